@@ -1,0 +1,203 @@
+/*
+ * qclhip — MI355X (gfx950) operator backend for the SCBA inner loop of
+ * QCLNEGF.jl: a third operator set beside the reference's educational (E0)
+ * and production (E1) operators, with the same argument meaning, array
+ * layout and error behaviour, callable from Julia (ccall), Python (ctypes)
+ * or C/C++.
+ *
+ * Conventions
+ *  - Every field argument is a DEVICE pointer to a complex (NE,Nk,Nb,Nb)
+ *    array in the reference layout: column-major, energy fastest, interleaved
+ *    (re,im) doubles (Julia ComplexF64): element (e,m,a,b) at
+ *    e + NE*(m + Nk*(a + Nb*b)). h is a device (Nk,Nb,Nb) complex array.
+ *  - Host arrays are named *_host. Grids (eps, wE, wk) are given once to
+ *    qclhip_grid_create and kept on the device.
+ *  - Work is enqueued on the context's stream. Entry points that return a
+ *    host scalar synchronize that stream; the others do not. Failures that
+ *    the device detects (a singular Dyson block, a number functional that is
+ *    not real, a Pauli-capacity violation) are latched in the context and
+ *    returned by qclhip_ctx_synchronize and by the next synchronizing call.
+ *  - Return codes: 0 ok; >0 one-based linear index e + NE*m + 1 of a singular
+ *    Dyson block (Julia: SingularException); QCLHIP_ERR_ARGUMENT
+ *    (ArgumentError), QCLHIP_ERR_DIMENSION (DimensionMismatch),
+ *    QCLHIP_ERR_DOMAIN (DomainError), QCLHIP_ERR_RUNTIME (HIP/rocBLAS
+ *    failure), QCLHIP_ERR_UNSUPPORTED (a size this build does not handle).
+ *  - Output fields must not alias inputs unless stated (the reference rejects
+ *    aliasing with ArgumentError, production_cache.jl:308-309).
+ *
+ * Reference citations are paths relative to the QCLNEGF.jl v0.2.0 root.
+ */
+#ifndef QCLHIP_H
+#define QCLHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QCLHIP_OK 0
+#define QCLHIP_ERR_ARGUMENT (-1)
+#define QCLHIP_ERR_DIMENSION (-2)
+#define QCLHIP_ERR_DOMAIN (-3)
+#define QCLHIP_ERR_RUNTIME (-4)
+#define QCLHIP_ERR_UNSUPPORTED (-5)
+
+typedef struct qclhip_ctx qclhip_ctx;
+typedef struct qclhip_grid qclhip_grid;
+typedef struct qclhip_shift qclhip_shift;
+typedef struct qclhip_kernel qclhip_kernel;
+typedef struct qclhip_hilbert qclhip_hilbert;
+
+/* ---- context ------------------------------------------------------------
+ * One context per host thread and device; owns the stream (or borrows the
+ * caller's), the rocBLAS handle, device status and scratch. Replaces the
+ * per-worker workspaces of the reference (production_cache.jl:63-117). */
+int qclhip_ctx_create(int device, void* stream, qclhip_ctx** out);
+int qclhip_ctx_destroy(qclhip_ctx* ctx);
+/* Waits for the stream, returns (and clears) the latched device status. */
+int qclhip_ctx_synchronize(qclhip_ctx* ctx);
+const char* qclhip_ctx_last_error(const qclhip_ctx* ctx);
+/* Build-time maximum basis size of the fused Dyson/Keldysh kernel. */
+int qclhip_max_basis(void);
+
+/* ---- grids (ModelGrids: reference/grids.jl:10-47) -----------------------
+ * eps, wE: energy nodes and trapezoid weights [NE]; wk: annular momentum
+ * weights [Nk]; gs: spin degeneracy g_s. */
+int qclhip_grid_create(qclhip_ctx* ctx, int NE, int Nk, int Nb, int gs,
+                       const double* eps_host, const double* wE_host,
+                       const double* wk_host, qclhip_grid** out);
+int qclhip_grid_destroy(qclhip_grid* grid);
+
+/* ---- A1: _retarded_green_production (production_operators.jl:1337-1385)
+ * G^R = inv(D/s)/s per block, D = (eps_e + i eta) I - h_m - SigmaR_em,
+ * s = max row sum |D| (written to scale[NE*Nk] when not NULL). sigR may be
+ * NULL (Sigma = 0, the seed of seed.jl:151-197). cond(D) (:1380) is a
+ * diagnostic that the reference never reads; it is not computed. */
+int qclhip_retarded_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h,
+                          const double* sigR, double eta, double* GR, double* scale);
+
+/* ---- A1+A2 fused: Dyson plus the raw Keldysh products
+ * G<raw = G^R S< G^R^dagger, G>raw = G^R S> G^R^dagger (production_state.jl:
+ * 14-48), and the weighted block traces for the number functionals.
+ * traces may be NULL (context scratch); else 2*NE*Nk complex. */
+int qclhip_dyson_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h,
+                         const double* sigR, const double* sigL, const double* sigG,
+                         double eta, double* GR, double* GL, double* GG, double* scale,
+                         double* traces);
+
+/* ---- A2: _current_green_production (production_state.jl:1-74) ----------
+ * total = fam[0] + fam[1] + ... (embedding first, then mechanisms in order;
+ * production_operators.jl:1447-1473); Dyson; Keldysh; number/holes
+ * (A3, production_operators.jl:1417-1445); paired coefficients (A4,
+ * reference/greens.jl:155-185); G< <- aG< - cG>, G> <- (1-c)G> - (1-a)G<;
+ * A = i(G^R - G^R^dagger) when A != NULL (A5). famR/famL/famG are HOST
+ * arrays of nfam device pointers. out_host (lambda, number, holes, a, c) may
+ * be NULL; when given, the call synchronizes. */
+int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h,
+                         int nfam, const double* const* famR, const double* const* famL,
+                         const double* const* famG, double target, double* GR, double* GL,
+                         double* GG, double* A, double* scale, double* out_host);
+
+/* A3: number functional N[G] = Re[-i g_s/(2 pi) sum wE wk tr G] with the
+ * realness gate (synchronizes). */
+int qclhip_number_functional(qclhip_ctx* ctx, const qclhip_grid* grid, const double* G,
+                             double* value_host);
+/* A5: A = i(G^R - G^R^dagger) (production_operators.jl:1387-1415). */
+int qclhip_spectral(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR, double* A);
+/* _green_state_production without a given G> (production_operators.jl:
+ * 1387-1415): G> = G^R - G^R^dagger + G<, A = i(G^R - G^R^dagger); either
+ * output may be NULL. */
+int qclhip_green_state(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR,
+                       const double* GL, double* GG, double* A);
+/* A4 applied with host coefficients (production_state.jl:62-70). */
+int qclhip_normalize_pair(qclhip_ctx* ctx, int64_t n, double* GL, double* GG, double a,
+                          double c);
+
+/* ---- A6: build_shift_plan / apply_energy_shift! (production_cache.jl:258-325)
+ * Non-cyclic two-point interpolation X(eps+delta); zero outside the window. */
+int qclhip_shift_create(qclhip_ctx* ctx, const qclhip_grid* grid, double delta,
+                        qclhip_shift** out);
+int qclhip_shift_destroy(qclhip_shift* shift);
+/* Y = S X over ncols columns of length NE; Y must not alias X. */
+int qclhip_apply_shift(qclhip_ctx* ctx, const qclhip_shift* shift, int64_t ncols,
+                       const double* X, double* Y);
+
+/* ---- A7: _production_static_contraction! (production_operators.jl:402-658)
+ * Kernel representations (production_contracts.jl:262-323), HOST matrices,
+ * column-major complex:
+ *   QCLHIP_KERNEL_DENSE: a = Kflat (Nc x Nc), rows (m,a,b), cols (m',c,d),
+ *     Nc = Nk*Nb^2 (production_cache.jl:387-409);
+ *   QCLHIP_KERNEL_MOMENTUM_INDEPENDENT: a = block (Nb^2 x Nb^2);
+ *   QCLHIP_KERNEL_SEPARABLE_IFR: a = left (Nb^2 x rank), b = right
+ *     (rank x Nb^2), c = radial (Nk x Nk);
+ *   QCLHIP_KERNEL_LOW_RANK: a = left (Nc x rank), b = right_adjoint (rank x Nc).
+ * The radial weights wk of the grid are folded into the device copy. */
+#define QCLHIP_KERNEL_DENSE 0
+#define QCLHIP_KERNEL_MOMENTUM_INDEPENDENT 1
+#define QCLHIP_KERNEL_SEPARABLE_IFR 2
+#define QCLHIP_KERNEL_LOW_RANK 3
+int qclhip_kernel_create(qclhip_ctx* ctx, const qclhip_grid* grid, int kind, int rank,
+                         const double* a_host, const double* b_host, const double* c_host,
+                         qclhip_kernel** out);
+int qclhip_kernel_destroy(qclhip_kernel* kernel);
+/* S <- alpha*qK*K[wk G] + beta*S; beta == 0 does not read S. */
+int qclhip_contract(qclhip_ctx* ctx, const qclhip_kernel* kernel, const double* G,
+                    double qK, double alpha, double beta, double* S);
+
+/* ---- A8: _production_lo_contraction, thermal N_LO (production_operators.jl:
+ * 1089-1162): S< = (N+1) K[S+ G<] + N K[S- G<], S> = (N+1) K[S- G>] + N K[S+ G>]
+ * with S+- the +-hbar*omega_LO shifts. */
+int qclhip_lo_contraction(qclhip_ctx* ctx, const qclhip_kernel* kernel, const double* GL,
+                          const double* GG, double qK, double NLO,
+                          const qclhip_shift* plus, const qclhip_shift* minus, double* SL,
+                          double* SG);
+
+/* ---- A9/A10: FFT Hilbert transform (fft_parallel.jl:27-291;
+ * production_operators.jl:735-884). Plan for ncols columns of length NE on a
+ * uniform grid; L = nextpow2(2NE-1); verify adds the 2L round-off witness. */
+int qclhip_hilbert_create(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t ncols,
+                          int verify, qclhip_hilbert** out);
+int qclhip_hilbert_destroy(qclhip_hilbert* plan);
+/* Lambda_e = sum_{e' != e} wE_e' Gamma_e' / (2 pi (eps_e - eps_e')).
+ * roundoff_host (may be NULL; requires verify) = ||L2 - L|| / (||L|| + 1e-14). */
+int qclhip_fft_hilbert(qclhip_ctx* ctx, const qclhip_hilbert* plan, const double* Gamma,
+                       double* Lambda, double* roundoff_host);
+/* Gamma = i(S> - S<), S^R = H[Gamma] - i Gamma/2; Gamma may be NULL. */
+int qclhip_retarded_self_energy(qclhip_ctx* ctx, const qclhip_hilbert* plan,
+                                const double* SL, const double* SG, double* SR,
+                                double* Gamma, double* roundoff_host);
+
+/* ---- A12: _embedding_component_production (production_operators.jl:886-922)
+ * S_em = T (shift X)_em T^dagger, zero outside the shift window. T is a
+ * device Nb x Nb complex matrix. */
+int qclhip_embedding_component(qclhip_ctx* ctx, const qclhip_grid* grid, const double* T,
+                               const double* X, const qclhip_shift* shift, double* S);
+
+/* ---- A13: elementwise family algebra (production_operators.jl:1447-1518) */
+/* out = x[0] + x[1] + ... (x is a HOST array of nx device pointers). */
+int qclhip_sum_fields(qclhip_ctx* ctx, int64_t n, int nx, const double* const* x, double* out);
+/* old <- (1-alpha) old + alpha cand. */
+int qclhip_mix(qclhip_ctx* ctx, int64_t n, double* old_, const double* cand, double alpha);
+
+/* ---- A14/A15: observables (production_iteration.jl:174-215) */
+/* Jbar = Re[g_s/(2 pi) sum_{e sorted by (|e|,e), m} wE wk tr(S>+ G< - S<+ G>)]. */
+int qclhip_boundary_flux(qclhip_ctx* ctx, const qclhip_grid* grid, const double* plusL,
+                         const double* plusG, const double* GL, const double* GG,
+                         double* Jbar_host);
+/* p_a = Re[-i g_s/(2 pi) sum wE wk G<_aa] / target, out_host[Nb]. */
+int qclhip_state_populations(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GL,
+                             double target, double* out_host);
+
+/* ---- A16: seed (seed.jl:28-128) */
+/* q_e = g_s/(2 pi) wE_e sum_m wk_m tr A_em; q_host[NE]. */
+int qclhip_seed_weights(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR,
+                        double* q_host);
+/* G< = -f(eps; mu, kBT) (G^R - G^R^dagger). */
+int qclhip_seed_lesser(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR, double mu,
+                       double kBT, double* GL);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,399 @@
+"""ctypes front end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker the HIP backend is compared against.
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may
+import this module; the product package never does.
+
+Fields are numpy complex128 arrays of shape (NE, Nk, Nb, Nb) in Fortran
+(column-major) order, i.e. exactly the memory layout of the reference's
+Array{ComplexF64,4}. All indices are 0-based.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ct.POINTER(ct.c_double)
+_ip = ct.POINTER(ct.c_int)
+_up = ct.POINTER(ct.c_ubyte)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: oracle returned {code}")
+        self.code = code
+
+
+class Kernel(ct.Structure):
+    _fields_ = [("kind", ct.c_int), ("rank", ct.c_int), ("a", _dp), ("b", _dp), ("c", _dp)]
+
+
+class Problem(ct.Structure):
+    _fields_ = [
+        ("NE", ct.c_int), ("Nk", ct.c_int), ("Nb", ct.c_int), ("nmech", ct.c_int),
+        ("gs", ct.c_int), ("energy_chunk", ct.c_int),
+        ("eps", _dp), ("wE", _dp), ("wk", _dp), ("h", _dp), ("Tplus", _dp), ("Tminus", _dp),
+        ("Ep", ct.c_double), ("hwLO", ct.c_double), ("NLO", ct.c_double),
+        ("target", ct.c_double), ("alpha_mix", ct.c_double),
+        ("kernels", ct.POINTER(Kernel)), ("qK", _dp), ("is_lo", _ip),
+    ]
+
+
+class State(ct.Structure):
+    _fields_ = [
+        ("scatR", ct.POINTER(_dp)), ("scatL", ct.POINTER(_dp)), ("scatG", ct.POINTER(_dp)),
+        ("plusR", _dp), ("plusL", _dp), ("plusG", _dp),
+        ("minusR", _dp), ("minusL", _dp), ("minusG", _dp),
+        ("embR", _dp), ("embL", _dp), ("embG", _dp),
+        ("GR", _dp), ("GL", _dp), ("GG", _dp), ("A", _dp), ("scale", _dp),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ct.CDLL(_LIB_PATH)
+        _lib.orc_number_functional.restype = ct.c_double
+        _lib.orc_boundary_flux_bar.restype = ct.c_double
+        _lib.orc_fermi.restype = ct.c_double
+    return _lib
+
+
+def P(x):
+    """Raw double pointer to a numpy array (complex arrays are interleaved)."""
+    if x is None:
+        return ct.cast(None, _dp)
+    assert x.flags.f_contiguous or x.flags.c_contiguous
+    return x.ctypes.data_as(_dp)
+
+
+def field(NE, Nk, Nb, dtype=np.complex128):
+    return np.zeros((NE, Nk, Nb, Nb), dtype=dtype, order="F")
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise OracleError(rc, what)
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+def retarded_green(eps, h, sigR, eta=0.0):
+    NE, (Nk, Nb) = len(eps), h.shape[:2]
+    GR = field(NE, Nk, Nb)
+    scale = np.zeros((NE, Nk), order="F")
+    rc = lib().orc_retarded_green(NE, Nk, Nb, P(eps), P(h), P(sigR), ct.c_double(eta), P(GR), P(scale))
+    _check(rc, "retarded_green")
+    return GR, scale
+
+
+def retarded_green_status(eps, h, sigR, eta=0.0):
+    NE, Nk, Nb, _ = sigR.shape
+    GR = field(NE, Nk, Nb)
+    scale = np.zeros((NE, Nk), order="F")
+    rc = lib().orc_retarded_green(NE, Nk, Nb, P(eps), P(h), P(sigR), ct.c_double(eta), P(GR), P(scale))
+    return rc, GR, scale
+
+
+def invert_block(A):
+    X = np.asfortranarray(A.astype(np.complex128))
+    rc = lib().orc_invert_block(X.shape[0], P(X))
+    _check(rc, "invert_block")
+    return X
+
+
+def spectral(GR):
+    NE, Nk, Nb, _ = GR.shape
+    A = field(NE, Nk, Nb)
+    lib().orc_spectral(NE, Nk, Nb, P(GR), P(A))
+    return A
+
+
+def keldysh(GR, S):
+    NE, Nk, Nb, _ = GR.shape
+    out = field(NE, Nk, Nb)
+    lib().orc_keldysh(NE, Nk, Nb, P(GR), P(S), P(out))
+    return out
+
+
+def number_functional(G, wE, wk, gs):
+    NE, Nk, Nb, _ = G.shape
+    st = ct.c_int(0)
+    v = lib().orc_number_functional(NE, Nk, Nb, P(G), P(wE), P(wk), gs, ct.byref(st))
+    _check(st.value, "number_functional")
+    return v
+
+
+def paired_coefficients(number, holes, target):
+    a, c, lam = ct.c_double(), ct.c_double(), ct.c_double()
+    rc = lib().orc_paired_coefficients(ct.c_double(number), ct.c_double(holes), ct.c_double(target),
+                                       ct.byref(a), ct.byref(c), ct.byref(lam))
+    _check(rc, "paired_coefficients")
+    return a.value, c.value, lam.value
+
+
+def normalize_pair(GL, GG, a, c):
+    lib().orc_normalize_pair(ct.c_longlong(GL.size), P(GL), P(GG), ct.c_double(a), ct.c_double(c))
+
+
+def current_green(eps, h, famR, famL, famG, wE, wk, gs, target):
+    """families: lists whose index 0 is the embedding total."""
+    NE, Nk, Nb, _ = famR[0].shape
+    nf = len(famR)
+    arr = lambda fs: (_dp * nf)(*[P(f) for f in fs])
+    GR, GL, GG, A = (field(NE, Nk, Nb) for _ in range(4))
+    scale = np.zeros((NE, Nk), order="F")
+    out3 = np.zeros(3)
+    rc = lib().orc_current_green(NE, Nk, Nb, P(eps), P(h), nf, arr(famR), arr(famL), arr(famG),
+                                 P(wE), P(wk), gs, ct.c_double(target), P(GR), P(GL), P(GG),
+                                 P(A), P(scale), P(out3))
+    _check(rc, "current_green")
+    return dict(GR=GR, GL=GL, GG=GG, A=A, scale=scale, lam=out3[0], number=out3[1], holes=out3[2])
+
+
+def shift_plan(eps, delta):
+    NE = len(eps)
+    left = np.zeros(NE, np.int32)
+    right = np.zeros(NE, np.int32)
+    wr = np.zeros(NE)
+    valid = np.zeros(NE, np.uint8)
+    rc = lib().orc_shift_plan(NE, P(eps), ct.c_double(delta), left.ctypes.data_as(_ip),
+                              right.ctypes.data_as(_ip), P(wr), valid.ctypes.data_as(_up))
+    _check(rc, "shift_plan")
+    return left, right, wr, valid
+
+
+def apply_shift(plan, X):
+    left, right, wr, valid = plan
+    Y = np.zeros_like(X, order="F")
+    NE = X.shape[0]
+    rc = lib().orc_apply_shift(NE, ct.c_longlong(X.size // NE), left.ctypes.data_as(_ip),
+                               right.ctypes.data_as(_ip), P(wr), valid.ctypes.data_as(_up), P(X), P(Y))
+    _check(rc, "apply_shift")
+    return Y
+
+
+def shift_matrix(eps, delta):
+    NE = len(eps)
+    W = np.zeros((NE, NE), order="F")
+    _check(lib().orc_shift_matrix(NE, P(eps), ct.c_double(delta), P(W)), "shift_matrix")
+    return W
+
+
+def apply_shift_dense(W, X):
+    Y = np.zeros_like(X, order="F")
+    NE = X.shape[0]
+    lib().orc_apply_shift_dense(NE, ct.c_longlong(X.size // NE), P(W), P(X), P(Y))
+    return Y
+
+
+def is_momentum_independent(K6):
+    Nk, _, Nb = K6.shape[:3]
+    return bool(lib().orc_is_momentum_independent(Nk, Nb, P(K6)))
+
+
+def kernel_dense_from_K6(K6):
+    Nk, Nb = K6.shape[0], K6.shape[2]
+    Nc = Nk * Nb * Nb
+    F = np.zeros((Nc, Nc), np.complex128, order="F")
+    lib().orc_kernel_dense_from_K6(Nk, Nb, P(K6), P(F))
+    return F
+
+
+def kernel_mi_from_K6(K6):
+    Nk, Nb = K6.shape[0], K6.shape[2]
+    B = np.zeros((Nb * Nb, Nb * Nb), np.complex128, order="F")
+    lib().orc_kernel_mi_from_K6(Nk, Nb, P(K6), P(B))
+    return B
+
+
+KIND = {"dense": 0, "momentum_independent": 1, "separable_ifr": 2, "low_rank": 3, "literal": 4}
+
+
+def make_kernel(kind, a, b=None, c=None, rank=0):
+    keep = [np.asfortranarray(x) if x is not None else None for x in (a, b, c)]
+    k = Kernel(KIND[kind], int(rank), P(keep[0]), P(keep[1]), P(keep[2]))
+    k._keep = keep
+    return k
+
+
+def contract(kernel, G, wk, qK, energy_chunk=128, alpha=1.0, beta=0.0, S=None):
+    NE, Nk, Nb, _ = G.shape
+    if S is None:
+        S = field(NE, Nk, Nb)
+    rc = lib().orc_contract(NE, Nk, Nb, ct.byref(kernel), P(G), P(wk), ct.c_double(qK),
+                            int(energy_chunk), ct.c_double(alpha), ct.c_double(beta), P(S))
+    _check(rc, "contract")
+    return S
+
+
+def uniform_spacing(eps):
+    d = ct.c_double()
+    _check(lib().orc_uniform_spacing(len(eps), P(eps), ct.byref(d)), "uniform_spacing")
+    return d.value
+
+
+def fft(x, inverse=False):
+    y = np.array(x, dtype=np.complex128)
+    lib().orc_fft(ct.c_longlong(len(y)), P(y), int(inverse))
+    return y
+
+
+def fft_hilbert(Gamma, eps, wE, roundoff=True):
+    NE = Gamma.shape[0]
+    Lam = np.zeros_like(Gamma, order="F")
+    r = ct.c_double(0.0)
+    rc = lib().orc_fft_hilbert_checked(NE, ct.c_longlong(Gamma.size // NE), P(eps), P(wE), P(Gamma),
+                                       P(Lam), ct.byref(r) if roundoff else None)
+    _check(rc, "fft_hilbert")
+    return (Lam, r.value) if roundoff else Lam
+
+
+def direct_hilbert(Gamma, eps, wE, reverse=False):
+    NE = Gamma.shape[0]
+    Lam = np.zeros_like(Gamma, order="F")
+    rc = lib().orc_direct_hilbert(NE, ct.c_longlong(Gamma.size // NE), P(eps), P(wE), P(Gamma),
+                                  P(Lam), int(reverse))
+    _check(rc, "direct_hilbert")
+    return Lam
+
+
+def retarded_self_energy_fft(SL, SG, eps, wE):
+    NE = SL.shape[0]
+    SR = np.zeros_like(SL, order="F")
+    Gam = np.zeros_like(SL, order="F")
+    r = ct.c_double(0.0)
+    rc = lib().orc_retarded_self_energy_fft(NE, ct.c_longlong(SL.size // NE), P(eps), P(wE), P(SL),
+                                            P(SG), P(SR), P(Gam), ct.byref(r))
+    _check(rc, "retarded_self_energy_fft")
+    return SR, Gam, r.value
+
+
+def lo_contraction(kernel, GL, GG, wk, qK, NLO, plan_plus, plan_minus, energy_chunk=128):
+    NE, Nk, Nb, _ = GL.shape
+    SL, SG = field(NE, Nk, Nb), field(NE, Nk, Nb)
+    pl, pr, pw, pv = plan_plus
+    ml, mr, mw, mv = plan_minus
+    ip = lambda a: a.ctypes.data_as(_ip)
+    up = lambda a: a.ctypes.data_as(_up)
+    rc = lib().orc_lo_contraction(NE, Nk, Nb, ct.byref(kernel), P(GL), P(GG), P(wk), ct.c_double(qK),
+                                  ct.c_double(NLO), ip(pl), ip(pr), P(pw), up(pv), ip(ml), ip(mr),
+                                  P(mw), up(mv), int(energy_chunk), P(SL), P(SG))
+    _check(rc, "lo_contraction")
+    return SL, SG
+
+
+def embedding_component(T, G, plan):
+    NE, Nk, Nb, _ = G.shape
+    S = field(NE, Nk, Nb)
+    left, right, wr, valid = plan
+    Tf = np.asfortranarray(T, dtype=np.complex128)
+    rc = lib().orc_embedding_component(NE, Nk, Nb, P(Tf), P(G), left.ctypes.data_as(_ip),
+                                       right.ctypes.data_as(_ip), P(wr), valid.ctypes.data_as(_up), P(S))
+    _check(rc, "embedding_component")
+    return S
+
+
+def mix(old, cand, alpha):
+    lib().orc_mix(ct.c_longlong(old.size), P(old), P(cand), ct.c_double(alpha))
+
+
+def boundary_flux_bar(eps, wE, wk, gs, plusL, plusG, GL, GG):
+    NE, Nk, Nb, _ = GL.shape
+    return lib().orc_boundary_flux_bar(NE, Nk, Nb, P(eps), P(wE), P(wk), gs, P(plusL), P(plusG),
+                                       P(GL), P(GG))
+
+
+def state_populations(wE, wk, gs, target, GL):
+    NE, Nk, Nb, _ = GL.shape
+    out = np.zeros(Nb)
+    lib().orc_state_populations(NE, Nk, Nb, P(wE), P(wk), gs, ct.c_double(target), P(GL), P(out))
+    return out
+
+
+def fermi(e, mu, kBT):
+    return lib().orc_fermi(ct.c_double(e), ct.c_double(mu), ct.c_double(kBT))
+
+
+def seed_weights(GR, wE, wk, gs):
+    NE, Nk, Nb, _ = GR.shape
+    q = np.zeros(NE)
+    _check(lib().orc_seed_weights(NE, Nk, Nb, P(GR), P(wE), P(wk), gs, P(q)), "seed_weights")
+    return q
+
+
+def seed_mu(eps, q, kBT, target):
+    mu = ct.c_double()
+    _check(lib().orc_seed_mu(len(eps), P(eps), P(q), ct.c_double(kBT), ct.c_double(target),
+                             ct.byref(mu)), "seed_mu")
+    return mu.value
+
+
+def seed_lesser(GR, eps, mu, kBT):
+    NE, Nk, Nb, _ = GR.shape
+    GL = field(NE, Nk, Nb)
+    lib().orc_seed_lesser(NE, Nk, Nb, P(GR), P(eps), ct.c_double(mu), ct.c_double(kBT), P(GL))
+    return GL
+
+
+class ScbaOracle:
+    """Holds an SCBA problem and state in host memory and runs oracle iterations."""
+
+    def __init__(self, prob: dict, state: dict):
+        self.prob = prob
+        self.state = state
+        NE, Nk, Nb = prob["NE"], prob["Nk"], prob["Nb"]
+        M = len(prob["kernels"])
+        self._kern = (Kernel * max(M, 1))(*prob["kernels"])
+        self._qK = np.array(prob["qK"], dtype=np.float64)
+        self._islo = np.array(prob["is_lo"], dtype=np.int32)
+        p = Problem()
+        p.NE, p.Nk, p.Nb, p.nmech, p.gs = NE, Nk, Nb, M, prob["gs"]
+        p.energy_chunk = prob.get("energy_chunk", 128)
+        for name in ("eps", "wE", "wk", "h", "Tplus", "Tminus"):
+            setattr(p, name, P(prob[name]))
+        p.Ep, p.hwLO, p.NLO = prob["Ep"], prob["hwLO"], prob["NLO"]
+        p.target, p.alpha_mix = prob["target"], prob["alpha_mix"]
+        p.kernels = self._kern
+        p.qK = P(self._qK)
+        p.is_lo = self._islo.ctypes.data_as(_ip)
+        self._p = p
+        s = State()
+        mk = lambda lst: (_dp * max(M, 1))(*[P(x) for x in lst])
+        self._scat = [mk(state["scatR"]), mk(state["scatL"]), mk(state["scatG"])]
+        s.scatR, s.scatL, s.scatG = (ct.cast(a, ct.POINTER(_dp)) for a in self._scat)
+        for name in ("plusR", "plusL", "plusG", "minusR", "minusL", "minusG", "embR", "embL", "embG",
+                     "GR", "GL", "GG", "A", "scale"):
+            setattr(s, name, P(state[name]))
+        self._s = s
+
+    def iterate(self):
+        out = np.zeros(5)
+        pops = np.zeros(self.prob["Nb"])
+        rc = lib().orc_scba_iteration(ct.byref(self._p), ct.byref(self._s), P(out), P(pops))
+        _check(rc, "scba_iteration")
+        return dict(lam=out[0], Jbar=out[1], roundoff=out[2], number=out[3], holes=out[4],
+                    populations=pops)

@@ -1,0 +1,154 @@
+// Shared definitions of the MI355X (gfx950) SCBA operator library.
+//
+// Data layout in HBM: every field is the reference's Array{ComplexF64,4} of
+// shape (NE,Nk,Nb,Nb) in column-major order (energy fastest), interleaved
+// (re,im) doubles. Element (e,m,a,b) sits at e + NE*(m + Nk*(a + Nb*b)), so a
+// block (e,m) is a strided Nb x Nb matrix with element stride NE*Nk and
+// consecutive energies of one element are contiguous in memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/qclhip.h"
+
+namespace qclhip {
+
+using c64 = double2;  // one complex double, (x,y) = (re,im)
+
+__host__ __device__ inline c64 cmk(double re, double im) { c64 z; z.x = re; z.y = im; return z; }
+__device__ inline c64 cadd(c64 a, c64 b) { return cmk(a.x + b.x, a.y + b.y); }
+__device__ inline c64 csub(c64 a, c64 b) { return cmk(a.x - b.x, a.y - b.y); }
+__device__ inline c64 cscale(double s, c64 a) { return cmk(s * a.x, s * a.y); }
+__device__ inline c64 cmul(c64 a, c64 b) {
+    return cmk(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+// c + a*b
+__device__ inline c64 cfma(c64 c, c64 a, c64 b) {
+    return cmk(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
+}
+// c - a*b
+__device__ inline c64 cfms(c64 c, c64 a, c64 b) {
+    return cmk(fma(-a.x, b.x, fma(a.y, b.y, c.x)), fma(-a.x, b.y, fma(-a.y, b.x, c.y)));
+}
+// c + a*conj(b)
+__device__ inline c64 cfmac(c64 c, c64 a, c64 b) {
+    return cmk(fma(a.x, b.x, fma(a.y, b.y, c.x)), fma(a.y, b.x, fma(-a.x, b.y, c.y)));
+}
+__device__ inline c64 cconj(c64 a) { return cmk(a.x, -a.y); }
+__device__ inline double cabs1(c64 a) { return fabs(a.x) + fabs(a.y); }
+// 1/z by Smith's algorithm (no overflow for |z| near the double range).
+__device__ inline c64 crecip(c64 z) {
+    if (fabs(z.x) >= fabs(z.y)) {
+        double r = z.y / z.x, d = z.x + z.y * r;
+        return cmk(1.0 / d, -r / d);
+    }
+    double r = z.x / z.y, d = z.y + z.x * r;
+    return cmk(r / d, -1.0 / d);
+}
+
+// Device-side status word shared by all kernels of one context:
+// code (QCLHIP_* error, 0 = fine) and the smallest one-based failing block.
+struct DeviceStatus {
+    int code;
+    int pad;
+    unsigned long long block;
+};
+
+// Paired-normalization coefficients produced on the device
+// (reference/greens.jl:155-185) and read by the normalization kernel.
+struct PairCoefficients {
+    double a, c, lambda, number, holes;
+};
+
+__device__ inline void report_status(DeviceStatus* st, int code, unsigned long long block) {
+    atomicCAS(&st->code, 0, code);
+    if (block) atomicMin(&st->block, block);
+}
+
+// Wave-local LDS ordering: LDS instructions of one wavefront execute in
+// order, so a compiler fence around a wave barrier makes one lane's LDS store
+// visible to the other lanes' later loads.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline double shfl_xor_d(double v, int mask) { return __shfl_xor(v, mask, 64); }
+__device__ inline int shfl_xor_i(int v, int mask) { return __shfl_xor(v, mask, 64); }
+__device__ inline double readlane_d(double v, int lane) {
+    long long bits = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), lane);
+    int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+}  // namespace qclhip
+
+// ---------------------------------------------------------------------------
+// Host-side context (opaque to callers).
+struct qclhip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = false;
+    rocblas_handle blas = nullptr;
+    qclhip::DeviceStatus* status = nullptr;         // device
+    qclhip::DeviceStatus* status_host = nullptr;    // pinned
+    qclhip::PairCoefficients* coef = nullptr;       // device
+    double* host_scalars = nullptr;                 // pinned, 64 doubles
+    void* workspace = nullptr;                      // device scratch, grows on demand
+    size_t workspace_bytes = 0;
+    int sm_count = 0;
+    size_t max_lds_per_block = 65536;
+    std::string last_error;
+};
+
+struct qclhip_grid {
+    int NE = 0, Nk = 0, Nb = 0, gs = 2;
+    double de = 0.0;        // uniform spacing (0 when the grid is not uniform)
+    double* eps = nullptr;  // device [NE]
+    double* wE = nullptr;   // device [NE]
+    double* wk = nullptr;   // device [Nk]
+    int* flux_order = nullptr;  // device [NE] energies sorted by (|e|, e)
+    std::vector<double> eps_host, wE_host, wk_host;
+};
+
+struct qclhip_shift {
+    int NE = 0;
+    double delta = 0.0;
+    int* left = nullptr;
+    int* right = nullptr;
+    double* wr = nullptr;
+    unsigned char* valid = nullptr;
+};
+
+namespace qclhip {
+
+int fail(qclhip_ctx* ctx, int code, const std::string& msg);
+int hip_fail(qclhip_ctx* ctx, hipError_t err, const char* what);
+// Device scratch of at least `bytes`, owned by the context.
+void* workspace(qclhip_ctx* ctx, size_t bytes);
+int launch_check(qclhip_ctx* ctx, const char* what);
+// Gamma = i (S> - S<) elementwise over n complex entries.
+int gamma_field(qclhip_ctx* ctx, long long n, const double* SL, const double* SG, double* Gamma);
+// Blocks for an elementwise grid-stride loop.
+inline unsigned grid_blocks(long long n, int threads, int sm_count) {
+    long long blocks = (n + threads - 1) / threads;
+    long long cap = (long long)(sm_count > 0 ? sm_count : 32) * 32;
+    if (blocks > cap) blocks = cap;
+    return (unsigned)(blocks < 1 ? 1 : blocks);
+}
+
+}  // namespace qclhip
+
+#define QCLHIP_TRY_HIP(ctx, expr)                                              \
+    do {                                                                       \
+        hipError_t _e = (expr);                                                \
+        if (_e != hipSuccess) return qclhip::hip_fail((ctx), _e, #expr);       \
+    } while (0)

@@ -1,0 +1,267 @@
+// Scattering-kernel contractions (production_operators.jl:402-658) and the
+// thermal LO map (:1089-1162).
+//
+// With G stored energy-fastest, G viewed as the column-major matrix
+// G2 (NE x Nc), Nc = Nk*Nb^2 (column q = m + Nk*(a + Nb*b)), every
+// representation is a short chain of plain GEMMs, which go to rocBLAS:
+//   dense:      S2 = f * G2 * (K diag(w))^T                 (NE x Nc)
+//   low rank:   S2 = f * (G2 * (R^+ diag(w))^T) * left^T
+//   momentum independent: X(e,(c,d)) = sum_m' wk_m' G(e,m',c,d) (kernel),
+//               Y = X * block^T, S(e,m,(a,b)) = f*Y(e,(a,b)) for every m
+//   separable IFR: C1 = Gm * right^T (Gm = G as (NE*Nk) x Nb^2),
+//               C2_j = C1_j * (radial diag(wk))^T per interface j (batched),
+//               S = f * C2 * left^T
+// with w(q) = wk(q mod Nk) folded into the device copy of the kernel.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+struct qclhip_kernel {
+    int kind = 0, rank = 0, NE = 0, Nk = 0, Nb = 0;
+    const qclhip_grid* grid = nullptr;
+    qclhip::c64* a = nullptr;
+    qclhip::c64* b = nullptr;
+    qclhip::c64* c = nullptr;
+};
+
+namespace qclhip {
+
+int lo_combine(qclhip_ctx* ctx, int NE, long long n, const c64* Y, double ca, const qclhip_shift* a, double cb,
+               const qclhip_shift* b, c64* out);
+
+namespace {
+
+__global__ void k_mi_reduce(int NE, int Nk, int N2, const double* __restrict__ wk, const c64* __restrict__ G,
+                            c64* __restrict__ X) {
+    const long long n = (long long)NE * N2;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int e = (int)(idx % NE);
+        const long long q = idx / NE;
+        c64 v = cmk(0.0, 0.0);
+        for (int mp = 0; mp < Nk; ++mp) {
+            c64 g = G[e + (long long)NE * (mp + (long long)Nk * q)];
+            v = cmk(v.x + wk[mp] * g.x, v.y + wk[mp] * g.y);
+        }
+        X[idx] = v;
+    }
+}
+
+__global__ void k_mi_broadcast(int NE, int Nk, int N2, const c64* __restrict__ Y, double f, double beta,
+                               c64* __restrict__ S) {
+    const long long n = (long long)NE * Nk * N2;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int e = (int)(idx % NE);
+        const long long q = idx / ((long long)NE * Nk);
+        c64 y = Y[e + (long long)NE * q];
+        c64 v = cmk(f * y.x, f * y.y);
+        if (beta != 0.0) {
+            c64 s = S[idx];
+            v = cmk(beta * s.x + v.x, beta * s.y + v.y);
+        }
+        S[idx] = v;
+    }
+}
+
+inline rocblas_double_complex rc(double re, double im = 0.0) {
+    rocblas_double_complex z(re, im);
+    return z;
+}
+
+inline const rocblas_double_complex* B_(const c64* p) { return reinterpret_cast<const rocblas_double_complex*>(p); }
+inline rocblas_double_complex* B_(c64* p) { return reinterpret_cast<rocblas_double_complex*>(p); }
+
+int blas_check(qclhip_ctx* ctx, rocblas_status st, const char* what) {
+    if (st != rocblas_status_success)
+        return fail(ctx, QCLHIP_ERR_RUNTIME, std::string(what) + ": rocBLAS status " + std::to_string((int)st));
+    return QCLHIP_OK;
+}
+
+// C (m x n, ld m) = alpha * A (m x k, ld lda) * B^T (B is n x k, ld ldb) + beta * C
+int gemm_nt(qclhip_ctx* ctx, long long m, long long n, long long k, double alpha, const c64* A, long long lda,
+            const c64* Bm, long long ldb, double beta, c64* C, long long ldc) {
+    rocblas_double_complex al = rc(alpha), be = rc(beta);
+    return blas_check(ctx,
+                      rocblas_zgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)m,
+                                    (rocblas_int)n, (rocblas_int)k, &al, B_(A), (rocblas_int)lda, B_(Bm),
+                                    (rocblas_int)ldb, &be, B_(C), (rocblas_int)ldc),
+                      "rocblas_zgemm");
+}
+
+}  // namespace
+
+// Device scratch that contract_core takes from the context workspace.
+size_t core_scratch_bytes(const qclhip_kernel* K) {
+    const qclhip_grid* g = K->grid;
+    const size_t NE = g->NE, Nk = g->Nk, N2 = (size_t)g->Nb * g->Nb, r = (size_t)std::max(1, K->rank);
+    switch (K->kind) {
+        case QCLHIP_KERNEL_LOW_RANK: return NE * r * sizeof(c64);
+        case QCLHIP_KERNEL_MOMENTUM_INDEPENDENT: return 2 * NE * N2 * sizeof(c64);
+        case QCLHIP_KERNEL_SEPARABLE_IFR: return 2 * NE * Nk * r * sizeof(c64);
+        default: return 0;
+    }
+}
+
+// S <- f K[w G] + beta S for the given representation.
+int contract_core(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double f, double beta, c64* S) {
+    const qclhip_grid* g = K->grid;
+    const long long NE = g->NE, Nk = g->Nk, N2 = (long long)g->Nb * g->Nb, Nc = Nk * N2;
+    switch (K->kind) {
+        case QCLHIP_KERNEL_DENSE:
+            return gemm_nt(ctx, NE, Nc, Nc, f, G, NE, K->a, Nc, beta, S, NE);
+        case QCLHIP_KERNEL_LOW_RANK: {
+            const long long r = K->rank;
+            c64* red = (c64*)workspace(ctx, (size_t)NE * r * sizeof(c64));
+            if (!red) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+            int st = gemm_nt(ctx, NE, r, Nc, 1.0, G, NE, K->b, r, 0.0, red, NE);
+            if (st) return st;
+            return gemm_nt(ctx, NE, Nc, r, f, red, NE, K->a, Nc, beta, S, NE);
+        }
+        case QCLHIP_KERNEL_MOMENTUM_INDEPENDENT: {
+            c64* X = (c64*)workspace(ctx, (size_t)2 * NE * N2 * sizeof(c64));
+            if (!X) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+            c64* Y = X + NE * N2;
+            hipLaunchKernelGGL(k_mi_reduce, dim3(grid_blocks(NE * N2, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
+                               (int)NE, (int)Nk, (int)N2, g->wk, G, X);
+            int st = launch_check(ctx, "k_mi_reduce");
+            if (st) return st;
+            st = gemm_nt(ctx, NE, N2, N2, 1.0, X, NE, K->a, N2, 0.0, Y, NE);
+            if (st) return st;
+            hipLaunchKernelGGL(k_mi_broadcast, dim3(grid_blocks(NE * Nk * N2, 256, ctx->sm_count)), dim3(256), 0,
+                               ctx->stream, (int)NE, (int)Nk, (int)N2, Y, f, beta, S);
+            return launch_check(ctx, "k_mi_broadcast");
+        }
+        case QCLHIP_KERNEL_SEPARABLE_IFR: {
+            const long long J = K->rank, M = NE * Nk;
+            c64* C1 = (c64*)workspace(ctx, (size_t)2 * M * J * sizeof(c64));
+            if (!C1) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+            c64* C2 = C1 + M * J;
+            int st = gemm_nt(ctx, M, J, N2, 1.0, G, M, K->b, J, 0.0, C1, M);
+            if (st) return st;
+            rocblas_double_complex one = rc(1.0), zero = rc(0.0);
+            st = blas_check(ctx,
+                            rocblas_zgemm_strided_batched(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
+                                                          (rocblas_int)NE, (rocblas_int)Nk, (rocblas_int)Nk, &one,
+                                                          B_(C1), (rocblas_int)NE, (rocblas_stride)M, B_(K->c),
+                                                          (rocblas_int)Nk, (rocblas_stride)0, &zero, B_(C2),
+                                                          (rocblas_int)NE, (rocblas_stride)M, (rocblas_int)J),
+                            "rocblas_zgemm_strided_batched");
+            if (st) return st;
+            return gemm_nt(ctx, M, N2, J, f, C2, M, K->a, N2, beta, S, M);
+        }
+        default:
+            return fail(ctx, QCLHIP_ERR_ARGUMENT, "unknown kernel representation");
+    }
+}
+
+}  // namespace qclhip
+
+using namespace qclhip;
+
+extern "C" {
+
+int qclhip_kernel_create(qclhip_ctx* ctx, const qclhip_grid* grid, int kind, int rank, const double* a_host,
+                         const double* b_host, const double* c_host, qclhip_kernel** out) {
+    if (!ctx || !grid || !out || !a_host) return QCLHIP_ERR_ARGUMENT;
+    *out = nullptr;
+    const long long Nk = grid->Nk, N2 = (long long)grid->Nb * grid->Nb, Nc = Nk * N2;
+    typedef std::complex<double> cd;
+    const cd* A = reinterpret_cast<const cd*>(a_host);
+    const cd* Bh = reinterpret_cast<const cd*>(b_host);
+    const cd* Ch = reinterpret_cast<const cd*>(c_host);
+    std::vector<cd> da, db, dc;
+    const std::vector<double>& wk = grid->wk_host;
+    switch (kind) {
+        case QCLHIP_KERNEL_DENSE:
+            da.resize((size_t)Nc * Nc);
+            for (long long qp = 0; qp < Nc; ++qp)
+                for (long long q = 0; q < Nc; ++q) da[q + Nc * qp] = A[q + Nc * qp] * wk[qp % Nk];
+            break;
+        case QCLHIP_KERNEL_MOMENTUM_INDEPENDENT:
+            da.assign(A, A + N2 * N2);
+            break;
+        case QCLHIP_KERNEL_SEPARABLE_IFR:
+            if (rank < 1 || !Bh || !Ch) return fail(ctx, QCLHIP_ERR_ARGUMENT, "IFR kernel needs left, right, radial");
+            da.assign(A, A + N2 * rank);
+            db.assign(Bh, Bh + (long long)rank * N2);
+            dc.resize((size_t)Nk * Nk);
+            for (long long mp = 0; mp < Nk; ++mp)
+                for (long long m = 0; m < Nk; ++m) dc[m + Nk * mp] = Ch[m + Nk * mp] * wk[mp];
+            break;
+        case QCLHIP_KERNEL_LOW_RANK:
+            if (rank < 1 || !Bh) return fail(ctx, QCLHIP_ERR_ARGUMENT, "low-rank kernel needs left and right_adjoint");
+            da.assign(A, A + Nc * rank);
+            db.resize((size_t)rank * Nc);
+            for (long long q = 0; q < Nc; ++q)
+                for (long long i = 0; i < rank; ++i) db[i + rank * q] = Bh[i + rank * q] * wk[q % Nk];
+            break;
+        default:
+            return fail(ctx, QCLHIP_ERR_ARGUMENT, "unknown kernel representation");
+    }
+    auto* K = new qclhip_kernel();
+    K->kind = kind;
+    K->rank = rank;
+    K->grid = grid;
+    K->Nk = (int)Nk;
+    K->Nb = grid->Nb;
+    bool ok = true;
+    auto upload = [&](std::vector<cd>& h, c64** d) {
+        if (h.empty() || !ok) return;
+        ok = hipMalloc(d, h.size() * sizeof(c64)) == hipSuccess &&
+             hipMemcpy(*d, h.data(), h.size() * sizeof(c64), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    upload(da, &K->a);
+    upload(db, &K->b);
+    upload(dc, &K->c);
+    if (!ok) {
+        qclhip_kernel_destroy(K);
+        return fail(ctx, QCLHIP_ERR_RUNTIME, "kernel allocation failed");
+    }
+    *out = K;
+    return QCLHIP_OK;
+}
+
+int qclhip_kernel_destroy(qclhip_kernel* K) {
+    if (!K) return QCLHIP_OK;
+    hipDeviceSynchronize();
+    if (K->a) hipFree(K->a);
+    if (K->b) hipFree(K->b);
+    if (K->c) hipFree(K->c);
+    delete K;
+    return QCLHIP_OK;
+}
+
+int qclhip_contract(qclhip_ctx* ctx, const qclhip_kernel* kernel, const double* G, double qK, double alpha,
+                    double beta, double* S) {
+    if (!ctx || !kernel || !G || !S) return QCLHIP_ERR_ARGUMENT;
+    if (G == S) return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
+    return contract_core(ctx, kernel, (const c64*)G, alpha * qK, beta, (c64*)S);
+}
+
+int qclhip_lo_contraction(qclhip_ctx* ctx, const qclhip_kernel* kernel, const double* GL, const double* GG,
+                          double qK, double NLO, const qclhip_shift* plus, const qclhip_shift* minus, double* SL,
+                          double* SG) {
+    if (!ctx || !kernel || !GL || !GG || !plus || !minus || !SL || !SG) return QCLHIP_ERR_ARGUMENT;
+    const qclhip_grid* g = kernel->grid;
+    if (plus->NE != g->NE || minus->NE != g->NE) return fail(ctx, QCLHIP_ERR_DIMENSION, "shift plan has the wrong energy size");
+    const long long n = (long long)g->NE * g->Nk * g->Nb * g->Nb;
+    // Y = qK K[w G] once per component, then both shifts of Y: the shifts act
+    // on energy and K on (m,a,b), so S K[G] = K[S G]. Y sits behind the
+    // scratch that contract_core uses, in one workspace request.
+    const size_t core = (core_scratch_bytes(kernel) + 255) / 256 * 256;
+    char* ws = (char*)workspace(ctx, core + (size_t)n * sizeof(c64));
+    if (!ws) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    c64* Y = (c64*)(ws + core);
+    int st = contract_core(ctx, kernel, (const c64*)GL, qK, 0.0, Y);
+    if (st) return st;
+    st = lo_combine(ctx, g->NE, n, Y, NLO + 1.0, plus, NLO, minus, (c64*)SL);
+    if (st) return st;
+    st = contract_core(ctx, kernel, (const c64*)GG, qK, 0.0, Y);
+    if (st) return st;
+    return lo_combine(ctx, g->NE, n, Y, NLO + 1.0, minus, NLO, plus, (c64*)SG);
+}
+
+}  // extern "C"

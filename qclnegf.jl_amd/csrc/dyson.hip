@@ -1,0 +1,867 @@
+// Fused Dyson / Keldysh kernel and the number-functional reductions.
+//
+// K_dyson_keldysh: one (e,m) block per wavefront. The 64 lanes form an 8x8
+// grid (lr = lane&7 row group, lc = lane>>3 column group); lane (lr,lc) keeps
+// the B x B sub-block rows lr*B.., columns lc*B.. of the matrix in registers
+// (N <= 8B, padding rows/columns are zero and never pivot).
+//
+// Inversion: in-place Gauss-Jordan with partial pivoting. The pivot of
+// column k is the first not-yet-used row of maximal |Re|+|Im| (the LAPACK
+// izamax choice of zgetrf, production_operators.jl:20-41, and of the owned
+// small inverse, :84-139). Rows are never swapped: the pivot row is
+// remembered and the permutation is undone when the inverse is written to
+// LDS. Per step the pivot column and the scaled pivot row travel through
+// LDS; every lane then does B*B complex FMAs on its own registers.
+//
+// Keldysh: with G^R in LDS, T = G^R S and G = T G^R^dagger are formed by the
+// same 8x8 lane grid, each lane accumulating its B x B block.
+#include <climits>
+
+#include "common.hpp"
+
+namespace qclhip {
+
+template <int B>
+struct DKShape {
+    static constexpr int NP = 8 * B;      // padded matrix size
+    static constexpr int LD = NP + 1;     // LDS row stride (complex elements)
+    static constexpr size_t bytes =
+        (size_t)2 * NP * LD * sizeof(c64) + 2 * NP * sizeof(c64) + 2 * NP * sizeof(int);
+};
+
+struct DKArgs {
+    int NE, Nk, N;
+    const double* eps;
+    const c64* h;
+    const c64* sigR;
+    const c64* sigL;
+    const c64* sigG;
+    double eta;
+    c64* GR;
+    c64* GL;
+    c64* GG;
+    double* scale;
+    c64* traces;  // [2][NE*Nk] raw traces of G< and G> blocks
+    DeviceStatus* status;
+    int keldysh;
+};
+
+// out[r][c] (lane block) = Gs[rows] * Xs[:, cols]  or  Xs[rows] * Gs[cols]^dagger
+template <int B, bool ADJ>
+__device__ inline void block_product(const c64* __restrict__ L, const c64* __restrict__ R, int N,
+                                     int lr, int lc, c64 (&out)[B][B]) {
+    constexpr int LD = DKShape<B>::LD;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) out[r][c] = cmk(0.0, 0.0);
+    for (int q = 0; q < N; ++q) {
+        c64 lv[B], rv[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) lv[r] = L[(lr * B + r) * LD + q];
+#pragma unroll
+        for (int c = 0; c < B; ++c) rv[c] = ADJ ? R[(lc * B + c) * LD + q] : R[q * LD + lc * B + c];
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c)
+                out[r][c] = ADJ ? cfmac(out[r][c], lv[r], rv[c]) : cfma(out[r][c], lv[r], rv[c]);
+    }
+}
+
+template <int B, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_dyson_keldysh(DKArgs p) {
+    constexpr int NP = DKShape<B>::NP;
+    constexpr int LD = DKShape<B>::LD;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const long long nblk = (long long)p.NE * p.Nk;
+    const long long blk = (long long)blockIdx.x * WAVES + wave;
+    if (blk >= nblk) return;  // whole wavefront leaves together
+
+    c64* Gs = reinterpret_cast<c64*>(smem + (size_t)wave * DKShape<B>::bytes);
+    c64* Xs = Gs + NP * LD;
+    c64* colbuf = Xs + NP * LD;
+    c64* rowbuf = colbuf + NP;
+    int* pivrow = reinterpret_cast<int*>(rowbuf + NP);
+    int* steprow = pivrow + NP;
+
+    const int N = p.N;
+    const int e = (int)(blk % p.NE);
+    const int m = (int)(blk / p.NE);
+    const size_t S = (size_t)nblk;  // stride between matrix elements of one block
+    const int lr = lane & 7, lc = lane >> 3;
+
+    // D = (eps_e + i eta) I - h_m - SigmaR_em (production_operators.jl:1360-1363)
+    c64 A[B][B];
+    const double ee = p.eps[e];
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr * B + r, j = lc * B + c;
+            c64 v = cmk(0.0, 0.0);
+            if (i < N && j < N) {
+                c64 hv = p.h[(size_t)m + (size_t)p.Nk * (i + (size_t)N * j)];
+                c64 sv = p.sigR ? p.sigR[blk + S * (i + (size_t)N * j)] : cmk(0.0, 0.0);
+                double dre = (i == j) ? ee : 0.0, dim = (i == j) ? p.eta : 0.0;
+                v = cmk(dre - hv.x - sv.x, dim - hv.y - sv.y);
+            }
+            A[r][c] = v;
+        }
+
+    // s = max_a sum_b |D_ab| (production_operators.jl:1364-1372)
+    double rows[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < B; ++c) acc += hypot(A[r][c].x, A[r][c].y);
+        acc += shfl_xor_d(acc, 8);
+        acc += shfl_xor_d(acc, 16);
+        acc += shfl_xor_d(acc, 32);
+        rows[r] = acc;
+    }
+    double s = rows[0];
+#pragma unroll
+    for (int r = 1; r < B; ++r) s = fmax(s, rows[r]);
+    s = fmax(s, shfl_xor_d(s, 1));
+    s = fmax(s, shfl_xor_d(s, 2));
+    s = fmax(s, shfl_xor_d(s, 4));
+    s = readlane_d(s, 0);
+    if (!(s > 0.0)) {
+        if (lane == 0) report_status(p.status, 1, (unsigned long long)blk + 1);
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) A[r][c] = cmk(A[r][c].x / s, A[r][c].y / s);
+
+    unsigned used = 0;  // bit r: own row lr*B+r already pivoted
+    bool singular = false;
+    for (int ck = 0; ck < 8 && !singular; ++ck) {
+#pragma unroll
+        for (int kk = 0; kk < B; ++kk) {
+            const int k = ck * B + kk;
+            if (k >= N || singular) break;
+            // Pivot column into LDS and the local pivot candidate.
+            double mag = -1.0;
+            int idx = INT_MAX;
+            if (lc == ck) {
+#pragma unroll
+                for (int r = 0; r < B; ++r) {
+                    const int i = lr * B + r;
+                    if (i < N) {
+                        colbuf[i] = A[r][kk];
+                        if (!((used >> r) & 1u)) {
+                            double mg = cabs1(A[r][kk]);
+                            if (mg > mag) { mag = mg; idx = i; }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int off = 1; off < 8; off <<= 1) {
+                double om = shfl_xor_d(mag, off);
+                int oi = shfl_xor_i(idx, off);
+                if (om > mag || (om == mag && oi < idx)) { mag = om; idx = oi; }
+            }
+            const int piv = __builtin_amdgcn_readlane(idx, 8 * ck);
+            const double pmag = readlane_d(mag, 8 * ck);
+            if (!(pmag > 0.0) || piv >= N) {
+                singular = true;
+                break;
+            }
+            wave_sync();
+            const c64 pv = colbuf[piv];
+            const c64 inv = crecip(pv);
+            const int pg = piv / B, pr = piv - pg * B;
+            if (lr == pg) {
+#pragma unroll
+                for (int r = 0; r < B; ++r)
+                    if (r == pr) {
+#pragma unroll
+                        for (int c = 0; c < B; ++c) {
+                            const int j = lc * B + c;
+                            if (j < N) rowbuf[j] = cmul(A[r][c], inv);
+                        }
+                    }
+            }
+            wave_sync();
+            c64 f[B], R[B];
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                const int i = lr * B + r;
+                f[r] = (i < N) ? colbuf[i] : cmk(0.0, 0.0);
+            }
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int j = lc * B + c;
+                R[c] = (j < N) ? rowbuf[j] : cmk(0.0, 0.0);
+            }
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int c = 0; c < B; ++c) A[r][c] = cfms(A[r][c], f[r], R[c]);
+            // Pivot row <- scaled pivot row; pivot column <- -f/pivot; pivot <- 1/pivot.
+            if (lr == pg) {
+#pragma unroll
+                for (int r = 0; r < B; ++r)
+                    if (r == pr) {
+#pragma unroll
+                        for (int c = 0; c < B; ++c) A[r][c] = R[c];
+                        used |= 1u << r;
+                    }
+            }
+            if (lc == ck) {
+#pragma unroll
+                for (int r = 0; r < B; ++r) {
+                    c64 t = cmul(f[r], inv);
+                    A[r][kk] = (lr == pg && r == pr) ? inv : cmk(-t.x, -t.y);
+                }
+            }
+            if (lane == 0) {
+                pivrow[k] = piv;
+                steprow[piv] = k;
+            }
+            wave_sync();
+        }
+    }
+    if (singular) {
+        if (lane == 0) report_status(p.status, 1, (unsigned long long)blk + 1);
+        return;
+    }
+    wave_sync();
+    // Undo the implicit pivoting: inverse[k(i)][p(j)] = M[i][j]; G = inverse / s.
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr * B + r, j = lc * B + c;
+            if (i < N && j < N) Gs[steprow[i] * LD + pivrow[j]] = cmk(A[r][c].x / s, A[r][c].y / s);
+        }
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr * B + r, j = lc * B + c;
+            if (i < N && j < N) p.GR[blk + S * (i + (size_t)N * j)] = Gs[i * LD + j];
+        }
+    if (lane == 0 && p.scale) p.scale[blk] = s;
+    if (!p.keldysh) return;
+
+    for (int comp = 0; comp < 2; ++comp) {
+        const c64* src = comp ? p.sigG : p.sigL;
+        c64* dst = comp ? p.GG : p.GL;
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int i = lr * B + r, j = lc * B + c;
+                if (i < N && j < N) Xs[i * LD + j] = src[blk + S * (i + (size_t)N * j)];
+            }
+        wave_sync();
+        c64 T[B][B];
+        block_product<B, false>(Gs, Xs, N, lr, lc, T);
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) Xs[(lr * B + r) * LD + lc * B + c] = T[r][c];
+        wave_sync();
+        block_product<B, true>(Xs, Gs, N, lr, lc, T);
+        c64 tr = cmk(0.0, 0.0);
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int i = lr * B + r, j = lc * B + c;
+                if (i < N && j < N) {
+                    dst[blk + S * (i + (size_t)N * j)] = T[r][c];
+                    if (i == j) tr = cadd(tr, T[r][c]);
+                }
+            }
+        if (p.traces) {
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                tr.x += shfl_xor_d(tr.x, off);
+                tr.y += shfl_xor_d(tr.y, off);
+            }
+            if (lane == 0) p.traces[(size_t)comp * nblk + blk] = tr;
+        }
+        wave_sync();
+    }
+}
+
+// Batched sandwich out_em = T (S X)_em T^dagger with a constant T (the
+// embedding components, production_operators.jl:886-922). Same lane grid and
+// LDS products as the Keldysh step; zero outside the shift window.
+struct SWArgs {
+    int NE, Nk, N;
+    const c64* T;
+    const c64* X;
+    const int* left;
+    const int* right;
+    const double* wr;
+    const unsigned char* valid;
+    c64* out;
+};
+
+template <int B, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_sandwich(SWArgs p) {
+    constexpr int NP = DKShape<B>::NP;
+    constexpr int LD = DKShape<B>::LD;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const long long nblk = (long long)p.NE * p.Nk;
+    const long long blk = (long long)blockIdx.x * WAVES + wave;
+    if (blk >= nblk) return;
+    c64* Gs = reinterpret_cast<c64*>(smem + (size_t)wave * DKShape<B>::bytes);
+    c64* Xs = Gs + NP * LD;
+    const int N = p.N;
+    const int e = (int)(blk % p.NE);
+    const size_t S = (size_t)nblk;
+    const int lr = lane & 7, lc = lane >> 3;
+    if (p.valid && !p.valid[e]) {
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int i = lr * B + r, j = lc * B + c;
+                if (i < N && j < N) p.out[blk + S * (i + (size_t)N * j)] = cmk(0.0, 0.0);
+            }
+        return;
+    }
+    int l = e, rr = e;
+    double th = 0.0;
+    if (p.valid) {
+        l = p.left[e];
+        rr = p.right[e];
+        th = p.wr[e];
+    }
+    const long long bl = blk - e + l, br = blk - e + rr;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr * B + r, j = lc * B + c;
+            if (i < N && j < N) {
+                Gs[i * LD + j] = p.T[i + (size_t)N * j];
+                const size_t off = S * (i + (size_t)N * j);
+                c64 xl = p.X[bl + off];
+                c64 v = xl;
+                if (!(l == rr || th == 0.0)) {
+                    c64 xr = p.X[br + off];
+                    v = cmk((1.0 - th) * xl.x + th * xr.x, (1.0 - th) * xl.y + th * xr.y);
+                }
+                Xs[i * LD + j] = v;
+            }
+        }
+    wave_sync();
+    c64 Tm[B][B];
+    block_product<B, false>(Gs, Xs, N, lr, lc, Tm);
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) Xs[(lr * B + r) * LD + lc * B + c] = Tm[r][c];
+    wave_sync();
+    block_product<B, true>(Xs, Gs, N, lr, lc, Tm);
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr * B + r, j = lc * B + c;
+            if (i < N && j < N) p.out[blk + S * (i + (size_t)N * j)] = Tm[r][c];
+        }
+}
+
+// Deterministic reduction of the weighted block traces in the reference's
+// term order linear = e*Nk + m (production_operators.jl:1426-1440), then the
+// realness gates, lambda and the paired coefficients (production_state.jl:
+// 49-70, reference/greens.jl:155-185). One workgroup; fixed partition.
+constexpr int kReduceThreads = 1024;
+
+__global__ __launch_bounds__(kReduceThreads) void k_number_pair(const c64* traces, int ncomp,
+                                                                  int NE, int Nk, const double* wE,
+                                                                  const double* wk, int gs,
+                                                                  double target, int pair,
+                                                                  PairCoefficients* coef,
+                                                                  DeviceStatus* st) {
+    __shared__ double red[2][2][kReduceThreads];
+    const long long nblk = (long long)NE * Nk;
+    const int t = threadIdx.x;
+    const long long per = (nblk + kReduceThreads - 1) / kReduceThreads;
+    const long long lo = t * per, hi = min(nblk, lo + per);
+    for (int comp = 0; comp < ncomp; ++comp) {
+        double sx = 0.0, sy = 0.0;
+        for (long long linear = lo; linear < hi; ++linear) {
+            int e = (int)(linear / Nk), mm = (int)(linear % Nk);
+            c64 tr = traces[(size_t)comp * nblk + e + (size_t)NE * mm];
+            double w = wE[e] * wk[mm];
+            sx += w * tr.x;
+            sy += w * tr.y;
+        }
+        red[comp][0][t] = sx;
+        red[comp][1][t] = sy;
+    }
+    __syncthreads();
+    for (int width = kReduceThreads / 2; width > 0; width >>= 1) {
+        if (t < width)
+            for (int comp = 0; comp < ncomp; ++comp) {
+                red[comp][0][t] += red[comp][0][t + width];
+                red[comp][1][t] += red[comp][1][t + width];
+            }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    const double k = -(double)gs / (2.0 * M_PI);  // value *= -i g_s/(2 pi)
+    double val[2], imv[2];
+    for (int comp = 0; comp < ncomp; ++comp) {
+        double x = red[comp][0][0], y = red[comp][1][0];
+        val[comp] = -y * k;  // Re[(x + iy)(0 + ik)]
+        imv[comp] = x * k;
+        if (!(fabs(imv[comp]) <= 1.0e-10 * fmax(fabs(val[comp]), 1.0))) {
+            report_status(st, QCLHIP_ERR_DOMAIN, 0);
+        }
+    }
+    PairCoefficients out;
+    out.number = val[0];
+    out.holes = ncomp > 1 ? -val[1] : 0.0;
+    out.lambda = out.number / target;
+    out.a = 1.0;
+    out.c = 0.0;
+    if (pair) {
+        if (!(isfinite(out.lambda) && out.lambda > 0)) report_status(st, QCLHIP_ERR_DOMAIN, 0);
+        double number = out.number, holes = out.holes;
+        if (!(isfinite(number) && number > 0) || !(isfinite(holes) && holes >= 0) ||
+            !(target <= number + holes)) {
+            report_status(st, QCLHIP_ERR_DOMAIN, 0);
+        } else if (number >= target) {
+            out.a = target / number;
+            out.c = 0.0;
+        } else {
+            out.a = 1.0;
+            out.c = (target - number) / holes;
+        }
+    }
+    *coef = out;
+}
+
+// Block traces of one field (for the standalone number functional).
+__global__ void k_block_traces(const c64* G, long long nblk, int N, c64* traces) {
+    long long blk = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= nblk) return;
+    c64 tr = cmk(0.0, 0.0);
+    for (int a = 0; a < N; ++a) tr = cadd(tr, G[blk + (size_t)nblk * (a + (size_t)N * a)]);
+    traces[blk] = tr;
+}
+
+__global__ void k_normalize_pair(long long n, c64* __restrict__ GL, c64* __restrict__ GG,
+                                 const PairCoefficients* coef, double a_host, double c_host) {
+    const double a = coef ? coef->a : a_host;
+    const double c = coef ? coef->c : c_host;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        c64 l = GL[i], g = GG[i];
+        GL[i] = cmk(a * l.x - c * g.x, a * l.y - c * g.y);
+        GG[i] = cmk((1.0 - c) * g.x - (1.0 - a) * l.x, (1.0 - c) * g.y - (1.0 - a) * l.y);
+    }
+}
+
+// A_em(a,b) = i (G(a,b) - conj(G(b,a))); consecutive threads = consecutive energies.
+__global__ void k_spectral(long long nblk, int N, const c64* __restrict__ GR, c64* __restrict__ A) {
+    long long n = nblk * N * N;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        long long blk = idx % nblk, ab = idx / nblk;
+        int a = (int)(ab % N), b = (int)(ab / N);
+        c64 g = GR[idx], gt = GR[blk + nblk * (b + (long long)N * a)];
+        double dx = g.x - gt.x, dy = g.y + gt.y;  // G - conj(G^T)
+        A[idx] = cmk(-dy, dx);                    // i * (dx + i dy)
+    }
+}
+
+// G> = (G - G^dagger) + G< and A = i (G - G^dagger), either optional.
+__global__ void k_green_state(long long nblk, int N, const c64* __restrict__ GR, const c64* __restrict__ GL,
+                              c64* __restrict__ GG, c64* __restrict__ A) {
+    long long n = nblk * N * N;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        long long blk = idx % nblk, ab = idx / nblk;
+        int a = (int)(ab % N), b = (int)(ab / N);
+        c64 g = GR[idx], gt = GR[blk + nblk * (b + (long long)N * a)];
+        double dx = g.x - gt.x, dy = g.y + gt.y;
+        if (GG) {
+            c64 l = GL[idx];
+            GG[idx] = cmk(dx + l.x, dy + l.y);
+        }
+        if (A) A[idx] = cmk(-dy, dx);
+    }
+}
+
+// G< = -f (G(a,b) - conj(G(b,a))) (seed.jl:113-126).
+__global__ void k_seed_lesser(int NE, long long nblk, int N, const c64* __restrict__ GR,
+                              const double* eps, double mu, double kBT, c64* __restrict__ GL) {
+    long long n = nblk * N * N;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        long long blk = idx % nblk, ab = idx / nblk;
+        int a = (int)(ab % N), b = (int)(ab / N);
+        int e = (int)(blk % NE);
+        double x = (eps[e] - mu) / kBT;
+        double f = x > 40.0 ? 0.0 : (x < -40.0 ? 1.0 : 1.0 / (exp(x) + 1.0));
+        c64 g = GR[idx], gt = GR[blk + nblk * (b + (long long)N * a)];
+        GL[idx] = cmk(-f * (g.x - gt.x), -f * (g.y + gt.y));
+    }
+}
+
+// q_e = g_s/(2 pi) wE_e sum_m wk_m sum_a Re[i (z - conj z)] (seed.jl:52-64).
+__global__ void k_seed_weights(int NE, int Nk, int N, const c64* __restrict__ GR, const double* wE,
+                               const double* wk, int gs, double* q) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NE) return;
+    const long long nblk = (long long)NE * Nk;
+    double msum = 0.0;
+    for (int mm = 0; mm < Nk; ++mm) {
+        double trA = 0.0;
+        for (int a = 0; a < N; ++a) {
+            c64 z = GR[e + (long long)NE * mm + nblk * (a + (long long)N * a)];
+            trA += -(z.y - (-z.y));  // Re[i (z - conj z)] = -(2 Im z)
+        }
+        msum += wk[mm] * trA;
+    }
+    q[e] = ((double)gs / (2.0 * M_PI)) * wE[e] * msum;
+}
+
+// out = x0 + x1 + ... (up to 16 addends per launch, in order).
+struct SumArgs {
+    const c64* x[16];
+    int nx;
+};
+
+__global__ void k_sum_fields(long long n, SumArgs a, int accumulate, c64* __restrict__ out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        c64 v = accumulate ? out[i] : a.x[0][i];
+        for (int f = accumulate ? 0 : 1; f < a.nx; ++f) v = cadd(v, a.x[f][i]);
+        out[i] = v;
+    }
+}
+
+__global__ void k_mix(long long n, c64* __restrict__ old_, const c64* __restrict__ cand, double alpha) {
+    const double beta = 1.0 - alpha;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        c64 o = old_[i], c = cand[i];
+        old_[i] = cmk(beta * o.x + alpha * c.x, beta * o.y + alpha * c.y);
+    }
+}
+
+inline unsigned grid_for(long long n, int threads, int sm_count) {
+    long long blocks = (n + threads - 1) / threads;
+    long long cap = (long long)sm_count * 32;
+    if (cap < 1) cap = 1024;
+    return (unsigned)std::max<long long>(1, std::min(blocks, cap));
+}
+
+template <int B>
+int launch_dk(qclhip_ctx* ctx, const DKArgs& a) {
+    constexpr int WAVES = DKShape<B>::bytes * 2 <= 48 * 1024 ? (DKShape<B>::bytes * 4 <= 48 * 1024 ? 4 : 2) : 1;
+    const size_t lds = DKShape<B>::bytes * WAVES;
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_dyson_keldysh<B, WAVES>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long long nblk = (long long)a.NE * a.Nk;
+    const unsigned blocks = (unsigned)((nblk + WAVES - 1) / WAVES);
+    hipLaunchKernelGGL((k_dyson_keldysh<B, WAVES>), dim3(blocks), dim3(64 * WAVES), lds, ctx->stream, a);
+    return launch_check(ctx, "k_dyson_keldysh");
+}
+
+template <int B>
+int launch_sw(qclhip_ctx* ctx, const SWArgs& a) {
+    constexpr int WAVES = DKShape<B>::bytes * 2 <= 48 * 1024 ? (DKShape<B>::bytes * 4 <= 48 * 1024 ? 4 : 2) : 1;
+    const size_t lds = DKShape<B>::bytes * WAVES;
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_sandwich<B, WAVES>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long long nblk = (long long)a.NE * a.Nk;
+    const unsigned blocks = (unsigned)((nblk + WAVES - 1) / WAVES);
+    hipLaunchKernelGGL((k_sandwich<B, WAVES>), dim3(blocks), dim3(64 * WAVES), lds, ctx->stream, a);
+    return launch_check(ctx, "k_sandwich");
+}
+
+int sandwich_dispatch(qclhip_ctx* ctx, const SWArgs& a) {
+    const int N = a.N;
+    if (N <= 8) return launch_sw<1>(ctx, a);
+    if (N <= 16) return launch_sw<2>(ctx, a);
+    if (N <= 24) return launch_sw<3>(ctx, a);
+    if (N <= 32) return launch_sw<4>(ctx, a);
+    if (N <= 40) return launch_sw<5>(ctx, a);
+    if (N <= 48) return launch_sw<6>(ctx, a);
+    return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
+}
+
+int dyson_keldysh_dispatch(qclhip_ctx* ctx, const DKArgs& a) {
+    const int N = a.N;
+    if (N <= 8) return launch_dk<1>(ctx, a);
+    if (N <= 16) return launch_dk<2>(ctx, a);
+    if (N <= 24) return launch_dk<3>(ctx, a);
+    if (N <= 32) return launch_dk<4>(ctx, a);
+    if (N <= 40) return launch_dk<5>(ctx, a);
+    if (N <= 48) return launch_dk<6>(ctx, a);
+    return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
+}
+
+}  // namespace qclhip
+
+using namespace qclhip;
+
+extern "C" {
+
+int qclhip_max_basis(void) { return 48; }
+
+int qclhip_dyson_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* sigR,
+                         const double* sigL, const double* sigG, double eta, double* GR, double* GL,
+                         double* GG, double* scale, double* traces) {
+    if (!ctx || !grid || !h || !GR) return QCLHIP_ERR_ARGUMENT;
+    const bool keldysh = sigL || sigG || GL || GG;
+    if (keldysh && !(sigL && sigG && GL && GG))
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "Keldysh products need sigL, sigG, GL and GG");
+    const long long nblk = (long long)grid->NE * grid->Nk;
+    if (keldysh && !traces) {
+        traces = (double*)workspace(ctx, (size_t)2 * nblk * sizeof(c64));
+        if (!traces) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    }
+    DKArgs a;
+    a.NE = grid->NE;
+    a.Nk = grid->Nk;
+    a.N = grid->Nb;
+    a.eps = grid->eps;
+    a.h = (const c64*)h;
+    a.sigR = (const c64*)sigR;
+    a.sigL = (const c64*)sigL;
+    a.sigG = (const c64*)sigG;
+    a.eta = eta;
+    a.GR = (c64*)GR;
+    a.GL = (c64*)GL;
+    a.GG = (c64*)GG;
+    a.scale = scale;
+    a.traces = (c64*)traces;
+    a.status = ctx->status;
+    a.keldysh = keldysh ? 1 : 0;
+    return dyson_keldysh_dispatch(ctx, a);
+}
+
+int qclhip_retarded_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* sigR,
+                          double eta, double* GR, double* scale) {
+    return qclhip_dyson_keldysh(ctx, grid, h, sigR, nullptr, nullptr, eta, GR, nullptr, nullptr, scale,
+                                nullptr);
+}
+
+int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
+                         const double* const* famR, const double* const* famL, const double* const* famG,
+                         double target, double* GR, double* GL, double* GG, double* A, double* scale,
+                         double* out_host) {
+    if (!ctx || !grid || !h || !famR || !famL || !famG || !GR || !GL || !GG) return QCLHIP_ERR_ARGUMENT;
+    if (nfam < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "at least the embedding family is required");
+    if (!(std::isfinite(target) && target > 0))
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "target charge must be finite and positive");
+    const long long nblk = (long long)grid->NE * grid->Nk;
+    const long long n = nblk * grid->Nb * grid->Nb;
+    // Scratch: traces, then (when nfam > 1) the three total fields.
+    size_t tr_bytes = (size_t)2 * nblk * sizeof(c64);
+    size_t tot_bytes = nfam > 1 ? (size_t)3 * n * sizeof(c64) : 0;
+    char* ws = (char*)workspace(ctx, tr_bytes + tot_bytes);
+    if (!ws) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    c64* traces = (c64*)ws;
+    const c64 *tR = (const c64*)famR[0], *tL = (const c64*)famL[0], *tG = (const c64*)famG[0];
+    if (nfam > 1) {
+        c64* tot = (c64*)(ws + tr_bytes);
+        const double* const* fams[3] = {famR, famL, famG};
+        for (int k = 0; k < 3; ++k) {
+            int done = 0;
+            while (done < nfam) {
+                SumArgs sa;
+                sa.nx = std::min(16, nfam - done);
+                for (int f = 0; f < sa.nx; ++f) sa.x[f] = (const c64*)fams[k][done + f];
+                hipLaunchKernelGGL(k_sum_fields, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
+                                   n, sa, done > 0 ? 1 : 0, tot + (size_t)k * n);
+                done += sa.nx;
+            }
+        }
+        int rc = launch_check(ctx, "k_sum_fields");
+        if (rc) return rc;
+        tR = tot;
+        tL = tot + n;
+        tG = tot + 2 * n;
+    }
+    DKArgs a;
+    a.NE = grid->NE;
+    a.Nk = grid->Nk;
+    a.N = grid->Nb;
+    a.eps = grid->eps;
+    a.h = (const c64*)h;
+    a.sigR = tR;
+    a.sigL = tL;
+    a.sigG = tG;
+    a.eta = 0.0;
+    a.GR = (c64*)GR;
+    a.GL = (c64*)GL;
+    a.GG = (c64*)GG;
+    a.scale = scale;
+    a.traces = traces;
+    a.status = ctx->status;
+    a.keldysh = 1;
+    int rc = dyson_keldysh_dispatch(ctx, a);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_number_pair, dim3(1), dim3(kReduceThreads), 0, ctx->stream, traces, 2, grid->NE,
+                       grid->Nk, grid->wE, grid->wk, grid->gs, target, 1, ctx->coef, ctx->status);
+    hipLaunchKernelGGL(k_normalize_pair, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, n,
+                       (c64*)GL, (c64*)GG, (const PairCoefficients*)ctx->coef, 0.0, 0.0);
+    if (A)
+        hipLaunchKernelGGL(k_spectral, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, nblk,
+                           grid->Nb, (const c64*)GR, (c64*)A);
+    rc = launch_check(ctx, "current_green");
+    if (rc) return rc;
+    if (out_host) {
+        QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, ctx->coef, sizeof(PairCoefficients),
+                                           hipMemcpyDeviceToHost, ctx->stream));
+        rc = qclhip_ctx_synchronize(ctx);
+        if (rc) return rc;
+        const PairCoefficients* pc = (const PairCoefficients*)ctx->host_scalars;
+        out_host[0] = pc->lambda;
+        out_host[1] = pc->number;
+        out_host[2] = pc->holes;
+        out_host[3] = pc->a;
+        out_host[4] = pc->c;
+    }
+    return QCLHIP_OK;
+}
+
+int qclhip_number_functional(qclhip_ctx* ctx, const qclhip_grid* grid, const double* G, double* value_host) {
+    if (!ctx || !grid || !G || !value_host) return QCLHIP_ERR_ARGUMENT;
+    const long long nblk = (long long)grid->NE * grid->Nk;
+    c64* traces = (c64*)workspace(ctx, (size_t)nblk * sizeof(c64));
+    if (!traces) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    hipLaunchKernelGGL(k_block_traces, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const c64*)G, nblk, grid->Nb, traces);
+    hipLaunchKernelGGL(k_number_pair, dim3(1), dim3(kReduceThreads), 0, ctx->stream, traces, 1, grid->NE,
+                       grid->Nk, grid->wE, grid->wk, grid->gs, 1.0, 0, ctx->coef, ctx->status);
+    int rc = launch_check(ctx, "number_functional");
+    if (rc) return rc;
+    QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, ctx->coef, sizeof(PairCoefficients),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+    rc = qclhip_ctx_synchronize(ctx);
+    if (rc) return rc;
+    *value_host = ((const PairCoefficients*)ctx->host_scalars)->number;
+    return QCLHIP_OK;
+}
+
+int qclhip_spectral(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR, double* A) {
+    if (!ctx || !grid || !GR || !A) return QCLHIP_ERR_ARGUMENT;
+    if (GR == A) return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
+    const long long nblk = (long long)grid->NE * grid->Nk;
+    const long long n = nblk * grid->Nb * grid->Nb;
+    hipLaunchKernelGGL(k_spectral, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, nblk,
+                       grid->Nb, (const c64*)GR, (c64*)A);
+    return launch_check(ctx, "k_spectral");
+}
+
+int qclhip_green_state(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR, const double* GL, double* GG,
+                       double* A) {
+    if (!ctx || !grid || !GR || (GG && !GL)) return QCLHIP_ERR_ARGUMENT;
+    if (GG == GR || A == GR || (GG && GG == A)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
+    const long long nblk = (long long)grid->NE * grid->Nk;
+    const long long n = nblk * grid->Nb * grid->Nb;
+    hipLaunchKernelGGL(k_green_state, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, nblk,
+                       grid->Nb, (const c64*)GR, (const c64*)GL, (c64*)GG, (c64*)A);
+    return launch_check(ctx, "k_green_state");
+}
+
+int qclhip_normalize_pair(qclhip_ctx* ctx, int64_t n, double* GL, double* GG, double a, double c) {
+    if (!ctx || !GL || !GG || n < 0) return QCLHIP_ERR_ARGUMENT;
+    if (n == 0) return QCLHIP_OK;
+    hipLaunchKernelGGL(k_normalize_pair, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
+                       (long long)n, (c64*)GL, (c64*)GG, (const PairCoefficients*)nullptr, a, c);
+    return launch_check(ctx, "k_normalize_pair");
+}
+
+int qclhip_seed_weights(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR, double* q_host) {
+    if (!ctx || !grid || !GR || !q_host) return QCLHIP_ERR_ARGUMENT;
+    double* q = (double*)workspace(ctx, (size_t)grid->NE * sizeof(double));
+    if (!q) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    hipLaunchKernelGGL(k_seed_weights, dim3((grid->NE + 255) / 256), dim3(256), 0, ctx->stream, grid->NE,
+                       grid->Nk, grid->Nb, (const c64*)GR, grid->wE, grid->wk, grid->gs, q);
+    int rc = launch_check(ctx, "k_seed_weights");
+    if (rc) return rc;
+    QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(q_host, q, (size_t)grid->NE * sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+    rc = qclhip_ctx_synchronize(ctx);
+    if (rc) return rc;
+    for (int e = 0; e < grid->NE; ++e)
+        if (!std::isfinite(q_host[e])) return fail(ctx, QCLHIP_ERR_DOMAIN, "seed spectral number weights must be finite");
+    return QCLHIP_OK;
+}
+
+int qclhip_seed_lesser(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR, double mu, double kBT,
+                       double* GL) {
+    if (!ctx || !grid || !GR || !GL) return QCLHIP_ERR_ARGUMENT;
+    if (!(kBT > 0)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "kBT must be positive");
+    const long long nblk = (long long)grid->NE * grid->Nk;
+    const long long n = nblk * grid->Nb * grid->Nb;
+    hipLaunchKernelGGL(k_seed_lesser, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
+                       grid->NE, nblk, grid->Nb, (const c64*)GR, grid->eps, mu, kBT, (c64*)GL);
+    return launch_check(ctx, "k_seed_lesser");
+}
+
+int qclhip_sum_fields(qclhip_ctx* ctx, int64_t n, int nx, const double* const* x, double* out) {
+    if (!ctx || !x || !out || nx < 1 || n < 0) return QCLHIP_ERR_ARGUMENT;
+    int done = 0;
+    while (done < nx) {
+        SumArgs sa;
+        sa.nx = std::min(16, nx - done);
+        for (int f = 0; f < sa.nx; ++f) sa.x[f] = (const c64*)x[done + f];
+        hipLaunchKernelGGL(k_sum_fields, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
+                           (long long)n, sa, done > 0 ? 1 : 0, (c64*)out);
+        done += sa.nx;
+    }
+    return launch_check(ctx, "k_sum_fields");
+}
+
+int qclhip_mix(qclhip_ctx* ctx, int64_t n, double* old_, const double* cand, double alpha) {
+    if (!ctx || !old_ || !cand || n < 0) return QCLHIP_ERR_ARGUMENT;
+    if (!(alpha > 0 && alpha <= 1)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "mixing factor must lie in (0,1]");
+    hipLaunchKernelGGL(k_mix, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, (long long)n,
+                       (c64*)old_, (const c64*)cand, alpha);
+    return launch_check(ctx, "k_mix");
+}
+
+int qclhip_embedding_component(qclhip_ctx* ctx, const qclhip_grid* grid, const double* T, const double* X,
+                               const qclhip_shift* shift, double* S) {
+    if (!ctx || !grid || !T || !X || !shift || !S) return QCLHIP_ERR_ARGUMENT;
+    if (shift->NE != grid->NE) return fail(ctx, QCLHIP_ERR_DIMENSION, "shift plan has the wrong energy size");
+    if (X == S) return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
+    SWArgs a;
+    a.NE = grid->NE;
+    a.Nk = grid->Nk;
+    a.N = grid->Nb;
+    a.T = (const c64*)T;
+    a.X = (const c64*)X;
+    a.left = shift->left;
+    a.right = shift->right;
+    a.wr = shift->wr;
+    a.valid = shift->valid;
+    a.out = (c64*)S;
+    return sandwich_dispatch(ctx, a);
+}
+
+}  // extern "C"

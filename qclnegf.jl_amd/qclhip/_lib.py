@@ -1,0 +1,137 @@
+"""ctypes binding of libqclhip.so (the C-ABI declared in include/qclhip.h).
+
+The library is required: importing this module on a machine without the
+built library raises immediately; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libqclhip.so"))
+
+OK = 0
+ERR_ARGUMENT = -1
+ERR_DIMENSION = -2
+ERR_DOMAIN = -3
+ERR_RUNTIME = -4
+ERR_UNSUPPORTED = -5
+
+KERNEL_DENSE = 0
+KERNEL_MOMENTUM_INDEPENDENT = 1
+KERNEL_SEPARABLE_IFR = 2
+KERNEL_LOW_RANK = 3
+
+
+class QCLHipError(RuntimeError):
+    """Base class; the subclasses mirror the Julia exception types."""
+
+
+class SingularException(QCLHipError):
+    def __init__(self, block: int, msg: str = ""):
+        super().__init__(f"singular Dyson block {block} {msg}".strip())
+        self.block = block
+
+
+class ArgumentError(QCLHipError):
+    pass
+
+
+class DimensionMismatch(QCLHipError):
+    pass
+
+
+class DomainError(QCLHipError):
+    pass
+
+
+class UnsupportedError(QCLHipError):
+    pass
+
+
+_vp = ct.c_void_p
+_dp = ct.POINTER(ct.c_double)
+_i64 = ct.c_int64
+
+_SIGNATURES = {
+    "qclhip_ctx_create": [ct.c_int, _vp, ct.POINTER(_vp)],
+    "qclhip_ctx_destroy": [_vp],
+    "qclhip_ctx_synchronize": [_vp],
+    "qclhip_ctx_last_error": [_vp],
+    "qclhip_max_basis": [],
+    "qclhip_grid_create": [_vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, ct.POINTER(_vp)],
+    "qclhip_grid_destroy": [_vp],
+    "qclhip_retarded_green": [_vp, _vp, _vp, _vp, ct.c_double, _vp, _vp],
+    "qclhip_dyson_keldysh": [_vp, _vp, _vp, _vp, _vp, _vp, ct.c_double, _vp, _vp, _vp, _vp, _vp],
+    "qclhip_current_green": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),
+                             ct.c_double, _vp, _vp, _vp, _vp, _vp, _dp],
+    "qclhip_number_functional": [_vp, _vp, _vp, _dp],
+    "qclhip_spectral": [_vp, _vp, _vp, _vp],
+    "qclhip_green_state": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "qclhip_normalize_pair": [_vp, _i64, _vp, _vp, ct.c_double, ct.c_double],
+    "qclhip_shift_create": [_vp, _vp, ct.c_double, ct.POINTER(_vp)],
+    "qclhip_shift_destroy": [_vp],
+    "qclhip_apply_shift": [_vp, _vp, _i64, _vp, _vp],
+    "qclhip_kernel_create": [_vp, _vp, ct.c_int, ct.c_int, _dp, _dp, _dp, ct.POINTER(_vp)],
+    "qclhip_kernel_destroy": [_vp],
+    "qclhip_contract": [_vp, _vp, _vp, ct.c_double, ct.c_double, ct.c_double, _vp],
+    "qclhip_lo_contraction": [_vp, _vp, _vp, _vp, ct.c_double, ct.c_double, _vp, _vp, _vp, _vp],
+    "qclhip_hilbert_create": [_vp, _vp, _i64, ct.c_int, ct.POINTER(_vp)],
+    "qclhip_hilbert_destroy": [_vp],
+    "qclhip_fft_hilbert": [_vp, _vp, _vp, _vp, _dp],
+    "qclhip_retarded_self_energy": [_vp, _vp, _vp, _vp, _vp, _vp, _dp],
+    "qclhip_embedding_component": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "qclhip_sum_fields": [_vp, _i64, ct.c_int, ct.POINTER(_vp), _vp],
+    "qclhip_mix": [_vp, _i64, _vp, _vp, ct.c_double],
+    "qclhip_boundary_flux": [_vp, _vp, _vp, _vp, _vp, _vp, _dp],
+    "qclhip_state_populations": [_vp, _vp, _vp, ct.c_double, _dp],
+    "qclhip_seed_weights": [_vp, _vp, _vp, _dp],
+    "qclhip_seed_lesser": [_vp, _vp, _vp, ct.c_double, ct.c_double, _vp],
+}
+
+_lib = None
+
+
+def declared_symbols():
+    return sorted(_SIGNATURES)
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (without touching the GPU) and set the signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"libqclhip.so not found at {path}; build it with `python qclnegf.jl_amd/build.py` "
+            "(the HIP backend has no CPU fallback)")
+    lib = ct.CDLL(path)
+    for name, argtypes in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = ct.c_char_p if name == "qclhip_ctx_last_error" else ct.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc: int, ctx=None, what: str = "") -> None:
+    """Map a C-ABI return code to the reference's exception types."""
+    if rc == OK:
+        return
+    msg = ""
+    if ctx is not None and _lib is not None:
+        raw = _lib.qclhip_ctx_last_error(ctx)
+        msg = raw.decode() if raw else ""
+    text = f"{what}: {msg}" if what else msg
+    if rc > 0:
+        raise SingularException(rc, f"({what})")
+    if rc == ERR_ARGUMENT:
+        raise ArgumentError(text)
+    if rc == ERR_DIMENSION:
+        raise DimensionMismatch(text)
+    if rc == ERR_DOMAIN:
+        raise DomainError(text)
+    if rc == ERR_UNSUPPORTED:
+        raise UnsupportedError(text)
+    raise QCLHipError(f"runtime failure ({rc}) {text}")

@@ -1,0 +1,473 @@
+"""Host-side mirror of the reference operator interface over the qclhip C-ABI.
+
+Names, argument meaning and error behaviour follow the production operators
+of QCLNEGF.jl (src/numerics/optimized/production_*.jl); fields live in device
+memory as torch complex128 tensors whose memory is the reference layout
+(NE,Nk,Nb,Nb) column-major, i.e. torch shape (Nb, Nb, Nk, NE) contiguous.
+torch supplies device memory and the stream; all arithmetic runs in the HIP
+kernels of libqclhip.so.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+
+_vp = ct.c_void_p
+_dp = ct.POINTER(ct.c_double)
+
+
+def _ptr(t):
+    return None if t is None else ct.c_void_p(t.data_ptr())
+
+
+def _host(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _chost(a):
+    """Complex host matrix (column-major) as an interleaved double pointer."""
+    a = np.asfortranarray(a, dtype=np.complex128)
+    return a, a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One qclhip context bound to a device and to torch's current stream."""
+
+    def __init__(self, device: int = 0):
+        lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError("qclhip needs a ROCm GPU; none is visible")
+        self.device = torch.device("cuda", device)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        h = _vp()
+        check(lib.qclhip_ctx_create(device, ct.c_void_p(stream), ct.byref(h)), None, "qclhip_ctx_create")
+        self.handle = h
+        self.lib = lib
+
+    def synchronize(self):
+        check(self.lib.qclhip_ctx_synchronize(self.handle), self.handle, "synchronize")
+
+    def close(self):
+        if self.handle:
+            self.lib.qclhip_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name, *args):
+        rc = getattr(self.lib, name)(self.handle, *args)
+        check(rc, self.handle, name)
+
+
+_default = {}
+
+
+def context(device: int = 0) -> Context:
+    if device not in _default:
+        _default[device] = Context(device)
+    return _default[device]
+
+
+# ---------------------------------------------------------------- layout helpers
+
+def field_shape(NE, Nk, Nb):
+    return (Nb, Nb, Nk, NE)
+
+
+def empty_field(grid, device=None):
+    return torch.empty(field_shape(grid.NE, grid.Nk, grid.Nb), dtype=torch.complex128,
+                       device=device or grid.ctx.device)
+
+
+def zeros_field(grid, device=None):
+    return torch.zeros(field_shape(grid.NE, grid.Nk, grid.Nb), dtype=torch.complex128,
+                       device=device or grid.ctx.device)
+
+
+def to_device(x, device="cuda"):
+    """numpy (NE,Nk,Nb,Nb) (any order) -> device tensor in the reference layout."""
+    x = np.asarray(x, dtype=np.complex128)
+    return torch.from_numpy(np.ascontiguousarray(x.T)).to(device)
+
+
+def to_host(t):
+    """Device tensor in the reference layout -> numpy (NE,Nk,Nb,Nb), Fortran order."""
+    return np.asfortranarray(t.detach().cpu().numpy().T)
+
+
+def h_to_device(h, device="cuda"):
+    """numpy h (Nk,Nb,Nb) -> device tensor (Nb,Nb,Nk)."""
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(h, np.complex128).T)).to(device)
+
+
+def matrix_to_device(T, device="cuda"):
+    """Column-major Nb x Nb complex matrix on the device (torch shape (Nb,Nb) = T^T)."""
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(T, np.complex128).T)).to(device)
+
+
+def _check_field(grid, t, name):
+    if t is None:
+        return
+    if t.dtype != torch.complex128 or not t.is_contiguous():
+        raise _lib.ArgumentError(f"{name} must be a contiguous complex128 tensor")
+    if tuple(t.shape) != field_shape(grid.NE, grid.Nk, grid.Nb):
+        raise _lib.DimensionMismatch(f"{name} has shape {tuple(t.shape)}, expected "
+                                     f"{field_shape(grid.NE, grid.Nk, grid.Nb)}")
+
+
+# ---------------------------------------------------------------- handles
+
+class Grid:
+    """ModelGrids on the device (reference/grids.jl:10-47) plus Nb and g_s."""
+
+    def __init__(self, eps, wE, wk, Nb, gs=2, ctx: Context | None = None):
+        self.ctx = ctx or context()
+        self.eps = np.ascontiguousarray(eps, np.float64)
+        self.wE = np.ascontiguousarray(wE, np.float64)
+        self.wk = np.ascontiguousarray(wk, np.float64)
+        self.NE, self.Nk, self.Nb, self.gs = len(self.eps), len(self.wk), int(Nb), int(gs)
+        if len(self.wE) != self.NE:
+            raise _lib.DimensionMismatch("energy data differ")
+        h = _vp()
+        self.ctx.call("qclhip_grid_create", self.NE, self.Nk, self.Nb, self.gs,
+                      self.eps.ctypes.data_as(_dp), self.wE.ctypes.data_as(_dp),
+                      self.wk.ctypes.data_as(_dp), ct.byref(h))
+        self.handle = h
+
+    @property
+    def nfield(self):
+        return self.NE * self.Nk * self.Nb * self.Nb
+
+    @property
+    def ncols(self):
+        return self.Nk * self.Nb * self.Nb
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.ctx.lib.qclhip_grid_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class EnergyShiftPlan:
+    """build_shift_plan(eps, delta) on the device (production_cache.jl:258-288)."""
+
+    def __init__(self, grid: Grid, delta: float):
+        self.grid, self.delta = grid, float(delta)
+        h = _vp()
+        grid.ctx.call("qclhip_shift_create", grid.handle, ct.c_double(delta), ct.byref(h))
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.grid.ctx.lib.qclhip_shift_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+_KINDS = {"dense": _lib.KERNEL_DENSE, "momentum_independent": _lib.KERNEL_MOMENTUM_INDEPENDENT,
+          "separable_ifr": _lib.KERNEL_SEPARABLE_IFR, "low_rank": _lib.KERNEL_LOW_RANK}
+
+
+class ProductionKernel:
+    """A scattering-kernel representation (production_contracts.jl:262-323).
+
+    dense: a = Kflat (Nc x Nc); momentum_independent: a = block (Nb^2 x Nb^2);
+    separable_ifr: a = left, b = right, c = radial; low_rank: a = left,
+    b = right_adjoint. Host matrices, column-major semantics (numpy arrays in
+    natural (row, column) indexing).
+    """
+
+    def __init__(self, grid: Grid, kind: str, a, b=None, c=None, rank: int = 0):
+        self.grid, self.kind, self.rank = grid, kind, int(rank)
+        keep = [_chost(x) if x is not None else (None, None) for x in (a, b, c)]
+        h = _vp()
+        grid.ctx.call("qclhip_kernel_create", grid.handle, _KINDS[kind], self.rank,
+                      keep[0][1], keep[1][1], keep[2][1], ct.byref(h))
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.grid.ctx.lib.qclhip_kernel_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class FFTHilbertPlan:
+    """ProductionFFTHilbertPlan (fft_parallel.jl:27-143) on the device."""
+
+    def __init__(self, grid: Grid, ncols: int | None = None, verify_roundoff: bool = True):
+        self.grid = grid
+        self.ncols = int(ncols if ncols is not None else grid.ncols)
+        self.verify = bool(verify_roundoff)
+        h = _vp()
+        grid.ctx.call("qclhip_hilbert_create", grid.handle, ct.c_int64(self.ncols), int(self.verify),
+                      ct.byref(h))
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.grid.ctx.lib.qclhip_hilbert_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- operators
+
+def retarded_green_production(grid: Grid, h, sigR=None, eta: float = 0.0, GR=None, scale=None,
+                              check_status: bool = True):
+    """_retarded_green_production (production_operators.jl:1337-1385).
+
+    Returns (GR, scale). Raises SingularException for a singular block."""
+    _check_field(grid, sigR, "sigmaR")
+    GR = empty_field(grid) if GR is None else GR
+    scale = torch.empty((grid.Nk, grid.NE), dtype=torch.float64, device=grid.ctx.device) \
+        if scale is None else scale
+    grid.ctx.call("qclhip_retarded_green", grid.handle, _ptr(h), _ptr(sigR), ct.c_double(eta),
+                  _ptr(GR), _ptr(scale))
+    if check_status:
+        grid.ctx.synchronize()
+    return GR, scale
+
+
+def dyson_keldysh(grid: Grid, h, sigR, sigL, sigG, GR=None, GL=None, GG=None, scale=None,
+                  traces=None, check_status: bool = False):
+    """Fused Dyson plus raw Keldysh products G^R S G^R^dagger (production_state.jl:14-48)."""
+    for t, n in ((sigR, "sigmaR"), (sigL, "sigmaL"), (sigG, "sigmaG")):
+        _check_field(grid, t, n)
+    GR = empty_field(grid) if GR is None else GR
+    GL = empty_field(grid) if GL is None else GL
+    GG = empty_field(grid) if GG is None else GG
+    grid.ctx.call("qclhip_dyson_keldysh", grid.handle, _ptr(h), _ptr(sigR), _ptr(sigL), _ptr(sigG),
+                  ct.c_double(0.0), _ptr(GR), _ptr(GL), _ptr(GG), _ptr(scale), _ptr(traces))
+    if check_status:
+        grid.ctx.synchronize()
+    return GR, GL, GG
+
+
+@dataclass
+class GreenState:
+    """GreenState (types.jl:733-740) with device fields."""
+    GR: torch.Tensor
+    GL: torch.Tensor
+    GG: torch.Tensor
+    A: torch.Tensor | None
+    scale: torch.Tensor
+
+
+def current_green_production(grid: Grid, h, families, target: float, state: GreenState | None = None,
+                             spectral: bool = True, return_scalars: bool = True):
+    """_current_green_production (production_state.jl:1-74).
+
+    families: list of (SR, SL, SG) device triples, the embedding total first,
+    then the scattering mechanisms in mechanism order. Returns
+    (GreenState, info) where info holds lambda, number, holes and the paired
+    coefficients (a, c) when return_scalars (the call then synchronizes)."""
+    nf = len(families)
+    for fam in families:
+        for t in fam:
+            _check_field(grid, t, "self-energy")
+    if state is None:
+        state = GreenState(empty_field(grid), empty_field(grid), empty_field(grid),
+                           empty_field(grid) if spectral else None,
+                           torch.empty((grid.Nk, grid.NE), dtype=torch.float64, device=grid.ctx.device))
+    arr = lambda k: (_vp * nf)(*[f[k].data_ptr() for f in families])
+    out = (ct.c_double * 5)()
+    grid.ctx.call("qclhip_current_green", grid.handle, _ptr(h), nf, arr(0), arr(1), arr(2),
+                  ct.c_double(target), _ptr(state.GR), _ptr(state.GL), _ptr(state.GG),
+                  _ptr(state.A if spectral else None), _ptr(state.scale),
+                  out if return_scalars else None)
+    info = dict(lam=out[0], number=out[1], holes=out[2], a=out[3], c=out[4]) if return_scalars else None
+    return state, info
+
+
+def number_functional(grid: Grid, G) -> float:
+    """_number_functional_production (production_operators.jl:1417-1445)."""
+    v = ct.c_double()
+    grid.ctx.call("qclhip_number_functional", grid.handle, _ptr(G), ct.byref(v))
+    return v.value
+
+
+def spectral_function(grid: Grid, GR, A=None):
+    A = empty_field(grid) if A is None else A
+    grid.ctx.call("qclhip_spectral", grid.handle, _ptr(GR), _ptr(A))
+    return A
+
+
+def normalize_keldysh_pair(GL, GG, a: float, c: float):
+    ctx = context(GL.device.index or 0)
+    ctx.call("qclhip_normalize_pair", ct.c_int64(GL.numel()), _ptr(GL), _ptr(GG), ct.c_double(a),
+             ct.c_double(c))
+
+
+def apply_energy_shift(plan: EnergyShiftPlan, X, Y=None):
+    """apply_energy_shift! (production_cache.jl:300-325)."""
+    Y = torch.empty_like(X) if Y is None else Y
+    ncols = X.numel() // plan.grid.NE
+    plan.grid.ctx.call("qclhip_apply_shift", plan.handle, ct.c_int64(ncols), _ptr(X), _ptr(Y))
+    return Y
+
+
+def production_static_contraction(kernel: ProductionKernel, G, qK: float = 1.0, alpha: float = 1.0,
+                                  beta: float = 0.0, S=None):
+    """_production_static_contraction! (production_operators.jl:402-658)."""
+    _check_field(kernel.grid, G, "G")
+    S = empty_field(kernel.grid) if S is None else S
+    kernel.grid.ctx.call("qclhip_contract", kernel.handle, _ptr(G), ct.c_double(qK), ct.c_double(alpha),
+                         ct.c_double(beta), _ptr(S))
+    return S
+
+
+def production_lo_contraction(kernel: ProductionKernel, GL, GG, qK: float, NLO: float,
+                              plus: EnergyShiftPlan, minus: EnergyShiftPlan, SL=None, SG=None):
+    """_production_lo_contraction with thermal N_LO (production_operators.jl:1089-1162)."""
+    SL = empty_field(kernel.grid) if SL is None else SL
+    SG = empty_field(kernel.grid) if SG is None else SG
+    kernel.grid.ctx.call("qclhip_lo_contraction", kernel.handle, _ptr(GL), _ptr(GG), ct.c_double(qK),
+                         ct.c_double(NLO), plus.handle, minus.handle, _ptr(SL), _ptr(SG))
+    return SL, SG
+
+
+def production_fft_hilbert_transform(plan: FFTHilbertPlan, Gamma, return_roundoff: bool = False,
+                                      Lambda=None):
+    """production_fft_hilbert_transform (fft_parallel.jl:247-291)."""
+    Lambda = torch.empty_like(Gamma) if Lambda is None else Lambda
+    r = ct.c_double(0.0)
+    plan.grid.ctx.call("qclhip_fft_hilbert", plan.handle, _ptr(Gamma), _ptr(Lambda),
+                       ct.byref(r) if return_roundoff else None)
+    return (Lambda, r.value) if return_roundoff else Lambda
+
+
+def retarded_self_energy_fft(plan: FFTHilbertPlan, SL, SG, return_roundoff: bool = False, SR=None,
+                             Gamma=None):
+    """_retarded_self_energy_fft_production (production_operators.jl:842-884).
+
+    Returns (SR, Gamma[, roundoff]); Gamma is only materialized when passed."""
+    SR = torch.empty_like(SL) if SR is None else SR
+    r = ct.c_double(0.0)
+    plan.grid.ctx.call("qclhip_retarded_self_energy", plan.handle, _ptr(SL), _ptr(SG), _ptr(SR),
+                       _ptr(Gamma), ct.byref(r) if return_roundoff else None)
+    return (SR, Gamma, r.value) if return_roundoff else (SR, Gamma)
+
+
+def embedding_component_production(grid: Grid, T, X, plan: EnergyShiftPlan, S=None):
+    """_embedding_component_production (production_operators.jl:886-922); T on the device."""
+    S = empty_field(grid) if S is None else S
+    grid.ctx.call("qclhip_embedding_component", grid.handle, _ptr(T), _ptr(X), plan.handle, _ptr(S))
+    return S
+
+
+def sum_fields(xs, out):
+    ctx = context(out.device.index or 0)
+    arr = (_vp * len(xs))(*[x.data_ptr() for x in xs])
+    ctx.call("qclhip_sum_fields", ct.c_int64(out.numel()), len(xs), arr, _ptr(out))
+    return out
+
+
+def mix_family_production(old, cand, alpha: float):
+    """_mix_family_production! for one component (production_operators.jl:1475-1491)."""
+    ctx = context(old.device.index or 0)
+    ctx.call("qclhip_mix", ct.c_int64(old.numel()), _ptr(old), _ptr(cand), ct.c_double(alpha))
+    return old
+
+
+def boundary_flux_bar(grid: Grid, plusL, plusG, GL, GG) -> float:
+    """_production_boundary_flux_bar (production_iteration.jl:189-201)."""
+    v = ct.c_double()
+    grid.ctx.call("qclhip_boundary_flux", grid.handle, _ptr(plusL), _ptr(plusG), _ptr(GL), _ptr(GG),
+                  ct.byref(v))
+    return v.value
+
+
+def state_populations(grid: Grid, GL, target: float):
+    """_production_state_populations (production_iteration.jl:204-215)."""
+    out = np.zeros(grid.Nb)
+    grid.ctx.call("qclhip_state_populations", grid.handle, _ptr(GL), ct.c_double(target),
+                  out.ctypes.data_as(_dp))
+    return out
+
+
+def seed_number_weights(grid: Grid, GR):
+    """_seed_number_weights_production (seed.jl:28-66); host vector q[NE]."""
+    q = np.zeros(grid.NE)
+    grid.ctx.call("qclhip_seed_weights", grid.handle, _ptr(GR), q.ctypes.data_as(_dp))
+    return q
+
+
+def seed_lesser(grid: Grid, GR, mu: float, kBT: float, GL=None):
+    """_seed_lesser_production! (seed.jl:99-128)."""
+    GL = empty_field(grid) if GL is None else GL
+    grid.ctx.call("qclhip_seed_lesser", grid.handle, _ptr(GR), ct.c_double(mu), ct.c_double(kBT), _ptr(GL))
+    return GL
+
+
+def fermi(e, mu, kBT):
+    """_fermi (reference/scba.jl:1-6), vectorized."""
+    x = (np.asarray(e, np.float64) - mu) / kBT
+    with np.errstate(over="ignore"):
+        f = 1.0 / (np.exp(x) + 1.0)
+    return np.where(x > 40.0, 0.0, np.where(x < -40.0, 1.0, f))
+
+
+def seed_chemical_potential(eps, q, kBT: float, target: float) -> float:
+    """Bisection of sum_e q_e f(e; mu) = target on [e1 - 50kT, eN + 50kT]
+    (seed.jl:175-182), down to adjacent floating-point numbers."""
+    number = lambda mu: float(np.sum(q * fermi(eps, mu, kBT))) - target
+    lo, hi = eps[0] - 50 * kBT, eps[-1] + 50 * kBT
+    flo, fhi = number(lo), number(hi)
+    if not flo * fhi <= 0:
+        raise _lib.ArgumentError("seed chemical-potential root is not bracketed")
+    if flo == 0:
+        return lo
+    if fhi == 0:
+        return hi
+    for _ in range(4000):
+        mid = lo + 0.5 * (hi - lo)
+        if mid <= lo or mid >= hi:
+            break
+        fm = number(mid)
+        if fm == 0:
+            return mid
+        if (fm < 0) == (flo < 0):
+            lo, flo = mid, fm
+        else:
+            hi, fhi = mid, fm
+    return lo if abs(flo) <= abs(fhi) else hi
+
+
+def seed_green_production(grid: Grid, h, eta: float, kBT: float, target: float, spectral: bool = True):
+    """_seed_green_production (seed.jl:151-197): Dyson with eta and Sigma = 0,
+    spectral number weights, bisection for mu, G< = -f (G^R - G^R^dagger)."""
+    GR, scale = retarded_green_production(grid, h, None, eta)
+    q = seed_number_weights(grid, GR)
+    mu = seed_chemical_potential(grid.eps, q, kBT, target)
+    GL = seed_lesser(grid, GR, mu, kBT)
+    GG, A = green_state(grid, GR, GL, spectral=spectral)
+    return GreenState(GR, GL, GG, A, scale), mu
+
+
+def green_state(grid: Grid, GR, GL, spectral: bool = True, GG=None, A=None):
+    """G> = G^R - G^A + G< and A = i(G^R - G^A) (production_operators.jl:1387-1415)."""
+    GG = empty_field(grid) if GG is None else GG
+    A = (empty_field(grid) if A is None else A) if spectral else None
+    grid.ctx.call("qclhip_green_state", grid.handle, _ptr(GR), _ptr(GL), _ptr(GG), _ptr(A))
+    return GG, A
