@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X SCBA operator backend (one GPU).
+
+A step is one GF-solve pass (A1+A2 of the production loop: Dyson inversion
+G^R = (E - h - Sigma^R)^-1, Keldysh products G^R Sigma^{<,>} G^A, number
+functionals and the paired normalization) over the synthetic batch
+N=24, NE=2048, Nk=64 (131,072 blocks) in FP64, inputs resident in HBM.
+value = GF solves/s. The same line carries the SCBA iteration rate of a
+full production iteration at the same shape, the roofline of the fused
+Dyson/Keldysh kernel (HIP-event timing of that kernel alone) and the CPU
+oracle's rate on a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "qclnegf.jl_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "GF solves/s + SCBA iter/s, FP32+refine vs FP64 at ≤1e-8 rel. current error"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gf_bench(steps, warmup):
+    import numpy as np
+    import torch
+
+    from qclhip import api, synthetic as syn
+
+    w = syn.gf_workload()
+    N, NE, Nk = w["N"], w["NE"], w["Nk"]
+    nblk = NE * Nk
+    g = api.Grid(w["eps"], w["wE"], w["wk"], N)
+    h = api.h_to_device(w["h"])
+    SR, SL, SG = syn.diag_fields_device(w)
+    fams = [(SR, SL, SG)]
+    st, info = api.current_green_production(g, h, fams, 1.0, spectral=False)
+    target = 0.9 * info["number"]  # exercises the paired map (a = 0.9)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        api.current_green_production(g, h, fams, target, state=st, spectral=False, return_scalars=False)
+
+    for _ in range(warmup):
+        step()
+    g.ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    g.ctx.synchronize()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    # Dominant kernel alone: the fused Dyson/Keldysh launch, timed with HIP
+    # events on the stream it runs on.
+    traces = torch.empty(4 * nblk, dtype=torch.float64, device="cuda")
+    reps = max(5, steps)
+    api.dyson_keldysh(g, h, SR, SL, SG, GR=st.GR, GL=st.GL, GG=st.GG, traces=traces)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        api.dyson_keldysh(g, h, SR, SL, SG, GR=st.GR, GL=st.GL, GG=st.GG, traces=traces)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    g.ctx.synchronize()
+    k_ms = e0.elapsed_time(e1) / reps
+    flops = nblk * 40.0 * N ** 3          # 8N^3 inverse + 2 x 16N^3 Keldysh
+    bytes_ = nblk * 16.0 * N * N * 6      # read S^R, S<, S>; write G^R, G<, G>
+    # Correctness witness of the timed state: N[G<] equals the target.
+    num = api.number_functional(g, st.GL)
+    return dict(N=N, NE=NE, Nk=Nk, nblk=nblk, step_s=dt, kernel_ms=k_ms, flops=flops, bytes=bytes_,
+                number_check=abs(num - target) / target, target=target)
+
+
+def scba_bench(iters, warmup):
+    import numpy as np
+
+    from qclhip import api, scba, synthetic as syn
+
+    w = syn.gf_workload()
+    N, NE, Nk = w["N"], w["NE"], w["Nk"]
+    rng = np.random.default_rng(syn.SEED + 7)
+    g = api.Grid(w["eps"], w["wE"], w["wk"], N)
+    mechs = [
+        scba.Mechanism("acoustic", api.ProductionKernel(g, "momentum_independent",
+                                                        syn.covariance_block(rng, N, 2, 0.04)), 1.0),
+    ]
+    l, r_, rad = syn.separable_factors(rng, N, Nk, J=3, scale=0.02)
+    mechs.append(scba.Mechanism("IFR", api.ProductionKernel(g, "separable_ifr", l, r_, rad, rank=3), 1.0))
+    lo_l, lo_r = syn.low_rank_factors(rng, N, Nk, r=4, scale=0.02)
+    mechs.append(scba.Mechanism("LO", api.ProductionKernel(g, "low_rank", lo_l, lo_r, rank=4), 1.0, True))
+    T = 0.05 * (rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N)))
+    h = api.h_to_device(w["h"])
+    kbt = w["kbt"]
+    hw = 0.36
+    # Seed at a fixed chemical potential; its number is the target.
+    GR0, _ = api.retarded_green_production(g, h, None, 0.05)
+    q = api.seed_number_weights(g, GR0)
+    del GR0
+    target = float(np.sum(q * api.fermi(w["eps"], w["mu"], kbt)))
+    seed_green, mu = api.seed_green_production(g, h, 0.05, kbt, target)
+    p = scba.ScbaProblem(g, h, api.matrix_to_device(T), api.matrix_to_device(T.conj().T), mechs, target,
+                         Ep=0.5, hwLO=hw, NLO=1.0 / math.expm1(hw / kbt), alpha_mix=0.3)
+    s = scba.initial_state(p, seed_green)
+    ws = scba.ScbaWorkspace(p)
+    for _ in range(warmup):
+        scba.scba_iteration(p, s, ws)
+    timings = {}
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        row = scba.scba_iteration(p, s, ws, timings)
+    dt = (time.perf_counter() - t0) / iters
+    return dict(iter_s=dt, phases_ms={k: 1e3 * v / iters for k, v in timings.items()}, last=row)
+
+
+def cpu_baseline(seconds=15.0):
+    """Oracle GF solves on a bounded sample (all NE energies, the first k momenta)."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    from qclhip import synthetic as syn
+
+    w = syn.gf_workload()
+    threads = orc.num_threads()
+
+    def run(k):
+        sub = dict(w)
+        sub["Nk"] = k
+        sub["gamma"] = w["gamma"][:, :k]
+        SR, SL, SG = syn.diag_fields_host(sub)
+        Z = np.zeros_like(SR)
+        h = np.asfortranarray(w["h"][:k])
+        t0 = time.perf_counter()
+        orc.current_green(w["eps"], h, [Z, SR], [Z, SL], [Z, SG], w["wE"], w["wk"][:k], 2, 1e-3)
+        return time.perf_counter() - t0
+
+    t1 = run(1)
+    k = int(max(1, min(w["Nk"], round(seconds / max(t1, 1e-3)))))
+    tk = run(k) if k > 1 else t1
+    blocks = w["NE"] * k
+    return dict(value=blocks / tk, unit="GF solves/s", cores=threads, kind="port",
+                sample=f"N=24 NE=2048, first {k} of 64 momenta ({blocks} blocks), {tk:.1f} s, "
+                       f"C++ oracle of the production Dyson/Keldysh/normalization, OpenMP {threads} threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scba-iters", type=int, default=3)
+    ap.add_argument("--no-scba", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the multi-GPU path is not part of this build "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs the MI355X", file=sys.stderr)
+        sys.exit(1)
+    log("gf batch: N=24 NE=2048 Nk=64 FP64")
+    gf = gf_bench(args.steps, args.warmup)
+    log(f"gf step {gf['step_s']*1e3:.3f} ms, fused kernel {gf['kernel_ms']:.3f} ms")
+    sc = None
+    if not args.no_scba:
+        log("scba iteration: N=24 NE=2048 Nk=64, 3 mechanisms + embedding")
+        sc = scba_bench(args.scba_iters, 1)
+        log(f"scba {sc['iter_s']*1e3:.1f} ms/iter {sc['phases_ms']}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle)")
+        cpu = cpu_baseline(args.cpu_seconds)
+        log(f"cpu {cpu['value']:.1f} solves/s on {cpu['cores']} threads")
+    k_s = gf["kernel_ms"] * 1e-3
+    achieved = gf["flops"] / k_s / 1e12
+    line = {
+        "metric": METRIC,
+        "value": gf["nblk"] / gf["step_s"],
+        "unit": "GF solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": gf["step_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "GF-solve batch: N=24 basis, NE=2048 energies on [-0.1,0.3] eV, Nk=64 momenta, "
+                               "diagonal Sigma^R with Im -U[0.5,5] meV, Fermi 77 K; FP64 plain path",
+                   "N": gf["N"], "NE": gf["NE"], "Nk": gf["Nk"], "solves_per_step": gf["nblk"],
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "k_dyson_keldysh<3,2>", "kernel_ms": gf["kernel_ms"],
+                     "hbm_achieved_GBs": gf["bytes"] / k_s / 1e9,
+                     "hbm_frac": gf["bytes"] / k_s / 1e9 / PEAK_HBM_GBS},
+        "number_check": gf["number_check"],
+    }
+    if sc is not None:
+        line["scba_iter_per_s"] = 1.0 / sc["iter_s"]
+        line["scba_ms_per_iter"] = sc["iter_s"] * 1e3
+        line["scba_phases_ms"] = sc["phases_ms"]
+        line["scba_config"] = ("N=24 NE=2048 Nk=64; momentum-independent (acoustic-like), separable IFR (J=3), "
+                               "low-rank LO (r=4, thermal), bulk embedding, FFT Hilbert with 2L witness, "
+                               "linear mixing; FP64")
+        line["scba_last"] = {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in sc["last"].items()
+                             if k != "populations"}
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

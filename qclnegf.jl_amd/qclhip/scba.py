@@ -1,0 +1,174 @@
+"""Host driver of the production SCBA loop over the device operators.
+
+Mirrors solve_scba_production (src/numerics/optimized/production_solver.jl:
+21-846) with linear mixing: per iteration the phases :dyson,
+:candidate_scattering, :candidate_embedding, :observables and :mixing run on
+the device in that order; only scalars (lambda, J, round-off, populations)
+come back to the host.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import api
+
+
+@dataclass
+class Mechanism:
+    """One enabled scattering mechanism (problem.kernels.enabled order)."""
+    name: str
+    kernel: api.ProductionKernel
+    qK: float
+    is_lo: bool = False
+
+
+@dataclass
+class ScbaProblem:
+    grid: api.Grid
+    h: torch.Tensor                 # device (Nb,Nb,Nk) = reference (Nk,Nb,Nb)
+    Tplus: torch.Tensor             # device Nb x Nb (column-major)
+    Tminus: torch.Tensor
+    mechanisms: list
+    target: float                   # N_D^2D in solver units
+    Ep: float                       # bias drop per period
+    hwLO: float = 0.36
+    NLO: float = 0.0                # thermal LO occupation 1/expm1(hw/kT)
+    alpha_mix: float = 0.3
+    verify_roundoff: bool = True
+
+    def __post_init__(self):
+        g = self.grid
+        self.plusE = api.EnergyShiftPlan(g, +self.Ep)
+        self.minusE = api.EnergyShiftPlan(g, -self.Ep)
+        self.plusLO = api.EnergyShiftPlan(g, +self.hwLO)
+        self.minusLO = api.EnergyShiftPlan(g, -self.hwLO)
+        self.hilbert = api.FFTHilbertPlan(g, g.ncols, verify_roundoff=self.verify_roundoff)
+
+
+@dataclass
+class Family:
+    R: torch.Tensor
+    L: torch.Tensor
+    G: torch.Tensor
+
+    def parts(self):
+        return (self.R, self.L, self.G)
+
+
+def new_family(grid):
+    return Family(api.zeros_field(grid), api.zeros_field(grid), api.zeros_field(grid))
+
+
+@dataclass
+class ScbaState:
+    green: api.GreenState
+    scattering: dict
+    plus: Family
+    minus: Family
+    embedding: Family
+    history: list = field(default_factory=list)
+
+
+def _candidate_scattering(p: ScbaProblem, green: api.GreenState, out: dict):
+    """_scattering_candidate_production (production_operators.jl:1258-1335)."""
+    roundoff = 0.0
+    for mech in p.mechanisms:
+        fam = out[mech.name]
+        if mech.is_lo:
+            api.production_lo_contraction(mech.kernel, green.GL, green.GG, mech.qK, p.NLO, p.plusLO,
+                                          p.minusLO, SL=fam.L, SG=fam.G)
+        else:
+            api.production_static_contraction(mech.kernel, green.GL, mech.qK, S=fam.L)
+            api.production_static_contraction(mech.kernel, green.GG, mech.qK, S=fam.G)
+        if p.verify_roundoff:
+            _, _, r = api.retarded_self_energy_fft(p.hilbert, fam.L, fam.G, return_roundoff=True, SR=fam.R)
+            roundoff = max(roundoff, r)
+        else:
+            api.retarded_self_energy_fft(p.hilbert, fam.L, fam.G, SR=fam.R)
+    return roundoff
+
+
+def _candidate_embedding(p: ScbaProblem, green: api.GreenState, plus: Family, minus: Family):
+    """_embedding_self_energy_production (production_operators.jl:924-941)."""
+    g = p.grid
+    for X, pk, mk in ((green.GR, plus.R, minus.R), (green.GL, plus.L, minus.L), (green.GG, plus.G, minus.G)):
+        api.embedding_component_production(g, p.Tplus, X, p.plusE, S=pk)
+        api.embedding_component_production(g, p.Tminus, X, p.minusE, S=mk)
+
+
+def initial_state(p: ScbaProblem, green: api.GreenState) -> ScbaState:
+    """Fresh candidates from a seed state become the first self-energies
+    (production_solver.jl:117-141)."""
+    g = p.grid
+    scat = {m.name: new_family(g) for m in p.mechanisms}
+    _candidate_scattering(p, green, scat)
+    plus, minus, emb = new_family(g), new_family(g), new_family(g)
+    _candidate_embedding(p, green, plus, minus)
+    for a, b, c in zip(plus.parts(), minus.parts(), emb.parts()):
+        api.sum_fields([a, b], c)
+    return ScbaState(green, scat, plus, minus, emb)
+
+
+class ScbaWorkspace:
+    """Candidate buffers reused across iterations."""
+
+    def __init__(self, p: ScbaProblem):
+        g = p.grid
+        self.scat = {m.name: new_family(g) for m in p.mechanisms}
+        self.plus = new_family(g)
+        self.minus = new_family(g)
+
+
+def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dict | None = None):
+    """One pass of the production loop (production_solver.jl:435-844)."""
+    g = p.grid
+    t0 = time.perf_counter()
+    fams = [s.embedding.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
+    green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=True)
+    s.green = green
+    t1 = time.perf_counter()
+    roundoff = _candidate_scattering(p, green, ws.scat)
+    t2 = time.perf_counter()
+    _candidate_embedding(p, green, ws.plus, ws.minus)
+    t3 = time.perf_counter()
+    Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
+    pops = api.state_populations(g, green.GL, p.target)
+    t4 = time.perf_counter()
+    for m in p.mechanisms:
+        for old, new in zip(s.scattering[m.name].parts(), ws.scat[m.name].parts()):
+            api.mix_family_production(old, new, p.alpha_mix)
+    for fam, cand in ((s.plus, ws.plus), (s.minus, ws.minus)):
+        for old, new in zip(fam.parts(), cand.parts()):
+            api.mix_family_production(old, new, p.alpha_mix)
+    for a, b, c in zip(s.plus.parts(), s.minus.parts(), s.embedding.parts()):
+        api.sum_fields([a, b], c)
+    g.ctx.synchronize()
+    t5 = time.perf_counter()
+    if timings is not None:
+        for k, v in (("dyson", t1 - t0), ("candidate_scattering", t2 - t1), ("candidate_embedding", t3 - t2),
+                     ("observables", t4 - t3), ("mixing", t5 - t4)):
+            timings[k] = timings.get(k, 0.0) + v
+    row = dict(lam=info["lam"], number=info["number"], holes=info["holes"], Jbar=Jbar, roundoff=roundoff,
+               populations=pops)
+    s.history.append(row)
+    return row
+
+
+def solve_scba_production(p: ScbaProblem, s: ScbaState, max_iterations: int, tol_population: float = 0.0):
+    """Run up to max_iterations passes; stops early when the relative
+    population change falls below tol_population (when > 0)."""
+    ws = ScbaWorkspace(p)
+    prev = None
+    for _ in range(max_iterations):
+        row = scba_iteration(p, s, ws)
+        if prev is not None and tol_population > 0:
+            change = np.linalg.norm(row["populations"] - prev) / (np.linalg.norm(row["populations"]) + 1e-14)
+            row["r_population"] = change
+            if change < tol_population:
+                break
+        prev = row["populations"]
+    return s

@@ -151,8 +151,36 @@ def separable_factors(rng, N, Nk, J=3, scale=0.05):
 
 
 def low_rank_factors(rng, N, Nk, r=4, scale=0.05):
-    """Low-rank kernel Kflat = left * right_adjoint with Nc = Nk N^2."""
+    """Low-rank LO-like kernel Kflat = left * right_adjoint (Nc = Nk N^2) with
+    left[(m,a,b),i] = s p_m u_i[a] conj(u_i[b]) and
+    right_adjoint[i,(m',c,d)] = p_m' conj(u_i[c]) u_i[d] for real positive p,
+    so K maps anti-Hermitian blocks to anti-Hermitian blocks."""
     Nc = Nk * N * N
-    left = scale * (rng.normal(size=(Nc, r)) + 1j * rng.normal(size=(Nc, r))) / math.sqrt(Nc)
-    radj = (rng.normal(size=(r, Nc)) + 1j * rng.normal(size=(r, Nc))) / math.sqrt(Nc)
+    p = 1.0 + 0.5 * np.cos(np.linspace(0.0, math.pi, Nk))
+    left = np.zeros((Nc, r), np.complex128)
+    radj = np.zeros((r, Nc), np.complex128)
+    for i in range(r):
+        u = rng.normal(size=N) + 1j * rng.normal(size=N)
+        u /= np.linalg.norm(u)
+        P = np.outer(u, u.conj())  # P[a,b] = u_a conj(u_b)
+        for b in range(N):
+            for a in range(N):
+                for m in range(Nk):
+                    q = m + Nk * (a + N * b)
+                    left[q, i] = scale * p[m] * P[a, b]
+                    radj[i, q] = p[m] * np.conj(P[a, b])
     return left, radj
+
+
+def dense_covariance_kernel(rng, N, Nk, nvert=2, scale=0.03):
+    """Dense Kflat[(m,a,b),(m',c,d)] = R[m,m'] sum_h V_h[a,c] conj(V_h[b,d]) with a
+    smooth positive R (a Hermiticity-preserving momentum-dependent map)."""
+    B = covariance_block(rng, N, nvert=nvert, scale=scale)
+    x = np.arange(Nk)
+    R = np.exp(-((x[:, None] - x[None, :]) / max(1.0, Nk / 3)) ** 2)
+    Nc = Nk * N * N
+    F = np.zeros((Nc, Nc), np.complex128)
+    for q in range(N * N):
+        for qp in range(N * N):
+            F[q * Nk:(q + 1) * Nk, qp * Nk:(qp + 1) * Nk] = R * B[q, qp]
+    return F
