@@ -65,6 +65,8 @@ def gf_bench(steps, warmup):
     g.ctx.synchronize()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    # Correctness witness of the timed state: N[G<] equals the target.
+    num = api.number_functional(g, st.GL)
     # Dominant kernel alone: the fused Dyson/Keldysh launch, timed with HIP
     # events on the stream it runs on.
     traces = torch.empty(4 * nblk, dtype=torch.float64, device="cuda")
@@ -80,8 +82,6 @@ def gf_bench(steps, warmup):
     k_ms = e0.elapsed_time(e1) / reps
     flops = nblk * 40.0 * N ** 3          # 8N^3 inverse + 2 x 16N^3 Keldysh
     bytes_ = nblk * 16.0 * N * N * 6      # read S^R, S<, S>; write G^R, G<, G>
-    # Correctness witness of the timed state: N[G<] equals the target.
-    num = api.number_functional(g, st.GL)
     return dict(N=N, NE=NE, Nk=Nk, nblk=nblk, step_s=dt, kernel_ms=k_ms, flops=flops, bytes=bytes_,
                 number_check=abs(num - target) / target, target=target)
 
@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--no-scba", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (the reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the multi-GPU path is not part of this build "
@@ -175,6 +177,13 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs the MI355X", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 1 GiB device-to-device copies")
     log("gf batch: N=24 NE=2048 Nk=64 FP64")
     gf = gf_bench(args.steps, args.warmup)
     log(f"gf step {gf['step_s']*1e3:.3f} ms, fused kernel {gf['kernel_ms']:.3f} ms")

@@ -50,9 +50,10 @@ typedef struct qclhip_kernel qclhip_kernel;
 typedef struct qclhip_hilbert qclhip_hilbert;
 
 /* ---- context ------------------------------------------------------------
- * One context per host thread and device; owns the stream (or borrows the
- * caller's), the rocBLAS handle, device status and scratch. Replaces the
- * per-worker workspaces of the reference (production_cache.jl:63-117). */
+ * One context per host thread and device; enqueues on the caller's stream
+ * (a hipStream_t; NULL is the default stream) and owns the rocBLAS handle,
+ * device status and scratch. Replaces the per-worker workspaces of the
+ * reference (production_cache.jl:63-117). */
 int qclhip_ctx_create(int device, void* stream, qclhip_ctx** out);
 int qclhip_ctx_destroy(qclhip_ctx* ctx);
 /* Waits for the stream, returns (and clears) the latched device status. */

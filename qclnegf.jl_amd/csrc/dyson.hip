@@ -29,13 +29,16 @@ struct DKShape {
         (size_t)2 * NP * LD * sizeof(c64) + 2 * NP * sizeof(c64) + 2 * NP * sizeof(int);
 };
 
+constexpr int kMaxFamilies = 8;
+
 struct DKArgs {
     int NE, Nk, N;
     const double* eps;
     const c64* h;
-    const c64* sigR;
-    const c64* sigL;
-    const c64* sigG;
+    const c64* famR[kMaxFamilies];  // summed in order: the total self-energy
+    const c64* famL[kMaxFamilies];
+    const c64* famG[kMaxFamilies];
+    int nfamR, nfamL, nfamG;
     double eta;
     c64* GR;
     c64* GL;
@@ -55,6 +58,7 @@ __device__ inline void block_product(const c64* __restrict__ L, const c64* __res
     for (int r = 0; r < B; ++r)
 #pragma unroll
         for (int c = 0; c < B; ++c) out[r][c] = cmk(0.0, 0.0);
+#pragma unroll 2
     for (int q = 0; q < N; ++q) {
         c64 lv[B], rv[B];
 #pragma unroll
@@ -69,230 +73,356 @@ __device__ inline void block_product(const c64* __restrict__ L, const c64* __res
     }
 }
 
-template <int B, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_dyson_keldysh(DKArgs p) {
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2):
+// give every XCD a contiguous range of tiles so that the energies sharing a
+// 128-byte line are served by one L2. Bijective for any block count.
+__device__ inline long long xcd_tile(long long b, long long nb) {
+    const long long q = nb >> 3, rem = nb & 7, x = b & 7, s = b >> 3;
+    return x * q + (x < rem ? x : rem) + s;
+}
+
+// Element (i,j) of the summed families of one block (the reference's
+// _total_family_production order: fam[0] + fam[1] + ...).
+__device__ inline c64 fam_load(const c64* const* fam, int nfam, size_t idx) {
+    c64 v = fam[0][idx];
+    for (int f = 1; f < nfam; ++f) v = cadd(v, fam[f][idx]);
+    return v;
+}
+
+// Cooperative tile transfer between HBM and the per-wave LDS matrices of a
+// workgroup whose W waves own the W consecutive energies blk0 .. blk0+W-1 of
+// one momentum. Thread f handles element (i,j) of energy w with
+// f = w + W*(i + N*j): consecutive threads touch consecutive energies, so one
+// instruction moves contiguous 16*W-byte segments. Wave w's matrix sits at
+// lds + w*wstride, element (i,j) at i*LD + j.
+template <int W, int LD>
+__device__ inline void tile_load(c64* lds, int wstride, const c64* const* fam, int nfam, long long blk0,
+                                 int nvalid, int N, size_t S) {
+    const int total = N * N * W;
+    int f = threadIdx.x;
+    const int w = f % W;
+    int ij = f / W;
+    int i = ij % N, j = ij / N;
+    for (; f < total; f += 64 * W) {
+        if (w < nvalid) lds[w * wstride + i * LD + j] = fam_load(fam, nfam, blk0 + w + S * (i + (size_t)N * j));
+        i += 64;
+        while (i >= N) {
+            i -= N;
+            ++j;
+        }
+    }
+}
+
+template <int W, int LD>
+__device__ inline void tile_store(const c64* lds, int wstride, c64* dst, long long blk0, unsigned wave_mask, int N,
+                                  size_t S) {
+    const int total = N * N * W;
+    int f = threadIdx.x;
+    const int w = f % W;
+    int ij = f / W;
+    int i = ij % N, j = ij / N;
+    const bool on = (wave_mask >> w) & 1u;
+    for (; f < total; f += 64 * W) {
+        if (on) dst[blk0 + w + S * (i + (size_t)N * j)] = lds[w * wstride + i * LD + j];
+        i += 64;
+        while (i >= N) {
+            i -= N;
+            ++j;
+        }
+    }
+}
+
+// 8-lane argmax (largest |Re|+|Im|, then smallest row) over the lanes of one
+// column group, by DPP lane permutes: xor 1, xor 2 (quad_perm) and the
+// half-row mirror (lane i <-> 7-i of each 8 lanes).
+template <int CTRL>
+__device__ inline void dpp_argmax_step(double& mag, int& idx) {
+    long long bits = __double_as_longlong(mag);
+    int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
+    int oi = __builtin_amdgcn_update_dpp(0, idx, CTRL, 0xF, 0xF, false);
+    double om = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    if (om > mag || (om == mag && oi < idx)) {
+        mag = om;
+        idx = oi;
+    }
+}
+
+__device__ inline void argmax8(double& mag, int& idx) {
+    dpp_argmax_step<0xB1>(mag, idx);   // quad_perm [1,0,3,2]
+    dpp_argmax_step<0x4E>(mag, idx);   // quad_perm [2,3,0,1]
+    dpp_argmax_step<0x141>(mag, idx);  // row_half_mirror
+}
+
+// Dyson inversion only (A1): one block per wavefront, 8x8 lane grid, B x B
+// register sub-blocks, Gauss-Jordan with partial pivoting. The workgroup
+// stages the W self-energy blocks through LDS with coalesced loads; during
+// the elimination only the pivot column and pivot row travel through LDS.
+// The inverse stays in the pivoted order in registers and is un-permuted
+// on its way back to LDS, G[steprow[i]][pivrow[j]] = M[i][j] / s, then
+// stored with coalesced writes.
+template <int B, int W>
+struct DyShape {
+    static constexpr int NP = 8 * B;
+    static constexpr int LD = NP + 1;
+    static constexpr size_t per_wave = (size_t)NP * LD * sizeof(c64) + 2 * NP * sizeof(c64) + 2 * NP * sizeof(int);
+    static constexpr size_t bytes = per_wave * W;
+};
+
+template <int B, int W>
+__global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p) {
+    constexpr int NP = DyShape<B, W>::NP;
+    constexpr int LD = DyShape<B, W>::LD;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ unsigned s_ok;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nte = (p.NE + W - 1) / W;
+    const long long t = xcd_tile(blockIdx.x, gridDim.x);
+    const int e0 = (int)(t % nte) * W;
+    const int m = (int)(t / nte);
+    const int nvalid = min(W, p.NE - e0);
+    const long long nblk = (long long)p.NE * p.Nk;
+    const long long blk0 = e0 + (long long)p.NE * m;
+    const int e = e0 + wave;
+    const long long blk = blk0 + wave;
+    const int wstride = NP * LD;
+    c64* Ms = reinterpret_cast<c64*>(smem) + (size_t)wave * wstride;
+    unsigned char* tail = smem + (size_t)W * wstride * sizeof(c64) + (size_t)wave * (2 * NP * sizeof(c64) + 2 * NP * sizeof(int));
+    c64* colbuf = reinterpret_cast<c64*>(tail);
+    c64* rowbuf = colbuf + NP;
+    int* pivrow = reinterpret_cast<int*>(rowbuf + NP);
+    int* steprow = pivrow + NP;
+    const int N = p.N;
+    const size_t S = (size_t)nblk;
+    const int lr = lane & 7, lc = lane >> 3;
+    if (threadIdx.x == 0) s_ok = 0;
+    if (p.nfamR) tile_load<W, LD>(reinterpret_cast<c64*>(smem), wstride, p.famR, p.nfamR, blk0, nvalid, N, S);
+    __syncthreads();
+    bool ok = wave < nvalid;
+    double s = 0.0, rs = 0.0;
+    c64 A[B][B];
+    if (ok) {
+        // D = (eps_e + i eta) I - h_m - SigmaR_em (production_operators.jl:1360-1363)
+        const double ee = p.eps[e];
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int i = lr * B + r, j = lc * B + c;
+                c64 v = cmk(0.0, 0.0);
+                if (i < N && j < N) {
+                    const c64 hv = p.h[(size_t)m + (size_t)p.Nk * (i + (size_t)N * j)];
+                    const c64 sv = p.nfamR ? Ms[i * LD + j] : cmk(0.0, 0.0);
+                    const double dre = (i == j) ? ee : 0.0, dim = (i == j) ? p.eta : 0.0;
+                    v = cmk(dre - hv.x - sv.x, dim - hv.y - sv.y);
+                }
+                A[r][c] = v;
+            }
+        // s = max_a sum_b |D_ab| (production_operators.jl:1364-1372)
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            double acc = 0.0;
+#pragma unroll
+            for (int c = 0; c < B; ++c) acc += hypot(A[r][c].x, A[r][c].y);
+            acc += shfl_xor_d(acc, 8);
+            acc += shfl_xor_d(acc, 16);
+            acc += shfl_xor_d(acc, 32);
+            s = fmax(s, acc);
+        }
+        s = fmax(s, shfl_xor_d(s, 1));
+        s = fmax(s, shfl_xor_d(s, 2));
+        s = fmax(s, shfl_xor_d(s, 4));
+        s = readlane_d(s, 0);
+        ok = s > 0.0;
+    }
+    if (ok) {
+        rs = 1.0 / s;
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) A[r][c] = cmk(A[r][c].x * rs, A[r][c].y * rs);
+        unsigned used = 0;  // bit r: own row lr*B+r already pivoted
+        for (int ck = 0; ck < 8 && ok; ++ck) {
+            if (ck * B >= N) break;
+#pragma unroll
+            for (int kk = 0; kk < B; ++kk) {
+                const int k = ck * B + kk;
+                if (k >= N || !ok) break;
+                double mag = -1.0;
+                int idx = 0x7fffffff;
+                if (lc == ck) {
+#pragma unroll
+                    for (int r = 0; r < B; ++r) {
+                        const int i = lr * B + r;
+                        if (i < N) {
+                            colbuf[i] = A[r][kk];
+                            if (!((used >> r) & 1u)) {
+                                const double mg = cabs1(A[r][kk]);
+                                if (mg > mag) { mag = mg; idx = i; }
+                            }
+                        }
+                    }
+                }
+                argmax8(mag, idx);
+                const int piv = __builtin_amdgcn_readlane(idx, 8 * ck);
+                const double pmag = readlane_d(mag, 8 * ck);
+                if (!(pmag > 0.0) || piv >= N) {
+                    ok = false;
+                    break;
+                }
+                const int pg = piv / B, pr = piv - pg * B;
+                if (lr == pg) {
+#pragma unroll
+                    for (int r = 0; r < B; ++r)
+                        if (r == pr) {
+#pragma unroll
+                            for (int c = 0; c < B; ++c) {
+                                const int j = lc * B + c;
+                                if (j < N) rowbuf[j] = A[r][c];
+                            }
+                        }
+                }
+                if (lane == 0) {
+                    pivrow[k] = piv;
+                    steprow[piv] = k;
+                }
+                wave_sync();
+                const c64 pv = colbuf[piv];
+                c64 f[B], R[B];
+#pragma unroll
+                for (int r = 0; r < B; ++r) {
+                    const int i = lr * B + r;
+                    f[r] = (i < N) ? colbuf[i] : cmk(0.0, 0.0);
+                }
+#pragma unroll
+                for (int c = 0; c < B; ++c) {
+                    const int j = lc * B + c;
+                    R[c] = (j < N) ? rowbuf[j] : cmk(0.0, 0.0);
+                }
+                const c64 inv = crecip(pv);
+#pragma unroll
+                for (int c = 0; c < B; ++c) R[c] = cmul(R[c], inv);
+#pragma unroll
+                for (int r = 0; r < B; ++r)
+#pragma unroll
+                    for (int c = 0; c < B; ++c) A[r][c] = cfms(A[r][c], f[r], R[c]);
+                // Pivot row <- scaled pivot row; pivot column <- -f/pivot; pivot <- 1/pivot.
+                if (lr == pg) {
+#pragma unroll
+                    for (int r = 0; r < B; ++r)
+                        if (r == pr) {
+#pragma unroll
+                            for (int c = 0; c < B; ++c) A[r][c] = R[c];
+                            used |= 1u << r;
+                        }
+                }
+                if (lc == ck) {
+#pragma unroll
+                    for (int r = 0; r < B; ++r) {
+                        const c64 tt = cmul(f[r], inv);
+                        A[r][kk] = (lr == pg && r == pr) ? inv : cmk(-tt.x, -tt.y);
+                    }
+                }
+                wave_sync();
+            }
+        }
+        if (!ok && lane == 0) report_status(p.status, 1, (unsigned long long)blk + 1);
+    } else if (wave < nvalid && lane == 0) {
+        report_status(p.status, 1, (unsigned long long)blk + 1);
+    }
+    if (ok) {
+        // Un-permute into LDS: G[k(i)][p(j)] = M[i][j] / s.
+        int si[B], pj[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) si[r] = (lr * B + r < N) ? steprow[lr * B + r] : 0;
+#pragma unroll
+        for (int c = 0; c < B; ++c) pj[c] = (lc * B + c < N) ? pivrow[lc * B + c] : 0;
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int i = lr * B + r, j = lc * B + c;
+                if (i < N && j < N) Ms[si[r] * LD + pj[c]] = cmk(A[r][c].x * rs, A[r][c].y * rs);
+            }
+        if (lane == 0) {
+            atomicOr(&s_ok, 1u << wave);
+            if (p.scale) p.scale[blk] = s;
+        }
+    }
+    __syncthreads();
+    tile_store<W, LD>(reinterpret_cast<const c64*>(smem), wstride, p.GR, blk0, s_ok, N, S);
+}
+
+// Keldysh products (A2): G<raw = G^R S< G^R^dagger and G>raw likewise, plus
+// the block traces. The workgroup stages G^R and S tiles through LDS with
+// coalesced loads; each lane accumulates its B x B block of T = G S and of
+// T G^dagger; results return through LDS with coalesced stores.
+template <int B, int W>
+__global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p) {
     constexpr int NP = DKShape<B>::NP;
     constexpr int LD = DKShape<B>::LD;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const int nte = (p.NE + W - 1) / W;
+    const long long t = xcd_tile(blockIdx.x, gridDim.x);
+    const int e0 = (int)(t % nte) * W;
+    const int m = (int)(t / nte);
+    const int nvalid = min(W, p.NE - e0);
     const long long nblk = (long long)p.NE * p.Nk;
-    const long long blk = (long long)blockIdx.x * WAVES + wave;
-    if (blk >= nblk) return;  // whole wavefront leaves together
-
-    c64* Gs = reinterpret_cast<c64*>(smem + (size_t)wave * DKShape<B>::bytes);
-    c64* Xs = Gs + NP * LD;
-    c64* colbuf = Xs + NP * LD;
-    c64* rowbuf = colbuf + NP;
-    int* pivrow = reinterpret_cast<int*>(rowbuf + NP);
-    int* steprow = pivrow + NP;
-
+    const long long blk0 = e0 + (long long)p.NE * m;
+    const long long blk = blk0 + wave;
+    const int wstride = NP * LD;
+    c64* G0 = reinterpret_cast<c64*>(smem);
+    c64* X0 = G0 + (size_t)W * wstride;
+    c64* Gs = G0 + (size_t)wave * wstride;
+    c64* Xs = X0 + (size_t)wave * wstride;
     const int N = p.N;
-    const int e = (int)(blk % p.NE);
-    const int m = (int)(blk / p.NE);
-    const size_t S = (size_t)nblk;  // stride between matrix elements of one block
+    const size_t S = (size_t)nblk;
     const int lr = lane & 7, lc = lane >> 3;
-
-    // D = (eps_e + i eta) I - h_m - SigmaR_em (production_operators.jl:1360-1363)
-    c64 A[B][B];
-    const double ee = p.eps[e];
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const int i = lr * B + r, j = lc * B + c;
-            c64 v = cmk(0.0, 0.0);
-            if (i < N && j < N) {
-                c64 hv = p.h[(size_t)m + (size_t)p.Nk * (i + (size_t)N * j)];
-                c64 sv = p.sigR ? p.sigR[blk + S * (i + (size_t)N * j)] : cmk(0.0, 0.0);
-                double dre = (i == j) ? ee : 0.0, dim = (i == j) ? p.eta : 0.0;
-                v = cmk(dre - hv.x - sv.x, dim - hv.y - sv.y);
-            }
-            A[r][c] = v;
-        }
-
-    // s = max_a sum_b |D_ab| (production_operators.jl:1364-1372)
-    double rows[B];
-#pragma unroll
-    for (int r = 0; r < B; ++r) {
-        double acc = 0.0;
-#pragma unroll
-        for (int c = 0; c < B; ++c) acc += hypot(A[r][c].x, A[r][c].y);
-        acc += shfl_xor_d(acc, 8);
-        acc += shfl_xor_d(acc, 16);
-        acc += shfl_xor_d(acc, 32);
-        rows[r] = acc;
-    }
-    double s = rows[0];
-#pragma unroll
-    for (int r = 1; r < B; ++r) s = fmax(s, rows[r]);
-    s = fmax(s, shfl_xor_d(s, 1));
-    s = fmax(s, shfl_xor_d(s, 2));
-    s = fmax(s, shfl_xor_d(s, 4));
-    s = readlane_d(s, 0);
-    if (!(s > 0.0)) {
-        if (lane == 0) report_status(p.status, 1, (unsigned long long)blk + 1);
-        return;
-    }
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) A[r][c] = cmk(A[r][c].x / s, A[r][c].y / s);
-
-    unsigned used = 0;  // bit r: own row lr*B+r already pivoted
-    bool singular = false;
-    for (int ck = 0; ck < 8 && !singular; ++ck) {
-#pragma unroll
-        for (int kk = 0; kk < B; ++kk) {
-            const int k = ck * B + kk;
-            if (k >= N || singular) break;
-            // Pivot column into LDS and the local pivot candidate.
-            double mag = -1.0;
-            int idx = INT_MAX;
-            if (lc == ck) {
-#pragma unroll
-                for (int r = 0; r < B; ++r) {
-                    const int i = lr * B + r;
-                    if (i < N) {
-                        colbuf[i] = A[r][kk];
-                        if (!((used >> r) & 1u)) {
-                            double mg = cabs1(A[r][kk]);
-                            if (mg > mag) { mag = mg; idx = i; }
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int off = 1; off < 8; off <<= 1) {
-                double om = shfl_xor_d(mag, off);
-                int oi = shfl_xor_i(idx, off);
-                if (om > mag || (om == mag && oi < idx)) { mag = om; idx = oi; }
-            }
-            const int piv = __builtin_amdgcn_readlane(idx, 8 * ck);
-            const double pmag = readlane_d(mag, 8 * ck);
-            if (!(pmag > 0.0) || piv >= N) {
-                singular = true;
-                break;
-            }
+    const bool active = wave < nvalid;
+    const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
+    const c64* gr_fam[1] = {p.GR};
+    tile_load<W, LD>(G0, wstride, gr_fam, 1, blk0, nvalid, N, S);
+    for (int comp = 0; comp < 2; ++comp) {
+        const c64* const* fam = comp ? p.famG : p.famL;
+        const int nf = comp ? p.nfamG : p.nfamL;
+        c64* dst = comp ? p.GG : p.GL;
+        tile_load<W, LD>(X0, wstride, fam, nf, blk0, nvalid, N, S);
+        __syncthreads();
+        c64 T[B][B];
+        if (active) {
+            block_product<B, false>(Gs, Xs, N, lr, lc, T);
             wave_sync();
-            const c64 pv = colbuf[piv];
-            const c64 inv = crecip(pv);
-            const int pg = piv / B, pr = piv - pg * B;
-            if (lr == pg) {
-#pragma unroll
-                for (int r = 0; r < B; ++r)
-                    if (r == pr) {
-#pragma unroll
-                        for (int c = 0; c < B; ++c) {
-                            const int j = lc * B + c;
-                            if (j < N) rowbuf[j] = cmul(A[r][c], inv);
-                        }
-                    }
-            }
-            wave_sync();
-            c64 f[B], R[B];
-#pragma unroll
-            for (int r = 0; r < B; ++r) {
-                const int i = lr * B + r;
-                f[r] = (i < N) ? colbuf[i] : cmk(0.0, 0.0);
-            }
-#pragma unroll
-            for (int c = 0; c < B; ++c) {
-                const int j = lc * B + c;
-                R[c] = (j < N) ? rowbuf[j] : cmk(0.0, 0.0);
-            }
 #pragma unroll
             for (int r = 0; r < B; ++r)
 #pragma unroll
-                for (int c = 0; c < B; ++c) A[r][c] = cfms(A[r][c], f[r], R[c]);
-            // Pivot row <- scaled pivot row; pivot column <- -f/pivot; pivot <- 1/pivot.
-            if (lr == pg) {
-#pragma unroll
-                for (int r = 0; r < B; ++r)
-                    if (r == pr) {
-#pragma unroll
-                        for (int c = 0; c < B; ++c) A[r][c] = R[c];
-                        used |= 1u << r;
-                    }
-            }
-            if (lc == ck) {
-#pragma unroll
-                for (int r = 0; r < B; ++r) {
-                    c64 t = cmul(f[r], inv);
-                    A[r][kk] = (lr == pg && r == pr) ? inv : cmk(-t.x, -t.y);
-                }
-            }
-            if (lane == 0) {
-                pivrow[k] = piv;
-                steprow[piv] = k;
-            }
+                for (int c = 0; c < B; ++c) Xs[(lr * B + r) * LD + lc * B + c] = T[r][c];
             wave_sync();
-        }
-    }
-    if (singular) {
-        if (lane == 0) report_status(p.status, 1, (unsigned long long)blk + 1);
-        return;
-    }
-    wave_sync();
-    // Undo the implicit pivoting: inverse[k(i)][p(j)] = M[i][j]; G = inverse / s.
+            block_product<B, true>(Xs, Gs, N, lr, lc, T);
+            wave_sync();
+            c64 tr = cmk(0.0, 0.0);
 #pragma unroll
-    for (int r = 0; r < B; ++r)
+            for (int r = 0; r < B; ++r)
 #pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const int i = lr * B + r, j = lc * B + c;
-            if (i < N && j < N) Gs[steprow[i] * LD + pivrow[j]] = cmk(A[r][c].x / s, A[r][c].y / s);
-        }
-    wave_sync();
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const int i = lr * B + r, j = lc * B + c;
-            if (i < N && j < N) p.GR[blk + S * (i + (size_t)N * j)] = Gs[i * LD + j];
-        }
-    if (lane == 0 && p.scale) p.scale[blk] = s;
-    if (!p.keldysh) return;
-
-    for (int comp = 0; comp < 2; ++comp) {
-        const c64* src = comp ? p.sigG : p.sigL;
-        c64* dst = comp ? p.GG : p.GL;
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int c = 0; c < B; ++c) {
-                const int i = lr * B + r, j = lc * B + c;
-                if (i < N && j < N) Xs[i * LD + j] = src[blk + S * (i + (size_t)N * j)];
-            }
-        wave_sync();
-        c64 T[B][B];
-        block_product<B, false>(Gs, Xs, N, lr, lc, T);
-        wave_sync();
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int c = 0; c < B; ++c) Xs[(lr * B + r) * LD + lc * B + c] = T[r][c];
-        wave_sync();
-        block_product<B, true>(Xs, Gs, N, lr, lc, T);
-        c64 tr = cmk(0.0, 0.0);
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int c = 0; c < B; ++c) {
-                const int i = lr * B + r, j = lc * B + c;
-                if (i < N && j < N) {
-                    dst[blk + S * (i + (size_t)N * j)] = T[r][c];
-                    if (i == j) tr = cadd(tr, T[r][c]);
+                for (int c = 0; c < B; ++c) {
+                    const int i = lr * B + r, j = lc * B + c;
+                    Xs[i * LD + j] = T[r][c];
+                    if (i == j && i < N) tr = cadd(tr, T[r][c]);
                 }
-            }
-        if (p.traces) {
+            if (p.traces) {
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                tr.x += shfl_xor_d(tr.x, off);
-                tr.y += shfl_xor_d(tr.y, off);
+                for (int off = 1; off < 64; off <<= 1) {
+                    tr.x += shfl_xor_d(tr.x, off);
+                    tr.y += shfl_xor_d(tr.y, off);
+                }
+                if (lane == 0) p.traces[(size_t)comp * nblk + blk] = tr;
             }
-            if (lane == 0) p.traces[(size_t)comp * nblk + blk] = tr;
         }
-        wave_sync();
+        __syncthreads();
+        tile_store<W, LD>(X0, wstride, dst, blk0, mask, N, S);
+        __syncthreads();
     }
 }
 
@@ -310,74 +440,85 @@ struct SWArgs {
     c64* out;
 };
 
-template <int B, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_sandwich(SWArgs p) {
+template <int B, int W>
+__global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
     constexpr int NP = DKShape<B>::NP;
     constexpr int LD = DKShape<B>::LD;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const int nte = (p.NE + W - 1) / W;
+    const long long t = xcd_tile(blockIdx.x, gridDim.x);
+    const int e0 = (int)(t % nte) * W;
+    const int m = (int)(t / nte);
+    const int nvalid = min(W, p.NE - e0);
     const long long nblk = (long long)p.NE * p.Nk;
-    const long long blk = (long long)blockIdx.x * WAVES + wave;
-    if (blk >= nblk) return;
-    c64* Gs = reinterpret_cast<c64*>(smem + (size_t)wave * DKShape<B>::bytes);
-    c64* Xs = Gs + NP * LD;
+    const long long mbase = (long long)p.NE * m;
+    const int wstride = NP * LD;
+    c64* Ts = reinterpret_cast<c64*>(smem);          // the constant T, shared
+    c64* X0 = Ts + wstride;
+    c64* Xs = X0 + (size_t)wave * wstride;
     const int N = p.N;
-    const int e = (int)(blk % p.NE);
     const size_t S = (size_t)nblk;
     const int lr = lane & 7, lc = lane >> 3;
-    if (p.valid && !p.valid[e]) {
+    for (int f = threadIdx.x; f < N * N; f += 64 * W) Ts[(f % N) * LD + f / N] = p.T[f];
+    // Shifted input X(e + delta) for the W energies: two coalesced gathers.
+    {
+        const int total = N * N * W;
+        int f = threadIdx.x;
+        const int w = f % W;
+        int ij = f / W;
+        int i = ij % N, j = ij / N;
+        const int e = e0 + w;
+        const bool valid = w < nvalid && (!p.valid || p.valid[e]);
+        int l = e, r = e;
+        double th = 0.0;
+        if (valid && p.valid) {
+            l = p.left[e];
+            r = p.right[e];
+            th = p.wr[e];
+        }
+        for (; f < total; f += 64 * W) {
+            if (w < nvalid) {
+                c64 v = cmk(0.0, 0.0);
+                if (valid) {
+                    const size_t off = mbase + S * (i + (size_t)N * j);
+                    const c64 xl = p.X[l + off];
+                    v = xl;
+                    if (!(l == r || th == 0.0)) {
+                        const c64 xr = p.X[r + off];
+                        v = cmk((1.0 - th) * xl.x + th * xr.x, (1.0 - th) * xl.y + th * xr.y);
+                    }
+                }
+                X0[w * wstride + i * LD + j] = v;
+            }
+            i += 64;
+            while (i >= N) {
+                i -= N;
+                ++j;
+            }
+        }
+    }
+    __syncthreads();
+    if (wave < nvalid) {
+        c64 Tm[B][B];
+        block_product<B, false>(Ts, Xs, N, lr, lc, Tm);
+        wave_sync();
 #pragma unroll
         for (int r = 0; r < B; ++r)
 #pragma unroll
-            for (int c = 0; c < B; ++c) {
-                const int i = lr * B + r, j = lc * B + c;
-                if (i < N && j < N) p.out[blk + S * (i + (size_t)N * j)] = cmk(0.0, 0.0);
-            }
-        return;
+            for (int c = 0; c < B; ++c) Xs[(lr * B + r) * LD + lc * B + c] = Tm[r][c];
+        wave_sync();
+        block_product<B, true>(Xs, Ts, N, lr, lc, Tm);
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) Xs[(lr * B + r) * LD + lc * B + c] = Tm[r][c];
     }
-    int l = e, rr = e;
-    double th = 0.0;
-    if (p.valid) {
-        l = p.left[e];
-        rr = p.right[e];
-        th = p.wr[e];
-    }
-    const long long bl = blk - e + l, br = blk - e + rr;
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const int i = lr * B + r, j = lc * B + c;
-            if (i < N && j < N) {
-                Gs[i * LD + j] = p.T[i + (size_t)N * j];
-                const size_t off = S * (i + (size_t)N * j);
-                c64 xl = p.X[bl + off];
-                c64 v = xl;
-                if (!(l == rr || th == 0.0)) {
-                    c64 xr = p.X[br + off];
-                    v = cmk((1.0 - th) * xl.x + th * xr.x, (1.0 - th) * xl.y + th * xr.y);
-                }
-                Xs[i * LD + j] = v;
-            }
-        }
-    wave_sync();
-    c64 Tm[B][B];
-    block_product<B, false>(Gs, Xs, N, lr, lc, Tm);
-    wave_sync();
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) Xs[(lr * B + r) * LD + lc * B + c] = Tm[r][c];
-    wave_sync();
-    block_product<B, true>(Xs, Gs, N, lr, lc, Tm);
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const int i = lr * B + r, j = lc * B + c;
-            if (i < N && j < N) p.out[blk + S * (i + (size_t)N * j)] = Tm[r][c];
-        }
+    __syncthreads();
+    const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
+    tile_store<W, LD>(X0, wstride, p.out, e0 + mbase, mask, N, S);
 }
 
 // Deterministic reduction of the weighted block traces in the reference's
@@ -570,32 +711,52 @@ inline unsigned grid_for(long long n, int threads, int sm_count) {
 }
 
 template <int B>
-int launch_dk(qclhip_ctx* ctx, const DKArgs& a) {
-    constexpr int WAVES = DKShape<B>::bytes * 2 <= 48 * 1024 ? (DKShape<B>::bytes * 4 <= 48 * 1024 ? 4 : 2) : 1;
-    const size_t lds = DKShape<B>::bytes * WAVES;
+constexpr int tile_waves(size_t per_wave) {
+    // Energies per workgroup: 4 (64-byte segments) when it fits in half the LDS
+    // of a CU, else fewer.
+    return per_wave * 4 <= 80 * 1024 ? 4 : (per_wave * 2 <= 80 * 1024 ? 2 : 1);
+}
+
+template <int B>
+int launch_dk(qclhip_ctx* ctx, const DKArgs& a, bool keldysh) {
+    {
+        constexpr int W = 4;
+        const size_t lds = DyShape<B, W>::bytes;
+        if (lds > ctx->max_lds_per_block)
+            return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
+        if (lds > 65536)
+            QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_dyson<B, W>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
+        hipLaunchKernelGGL((k_dyson<B, W>), dim3((unsigned)tiles), dim3(64 * W), lds, ctx->stream, a);
+        int rc = launch_check(ctx, "k_dyson");
+        if (rc || !keldysh) return rc;
+    }
+    constexpr size_t per_wave = 2 * (size_t)DKShape<B>::NP * DKShape<B>::LD * sizeof(c64);
+    constexpr int W = tile_waves<B>(per_wave);
+    const size_t lds = per_wave * W;
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
     if (lds > 65536)
-        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_dyson_keldysh<B, WAVES>,
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_keldysh<B, W>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const long long nblk = (long long)a.NE * a.Nk;
-    const unsigned blocks = (unsigned)((nblk + WAVES - 1) / WAVES);
-    hipLaunchKernelGGL((k_dyson_keldysh<B, WAVES>), dim3(blocks), dim3(64 * WAVES), lds, ctx->stream, a);
-    return launch_check(ctx, "k_dyson_keldysh");
+    const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
+    hipLaunchKernelGGL((k_keldysh<B, W>), dim3((unsigned)tiles), dim3(64 * W), lds, ctx->stream, a);
+    return launch_check(ctx, "k_keldysh");
 }
 
 template <int B>
 int launch_sw(qclhip_ctx* ctx, const SWArgs& a) {
-    constexpr int WAVES = DKShape<B>::bytes * 2 <= 48 * 1024 ? (DKShape<B>::bytes * 4 <= 48 * 1024 ? 4 : 2) : 1;
-    const size_t lds = DKShape<B>::bytes * WAVES;
+    constexpr size_t per_wave = (size_t)DKShape<B>::NP * DKShape<B>::LD * sizeof(c64);
+    constexpr int W = tile_waves<B>(2 * per_wave);
+    const size_t lds = per_wave * (W + 1);
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
     if (lds > 65536)
-        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_sandwich<B, WAVES>,
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_sandwich<B, W>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const long long nblk = (long long)a.NE * a.Nk;
-    const unsigned blocks = (unsigned)((nblk + WAVES - 1) / WAVES);
-    hipLaunchKernelGGL((k_sandwich<B, WAVES>), dim3(blocks), dim3(64 * WAVES), lds, ctx->stream, a);
+    const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
+    hipLaunchKernelGGL((k_sandwich<B, W>), dim3((unsigned)tiles), dim3(64 * W), lds, ctx->stream, a);
     return launch_check(ctx, "k_sandwich");
 }
 
@@ -610,14 +771,14 @@ int sandwich_dispatch(qclhip_ctx* ctx, const SWArgs& a) {
     return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
 }
 
-int dyson_keldysh_dispatch(qclhip_ctx* ctx, const DKArgs& a) {
+int dyson_keldysh_dispatch(qclhip_ctx* ctx, const DKArgs& a, bool keldysh) {
     const int N = a.N;
-    if (N <= 8) return launch_dk<1>(ctx, a);
-    if (N <= 16) return launch_dk<2>(ctx, a);
-    if (N <= 24) return launch_dk<3>(ctx, a);
-    if (N <= 32) return launch_dk<4>(ctx, a);
-    if (N <= 40) return launch_dk<5>(ctx, a);
-    if (N <= 48) return launch_dk<6>(ctx, a);
+    if (N <= 8) return launch_dk<1>(ctx, a, keldysh);
+    if (N <= 16) return launch_dk<2>(ctx, a, keldysh);
+    if (N <= 24) return launch_dk<3>(ctx, a, keldysh);
+    if (N <= 32) return launch_dk<4>(ctx, a, keldysh);
+    if (N <= 40) return launch_dk<5>(ctx, a, keldysh);
+    if (N <= 48) return launch_dk<6>(ctx, a, keldysh);
     return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
 }
 
@@ -641,15 +802,17 @@ int qclhip_dyson_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
         traces = (double*)workspace(ctx, (size_t)2 * nblk * sizeof(c64));
         if (!traces) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
     }
-    DKArgs a;
+    DKArgs a{};
     a.NE = grid->NE;
     a.Nk = grid->Nk;
     a.N = grid->Nb;
     a.eps = grid->eps;
     a.h = (const c64*)h;
-    a.sigR = (const c64*)sigR;
-    a.sigL = (const c64*)sigL;
-    a.sigG = (const c64*)sigG;
+    a.famR[0] = (const c64*)sigR;
+    a.famL[0] = (const c64*)sigL;
+    a.famG[0] = (const c64*)sigG;
+    a.nfamR = sigR ? 1 : 0;
+    a.nfamL = a.nfamG = 1;
     a.eta = eta;
     a.GR = (c64*)GR;
     a.GL = (c64*)GL;
@@ -658,7 +821,7 @@ int qclhip_dyson_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
     a.traces = (c64*)traces;
     a.status = ctx->status;
     a.keldysh = keldysh ? 1 : 0;
-    return dyson_keldysh_dispatch(ctx, a);
+    return dyson_keldysh_dispatch(ctx, a, keldysh);
 }
 
 int qclhip_retarded_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* sigR,
@@ -677,14 +840,28 @@ int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
         return fail(ctx, QCLHIP_ERR_ARGUMENT, "target charge must be finite and positive");
     const long long nblk = (long long)grid->NE * grid->Nk;
     const long long n = nblk * grid->Nb * grid->Nb;
-    // Scratch: traces, then (when nfam > 1) the three total fields.
+    // Scratch: traces, then (more families than one launch takes) the totals.
+    const bool fused = nfam <= kMaxFamilies;
     size_t tr_bytes = (size_t)2 * nblk * sizeof(c64);
-    size_t tot_bytes = nfam > 1 ? (size_t)3 * n * sizeof(c64) : 0;
+    size_t tot_bytes = fused ? 0 : (size_t)3 * n * sizeof(c64);
     char* ws = (char*)workspace(ctx, tr_bytes + tot_bytes);
     if (!ws) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
     c64* traces = (c64*)ws;
-    const c64 *tR = (const c64*)famR[0], *tL = (const c64*)famL[0], *tG = (const c64*)famG[0];
-    if (nfam > 1) {
+    DKArgs a{};
+    a.NE = grid->NE;
+    a.Nk = grid->Nk;
+    a.N = grid->Nb;
+    a.eps = grid->eps;
+    a.h = (const c64*)h;
+    if (fused) {
+        // The family sum happens in the kernels' loads, in the reference order.
+        for (int f = 0; f < nfam; ++f) {
+            a.famR[f] = (const c64*)famR[f];
+            a.famL[f] = (const c64*)famL[f];
+            a.famG[f] = (const c64*)famG[f];
+        }
+        a.nfamR = a.nfamL = a.nfamG = nfam;
+    } else {
         c64* tot = (c64*)(ws + tr_bytes);
         const double* const* fams[3] = {famR, famL, famG};
         for (int k = 0; k < 3; ++k) {
@@ -700,19 +877,11 @@ int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
         }
         int rc = launch_check(ctx, "k_sum_fields");
         if (rc) return rc;
-        tR = tot;
-        tL = tot + n;
-        tG = tot + 2 * n;
+        a.famR[0] = tot;
+        a.famL[0] = tot + n;
+        a.famG[0] = tot + 2 * n;
+        a.nfamR = a.nfamL = a.nfamG = 1;
     }
-    DKArgs a;
-    a.NE = grid->NE;
-    a.Nk = grid->Nk;
-    a.N = grid->Nb;
-    a.eps = grid->eps;
-    a.h = (const c64*)h;
-    a.sigR = tR;
-    a.sigL = tL;
-    a.sigG = tG;
     a.eta = 0.0;
     a.GR = (c64*)GR;
     a.GL = (c64*)GL;
@@ -721,7 +890,7 @@ int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
     a.traces = traces;
     a.status = ctx->status;
     a.keldysh = 1;
-    int rc = dyson_keldysh_dispatch(ctx, a);
+    int rc = dyson_keldysh_dispatch(ctx, a, true);
     if (rc) return rc;
     hipLaunchKernelGGL(k_number_pair, dim3(1), dim3(kReduceThreads), 0, ctx->stream, traces, 2, grid->NE,
                        grid->Nk, grid->wE, grid->wk, grid->gs, target, 1, ctx->coef, ctx->status);

@@ -7,10 +7,11 @@
 // (L = nextpow2(2NE-1)), extracting y[e + NE - 1]. The round-off witness
 // repeats the transform with 2L and reports ||L2 - L|| / (||L|| + 1e-14).
 //
-// One workgroup per column. Each thread keeps 16 complex values in registers
-// (T = L/16 threads); every Stockham pass does radix-R butterflies in
-// registers and exchanges data through LDS one real component at a time, so
-// the LDS footprint is L doubles.
+// One workgroup per column. Each thread keeps 8 complex values in registers
+// (T = L/8 threads, 16 values above L = 8192); every Stockham pass does
+// radix-R butterflies in registers and exchanges data through LDS one real
+// component at a time, so the LDS footprint is L doubles. FFT lengths are
+// template parameters: all index arithmetic is shifts and masks.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -24,8 +25,6 @@ struct qclhip_hilbert {
     long long ncols = 0;
     int L = 0;
     int verify = 0;
-    int radix[2][16] = {};  // pass radices for L and 2L
-    int npass[2] = {0, 0};
     qclhip::c64* twiddle = nullptr;  // exp(-2 pi i idx / Lmax), idx < Lmax
     qclhip::c64* kf[2] = {nullptr, nullptr};  // kernel spectra for L and 2L
     int Lmax = 0;
@@ -100,19 +99,38 @@ __device__ inline void dft(double* re, double* im) {
     }
 }
 
-template <int V, int R, bool INV>
-__device__ inline void butterflies(double* re, double* im, int t, int T, int L, int Ns,
-                                   const c64* __restrict__ tw, int twstride) {
+// ---- compile-time Stockham plan for a power-of-two length L ---------------
+// Passes use the base radix except the last, which takes the remaining factor; the
+// inverse runs the radices in reverse order so that the forward outputs are
+// already in the inverse's input order.
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x >> 1); }
+// 8 values per thread (radix-8 passes) up to L = 8192 (1024 threads); 16
+// above that.
+constexpr int vals_per_thread(int L) { return L < 8 ? L : (L <= 8192 ? 8 : 16); }
+constexpr int threads_for(int L) { return L / vals_per_thread(L); }
+constexpr int base_log(int L) { return ilog2c(vals_per_thread(L)); }
+constexpr int npasses(int L) { return (ilog2c(L) + base_log(L) - 1) / base_log(L); }
+constexpr int fwd_radix(int L, int p) {
+    return p < npasses(L) - 1 ? vals_per_thread(L) : (1 << (ilog2c(L) - base_log(L) * (npasses(L) - 1)));
+}
+constexpr int inv_radix(int L, int q) { return fwd_radix(L, npasses(L) - 1 - q); }
+constexpr int fwd_span(int L, int p) { return p == 0 ? 1 : fwd_span(L, p - 1) * fwd_radix(L, p - 1); }
+constexpr int inv_span(int L, int q) { return q == 0 ? 1 : inv_span(L, q - 1) * inv_radix(L, q - 1); }
+
+// One Stockham pass: butterfly j (inputs j + r L/R) with twiddles
+// exp(-+2 pi i r (j mod NS) / (NS R)) and an in-register DFT of R points.
+template <int L, int R, int NS, bool INV>
+__device__ inline void stockham_pass(double* re, double* im, int t, const c64* __restrict__ tw, int twstride) {
+    constexpr int V = vals_per_thread(L), T = threads_for(L);
 #pragma unroll
     for (int i = 0; i < V / R; ++i) {
         const int j = t + i * T;
-        if (j >= L / R) break;
-        const int k = j % Ns;
-        if (Ns > 1) {
-            const int step = (L / (Ns * R)) * twstride;
+        if (NS > 1) {
+            const int k = j & (NS - 1);
+            const int step = (L / (NS * R)) * twstride;
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                c64 w = tw[r * k * step];
+                const c64 w = tw[r * k * step];
                 const double wi = INV ? -w.y : w.y;
                 const double xr = re[i * R + r], xi = im[i * R + r];
                 re[i * R + r] = w.x * xr - wi * xi;
@@ -123,66 +141,48 @@ __device__ inline void butterflies(double* re, double* im, int t, int T, int L, 
     }
 }
 
-template <int V>
-__device__ inline void run_butterflies(int R, bool inv, double* re, double* im, int t, int T, int L,
-                                       int Ns, const c64* tw, int twstride) {
-    if (inv) {
-        switch (R) {
-            case 16: butterflies<V, 16, true>(re, im, t, T, L, Ns, tw, twstride); break;
-            case 8: butterflies<V, 8, true>(re, im, t, T, L, Ns, tw, twstride); break;
-            case 4: butterflies<V, 4, true>(re, im, t, T, L, Ns, tw, twstride); break;
-            default: butterflies<V, 2, true>(re, im, t, T, L, Ns, tw, twstride); break;
+// Redistribute through LDS (one real component at a time) from the outputs
+// of a pass (radix R, span NS) to the inputs of the next pass (radix RN).
+template <int L, int R, int NS, int RN>
+__device__ inline void stockham_exchange(double* re, double* im, double* lds, int t) {
+    constexpr int V = vals_per_thread(L), T = threads_for(L);
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp) {
+        double* v = comp ? im : re;
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < V; ++f) {
+            const int i = f / R, r = f % R;
+            const int j = t + i * T;
+            lds[(j / NS) * NS * R + (j & (NS - 1)) + r * NS] = v[f];
         }
-    } else {
-        switch (R) {
-            case 16: butterflies<V, 16, false>(re, im, t, T, L, Ns, tw, twstride); break;
-            case 8: butterflies<V, 8, false>(re, im, t, T, L, Ns, tw, twstride); break;
-            case 4: butterflies<V, 4, false>(re, im, t, T, L, Ns, tw, twstride); break;
-            default: butterflies<V, 2, false>(re, im, t, T, L, Ns, tw, twstride); break;
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < V; ++f) {
+            const int i = f / RN, r = f % RN;
+            const int j = t + i * T;
+            v[f] = lds[j + r * (L / RN)];
         }
     }
 }
 
-// Index of value (i, r) after a pass of radix R with span Ns.
-__device__ inline int out_index(int j, int r, int R, int Ns) { return (j / Ns) * Ns * R + (j % Ns) + r * Ns; }
-
-// Full-length transform. On entry the registers hold the inputs of the first
-// pass (value (i,r) = x[j + r*L/R0]); on exit the outputs of the last pass
-// (value (i,r) at index j + r*L/Rlast).
-template <int V>
-__device__ void fft_passes(double* re, double* im, double* lds, int t, int T, int L, const int* radix,
-                           int npass, bool inv, const c64* tw, int twstride) {
-    int Ns = 1;
-    for (int p = 0; p < npass; ++p) {
-        const int R = inv ? radix[npass - 1 - p] : radix[p];
-        run_butterflies<V>(R, inv, re, im, t, T, L, Ns, tw, twstride);
-        if (p == npass - 1) break;
-        const int Rn = inv ? radix[npass - 2 - p] : radix[p + 1];
-        // Register index f = i*R + r is static in every unrolled loop below.
-        for (int comp = 0; comp < 2; ++comp) {
-            double* v = comp ? im : re;
-            __syncthreads();
-#pragma unroll
-            for (int f = 0; f < V; ++f) {
-                const int i = f / R, r = f % R;
-                const int j = t + i * T;
-                if (j < L / R) lds[out_index(j, r, R, Ns)] = v[f];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int f = 0; f < V; ++f) {
-                const int i = f / Rn, r = f % Rn;
-                const int j = t + i * T;
-                if (j < L / Rn) v[f] = lds[j + r * (L / Rn)];
-            }
+template <int L, int P, bool INV>
+struct StockhamPasses {
+    static __device__ inline void run(double* re, double* im, double* lds, int t, const c64* tw, int twstride) {
+        constexpr int NP = npasses(L);
+        constexpr int R = INV ? inv_radix(L, P) : fwd_radix(L, P);
+        constexpr int NS = INV ? inv_span(L, P) : fwd_span(L, P);
+        stockham_pass<L, R, NS, INV>(re, im, t, tw, twstride);
+        if constexpr (P + 1 < NP) {
+            constexpr int RN = INV ? inv_radix(L, P + 1) : fwd_radix(L, P + 1);
+            stockham_exchange<L, R, NS, RN>(re, im, lds, t);
+            StockhamPasses<L, P + 1, INV>::run(re, im, lds, t, tw, twstride);
         }
-        Ns *= R;
     }
-}
+};
 
 struct HilbertArgs {
-    int NE, L, npass;
-    int radix[16];
+    int NE;
     long long ncols;
     const double* wE;
     const c64* gamma;   // Gamma input, or NULL when (SL, SG) are given
@@ -205,62 +205,54 @@ __device__ inline c64 load_gamma(const HilbertArgs& a, size_t idx) {
     return cmk(-(g.y - l.y), g.x - l.x);
 }
 
-template <int V>
-__global__ __launch_bounds__(512) void k_hilbert(HilbertArgs a) {
+template <int L>
+__global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
+    constexpr int V = vals_per_thread(L), T = threads_for(L);
+    constexpr int NPASS = npasses(L);
+    constexpr int R0 = fwd_radix(L, 0), RL = fwd_radix(L, NPASS - 1);
     extern __shared__ __align__(16) double lds[];
-    const int t = threadIdx.x, T = blockDim.x;
+    const int t = threadIdx.x;
     const long long col = blockIdx.x;
-    const int L = a.L, NE = a.NE;
+    const int NE = a.NE;
     const size_t base = (size_t)col * NE;
     double re[V], im[V];
-    // Load wE*Gamma, zero padded, into the first pass's input order.
-    {
-        const int R0 = a.radix[0];
-#pragma unroll
-        for (int f = 0; f < V; ++f) {
-            const int i = f / R0, r = f % R0;
-            const int j = t + i * T;
-            const int e = j + r * (L / R0);
-            double xr = 0.0, xi = 0.0;
-            if (j < L / R0 && e < NE) {
-                c64 gm = load_gamma(a, base + e);
-                xr = a.wE[e] * gm.x;
-                xi = a.wE[e] * gm.y;
-            }
-            re[f] = xr;
-            im[f] = xi;
-        }
-    }
-    fft_passes<V>(re, im, lds, t, T, L, a.radix, a.npass, false, a.tw, a.twstride);
-    // Multiply by the kernel spectrum (outputs at j + r*L/Rlast).
-    const int Rl = a.radix[a.npass - 1];
+    // Load wE*Gamma, zero padded, in the first pass's input order.
 #pragma unroll
     for (int f = 0; f < V; ++f) {
-        const int i = f / Rl, r = f % Rl;
-        const int j = t + i * T;
-        if (j < L / Rl) {
-            c64 k = a.kf[j + r * (L / Rl)];
-            const double xr = re[f], xi = im[f];
-            re[f] = k.x * xr - k.y * xi;
-            im[f] = k.x * xi + k.y * xr;
+        const int i = f / R0, r = f % R0;
+        const int e = t + i * T + r * (L / R0);
+        double xr = 0.0, xi = 0.0;
+        if (e < NE) {
+            const c64 gm = load_gamma(a, base + e);
+            xr = a.wE[e] * gm.x;
+            xi = a.wE[e] * gm.y;
         }
+        re[f] = xr;
+        im[f] = xi;
     }
-    // Inverse transform (radices in reverse order, so no exchange is needed).
-    fft_passes<V>(re, im, lds, t, T, L, a.radix, a.npass, true, a.tw, a.twstride);
+    StockhamPasses<L, 0, false>::run(re, im, lds, t, a.tw, a.twstride);
+    // Kernel spectrum, at the forward outputs j + r L/RL.
+#pragma unroll
+    for (int f = 0; f < V; ++f) {
+        const int i = f / RL, r = f % RL;
+        const c64 k = a.kf[t + i * T + r * (L / RL)];
+        const double xr = re[f], xi = im[f];
+        re[f] = k.x * xr - k.y * xi;
+        im[f] = k.x * xi + k.y * xr;
+    }
+    StockhamPasses<L, 0, true>::run(re, im, lds, t, a.tw, a.twstride);
+    // Outputs of the inverse sit at j + r L/R0; Lambda_e = y[e + NE - 1] / L.
     const double invL = 1.0 / (double)L;
-    const int Rf = a.radix[0];
     double dn = 0.0, nn = 0.0;
 #pragma unroll
     for (int f = 0; f < V; ++f) {
-        const int i = f / Rf, r = f % Rf;
-        const int j = t + i * T;
-        const int idx = j + r * (L / Rf);
-        const int e = idx - (NE - 1);
-        if (j >= L / Rf || e < 0 || e >= NE) continue;
-        c64 lam = cmk(re[f] * invL, im[f] * invL);
+        const int i = f / R0, r = f % R0;
+        const int e = t + i * T + r * (L / R0) - (NE - 1);
+        if (e < 0 || e >= NE) continue;
+        const c64 lam = cmk(re[f] * invL, im[f] * invL);
         const size_t gi = base + e;
         if (a.mode == 2) {
-            c64 ref = a.lambda_ref[gi];
+            const c64 ref = a.lambda_ref[gi];
             const double dx = lam.x - ref.x, dy = lam.y - ref.y;
             dn += dx * dx + dy * dy;
             nn += ref.x * ref.x + ref.y * ref.y;
@@ -268,24 +260,23 @@ __global__ __launch_bounds__(512) void k_hilbert(HilbertArgs a) {
         }
         if (a.lambda_copy) a.lambda_copy[gi] = lam;
         if (a.mode == 1) {
-            c64 gm = load_gamma(a, gi);
-            // S^R = Lambda - 0.5 i Gamma
-            a.out[gi] = cmk(lam.x + 0.5 * gm.y, lam.y - 0.5 * gm.x);
+            const c64 gm = load_gamma(a, gi);
+            a.out[gi] = cmk(lam.x + 0.5 * gm.y, lam.y - 0.5 * gm.x);  // Lambda - 0.5 i Gamma
         } else {
             a.out[gi] = lam;
         }
     }
     if (a.mode != 2) return;
-    // Deterministic block reduction of the witness sums (fixed tree in LDS;
-    // the transform no longer needs it).
+    // Deterministic block reduction of the witness sums (fixed tree in LDS).
     double* rd = lds;
     double* rn = lds + T;
     __syncthreads();
     rd[t] = dn;
     rn[t] = nn;
     __syncthreads();
-    for (int width = 256; width > 0; width >>= 1) {
-        if (t < width && t + width < T) {
+#pragma unroll
+    for (int width = T / 2; width > 0; width >>= 1) {
+        if (t < width) {
             rd[t] += rd[t + width];
             rn[t] += rn[t + width];
         }
@@ -317,17 +308,6 @@ __global__ __launch_bounds__(1024) void k_witness_reduce(const double* partial, 
         __syncthreads();
     }
     if (t == 0) out[0] = sqrt(red[0][0]) / (sqrt(red[1][0]) + 1e-14);
-}
-
-// Host: radices for a power-of-two length (16s first, remainder last).
-int plan_radices(int L, int* radix) {
-    int n = 0, rest = L;
-    while (rest >= 16 && (rest % 16) == 0) {
-        radix[n++] = 16;
-        rest /= 16;
-    }
-    if (rest > 1) radix[n++] = rest;  // 2, 4 or 8
-    return n;
 }
 
 // Host FFT in long double for the plan tables.
@@ -371,27 +351,41 @@ std::vector<c64> kernel_spectrum(int NE, int L, double de) {
 
 }  // namespace
 
+template <int L>
+int launch_hilbert_len(qclhip_ctx* ctx, long long ncols, const HilbertArgs& a) {
+    constexpr int T = threads_for(L);
+    const size_t lds = (size_t)(L > 2 * T ? L : 2 * T) * sizeof(double);
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "FFT length exceeds the LDS of one workgroup");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_hilbert<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                (int)lds));
+    hipLaunchKernelGGL(k_hilbert<L>, dim3((unsigned)ncols), dim3(T), lds, ctx->stream, a);
+    return launch_check(ctx, "k_hilbert");
+}
+
 int hilbert_launch(qclhip_ctx* ctx, const qclhip_hilbert* p, const HilbertArgs& a_in, int which) {
     HilbertArgs a = a_in;
-    a.L = which ? 2 * p->L : p->L;
-    a.npass = p->npass[which];
-    for (int i = 0; i < 16; ++i) a.radix[i] = p->radix[which][i];
+    const int L = which ? 2 * p->L : p->L;
     a.kf = p->kf[which];
     a.tw = p->twiddle;
-    a.twstride = p->Lmax / a.L;
-    const int Vv = a.L <= 8192 ? 16 : 32;
-    const int T = std::max(1, a.L / Vv);
-    const size_t lds = (size_t)std::max(a.L, 2 * T) * sizeof(double);
-    if (lds > ctx->max_lds_per_block || T > 512)
-        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "FFT length exceeds the LDS of one workgroup");
-    const void* fn = Vv == 16 ? (const void*)k_hilbert<16> : (const void*)k_hilbert<32>;
-    if (lds > 65536)
-        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (Vv == 16)
-        hipLaunchKernelGGL(k_hilbert<16>, dim3((unsigned)p->ncols), dim3(T), lds, ctx->stream, a);
-    else
-        hipLaunchKernelGGL(k_hilbert<32>, dim3((unsigned)p->ncols), dim3(T), lds, ctx->stream, a);
-    return launch_check(ctx, "k_hilbert");
+    a.twstride = p->Lmax / L;
+    switch (L) {
+        case 4: return launch_hilbert_len<4>(ctx, p->ncols, a);
+        case 8: return launch_hilbert_len<8>(ctx, p->ncols, a);
+        case 16: return launch_hilbert_len<16>(ctx, p->ncols, a);
+        case 32: return launch_hilbert_len<32>(ctx, p->ncols, a);
+        case 64: return launch_hilbert_len<64>(ctx, p->ncols, a);
+        case 128: return launch_hilbert_len<128>(ctx, p->ncols, a);
+        case 256: return launch_hilbert_len<256>(ctx, p->ncols, a);
+        case 512: return launch_hilbert_len<512>(ctx, p->ncols, a);
+        case 1024: return launch_hilbert_len<1024>(ctx, p->ncols, a);
+        case 2048: return launch_hilbert_len<2048>(ctx, p->ncols, a);
+        case 4096: return launch_hilbert_len<4096>(ctx, p->ncols, a);
+        case 8192: return launch_hilbert_len<8192>(ctx, p->ncols, a);
+        case 16384: return launch_hilbert_len<16384>(ctx, p->ncols, a);
+        default: return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "FFT length outside 4..16384");
+    }
 }
 
 int hilbert_run(qclhip_ctx* ctx, const qclhip_hilbert* p, const double* Gamma, const double* SL,
@@ -462,8 +456,6 @@ int qclhip_hilbert_create(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t ncol
     p->L = L;
     p->Lmax = Lmax;
     p->verify = verify ? 1 : 0;
-    p->npass[0] = plan_radices(L, p->radix[0]);
-    p->npass[1] = plan_radices(2 * L, p->radix[1]);
     // Twiddles exp(-2 pi i idx/Lmax) in long double, rounded once.
     std::vector<c64> tw(Lmax);
     const long double pi = 3.141592653589793238462643383279502884L;

@@ -68,15 +68,10 @@ int qclhip_ctx_create(int device, void* stream, qclhip_ctx** out) {
     if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, device) == hipSuccess &&
         (size_t)optin > ctx->max_lds_per_block)
         ctx->max_lds_per_block = (size_t)optin;
-    if (stream) {
-        ctx->stream = (hipStream_t)stream;
-    } else {
-        if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-            delete ctx;
-            return QCLHIP_ERR_RUNTIME;
-        }
-        ctx->owns_stream = true;
-    }
+    // The caller's stream; NULL is HIP's default stream (torch's default
+    // stream), so work stays ordered with the caller's own kernels and copies.
+    ctx->stream = (hipStream_t)stream;
+    ctx->owns_stream = false;
     bool ok = rocblas_create_handle(&ctx->blas) == rocblas_status_success &&
               rocblas_set_stream(ctx->blas, ctx->stream) == rocblas_status_success &&
               rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host) == rocblas_status_success;
