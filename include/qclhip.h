@@ -180,6 +180,12 @@ int qclhip_embedding_component(qclhip_ctx* ctx, const qclhip_grid* grid, const d
 int qclhip_sum_fields(qclhip_ctx* ctx, int64_t n, int nx, const double* const* x, double* out);
 /* old <- (1-alpha) old + alpha cand. */
 int qclhip_mix(qclhip_ctx* ctx, int64_t n, double* old_, const double* cand, double alpha);
+/* The same for npairs (old[p], cand[p]) field pairs in one pass
+ * (_mix_mechanisms_production! and the plus/minus families,
+ * production_solver.jl:819-832); old and cand are HOST arrays of device
+ * pointers. */
+int qclhip_mix_fields(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
+                      const double* const* cand, double alpha);
 
 /* ---- A14/A15: observables (production_iteration.jl:174-215) */
 /* Jbar = Re[g_s/(2 pi) sum_{e sorted by (|e|,e), m} wE wk tr(S>+ G< - S<+ G>)]. */

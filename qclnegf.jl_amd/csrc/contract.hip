@@ -91,6 +91,121 @@ int gemm_nt(qclhip_ctx* ctx, long long m, long long n, long long k, double alpha
                       "rocblas_zgemm");
 }
 
+// Streaming kernels for the thin factors (rank J or r of a few), where a
+// plain GEMM library tiles poorly: each thread owns one row of the tall
+// operand and walks its long dimension, so every load and store is
+// coalesced across threads and the field is read or written exactly once.
+constexpr int kMaxRank = 16;
+
+// C[row, j] = sum_q A[row + M q] * Bm[j + J q]  (q < K), J <= kMaxRank.
+__global__ void k_thin_reduce(long long M, int K, int J, const c64* __restrict__ A, const c64* __restrict__ Bm,
+                              c64* __restrict__ C) {
+    extern __shared__ c64 sb[];
+    for (int idx = threadIdx.x; idx < J * K; idx += blockDim.x) sb[idx] = Bm[idx];
+    __syncthreads();
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= M) return;
+    c64 acc[kMaxRank];
+#pragma unroll
+    for (int j = 0; j < kMaxRank; ++j) acc[j] = cmk(0.0, 0.0);
+    for (int q = 0; q < K; ++q) {
+        const c64 a = A[row + M * q];
+#pragma unroll
+        for (int j = 0; j < kMaxRank; ++j)
+            if (j < J) acc[j] = cfma(acc[j], a, sb[j + J * q]);
+    }
+    for (int j = 0; j < J; ++j) C[row + M * j] = acc[j];
+}
+
+// S[row + M q] = f * sum_j C[row + M j] * Bm[q + K j] + beta * S  (q < K).
+__global__ void k_thin_expand(long long M, int K, int J, const c64* __restrict__ C, const c64* __restrict__ Bm,
+                              double f, double beta, c64* __restrict__ S) {
+    extern __shared__ c64 sb[];
+    for (int idx = threadIdx.x; idx < J * K; idx += blockDim.x) sb[idx] = Bm[idx];
+    __syncthreads();
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= M) return;
+    c64 c[kMaxRank];
+#pragma unroll
+    for (int j = 0; j < kMaxRank; ++j) c[j] = (j < J) ? C[row + M * j] : cmk(0.0, 0.0);
+    for (int q = 0; q < K; ++q) {
+        c64 v = cmk(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < kMaxRank; ++j)
+            if (j < J) v = cfma(v, c[j], sb[q + K * j]);
+        v = cmk(f * v.x, f * v.y);
+        if (beta != 0.0) {
+            const c64 o = S[row + M * q];
+            v = cmk(beta * o.x + v.x, beta * o.y + v.y);
+        }
+        S[row + M * q] = v;
+    }
+}
+
+// Long reduction of the low-rank factor: partial[chunk][e][i] over a chunk
+// of the compound index q (rows e < NE, ranks i < r).
+constexpr int kQChunk = 256;
+__global__ void k_lowrank_partial(int NE, long long Nc, int r, const c64* __restrict__ G,
+                                  const c64* __restrict__ RAw, c64* __restrict__ partial) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q0 = (long long)blockIdx.y * kQChunk;
+    if (e >= NE) return;
+    c64 acc[kMaxRank];
+#pragma unroll
+    for (int i = 0; i < kMaxRank; ++i) acc[i] = cmk(0.0, 0.0);
+    const long long q1 = min(Nc, q0 + kQChunk);
+    for (long long q = q0; q < q1; ++q) {
+        const c64 g = G[e + (long long)NE * q];
+#pragma unroll
+        for (int i = 0; i < kMaxRank; ++i)
+            if (i < r) acc[i] = cfma(acc[i], g, RAw[i + (long long)r * q]);
+    }
+    for (int i = 0; i < r; ++i) partial[((long long)blockIdx.y * r + i) * NE + e] = acc[i];
+}
+
+// red[e + NE i] = sum over chunks, in chunk order.
+__global__ void k_lowrank_sum(int NE, int r, int nchunk, const c64* __restrict__ partial, c64* __restrict__ red) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)NE * r) return;
+    const int e = (int)(idx % NE), i = (int)(idx / NE);
+    c64 v = cmk(0.0, 0.0);
+    for (int ch = 0; ch < nchunk; ++ch) v = cadd(v, partial[((long long)ch * r + i) * NE + e]);
+    red[e + (long long)NE * i] = v;
+}
+
+// S2[e + NE q] = f sum_i red[e + NE i] left[q + Nc i] + beta S2.
+__global__ void k_lowrank_expand(int NE, long long Nc, int r, const c64* __restrict__ red,
+                                 const c64* __restrict__ left, double f, double beta, c64* __restrict__ S) {
+    const long long n = (long long)NE * Nc;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int e = (int)(idx % NE);
+        const long long q = idx / NE;
+        c64 v = cmk(0.0, 0.0);
+        for (int i = 0; i < r; ++i) v = cfma(v, red[e + (long long)NE * i], left[q + Nc * i]);
+        v = cmk(f * v.x, f * v.y);
+        if (beta != 0.0) {
+            const c64 o = S[idx];
+            v = cmk(beta * o.x + v.x, beta * o.y + v.y);
+        }
+        S[idx] = v;
+    }
+}
+
+// IFR radial step: C2[(e,m), j] = sum_m' Rw[m, m'] C1[(e,m'), j] for every (e, j).
+__global__ void k_ifr_radial(int NE, int Nk, int J, const c64* __restrict__ C1, const c64* __restrict__ Rw,
+                             c64* __restrict__ C2) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long M = (long long)NE * Nk;
+    if (idx >= M * J) return;
+    const int e = (int)(idx % NE);
+    const long long rest = idx / NE;
+    const int m = (int)(rest % Nk), j = (int)(rest / Nk);
+    c64 v = cmk(0.0, 0.0);
+    for (int mp = 0; mp < Nk; ++mp) v = cfma(v, Rw[m + (long long)Nk * mp], C1[e + (long long)NE * mp + M * j]);
+    C2[e + (long long)NE * m + M * j] = v;
+}
+
 }  // namespace
 
 // Device scratch that contract_core takes from the context workspace.
@@ -98,7 +213,7 @@ size_t core_scratch_bytes(const qclhip_kernel* K) {
     const qclhip_grid* g = K->grid;
     const size_t NE = g->NE, Nk = g->Nk, N2 = (size_t)g->Nb * g->Nb, r = (size_t)std::max(1, K->rank);
     switch (K->kind) {
-        case QCLHIP_KERNEL_LOW_RANK: return NE * r * sizeof(c64);
+        case QCLHIP_KERNEL_LOW_RANK: return (NE * r + ((Nk * N2 + 255) / 256) * r * NE) * sizeof(c64);
         case QCLHIP_KERNEL_MOMENTUM_INDEPENDENT: return 2 * NE * N2 * sizeof(c64);
         case QCLHIP_KERNEL_SEPARABLE_IFR: return 2 * NE * Nk * r * sizeof(c64);
         default: return 0;
@@ -114,11 +229,27 @@ int contract_core(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double 
             return gemm_nt(ctx, NE, Nc, Nc, f, G, NE, K->a, Nc, beta, S, NE);
         case QCLHIP_KERNEL_LOW_RANK: {
             const long long r = K->rank;
-            c64* red = (c64*)workspace(ctx, (size_t)NE * r * sizeof(c64));
+            if (r > kMaxRank) {
+                c64* red = (c64*)workspace(ctx, (size_t)NE * r * sizeof(c64));
+                if (!red) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+                int st = gemm_nt(ctx, NE, r, Nc, 1.0, G, NE, K->b, r, 0.0, red, NE);
+                if (st) return st;
+                return gemm_nt(ctx, NE, Nc, r, f, red, NE, K->a, Nc, beta, S, NE);
+            }
+            const long long nchunk = (Nc + kQChunk - 1) / kQChunk;
+            c64* red = (c64*)workspace(ctx, (size_t)(NE * r + nchunk * r * NE) * sizeof(c64));
             if (!red) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
-            int st = gemm_nt(ctx, NE, r, Nc, 1.0, G, NE, K->b, r, 0.0, red, NE);
-            if (st) return st;
-            return gemm_nt(ctx, NE, Nc, r, f, red, NE, K->a, Nc, beta, S, NE);
+            c64* partial = red + NE * r;
+            hipLaunchKernelGGL(k_lowrank_partial, dim3((unsigned)((NE + 255) / 256), (unsigned)nchunk), dim3(256), 0,
+                               ctx->stream, (int)NE, Nc, (int)r, G, K->b, partial);
+            hipLaunchKernelGGL(k_lowrank_sum, dim3((unsigned)((NE * r + 255) / 256)), dim3(256), 0, ctx->stream,
+                               (int)NE, (int)r, (int)nchunk, partial, red);
+            // S2 (NE x Nc) = f red left^T + beta S2, streamed over q.
+            const size_t lds = 0;
+            (void)lds;
+            hipLaunchKernelGGL(k_lowrank_expand, dim3(grid_blocks(NE * Nc, 256, ctx->sm_count)), dim3(256), 0,
+                               ctx->stream, (int)NE, Nc, (int)r, red, K->a, f, beta, S);
+            return launch_check(ctx, "low-rank contraction");
         }
         case QCLHIP_KERNEL_MOMENTUM_INDEPENDENT: {
             c64* X = (c64*)workspace(ctx, (size_t)2 * NE * N2 * sizeof(c64));
@@ -136,21 +267,20 @@ int contract_core(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double 
         }
         case QCLHIP_KERNEL_SEPARABLE_IFR: {
             const long long J = K->rank, M = NE * Nk;
+            if (J > kMaxRank || (size_t)J * N2 * sizeof(c64) > 60 * 1024)
+                return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "separable kernel rank too large for this build");
             c64* C1 = (c64*)workspace(ctx, (size_t)2 * M * J * sizeof(c64));
             if (!C1) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
             c64* C2 = C1 + M * J;
-            int st = gemm_nt(ctx, M, J, N2, 1.0, G, M, K->b, J, 0.0, C1, M);
-            if (st) return st;
-            rocblas_double_complex one = rc(1.0), zero = rc(0.0);
-            st = blas_check(ctx,
-                            rocblas_zgemm_strided_batched(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
-                                                          (rocblas_int)NE, (rocblas_int)Nk, (rocblas_int)Nk, &one,
-                                                          B_(C1), (rocblas_int)NE, (rocblas_stride)M, B_(K->c),
-                                                          (rocblas_int)Nk, (rocblas_stride)0, &zero, B_(C2),
-                                                          (rocblas_int)NE, (rocblas_stride)M, (rocblas_int)J),
-                            "rocblas_zgemm_strided_batched");
-            if (st) return st;
-            return gemm_nt(ctx, M, N2, J, f, C2, M, K->a, N2, beta, S, M);
+            const size_t lds = (size_t)J * N2 * sizeof(c64);
+            const unsigned blocks = (unsigned)((M + 255) / 256);
+            hipLaunchKernelGGL(k_thin_reduce, dim3(blocks), dim3(256), lds, ctx->stream, M, (int)N2, (int)J, G,
+                               K->b, C1);
+            hipLaunchKernelGGL(k_ifr_radial, dim3((unsigned)((M * J + 255) / 256)), dim3(256), 0, ctx->stream,
+                               (int)NE, (int)Nk, (int)J, C1, K->c, C2);
+            hipLaunchKernelGGL(k_thin_expand, dim3(blocks), dim3(256), lds, ctx->stream, M, (int)N2, (int)J, C2,
+                               K->a, f, beta, S);
+            return launch_check(ctx, "separable contraction");
         }
         default:
             return fail(ctx, QCLHIP_ERR_ARGUMENT, "unknown kernel representation");

@@ -703,6 +703,25 @@ __global__ void k_mix(long long n, c64* __restrict__ old_, const c64* __restrict
     }
 }
 
+// All (old, candidate) pairs of one mixing phase in a single pass.
+constexpr int kMaxMixPairs = 32;
+struct MixArgs {
+    c64* old_[kMaxMixPairs];
+    const c64* cand[kMaxMixPairs];
+    int npairs;
+};
+
+__global__ void k_mix_many(long long n, MixArgs a, double alpha) {
+    const double beta = 1.0 - alpha;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        for (int p = 0; p < a.npairs; ++p) {
+            const c64 o = a.old_[p][i], c = a.cand[p][i];
+            a.old_[p][i] = cmk(beta * o.x + alpha * c.x, beta * o.y + alpha * c.y);
+        }
+    }
+}
+
 inline unsigned grid_for(long long n, int threads, int sm_count) {
     long long blocks = (n + threads - 1) / threads;
     long long cap = (long long)sm_count * 32;
@@ -1004,6 +1023,24 @@ int qclhip_sum_fields(qclhip_ctx* ctx, int64_t n, int nx, const double* const* x
         done += sa.nx;
     }
     return launch_check(ctx, "k_sum_fields");
+}
+
+int qclhip_mix_fields(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_, const double* const* cand,
+                      double alpha) {
+    if (!ctx || !old_ || !cand || n < 0 || npairs < 0) return QCLHIP_ERR_ARGUMENT;
+    if (!(alpha > 0 && alpha <= 1)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "mixing factor must lie in (0,1]");
+    for (int done = 0; done < npairs;) {
+        MixArgs a;
+        a.npairs = std::min(kMaxMixPairs, npairs - done);
+        for (int p = 0; p < a.npairs; ++p) {
+            a.old_[p] = (c64*)old_[done + p];
+            a.cand[p] = (const c64*)cand[done + p];
+        }
+        hipLaunchKernelGGL(k_mix_many, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
+                           (long long)n, a, alpha);
+        done += a.npairs;
+    }
+    return launch_check(ctx, "k_mix_many");
 }
 
 int qclhip_mix(qclhip_ctx* ctx, int64_t n, double* old_, const double* cand, double alpha) {

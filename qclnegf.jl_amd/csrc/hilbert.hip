@@ -104,9 +104,9 @@ __device__ inline void dft(double* re, double* im) {
 // inverse runs the radices in reverse order so that the forward outputs are
 // already in the inverse's input order.
 constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x >> 1); }
-// 8 values per thread (radix-8 passes) up to L = 8192 (1024 threads); 16
-// above that.
-constexpr int vals_per_thread(int L) { return L < 8 ? L : (L <= 8192 ? 8 : 16); }
+// 16 values per thread (radix-16 passes) up to L = 4096, 8 (radix 8, 1024
+// threads) at 8192 and 16 above.
+constexpr int vals_per_thread(int L) { return L < 16 ? L : (L == 8192 ? 8 : 16); }
 constexpr int threads_for(int L) { return L / vals_per_thread(L); }
 constexpr int base_log(int L) { return ilog2c(vals_per_thread(L)); }
 constexpr int npasses(int L) { return (ilog2c(L) + base_log(L) - 1) / base_log(L); }
@@ -141,6 +141,10 @@ __device__ inline void stockham_pass(double* re, double* im, int t, const c64* _
     }
 }
 
+// LDS slot of element x: one pad double per 32 keeps the stride-R writes of
+// the first pass off a single bank.
+__device__ constexpr int lds_slot(int x) { return x + (x >> 5); }
+
 // Redistribute through LDS (one real component at a time) from the outputs
 // of a pass (radix R, span NS) to the inputs of the next pass (radix RN).
 template <int L, int R, int NS, int RN>
@@ -154,14 +158,14 @@ __device__ inline void stockham_exchange(double* re, double* im, double* lds, in
         for (int f = 0; f < V; ++f) {
             const int i = f / R, r = f % R;
             const int j = t + i * T;
-            lds[(j / NS) * NS * R + (j & (NS - 1)) + r * NS] = v[f];
+            lds[lds_slot((j / NS) * NS * R + (j & (NS - 1)) + r * NS)] = v[f];
         }
         __syncthreads();
 #pragma unroll
         for (int f = 0; f < V; ++f) {
             const int i = f / RN, r = f % RN;
             const int j = t + i * T;
-            v[f] = lds[j + r * (L / RN)];
+            v[f] = lds[lds_slot(j + r * (L / RN))];
         }
     }
 }
@@ -354,7 +358,7 @@ std::vector<c64> kernel_spectrum(int NE, int L, double de) {
 template <int L>
 int launch_hilbert_len(qclhip_ctx* ctx, long long ncols, const HilbertArgs& a) {
     constexpr int T = threads_for(L);
-    const size_t lds = (size_t)(L > 2 * T ? L : 2 * T) * sizeof(double);
+    const size_t lds = (size_t)((L + L / 32) > 2 * T ? (L + L / 32) : 2 * T) * sizeof(double);
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "FFT length exceeds the LDS of one workgroup");
     if (lds > 65536)

@@ -84,6 +84,7 @@ _SIGNATURES = {
     "qclhip_embedding_component": [_vp, _vp, _vp, _vp, _vp, _vp],
     "qclhip_sum_fields": [_vp, _i64, ct.c_int, ct.POINTER(_vp), _vp],
     "qclhip_mix": [_vp, _i64, _vp, _vp, ct.c_double],
+    "qclhip_mix_fields": [_vp, _i64, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_double],
     "qclhip_boundary_flux": [_vp, _vp, _vp, _vp, _vp, _vp, _dp],
     "qclhip_state_populations": [_vp, _vp, _vp, ct.c_double, _dp],
     "qclhip_seed_weights": [_vp, _vp, _vp, _dp],

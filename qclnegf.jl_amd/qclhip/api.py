@@ -390,6 +390,17 @@ def mix_family_production(old, cand, alpha: float):
     return old
 
 
+def mix_fields(pairs, alpha: float):
+    """Mix every (old, candidate) pair in one pass: old <- (1-alpha) old + alpha cand."""
+    if not pairs:
+        return
+    ctx = context(pairs[0][0].device.index or 0)
+    n = pairs[0][0].numel()
+    olds = (_vp * len(pairs))(*[o.data_ptr() for o, _ in pairs])
+    cands = (_vp * len(pairs))(*[c.data_ptr() for _, c in pairs])
+    ctx.call("qclhip_mix_fields", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha))
+
+
 def boundary_flux_bar(grid: Grid, plusL, plusG, GL, GG) -> float:
     """_production_boundary_flux_bar (production_iteration.jl:189-201)."""
     v = ct.c_double()

@@ -65,12 +65,21 @@ def new_family(grid):
 
 @dataclass
 class ScbaState:
+    """SCBA state. The embedding total plus + minus is never stored: the Dyson
+    kernels sum the families in their loads in the reference order
+    ((plus + minus) + mechanism_1 + ...), and `embedding()` forms it on request."""
     green: api.GreenState
     scattering: dict
     plus: Family
     minus: Family
-    embedding: Family
     history: list = field(default_factory=list)
+
+    def embedding(self) -> Family:
+        g = self.plus.R
+        out = Family(torch.empty_like(g), torch.empty_like(g), torch.empty_like(g))
+        for a, b, c in zip(self.plus.parts(), self.minus.parts(), out.parts()):
+            api.sum_fields([a, b], c)
+        return out
 
 
 def _candidate_scattering(p: ScbaProblem, green: api.GreenState, out: dict):
@@ -106,11 +115,9 @@ def initial_state(p: ScbaProblem, green: api.GreenState) -> ScbaState:
     g = p.grid
     scat = {m.name: new_family(g) for m in p.mechanisms}
     _candidate_scattering(p, green, scat)
-    plus, minus, emb = new_family(g), new_family(g), new_family(g)
+    plus, minus = new_family(g), new_family(g)
     _candidate_embedding(p, green, plus, minus)
-    for a, b, c in zip(plus.parts(), minus.parts(), emb.parts()):
-        api.sum_fields([a, b], c)
-    return ScbaState(green, scat, plus, minus, emb)
+    return ScbaState(green, scat, plus, minus)
 
 
 class ScbaWorkspace:
@@ -127,7 +134,7 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     """One pass of the production loop (production_solver.jl:435-844)."""
     g = p.grid
     t0 = time.perf_counter()
-    fams = [s.embedding.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
+    fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
     green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=True)
     s.green = green
     t1 = time.perf_counter()
@@ -138,14 +145,12 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
     pops = api.state_populations(g, green.GL, p.target)
     t4 = time.perf_counter()
+    pairs = []
     for m in p.mechanisms:
-        for old, new in zip(s.scattering[m.name].parts(), ws.scat[m.name].parts()):
-            api.mix_family_production(old, new, p.alpha_mix)
+        pairs += list(zip(s.scattering[m.name].parts(), ws.scat[m.name].parts()))
     for fam, cand in ((s.plus, ws.plus), (s.minus, ws.minus)):
-        for old, new in zip(fam.parts(), cand.parts()):
-            api.mix_family_production(old, new, p.alpha_mix)
-    for a, b, c in zip(s.plus.parts(), s.minus.parts(), s.embedding.parts()):
-        api.sum_fields([a, b], c)
+        pairs += list(zip(fam.parts(), cand.parts()))
+    api.mix_fields(pairs, p.alpha_mix)
     g.ctx.synchronize()
     t5 = time.perf_counter()
     if timings is not None:
