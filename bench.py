@@ -27,6 +27,20 @@ for _p in (ROOT, os.path.join(ROOT, "qclnegf.jl_amd")):
         sys.path.insert(0, _p)
 
 METRIC = "GF solves/s + SCBA iter/s, FP32+refine vs FP64 at ≤1e-8 rel. current error"
+# HBM bytes per launch pair (k_dyson + k_keldysh) from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes, written by the profile summary below (None if absent).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gf_solve_traffic.txt")
+
+
+def read_traffic():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -217,8 +231,9 @@ def main():
                    "N": gf["N"], "NE": gf["NE"], "Nk": gf["Nk"], "solves_per_step": gf["nblk"],
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "k_dyson_keldysh<3,2>", "kernel_ms": gf["kernel_ms"],
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": read_traffic(),
+                     "kernel": "k_dyson<3,4> + k_keldysh<3,4> (one GF-solve launch pair)",
+                     "kernel_ms": gf["kernel_ms"],
                      "hbm_achieved_GBs": gf["bytes"] / k_s / 1e9,
                      "hbm_frac": gf["bytes"] / k_s / 1e9 / PEAK_HBM_GBS},
         "number_check": gf["number_check"],
