@@ -70,6 +70,34 @@ int qclhip_grid_create(qclhip_ctx* ctx, int NE, int Nk, int Nb, int gs,
                        const double* wk_host, qclhip_grid** out);
 int qclhip_grid_destroy(qclhip_grid* grid);
 
+/* ---- GF-solve precision (the plan's precision option) --------------------
+ * The reference solves in FP64 only (production_operators.jl:16-66, LAPACK
+ * zgetrf/zgetri). QCLHIP_PRECISION_FP32_REFINE factors each Dyson block in
+ * FP32 and applies refine_steps Newton steps R = I - (D/s) X,
+ * X <- X + X R with the residual in FP64; a block whose predicted residual
+ * ||R_1||^(2^refine_steps) exceeds refine_tol (or whose FP32 elimination
+ * meets a zero pivot) is re-solved in FP64. The Keldysh products stay FP64
+ * (on the engine of qclhip_grid_set_engine). QCLHIP_PRECISION_FP32_PRODUCTS
+ * adds FP32 matrix-core Keldysh products: G</G> then carry FP32 rounding
+ * (~1e-7 relative), and the number functionals of the grid gate realness at
+ * 1e-5 relative instead of the reference's 1e-10 (numerical_invariants.jl:5).
+ * Outputs stay ComplexF64 in the same layout. Applies to retarded_green,
+ * dyson_keldysh and current_green on this grid. refine_steps in [0, 4];
+ * refine_tol <= 0 selects the default 1e-10. */
+#define QCLHIP_PRECISION_FP64 0
+#define QCLHIP_PRECISION_FP32_REFINE 1
+#define QCLHIP_PRECISION_FP32_PRODUCTS 2
+int qclhip_grid_set_precision(qclhip_ctx* ctx, qclhip_grid* grid, int precision,
+                              int refine_steps, double refine_tol);
+/* Engine of the FP64 Keldysh products: vector ALUs (default) or the FP64
+ * matrix cores. Results agree to rounding. */
+#define QCLHIP_ENGINE_VECTOR 0
+#define QCLHIP_ENGINE_MATRIX 1
+int qclhip_grid_set_engine(qclhip_ctx* ctx, qclhip_grid* grid, int keldysh_engine);
+/* Blocks of the grid's last mixed-precision Dyson call that the guard sent
+ * to the FP64 re-solve (0 for an FP64 grid). Synchronizes. */
+int qclhip_grid_refine_fallbacks(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t* count_host);
+
 /* ---- A1: _retarded_green_production (production_operators.jl:1337-1385)
  * G^R = inv(D/s)/s per block, D = (eps_e + i eta) I - h_m - SigmaR_em,
  * s = max row sum |D| (written to scale[NE*Nk] when not NULL). sigR may be

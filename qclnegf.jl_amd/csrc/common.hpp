@@ -117,6 +117,12 @@ struct qclhip_grid {
     double* wk = nullptr;   // device [Nk]
     int* flux_order = nullptr;  // device [NE] energies sorted by (|e|, e)
     std::vector<double> eps_host, wE_host, wk_host;
+    // GF-solve precision and engines (qclhip_grid_set_precision / _set_engine).
+    int precision = QCLHIP_PRECISION_FP64;
+    int refine_steps = 2;
+    double refine_tol = 1e-10;
+    int keldysh_engine = QCLHIP_ENGINE_VECTOR;
+    unsigned* block_flags = nullptr;  // device, one per (e,m) block (mixed precision)
 };
 
 struct qclhip_shift {

@@ -1,0 +1,208 @@
+// Per-block (e,m) tile machinery shared by the Dyson/Keldysh kernels:
+// argument block, LDS shapes, XCD-aware tile order, coalesced tile transfers
+// and the 8-lane DPP argmax of the pivot search.
+#pragma once
+
+#include "common.hpp"
+
+namespace qclhip {
+
+template <int B>
+struct DKShape {
+    static constexpr int NP = 8 * B;      // padded matrix size
+    static constexpr int LD = NP + 1;     // LDS row stride (complex elements)
+    static constexpr size_t bytes =
+        (size_t)2 * NP * LD * sizeof(c64) + 2 * NP * sizeof(c64) + 2 * NP * sizeof(int);
+};
+
+constexpr int kMaxFamilies = 8;
+// Energies per workgroup of the FP64 Dyson kernel.
+constexpr int kDysonTileWaves = 4;
+
+struct DKArgs {
+    int NE, Nk, N;
+    const double* eps;
+    const c64* h;
+    const c64* famR[kMaxFamilies];  // summed in order: the total self-energy
+    const c64* famL[kMaxFamilies];
+    const c64* famG[kMaxFamilies];
+    int nfamR, nfamL, nfamG;
+    double eta;
+    c64* GR;
+    c64* GL;
+    c64* GG;
+    double* scale;
+    c64* traces;  // [2][NE*Nk] raw traces of G< and G> blocks
+    DeviceStatus* status;
+    int keldysh;
+    // Mixed precision (FP32 factorization + FP64 refinement): refinement
+    // steps, the bound on the predicted final residual norm, and one flag
+    // per block (1 = re-solve in FP64). When block_flags is set for the FP64
+    // Dyson kernel it only redoes tiles that hold a flagged block.
+    int refine;
+    double refine_tol;
+    unsigned* block_flags;
+};
+
+// out[r][c] (lane block) = Gs[rows] * Xs[:, cols]  or  Xs[rows] * Gs[cols]^dagger
+template <int B, bool ADJ>
+__device__ inline void block_product(const c64* __restrict__ L, const c64* __restrict__ R, int N,
+                                     int lr, int lc, c64 (&out)[B][B]) {
+    constexpr int LD = DKShape<B>::LD;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) out[r][c] = cmk(0.0, 0.0);
+#pragma unroll 2
+    for (int q = 0; q < N; ++q) {
+        c64 lv[B], rv[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) lv[r] = L[(lr * B + r) * LD + q];
+#pragma unroll
+        for (int c = 0; c < B; ++c) rv[c] = ADJ ? R[(lc * B + c) * LD + q] : R[q * LD + lc * B + c];
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c)
+                out[r][c] = ADJ ? cfmac(out[r][c], lv[r], rv[c]) : cfma(out[r][c], lv[r], rv[c]);
+    }
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2):
+// give every XCD a contiguous range of tiles so that the energies sharing a
+// 128-byte line are served by one L2. Bijective for any block count.
+__device__ inline long long xcd_tile(long long b, long long nb) {
+    const long long q = nb >> 3, rem = nb & 7, x = b & 7, s = b >> 3;
+    return x * q + (x < rem ? x : rem) + s;
+}
+
+// Element (i,j) of the summed families of one block (the reference's
+// _total_family_production order: fam[0] + fam[1] + ...).
+__device__ inline c64 fam_load(const c64* const* fam, int nfam, size_t idx) {
+    c64 v = fam[0][idx];
+    for (int f = 1; f < nfam; ++f) v = cadd(v, fam[f][idx]);
+    return v;
+}
+
+// Cooperative tile transfer between HBM and the per-wave LDS matrices of a
+// workgroup whose W waves own the W consecutive energies blk0 .. blk0+W-1 of
+// one momentum. Thread f handles element (i,j) of energy w with
+// f = w + W*(i + N*j): consecutive threads touch consecutive energies, so one
+// instruction moves contiguous 16*W-byte segments. Wave w's matrix sits at
+// lds + w*wstride, element (i,j) at i*LD + j.
+template <int W, int LD>
+__device__ inline void tile_load(c64* lds, int wstride, const c64* const* fam, int nfam, long long blk0,
+                                 int nvalid, int N, size_t S) {
+    const int total = N * N * W;
+    int f = threadIdx.x;
+    const int w = f % W;
+    int ij = f / W;
+    int i = ij % N, j = ij / N;
+    for (; f < total; f += 64 * W) {
+        if (w < nvalid) lds[w * wstride + i * LD + j] = fam_load(fam, nfam, blk0 + w + S * (i + (size_t)N * j));
+        i += 64;
+        while (i >= N) {
+            i -= N;
+            ++j;
+        }
+    }
+}
+
+template <int W, int LD>
+__device__ inline void tile_store(const c64* lds, int wstride, c64* dst, long long blk0, unsigned wave_mask, int N,
+                                  size_t S) {
+    const int total = N * N * W;
+    int f = threadIdx.x;
+    const int w = f % W;
+    int ij = f / W;
+    int i = ij % N, j = ij / N;
+    const bool on = (wave_mask >> w) & 1u;
+    for (; f < total; f += 64 * W) {
+        if (on) dst[blk0 + w + S * (i + (size_t)N * j)] = lds[w * wstride + i * LD + j];
+        i += 64;
+        while (i >= N) {
+            i -= N;
+            ++j;
+        }
+    }
+}
+
+// 8-lane argmax (largest |Re|+|Im|, then smallest row) over the lanes of one
+// column group, by DPP lane permutes: xor 1, xor 2 (quad_perm) and the
+// half-row mirror (lane i <-> 7-i of each 8 lanes).
+template <int CTRL>
+__device__ inline void dpp_argmax_step(double& mag, int& idx) {
+    long long bits = __double_as_longlong(mag);
+    int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
+    int oi = __builtin_amdgcn_update_dpp(0, idx, CTRL, 0xF, 0xF, false);
+    double om = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    if (om > mag || (om == mag && oi < idx)) {
+        mag = om;
+        idx = oi;
+    }
+}
+
+__device__ inline void argmax8(double& mag, int& idx) {
+    dpp_argmax_step<0xB1>(mag, idx);   // quad_perm [1,0,3,2]
+    dpp_argmax_step<0x4E>(mag, idx);   // quad_perm [2,3,0,1]
+    dpp_argmax_step<0x141>(mag, idx);  // row_half_mirror
+}
+
+// Converting variants of the cooperative tile transfer: the LDS tile holds
+// elements of type V (float2 or double2); HBM fields stay complex double.
+template <typename V>
+__device__ inline V to_lds(c64 z) {
+    V v;
+    v.x = z.x;
+    v.y = z.y;
+    return v;
+}
+
+template <int W, int LD, typename V>
+__device__ inline void tile_load_as(V* lds, int wstride, const c64* const* fam, int nfam, long long blk0, int nvalid,
+                                    int N, size_t S) {
+    const int total = N * N * W;
+    int f = threadIdx.x;
+    const int w = f % W;
+    int ij = f / W;
+    int i = ij % N, j = ij / N;
+    for (; f < total; f += 64 * W) {
+        if (w < nvalid) lds[w * wstride + i * LD + j] = to_lds<V>(fam_load(fam, nfam, blk0 + w + S * (i + (size_t)N * j)));
+        i += 64;
+        while (i >= N) {
+            i -= N;
+            ++j;
+        }
+    }
+}
+
+template <int W, int LD, typename V>
+__device__ inline void tile_store_from(const V* lds, int wstride, c64* dst, long long blk0, unsigned wave_mask, int N,
+                                       size_t S) {
+    const int total = N * N * W;
+    int f = threadIdx.x;
+    const int w = f % W;
+    int ij = f / W;
+    int i = ij % N, j = ij / N;
+    const bool on = (wave_mask >> w) & 1u;
+    for (; f < total; f += 64 * W) {
+        if (on) {
+            const V v = lds[w * wstride + i * LD + j];
+            dst[blk0 + w + S * (i + (size_t)N * j)] = cmk((double)v.x, (double)v.y);
+        }
+        i += 64;
+        while (i >= N) {
+            i -= N;
+            ++j;
+        }
+    }
+}
+
+// Launchers (dyson.hip, mixed.hip).
+int launch_dyson_fp64(qclhip_ctx* ctx, const DKArgs& a);
+int launch_keldysh_valu(qclhip_ctx* ctx, const DKArgs& a);
+int launch_keldysh_mfma(qclhip_ctx* ctx, const DKArgs& a, bool fp32);
+int launch_mixed_dyson(qclhip_ctx* ctx, const DKArgs& a);
+
+}  // namespace qclhip

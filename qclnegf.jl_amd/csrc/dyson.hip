@@ -1,6 +1,7 @@
-// Fused Dyson / Keldysh kernel and the number-functional reductions.
+// Dyson / Keldysh kernels (FP64, vector ALUs) and the number-functional
+// reductions.
 //
-// K_dyson_keldysh: one (e,m) block per wavefront. The 64 lanes form an 8x8
+// One (e,m) block per wavefront. The 64 lanes form an 8x8
 // grid (lr = lane&7 row group, lc = lane>>3 column group); lane (lr,lc) keeps
 // the B x B sub-block rows lr*B.., columns lc*B.. of the matrix in registers
 // (N <= 8B, padding rows/columns are zero and never pivot).
@@ -17,142 +18,9 @@
 // same 8x8 lane grid, each lane accumulating its B x B block.
 #include <climits>
 
-#include "common.hpp"
+#include "dk_common.hpp"
 
 namespace qclhip {
-
-template <int B>
-struct DKShape {
-    static constexpr int NP = 8 * B;      // padded matrix size
-    static constexpr int LD = NP + 1;     // LDS row stride (complex elements)
-    static constexpr size_t bytes =
-        (size_t)2 * NP * LD * sizeof(c64) + 2 * NP * sizeof(c64) + 2 * NP * sizeof(int);
-};
-
-constexpr int kMaxFamilies = 8;
-
-struct DKArgs {
-    int NE, Nk, N;
-    const double* eps;
-    const c64* h;
-    const c64* famR[kMaxFamilies];  // summed in order: the total self-energy
-    const c64* famL[kMaxFamilies];
-    const c64* famG[kMaxFamilies];
-    int nfamR, nfamL, nfamG;
-    double eta;
-    c64* GR;
-    c64* GL;
-    c64* GG;
-    double* scale;
-    c64* traces;  // [2][NE*Nk] raw traces of G< and G> blocks
-    DeviceStatus* status;
-    int keldysh;
-};
-
-// out[r][c] (lane block) = Gs[rows] * Xs[:, cols]  or  Xs[rows] * Gs[cols]^dagger
-template <int B, bool ADJ>
-__device__ inline void block_product(const c64* __restrict__ L, const c64* __restrict__ R, int N,
-                                     int lr, int lc, c64 (&out)[B][B]) {
-    constexpr int LD = DKShape<B>::LD;
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) out[r][c] = cmk(0.0, 0.0);
-#pragma unroll 2
-    for (int q = 0; q < N; ++q) {
-        c64 lv[B], rv[B];
-#pragma unroll
-        for (int r = 0; r < B; ++r) lv[r] = L[(lr * B + r) * LD + q];
-#pragma unroll
-        for (int c = 0; c < B; ++c) rv[c] = ADJ ? R[(lc * B + c) * LD + q] : R[q * LD + lc * B + c];
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int c = 0; c < B; ++c)
-                out[r][c] = ADJ ? cfmac(out[r][c], lv[r], rv[c]) : cfma(out[r][c], lv[r], rv[c]);
-    }
-}
-
-// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2):
-// give every XCD a contiguous range of tiles so that the energies sharing a
-// 128-byte line are served by one L2. Bijective for any block count.
-__device__ inline long long xcd_tile(long long b, long long nb) {
-    const long long q = nb >> 3, rem = nb & 7, x = b & 7, s = b >> 3;
-    return x * q + (x < rem ? x : rem) + s;
-}
-
-// Element (i,j) of the summed families of one block (the reference's
-// _total_family_production order: fam[0] + fam[1] + ...).
-__device__ inline c64 fam_load(const c64* const* fam, int nfam, size_t idx) {
-    c64 v = fam[0][idx];
-    for (int f = 1; f < nfam; ++f) v = cadd(v, fam[f][idx]);
-    return v;
-}
-
-// Cooperative tile transfer between HBM and the per-wave LDS matrices of a
-// workgroup whose W waves own the W consecutive energies blk0 .. blk0+W-1 of
-// one momentum. Thread f handles element (i,j) of energy w with
-// f = w + W*(i + N*j): consecutive threads touch consecutive energies, so one
-// instruction moves contiguous 16*W-byte segments. Wave w's matrix sits at
-// lds + w*wstride, element (i,j) at i*LD + j.
-template <int W, int LD>
-__device__ inline void tile_load(c64* lds, int wstride, const c64* const* fam, int nfam, long long blk0,
-                                 int nvalid, int N, size_t S) {
-    const int total = N * N * W;
-    int f = threadIdx.x;
-    const int w = f % W;
-    int ij = f / W;
-    int i = ij % N, j = ij / N;
-    for (; f < total; f += 64 * W) {
-        if (w < nvalid) lds[w * wstride + i * LD + j] = fam_load(fam, nfam, blk0 + w + S * (i + (size_t)N * j));
-        i += 64;
-        while (i >= N) {
-            i -= N;
-            ++j;
-        }
-    }
-}
-
-template <int W, int LD>
-__device__ inline void tile_store(const c64* lds, int wstride, c64* dst, long long blk0, unsigned wave_mask, int N,
-                                  size_t S) {
-    const int total = N * N * W;
-    int f = threadIdx.x;
-    const int w = f % W;
-    int ij = f / W;
-    int i = ij % N, j = ij / N;
-    const bool on = (wave_mask >> w) & 1u;
-    for (; f < total; f += 64 * W) {
-        if (on) dst[blk0 + w + S * (i + (size_t)N * j)] = lds[w * wstride + i * LD + j];
-        i += 64;
-        while (i >= N) {
-            i -= N;
-            ++j;
-        }
-    }
-}
-
-// 8-lane argmax (largest |Re|+|Im|, then smallest row) over the lanes of one
-// column group, by DPP lane permutes: xor 1, xor 2 (quad_perm) and the
-// half-row mirror (lane i <-> 7-i of each 8 lanes).
-template <int CTRL>
-__device__ inline void dpp_argmax_step(double& mag, int& idx) {
-    long long bits = __double_as_longlong(mag);
-    int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
-    int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
-    int oi = __builtin_amdgcn_update_dpp(0, idx, CTRL, 0xF, 0xF, false);
-    double om = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-    if (om > mag || (om == mag && oi < idx)) {
-        mag = om;
-        idx = oi;
-    }
-}
-
-__device__ inline void argmax8(double& mag, int& idx) {
-    dpp_argmax_step<0xB1>(mag, idx);   // quad_perm [1,0,3,2]
-    dpp_argmax_step<0x4E>(mag, idx);   // quad_perm [2,3,0,1]
-    dpp_argmax_step<0x141>(mag, idx);  // row_half_mirror
-}
 
 // Dyson inversion only (A1): one block per wavefront, 8x8 lane grid, B x B
 // register sub-blocks, Gauss-Jordan with partial pivoting. The workgroup
@@ -184,6 +52,11 @@ __global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p) {
     const int nvalid = min(W, p.NE - e0);
     const long long nblk = (long long)p.NE * p.Nk;
     const long long blk0 = e0 + (long long)p.NE * m;
+    if (p.block_flags) {  // FP64 re-solve of tiles holding a flagged block only
+        unsigned any = 0;
+        for (int w = 0; w < nvalid; ++w) any |= p.block_flags[blk0 + w];
+        if (!any) return;
+    }
     const int e = e0 + wave;
     const long long blk = blk0 + wave;
     const int wstride = NP * LD;
@@ -525,12 +398,14 @@ __global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
 // term order linear = e*Nk + m (production_operators.jl:1426-1440), then the
 // realness gates, lambda and the paired coefficients (production_state.jl:
 // 49-70, reference/greens.jl:155-185). One workgroup; fixed partition.
+// The realness limit is the reference's 1e-10 (numerical_invariants.jl:5)
+// for FP64 fields and realness_limit() of the grid otherwise.
 constexpr int kReduceThreads = 1024;
 
 __global__ __launch_bounds__(kReduceThreads) void k_number_pair(const c64* traces, int ncomp,
                                                                   int NE, int Nk, const double* wE,
                                                                   const double* wk, int gs,
-                                                                  double target, int pair,
+                                                                  double target, int pair, double limit,
                                                                   PairCoefficients* coef,
                                                                   DeviceStatus* st) {
     __shared__ double red[2][2][kReduceThreads];
@@ -566,7 +441,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_number_pair(const c64* trace
         double x = red[comp][0][0], y = red[comp][1][0];
         val[comp] = -y * k;  // Re[(x + iy)(0 + ik)]
         imv[comp] = x * k;
-        if (!(fabs(imv[comp]) <= 1.0e-10 * fmax(fabs(val[comp]), 1.0))) {
+        if (!(fabs(imv[comp]) <= limit * fmax(fabs(val[comp]), 1.0))) {
             report_status(st, QCLHIP_ERR_DOMAIN, 0);
         }
     }
@@ -737,20 +612,21 @@ constexpr int tile_waves(size_t per_wave) {
 }
 
 template <int B>
-int launch_dk(qclhip_ctx* ctx, const DKArgs& a, bool keldysh) {
-    {
-        constexpr int W = 4;
-        const size_t lds = DyShape<B, W>::bytes;
-        if (lds > ctx->max_lds_per_block)
-            return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
-        if (lds > 65536)
-            QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_dyson<B, W>,
-                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
-        hipLaunchKernelGGL((k_dyson<B, W>), dim3((unsigned)tiles), dim3(64 * W), lds, ctx->stream, a);
-        int rc = launch_check(ctx, "k_dyson");
-        if (rc || !keldysh) return rc;
-    }
+int launch_dyson(qclhip_ctx* ctx, const DKArgs& a) {
+    constexpr int W = kDysonTileWaves;
+    const size_t lds = DyShape<B, W>::bytes;
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_dyson<B, W>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
+    hipLaunchKernelGGL((k_dyson<B, W>), dim3((unsigned)tiles), dim3(64 * W), lds, ctx->stream, a);
+    return launch_check(ctx, "k_dyson");
+}
+
+template <int B>
+int launch_keldysh(qclhip_ctx* ctx, const DKArgs& a) {
     constexpr size_t per_wave = 2 * (size_t)DKShape<B>::NP * DKShape<B>::LD * sizeof(c64);
     constexpr int W = tile_waves<B>(per_wave);
     const size_t lds = per_wave * W;
@@ -790,15 +666,56 @@ int sandwich_dispatch(qclhip_ctx* ctx, const SWArgs& a) {
     return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
 }
 
-int dyson_keldysh_dispatch(qclhip_ctx* ctx, const DKArgs& a, bool keldysh) {
+int launch_dyson_fp64(qclhip_ctx* ctx, const DKArgs& a) {
     const int N = a.N;
-    if (N <= 8) return launch_dk<1>(ctx, a, keldysh);
-    if (N <= 16) return launch_dk<2>(ctx, a, keldysh);
-    if (N <= 24) return launch_dk<3>(ctx, a, keldysh);
-    if (N <= 32) return launch_dk<4>(ctx, a, keldysh);
-    if (N <= 40) return launch_dk<5>(ctx, a, keldysh);
-    if (N <= 48) return launch_dk<6>(ctx, a, keldysh);
+    if (N <= 8) return launch_dyson<1>(ctx, a);
+    if (N <= 16) return launch_dyson<2>(ctx, a);
+    if (N <= 24) return launch_dyson<3>(ctx, a);
+    if (N <= 32) return launch_dyson<4>(ctx, a);
+    if (N <= 40) return launch_dyson<5>(ctx, a);
+    if (N <= 48) return launch_dyson<6>(ctx, a);
     return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
+}
+
+int launch_keldysh_valu(qclhip_ctx* ctx, const DKArgs& a) {
+    const int N = a.N;
+    if (N <= 8) return launch_keldysh<1>(ctx, a);
+    if (N <= 16) return launch_keldysh<2>(ctx, a);
+    if (N <= 24) return launch_keldysh<3>(ctx, a);
+    if (N <= 32) return launch_keldysh<4>(ctx, a);
+    if (N <= 40) return launch_keldysh<5>(ctx, a);
+    if (N <= 48) return launch_keldysh<6>(ctx, a);
+    return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
+}
+
+// Dyson (+ Keldysh) at the grid's precision. FP64: k_dyson. FP32 +
+// refinement: k_dyson_mixed, then the FP64 re-solve of flagged tiles. The
+// Keldysh products run in FP64 on the engine the grid selects, or in FP32 on
+// the matrix cores (QCLHIP_PRECISION_FP32_PRODUCTS).
+// Realness gate of the number functional: the reference's limit for FP64;
+// FP32 Keldysh products leave an imaginary part of order 1e-7 relative.
+double realness_limit(const qclhip_grid* grid) {
+    return grid->precision == QCLHIP_PRECISION_FP32_PRODUCTS ? 1.0e-5 : 1.0e-10;
+}
+
+int dyson_keldysh_dispatch(qclhip_ctx* ctx, const qclhip_grid* grid, DKArgs a, bool keldysh) {
+    if (a.N > 48) return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
+    int rc;
+    if (grid->precision != QCLHIP_PRECISION_FP64) {
+        a.block_flags = grid->block_flags;  // one flag per block, owned by the grid
+        if (!a.block_flags) return fail(ctx, QCLHIP_ERR_RUNTIME, "grid has no refinement flags");
+        a.refine = grid->refine_steps;
+        a.refine_tol = grid->refine_tol;
+        rc = launch_mixed_dyson(ctx, a);
+        if (rc || !keldysh) return rc;
+        if (grid->precision == QCLHIP_PRECISION_FP32_PRODUCTS) return launch_keldysh_mfma(ctx, a, true);
+    } else {
+        a.block_flags = nullptr;
+        rc = launch_dyson_fp64(ctx, a);
+        if (rc || !keldysh) return rc;
+    }
+    if (grid->keldysh_engine == QCLHIP_ENGINE_MATRIX) return launch_keldysh_mfma(ctx, a, false);
+    return launch_keldysh_valu(ctx, a);
 }
 
 }  // namespace qclhip
@@ -840,7 +757,7 @@ int qclhip_dyson_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
     a.traces = (c64*)traces;
     a.status = ctx->status;
     a.keldysh = keldysh ? 1 : 0;
-    return dyson_keldysh_dispatch(ctx, a, keldysh);
+    return dyson_keldysh_dispatch(ctx, grid, a, keldysh);
 }
 
 int qclhip_retarded_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* sigR,
@@ -909,10 +826,10 @@ int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
     a.traces = traces;
     a.status = ctx->status;
     a.keldysh = 1;
-    int rc = dyson_keldysh_dispatch(ctx, a, true);
+    int rc = dyson_keldysh_dispatch(ctx, grid, a, true);
     if (rc) return rc;
     hipLaunchKernelGGL(k_number_pair, dim3(1), dim3(kReduceThreads), 0, ctx->stream, traces, 2, grid->NE,
-                       grid->Nk, grid->wE, grid->wk, grid->gs, target, 1, ctx->coef, ctx->status);
+                       grid->Nk, grid->wE, grid->wk, grid->gs, target, 1, realness_limit(grid), ctx->coef, ctx->status);
     hipLaunchKernelGGL(k_normalize_pair, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, n,
                        (c64*)GL, (c64*)GG, (const PairCoefficients*)ctx->coef, 0.0, 0.0);
     if (A)
@@ -943,7 +860,8 @@ int qclhip_number_functional(qclhip_ctx* ctx, const qclhip_grid* grid, const dou
     hipLaunchKernelGGL(k_block_traces, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, ctx->stream,
                        (const c64*)G, nblk, grid->Nb, traces);
     hipLaunchKernelGGL(k_number_pair, dim3(1), dim3(kReduceThreads), 0, ctx->stream, traces, 1, grid->NE,
-                       grid->Nk, grid->wE, grid->wk, grid->gs, 1.0, 0, ctx->coef, ctx->status);
+                       grid->Nk, grid->wE, grid->wk, grid->gs, 1.0, 0, realness_limit(grid), ctx->coef,
+                       ctx->status);
     int rc = launch_check(ctx, "number_functional");
     if (rc) return rc;
     QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, ctx->coef, sizeof(PairCoefficients),

@@ -189,7 +189,32 @@ int qclhip_grid_destroy(qclhip_grid* g) {
     if (g->wE) hipFree(g->wE);
     if (g->wk) hipFree(g->wk);
     if (g->flux_order) hipFree(g->flux_order);
+    if (g->block_flags) hipFree(g->block_flags);
     delete g;
+    return QCLHIP_OK;
+}
+
+int qclhip_grid_set_precision(qclhip_ctx* ctx, qclhip_grid* g, int precision, int refine_steps, double refine_tol) {
+    if (!ctx || !g) return QCLHIP_ERR_ARGUMENT;
+    if (precision != QCLHIP_PRECISION_FP64 && precision != QCLHIP_PRECISION_FP32_REFINE &&
+        precision != QCLHIP_PRECISION_FP32_PRODUCTS)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "precision must be QCLHIP_PRECISION_FP64, _FP32_REFINE or _FP32_PRODUCTS");
+    if (refine_steps < 0 || refine_steps > 4) return fail(ctx, QCLHIP_ERR_ARGUMENT, "refine_steps must be in [0, 4]");
+    if (!(refine_tol <= 0.0 || std::isfinite(refine_tol)))
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "refine_tol must be finite");
+    if (precision != QCLHIP_PRECISION_FP64 && !g->block_flags)
+        QCLHIP_TRY_HIP(ctx, hipMalloc(&g->block_flags, (size_t)g->NE * g->Nk * sizeof(unsigned)));
+    g->precision = precision;
+    g->refine_steps = refine_steps;
+    g->refine_tol = refine_tol > 0.0 ? refine_tol : 1e-10;
+    return QCLHIP_OK;
+}
+
+int qclhip_grid_set_engine(qclhip_ctx* ctx, qclhip_grid* g, int keldysh_engine) {
+    if (!ctx || !g) return QCLHIP_ERR_ARGUMENT;
+    if (keldysh_engine != QCLHIP_ENGINE_VECTOR && keldysh_engine != QCLHIP_ENGINE_MATRIX)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "engine must be QCLHIP_ENGINE_VECTOR or _MATRIX");
+    g->keldysh_engine = keldysh_engine;
     return QCLHIP_OK;
 }
 

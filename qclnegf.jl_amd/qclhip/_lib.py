@@ -62,6 +62,9 @@ _SIGNATURES = {
     "qclhip_max_basis": [],
     "qclhip_grid_create": [_vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, ct.POINTER(_vp)],
     "qclhip_grid_destroy": [_vp],
+    "qclhip_grid_set_precision": [_vp, _vp, ct.c_int, ct.c_int, ct.c_double],
+    "qclhip_grid_set_engine": [_vp, _vp, ct.c_int],
+    "qclhip_grid_refine_fallbacks": [_vp, _vp, ct.POINTER(ct.c_int64)],
     "qclhip_retarded_green": [_vp, _vp, _vp, _vp, ct.c_double, _vp, _vp],
     "qclhip_dyson_keldysh": [_vp, _vp, _vp, _vp, _vp, _vp, ct.c_double, _vp, _vp, _vp, _vp, _vp],
     "qclhip_current_green": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),

@@ -144,6 +144,42 @@ class Grid:
                       self.eps.ctypes.data_as(_dp), self.wE.ctypes.data_as(_dp),
                       self.wk.ctypes.data_as(_dp), ct.byref(h))
         self.handle = h
+        self.precision, self.engine = "fp64", "vector"
+
+    PRECISION_FP64 = 0
+    PRECISION_FP32_REFINE = 1
+    PRECISION_FP32_PRODUCTS = 2
+    ENGINE_VECTOR = 0
+    ENGINE_MATRIX = 1
+
+    def set_precision(self, precision: str = "fp64", refine_steps: int = 2, refine_tol: float = 0.0):
+        """GF-solve precision of this grid: "fp64" (the reference's),
+        "fp32_refine" (FP32 Dyson factorization + refine_steps FP64 Newton
+        steps, FP64 re-solve of blocks whose predicted residual exceeds
+        refine_tol; FP64 Keldysh products) or "fp32_products" (the same Dyson
+        step with FP32 matrix-core Keldysh products)."""
+        code = {"fp64": self.PRECISION_FP64, "fp32_refine": self.PRECISION_FP32_REFINE,
+                "fp32_products": self.PRECISION_FP32_PRODUCTS}.get(precision)
+        if code is None:
+            raise _lib.ArgumentError(f"unknown precision {precision!r}")
+        self.ctx.call("qclhip_grid_set_precision", self.handle, code, int(refine_steps), ct.c_double(refine_tol))
+        self.precision = precision
+        return self
+
+    def set_engine(self, engine: str = "vector"):
+        """Engine of the FP64 Keldysh products: "vector" or "matrix" cores."""
+        code = {"vector": self.ENGINE_VECTOR, "matrix": self.ENGINE_MATRIX}.get(engine)
+        if code is None:
+            raise _lib.ArgumentError(f"unknown engine {engine!r}")
+        self.ctx.call("qclhip_grid_set_engine", self.handle, code)
+        self.engine = engine
+        return self
+
+    def refine_fallbacks(self) -> int:
+        """Blocks of the last mixed-precision Dyson call re-solved in FP64."""
+        n = ct.c_int64()
+        self.ctx.call("qclhip_grid_refine_fallbacks", self.handle, ct.byref(n))
+        return n.value
 
     @property
     def nfield(self):

@@ -92,7 +92,8 @@ def gf_workload(N=24, NE=2048, Nk=64, seed=SEED, T_K=77.0, mu_eV=0.1):
 
 
 def diag_fields_host(w):
-    """Dense host fields (NE,Nk,N,N) Fortran order of the workload: (SR, SL, SG)."""
+    """Dense host fields (NE,Nk,N,N) Fortran order of the workload (or of an
+    injection_family dict): (SR, SL, SG)."""
     NE, Nk, N = w["NE"], w["Nk"], w["N"]
     SR = np.zeros((NE, Nk, N, N), np.complex128, order="F")
     SL = np.zeros_like(SR)
@@ -106,8 +107,22 @@ def diag_fields_host(w):
     return SR, SL, SG
 
 
+def injection_family(w, dmu_eV=0.05, gamma_meV=(0.5, 2.0), seed=SEED + 1):
+    """Diagonal family of a reservoir at mu + dmu (the `plus` embedding family of
+    the current functional): dict(gamma (NE,Nk,N), f (NE)). Together with the
+    workload's own family it drives a non-zero current."""
+    rng = np.random.default_rng(seed)
+    NE, Nk, N = w["NE"], w["Nk"], w["N"]
+    gamma = 2.0 * rng.uniform(gamma_meV[0] * 1e-3, gamma_meV[1] * 1e-3, size=(NE, Nk, N)) / E0_EV
+    x = (w["eps"] - w["mu"] - dmu_eV / E0_EV) / w["kbt"]
+    with np.errstate(over="ignore"):
+        f = np.where(x > 40, 0.0, np.where(x < -40, 1.0, 1.0 / (np.exp(np.clip(x, -700, 700)) + 1.0)))
+    return dict(N=N, NE=NE, Nk=Nk, gamma=gamma, f=f)
+
+
 def diag_fields_device(w, device="cuda"):
-    """The same fields built directly in device memory (torch shape (N,N,Nk,NE))."""
+    """The same fields built directly in device memory (torch shape (N,N,Nk,NE)).
+    `w` may also be an injection_family dict."""
     import torch
     NE, Nk, N = w["NE"], w["Nk"], w["N"]
     g = torch.from_numpy(np.ascontiguousarray(np.transpose(w["gamma"], (2, 1, 0)))).to(device)  # (N,Nk,NE)
