@@ -59,7 +59,7 @@ int qclhip_ctx_destroy(qclhip_ctx* ctx);
 /* Waits for the stream, returns (and clears) the latched device status. */
 int qclhip_ctx_synchronize(qclhip_ctx* ctx);
 const char* qclhip_ctx_last_error(const qclhip_ctx* ctx);
-/* Build-time maximum basis size of the fused Dyson/Keldysh kernel. */
+/* Build-time maximum basis size of the Dyson/Keldysh kernels. */
 int qclhip_max_basis(void);
 
 /* ---- grids (ModelGrids: reference/grids.jl:10-47) -----------------------
@@ -106,7 +106,7 @@ int qclhip_grid_refine_fallbacks(qclhip_ctx* ctx, const qclhip_grid* grid, int64
 int qclhip_retarded_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h,
                           const double* sigR, double eta, double* GR, double* scale);
 
-/* ---- A1+A2 fused: Dyson plus the raw Keldysh products
+/* ---- A1+A2: Dyson plus the raw Keldysh products
  * G<raw = G^R S< G^R^dagger, G>raw = G^R S> G^R^dagger (production_state.jl:
  * 14-48), and the weighted block traces for the number functionals.
  * traces may be NULL (context scratch); else 2*NE*Nk complex. */
@@ -231,6 +231,52 @@ int qclhip_seed_weights(qclhip_ctx* ctx, const qclhip_grid* grid, const double* 
 /* G< = -f(eps; mu, kBT) (G^R - G^R^dagger). */
 int qclhip_seed_lesser(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR, double mu,
                        double kBT, double* GL);
+
+/* ---- A19: adjoint (vector-Jacobian products) ----------------------------
+ * No counterpart in the reference, which lists the adjoint as missing
+ * (algorithm_registry.jl:226-249). For a real functional J of a complex
+ * field Z the gradient is Zbar = dJ/dRe Z + i dJ/dIm Z (dJ = Re sum
+ * conj(Zbar) dZ). Entries marked "accumulates" add into their outputs. */
+/* Reverse of qclhip_current_green: given the forward's self-energy families
+ * (famL, famG: HOST arrays of nfam device pointers; Sigma^R is not needed),
+ * target, its out_host coefficients coef_host = (lambda, number, holes, a, c),
+ * its outputs GR, GL, GG and the gradients GRbar (may be NULL), GLbar,
+ * GGbar, writes the gradients SRbar, SLbar, SGbar of the TOTAL self-energy
+ * (the gradient of every family) and, when hbar != NULL, the gradient of h
+ * (Nk,Nb,Nb). Raw Keldysh fields are recovered from the paired map
+ * (DomainError when a = c). */
+int qclhip_current_green_adjoint(qclhip_ctx* ctx, const qclhip_grid* grid, int nfam,
+                                 const double* const* famL, const double* const* famG,
+                                 double target, const double* coef_host, const double* GR,
+                                 const double* GL, const double* GG, const double* GRbar,
+                                 const double* GLbar, const double* GGbar, double* SRbar,
+                                 double* SLbar, double* SGbar, double* hbar);
+/* Reverse of qclhip_boundary_flux for the scalar gradient Jbar; accumulates
+ * into any non-NULL of plusLbar, plusGbar, GLbar, GGbar. */
+int qclhip_boundary_flux_adjoint(qclhip_ctx* ctx, const qclhip_grid* grid, double Jbar,
+                                 const double* plusL, const double* plusG, const double* GL,
+                                 const double* GG, double* plusLbar, double* plusGbar,
+                                 double* GLbar, double* GGbar);
+/* X (+)= alpha S^T Y: transpose of apply_energy_shift! (production_cache.jl:
+ * 300-325); accumulates when accumulate != 0. */
+int qclhip_shift_transpose(qclhip_ctx* ctx, const qclhip_grid* grid, const qclhip_shift* shift,
+                           const double* Y, double alpha, int accumulate, double* X);
+/* Conjugate transpose (K diag(wk))^dagger of a kernel representation, in the
+ * same representation: qclhip_contract with it maps Sbar to Gbar. */
+int qclhip_kernel_adjoint(qclhip_ctx* ctx, const qclhip_kernel* kernel, qclhip_kernel** out);
+/* Reverse of qclhip_retarded_self_energy: Gamma_bar = -wE * H_raw[SRbar]
+ * + (i/2) SRbar (H_raw the transform without the wE weights; its kernel is
+ * antisymmetric); accumulates SLbar += i Gamma_bar, SGbar -= i Gamma_bar. */
+int qclhip_retarded_self_energy_adjoint(qclhip_ctx* ctx, const qclhip_hilbert* plan,
+                                        const double* SRbar, double* SLbar, double* SGbar);
+/* X_em <- (X_em - X_em^dagger)/2 in place: the projection onto the
+ * anti-Hermitian blocks on which S< and S> live (the tangent space of the
+ * SCBA state used by the fixed-point adjoint). */
+int qclhip_antihermitian_part(qclhip_ctx* ctx, const qclhip_grid* grid, double* X);
+/* Reverse of qclhip_embedding_component; accumulates
+ * Xbar += shift^T (T^dagger Sbar T). */
+int qclhip_embedding_adjoint(qclhip_ctx* ctx, const qclhip_grid* grid, const double* T,
+                             const qclhip_shift* shift, const double* Sbar, double* Xbar);
 
 #ifdef __cplusplus
 }

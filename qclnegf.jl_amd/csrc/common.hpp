@@ -132,6 +132,10 @@ struct qclhip_shift {
     int* right = nullptr;
     double* wr = nullptr;
     unsigned char* valid = nullptr;
+    // Transpose of the plan (CSR over source nodes): tptr[NE+1], tsrc, tw.
+    int* tptr = nullptr;
+    int* tsrc = nullptr;
+    double* tw = nullptr;
 };
 
 namespace qclhip {

@@ -20,6 +20,10 @@
 
 struct qclhip_kernel {
     int kind = 0, rank = 0, NE = 0, Nk = 0, Nb = 0;
+    // Momentum-independent form: 0 = the quadrature weights enter the
+    // momentum sum (the forward map), 1 = they scale the broadcast output
+    // (its adjoint).
+    int weights_out = 0;
     const qclhip_grid* grid = nullptr;
     qclhip::c64* a = nullptr;
     qclhip::c64* b = nullptr;
@@ -43,21 +47,23 @@ __global__ void k_mi_reduce(int NE, int Nk, int N2, const double* __restrict__ w
         c64 v = cmk(0.0, 0.0);
         for (int mp = 0; mp < Nk; ++mp) {
             c64 g = G[e + (long long)NE * (mp + (long long)Nk * q)];
-            v = cmk(v.x + wk[mp] * g.x, v.y + wk[mp] * g.y);
+            const double w = wk ? wk[mp] : 1.0;
+            v = cmk(v.x + w * g.x, v.y + w * g.y);
         }
         X[idx] = v;
     }
 }
 
 __global__ void k_mi_broadcast(int NE, int Nk, int N2, const c64* __restrict__ Y, double f, double beta,
-                               c64* __restrict__ S) {
+                               const double* __restrict__ wout, c64* __restrict__ S) {
     const long long n = (long long)NE * Nk * N2;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
          idx += (long long)gridDim.x * blockDim.x) {
         const int e = (int)(idx % NE);
         const long long q = idx / ((long long)NE * Nk);
         c64 y = Y[e + (long long)NE * q];
-        c64 v = cmk(f * y.x, f * y.y);
+        const double fw = wout ? f * wout[(idx / NE) % Nk] : f;
+        c64 v = cmk(fw * y.x, fw * y.y);
         if (beta != 0.0) {
             c64 s = S[idx];
             v = cmk(beta * s.x + v.x, beta * s.y + v.y);
@@ -256,13 +262,14 @@ int contract_core(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double 
             if (!X) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
             c64* Y = X + NE * N2;
             hipLaunchKernelGGL(k_mi_reduce, dim3(grid_blocks(NE * N2, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
-                               (int)NE, (int)Nk, (int)N2, g->wk, G, X);
+                               (int)NE, (int)Nk, (int)N2, K->weights_out ? (const double*)nullptr : g->wk, G, X);
             int st = launch_check(ctx, "k_mi_reduce");
             if (st) return st;
             st = gemm_nt(ctx, NE, N2, N2, 1.0, X, NE, K->a, N2, 0.0, Y, NE);
             if (st) return st;
             hipLaunchKernelGGL(k_mi_broadcast, dim3(grid_blocks(NE * Nk * N2, 256, ctx->sm_count)), dim3(256), 0,
-                               ctx->stream, (int)NE, (int)Nk, (int)N2, Y, f, beta, S);
+                               ctx->stream, (int)NE, (int)Nk, (int)N2, Y, f, beta,
+                               K->weights_out ? (const double*)g->wk : nullptr, S);
             return launch_check(ctx, "k_mi_broadcast");
         }
         case QCLHIP_KERNEL_SEPARABLE_IFR: {
@@ -351,6 +358,85 @@ int qclhip_kernel_create(qclhip_ctx* ctx, const qclhip_grid* grid, int kind, int
         return fail(ctx, QCLHIP_ERR_RUNTIME, "kernel allocation failed");
     }
     *out = K;
+    return QCLHIP_OK;
+}
+
+// Conjugate transpose of the weighted map S = K diag(w) G: G_bar =
+// (K diag(w))^dagger S_bar, in the same representation (the reverse of the
+// static contraction, production_operators.jl:402-658). Built once from the
+// device copy of the forward kernel.
+int qclhip_kernel_adjoint(qclhip_ctx* ctx, const qclhip_kernel* K, qclhip_kernel** out) {
+    if (!ctx || !K || !out) return QCLHIP_ERR_ARGUMENT;
+    *out = nullptr;
+    typedef std::complex<double> cd;
+    const long long Nk = K->Nk, N2 = (long long)K->Nb * K->Nb, Nc = Nk * N2, r = K->rank;
+    auto fetch = [&](const c64* d, size_t n, std::vector<cd>& h) -> bool {
+        h.resize(n);
+        return hipMemcpy(h.data(), d, n * sizeof(c64), hipMemcpyDeviceToHost) == hipSuccess;
+    };
+    QCLHIP_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<cd> a, b, c, da, db, dc;
+    bool ok = true;
+    switch (K->kind) {
+        case QCLHIP_KERNEL_DENSE:
+            ok = fetch(K->a, (size_t)Nc * Nc, a);
+            da.resize(a.size());
+            for (long long qp = 0; qp < Nc; ++qp)
+                for (long long q = 0; q < Nc; ++q) da[qp + Nc * q] = std::conj(a[q + Nc * qp]);
+            break;
+        case QCLHIP_KERNEL_MOMENTUM_INDEPENDENT:
+            ok = fetch(K->a, (size_t)N2 * N2, a);
+            da.resize(a.size());
+            for (long long qp = 0; qp < N2; ++qp)
+                for (long long q = 0; q < N2; ++q) da[qp + N2 * q] = std::conj(a[q + N2 * qp]);
+            break;
+        case QCLHIP_KERNEL_SEPARABLE_IFR:
+            ok = fetch(K->a, (size_t)N2 * r, a) && fetch(K->b, (size_t)r * N2, b) && fetch(K->c, (size_t)Nk * Nk, c);
+            da.resize(a.size());
+            db.resize(b.size());
+            dc.resize(c.size());
+            for (long long j = 0; j < r; ++j)
+                for (long long q = 0; q < N2; ++q) {
+                    da[q + N2 * j] = std::conj(b[j + r * q]);  // left' = right^dagger
+                    db[j + r * q] = std::conj(a[q + N2 * j]);  // right' = left^dagger
+                }
+            for (long long mp = 0; mp < Nk; ++mp)
+                for (long long m = 0; m < Nk; ++m) dc[mp + Nk * m] = std::conj(c[m + Nk * mp]);
+            break;
+        case QCLHIP_KERNEL_LOW_RANK:
+            ok = fetch(K->a, (size_t)Nc * r, a) && fetch(K->b, (size_t)r * Nc, b);
+            da.resize(a.size());
+            db.resize(b.size());
+            for (long long i = 0; i < r; ++i)
+                for (long long q = 0; q < Nc; ++q) {
+                    da[q + Nc * i] = std::conj(b[i + r * q]);
+                    db[i + r * q] = std::conj(a[q + Nc * i]);
+                }
+            break;
+        default:
+            return fail(ctx, QCLHIP_ERR_ARGUMENT, "unknown kernel representation");
+    }
+    if (!ok) return fail(ctx, QCLHIP_ERR_RUNTIME, "kernel download failed");
+    auto* A = new qclhip_kernel();
+    A->kind = K->kind;
+    A->rank = K->rank;
+    A->grid = K->grid;
+    A->Nk = K->Nk;
+    A->Nb = K->Nb;
+    A->weights_out = K->kind == QCLHIP_KERNEL_MOMENTUM_INDEPENDENT ? 1 - K->weights_out : 0;
+    auto upload = [&](std::vector<cd>& h, c64** d) {
+        if (h.empty() || !ok) return;
+        ok = hipMalloc(d, h.size() * sizeof(c64)) == hipSuccess &&
+             hipMemcpy(*d, h.data(), h.size() * sizeof(c64), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    upload(da, &A->a);
+    upload(db, &A->b);
+    upload(dc, &A->c);
+    if (!ok) {
+        qclhip_kernel_destroy(A);
+        return fail(ctx, QCLHIP_ERR_RUNTIME, "kernel allocation failed");
+    }
+    *out = A;
     return QCLHIP_OK;
 }
 

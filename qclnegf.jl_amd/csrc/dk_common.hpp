@@ -108,7 +108,7 @@ __device__ inline void tile_load(c64* lds, int wstride, const c64* const* fam, i
     }
 }
 
-template <int W, int LD>
+template <int W, int LD, bool ACC = false>
 __device__ inline void tile_store(const c64* lds, int wstride, c64* dst, long long blk0, unsigned wave_mask, int N,
                                   size_t S) {
     const int total = N * N * W;
@@ -118,7 +118,11 @@ __device__ inline void tile_store(const c64* lds, int wstride, c64* dst, long lo
     int i = ij % N, j = ij / N;
     const bool on = (wave_mask >> w) & 1u;
     for (; f < total; f += 64 * W) {
-        if (on) dst[blk0 + w + S * (i + (size_t)N * j)] = lds[w * wstride + i * LD + j];
+        if (on) {
+            const size_t idx = blk0 + w + S * (i + (size_t)N * j);
+            const c64 v = lds[w * wstride + i * LD + j];
+            dst[idx] = ACC ? cadd(dst[idx], v) : v;
+        }
         i += 64;
         while (i >= N) {
             i -= N;
@@ -204,5 +208,8 @@ int launch_dyson_fp64(qclhip_ctx* ctx, const DKArgs& a);
 int launch_keldysh_valu(qclhip_ctx* ctx, const DKArgs& a);
 int launch_keldysh_mfma(qclhip_ctx* ctx, const DKArgs& a, bool fp32);
 int launch_mixed_dyson(qclhip_ctx* ctx, const DKArgs& a);
+// out[e, col] (+)= alpha * (S^T Y)[e, col] over ncols columns (adjoint.hip).
+int shift_transpose(qclhip_ctx* ctx, const qclhip_shift* s, long long ncols, const c64* Y, double alpha,
+                    int accumulate, c64* out);
 
 }  // namespace qclhip

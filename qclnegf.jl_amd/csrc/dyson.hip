@@ -311,6 +311,8 @@ struct SWArgs {
     const double* wr;
     const unsigned char* valid;
     c64* out;
+    int tadj;        // 1: use T^dagger in place of T (T^dagger X T)
+    int accumulate;  // 1: out += result
 };
 
 template <int B, int W>
@@ -334,7 +336,12 @@ __global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
     const int N = p.N;
     const size_t S = (size_t)nblk;
     const int lr = lane & 7, lc = lane >> 3;
-    for (int f = threadIdx.x; f < N * N; f += 64 * W) Ts[(f % N) * LD + f / N] = p.T[f];
+    for (int f = threadIdx.x; f < N * N; f += 64 * W) {
+        if (p.tadj)
+            Ts[(f / N) * LD + f % N] = cconj(p.T[f]);
+        else
+            Ts[(f % N) * LD + f / N] = p.T[f];
+    }
     // Shifted input X(e + delta) for the W energies: two coalesced gathers.
     {
         const int total = N * N * W;
@@ -391,7 +398,10 @@ __global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
     }
     __syncthreads();
     const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
-    tile_store<W, LD>(X0, wstride, p.out, e0 + mbase, mask, N, S);
+    if (p.accumulate)
+        tile_store<W, LD, true>(X0, wstride, p.out, e0 + mbase, mask, N, S);
+    else
+        tile_store<W, LD>(X0, wstride, p.out, e0 + mbase, mask, N, S);
 }
 
 // Deterministic reduction of the weighted block traces in the reference's
@@ -974,7 +984,7 @@ int qclhip_embedding_component(qclhip_ctx* ctx, const qclhip_grid* grid, const d
     if (!ctx || !grid || !T || !X || !shift || !S) return QCLHIP_ERR_ARGUMENT;
     if (shift->NE != grid->NE) return fail(ctx, QCLHIP_ERR_DIMENSION, "shift plan has the wrong energy size");
     if (X == S) return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
-    SWArgs a;
+    SWArgs a{};
     a.NE = grid->NE;
     a.Nk = grid->Nk;
     a.N = grid->Nb;
@@ -985,6 +995,31 @@ int qclhip_embedding_component(qclhip_ctx* ctx, const qclhip_grid* grid, const d
     a.wr = shift->wr;
     a.valid = shift->valid;
     a.out = (c64*)S;
+    return sandwich_dispatch(ctx, a);
+}
+
+// Reverse of the embedding component: Xbar += S^T (T^dagger Sbar T); the
+// shift acts on energy and T on the block indices, so the transpose of the
+// shift is applied first, then the sandwich with T^dagger.
+int qclhip_embedding_adjoint(qclhip_ctx* ctx, const qclhip_grid* grid, const double* T, const qclhip_shift* shift,
+                             const double* Sbar, double* Xbar) {
+    if (!ctx || !grid || !T || !shift || !Sbar || !Xbar) return QCLHIP_ERR_ARGUMENT;
+    if (shift->NE != grid->NE) return fail(ctx, QCLHIP_ERR_DIMENSION, "shift plan has the wrong energy size");
+    if (Sbar == Xbar) return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
+    const long long ncols = (long long)grid->Nk * grid->Nb * grid->Nb;
+    c64* Z = (c64*)workspace(ctx, (size_t)grid->NE * ncols * sizeof(c64));
+    if (!Z) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    int rc = shift_transpose(ctx, shift, ncols, (const c64*)Sbar, 1.0, 0, Z);
+    if (rc) return rc;
+    SWArgs a{};
+    a.NE = grid->NE;
+    a.Nk = grid->Nk;
+    a.N = grid->Nb;
+    a.T = (const c64*)T;
+    a.X = Z;
+    a.out = (c64*)Xbar;
+    a.tadj = 1;
+    a.accumulate = 1;
     return sandwich_dispatch(ctx, a);
 }
 

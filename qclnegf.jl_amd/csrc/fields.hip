@@ -162,11 +162,41 @@ int lo_combine(qclhip_ctx* ctx, int NE, long long n, const c64* Y, double ca, co
     return launch_check(ctx, "k_lo_combine");
 }
 
+// X_em <- (X_em - X_em^dagger) / 2 in place; one thread per pair a <= b.
+__global__ void k_antihermitian_part(long long nblk, int N, c64* X) {
+    const long long npair = (long long)N * (N + 1) / 2;
+    const long long n = nblk * npair;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long blk = idx % nblk;
+        long long pr = idx / nblk;
+        int b = 0;
+        while (pr >= b + 1) {
+            pr -= b + 1;
+            ++b;
+        }
+        const int a = (int)pr;  // a <= b
+        const size_t ab = blk + nblk * (a + (size_t)N * b), ba = blk + nblk * (b + (size_t)N * a);
+        const c64 x = X[ab], y = X[ba];
+        X[ab] = cmk(0.5 * (x.x - y.x), 0.5 * (x.y + y.y));
+        if (a != b) X[ba] = cmk(0.5 * (y.x - x.x), 0.5 * (y.y + x.y));
+    }
+}
+
 }  // namespace qclhip
 
 using namespace qclhip;
 
 extern "C" {
+
+int qclhip_antihermitian_part(qclhip_ctx* ctx, const qclhip_grid* grid, double* X) {
+    if (!ctx || !grid || !X) return QCLHIP_ERR_ARGUMENT;
+    const long long nblk = (long long)grid->NE * grid->Nk;
+    const long long n = nblk * grid->Nb * (grid->Nb + 1) / 2;
+    hipLaunchKernelGGL(k_antihermitian_part, dim3(grid_blocks(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
+                       nblk, grid->Nb, (c64*)X);
+    return launch_check(ctx, "k_antihermitian_part");
+}
 
 int qclhip_apply_shift(qclhip_ctx* ctx, const qclhip_shift* shift, int64_t ncols, const double* X, double* Y) {
     if (!ctx || !shift || !X || !Y || ncols < 0) return QCLHIP_ERR_ARGUMENT;

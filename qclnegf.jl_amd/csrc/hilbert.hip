@@ -199,7 +199,8 @@ struct HilbertArgs {
     const c64* tw;
     int twstride;
     const c64* kf;
-    int mode;           // 0: Lambda, 1: S^R, 2: witness only
+    int mode;           // 0: Lambda, 1: S^R, 2: witness only, 3: adjoint of S^R
+    c64* out2;          // adjoint mode: S>_bar (accumulated); out holds S<_bar
 };
 
 __device__ inline c64 load_gamma(const HilbertArgs& a, size_t idx) {
@@ -228,8 +229,9 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
         double xr = 0.0, xi = 0.0;
         if (e < NE) {
             const c64 gm = load_gamma(a, base + e);
-            xr = a.wE[e] * gm.x;
-            xi = a.wE[e] * gm.y;
+            const double we = a.mode == 3 ? 1.0 : a.wE[e];  // the adjoint weights the output
+            xr = we * gm.x;
+            xi = we * gm.y;
         }
         re[f] = xr;
         im[f] = xi;
@@ -260,6 +262,16 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
             const double dx = lam.x - ref.x, dy = lam.y - ref.y;
             dn += dx * dx + dy * dy;
             nn += ref.x * ref.x + ref.y * ref.y;
+            continue;
+        }
+        if (a.mode == 3) {
+            // Gamma_bar = -wE * M SR_bar + (i/2) SR_bar (M antisymmetric);
+            // S<_bar += i Gamma_bar, S>_bar -= i Gamma_bar.
+            const c64 rb = a.gamma[gi];
+            const double we = a.wE[e];
+            const c64 ig = cmk(we * lam.y - 0.5 * rb.x, -we * lam.x - 0.5 * rb.y);
+            a.out[gi] = cadd(a.out[gi], ig);
+            a.out2[gi] = csub(a.out2[gi], ig);
             continue;
         }
         if (a.lambda_copy) a.lambda_copy[gi] = lam;
@@ -498,6 +510,22 @@ int qclhip_fft_hilbert(qclhip_ctx* ctx, const qclhip_hilbert* plan, const double
     if (!ctx || !plan || !Gamma || !Lambda) return QCLHIP_ERR_ARGUMENT;
     if (Gamma == Lambda) return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
     return hilbert_run(ctx, plan, Gamma, nullptr, nullptr, Lambda, 0, roundoff_host);
+}
+
+int qclhip_retarded_self_energy_adjoint(qclhip_ctx* ctx, const qclhip_hilbert* plan, const double* SRbar,
+                                        double* SLbar, double* SGbar) {
+    if (!ctx || !plan || !SRbar || !SLbar || !SGbar) return QCLHIP_ERR_ARGUMENT;
+    if (SLbar == SGbar || SRbar == SLbar || SRbar == SGbar)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
+    HilbertArgs a{};
+    a.NE = plan->NE;
+    a.ncols = plan->ncols;
+    a.wE = plan->grid->wE;
+    a.gamma = (const c64*)SRbar;
+    a.out = (c64*)SLbar;
+    a.out2 = (c64*)SGbar;
+    a.mode = 3;
+    return hilbert_launch(ctx, plan, a, 0);
 }
 
 int qclhip_retarded_self_energy(qclhip_ctx* ctx, const qclhip_hilbert* plan, const double* SL, const double* SG,

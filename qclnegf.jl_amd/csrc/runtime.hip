@@ -249,10 +249,38 @@ int qclhip_shift_create(qclhip_ctx* ctx, const qclhip_grid* grid, double delta, 
         wr[e] = (target - eps[j]) / (eps[j + 1] - eps[j]);
         valid[e] = 1;
     }
+    // Transpose (adjoint) of the interpolation as a gather per source node j:
+    // X[j] += sum over valid e with left_e = j of (1 - wr_e) Y[e] and with
+    // right_e = j of wr_e Y[e], in ascending e (deterministic).
+    std::vector<std::vector<std::pair<int, double>>> rows(N);
+    for (int e = 0; e < N; ++e) {
+        if (!valid[e]) continue;
+        if (1.0 - wr[e] != 0.0) rows[left[e]].push_back({e, 1.0 - wr[e]});
+        if (wr[e] != 0.0) rows[right[e]].push_back({e, wr[e]});
+    }
+    std::vector<int> tptr(N + 1, 0), tsrc;
+    std::vector<double> tw;
+    for (int jj = 0; jj < N; ++jj) {
+        for (auto& pr : rows[jj]) {
+            tsrc.push_back(pr.first);
+            tw.push_back(pr.second);
+        }
+        tptr[jj + 1] = (int)tsrc.size();
+    }
+    if (tsrc.empty()) {  // keep device allocations non-empty
+        tsrc.push_back(0);
+        tw.push_back(0.0);
+    }
     auto* s = new qclhip_shift();
     s->NE = N;
     s->delta = delta;
-    bool ok = hipMalloc(&s->left, N * sizeof(int)) == hipSuccess &&
+    bool ok = hipMalloc(&s->tptr, (N + 1) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&s->tsrc, tsrc.size() * sizeof(int)) == hipSuccess &&
+              hipMalloc(&s->tw, tw.size() * sizeof(double)) == hipSuccess &&
+              hipMemcpy(s->tptr, tptr.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->tsrc, tsrc.data(), tsrc.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMalloc(&s->left, N * sizeof(int)) == hipSuccess &&
               hipMalloc(&s->right, N * sizeof(int)) == hipSuccess &&
               hipMalloc(&s->wr, N * sizeof(double)) == hipSuccess &&
               hipMalloc(&s->valid, N) == hipSuccess;
@@ -275,6 +303,9 @@ int qclhip_shift_destroy(qclhip_shift* s) {
     if (s->right) hipFree(s->right);
     if (s->wr) hipFree(s->wr);
     if (s->valid) hipFree(s->valid);
+    if (s->tptr) hipFree(s->tptr);
+    if (s->tsrc) hipFree(s->tsrc);
+    if (s->tw) hipFree(s->tw);
     delete s;
     return QCLHIP_OK;
 }

@@ -92,6 +92,14 @@ _SIGNATURES = {
     "qclhip_state_populations": [_vp, _vp, _vp, ct.c_double, _dp],
     "qclhip_seed_weights": [_vp, _vp, _vp, _dp],
     "qclhip_seed_lesser": [_vp, _vp, _vp, ct.c_double, ct.c_double, _vp],
+    "qclhip_current_green_adjoint": [_vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_double, _dp,
+                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "qclhip_boundary_flux_adjoint": [_vp, _vp, ct.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "qclhip_shift_transpose": [_vp, _vp, _vp, _vp, ct.c_double, ct.c_int, _vp],
+    "qclhip_kernel_adjoint": [_vp, _vp, ct.POINTER(_vp)],
+    "qclhip_retarded_self_energy_adjoint": [_vp, _vp, _vp, _vp, _vp],
+    "qclhip_embedding_adjoint": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "qclhip_antihermitian_part": [_vp, _vp, _vp],
 }
 
 _lib = None

@@ -1,0 +1,148 @@
+"""Fixed-point adjoint of the production SCBA loop (no reference counterpart:
+algorithm_registry.jl:226-249 lists the adjoint as missing).
+
+At a converged state s* = F(s*) (F: current_green, then the scattering and
+embedding candidates of production_solver.jl:435-844) the current functional
+J = Jbar(s*.plus, G(s*)) (production_iteration.jl:189-201) has the gradient
+
+    dJ/dh = dJ/dh|_G + (dF/dh)^H mu,   mu = (dF/ds)^H mu + dJ/ds,
+
+and mu is found by the damped iteration mu <- (1-alpha) mu + alpha[(dF/ds)^H mu
++ dJ/ds] with the forward's mixing alpha, restricted to the tangent space of
+the state (S< and S> blocks anti-Hermitian; S^R free). There it converges at
+the forward's linear rate (the iteration matrix is the transpose of the
+forward's on that space). Every
+reverse operator runs on the device through the C-ABI (qclhip_*_adjoint,
+qclhip_shift_transpose, qclhip_kernel_adjoint).
+
+Gradient convention: Zbar = dJ/dRe Z + i dJ/dIm Z (dJ = Re sum conj(Zbar) dZ).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import api
+from .scba import Family, ScbaProblem, ScbaState, new_family, scba_iteration, ScbaWorkspace
+
+
+def converge(p: ScbaProblem, s: ScbaState, tol: float = 1e-15, max_iterations: int = 5000, min_iterations: int = 5):
+    """Iterate the forward loop until |dJ| <= tol |J| (J the boundary flux)."""
+    ws = ScbaWorkspace(p)
+    prev = None
+    it = 0
+    for it in range(1, max_iterations + 1):
+        row = scba_iteration(p, s, ws)
+        J = row["Jbar"]
+        if prev is not None and it >= min_iterations and abs(J - prev) <= tol * abs(J):
+            break
+        prev = J
+    return it
+
+
+class _Families:
+    """The adjoint state mu, one Family of gradients per SCBA family."""
+
+    def __init__(self, p: ScbaProblem):
+        g = p.grid
+        self.plus = new_family(g)
+        self.minus = new_family(g)
+        self.scat = {m.name: new_family(g) for m in p.mechanisms}
+
+    def all(self):
+        return [self.plus, self.minus] + list(self.scat.values())
+
+
+class ScbaAdjoint:
+    """Reverse operators of one problem: adjoint kernels built once."""
+
+    def __init__(self, p: ScbaProblem):
+        self.p = p
+        self.kadj = {m.name: m.kernel.adjoint() for m in p.mechanisms}
+
+    # -- reverse of the candidates: mu (families) -> (GRbar, GLbar, GGbar)
+    def candidates_vjp(self, mu: _Families, out):
+        p, g = self.p, self.p.grid
+        GRb, GLb, GGb = out
+        tmpL, tmpG, Z = api.empty_field(g), api.empty_field(g), api.empty_field(g)
+        for m in p.mechanisms:
+            f = mu.scat[m.name]
+            tmpL.copy_(f.L)
+            tmpG.copy_(f.G)
+            api.retarded_self_energy_adjoint(p.hilbert, f.R, tmpL, tmpG)
+            K = self.kadj[m.name]
+            if m.is_lo:
+                # S< = qK[(N+1) S+ K[G<] + N S- K[G<]],  S> = qK[(N+1) S- K[G>] + N S+ K[G>]]
+                api.shift_transpose(p.plusLO, tmpL, Z, alpha=p.NLO + 1.0)
+                api.shift_transpose(p.minusLO, tmpL, Z, alpha=p.NLO, accumulate=True)
+                api.production_static_contraction(K, Z, m.qK, beta=1.0, S=GLb)
+                api.shift_transpose(p.minusLO, tmpG, Z, alpha=p.NLO + 1.0)
+                api.shift_transpose(p.plusLO, tmpG, Z, alpha=p.NLO, accumulate=True)
+                api.production_static_contraction(K, Z, m.qK, beta=1.0, S=GGb)
+            else:
+                api.production_static_contraction(K, tmpL, m.qK, beta=1.0, S=GLb)
+                api.production_static_contraction(K, tmpG, m.qK, beta=1.0, S=GGb)
+        for T, plan, f in ((p.Tplus, p.plusE, mu.plus), (p.Tminus, p.minusE, mu.minus)):
+            api.embedding_adjoint(g, T, plan, f.R, GRb)
+            api.embedding_adjoint(g, T, plan, f.L, GLb)
+            api.embedding_adjoint(g, T, plan, f.G, GGb)
+
+    def gradient(self, s: ScbaState, tol: float = 1e-13, max_iterations: int = 5000, alpha: float | None = None):
+        """dJ/dh (device tensor (Nb,Nb,Nk), the reference layout of h) at the
+        converged state s, plus the adjoint state and iteration count."""
+        p, g = self.p, self.p.grid
+        alpha = p.alpha_mix if alpha is None else alpha
+        fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
+        green, info = api.current_green_production(g, p.h, fams, p.target, spectral=False)
+        # dJ/dG (constant) and the direct dJ/dplus.
+        JL, JG = api.zeros_field(g), api.zeros_field(g)
+        dplusL, dplusG = api.zeros_field(g), api.zeros_field(g)
+        api.boundary_flux_adjoint(g, 1.0, s.plus.L, s.plus.G, green.GL, green.GG, dplusL, dplusG, JL, JG)
+        api.antihermitian_part(g, dplusL)
+        api.antihermitian_part(g, dplusG)
+        mu = _Families(p)
+        new = _Families(p)
+        GRb, GLb, GGb = api.empty_field(g), api.empty_field(g), api.empty_field(g)
+        sig = (api.empty_field(g), api.empty_field(g), api.empty_field(g))
+        it, change = 0, float("inf")
+        history = []
+        for it in range(1, max_iterations + 1):
+            GRb.zero_()
+            GLb.copy_(JL)
+            GGb.copy_(JG)
+            self.candidates_vjp(mu, (GRb, GLb, GGb))
+            api.current_green_adjoint(g, fams, p.target, info, green, GLb, GGb, GRbar=GRb, want_hbar=False,
+                                      out=sig)
+            # Project S< and S> onto the anti-Hermitian blocks: the forward
+            # state lives there, and outside it the linearized map keeps a
+            # neutral mode (eigenvalue 1) that the forward never excites.
+            api.antihermitian_part(g, sig[1])
+            api.antihermitian_part(g, sig[2])
+            for f in new.all():
+                f.R.copy_(sig[0])
+                f.L.copy_(sig[1])
+                f.G.copy_(sig[2])
+            new.plus.L.add_(dplusL)
+            new.plus.G.add_(dplusG)
+            num = 0.0
+            den = 0.0
+            for a, b in zip(mu.all(), new.all()):
+                for x, y in zip(a.parts(), b.parts()):
+                    num += float(torch.linalg.vector_norm(y - x)) ** 2
+                    den += float(torch.linalg.vector_norm(y)) ** 2
+            pairs = []
+            for a, b in zip(mu.all(), new.all()):
+                pairs += list(zip(a.parts(), b.parts()))
+            api.mix_fields(pairs, alpha)
+            change = (num / max(den, 1e-300)) ** 0.5
+            if it == 1 or it % 100 == 0:
+                history.append((it, change))
+            if change <= tol:
+                break
+        GRb.zero_()
+        GLb.copy_(JL)
+        GGb.copy_(JG)
+        self.candidates_vjp(mu, (GRb, GLb, GGb))
+        _, hbar = api.current_green_adjoint(g, fams, p.target, info, green, GLb, GGb, GRbar=GRb, want_hbar=True,
+                                            out=sig)
+        return hbar, dict(iterations=it, change=change, info=info, history=history)

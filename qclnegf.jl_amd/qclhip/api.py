@@ -229,13 +229,22 @@ class ProductionKernel:
     natural (row, column) indexing).
     """
 
-    def __init__(self, grid: Grid, kind: str, a, b=None, c=None, rank: int = 0):
+    def __init__(self, grid: Grid, kind: str, a=None, b=None, c=None, rank: int = 0, _handle=None):
         self.grid, self.kind, self.rank = grid, kind, int(rank)
+        if _handle is not None:
+            self.handle = _handle
+            return
         keep = [_chost(x) if x is not None else (None, None) for x in (a, b, c)]
         h = _vp()
         grid.ctx.call("qclhip_kernel_create", grid.handle, _KINDS[kind], self.rank,
                       keep[0][1], keep[1][1], keep[2][1], ct.byref(h))
         self.handle = h
+
+    def adjoint(self) -> "ProductionKernel":
+        """(K diag(wk))^dagger in the same representation (qclhip_kernel_adjoint)."""
+        h = _vp()
+        self.grid.ctx.call("qclhip_kernel_adjoint", self.handle, ct.byref(h))
+        return ProductionKernel(self.grid, self.kind, rank=self.rank, _handle=h)
 
     def __del__(self):
         try:
@@ -518,3 +527,66 @@ def green_state(grid: Grid, GR, GL, spectral: bool = True, GG=None, A=None):
     A = (empty_field(grid) if A is None else A) if spectral else None
     grid.ctx.call("qclhip_green_state", grid.handle, _ptr(GR), _ptr(GL), _ptr(GG), _ptr(A))
     return GG, A
+
+
+# ---- A19: adjoint (vector-Jacobian products); no reference counterpart ----
+# Gradient convention: Zbar = dJ/dRe Z + i dJ/dIm Z for a real functional J.
+
+def current_green_adjoint(grid: Grid, families, target: float, info: dict, green: GreenState, GLbar, GGbar,
+                          GRbar=None, want_hbar: bool = True, out=None):
+    """Reverse of current_green_production: gradients (SRbar, SLbar, SGbar) of
+    the total self-energy (= of every family) and hbar (Nb,Nb,Nk) from the
+    gradients of the normalized outputs. `info` is the forward's scalars."""
+    nf = len(families)
+    arr = lambda k: (_vp * nf)(*[f[k].data_ptr() for f in families])
+    SRb, SLb, SGb = out if out is not None else (empty_field(grid), empty_field(grid), empty_field(grid))
+    hbar = torch.empty((grid.Nb, grid.Nb, grid.Nk), dtype=torch.complex128, device=grid.ctx.device) \
+        if want_hbar else None
+    coef = (ct.c_double * 5)(info["lam"], info["number"], info["holes"], info["a"], info["c"])
+    grid.ctx.call("qclhip_current_green_adjoint", grid.handle, nf, arr(1), arr(2), ct.c_double(target), coef,
+                  _ptr(green.GR), _ptr(green.GL), _ptr(green.GG), _ptr(GRbar), _ptr(GLbar), _ptr(GGbar),
+                  _ptr(SRb), _ptr(SLb), _ptr(SGb), _ptr(hbar))
+    return (SRb, SLb, SGb), hbar
+
+
+def boundary_flux_adjoint(grid: Grid, Jbar: float, plusL, plusG, GL, GG, plusLbar=None, plusGbar=None,
+                          GLbar=None, GGbar=None):
+    """Reverse of boundary_flux_bar; accumulates into the given gradients."""
+    grid.ctx.call("qclhip_boundary_flux_adjoint", grid.handle, ct.c_double(Jbar), _ptr(plusL), _ptr(plusG),
+                  _ptr(GL), _ptr(GG), _ptr(plusLbar), _ptr(plusGbar), _ptr(GLbar), _ptr(GGbar))
+
+
+def shift_transpose(plan: EnergyShiftPlan, Y, X=None, alpha: float = 1.0, accumulate: bool = False):
+    """X (+)= alpha S^T Y, the transpose of apply_energy_shift."""
+    X = torch.empty_like(Y) if X is None else X
+    plan.grid.ctx.call("qclhip_shift_transpose", plan.grid.handle, plan.handle, _ptr(Y), ct.c_double(alpha),
+                       int(accumulate), _ptr(X))
+    return X
+
+
+def retarded_self_energy_adjoint(plan: FFTHilbertPlan, SRbar, SLbar, SGbar):
+    """Reverse of retarded_self_energy_fft; accumulates into SLbar, SGbar."""
+    plan.grid.ctx.call("qclhip_retarded_self_energy_adjoint", plan.handle, _ptr(SRbar), _ptr(SLbar), _ptr(SGbar))
+
+
+def embedding_adjoint(grid: Grid, T, plan: EnergyShiftPlan, Sbar, Xbar):
+    """Reverse of embedding_component_production; accumulates
+    Xbar += S^T (T^dagger Sbar T)."""
+    grid.ctx.call("qclhip_embedding_adjoint", grid.handle, _ptr(T), plan.handle, _ptr(Sbar), _ptr(Xbar))
+
+
+def antihermitian_part(grid: Grid, X):
+    """X_em <- (X_em - X_em^dagger)/2 in place."""
+    grid.ctx.call("qclhip_antihermitian_part", grid.handle, _ptr(X))
+    return X
+
+
+def potential_gradient(hbar, Phi):
+    """Host chain rule to a potential sampled on the real-space grid:
+    dJ/dU_g = sum_m Re sum_ab conj(hbar_m[a,b]) conj(Phi[g,a]) Phi[g,b]
+    (dh_m/dU_g = Phi[g,:]^* Phi[g,:]^T, basis.jl:300). hbar: (Nb,Nb,Nk) device
+    tensor in the reference layout (element [b,a,m] is hbar_m[a,b]); Phi:
+    (Ng, Nb) host array."""
+    H = hbar.sum(dim=2).cpu().numpy().T  # sum over m; [a,b]
+    P = np.asarray(Phi)
+    return np.real(np.einsum("ab,ga,gb->g", np.conj(H), np.conj(P), P))
