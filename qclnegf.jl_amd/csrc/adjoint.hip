@@ -123,7 +123,7 @@ __global__ __launch_bounds__(64 * W) void k_green_adjoint(GAArgs p) {
     const int nvalid = min(W, p.NE - e0);
     const long long nblk = (long long)p.NE * p.Nk;
     const long long blk0 = e0 + (long long)p.NE * m;
-    const int wstride = NP * LD;
+    const int wstride = TileStride<B>::value;
     c64* G0 = reinterpret_cast<c64*>(smem);
     c64* X0 = G0 + (size_t)W * wstride;
     c64* Y0 = X0 + (size_t)W * wstride;
@@ -364,7 +364,7 @@ int shift_transpose(qclhip_ctx* ctx, const qclhip_shift* s, long long ncols, con
 
 template <int B>
 int launch_green_adjoint(qclhip_ctx* ctx, const GAArgs& a) {
-    constexpr size_t per_wave = 4 * (size_t)(8 * B) * (8 * B + 1) * sizeof(c64);
+    constexpr size_t per_wave = 4 * (size_t)TileStride<B>::value * sizeof(c64);
     constexpr int W = per_wave * 2 <= 80 * 1024 ? 2 : 1;
     const size_t lds = per_wave * W;
     if (lds > ctx->max_lds_per_block)

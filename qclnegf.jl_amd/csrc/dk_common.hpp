@@ -7,12 +7,22 @@
 
 namespace qclhip {
 
+// LDS stride between the per-wave tiles of one workgroup, in elements: the
+// (8B) x (8B+1) tile rounded up so that the stride is 2 (mod 8). Cooperative
+// transfers put the same (i,j) of consecutive energies in consecutive lanes;
+// with this stride those addresses start 8 dwords (complex double) or 4
+// dwords (complex float) apart instead of falling into the same LDS banks.
+template <int B>
+struct TileStride {
+    static constexpr int value = ((8 * B) * (8 * B + 1) + 7) / 8 * 8 + 2;
+};
+
 template <int B>
 struct DKShape {
     static constexpr int NP = 8 * B;      // padded matrix size
     static constexpr int LD = NP + 1;     // LDS row stride (complex elements)
     static constexpr size_t bytes =
-        (size_t)2 * NP * LD * sizeof(c64) + 2 * NP * sizeof(c64) + 2 * NP * sizeof(int);
+        (size_t)2 * TileStride<B>::value * sizeof(c64) + 2 * NP * sizeof(c64) + 2 * NP * sizeof(int);
 };
 
 constexpr int kMaxFamilies = 8;
@@ -203,6 +213,10 @@ __device__ inline void tile_store_from(const V* lds, int wstride, c64* dst, long
     }
 }
 
+// Workgroups of a persistent launch: the number resident on the device at
+// once (occupancy x CUs), a multiple of 8 so that each workgroup keeps one
+// XCD, and at most the number of tiles (runtime.hip).
+int persistent_grid(qclhip_ctx* ctx, const void* kernel, int threads, size_t lds, long long ntiles, unsigned* grid);
 // Launchers (dyson.hip, mixed.hip).
 int launch_dyson_fp64(qclhip_ctx* ctx, const DKArgs& a);
 int launch_keldysh_valu(qclhip_ctx* ctx, const DKArgs& a);
@@ -211,5 +225,66 @@ int launch_mixed_dyson(qclhip_ctx* ctx, const DKArgs& a);
 // out[e, col] (+)= alpha * (S^T Y)[e, col] over ncols columns (adjoint.hip).
 int shift_transpose(qclhip_ctx* ctx, const qclhip_shift* s, long long ncols, const c64* Y, double alpha,
                     int accumulate, c64* out);
+
+// Register-staged tile transfer for persistent kernels. Thread f of a
+// W-wave workgroup owns, in every tile, the elements ij = f/W + 64k
+// (k < EPT) of energy w = f % W: the mapping of tile_load, so loads stay
+// coalesced (consecutive threads, consecutive energies). The next tile's
+// inputs are fetched into registers while the current tile computes and
+// then written to LDS in one step.
+template <int B, int W>
+struct TileMap {
+    static constexpr int EPT = ((8 * B) * (8 * B) + 63) / 64;
+    static constexpr int LD = 8 * B + 1;
+    int w, n;
+    int gidx[EPT];  // i + N j
+    int lidx[EPT];  // i LD + j
+    __device__ inline void init(int N) {
+        w = threadIdx.x % W;
+        const int first = threadIdx.x / W;
+        n = 0;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int ij = first + 64 * k;
+            const int i = ij % N, j = ij / N;
+            const bool on = ij < N * N;
+            gidx[k] = on ? i + N * j : 0;
+            lidx[k] = on ? i * LD + j : 0;
+            n += on ? 1 : 0;
+        }
+    }
+    __device__ inline void fetch(c64 (&r)[EPT], const c64* const* fam, int nfam, long long blk0, int nvalid,
+                                 size_t S) const {
+        const bool on = w < nvalid;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k)
+            if (on && k < n) r[k] = fam_load(fam, nfam, blk0 + w + S * (size_t)gidx[k]);
+    }
+    __device__ inline void put(c64* lds, int wstride, const c64 (&r)[EPT], int nvalid) const {
+        const bool on = w < nvalid;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k)
+            if (on && k < n) lds[w * wstride + lidx[k]] = r[k];
+    }
+};
+
+// Tile d of a persistent loop: XCD-aware tile index, first energy, momentum,
+// valid energies and first block.
+struct TileCoord {
+    int e0, m, nvalid;
+    long long blk0;
+};
+
+template <int W>
+__device__ inline TileCoord tile_coord(long long d, long long ntiles, int NE) {
+    const int nte = (NE + W - 1) / W;
+    const long long t = xcd_tile(d, ntiles);
+    TileCoord c;
+    c.e0 = (int)(t % nte) * W;
+    c.m = (int)(t / nte);
+    c.nvalid = min(W, NE - c.e0);
+    c.blk0 = c.e0 + (long long)NE * c.m;
+    return c;
+}
 
 }  // namespace qclhip

@@ -33,33 +33,21 @@ template <int B, int W>
 struct DyShape {
     static constexpr int NP = 8 * B;
     static constexpr int LD = NP + 1;
-    static constexpr size_t per_wave = (size_t)NP * LD * sizeof(c64) + 2 * NP * sizeof(c64) + 2 * NP * sizeof(int);
+    static constexpr size_t per_wave =
+        (size_t)TileStride<B>::value * sizeof(c64) + 2 * NP * sizeof(c64) + 2 * NP * sizeof(int);
     static constexpr size_t bytes = per_wave * W;
 };
 
 template <int B, int W>
-__global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p) {
+__global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p, long long ntiles) {
     constexpr int NP = DyShape<B, W>::NP;
     constexpr int LD = DyShape<B, W>::LD;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ unsigned s_ok;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int nte = (p.NE + W - 1) / W;
-    const long long t = xcd_tile(blockIdx.x, gridDim.x);
-    const int e0 = (int)(t % nte) * W;
-    const int m = (int)(t / nte);
-    const int nvalid = min(W, p.NE - e0);
     const long long nblk = (long long)p.NE * p.Nk;
-    const long long blk0 = e0 + (long long)p.NE * m;
-    if (p.block_flags) {  // FP64 re-solve of tiles holding a flagged block only
-        unsigned any = 0;
-        for (int w = 0; w < nvalid; ++w) any |= p.block_flags[blk0 + w];
-        if (!any) return;
-    }
-    const int e = e0 + wave;
-    const long long blk = blk0 + wave;
-    const int wstride = NP * LD;
+    const int wstride = TileStride<B>::value;
     c64* Ms = reinterpret_cast<c64*>(smem) + (size_t)wave * wstride;
     unsigned char* tail = smem + (size_t)W * wstride * sizeof(c64) + (size_t)wave * (2 * NP * sizeof(c64) + 2 * NP * sizeof(int));
     c64* colbuf = reinterpret_cast<c64*>(tail);
@@ -69,9 +57,54 @@ __global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p) {
     const int N = p.N;
     const size_t S = (size_t)nblk;
     const int lr = lane & 7, lc = lane >> 3;
+    // Persistent loop over tiles (optionally fetching the next tile's
+    // self-energy into registers while the current one is eliminated). The
+    // FP64 re-solve of flagged tiles (block_flags set) loads on demand.
+    // Register prefetch is off: its registers would cost this kernel one of
+    // its three resident waves per SIMD, which it needs to hide the latency
+    // of the pivot chain (measured: slower with prefetch).
+    constexpr bool kPrefetch = false;
+    TileMap<B, W> tm;
+    if constexpr (kPrefetch) tm.init(N);
+    c64 rS[kPrefetch ? TileMap<B, W>::EPT : 1];
+    const bool prefetch = kPrefetch && p.block_flags == nullptr && p.nfamR > 0;
+    long long d = blockIdx.x;
+    if constexpr (kPrefetch) {
+        if (prefetch && d < ntiles) {
+            const TileCoord c = tile_coord<W>(d, ntiles, p.NE);
+            tm.fetch(rS, p.famR, p.nfamR, c.blk0, c.nvalid, S);
+        }
+    }
+    for (; d < ntiles; d += gridDim.x) {
+    const TileCoord tc = tile_coord<W>(d, ntiles, p.NE);
+    const int e0 = tc.e0, m = tc.m, nvalid = tc.nvalid;
+    const long long blk0 = tc.blk0;
+    if (p.block_flags) {  // FP64 re-solve of tiles holding a flagged block only
+        unsigned any = 0;
+        for (int w = 0; w < nvalid; ++w) any |= p.block_flags[blk0 + w];
+        if (!any) continue;
+    }
+    const int e = e0 + wave;
+    const long long blk = blk0 + wave;
+    __syncthreads();  // the previous tile's LDS is free
     if (threadIdx.x == 0) s_ok = 0;
-    if (p.nfamR) tile_load<W, LD>(reinterpret_cast<c64*>(smem), wstride, p.famR, p.nfamR, blk0, nvalid, N, S);
+    if (p.nfamR) {
+        if constexpr (kPrefetch) {
+            if (prefetch)
+                tm.put(reinterpret_cast<c64*>(smem), wstride, rS, nvalid);
+            else
+                tile_load<W, LD>(reinterpret_cast<c64*>(smem), wstride, p.famR, p.nfamR, blk0, nvalid, N, S);
+        } else {
+            tile_load<W, LD>(reinterpret_cast<c64*>(smem), wstride, p.famR, p.nfamR, blk0, nvalid, N, S);
+        }
+    }
     __syncthreads();
+    if constexpr (kPrefetch) {
+        if (prefetch && d + gridDim.x < ntiles) {
+            const TileCoord c = tile_coord<W>(d + gridDim.x, ntiles, p.NE);
+            tm.fetch(rS, p.famR, p.nfamR, c.blk0, c.nvalid, S);
+        }
+    }
     bool ok = wave < nvalid;
     double s = 0.0, rs = 0.0;
     c64 A[B][B];
@@ -225,6 +258,7 @@ __global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p) {
     }
     __syncthreads();
     tile_store<W, LD>(reinterpret_cast<const c64*>(smem), wstride, p.GR, blk0, s_ok, N, S);
+    }
 }
 
 // Keldysh products (A2): G<raw = G^R S< G^R^dagger and G>raw likewise, plus
@@ -232,21 +266,14 @@ __global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p) {
 // coalesced loads; each lane accumulates its B x B block of T = G S and of
 // T G^dagger; results return through LDS with coalesced stores.
 template <int B, int W>
-__global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p) {
+__global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) {
     constexpr int NP = DKShape<B>::NP;
     constexpr int LD = DKShape<B>::LD;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int nte = (p.NE + W - 1) / W;
-    const long long t = xcd_tile(blockIdx.x, gridDim.x);
-    const int e0 = (int)(t % nte) * W;
-    const int m = (int)(t / nte);
-    const int nvalid = min(W, p.NE - e0);
     const long long nblk = (long long)p.NE * p.Nk;
-    const long long blk0 = e0 + (long long)p.NE * m;
-    const long long blk = blk0 + wave;
-    const int wstride = NP * LD;
+    const int wstride = TileStride<B>::value;
     c64* G0 = reinterpret_cast<c64*>(smem);
     c64* X0 = G0 + (size_t)W * wstride;
     c64* Gs = G0 + (size_t)wave * wstride;
@@ -254,48 +281,90 @@ __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p) {
     const int N = p.N;
     const size_t S = (size_t)nblk;
     const int lr = lane & 7, lc = lane >> 3;
-    const bool active = wave < nvalid;
-    const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
+    (void)NP;
+    // Persistent loop: G^R and S< of the next tile, and S> of the current
+    // one, are fetched into registers while the products run.
+    // Register prefetch for B <= 3 (N <= 24); larger tiles would spill and
+    // load straight into LDS.
+    constexpr bool kPrefetch = B <= 3;
+    TileMap<B, W> tm;
+    tm.init(N);
+    constexpr int EPT = kPrefetch ? TileMap<B, W>::EPT : 1;
+    c64 rG[EPT], rS[EPT];
     const c64* gr_fam[1] = {p.GR};
-    tile_load<W, LD>(G0, wstride, gr_fam, 1, blk0, nvalid, N, S);
-    for (int comp = 0; comp < 2; ++comp) {
-        const c64* const* fam = comp ? p.famG : p.famL;
-        const int nf = comp ? p.nfamG : p.nfamL;
-        c64* dst = comp ? p.GG : p.GL;
-        tile_load<W, LD>(X0, wstride, fam, nf, blk0, nvalid, N, S);
-        __syncthreads();
-        c64 T[B][B];
-        if (active) {
-            block_product<B, false>(Gs, Xs, N, lr, lc, T);
-            wave_sync();
-#pragma unroll
-            for (int r = 0; r < B; ++r)
-#pragma unroll
-                for (int c = 0; c < B; ++c) Xs[(lr * B + r) * LD + lc * B + c] = T[r][c];
-            wave_sync();
-            block_product<B, true>(Xs, Gs, N, lr, lc, T);
-            wave_sync();
-            c64 tr = cmk(0.0, 0.0);
-#pragma unroll
-            for (int r = 0; r < B; ++r)
-#pragma unroll
-                for (int c = 0; c < B; ++c) {
-                    const int i = lr * B + r, j = lc * B + c;
-                    Xs[i * LD + j] = T[r][c];
-                    if (i == j && i < N) tr = cadd(tr, T[r][c]);
-                }
-            if (p.traces) {
-#pragma unroll
-                for (int off = 1; off < 64; off <<= 1) {
-                    tr.x += shfl_xor_d(tr.x, off);
-                    tr.y += shfl_xor_d(tr.y, off);
-                }
-                if (lane == 0) p.traces[(size_t)comp * nblk + blk] = tr;
-            }
+    long long d = blockIdx.x;
+    if constexpr (kPrefetch) {
+        if (d < ntiles) {
+            const TileCoord c = tile_coord<W>(d, ntiles, p.NE);
+            tm.fetch(rG, gr_fam, 1, c.blk0, c.nvalid, S);
+            tm.fetch(rS, p.famL, p.nfamL, c.blk0, c.nvalid, S);
+        }
+    }
+    for (; d < ntiles; d += gridDim.x) {
+        const TileCoord tc = tile_coord<W>(d, ntiles, p.NE);
+        const int nvalid = tc.nvalid;
+        const long long blk0 = tc.blk0;
+        const long long blk = blk0 + wave;
+        const bool active = wave < nvalid;
+        const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
+        __syncthreads();  // the previous tile's LDS is free
+        if constexpr (kPrefetch) {
+            tm.put(G0, wstride, rG, nvalid);
+            tm.put(X0, wstride, rS, nvalid);
+        } else {
+            tile_load<W, LD>(G0, wstride, gr_fam, 1, blk0, nvalid, N, S);
+            tile_load<W, LD>(X0, wstride, p.famL, p.nfamL, blk0, nvalid, N, S);
         }
         __syncthreads();
-        tile_store<W, LD>(X0, wstride, dst, blk0, mask, N, S);
-        __syncthreads();
+        if constexpr (kPrefetch) tm.fetch(rS, p.famG, p.nfamG, blk0, nvalid, S);
+        for (int comp = 0; comp < 2; ++comp) {
+            c64* dst = comp ? p.GG : p.GL;
+            if (comp == 1) {
+                if constexpr (kPrefetch) {
+                    tm.put(X0, wstride, rS, nvalid);
+                    if (d + gridDim.x < ntiles) {
+                        const TileCoord c = tile_coord<W>(d + gridDim.x, ntiles, p.NE);
+                        tm.fetch(rG, gr_fam, 1, c.blk0, c.nvalid, S);
+                        tm.fetch(rS, p.famL, p.nfamL, c.blk0, c.nvalid, S);
+                    }
+                } else {
+                    tile_load<W, LD>(X0, wstride, p.famG, p.nfamG, blk0, nvalid, N, S);
+                }
+                __syncthreads();
+            }
+            c64 T[B][B];
+            if (active) {
+                block_product<B, false>(Gs, Xs, N, lr, lc, T);
+                wave_sync();
+#pragma unroll
+                for (int r = 0; r < B; ++r)
+#pragma unroll
+                    for (int c = 0; c < B; ++c) Xs[(lr * B + r) * LD + lc * B + c] = T[r][c];
+                wave_sync();
+                block_product<B, true>(Xs, Gs, N, lr, lc, T);
+                wave_sync();
+                c64 tr = cmk(0.0, 0.0);
+#pragma unroll
+                for (int r = 0; r < B; ++r)
+#pragma unroll
+                    for (int c = 0; c < B; ++c) {
+                        const int i = lr * B + r, j = lc * B + c;
+                        Xs[i * LD + j] = T[r][c];
+                        if (i == j && i < N) tr = cadd(tr, T[r][c]);
+                    }
+                if (p.traces) {
+#pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) {
+                        tr.x += shfl_xor_d(tr.x, off);
+                        tr.y += shfl_xor_d(tr.y, off);
+                    }
+                    if (lane == 0) p.traces[(size_t)comp * nblk + blk] = tr;
+                }
+            }
+            __syncthreads();
+            tile_store<W, LD>(X0, wstride, dst, blk0, mask, N, S);
+            __syncthreads();
+        }
     }
 }
 
@@ -329,7 +398,7 @@ __global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
     const int nvalid = min(W, p.NE - e0);
     const long long nblk = (long long)p.NE * p.Nk;
     const long long mbase = (long long)p.NE * m;
-    const int wstride = NP * LD;
+    const int wstride = TileStride<B>::value;
     c64* Ts = reinterpret_cast<c64*>(smem);          // the constant T, shared
     c64* X0 = Ts + wstride;
     c64* Xs = X0 + (size_t)wave * wstride;
@@ -631,13 +700,15 @@ int launch_dyson(qclhip_ctx* ctx, const DKArgs& a) {
         QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_dyson<B, W>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
-    hipLaunchKernelGGL((k_dyson<B, W>), dim3((unsigned)tiles), dim3(64 * W), lds, ctx->stream, a);
+    // One tile per workgroup (the loop runs once): measured faster than a
+    // persistent grid for the elimination, which hides latency by occupancy.
+    hipLaunchKernelGGL((k_dyson<B, W>), dim3((unsigned)tiles), dim3(64 * W), lds, ctx->stream, a, tiles);
     return launch_check(ctx, "k_dyson");
 }
 
 template <int B>
 int launch_keldysh(qclhip_ctx* ctx, const DKArgs& a) {
-    constexpr size_t per_wave = 2 * (size_t)DKShape<B>::NP * DKShape<B>::LD * sizeof(c64);
+    constexpr size_t per_wave = 2 * (size_t)TileStride<B>::value * sizeof(c64);
     constexpr int W = tile_waves<B>(per_wave);
     const size_t lds = per_wave * W;
     if (lds > ctx->max_lds_per_block)
@@ -646,13 +717,16 @@ int launch_keldysh(qclhip_ctx* ctx, const DKArgs& a) {
         QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_keldysh<B, W>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
-    hipLaunchKernelGGL((k_keldysh<B, W>), dim3((unsigned)tiles), dim3(64 * W), lds, ctx->stream, a);
+    unsigned grid = 0;
+    int rc = persistent_grid(ctx, (const void*)k_keldysh<B, W>, 64 * W, lds, tiles, &grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_keldysh<B, W>), dim3(grid), dim3(64 * W), lds, ctx->stream, a, tiles);
     return launch_check(ctx, "k_keldysh");
 }
 
 template <int B>
 int launch_sw(qclhip_ctx* ctx, const SWArgs& a) {
-    constexpr size_t per_wave = (size_t)DKShape<B>::NP * DKShape<B>::LD * sizeof(c64);
+    constexpr size_t per_wave = (size_t)TileStride<B>::value * sizeof(c64);
     constexpr int W = tile_waves<B>(2 * per_wave);
     const size_t lds = per_wave * (W + 1);
     if (lds > ctx->max_lds_per_block)

@@ -159,7 +159,7 @@ __global__ __launch_bounds__(64 * W) void k_keldysh_mfma(DKArgs p) {
     const long long nblk = (long long)p.NE * p.Nk;
     const long long blk0 = e0 + (long long)p.NE * m;
     const long long blk = blk0 + wave;
-    const int wstride = NP * LD;
+    const int wstride = TileStride<B>::value;
     V* G0 = reinterpret_cast<V*>(smem);
     V* X0 = G0 + (size_t)W * wstride;
     V* Gs = G0 + (size_t)wave * wstride;
@@ -351,7 +351,7 @@ template <int B, int W>
 struct MxShape {
     static constexpr int NP = 8 * B;
     static constexpr int LD = NP + 1;
-    static constexpr size_t per_wave = (size_t)NP * LD * (sizeof(c64) + 2 * sizeof(float2));
+    static constexpr size_t per_wave = (size_t)TileStride<B>::value * (sizeof(c64) + 2 * sizeof(float2));
     static constexpr size_t bytes = per_wave * W;
 };
 
@@ -375,7 +375,7 @@ __global__ __launch_bounds__(64 * W) void k_dyson_mixed(DKArgs p) {
     const long long blk0 = e0 + (long long)p.NE * m;
     const int e = e0 + wave;
     const long long blk = blk0 + wave;
-    const int wstride = NP * LD;
+    const int wstride = TileStride<B>::value;
     c64* D0 = reinterpret_cast<c64*>(smem);
     float2* X0all = reinterpret_cast<float2*>(D0 + (size_t)W * wstride);
     float2* R0all = X0all + (size_t)W * wstride;
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(64 * W) void k_dyson_mixed(DKArgs p) {
 template <typename T, int B>
 int launch_keldysh_mfma_impl(qclhip_ctx* ctx, const DKArgs& a) {
     using V = typename Mfma<T>::cplx;
-    constexpr size_t per_wave = 2 * (size_t)(8 * B) * (8 * B + 1) * sizeof(V);
+    constexpr size_t per_wave = 2 * (size_t)TileStride<B>::value * sizeof(V);
     constexpr int W = per_wave * 4 <= 80 * 1024 ? 4 : (per_wave * 2 <= 80 * 1024 ? 2 : 1);
     const size_t lds = per_wave * W;
     if (lds > ctx->max_lds_per_block)

@@ -36,6 +36,15 @@ void* workspace(qclhip_ctx* ctx, size_t bytes) {
     return ctx->workspace;
 }
 
+int persistent_grid(qclhip_ctx* ctx, const void* kernel, int threads, size_t lds, long long ntiles, unsigned* grid) {
+    int per_cu = 0;
+    QCLHIP_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
+    long long g = (long long)std::max(per_cu, 1) * std::max(ctx->sm_count, 1);
+    g = std::max<long long>(8, g / 8 * 8);
+    *grid = (unsigned)std::max<long long>(1, std::min(g, ntiles));
+    return QCLHIP_OK;
+}
+
 int launch_check(qclhip_ctx* ctx, const char* what) {
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return hip_fail(ctx, err, what);
