@@ -48,6 +48,7 @@ typedef struct qclhip_grid qclhip_grid;
 typedef struct qclhip_shift qclhip_shift;
 typedef struct qclhip_kernel qclhip_kernel;
 typedef struct qclhip_hilbert qclhip_hilbert;
+typedef struct qclhip_batch qclhip_batch;
 
 /* ---- context ------------------------------------------------------------
  * One context per host thread and device; enqueues on the caller's stream
@@ -231,6 +232,26 @@ int qclhip_seed_weights(qclhip_ctx* ctx, const qclhip_grid* grid, const double* 
 /* G< = -f(eps; mu, kBT) (G^R - G^R^dagger). */
 int qclhip_seed_lesser(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR, double mu,
                        double kBT, double* GL);
+
+/* ---- batched GF solve over independent instances -------------------------
+ * No counterpart in the reference, which solves one problem per
+ * solve_production call (src/QCLNEGF.jl:982-983); a bias sweep calls it once
+ * per point. Instances have their own grids (N_E, N_k, N_b may differ);
+ * every kernel of current_green runs one launch per tile class over all
+ * instances of that class. FP64 (the grids' precision option is ignored). */
+int qclhip_batch_create(qclhip_ctx* ctx, int ninst, const qclhip_grid* const* grids,
+                        qclhip_batch** out);
+int qclhip_batch_destroy(qclhip_batch* batch);
+/* qclhip_current_green for every instance i: h[i], families
+ * famR/famL/famG[i*nfam + f] (HOST arrays of device pointers, nfam <= 8, the
+ * same count for all instances), target_host[i], outputs GR/GL/GG[i];
+ * out_host[5i..5i+4] = (lambda, number, holes, a, c) when not NULL (the call
+ * then synchronizes). */
+int qclhip_batch_current_green(qclhip_ctx* ctx, qclhip_batch* batch, const double* const* h,
+                               int nfam, const double* const* famR, const double* const* famL,
+                               const double* const* famG, const double* target_host,
+                               double* const* GR, double* const* GL, double* const* GG,
+                               double* out_host);
 
 /* ---- A19: adjoint (vector-Jacobian products) ----------------------------
  * No counterpart in the reference, which lists the adjoint as missing

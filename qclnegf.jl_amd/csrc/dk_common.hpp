@@ -213,6 +213,34 @@ __device__ inline void tile_store_from(const V* lds, int wstride, c64* dst, long
     }
 }
 
+// Batched: the workgroups of one launch cover the tiles of every instance of
+// one tile class (instances listed in `cls`, tile prefix sums in `begin`).
+struct BatchClass {
+    const DKArgs* insts;      // device table, one entry per instance
+    const int* inst;          // instances of this class
+    const long long* begin;   // [n+1] tile prefix over them
+    int n;
+};
+
+// Batched number functionals and paired coefficients: one workgroup per
+// instance.
+struct BatchScalars {
+    const c64* traces;
+    int NE, Nk, gs;
+    const double* wE;
+    const double* wk;
+    double target, limit;
+    PairCoefficients* coef;
+    c64* GL;
+    c64* GG;
+    long long n;
+};
+
+// Batched launches (dyson.hip): one launch per tile class and kernel.
+int batch_tile_waves(int B, bool keldysh);
+int launch_dk_batch(qclhip_ctx* ctx, int B, bool keldysh, const BatchClass& c, long long total);
+int launch_pair_batch(qclhip_ctx* ctx, const BatchScalars* d, int ninst, long long maxn);
+
 // Workgroups of a persistent launch: the number resident on the device at
 // once (occupancy x CUs), a multiple of 8 so that each workgroup keeps one
 // XCD, and at most the number of tiles (runtime.hip).

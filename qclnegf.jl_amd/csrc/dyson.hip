@@ -38,12 +38,12 @@ struct DyShape {
     static constexpr size_t bytes = per_wave * W;
 };
 
+// One tile (W consecutive energies of one momentum) of the FP64 Dyson step;
+// called by the single-instance and the batched kernels.
 template <int B, int W>
-__global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p, long long ntiles) {
+__device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned char* smem, unsigned& s_ok) {
     constexpr int NP = DyShape<B, W>::NP;
     constexpr int LD = DyShape<B, W>::LD;
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ unsigned s_ok;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const long long nblk = (long long)p.NE * p.Nk;
@@ -57,54 +57,13 @@ __global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p, long long ntiles) {
     const int N = p.N;
     const size_t S = (size_t)nblk;
     const int lr = lane & 7, lc = lane >> 3;
-    // Persistent loop over tiles (optionally fetching the next tile's
-    // self-energy into registers while the current one is eliminated). The
-    // FP64 re-solve of flagged tiles (block_flags set) loads on demand.
-    // Register prefetch is off: its registers would cost this kernel one of
-    // its three resident waves per SIMD, which it needs to hide the latency
-    // of the pivot chain (measured: slower with prefetch).
-    constexpr bool kPrefetch = false;
-    TileMap<B, W> tm;
-    if constexpr (kPrefetch) tm.init(N);
-    c64 rS[kPrefetch ? TileMap<B, W>::EPT : 1];
-    const bool prefetch = kPrefetch && p.block_flags == nullptr && p.nfamR > 0;
-    long long d = blockIdx.x;
-    if constexpr (kPrefetch) {
-        if (prefetch && d < ntiles) {
-            const TileCoord c = tile_coord<W>(d, ntiles, p.NE);
-            tm.fetch(rS, p.famR, p.nfamR, c.blk0, c.nvalid, S);
-        }
-    }
-    for (; d < ntiles; d += gridDim.x) {
-    const TileCoord tc = tile_coord<W>(d, ntiles, p.NE);
     const int e0 = tc.e0, m = tc.m, nvalid = tc.nvalid;
     const long long blk0 = tc.blk0;
-    if (p.block_flags) {  // FP64 re-solve of tiles holding a flagged block only
-        unsigned any = 0;
-        for (int w = 0; w < nvalid; ++w) any |= p.block_flags[blk0 + w];
-        if (!any) continue;
-    }
     const int e = e0 + wave;
     const long long blk = blk0 + wave;
-    __syncthreads();  // the previous tile's LDS is free
     if (threadIdx.x == 0) s_ok = 0;
-    if (p.nfamR) {
-        if constexpr (kPrefetch) {
-            if (prefetch)
-                tm.put(reinterpret_cast<c64*>(smem), wstride, rS, nvalid);
-            else
-                tile_load<W, LD>(reinterpret_cast<c64*>(smem), wstride, p.famR, p.nfamR, blk0, nvalid, N, S);
-        } else {
-            tile_load<W, LD>(reinterpret_cast<c64*>(smem), wstride, p.famR, p.nfamR, blk0, nvalid, N, S);
-        }
-    }
+    if (p.nfamR) tile_load<W, LD>(reinterpret_cast<c64*>(smem), wstride, p.famR, p.nfamR, blk0, nvalid, N, S);
     __syncthreads();
-    if constexpr (kPrefetch) {
-        if (prefetch && d + gridDim.x < ntiles) {
-            const TileCoord c = tile_coord<W>(d + gridDim.x, ntiles, p.NE);
-            tm.fetch(rS, p.famR, p.nfamR, c.blk0, c.nvalid, S);
-        }
-    }
     bool ok = wave < nvalid;
     double s = 0.0, rs = 0.0;
     c64 A[B][B];
@@ -258,13 +217,92 @@ __global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p, long long ntiles) {
     }
     __syncthreads();
     tile_store<W, LD>(reinterpret_cast<const c64*>(smem), wstride, p.GR, blk0, s_ok, N, S);
+}
+
+// One tile per workgroup. With block_flags set (the FP64 re-solve of the
+// mixed-precision path) only tiles holding a flagged block are solved.
+template <int B, int W>
+__global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p, long long ntiles) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ unsigned s_ok;
+    const TileCoord tc = tile_coord<W>(blockIdx.x, ntiles, p.NE);
+    if (p.block_flags) {
+        unsigned any = 0;
+        for (int w = 0; w < tc.nvalid; ++w) any |= p.block_flags[tc.blk0 + w];
+        if (!any) return;
     }
+    dyson_tile<B, W>(p, tc, smem, s_ok);
+}
+
+__device__ inline int batch_find(const BatchClass& c, long long g) {
+    int lo = 0, hi = c.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (c.begin[mid] <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int B, int W>
+__global__ __launch_bounds__(64 * W) void k_dyson_batch(BatchClass c) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ unsigned s_ok;
+    const long long total = c.begin[c.n];
+    const long long g = xcd_tile(blockIdx.x, total);
+    const int j = batch_find(c, g);
+    const DKArgs p = c.insts[c.inst[j]];
+    const long long t = g - c.begin[j];
+    const int nte = (p.NE + W - 1) / W;
+    TileCoord tc;
+    tc.e0 = (int)(t % nte) * W;
+    tc.m = (int)(t / nte);
+    tc.nvalid = min(W, p.NE - tc.e0);
+    tc.blk0 = tc.e0 + (long long)p.NE * tc.m;
+    dyson_tile<B, W>(p, tc, smem, s_ok);
 }
 
 // Keldysh products (A2): G<raw = G^R S< G^R^dagger and G>raw likewise, plus
 // the block traces. The workgroup stages G^R and S tiles through LDS with
 // coalesced loads; each lane accumulates its B x B block of T = G S and of
 // T G^dagger; results return through LDS with coalesced stores.
+// One Keldysh component of one block (one wavefront): Xs <- G Xs G^dagger
+// with G in Gs (both LDS tiles), and the block trace to *trace when given.
+template <int B>
+__device__ inline void keldysh_products(c64* Gs, c64* Xs, int N, int lane, c64* trace) {
+    constexpr int LD = DKShape<B>::LD;
+    const int lr = lane & 7, lc = lane >> 3;
+    c64 T[B][B];
+    block_product<B, false>(Gs, Xs, N, lr, lc, T);
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) Xs[(lr * B + r) * LD + lc * B + c] = T[r][c];
+    wave_sync();
+    block_product<B, true>(Xs, Gs, N, lr, lc, T);
+    wave_sync();
+    c64 tr = cmk(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr * B + r, j = lc * B + c;
+            Xs[i * LD + j] = T[r][c];
+            if (i == j && i < N) tr = cadd(tr, T[r][c]);
+        }
+    if (trace) {
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            tr.x += shfl_xor_d(tr.x, off);
+            tr.y += shfl_xor_d(tr.y, off);
+        }
+        if (lane == 0) *trace = tr;
+    }
+}
+
 template <int B, int W>
 __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) {
     constexpr int NP = DKShape<B>::NP;
@@ -332,39 +370,52 @@ __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) 
                 }
                 __syncthreads();
             }
-            c64 T[B][B];
-            if (active) {
-                block_product<B, false>(Gs, Xs, N, lr, lc, T);
-                wave_sync();
-#pragma unroll
-                for (int r = 0; r < B; ++r)
-#pragma unroll
-                    for (int c = 0; c < B; ++c) Xs[(lr * B + r) * LD + lc * B + c] = T[r][c];
-                wave_sync();
-                block_product<B, true>(Xs, Gs, N, lr, lc, T);
-                wave_sync();
-                c64 tr = cmk(0.0, 0.0);
-#pragma unroll
-                for (int r = 0; r < B; ++r)
-#pragma unroll
-                    for (int c = 0; c < B; ++c) {
-                        const int i = lr * B + r, j = lc * B + c;
-                        Xs[i * LD + j] = T[r][c];
-                        if (i == j && i < N) tr = cadd(tr, T[r][c]);
-                    }
-                if (p.traces) {
-#pragma unroll
-                    for (int off = 1; off < 64; off <<= 1) {
-                        tr.x += shfl_xor_d(tr.x, off);
-                        tr.y += shfl_xor_d(tr.y, off);
-                    }
-                    if (lane == 0) p.traces[(size_t)comp * nblk + blk] = tr;
-                }
-            }
+            if (active) keldysh_products<B>(Gs, Xs, N, lane, p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr);
             __syncthreads();
             tile_store<W, LD>(X0, wstride, dst, blk0, mask, N, S);
             __syncthreads();
         }
+    }
+}
+
+// Batched Keldysh products: one tile per workgroup over the tiles of every
+// instance of one tile class (no prefetch).
+template <int B, int W>
+__global__ __launch_bounds__(64 * W) void k_keldysh_batch(BatchClass cl) {
+    constexpr int LD = DKShape<B>::LD;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const long long total = cl.begin[cl.n];
+    const long long g = xcd_tile(blockIdx.x, total);
+    const int jj = batch_find(cl, g);
+    const DKArgs p = cl.insts[cl.inst[jj]];
+    const long long t = g - cl.begin[jj];
+    const int nte = (p.NE + W - 1) / W;
+    const int e0 = (int)(t % nte) * W;
+    const int m = (int)(t / nte);
+    const int nvalid = min(W, p.NE - e0);
+    const long long nblk = (long long)p.NE * p.Nk;
+    const long long blk0 = e0 + (long long)p.NE * m;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const long long blk = blk0 + wave;
+    const int wstride = TileStride<B>::value;
+    c64* G0 = reinterpret_cast<c64*>(smem);
+    c64* X0 = G0 + (size_t)W * wstride;
+    c64* Gs = G0 + (size_t)wave * wstride;
+    c64* Xs = X0 + (size_t)wave * wstride;
+    const int N = p.N;
+    const size_t S = (size_t)nblk;
+    const bool active = wave < nvalid;
+    const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
+    const c64* gr_fam[1] = {p.GR};
+    tile_load<W, LD>(G0, wstride, gr_fam, 1, blk0, nvalid, N, S);
+    for (int comp = 0; comp < 2; ++comp) {
+        tile_load<W, LD>(X0, wstride, comp ? p.famG : p.famL, comp ? p.nfamG : p.nfamL, blk0, nvalid, N, S);
+        __syncthreads();
+        if (active) keldysh_products<B>(Gs, Xs, N, lane, p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr);
+        __syncthreads();
+        tile_store<W, LD>(X0, wstride, comp ? p.GG : p.GL, blk0, mask, N, S);
+        __syncthreads();
     }
 }
 
@@ -481,12 +532,9 @@ __global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
 // for FP64 fields and realness_limit() of the grid otherwise.
 constexpr int kReduceThreads = 1024;
 
-__global__ __launch_bounds__(kReduceThreads) void k_number_pair(const c64* traces, int ncomp,
-                                                                  int NE, int Nk, const double* wE,
-                                                                  const double* wk, int gs,
-                                                                  double target, int pair, double limit,
-                                                                  PairCoefficients* coef,
-                                                                  DeviceStatus* st) {
+__device__ inline void number_pair_body(const c64* traces, int ncomp, int NE, int Nk, const double* wE,
+                                        const double* wk, int gs, double target, int pair, double limit,
+                                        PairCoefficients* coef, DeviceStatus* st) {
     __shared__ double red[2][2][kReduceThreads];
     const long long nblk = (long long)NE * Nk;
     const int t = threadIdx.x;
@@ -545,6 +593,33 @@ __global__ __launch_bounds__(kReduceThreads) void k_number_pair(const c64* trace
         }
     }
     *coef = out;
+}
+
+__global__ __launch_bounds__(kReduceThreads) void k_number_pair(const c64* traces, int ncomp,
+                                                                  int NE, int Nk, const double* wE,
+                                                                  const double* wk, int gs,
+                                                                  double target, int pair, double limit,
+                                                                  PairCoefficients* coef,
+                                                                  DeviceStatus* st) {
+    number_pair_body(traces, ncomp, NE, Nk, wE, wk, gs, target, pair, limit, coef, st);
+}
+
+
+__global__ __launch_bounds__(kReduceThreads) void k_number_pair_batch(const BatchScalars* b, DeviceStatus* st) {
+    const BatchScalars& d = b[blockIdx.x];
+    number_pair_body(d.traces, 2, d.NE, d.Nk, d.wE, d.wk, d.gs, d.target, 1, d.limit, d.coef, st);
+}
+
+// Batched paired map: blockIdx.y is the instance.
+__global__ void k_normalize_pair_batch(const BatchScalars* b) {
+    const BatchScalars& d = b[blockIdx.y];
+    const double a = d.coef->a, c = d.coef->c;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const c64 l = d.GL[i], g = d.GG[i];
+        d.GL[i] = cmk(a * l.x - c * g.x, a * l.y - c * g.y);
+        d.GG[i] = cmk((1.0 - c) * g.x - (1.0 - a) * l.x, (1.0 - c) * g.y - (1.0 - a) * l.y);
+    }
 }
 
 // Block traces of one field (for the standalone number functional).
@@ -800,6 +875,68 @@ int dyson_keldysh_dispatch(qclhip_ctx* ctx, const qclhip_grid* grid, DKArgs a, b
     }
     if (grid->keldysh_engine == QCLHIP_ENGINE_MATRIX) return launch_keldysh_mfma(ctx, a, false);
     return launch_keldysh_valu(ctx, a);
+}
+
+template <int B>
+int launch_dyson_batch_b(qclhip_ctx* ctx, const BatchClass& c, long long total) {
+    constexpr int W = kDysonTileWaves;
+    const size_t lds = DyShape<B, W>::bytes;
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_dyson_batch<B, W>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_dyson_batch<B, W>), dim3((unsigned)total), dim3(64 * W), lds, ctx->stream, c);
+    return launch_check(ctx, "k_dyson_batch");
+}
+
+template <int B>
+constexpr int keldysh_waves() {
+    return tile_waves<B>(2 * (size_t)TileStride<B>::value * sizeof(c64));
+}
+
+template <int B>
+int launch_keldysh_batch_b(qclhip_ctx* ctx, const BatchClass& c, long long total) {
+    constexpr int W = keldysh_waves<B>();
+    const size_t lds = 2 * (size_t)TileStride<B>::value * sizeof(c64) * W;
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_keldysh_batch<B, W>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_keldysh_batch<B, W>), dim3((unsigned)total), dim3(64 * W), lds, ctx->stream, c);
+    return launch_check(ctx, "k_keldysh_batch");
+}
+
+int batch_tile_waves(int B, bool keldysh) {
+    if (!keldysh) return kDysonTileWaves;
+    switch (B) {
+        case 1: return keldysh_waves<1>();
+        case 2: return keldysh_waves<2>();
+        case 3: return keldysh_waves<3>();
+        case 4: return keldysh_waves<4>();
+        case 5: return keldysh_waves<5>();
+        default: return keldysh_waves<6>();
+    }
+}
+
+int launch_dk_batch(qclhip_ctx* ctx, int B, bool keldysh, const BatchClass& c, long long total) {
+    switch (B) {
+        case 1: return keldysh ? launch_keldysh_batch_b<1>(ctx, c, total) : launch_dyson_batch_b<1>(ctx, c, total);
+        case 2: return keldysh ? launch_keldysh_batch_b<2>(ctx, c, total) : launch_dyson_batch_b<2>(ctx, c, total);
+        case 3: return keldysh ? launch_keldysh_batch_b<3>(ctx, c, total) : launch_dyson_batch_b<3>(ctx, c, total);
+        case 4: return keldysh ? launch_keldysh_batch_b<4>(ctx, c, total) : launch_dyson_batch_b<4>(ctx, c, total);
+        case 5: return keldysh ? launch_keldysh_batch_b<5>(ctx, c, total) : launch_dyson_batch_b<5>(ctx, c, total);
+        case 6: return keldysh ? launch_keldysh_batch_b<6>(ctx, c, total) : launch_dyson_batch_b<6>(ctx, c, total);
+        default: return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
+    }
+}
+
+int launch_pair_batch(qclhip_ctx* ctx, const BatchScalars* d, int ninst, long long maxn) {
+    hipLaunchKernelGGL(k_number_pair_batch, dim3(ninst), dim3(kReduceThreads), 0, ctx->stream, d, ctx->status);
+    const unsigned blocks = std::max(1u, std::min(grid_for(maxn, 256, ctx->sm_count), 65535u));
+    hipLaunchKernelGGL(k_normalize_pair_batch, dim3(blocks, ninst), dim3(256), 0, ctx->stream, d);
+    return launch_check(ctx, "batched number functionals");
 }
 
 }  // namespace qclhip

@@ -100,6 +100,10 @@ _SIGNATURES = {
     "qclhip_retarded_self_energy_adjoint": [_vp, _vp, _vp, _vp, _vp],
     "qclhip_embedding_adjoint": [_vp, _vp, _vp, _vp, _vp, _vp],
     "qclhip_antihermitian_part": [_vp, _vp, _vp],
+    "qclhip_batch_create": [_vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp)],
+    "qclhip_batch_destroy": [_vp],
+    "qclhip_batch_current_green": [_vp, _vp, ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),
+                                   ct.POINTER(_vp), _dp, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp), _dp],
 }
 
 _lib = None

@@ -590,3 +590,49 @@ def potential_gradient(hbar, Phi):
     H = hbar.sum(dim=2).cpu().numpy().T  # sum over m; [a,b]
     P = np.asarray(Phi)
     return np.real(np.einsum("ab,ga,gb->g", np.conj(H), np.conj(P), P))
+
+class Batch:
+    """Independent instances with their own grids (uneven N_E, N_k, N_b),
+    solved together: one launch per kernel and tile class (qclhip_batch_*)."""
+
+    def __init__(self, grids):
+        self.grids = list(grids)
+        self.ctx = self.grids[0].ctx
+        arr = (_vp * len(self.grids))(*[g.handle for g in self.grids])
+        h = _vp()
+        self.ctx.call("qclhip_batch_create", len(self.grids), arr, ct.byref(h))
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.ctx.lib.qclhip_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def batch_current_green(batch: Batch, hs, families, targets, states=None):
+    """current_green_production for every instance of the batch. families[i]
+    is instance i's list of (SR, SL, SG) triples (the same count for all).
+    Returns (list of GreenState without A, list of info dicts)."""
+    n = len(batch.grids)
+    nf = len(families[0])
+    if any(len(f) != nf for f in families):
+        raise _lib.ArgumentError("every instance needs the same number of families")
+    for g, fams in zip(batch.grids, families):
+        for fam in fams:
+            for t in fam:
+                _check_field(g, t, "self-energy")
+    if states is None:
+        states = [GreenState(empty_field(g), empty_field(g), empty_field(g), None, None) for g in batch.grids]
+    fam = lambda k: (_vp * (n * nf))(*[families[i][f][k].data_ptr() for i in range(n) for f in range(nf)])
+    ptrs = lambda xs: (_vp * n)(*[x.data_ptr() for x in xs])
+    tg = np.ascontiguousarray(targets, dtype=np.float64)
+    out = (ct.c_double * (5 * n))()
+    batch.ctx.call("qclhip_batch_current_green", batch.handle, ptrs(hs), nf, fam(0), fam(1), fam(2),
+                   tg.ctypes.data_as(_dp), ptrs([s.GR for s in states]), ptrs([s.GL for s in states]),
+                   ptrs([s.GG for s in states]), out)
+    infos = [dict(lam=out[5 * i], number=out[5 * i + 1], holes=out[5 * i + 2], a=out[5 * i + 3], c=out[5 * i + 4])
+             for i in range(n)]
+    return states, infos
