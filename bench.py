@@ -125,7 +125,7 @@ def gf_bench(steps, warmup, variants=VARIANTS):
     return dict(N=N, NE=NE, Nk=Nk, nblk=nblk, flops=flops, bytes=bytes_, variants=out)
 
 
-def scba_bench(iters, warmup):
+def scba_bench(iters, warmup, adjoint=True):
     import numpy as np
 
     from qclhip import api, scba, synthetic as syn
@@ -163,7 +163,67 @@ def scba_bench(iters, warmup):
     for _ in range(iters):
         row = scba.scba_iteration(p, s, ws, timings)
     dt = (time.perf_counter() - t0) / iters
-    return dict(iter_s=dt, phases_ms={k: 1e3 * v / iters for k, v in timings.items()}, last=row)
+    out = dict(iter_s=dt, phases_ms={k: 1e3 * v / iters for k, v in timings.items()}, last=row)
+    if adjoint:
+        # Fixed-point adjoint iterations at the same size (the gradient dJ/dh):
+        # the cost of one reverse sweep, from the difference of 1 and 4 steps.
+        from qclhip import adjoint as adj
+        import torch
+        A = adj.ScbaAdjoint(p)
+        A.gradient(s, tol=0.0, max_iterations=1)
+        times = []
+        for k in (1, 4):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            A.gradient(s, tol=0.0, max_iterations=k)
+            g.ctx.synchronize()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        out["adjoint_iter_ms"] = (times[1] - times[0]) / 3 * 1e3
+        out["adjoint_gradient_1iter_ms"] = times[0] * 1e3
+    return out
+
+
+def batch_bench(reps=3, ninst=16, Nk=16, N_range=(12, 48), NE_range=(1024, 4096)):
+    """Uneven sweep: ninst instances with N and NE uniform in the given ranges,
+    Nk momenta, two families each; one batched current_green call against
+    the same instances solved one call at a time."""
+    import numpy as np
+    import torch
+
+    from qclhip import api, synthetic as syn
+
+    rng = np.random.default_rng(syn.SEED + 4)
+    grids, hs, fams, shapes = [], [], [], []
+    for i in range(ninst):
+        N, NE = int(rng.integers(N_range[0], N_range[1] + 1)), int(rng.integers(NE_range[0], NE_range[1] + 1))
+        w = syn.gf_workload(N=N, NE=NE, Nk=Nk, seed=syn.SEED + 10 + i)
+        grids.append(api.Grid(w["eps"], w["wE"], w["wk"], N))
+        hs.append(api.h_to_device(w["h"]))
+        fams.append([syn.diag_fields_device(syn.injection_family(w)), syn.diag_fields_device(w)])
+        shapes.append((N, NE))
+    batch = api.Batch(grids)
+    targets = [1.0] * ninst
+    states, _ = api.batch_current_green(batch, hs, fams, targets)
+    nblk = sum(NE * Nk for _, NE in shapes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        api.batch_current_green(batch, hs, fams, targets, states=states)
+    grids[0].ctx.synchronize()
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for i in range(ninst):
+            api.current_green_production(grids[i], hs[i], fams[i], targets[i], state=states[i], spectral=False,
+                                         return_scalars=False)
+    grids[0].ctx.synchronize()
+    torch.cuda.synchronize()
+    ts = (time.perf_counter() - t0) / reps
+    return dict(instances=ninst, Nk=Nk, solves=nblk, batch_ms=tb * 1e3, batch_solves_per_s=nblk / tb,
+                one_by_one_ms=ts * 1e3, one_by_one_solves_per_s=nblk / ts,
+                shapes=f"N in U{{{N_range[0]}..{N_range[1]}}}, NE in U{{{NE_range[0]}..{NE_range[1]}}}")
 
 
 def cpu_baseline(seconds=15.0):
@@ -205,6 +265,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scba-iters", type=int, default=3)
     ap.add_argument("--no-scba", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -236,6 +297,18 @@ def main():
         log("scba iteration: N=24 NE=2048 Nk=64, 3 mechanisms + embedding")
         sc = scba_bench(args.scba_iters, 1)
         log(f"scba {sc['iter_s']*1e3:.1f} ms/iter {sc['phases_ms']}")
+        if "adjoint_iter_ms" in sc:
+            log(f"adjoint {sc['adjoint_iter_ms']:.1f} ms/iter")
+    bt = None
+    if not args.no_batch:
+        bt = {}
+        for name, kw in (("uneven_large", dict(ninst=16, Nk=16)),
+                         ("small_instances", dict(ninst=64, Nk=5, N_range=(3, 8), NE_range=(49, 200), reps=10))):
+            log(f"batched sweep {name}: {kw}")
+            r = batch_bench(**kw)
+            bt[name] = r
+            log(f"batch {r['batch_ms']:.2f} ms ({r['batch_solves_per_s']/1e6:.2f} M solves/s), one call per "
+                f"instance {r['one_by_one_ms']:.2f} ms")
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle)")
@@ -294,6 +367,12 @@ def main():
                                "linear mixing; FP64")
         line["scba_last"] = {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in sc["last"].items()
                              if k != "populations"}
+        if "adjoint_iter_ms" in sc:
+            line["adjoint"] = {"iter_ms": sc["adjoint_iter_ms"], "gradient_with_1_iter_ms": sc["adjoint_gradient_1iter_ms"],
+                               "config": "N=24 NE=2048 Nk=64, same problem as the SCBA iteration: one damped "
+                                         "fixed-point step = candidate reverse + current_green reverse + mixing"}
+    if bt is not None:
+        line["batch"] = bt
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

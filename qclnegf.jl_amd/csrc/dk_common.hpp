@@ -90,7 +90,11 @@ __device__ inline long long xcd_tile(long long b, long long nb) {
 // _total_family_production order: fam[0] + fam[1] + ...).
 __device__ inline c64 fam_load(const c64* const* fam, int nfam, size_t idx) {
     c64 v = fam[0][idx];
-    for (int f = 1; f < nfam; ++f) v = cadd(v, fam[f][idx]);
+    // Statically unrolled so that a family table copied into a kernel's
+    // registers is never indexed dynamically (which would move it to scratch).
+#pragma unroll
+    for (int f = 1; f < kMaxFamilies; ++f)
+        if (f < nfam) v = cadd(v, fam[f][idx]);
     return v;
 }
 

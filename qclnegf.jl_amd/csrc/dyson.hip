@@ -378,44 +378,48 @@ __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) 
     }
 }
 
-// Batched Keldysh products: one tile per workgroup over the tiles of every
-// instance of one tile class (no prefetch).
+// Batched Keldysh products: a persistent grid over the tiles of every
+// instance of one tile class (each tile finds its instance; no prefetch).
 template <int B, int W>
 __global__ __launch_bounds__(64 * W) void k_keldysh_batch(BatchClass cl) {
     constexpr int LD = DKShape<B>::LD;
     extern __shared__ __align__(16) unsigned char smem[];
     const long long total = cl.begin[cl.n];
-    const long long g = xcd_tile(blockIdx.x, total);
-    const int jj = batch_find(cl, g);
-    const DKArgs p = cl.insts[cl.inst[jj]];
-    const long long t = g - cl.begin[jj];
-    const int nte = (p.NE + W - 1) / W;
-    const int e0 = (int)(t % nte) * W;
-    const int m = (int)(t / nte);
-    const int nvalid = min(W, p.NE - e0);
-    const long long nblk = (long long)p.NE * p.Nk;
-    const long long blk0 = e0 + (long long)p.NE * m;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const long long blk = blk0 + wave;
     const int wstride = TileStride<B>::value;
     c64* G0 = reinterpret_cast<c64*>(smem);
     c64* X0 = G0 + (size_t)W * wstride;
     c64* Gs = G0 + (size_t)wave * wstride;
     c64* Xs = X0 + (size_t)wave * wstride;
-    const int N = p.N;
-    const size_t S = (size_t)nblk;
-    const bool active = wave < nvalid;
-    const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
-    const c64* gr_fam[1] = {p.GR};
-    tile_load<W, LD>(G0, wstride, gr_fam, 1, blk0, nvalid, N, S);
-    for (int comp = 0; comp < 2; ++comp) {
-        tile_load<W, LD>(X0, wstride, comp ? p.famG : p.famL, comp ? p.nfamG : p.nfamL, blk0, nvalid, N, S);
-        __syncthreads();
-        if (active) keldysh_products<B>(Gs, Xs, N, lane, p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr);
-        __syncthreads();
-        tile_store<W, LD>(X0, wstride, comp ? p.GG : p.GL, blk0, mask, N, S);
-        __syncthreads();
+    for (long long d = blockIdx.x; d < total; d += gridDim.x) {
+        const long long g = xcd_tile(d, total);
+        const int jj = batch_find(cl, g);
+        const DKArgs p = cl.insts[cl.inst[jj]];
+        const long long t = g - cl.begin[jj];
+        const int nte = (p.NE + W - 1) / W;
+        const int e0 = (int)(t % nte) * W;
+        const int m = (int)(t / nte);
+        const int nvalid = min(W, p.NE - e0);
+        const long long nblk = (long long)p.NE * p.Nk;
+        const long long blk0 = e0 + (long long)p.NE * m;
+        const long long blk = blk0 + wave;
+        const int N = p.N;
+        const size_t S = (size_t)nblk;
+        const bool active = wave < nvalid;
+        const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
+        const c64* gr_fam[1] = {p.GR};
+        __syncthreads();  // the previous tile's LDS is free
+        tile_load<W, LD>(G0, wstride, gr_fam, 1, blk0, nvalid, N, S);
+        for (int comp = 0; comp < 2; ++comp) {
+            tile_load<W, LD>(X0, wstride, comp ? p.famG : p.famL, comp ? p.nfamG : p.nfamL, blk0, nvalid, N, S);
+            __syncthreads();
+            if (active)
+                keldysh_products<B>(Gs, Xs, N, lane, p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr);
+            __syncthreads();
+            tile_store<W, LD>(X0, wstride, comp ? p.GG : p.GL, blk0, mask, N, S);
+            __syncthreads();
+        }
     }
 }
 
@@ -904,7 +908,10 @@ int launch_keldysh_batch_b(qclhip_ctx* ctx, const BatchClass& c, long long total
     if (lds > 65536)
         QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_keldysh_batch<B, W>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_keldysh_batch<B, W>), dim3((unsigned)total), dim3(64 * W), lds, ctx->stream, c);
+    unsigned grid = 0;
+    int rc = persistent_grid(ctx, (const void*)k_keldysh_batch<B, W>, 64 * W, lds, total, &grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_keldysh_batch<B, W>), dim3(grid), dim3(64 * W), lds, ctx->stream, c);
     return launch_check(ctx, "k_keldysh_batch");
 }
 
