@@ -125,13 +125,15 @@ def gf_bench(steps, warmup, variants=VARIANTS):
     return dict(N=N, NE=NE, Nk=Nk, nblk=nblk, flops=flops, bytes=bytes_, variants=out)
 
 
-def scba_bench(iters, warmup, adjoint=True):
+def scba_problem(N=24, NE=2048, Nk=64):
+    """The SCBA benchmark problem: seeded synthetic Hamiltonian, three
+    mechanisms (momentum-independent, separable IFR, low-rank thermal LO),
+    bulk embedding, seed state at a fixed chemical potential."""
     import numpy as np
 
     from qclhip import api, scba, synthetic as syn
 
-    w = syn.gf_workload()
-    N, NE, Nk = w["N"], w["NE"], w["Nk"]
+    w = syn.gf_workload(N=N, NE=NE, Nk=Nk)
     rng = np.random.default_rng(syn.SEED + 7)
     g = api.Grid(w["eps"], w["wE"], w["wk"], N)
     mechs = [
@@ -155,6 +157,14 @@ def scba_bench(iters, warmup, adjoint=True):
     p = scba.ScbaProblem(g, h, api.matrix_to_device(T), api.matrix_to_device(T.conj().T), mechs, target,
                          Ep=0.5, hwLO=hw, NLO=1.0 / math.expm1(hw / kbt), alpha_mix=0.3)
     s = scba.initial_state(p, seed_green)
+    return p, s
+
+
+def scba_bench(iters, warmup, adjoint=True):
+    from qclhip import scba
+
+    p, s = scba_problem()
+    g = p.grid
     ws = scba.ScbaWorkspace(p)
     for _ in range(warmup):
         scba.scba_iteration(p, s, ws)
@@ -181,6 +191,34 @@ def scba_bench(iters, warmup, adjoint=True):
             times.append(time.perf_counter() - t0)
         out["adjoint_iter_ms"] = (times[1] - times[0]) / 3 * 1e3
         out["adjoint_gradient_1iter_ms"] = times[0] * 1e3
+    return out
+
+
+def graph_bench(iters=200):
+    """Small SCBA problem (the tutorial's size: N=3, NE=49, Nk=5), launch-bound:
+    eager iterations against replays of one captured iteration."""
+    import torch
+
+    from qclhip import scba
+
+    out = {}
+    for name, (N, NE, Nk) in (("tutorial_N3_NE49_Nk5", (3, 49, 5)), ("small_N8_NE256_Nk8", (8, 256, 8))):
+        p, s = scba_problem(N, NE, Nk)
+        ws = scba.ScbaWorkspace(p)
+        for _ in range(3):
+            scba.scba_iteration(p, s, ws)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            scba.scba_iteration(p, s, ws)
+        te = (time.perf_counter() - t0) / iters
+        graph = scba.IterationGraph(p, s, ws)
+        graph.replay()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            graph.replay()
+        tg = (time.perf_counter() - t0) / iters
+        out[name] = dict(eager_ms_per_iter=te * 1e3, graph_ms_per_iter=tg * 1e3, speedup=te / tg)
     return out
 
 
@@ -266,6 +304,7 @@ def main():
     ap.add_argument("--scba-iters", type=int, default=3)
     ap.add_argument("--no-scba", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -299,6 +338,11 @@ def main():
         log(f"scba {sc['iter_s']*1e3:.1f} ms/iter {sc['phases_ms']}")
         if "adjoint_iter_ms" in sc:
             log(f"adjoint {sc['adjoint_iter_ms']:.1f} ms/iter")
+    gr = None
+    if not args.no_graph:
+        log("graph replay of small SCBA iterations")
+        gr = graph_bench()
+        log(f"graph {gr}")
     bt = None
     if not args.no_batch:
         bt = {}
@@ -371,6 +415,8 @@ def main():
             line["adjoint"] = {"iter_ms": sc["adjoint_iter_ms"], "gradient_with_1_iter_ms": sc["adjoint_gradient_1iter_ms"],
                                "config": "N=24 NE=2048 Nk=64, same problem as the SCBA iteration: one damped "
                                          "fixed-point step = candidate reverse + current_green reverse + mixing"}
+    if gr is not None:
+        line["graph"] = gr
     if bt is not None:
         line["batch"] = bt
     if cpu is not None:

@@ -60,6 +60,30 @@ int qclhip_ctx_destroy(qclhip_ctx* ctx);
 /* Waits for the stream, returns (and clears) the latched device status. */
 int qclhip_ctx_synchronize(qclhip_ctx* ctx);
 const char* qclhip_ctx_last_error(const qclhip_ctx* ctx);
+/* Moves the context (and its rocBLAS handle) to another stream. */
+int qclhip_ctx_set_stream(qclhip_ctx* ctx, void* stream);
+/* Deferred results: while dev_buffer != NULL, the entries that return host
+ * scalars (current_green's out_host, number_functional, boundary_flux,
+ * state_populations, the FFT round-off witness) neither synchronize nor
+ * write their host outputs; they copy their values, on the stream, to
+ * dev_buffer[slot...] (current_green: lambda, number, holes, a, c; the others
+ * in their host order) and advance slot. qclhip_ctx_result_slot gives the
+ * next slot. NULL ends the mode; every call resets slot to 0. This is what
+ * makes a loop iteration capturable as a graph. */
+int qclhip_ctx_defer_results(qclhip_ctx* ctx, double* dev_buffer, int capacity);
+int qclhip_ctx_result_slot(const qclhip_ctx* ctx);
+
+/* ---- graphs (the plan's graph API) ---------------------------------------
+ * Everything enqueued on the context's stream (which must not be the
+ * default stream) between begin and end becomes one executable graph;
+ * launch replays it on the stream. No allocation may happen during capture:
+ * run the captured sequence once eagerly first (it sizes the scratch), and
+ * call synchronizing entries only in deferred-results mode. */
+typedef struct qclhip_graph qclhip_graph;
+int qclhip_graph_begin(qclhip_ctx* ctx);
+int qclhip_graph_end(qclhip_ctx* ctx, qclhip_graph** out);
+int qclhip_graph_launch(qclhip_ctx* ctx, const qclhip_graph* graph);
+int qclhip_graph_destroy(qclhip_graph* graph);
 /* Build-time maximum basis size of the Dyson/Keldysh kernels. */
 int qclhip_max_basis(void);
 

@@ -107,6 +107,12 @@ struct qclhip_ctx {
     int sm_count = 0;
     size_t max_lds_per_block = 65536;
     std::string last_error;
+    // Deferred results: entries that return host scalars write them to
+    // defer_dev[defer_next...] on the device instead and do not synchronize.
+    double* defer_dev = nullptr;
+    int defer_cap = 0;
+    int defer_next = 0;
+    bool capturing = false;  // inside qclhip_graph_begin/_end
 };
 
 struct qclhip_grid {
@@ -145,6 +151,12 @@ int hip_fail(qclhip_ctx* ctx, hipError_t err, const char* what);
 // Device scratch of at least `bytes`, owned by the context.
 void* workspace(qclhip_ctx* ctx, size_t bytes);
 int launch_check(qclhip_ctx* ctx, const char* what);
+// In deferred-results mode, copy n device doubles to the next result slots
+// and set *deferred; otherwise leave *deferred false.
+int defer_result(qclhip_ctx* ctx, const double* src, int n, bool* deferred);
+// The same for the paired coefficients, in out_host order
+// (lambda, number, holes, a, c).
+int defer_coefficients(qclhip_ctx* ctx, const PairCoefficients* src, bool* deferred);
 // Gamma = i (S> - S<) elementwise over n complex entries.
 int gamma_field(qclhip_ctx* ctx, long long n, const double* SL, const double* SG, double* Gamma);
 // Blocks for an elementwise grid-stride loop.

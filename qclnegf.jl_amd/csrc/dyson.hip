@@ -17,6 +17,7 @@
 // Keldysh: with G^R in LDS, T = G^R S and G = T G^R^dagger are formed by the
 // same 8x8 lane grid, each lane accumulating its B x B block.
 #include <climits>
+#include <cstddef>
 
 #include "dk_common.hpp"
 
@@ -1066,6 +1067,9 @@ int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
     rc = launch_check(ctx, "current_green");
     if (rc) return rc;
     if (out_host) {
+        bool deferred = false;
+        rc = defer_coefficients(ctx, ctx->coef, &deferred);
+        if (rc || deferred) return rc;
         QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, ctx->coef, sizeof(PairCoefficients),
                                            hipMemcpyDeviceToHost, ctx->stream));
         rc = qclhip_ctx_synchronize(ctx);
@@ -1092,6 +1096,10 @@ int qclhip_number_functional(qclhip_ctx* ctx, const qclhip_grid* grid, const dou
                        ctx->status);
     int rc = launch_check(ctx, "number_functional");
     if (rc) return rc;
+    bool deferred = false;
+    rc = defer_result(ctx, (const double*)((const char*)ctx->coef + offsetof(PairCoefficients, number)), 1,
+                      &deferred);
+    if (rc || deferred) return rc;
     QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, ctx->coef, sizeof(PairCoefficients),
                                        hipMemcpyDeviceToHost, ctx->stream));
     rc = qclhip_ctx_synchronize(ctx);

@@ -223,6 +223,9 @@ int qclhip_boundary_flux(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
                        terms, grid->gs, res);
     int rc = launch_check(ctx, "boundary_flux");
     if (rc) return rc;
+    bool deferred = false;
+    rc = defer_result(ctx, res, 1, &deferred);
+    if (rc || deferred) return rc;
     QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     rc = qclhip_ctx_synchronize(ctx);
     if (rc) return rc;
@@ -240,6 +243,9 @@ int qclhip_state_populations(qclhip_ctx* ctx, const qclhip_grid* grid, const dou
                        grid->wE, grid->wk, grid->gs, target, (const c64*)GL, res);
     int rc = launch_check(ctx, "k_populations");
     if (rc) return rc;
+    bool deferred = false;
+    rc = defer_result(ctx, res, grid->Nb, &deferred);
+    if (rc || deferred) return rc;
     QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(out_host, res, (size_t)grid->Nb * sizeof(double), hipMemcpyDeviceToHost,
                                        ctx->stream));
     return qclhip_ctx_synchronize(ctx);

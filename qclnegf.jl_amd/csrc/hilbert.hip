@@ -440,6 +440,9 @@ int hilbert_run(qclhip_ctx* ctx, const qclhip_hilbert* p, const double* Gamma, c
     hipLaunchKernelGGL(k_witness_reduce, dim3(1), dim3(1024), 0, ctx->stream, partial, p->ncols, res);
     rc = launch_check(ctx, "k_witness_reduce");
     if (rc) return rc;
+    bool deferred = false;
+    rc = defer_result(ctx, res, 1, &deferred);
+    if (rc || deferred) return rc;
     QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     rc = qclhip_ctx_synchronize(ctx);
     if (rc) return rc;

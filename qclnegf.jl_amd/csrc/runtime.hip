@@ -20,6 +20,7 @@ int hip_fail(qclhip_ctx* ctx, hipError_t err, const char* what) {
 
 void* workspace(qclhip_ctx* ctx, size_t bytes) {
     if (bytes <= ctx->workspace_bytes) return ctx->workspace;
+    if (ctx->capturing) return nullptr;  // no allocation under graph capture
     if (ctx->workspace) {
         // Work already enqueued may still read the old scratch.
         if (hipStreamSynchronize(ctx->stream) != hipSuccess) return nullptr;
@@ -43,6 +44,40 @@ int persistent_grid(qclhip_ctx* ctx, const void* kernel, int threads, size_t lds
     g = std::max<long long>(8, g / 8 * 8);
     *grid = (unsigned)std::max<long long>(1, std::min(g, ntiles));
     return QCLHIP_OK;
+}
+
+__global__ void k_copy_doubles(const double* src, int n, double* dst) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+__global__ void k_copy_coefficients(const PairCoefficients* c, double* dst) {
+    if (threadIdx.x == 0) {
+        dst[0] = c->lambda;
+        dst[1] = c->number;
+        dst[2] = c->holes;
+        dst[3] = c->a;
+        dst[4] = c->c;
+    }
+}
+
+int defer_result(qclhip_ctx* ctx, const double* src, int n, bool* deferred) {
+    *deferred = false;
+    if (!ctx->defer_dev) return QCLHIP_OK;
+    if (ctx->defer_next + n > ctx->defer_cap) return fail(ctx, QCLHIP_ERR_ARGUMENT, "deferred-result buffer is full");
+    hipLaunchKernelGGL(k_copy_doubles, dim3(1), dim3(256), 0, ctx->stream, src, n, ctx->defer_dev + ctx->defer_next);
+    ctx->defer_next += n;
+    *deferred = true;
+    return launch_check(ctx, "k_copy_doubles");
+}
+
+int defer_coefficients(qclhip_ctx* ctx, const PairCoefficients* src, bool* deferred) {
+    *deferred = false;
+    if (!ctx->defer_dev) return QCLHIP_OK;
+    if (ctx->defer_next + 5 > ctx->defer_cap) return fail(ctx, QCLHIP_ERR_ARGUMENT, "deferred-result buffer is full");
+    hipLaunchKernelGGL(k_copy_coefficients, dim3(1), dim3(64), 0, ctx->stream, src, ctx->defer_dev + ctx->defer_next);
+    ctx->defer_next += 5;
+    *deferred = true;
+    return launch_check(ctx, "k_copy_coefficients");
 }
 
 int launch_check(qclhip_ctx* ctx, const char* what) {
@@ -116,8 +151,73 @@ int qclhip_ctx_destroy(qclhip_ctx* ctx) {
 
 // The latched device status is read back and cleared here. Singular blocks
 // win over other codes, like the reference's SingularException.
+int qclhip_ctx_set_stream(qclhip_ctx* ctx, void* stream) {
+    if (!ctx) return QCLHIP_ERR_ARGUMENT;
+    if (ctx->capturing) return fail(ctx, QCLHIP_ERR_ARGUMENT, "cannot change the stream during graph capture");
+    ctx->stream = (hipStream_t)stream;
+    if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
+        return fail(ctx, QCLHIP_ERR_RUNTIME, "rocblas_set_stream failed");
+    return QCLHIP_OK;
+}
+
+int qclhip_ctx_defer_results(qclhip_ctx* ctx, double* dev_buffer, int capacity) {
+    if (!ctx || (dev_buffer && capacity <= 0)) return QCLHIP_ERR_ARGUMENT;
+    ctx->defer_dev = dev_buffer;
+    ctx->defer_cap = dev_buffer ? capacity : 0;
+    ctx->defer_next = 0;
+    return QCLHIP_OK;
+}
+
+int qclhip_ctx_result_slot(const qclhip_ctx* ctx) { return ctx ? ctx->defer_next : QCLHIP_ERR_ARGUMENT; }
+
+struct qclhip_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int qclhip_graph_begin(qclhip_ctx* ctx) {
+    if (!ctx) return QCLHIP_ERR_ARGUMENT;
+    if (!ctx->stream) return fail(ctx, QCLHIP_ERR_ARGUMENT, "the default stream cannot be captured; set a stream");
+    if (ctx->capturing) return fail(ctx, QCLHIP_ERR_ARGUMENT, "capture already in progress");
+    QCLHIP_TRY_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    return QCLHIP_OK;
+}
+
+int qclhip_graph_end(qclhip_ctx* ctx, qclhip_graph** out) {
+    if (!ctx || !out) return QCLHIP_ERR_ARGUMENT;
+    *out = nullptr;
+    if (!ctx->capturing) return fail(ctx, QCLHIP_ERR_ARGUMENT, "no capture in progress");
+    ctx->capturing = false;
+    auto* g = new qclhip_graph();
+    hipError_t err = hipStreamEndCapture(ctx->stream, &g->graph);
+    if (err == hipSuccess) err = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+        if (g->graph) hipGraphDestroy(g->graph);
+        delete g;
+        return hip_fail(ctx, err, "graph capture");
+    }
+    *out = g;
+    return QCLHIP_OK;
+}
+
+int qclhip_graph_launch(qclhip_ctx* ctx, const qclhip_graph* g) {
+    if (!ctx || !g) return QCLHIP_ERR_ARGUMENT;
+    QCLHIP_TRY_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    return QCLHIP_OK;
+}
+
+int qclhip_graph_destroy(qclhip_graph* g) {
+    if (!g) return QCLHIP_OK;
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return QCLHIP_OK;
+}
+
 int qclhip_ctx_synchronize(qclhip_ctx* ctx) {
     if (!ctx) return QCLHIP_ERR_ARGUMENT;
+    if (ctx->capturing) return fail(ctx, QCLHIP_ERR_ARGUMENT, "synchronizing entry called during graph capture");
     hipError_t err = hipMemcpyAsync(ctx->status_host, ctx->status, sizeof(DeviceStatus),
                                     hipMemcpyDeviceToHost, ctx->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);

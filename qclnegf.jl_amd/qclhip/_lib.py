@@ -59,6 +59,13 @@ _SIGNATURES = {
     "qclhip_ctx_destroy": [_vp],
     "qclhip_ctx_synchronize": [_vp],
     "qclhip_ctx_last_error": [_vp],
+    "qclhip_ctx_set_stream": [_vp, _vp],
+    "qclhip_ctx_defer_results": [_vp, _vp, ct.c_int],
+    "qclhip_ctx_result_slot": [_vp],
+    "qclhip_graph_begin": [_vp],
+    "qclhip_graph_end": [_vp, ct.POINTER(_vp)],
+    "qclhip_graph_launch": [_vp, _vp],
+    "qclhip_graph_destroy": [_vp],
     "qclhip_max_basis": [],
     "qclhip_grid_create": [_vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, ct.POINTER(_vp)],
     "qclhip_grid_destroy": [_vp],

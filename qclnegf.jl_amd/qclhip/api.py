@@ -69,6 +69,59 @@ class Context:
         rc = getattr(self.lib, name)(self.handle, *args)
         check(rc, self.handle, name)
 
+    # -- streams, deferred results and graphs --------------------------------
+    deferring = False
+
+    def set_stream(self, stream):
+        """Enqueue on `stream` (a torch.cuda.Stream) from now on."""
+        self.call("qclhip_ctx_set_stream", ct.c_void_p(stream.cuda_stream))
+
+    def defer(self, buffer):
+        """Deferred-results mode into the device float64 tensor `buffer`
+        (None ends it): scalar-returning calls return Deferred slots."""
+        self.call("qclhip_ctx_defer_results", _ptr(buffer), 0 if buffer is None else buffer.numel())
+        self.deferring = buffer is not None
+
+    def result_slot(self) -> int:
+        return int(self.lib.qclhip_ctx_result_slot(self.handle))
+
+    def graph_begin(self):
+        self.call("qclhip_graph_begin")
+
+    def graph_end(self) -> "Graph":
+        h = _vp()
+        self.call("qclhip_graph_end", ct.byref(h))
+        return Graph(self, h)
+
+
+class Graph:
+    """An executable HIP graph captured from a context's stream."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx, self.handle = ctx, handle
+
+    def launch(self):
+        self.ctx.call("qclhip_graph_launch", self.handle)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.ctx.lib.qclhip_graph_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+@dataclass
+class Deferred:
+    """Where a deferred result will land in the context's result buffer."""
+    slot: int
+    n: int
+
+    def read(self, host_buffer):
+        v = host_buffer[self.slot:self.slot + self.n]
+        return float(v[0]) if self.n == 1 else np.array(v)
+
 
 _default = {}
 
@@ -337,10 +390,13 @@ def current_green_production(grid: Grid, h, families, target: float, state: Gree
                            torch.empty((grid.Nk, grid.NE), dtype=torch.float64, device=grid.ctx.device))
     arr = lambda k: (_vp * nf)(*[f[k].data_ptr() for f in families])
     out = (ct.c_double * 5)()
+    slot0 = grid.ctx.result_slot() if grid.ctx.deferring else 0
     grid.ctx.call("qclhip_current_green", grid.handle, _ptr(h), nf, arr(0), arr(1), arr(2),
                   ct.c_double(target), _ptr(state.GR), _ptr(state.GL), _ptr(state.GG),
                   _ptr(state.A if spectral else None), _ptr(state.scale),
                   out if return_scalars else None)
+    if return_scalars and grid.ctx.deferring:
+        return state, Deferred(slot0, 5)
     info = dict(lam=out[0], number=out[1], holes=out[2], a=out[3], c=out[4]) if return_scalars else None
     return state, info
 
@@ -348,8 +404,9 @@ def current_green_production(grid: Grid, h, families, target: float, state: Gree
 def number_functional(grid: Grid, G) -> float:
     """_number_functional_production (production_operators.jl:1417-1445)."""
     v = ct.c_double()
+    slot0 = grid.ctx.result_slot() if grid.ctx.deferring else 0
     grid.ctx.call("qclhip_number_functional", grid.handle, _ptr(G), ct.byref(v))
-    return v.value
+    return Deferred(slot0, 1) if grid.ctx.deferring else v.value
 
 
 def spectral_function(grid: Grid, GR, A=None):
@@ -409,9 +466,13 @@ def retarded_self_energy_fft(plan: FFTHilbertPlan, SL, SG, return_roundoff: bool
     Returns (SR, Gamma[, roundoff]); Gamma is only materialized when passed."""
     SR = torch.empty_like(SL) if SR is None else SR
     r = ct.c_double(0.0)
-    plan.grid.ctx.call("qclhip_retarded_self_energy", plan.handle, _ptr(SL), _ptr(SG), _ptr(SR),
-                       _ptr(Gamma), ct.byref(r) if return_roundoff else None)
-    return (SR, Gamma, r.value) if return_roundoff else (SR, Gamma)
+    ctx = plan.grid.ctx
+    slot0 = ctx.result_slot() if ctx.deferring else 0
+    ctx.call("qclhip_retarded_self_energy", plan.handle, _ptr(SL), _ptr(SG), _ptr(SR),
+             _ptr(Gamma), ct.byref(r) if return_roundoff else None)
+    if return_roundoff:
+        return SR, Gamma, (Deferred(slot0, 1) if ctx.deferring else r.value)
+    return SR, Gamma
 
 
 def embedding_component_production(grid: Grid, T, X, plan: EnergyShiftPlan, S=None):
@@ -449,17 +510,19 @@ def mix_fields(pairs, alpha: float):
 def boundary_flux_bar(grid: Grid, plusL, plusG, GL, GG) -> float:
     """_production_boundary_flux_bar (production_iteration.jl:189-201)."""
     v = ct.c_double()
+    slot0 = grid.ctx.result_slot() if grid.ctx.deferring else 0
     grid.ctx.call("qclhip_boundary_flux", grid.handle, _ptr(plusL), _ptr(plusG), _ptr(GL), _ptr(GG),
                   ct.byref(v))
-    return v.value
+    return Deferred(slot0, 1) if grid.ctx.deferring else v.value
 
 
 def state_populations(grid: Grid, GL, target: float):
     """_production_state_populations (production_iteration.jl:204-215)."""
     out = np.zeros(grid.Nb)
+    slot0 = grid.ctx.result_slot() if grid.ctx.deferring else 0
     grid.ctx.call("qclhip_state_populations", grid.handle, _ptr(GL), ct.c_double(target),
                   out.ctypes.data_as(_dp))
-    return out
+    return Deferred(slot0, grid.Nb) if grid.ctx.deferring else out
 
 
 def seed_number_weights(grid: Grid, GR):

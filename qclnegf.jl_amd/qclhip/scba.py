@@ -83,8 +83,10 @@ class ScbaState:
 
 
 def _candidate_scattering(p: ScbaProblem, green: api.GreenState, out: dict):
-    """_scattering_candidate_production (production_operators.jl:1258-1335)."""
-    roundoff = 0.0
+    """_scattering_candidate_production (production_operators.jl:1258-1335).
+    Returns the largest round-off witness (a list of Deferred slots in
+    deferred-results mode)."""
+    roundoff = [] if p.grid.ctx.deferring else 0.0
     for mech in p.mechanisms:
         fam = out[mech.name]
         if mech.is_lo:
@@ -95,7 +97,7 @@ def _candidate_scattering(p: ScbaProblem, green: api.GreenState, out: dict):
             api.production_static_contraction(mech.kernel, green.GG, mech.qK, S=fam.G)
         if p.verify_roundoff:
             _, _, r = api.retarded_self_energy_fft(p.hilbert, fam.L, fam.G, return_roundoff=True, SR=fam.R)
-            roundoff = max(roundoff, r)
+            roundoff = roundoff + [r] if isinstance(roundoff, list) else max(roundoff, r)
         else:
             api.retarded_self_energy_fft(p.hilbert, fam.L, fam.G, SR=fam.R)
     return roundoff
@@ -151,6 +153,9 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     for fam, cand in ((s.plus, ws.plus), (s.minus, ws.minus)):
         pairs += list(zip(fam.parts(), cand.parts()))
     api.mix_fields(pairs, p.alpha_mix)
+    if g.ctx.deferring:  # captured or deferred: results are slots of the result buffer
+        row = dict(info=info, Jbar=Jbar, roundoff=roundoff, populations=pops)
+        return row
     g.ctx.synchronize()
     t5 = time.perf_counter()
     if timings is not None:
@@ -177,3 +182,54 @@ def solve_scba_production(p: ScbaProblem, s: ScbaState, max_iterations: int, tol
                 break
         prev = row["populations"]
     return s
+
+
+class IterationGraph:
+    """One production iteration captured as a HIP graph and replayed: for the
+    small, launch-bound problems (tens of kernels of a few microseconds per
+    iteration). Construction runs one eager iteration (it sizes the scratch
+    and allocates the Green state), then captures the next iteration on a
+    side stream in deferred-results mode; capture itself runs nothing.
+    replay() runs one iteration and returns its row."""
+
+    def __init__(self, p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace):
+        self.p, self.s, self.ws = p, s, ws
+        ctx = p.grid.ctx
+        scba_iteration(p, s, ws)
+        nres = 5 + 1 + p.grid.Nb + len(p.mechanisms) + 8
+        self.buffer = torch.zeros(nres, dtype=torch.float64, device=ctx.device)
+        self.stream = torch.cuda.Stream(ctx.device)
+        torch.cuda.synchronize()
+        base = torch.cuda.current_stream(ctx.device)
+        with torch.cuda.stream(self.stream):
+            ctx.set_stream(self.stream)
+            try:
+                ctx.defer(self.buffer)
+                ctx.graph_begin()
+                try:
+                    self.slots = scba_iteration(p, s, ws)
+                finally:
+                    self.graph = ctx.graph_end()
+                    ctx.defer(None)
+            finally:
+                ctx.set_stream(base)
+
+    def replay(self):
+        ctx = self.p.grid.ctx
+        base = torch.cuda.current_stream(ctx.device)
+        self.stream.wait_stream(base)
+        ctx.set_stream(self.stream)
+        try:
+            self.graph.launch()
+            ctx.synchronize()
+        finally:
+            ctx.set_stream(base)
+        base.wait_stream(self.stream)
+        host = self.buffer.cpu().numpy()
+        info = self.slots["info"].read(host)
+        ro = self.slots["roundoff"]
+        row = dict(lam=info[0], number=info[1], holes=info[2], Jbar=self.slots["Jbar"].read(host),
+                   roundoff=max([r.read(host) for r in ro]) if isinstance(ro, list) and ro else 0.0,
+                   populations=self.slots["populations"].read(host))
+        self.s.history.append(row)
+        return row
