@@ -117,12 +117,14 @@ def gf_bench(steps, warmup, variants=VARIANTS):
                          fp64_fallback_fraction=fallback)
         del st, g
         torch.cuda.empty_cache()
+    tune = api.Grid(w["eps"], w["wE"], w["wk"], N).autotune()
+    log(f"  autotune: {tune}")
     ref = out["fp64_vector"]["J"] if "fp64_vector" in out else None
     for v in out.values():
         v["current_rel_err"] = abs(v["J"] - ref) / abs(ref) if ref else None
     flops = nblk * 40.0 * N ** 3          # 8N^3 inverse + 2 x 16N^3 Keldysh
     bytes_ = nblk * 16.0 * N * N * 6      # read S^R, S<, S>; write G^R, G<, G>
-    return dict(N=N, NE=NE, Nk=Nk, nblk=nblk, flops=flops, bytes=bytes_, variants=out)
+    return dict(N=N, NE=NE, Nk=Nk, nblk=nblk, flops=flops, bytes=bytes_, variants=out, autotune=tune)
 
 
 def scba_problem(N=24, NE=2048, Nk=64):
@@ -401,6 +403,7 @@ def main():
                      "hbm_frac": gf["bytes"] / k_s / 1e9 / PEAK_HBM_GBS},
         "number_check": best["number_check"],
         "variants": {k: {kk: vv for kk, vv in v.items() if kk != "precision"} for k, v in gf["variants"].items()},
+        "autotune": gf["autotune"],
     }
     if sc is not None:
         line["scba_iter_per_s"] = 1.0 / sc["iter_s"]

@@ -119,6 +119,12 @@ int qclhip_grid_set_precision(qclhip_ctx* ctx, qclhip_grid* grid, int precision,
 #define QCLHIP_ENGINE_VECTOR 0
 #define QCLHIP_ENGINE_MATRIX 1
 int qclhip_grid_set_engine(qclhip_ctx* ctx, qclhip_grid* grid, int keldysh_engine);
+/* Autotuner: times both FP64 Keldysh engines on synthetic blocks of the
+ * grid's shape (a sub-grid of at most 512 energies) and selects the faster
+ * for the grid; the choice is cached in the context per (Nb, NE, Nk).
+ * ms_host (may be NULL) receives the two times (vector, matrix) in ms.
+ * Allocates temporary device memory; not during graph capture. */
+int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host);
 /* Blocks of the grid's last mixed-precision Dyson call that the guard sent
  * to the FP64 re-solve (0 for an FP64 grid). Synchronizes. */
 int qclhip_grid_refine_fallbacks(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t* count_host);

@@ -1,0 +1,139 @@
+// Autotuner of the GF-solve launch configuration. For a grid shape it times
+// the FP64 Keldysh engines (vector ALUs, matrix cores) on synthetic blocks of
+// that shape and keeps the faster one for the grid, caching the choice per
+// (Nb, NE, Nk) in the context. Timing uses a sub-grid of at most 512 energies
+// at the grid's Nk and Nb: the kernels' throughput per block depends on the
+// block size and occupancy, not on the number of tiles once the device is
+// full.
+#include <algorithm>
+#include <cmath>
+
+#include "dk_common.hpp"
+
+namespace qclhip {
+
+// Synthetic, well-conditioned inputs: h diagonal, Sigma^R = -i(0.05 + u) I,
+// Sigma< = i f g I, Sigma> = -i (1-f) g I, with u, f, g from a hash.
+__device__ inline double hash01(unsigned long long x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void k_tune_fields(long long nblk, int N, c64* SR, c64* SL, c64* SG) {
+    const long long n = nblk * N * N;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long blk = i % nblk;
+        const long long ab = i / nblk;
+        const bool diag = (ab % N) == (ab / N);
+        const double u = hash01(2 * i + 1), f = hash01(2 * i + 2);
+        SR[i] = diag ? cmk(0.0, -(0.05 + 0.1 * u)) : cmk(1e-3 * (u - 0.5), 0.0);
+        SL[i] = diag ? cmk(0.0, f * 0.1) : cmk(0.0, 0.0);
+        SG[i] = diag ? cmk(0.0, -(1.0 - f) * 0.1) : cmk(0.0, 0.0);
+        (void)blk;
+    }
+}
+
+__global__ void k_tune_h(int Nk, int N, c64* h) {
+    const int n = Nk * N * N;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int ab = i / Nk;
+        h[i] = ((ab % N) == (ab / N)) ? cmk(0.2 * (ab % N) / N, 0.0) : cmk(0.0, 0.0);
+    }
+}
+
+}  // namespace qclhip
+
+using namespace qclhip;
+
+extern "C" {
+
+int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host) {
+    if (!ctx || !grid) return QCLHIP_ERR_ARGUMENT;
+    if (ctx->capturing) return fail(ctx, QCLHIP_ERR_ARGUMENT, "autotuning is not possible during graph capture");
+    for (const auto& t : ctx->tune_cache)
+        if (t.Nb == grid->Nb && t.NE == grid->NE && t.Nk == grid->Nk) {
+            grid->keldysh_engine = t.engine;
+            if (ms_host) {
+                ms_host[0] = t.ms_vector;
+                ms_host[1] = t.ms_matrix;
+            }
+            return QCLHIP_OK;
+        }
+    const int NE = std::min(grid->NE, 512), Nk = grid->Nk, N = grid->Nb;
+    const long long nblk = (long long)NE * Nk, n = nblk * N * N;
+    c64 *h = nullptr, *f = nullptr;
+    double* eps = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (h) hipFree(h);
+        if (f) hipFree(f);
+        if (eps) hipFree(eps);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+    };
+    if (hipMalloc(&h, (size_t)Nk * N * N * sizeof(c64)) != hipSuccess ||
+        hipMalloc(&f, (size_t)6 * n * sizeof(c64)) != hipSuccess || hipMalloc(&eps, NE * sizeof(double)) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        cleanup();
+        return fail(ctx, QCLHIP_ERR_RUNTIME, "autotune allocation failed");
+    }
+    // eps: the first NE energies of the grid.
+    if (hipMemcpyAsync(eps, grid->eps, NE * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+        cleanup();
+        return fail(ctx, QCLHIP_ERR_RUNTIME, "autotune copy failed");
+    }
+    hipLaunchKernelGGL(k_tune_h, dim3(64), dim3(256), 0, ctx->stream, Nk, N, h);
+    hipLaunchKernelGGL(k_tune_fields, dim3(grid_blocks(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, nblk, N, f,
+                       f + n, f + 2 * n);
+    int rc = launch_check(ctx, "autotune fields");
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    DKArgs a{};
+    a.NE = NE;
+    a.Nk = Nk;
+    a.N = N;
+    a.eps = eps;
+    a.h = h;
+    a.famR[0] = f;
+    a.famL[0] = f + n;
+    a.famG[0] = f + 2 * n;
+    a.nfamR = a.nfamL = a.nfamG = 1;
+    a.GR = f + 3 * n;
+    a.GL = f + 4 * n;
+    a.GG = f + 5 * n;
+    a.status = ctx->status;
+    a.keldysh = 1;
+    rc = launch_dyson_fp64(ctx, a);
+    float ms[2] = {0.f, 0.f};
+    for (int engine = 0; engine < 2 && !rc; ++engine) {
+        rc = engine ? launch_keldysh_mfma(ctx, a, false) : launch_keldysh_valu(ctx, a);  // warm-up
+        if (rc) break;
+        const int reps = 3;
+        hipEventRecord(e0, ctx->stream);
+        for (int r = 0; r < reps && !rc; ++r)
+            rc = engine ? launch_keldysh_mfma(ctx, a, false) : launch_keldysh_valu(ctx, a);
+        hipEventRecord(e1, ctx->stream);
+        if (!rc && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms[engine], e0, e1);
+        ms[engine] /= reps;
+    }
+    if (!rc) rc = qclhip_ctx_synchronize(ctx);
+    cleanup();
+    if (rc) return rc;
+    const int engine = ms[1] < ms[0] ? QCLHIP_ENGINE_MATRIX : QCLHIP_ENGINE_VECTOR;
+    grid->keldysh_engine = engine;
+    ctx->tune_cache.push_back({grid->Nb, grid->NE, grid->Nk, engine, ms[0], ms[1]});
+    if (ms_host) {
+        ms_host[0] = ms[0];
+        ms_host[1] = ms[1];
+    }
+    return QCLHIP_OK;
+}
+
+}  // extern "C"

@@ -113,6 +113,13 @@ struct qclhip_ctx {
     int defer_cap = 0;
     int defer_next = 0;
     bool capturing = false;  // inside qclhip_graph_begin/_end
+    // Autotune cache: (Nb, NE, Nk) -> chosen Keldysh engine and the two
+    // measured times (ms).
+    struct TuneEntry {
+        int Nb, NE, Nk, engine;
+        float ms_vector, ms_matrix;
+    };
+    std::vector<TuneEntry> tune_cache;
 };
 
 struct qclhip_grid {

@@ -72,6 +72,7 @@ _SIGNATURES = {
     "qclhip_grid_set_precision": [_vp, _vp, ct.c_int, ct.c_int, ct.c_double],
     "qclhip_grid_set_engine": [_vp, _vp, ct.c_int],
     "qclhip_grid_refine_fallbacks": [_vp, _vp, ct.POINTER(ct.c_int64)],
+    "qclhip_grid_autotune": [_vp, _vp, _dp],
     "qclhip_retarded_green": [_vp, _vp, _vp, _vp, ct.c_double, _vp, _vp],
     "qclhip_dyson_keldysh": [_vp, _vp, _vp, _vp, _vp, _vp, ct.c_double, _vp, _vp, _vp, _vp, _vp],
     "qclhip_current_green": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),

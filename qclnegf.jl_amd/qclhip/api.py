@@ -228,6 +228,14 @@ class Grid:
         self.engine = engine
         return self
 
+    def autotune(self):
+        """Pick the faster FP64 Keldysh engine for this grid's shape (cached
+        per shape in the context); returns the two times (ms)."""
+        ms = (ct.c_double * 2)()
+        self.ctx.call("qclhip_grid_autotune", self.handle, ms)
+        self.engine = "matrix" if ms[1] < ms[0] else "vector"
+        return dict(vector_ms=ms[0], matrix_ms=ms[1], engine=self.engine)
+
     def refine_fallbacks(self) -> int:
         """Blocks of the last mixed-precision Dyson call re-solved in FP64."""
         n = ct.c_int64()

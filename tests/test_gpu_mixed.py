@@ -181,3 +181,17 @@ def test_mixed_current_error_driven(hip, orc):
     del st32
     Jp, stp, infop = _driven_current(hip, w, inj, "fp32_products")
     assert abs(Jp - J64) <= 1e-6 * abs(J64)
+
+
+def test_autotune_selects_engine_and_keeps_parity(hip, orc):
+    w, SR, SL, SG = _dense_case(24, NE=64, Nk=4)
+    g = hip.Grid(w["eps"], w["wE"], w["wk"], 24)
+    t = g.autotune()
+    assert t["vector_ms"] > 0 and t["matrix_ms"] > 0 and t["engine"] in ("vector", "matrix")
+    g2 = hip.Grid(w["eps"], w["wE"], w["wk"], 24)
+    assert g2.autotune() == t  # cached per shape in the context
+    GR_o, _ = orc.retarded_green(w["eps"], w["h"], SR)
+    GL_o = orc.keldysh(GR_o, SL)
+    GR, GL, GG, tr = _run_dk(hip, g, w, SR, SL, SG)
+    err = float(rel(GL, GL_o))
+    assert err < TOL
