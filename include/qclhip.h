@@ -329,6 +329,25 @@ int qclhip_antihermitian_part(qclhip_ctx* ctx, const qclhip_grid* grid, double* 
 int qclhip_embedding_adjoint(qclhip_ctx* ctx, const qclhip_grid* grid, const double* T,
                              const qclhip_shift* shift, const double* Sbar, double* Xbar);
 
+/* production_residual_suite (src/numerics/optimized/residuals.jl:537-772):
+ * out_host = r_D, r_A, r_K, r_PSD, r_caus of one SCBA iteration, from the
+ * Green functions (GR, GL, GG, A), h (Nk,Nb,Nb), the families of the total
+ * self-energy (summed in list order) and the unmixed candidate S< (families
+ * summed likewise). One wavefront per (e,m) block; Hermitian eigenvalues by a
+ * parallel-ordered Jacobi method; per-block values reduced in a fixed order
+ * (sums for r_A and r_K, maxima otherwise; a non-finite value gives Inf).
+ * Deferrable (5 doubles). */
+int qclhip_residual_suite(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,
+                          const double* GL, const double* GG, const double* A, int nfam,
+                          const double* const* famR, const double* const* famL, const double* const* famG,
+                          int ncand, const double* const* candL, double* out_host);
+/* production_selfenergy_residual (src/numerics/optimized/residuals.jl:
+ * 802-865): max over (R, <, >) of ||cand - old||_F / (||cand||_F + 1e-14)
+ * over n complex entries each; Inf when any entry is non-finite.
+ * Deferrable (1 double). */
+int qclhip_selfenergy_residual(qclhip_ctx* ctx, int64_t n, const double* const* old_, const double* const* cand,
+                               double* out_host);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,10 +57,9 @@ class State(ct.Structure):
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with the committed Makefile."""
-    if force or not os.path.exists(_LIB_PATH):
-        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
-                       stdout=subprocess.DEVNULL)
+    """Compile liboracle.so with the committed Makefile (make rebuilds it
+    when a source is newer than the library)."""
+    subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
     return _LIB_PATH
 
 
@@ -397,3 +396,29 @@ class ScbaOracle:
         _check(rc, "scba_iteration")
         return dict(lam=out[0], Jbar=out[1], roundoff=out[2], number=out[3], holes=out[4],
                     populations=pops)
+
+def hermitian_eigvals(H):
+    H = np.asfortranarray(H, dtype=np.complex128)
+    n = H.shape[0]
+    w = np.zeros(n)
+    lib().orc_hermitian_eigvals(n, P(H), P(w))
+    return w
+
+
+def residual_suite(eps, h, GR, GL, GG, A, SR, SL, SG, cand_SL):
+    """production_residual_suite: dict r_D, r_A, r_K, r_PSD, r_caus."""
+    NE, Nk, Nb, _ = GR.shape
+    out = np.zeros(5)
+    arrs = [np.asfortranarray(x, dtype=np.complex128) for x in (h, GR, GL, GG, A, SR, SL, SG, cand_SL)]
+    rc = lib().orc_residual_suite(NE, Nk, Nb, P(np.asarray(eps, dtype=np.float64)), *[P(x) for x in arrs], P(out))
+    _check(rc, "residual_suite")
+    return dict(zip(("r_D", "r_A", "r_K", "r_PSD", "r_caus"), out))
+
+
+def selfenergy_residual(old, cand):
+    """production_selfenergy_residual; old, cand = (SR, SL, SG)."""
+    o = [np.asfortranarray(x, dtype=np.complex128) for x in old]
+    c = [np.asfortranarray(x, dtype=np.complex128) for x in cand]
+    arr = lambda xs: (_dp * 3)(*[P(x) for x in xs])
+    lib().orc_selfenergy_residual.restype = ct.c_double
+    return lib().orc_selfenergy_residual(ct.c_longlong(o[0].size), arr(o), arr(c))

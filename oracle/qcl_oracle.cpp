@@ -1036,3 +1036,226 @@ int orc_scba_iteration(const orc_problem* p, orc_state* s, double* out, double* 
 }
 
 }  // extern "C"
+
+namespace {
+// Eigenvalues of a Hermitian n x n block (column-major) by cyclic two-sided
+// complex Jacobi rotations, ascending. Each rotation J = diag(1, u*) R with
+// u = H_pq/|H_pq| and R the real symmetric Schur rotation of
+// [[H_pp, |H_pq|], [|H_pq|, H_qq]] (Golub-Van Loan 8.5.2); H <- J^H H J.
+void herm_eigvals(int n, const cd* Hin, double* w) {
+    std::vector<cd> H((size_t)n * n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) H[i + n * j] = 0.5 * (Hin[i + n * j] + std::conj(Hin[j + n * i]));
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const double v = std::norm(H[i + n * j]);
+                tot += v;
+                if (i != j) off += v;
+            }
+        if (off == 0.0 || off <= 1e-34 * tot) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const cd b = H[p + n * q];
+                const double ab = std::abs(b);
+                if (ab == 0.0) continue;
+                const double a = H[p + n * p].real(), d = H[q + n * q].real();
+                const double tau = (d - a) / (2.0 * ab);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                const cd u = b / ab;
+                for (int i = 0; i < n; ++i) {  // columns: H J
+                    const cd hp = H[i + n * p], hq = H[i + n * q];
+                    H[i + n * p] = c * hp - s * std::conj(u) * hq;
+                    H[i + n * q] = s * hp + c * std::conj(u) * hq;
+                }
+                for (int j = 0; j < n; ++j) {  // rows: J^H H
+                    const cd hp = H[p + n * j], hq = H[q + n * j];
+                    H[p + n * j] = c * hp - s * u * hq;
+                    H[q + n * j] = s * hp + c * u * hq;
+                }
+                H[p + n * q] = H[q + n * p] = cd(0.0, 0.0);
+                H[p + n * p] = cd(H[p + n * p].real(), 0.0);
+                H[q + n * q] = cd(H[q + n * q].real(), 0.0);
+            }
+    }
+    for (int i = 0; i < n; ++i) w[i] = H[i + n * i].real();
+    std::sort(w, w + n);
+}
+
+bool all_finite(int n, const cd* x) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(x[i].real()) || !std::isfinite(x[i].imag())) return false;
+    return true;
+}
+
+double frob2(int n2, const cd* x) {
+    double s = 0.0;
+    for (int i = 0; i < n2; ++i) s += std::norm(x[i]);
+    return s;
+}
+
+// Positivity ratio of the Hermitian part of f X (residuals.jl:413-452 with
+// the witness of positivity_diagnostics.jl:61-96): max(0, -lambda_min -
+// 64 n eps max(|lambda|_max, 1)) / max(|lambda|_max, floatmin).
+double negative_part(int n, const cd* X, cd f, cd* H, double* w) {
+    if (!all_finite(n * n, X)) return INFINITY;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) H[i + n * j] = 0.5 * (f * X[i + n * j] + std::conj(f * X[j + n * i]));
+    herm_eigvals(n, H, w);
+    double big = 0.0;
+    for (int i = 0; i < n; ++i) big = std::max(big, std::fabs(w[i]));
+    const double budget = 64.0 * n * 2.220446049250313e-16 * std::max(big, 1.0);
+    const double absolute = std::max(0.0, -w[0]);
+    return std::max(0.0, absolute - budget) / std::max(big, 2.2250738585072014e-308);
+}
+
+// Causality of one retarded block (residuals.jl:455-492): lambda_max of
+// Im S^R = (S - S^dagger)/(2i) over the spectral norm of S^R (+1e-14).
+double causality_block(int n, const cd* S, cd* H, cd* T, double* w) {
+    if (!all_finite(n * n, S)) return INFINITY;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) H[i + n * j] = (S[i + n * j] - std::conj(S[j + n * i])) / cd(0.0, 2.0);
+    herm_eigvals(n, H, w);
+    const double numerator = std::max(0.0, w[n - 1]);
+    if (numerator == 0.0) return 0.0;
+    // ||S||_2 = sqrt(lambda_max(S^dagger S))
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            cd v(0.0, 0.0);
+            for (int c = 0; c < n; ++c) v += std::conj(S[c + n * i]) * S[c + n * j];
+            T[i + n * j] = v;
+        }
+    herm_eigvals(n, T, w);
+    const double norm2 = std::sqrt(std::max(0.0, w[n - 1]));
+    return numerator / (norm2 + 1e-14);
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_hermitian_eigvals(int n, const double* H, double* w) { herm_eigvals(n, C(H), w); }
+
+/* production_residual_suite (residuals.jl:537-772): out = r_D, r_A, r_K,
+ * r_PSD, r_caus. Totals S* and the unmixed candidate S< of the iteration. */
+int orc_residual_suite(int NE, int Nk, int Nb, const double* eps, const double* h_, const double* GR_,
+                       const double* GL_, const double* GG_, const double* A_, const double* SR_,
+                       const double* SL_, const double* SG_, const double* cand_, double* out) {
+    const cd *h = C(h_), *GRf = C(GR_), *GLf = C(GL_), *GGf = C(GG_), *Af = C(A_);
+    const cd *SRf = C(SR_), *SLf = C(SL_), *SGf = C(SG_), *Cf = C(cand_);
+    const long long nblk = (long long)NE * Nk;
+    const int n = Nb, n2 = Nb * Nb;
+    std::vector<double> specN(nblk), specD(nblk), kelN(nblk), kelD(nblk), rDv(nblk), psdv(nblk), causv(nblk);
+#pragma omp parallel
+    {
+        std::vector<cd> G(n2), GL(n2), GG(n2), A(n2), SR(n2), SL(n2), SG(n2), CL(n2), D(n2), Gm(n2), T1(n2), T2(n2),
+            H(n2), T3(n2);
+        std::vector<double> w(n);
+#pragma omp for schedule(static)
+        for (long long linear = 0; linear < nblk; ++linear) {
+            const int e = (int)(linear / Nk), m = (int)(linear % Nk);  // the reference's order: m fastest
+            load_block(GRf, e, m, NE, Nk, Nb, G.data());
+            load_block(GLf, e, m, NE, Nk, Nb, GL.data());
+            load_block(GGf, e, m, NE, Nk, Nb, GG.data());
+            load_block(Af, e, m, NE, Nk, Nb, A.data());
+            load_block(SRf, e, m, NE, Nk, Nb, SR.data());
+            load_block(SLf, e, m, NE, Nk, Nb, SL.data());
+            load_block(SGf, e, m, NE, Nk, Nb, SG.data());
+            load_block(Cf, e, m, NE, Nk, Nb, CL.data());
+            // r_D = ||D G - I||_F / (||D||_F ||G||_F + sqrt(Nb)),  D = eps - h - S^R
+            for (int b = 0; b < n; ++b)
+                for (int a = 0; a < n; ++a)
+                    D[a + n * b] = (a == b ? cd(eps[e], 0.0) : cd(0.0, 0.0)) - h[(size_t)m + (size_t)Nk * (a + (size_t)n * b)] -
+                                   SR[a + n * b];
+            matmul(n, D.data(), G.data(), T1.data());
+            for (int a = 0; a < n; ++a) T1[a + n * a] -= 1.0;
+            {
+                const double num = std::sqrt(frob2(n2, T1.data()));
+                const double den = std::sqrt(frob2(n2, D.data())) * std::sqrt(frob2(n2, G.data())) + std::sqrt((double)n);
+                rDv[linear] = std::isfinite(num) && std::isfinite(den) ? num / den : INFINITY;
+            }
+            // Gamma = i (S> - S<)
+            for (int i = 0; i < n2; ++i) Gm[i] = cd(0.0, 1.0) * (SG[i] - SL[i]);
+            // r_A terms: ||A - G Gamma G^dagger||^2, ||A||^2
+            matmul(n, G.data(), Gm.data(), T1.data());
+            matmul_adj(n, T1.data(), G.data(), T2.data());
+            for (int i = 0; i < n2; ++i) T2[i] = A[i] - T2[i];
+            specN[linear] = frob2(n2, T2.data());
+            specD[linear] = frob2(n2, A.data());
+            // r_K terms against the unmixed candidate S<
+            matmul(n, G.data(), CL.data(), T1.data());
+            matmul_adj(n, T1.data(), G.data(), T2.data());
+            for (int i = 0; i < n2; ++i) T2[i] = GL[i] - T2[i];
+            kelN[linear] = frob2(n2, T2.data());
+            kelD[linear] = frob2(n2, GL.data());
+            // r_PSD over A, -iG<, iG>, Gamma, -i G S< G^dagger, i G S> G^dagger
+            double psd = 0.0;
+            psd = std::max(psd, negative_part(n, A.data(), cd(1.0, 0.0), H.data(), w.data()));
+            psd = std::max(psd, negative_part(n, GL.data(), cd(0.0, -1.0), H.data(), w.data()));
+            psd = std::max(psd, negative_part(n, GG.data(), cd(0.0, 1.0), H.data(), w.data()));
+            psd = std::max(psd, negative_part(n, Gm.data(), cd(1.0, 0.0), H.data(), w.data()));
+            matmul(n, G.data(), SL.data(), T1.data());
+            matmul_adj(n, T1.data(), G.data(), T2.data());
+            psd = std::max(psd, negative_part(n, T2.data(), cd(0.0, -1.0), H.data(), w.data()));
+            matmul(n, G.data(), SG.data(), T1.data());
+            matmul_adj(n, T1.data(), G.data(), T2.data());
+            psd = std::max(psd, negative_part(n, T2.data(), cd(0.0, 1.0), H.data(), w.data()));
+            psdv[linear] = psd;
+            causv[linear] = causality_block(n, SR.data(), H.data(), T3.data(), w.data());
+        }
+    }
+    // The reference's canonical reductions (residuals.jl:495-512): a
+    // non-finite entry makes the sum or maximum Inf.
+    auto csum = [](const std::vector<double>& v) {
+        double r = 0.0;
+        for (double x : v) {
+            if (!std::isfinite(x)) return (double)INFINITY;
+            r += x;
+            if (!std::isfinite(r)) return (double)INFINITY;
+        }
+        return r;
+    };
+    auto cmax = [](const std::vector<double>& v) {
+        double r = 0.0;
+        for (double x : v) {
+            if (!std::isfinite(x)) return (double)INFINITY;
+            r = std::max(r, x);
+        }
+        return r;
+    };
+    auto ratio = [](double num, double den) {
+        const double v = std::sqrt(num) / (std::sqrt(den) + 1e-14);
+        return std::isfinite(v) ? v : (double)INFINITY;
+    };
+    out[0] = cmax(rDv);
+    out[1] = ratio(csum(specN), csum(specD));
+    out[2] = ratio(csum(kelN), csum(kelD));
+    out[3] = cmax(psdv);
+    out[4] = cmax(causv);
+    return 0;
+}
+
+/* production_selfenergy_residual (residuals.jl:802-865): max over R, <, >
+ * of ||cand - old||_F / (||cand||_F + 1e-14). */
+double orc_selfenergy_residual(long long n, const double* const* old_, const double* const* cand_) {
+    double r = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const cd *o = C(old_[k]), *c = C(cand_[k]);
+        double dn = 0.0, cn = 0.0;
+        bool finite = true;
+        for (long long i = 0; i < n; ++i) {
+            if (!std::isfinite(c[i].real()) || !std::isfinite(c[i].imag()) || !std::isfinite(o[i].real()) ||
+                !std::isfinite(o[i].imag()))
+                finite = false;
+            dn += std::norm(c[i] - o[i]);
+            cn += std::norm(c[i]);
+        }
+        const double v = std::sqrt(dn) / (std::sqrt(cn) + 1e-14);
+        r = std::max(r, finite && std::isfinite(v) ? v : (double)INFINITY);
+    }
+    return r;
+}
+
+}  // extern "C"

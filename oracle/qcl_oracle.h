@@ -125,6 +125,17 @@ typedef struct {
 int orc_scba_iteration(const orc_problem* p, orc_state* s, double* out /* lambda, J,
                        roundoff, number, holes */, double* populations);
 
+/* Eigenvalues (ascending) of a Hermitian n x n block, cyclic Jacobi. */
+void orc_hermitian_eigvals(int n, const double* H, double* w);
+/* production_residual_suite (residuals.jl:537-772): out = r_D, r_A, r_K,
+ * r_PSD, r_caus. */
+int orc_residual_suite(int NE, int Nk, int Nb, const double* eps, const double* h, const double* GR,
+                       const double* GL, const double* GG, const double* A, const double* SR,
+                       const double* SL, const double* SG, const double* candSL, double* out);
+/* production_selfenergy_residual (residuals.jl:802-865); old and cand are
+ * arrays of the three component pointers (R, <, >) of n complex entries. */
+double orc_selfenergy_residual(long long n, const double* const* old_, const double* const* cand);
+
 int orc_num_threads(void);
 void orc_set_num_threads(int n);
 

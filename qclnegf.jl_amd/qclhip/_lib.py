@@ -108,6 +108,9 @@ _SIGNATURES = {
     "qclhip_retarded_self_energy_adjoint": [_vp, _vp, _vp, _vp, _vp],
     "qclhip_embedding_adjoint": [_vp, _vp, _vp, _vp, _vp, _vp],
     "qclhip_antihermitian_part": [_vp, _vp, _vp],
+    "qclhip_residual_suite": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),
+                              ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp), _dp],
+    "qclhip_selfenergy_residual": [_vp, ct.c_int64, ct.POINTER(_vp), ct.POINTER(_vp), _dp],
     "qclhip_batch_create": [_vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp)],
     "qclhip_batch_destroy": [_vp],
     "qclhip_batch_current_green": [_vp, _vp, ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),

@@ -652,6 +652,51 @@ def antihermitian_part(grid: Grid, X):
     return X
 
 
+def residual_suite(grid: Grid, h, green, families, candidate_lesser):
+    """production_residual_suite (residuals.jl:537-772) on the device.
+
+    green: a GreenState with A filled; families: (SR, SL, SG) triples of the
+    total self-energy; candidate_lesser: the unmixed candidate S< as a list of
+    fields summed in order. Returns dict r_D, r_A, r_K, r_PSD, r_caus, or a
+    Deferred of 5 values in deferred-results mode."""
+    if green.A is None:
+        raise _lib.ArgumentError("the residual suite needs the spectral function A")
+    for fam in families:
+        for t in fam:
+            _check_field(grid, t, "self-energy")
+    cands = list(candidate_lesser)
+    for t in cands:
+        _check_field(grid, t, "candidate")
+    nf = len(families)
+    arr = lambda k: (_vp * nf)(*[f[k].data_ptr() for f in families])
+    carr = (_vp * len(cands))(*[t.data_ptr() for t in cands])
+    out = (ct.c_double * 5)()
+    slot0 = grid.ctx.result_slot() if grid.ctx.deferring else 0
+    grid.ctx.call("qclhip_residual_suite", grid.handle, _ptr(h), _ptr(green.GR), _ptr(green.GL), _ptr(green.GG),
+                  _ptr(green.A), nf, arr(0), arr(1), arr(2), len(cands), carr, out)
+    if grid.ctx.deferring:
+        return Deferred(slot0, 5)
+    return dict(r_D=out[0], r_A=out[1], r_K=out[2], r_PSD=out[3], r_caus=out[4])
+
+
+def selfenergy_residual(old, candidate):
+    """production_selfenergy_residual (residuals.jl:802-865): old and
+    candidate are (SR, SL, SG) triples of equal shapes."""
+    old, candidate = list(old), list(candidate)
+    if len(old) != 3 or len(candidate) != 3:
+        raise _lib.ArgumentError("self-energy residual takes (R, <, >) triples")
+    n = old[0].numel()
+    for t in old + candidate:
+        if t.dtype != torch.complex128 or t.numel() != n or not t.is_contiguous() or t.device != old[0].device:
+            raise _lib.DimensionMismatch("self-energy components must be contiguous complex128 fields of one shape")
+    ctx = context(old[0].device.index or 0)
+    out = ct.c_double()
+    slot0 = ctx.result_slot() if ctx.deferring else 0
+    ctx.call("qclhip_selfenergy_residual", ct.c_int64(n), (_vp * 3)(*[t.data_ptr() for t in old]),
+             (_vp * 3)(*[t.data_ptr() for t in candidate]), ct.byref(out))
+    return Deferred(slot0, 1) if ctx.deferring else out.value
+
+
 def potential_gradient(hbar, Phi):
     """Host chain rule to a potential sampled on the real-space grid:
     dJ/dU_g = sum_m Re sum_ab conj(hbar_m[a,b]) conj(Phi[g,a]) Phi[g,b]

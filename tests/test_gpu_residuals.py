@@ -1,0 +1,209 @@
+"""Residual suite and self-energy residual on the device against the oracle
+(residuals.jl:537-865): random states whose residuals are O(1) (relative bar
+1e-10: eigenvalue-based terms carry the Jacobi method's rounding), closed
+forms (G = 0, exact inverses, known negative and causal-violating
+eigenvalues), non-finite blocks (Inf), every tile class N_b = 1..48, an SCBA
+state (near-zero residuals, absolute bar) and the deferred-results mode."""
+import numpy as np
+import pytest
+
+import scba_fixture as fx
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(float).eps
+KEYS = ("r_D", "r_A", "r_K", "r_PSD", "r_caus")
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need the MI355X"
+    from qclhip import api
+    api.context(0)
+    return api
+
+
+def _rand(rng, shape, scale=1.0):
+    return np.asfortranarray(scale * (rng.normal(size=shape) + 1j * rng.normal(size=shape)))
+
+
+def _random_state(NE, Nk, Nb, seed, nfam=2, ncand=2):
+    rng = np.random.default_rng(seed)
+    shape = (NE, Nk, Nb, Nb)
+    s = dict(eps=np.linspace(-0.3, 0.4, NE), h=_rand(rng, (Nk, Nb, Nb), 0.2))
+    for k in ("GR", "GL", "GG", "A"):
+        s[k] = _rand(rng, shape)
+    s["fams"] = [[_rand(rng, shape, 0.1) for _ in range(3)] for _ in range(nfam)]
+    s["cands"] = [_rand(rng, shape, 0.1) for _ in range(ncand)]
+    return s
+
+
+def _oracle(orc, s):
+    tot = [np.asfortranarray(sum(f[k] for f in s["fams"])) for k in range(3)]
+    cand = np.asfortranarray(sum(s["cands"]))
+    return orc.residual_suite(s["eps"], s["h"], s["GR"], s["GL"], s["GG"], s["A"], *tot, cand)
+
+
+def _device(hip, s):
+    import torch
+    NE, Nk, Nb = s["GR"].shape[:3]
+    g = hip.Grid(s["eps"], np.full(NE, 0.01), np.full(Nk, 1.0 / Nk), Nb)
+    green = hip.GreenState(*(hip.to_device(s[k]) for k in ("GR", "GL", "GG", "A")),
+                           torch.zeros((Nk, NE), dtype=torch.float64, device="cuda"))
+    fams = [tuple(hip.to_device(x) for x in f) for f in s["fams"]]
+    cands = [hip.to_device(x) for x in s["cands"]]
+    return g, hip.h_to_device(s["h"]), green, fams, cands
+
+
+def _gpu(hip, s):
+    g, h, green, fams, cands = _device(hip, s)
+    return hip.residual_suite(g, h, green, fams, cands)
+
+
+@pytest.mark.parametrize("Nb", [1, 2, 3, 5, 8, 9, 16, 17, 24, 31, 33, 40, 48])
+def test_random_states_vs_oracle(hip, orc, Nb):
+    s = _random_state(NE=7, Nk=3, Nb=Nb, seed=100 + Nb)
+    ref = _oracle(orc, s)
+    got = _gpu(hip, s)
+    for k in KEYS:
+        assert ref[k] > 0.0, k  # the random state violates every gate
+        assert abs(got[k] - ref[k]) <= 1e-10 * ref[k], (k, got[k], ref[k])
+
+
+def test_single_family_and_many_candidates(hip, orc):
+    s = _random_state(NE=9, Nk=2, Nb=6, seed=5, nfam=1, ncand=8)
+    ref = _oracle(orc, s)
+    got = _gpu(hip, s)
+    for k in KEYS:
+        assert abs(got[k] - ref[k]) <= 1e-10 * ref[k], k
+
+
+def _zero_state(NE=3, Nk=2, Nb=4):
+    z = lambda: np.zeros((NE, Nk, Nb, Nb), np.complex128, order="F")
+    return dict(eps=np.linspace(0, 1, NE), h=np.zeros((Nk, Nb, Nb), np.complex128, order="F"),
+                GR=z(), GL=z(), GG=z(), A=z(), fams=[[z(), z(), z()]], cands=[z()])
+
+
+def test_zero_green_function(hip):
+    r = _gpu(hip, _zero_state())
+    assert r["r_D"] == pytest.approx(1.0, rel=1e-15)
+    assert r["r_A"] == 0.0 and r["r_K"] == 0.0 and r["r_PSD"] == 0.0 and r["r_caus"] == 0.0
+
+
+def test_exact_inverse(hip):
+    rng = np.random.default_rng(5)
+    s = _zero_state(NE=5, Nk=3, Nb=12)
+    for e in range(5):
+        for m in range(3):
+            S = 0.1 * (rng.normal(size=(12, 12)) + 1j * rng.normal(size=(12, 12)))
+            s["fams"][0][0][e, m] = S
+            s["GR"][e, m] = np.linalg.inv(s["eps"][e] * np.eye(12) - S)
+    assert _gpu(hip, s)["r_D"] < 1e-14
+
+
+@pytest.mark.parametrize("field,factor", [("A", 1.0), ("GL", 1j), ("GG", -1j)])
+def test_negative_eigenvalue_closed_form(hip, field, factor):
+    rng = np.random.default_rng(7)
+    s = _zero_state()
+    Q, _ = np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))
+    s[field][1, 0] = factor * (Q @ np.diag([-0.3, 0.5, 1.0, 2.0]) @ Q.conj().T)
+    assert _gpu(hip, s)["r_PSD"] == pytest.approx((0.3 - 64 * 4 * EPS * 2.0) / 2.0, rel=1e-13)
+
+
+def test_causality_closed_form(hip):
+    s = _zero_state()
+    Q, _ = np.linalg.qr(np.random.default_rng(2).normal(size=(4, 4)) + 0j)
+    s["fams"][0][0][2, 1] = 1j * (Q @ np.diag([0.2, -0.5, -1.0, 0.0]) @ Q.T)
+    assert _gpu(hip, s)["r_caus"] == pytest.approx(0.2 / (1.0 + 1e-14), rel=1e-13)
+    s["fams"][0][0][2, 1] = -1j * np.eye(4)
+    assert _gpu(hip, s)["r_caus"] == 0.0
+
+
+def test_spectral_and_keldysh_closed_form(hip):
+    s = _zero_state()
+    I = np.eye(4)
+    s["GR"][:, :] = I
+    s["A"][:, :] = 2 * I
+    s["fams"][0][2][:, :] = -1.5j * I
+    s["fams"][0][1][:, :] = 1.5j * I
+    s["GL"][:, :] = 2j * I
+    s["cands"][0][:, :] = 1j * I
+    r = _gpu(hip, s)
+    assert r["r_A"] == pytest.approx(0.5, rel=1e-14)
+    assert r["r_K"] == pytest.approx(0.5, rel=1e-14)
+
+
+@pytest.mark.parametrize("where", ["GR", "SR", "A", "cand"])
+def test_nonfinite_block_gives_inf(hip, orc, where):
+    s = _random_state(NE=4, Nk=2, Nb=5, seed=11)
+    target = {"GR": s["GR"], "SR": s["fams"][1][0], "A": s["A"], "cand": s["cands"][0]}[where]
+    target[2, 1, 3, 0] = np.nan
+    ref = _oracle(orc, s)
+    got = _gpu(hip, s)
+    for k in KEYS:
+        assert (got[k] == np.inf) == (ref[k] == np.inf), (k, got[k], ref[k])
+        if ref[k] != np.inf:
+            assert abs(got[k] - ref[k]) <= 1e-10 * ref[k], k
+    assert any(got[k] == np.inf for k in KEYS)
+
+
+def _scba_state(prob):
+    full, state, green = fx.oracle_state(prob)
+    from oracle import oracle as orc
+    fams = [[state["embR"], state["embL"], state["embG"]]] + \
+        [[state["scatR"][i], state["scatL"][i], state["scatG"][i]] for i in range(len(prob["mechanisms"]))]
+    new = orc.current_green(prob["eps"], prob["h"], [f[0] for f in fams], [f[1] for f in fams],
+                            [f[2] for f in fams], prob["wE"], prob["wk"], prob["gs"], full["target"])
+    new = dict(GR=new["GR"], GL=new["GL"], GG=new["GG"], A=orc.spectral(new["GR"]))
+    scat, plus, minus, emb = fx.oracle_candidates(prob, full["kernels"], new)
+    cands = [emb[1]] + [c[1] for c in scat]
+    return dict(eps=prob["eps"], h=prob["h"], fams=fams, cands=cands, **new), (emb, plus, minus, scat, state)
+
+
+@pytest.mark.parametrize("N", [4, 10])
+def test_scba_state_vs_oracle(hip, orc, N):
+    prob = fx.make_problem(N=N, NE=48, Nk=3)
+    s, _ = _scba_state(prob)
+    ref = _oracle(orc, s)
+    got = _gpu(hip, s)
+    # a consistent state: r_D, r_A at rounding level, r_PSD and r_caus ~0
+    assert ref["r_D"] < 1e-13 and got["r_D"] < 1e-13
+    for k in ("r_A", "r_K"):
+        assert abs(got[k] - ref[k]) <= 1e-10 * ref[k] + 1e-13, k
+    for k in ("r_PSD", "r_caus"):
+        assert abs(got[k] - ref[k]) <= 1e-12, k
+
+
+def test_selfenergy_residual_vs_oracle(hip, orc):
+    prob = fx.make_problem(N=6, NE=40, Nk=3)
+    s, (emb, plus, minus, scat, state) = _scba_state(prob)
+    old = [state["embR"], state["embL"], state["embG"]]
+    ref = orc.selfenergy_residual(old, emb)
+    got = hip.selfenergy_residual([hip.to_device(x) for x in old], [hip.to_device(x) for x in emb])
+    assert ref > 0 and abs(got - ref) <= 1e-12 * ref
+    dev = [hip.to_device(x) for x in emb]
+    assert hip.selfenergy_residual(dev, dev) == 0.0
+    bad = [x.copy() for x in emb]
+    bad[2][0, 0, 0, 0] = np.inf
+    assert hip.selfenergy_residual([hip.to_device(x) for x in old], [hip.to_device(x) for x in bad]) == np.inf
+    assert orc.selfenergy_residual(old, bad) == np.inf
+
+
+def test_deferred_matches_eager(hip, orc):
+    import torch
+    s = _random_state(NE=6, Nk=2, Nb=7, seed=21)
+    g, h, green, fams, cands = _device(hip, s)
+    eager = hip.residual_suite(g, h, green, fams, cands)
+    sig = hip.selfenergy_residual(fams[0], fams[1])
+    buf = torch.zeros(8, dtype=torch.float64, device="cuda")
+    g.ctx.defer(buf)
+    try:
+        d1 = hip.residual_suite(g, h, green, fams, cands)
+        d2 = hip.selfenergy_residual(fams[0], fams[1])
+    finally:
+        g.ctx.defer(None)
+    g.ctx.synchronize()
+    host = buf.cpu().numpy()
+    assert list(d1.read(host)) == [eager[k] for k in KEYS]
+    assert d2.read(host) == sig
