@@ -4,11 +4,9 @@
 //
 // One wavefront per (e,m) block. Block products reuse the lane-grid product
 // of the Keldysh kernels; the positivity and causality checks need Hermitian
-// eigenvalues, computed by a parallel-ordered two-sided Jacobi method on the
-// block in LDS: each round rotates N/2 disjoint pairs (round-robin
-// "circle" schedule) at once, J = diag(1, u*) R with u = H_pq/|H_pq| and R the
-// real symmetric Schur rotation (Golub-Van Loan 8.5.2), H <- J^H H J; sweeps
-// stop when the off-diagonal mass is below 1e-34 of the total.
+// extreme eigenvalues, computed by a Householder reduction to tridiagonal
+// form in registers and Sturm multisection (eig_core below), after a
+// Cholesky screen that settles positive blocks.
 // Per-block values are reduced afterwards in a fixed order.
 #include "dk_common.hpp"
 
@@ -80,134 +78,296 @@ __device__ inline double tile_frob2(const c64* T, int N, int lane, double cdiag)
 
 // Hermitian part of f X into H: H = (f X + (f X)^dagger) / 2; f in {1, -i, i}
 // as (fr, fi).
-template <int LD>
-__device__ inline void herm_part(const c64* X, c64* H, int N, int lane, double fr, double fi) {
-    for (int f = lane; f < N * N; f += 64) {
-        const int i = f / N, j = f % N;
-        const c64 a = X[i * LD + j], b = X[j * LD + i];
-        const c64 fa = cmk(fr * a.x - fi * a.y, fr * a.y + fi * a.x);
-        const c64 fb = cmk(fr * b.x - fi * b.y, fr * b.y + fi * b.x);
-        H[i * LD + j] = cmk(0.5 * (fa.x + fb.x), 0.5 * (fa.y - fb.y));
-    }
-}
-
-struct JacobiPair {
-    double c, s;
-    c64 u;
-    int p, q;
+// LDS vectors of one wave's tridiagonal reduction and Cholesky screen.
+struct EigWork {
+    c64 u[48];
+    c64 q[48];
+    double d[48];
+    double e2[48];
 };
 
-// Eigenvalues of the Hermitian tile H (destroyed); returns lambda_min,
-// lambda_max, max |lambda| (wave-uniform). prs: LDS scratch of 24 pairs.
-template <int LD>
-__device__ inline void wave_eig_extremes(c64* H, int N, int lane, JacobiPair* prs, double& lmin, double& lmax,
-                                         double& lbig) {
-    const int ne = (N + 1) & ~1;  // even number of slots; slot N (odd N) is idle
-    const int mm = ne - 1;        // odd
-    const int npairs = ne / 2;
-    for (int sweep = 0; sweep < 60 && N > 1; ++sweep) {
-        double off = 0.0, tot = 0.0;
-        for (int f = lane; f < N * N; f += 64) {
-            const int i = f / N, j = f % N;
-            const c64 v = H[i * LD + j];
-            const double a = v.x * v.x + v.y * v.y;
-            tot += a;
-            if (i != j) off += a;
-        }
-        off = wave_sum(off);
-        tot = wave_sum(tot);
-        if (off == 0.0 || off <= 1e-34 * tot) break;
-        for (int r = 0; r < mm; ++r) {
-            const int ks = (int)((long long)r * ((mm + 1) / 2) % mm);  // 2 ks = r (mod mm)
-            if (lane < npairs) {
-                int p, q;
-                if (lane == 0) {
-                    p = ks;
-                    q = mm;
-                } else {
-                    p = (ks + lane) % mm;
-                    q = (ks - lane + mm) % mm;
-                }
-                JacobiPair jp;
-                jp.p = p;
-                jp.q = q;
-                jp.c = 1.0;
-                jp.s = 0.0;
-                jp.u = cmk(1.0, 0.0);
-                if (p < N && q < N) {
-                    const c64 b = H[p * LD + q];
-                    const double ab = sqrt(b.x * b.x + b.y * b.y);
-                    if (ab > 0.0) {
-                        const double a = H[p * LD + p].x, d = H[q * LD + q].x;
-                        const double tau = (d - a) / (2.0 * ab);
-                        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        jp.c = 1.0 / sqrt(1.0 + t * t);
-                        jp.s = t * jp.c;
-                        jp.u = cmk(b.x / ab, b.y / ab);
-                    }
-                }
-                prs[lane] = jp;
-            }
-            wave_sync();
-            // columns: H <- H J  (col p = c col_p - s u* col_q, col q = s col_p + c u* col_q)
-            for (int t = lane; t < npairs * N; t += 64) {
-                const JacobiPair jp = prs[t / N];
-                const int i = t % N;
-                if (jp.q >= N || jp.p >= N || jp.s == 0.0) continue;
-                const c64 hp = H[i * LD + jp.p], hq = H[i * LD + jp.q];
-                const c64 uq = cmul(cconj(jp.u), hq);
-                H[i * LD + jp.p] = cmk(jp.c * hp.x - jp.s * uq.x, jp.c * hp.y - jp.s * uq.y);
-                H[i * LD + jp.q] = cmk(jp.s * hp.x + jp.c * uq.x, jp.s * hp.y + jp.c * uq.y);
-            }
-            wave_sync();
-            // rows: H <- J^H H  (row p = c row_p - s u row_q, row q = s row_p + c u row_q)
-            for (int t = lane; t < npairs * N; t += 64) {
-                const JacobiPair jp = prs[t / N];
-                const int j = t % N;
-                if (jp.q >= N || jp.p >= N || jp.s == 0.0) continue;
-                const c64 hp = H[jp.p * LD + j], hq = H[jp.q * LD + j];
-                const c64 uq = cmul(jp.u, hq);
-                c64 np = cmk(jp.c * hp.x - jp.s * uq.x, jp.c * hp.y - jp.s * uq.y);
-                c64 nq = cmk(jp.s * hp.x + jp.c * uq.x, jp.s * hp.y + jp.c * uq.y);
-                // The rotated pair is diagonal with a real diagonal in exact
-                // arithmetic: store it so (rows p and q belong to this pair).
-                if (j == jp.p) {
-                    np.y = 0.0;
-                    nq = cmk(0.0, 0.0);
-                } else if (j == jp.q) {
-                    np = cmk(0.0, 0.0);
-                    nq.y = 0.0;
-                }
-                H[jp.p * LD + j] = np;
-                H[jp.q * LD + j] = nq;
-            }
-            wave_sync();
-        }
-    }
-    double mn = INFINITY, mx = -INFINITY, bg = 0.0, bad = 0.0;
-    for (int i = lane; i < N; i += 64) {
-        const double v = H[i * LD + i].x;
-        if (!isfinite(v)) bad = 1.0;
-        mn = fmin(mn, v);
-        mx = fmax(mx, v);
-        bg = fmax(bg, fabs(v));
-    }
-    lmin = wave_min(mn);
-    lmax = wave_max(mx);
-    lbig = wave_max(bg);
-    if (wave_max(bad) != 0.0) lmin = lmax = lbig = NAN;
-    wave_sync();  // the caller may overwrite H next
+// Sums over the 8 lanes of one row group (lane bits 3..5) and of one column
+// group (lane bits 0..2) of the 8 x 8 lane grid.
+__device__ inline double sum_over_lc(double v) {
+    v += shfl_xor_d(v, 8);
+    v += shfl_xor_d(v, 16);
+    return v + shfl_xor_d(v, 32);
+}
+__device__ inline double sum_over_lr(double v) {
+    v += shfl_xor_d(v, 1);
+    v += shfl_xor_d(v, 2);
+    return v + shfl_xor_d(v, 4);
 }
 
-// 1 when every element of the N x N tile is finite (wave-uniform).
-template <int LD>
-__device__ inline bool tile_finite(const c64* T, int N, int lane) {
-    double bad = 0.0;
-    for (int f = lane; f < N * N; f += 64) {
-        const c64 v = T[(f / N) * LD + f % N];
-        if (!(isfinite(v.x) && isfinite(v.y))) bad = 1.0;
+// Hermitian part of f X, f = fr + i fi, from an LDS tile into the lane-grid
+// registers (rows lr*B + r, columns lc*B + c; zero outside N x N).
+template <int B, int LD>
+__device__ inline void load_herm(const c64* X, int N, int lane, double fr, double fi, c64 (&a)[B][B]) {
+    const int lr = lane & 7, lc = lane >> 3;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr * B + r, j = lc * B + c;
+            if (i < N && j < N) {
+                const c64 x = X[i * LD + j], y = X[j * LD + i];
+                const double ax = fr * x.x - fi * x.y, ay = fr * x.y + fi * x.x;
+                const double bx = fr * y.x - fi * y.y, by = fr * y.y + fi * y.x;
+                a[r][c] = cmk(0.5 * (ax + bx), 0.5 * (ay - by));
+            } else {
+                a[r][c] = cmk(0.0, 0.0);
+            }
+        }
+}
+
+struct BlockScale {
+    bool bad;     // a non-finite entry
+    double amax;  // largest |Re|, |Im|
+    double fro2;  // squared Frobenius norm
+};
+
+// Wave-uniform scale figures of the N x N part of a register block.
+template <int B>
+__device__ inline BlockScale block_scale(const c64 (&a)[B][B], int N, int lane) {
+    const int lr = lane & 7, lc = lane >> 3;
+    double amax = 0.0, fro2 = 0.0, bad = 0.0;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+            if (lr * B + r < N && lc * B + c < N) {
+                const c64 v = a[r][c];
+                if (!(isfinite(v.x) && isfinite(v.y))) bad = 1.0;
+                amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
+                fro2 += v.x * v.x + v.y * v.y;
+            }
+    BlockScale s;
+    s.bad = wave_max(bad) != 0.0;
+    s.amax = wave_max(amax);
+    s.fro2 = wave_sum(fro2);
+    return s;
+}
+
+template <int B>
+__device__ inline void scale_regs(c64 (&a)[B][B], double s) {
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) a[r][c] = cmk(a[r][c].x * s, a[r][c].y * s);
+}
+
+// True when the Hermitian register block a + shift I (N x N part) has a
+// Cholesky factorization a + shift I = L L^H, i.e. every pivot of the
+// right-looking elimination is positive; a is destroyed. The column of each
+// step is passed through LDS.
+template <int B>
+__device__ inline bool chol_screen(c64 (&a)[B][B], int N, int lane, EigWork* w, double shift) {
+    const int lr = lane & 7, lc = lane >> 3;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+            if (lr * B + r == lc * B + c) a[r][c].x += shift;
+    for (int k = 0; k < N; ++k) {
+        const int kg = k / B, kc = k - kg * B;
+        if (lc == kg) {
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                if (c != kc) continue;
+#pragma unroll
+                for (int r = 0; r < B; ++r) {
+                    const int i = lr * B + r;
+                    if (i >= k && i < N) w->u[i] = a[r][c];
+                }
+            }
+        }
+        wave_sync();
+        const double piv = w->u[k].x;
+        if (!(piv > 0.0)) {
+            wave_sync();
+            return false;
+        }
+        const double rs = 1.0 / sqrt(piv);
+        c64 li[B], lj[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int i = lr * B + r, j = lc * B + r;
+            li[r] = (i > k && i < N) ? cscale(rs, w->u[i]) : cmk(0.0, 0.0);
+            lj[r] = (j > k && j < N) ? cscale(rs, w->u[j]) : cmk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) a[r][c] = cfms(a[r][c], li[r], cconj(lj[c]));
+        wave_sync();  // the next step rewrites u
     }
-    return wave_max(bad) == 0.0;
+    return true;
+}
+
+// Number of eigenvalues below x of the real symmetric tridiagonal matrix
+// with diagonal d and squared off-diagonal e2: the negative pivots of
+// LDL^T(T - x I), a zero pivot replaced by -pivmin (Kahan's count, as in
+// LAPACK's bisection).
+__device__ inline int sturm_count(const double* d, const double* e2, int N, double x, double pivmin) {
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    int cnt = q < 0.0;
+    for (int i = 1; i < N; ++i) {
+        q = (d[i] - x) - e2[i - 1] / q;
+        if (fabs(q) < pivmin) q = -pivmin;
+        cnt += q < 0.0;
+    }
+    return cnt;
+}
+
+// Narrow [a, b] to the sub-interval that holds the transition of a
+// monotone predicate sampled at x_j = a + (b - a)(j + 1)/33, j = 0..31
+// (bit j of mask set where the predicate holds).
+__device__ inline void multisect(double& a, double& b, unsigned mask) {
+    const double w = b - a;
+    if (mask == 0u) {
+        a = a + w * (32 / 33.0);
+        return;
+    }
+    const int f = __builtin_ctz(mask);
+    const double nb = a + w * ((f + 1) / 33.0);
+    if (f > 0) a = a + w * (f / 33.0);
+    b = nb;
+}
+
+// Smallest and largest eigenvalue of the finite Hermitian register block a
+// (N x N part; a is destroyed; entries outside are ignored), wave-uniform.
+// a is reduced to real symmetric tridiagonal form by N-2 Householder
+// reflections applied in registers on the 8 x 8 lane grid
+// (P = I - beta u u^H with u = x + e^{i arg x_0} ||x|| e_1, A <- P A P
+// through p = beta A u, q = p - (beta/2)(u^H p) u, A <- A - u q^H - q u^H),
+// and the two extreme eigenvalues are located by 32-point Sturm
+// multisection each (half-waves), to eps * ||T|| absolute. Callers scale a
+// to max |a_ij| ~ 1 first.
+template <int B>
+__device__ inline void eig_core(c64 (&a)[B][B], int N, int lane, EigWork* w, double& lmin, double& lmax) {
+    constexpr double kEps = 2.220446049250313e-16, kTiny = 2.2250738585072014e-308;
+    const int lr = lane & 7, lc = lane >> 3;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+            if (lr * B + r >= N || lc * B + c >= N) a[r][c] = cmk(0.0, 0.0);
+    for (int k = 0; k + 2 < N; ++k) {
+        const int kg = k / B, kc = k - kg * B;  // column k: lanes lc == kg, register column kc
+        double part = 0.0;
+        if (lc == kg) {
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                if (c != kc) continue;
+#pragma unroll
+                for (int r = 0; r < B; ++r) {
+                    const int i = lr * B + r;
+                    if (i > k && i < N) {
+                        w->u[i] = a[r][c];
+                        part += a[r][c].x * a[r][c].x + a[r][c].y * a[r][c].y;
+                    }
+                    if (i == k) w->d[k] = a[r][c].x;
+                }
+            }
+        }
+        const double nrm2 = wave_sum(part);
+        wave_sync();
+        const c64 x0 = w->u[k + 1];
+        if (lane == 0) w->e2[k] = nrm2;
+        if (nrm2 == 0.0) {  // column already reduced
+            wave_sync();
+            continue;
+        }
+        const double ax0 = sqrt(x0.x * x0.x + x0.y * x0.y), xn = sqrt(nrm2);
+        const c64 ph = ax0 > 0.0 ? cmk(x0.x / ax0, x0.y / ax0) : cmk(1.0, 0.0);
+        const c64 u1 = cmk(x0.x + ph.x * xn, x0.y + ph.y * xn);
+        const double beta = 1.0 / (nrm2 + xn * ax0);
+        c64 ur[B], uc[B], p[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int i = lr * B + r;
+            ur[r] = i == k + 1 ? u1 : (i > k + 1 && i < N ? w->u[i] : cmk(0.0, 0.0));
+            const int j = lc * B + r;
+            uc[r] = j == k + 1 ? u1 : (j > k + 1 && j < N ? w->u[j] : cmk(0.0, 0.0));
+        }
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            p[r] = cmk(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < B; ++c) p[r] = cfma(p[r], a[r][c], uc[c]);
+        }
+        double kd = 0.0;
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int i = lr * B + r;
+            p[r] = cmk(beta * sum_over_lc(p[r].x), beta * sum_over_lc(p[r].y));
+            if (i <= k || i >= N) p[r] = cmk(0.0, 0.0);
+            kd += ur[r].x * p[r].x + ur[r].y * p[r].y;  // Re(conj(u_i) p_i)
+        }
+        const double K = 0.5 * beta * sum_over_lr(kd);
+#pragma unroll
+        for (int r = 0; r < B; ++r) p[r] = cmk(p[r].x - K * ur[r].x, p[r].y - K * ur[r].y);  // q rows
+        if (lc == 0) {
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+                if (lr * B + r < N) w->q[lr * B + r] = p[r];
+        }
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int j = lc * B + c;
+            const c64 qc = (j > k && j < N) ? w->q[j] : cmk(0.0, 0.0);
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                const int i = lr * B + r;
+                if (i > k && i < N && j > k && j < N) {
+                    // a -= u_i conj(q_j) + q_i conj(u_j)
+                    a[r][c] = cfms(a[r][c], ur[r], cconj(qc));
+                    a[r][c] = cfms(a[r][c], p[r], cconj(uc[c]));
+                }
+            }
+        }
+        wave_sync();  // the next step rewrites u and q
+    }
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr * B + r, j = lc * B + c;
+            if (i == j && i < N && i >= N - 2) w->d[i] = a[r][c].x;
+            if (i == N - 1 && j == N - 2) w->e2[N - 2] = a[r][c].x * a[r][c].x + a[r][c].y * a[r][c].y;
+        }
+    wave_sync();
+    double e2max = 0.0, glo = INFINITY, ghi = -INFINITY;
+    for (int i = lane; i < N; i += 64) {
+        const double ri = (i > 0 ? sqrt(w->e2[i - 1]) : 0.0) + (i + 1 < N ? sqrt(w->e2[i]) : 0.0);
+        glo = fmin(glo, w->d[i] - ri);
+        ghi = fmax(ghi, w->d[i] + ri);
+        if (i + 1 < N) e2max = fmax(e2max, w->e2[i]);
+    }
+    glo = wave_min(glo);
+    ghi = wave_max(ghi);
+    e2max = wave_max(e2max);
+    const double pivmin = kTiny * fmax(1.0, e2max);
+    const double tnorm = fmax(fabs(glo), fabs(ghi));
+    const double widen = 2.0 * kEps * tnorm * N + 2.0 * pivmin;
+    double a0 = glo - widen, b0 = ghi + widen, a1 = a0, b1 = b0;
+    const int half = lane >> 5, jx = lane & 31;
+    for (int round = 0; round < 24; ++round) {
+        const double t0 = fmax(2.0 * kEps * fmax(fabs(a0), fabs(b0)), kEps * tnorm);
+        const double t1 = fmax(2.0 * kEps * fmax(fabs(a1), fabs(b1)), kEps * tnorm);
+        if (b0 - a0 <= t0 && b1 - a1 <= t1) break;
+        const double lo = half ? a1 : a0, hi = half ? b1 : b0;
+        const double x = lo + (hi - lo) * ((jx + 1) / 33.0);
+        const int cnt = sturm_count(w->d, w->e2, N, x, pivmin);
+        const unsigned long long m = __ballot(half ? cnt >= N : cnt >= 1);
+        multisect(a0, b0, (unsigned)(m & 0xffffffffull));
+        multisect(a1, b1, (unsigned)(m >> 32));
+    }
+    lmin = 0.5 * (a0 + b0);
+    lmax = 0.5 * (a1 + b1);
+    wave_sync();  // d and e2 are rewritten by the next call
 }
 
 // The positivity ratio of the reference's witness (positivity_diagnostics.jl:
@@ -219,16 +379,88 @@ __device__ inline double psd_ratio(double lmin, double lbig, int N) {
     return fmax(0.0, fmax(0.0, -lmin) - budget) / fmax(lbig, 2.2250738585072014e-308);
 }
 
-// Positivity ratio of the Hermitian part of f X (X intact, E overwritten);
-// Inf for a non-finite block (residuals.jl:413-452).
-template <int LD>
-__device__ inline double psd_term(const c64* X, c64* E, int N, int lane, JacobiPair* prs, double fr, double fi) {
-    if (!tile_finite<LD>(X, N, lane)) return INFINITY;
-    herm_part<LD>(X, E, N, lane, fr, fi);
-    wave_sync();
-    double lmin, lmax, lbig;
-    wave_eig_extremes<LD>(E, N, lane, prs, lmin, lmax, lbig);
-    return psd_ratio(lmin, lbig, N);
+// Positivity ratio of the Hermitian part H of f X (tile X intact); Inf for
+// a non-finite block (residuals.jl:413-452). Screen first: the ratio is 0
+// whenever lambda_min(H) >= -budget, and max |lambda| >= ||H||_F / sqrt(N)
+// bounds the budget from below, so a Cholesky factorization of H + delta I
+// with delta = 32 N eps max(||H||_F / sqrt(N), 1) (half that bound) settles
+// the block; its backward error (<= N(N+1) eps ||H||) stays within the other
+// half of the budget. Otherwise the extreme eigenvalues are computed.
+template <int B, int LD>
+__device__ inline double psd_from_tile(const c64* X, int N, int lane, EigWork* w, double fr, double fi) {
+    constexpr double kEps = 2.220446049250313e-16;
+    c64 a[B][B];
+    load_herm<B, LD>(X, N, lane, fr, fi, a);
+    const BlockScale s = block_scale<B>(a, N, lane);
+    if (s.bad) return INFINITY;
+    if (s.amax == 0.0) return 0.0;
+    int ex;
+    frexp(s.amax, &ex);
+    const double sc = ldexp(1.0, -ex);
+    scale_regs<B>(a, sc);
+    const double delta = 32.0 * N * kEps * fmax(sqrt(s.fro2 / N), 1.0);
+    if (chol_screen<B>(a, N, lane, w, delta * sc)) return 0.0;
+    load_herm<B, LD>(X, N, lane, fr, fi, a);
+    scale_regs<B>(a, sc);
+    double lmin, lmax;
+    eig_core<B>(a, N, lane, w, lmin, lmax);
+    lmin = ldexp(lmin, ex);
+    lmax = ldexp(lmax, ex);
+    return psd_ratio(lmin, fmax(fabs(lmin), fabs(lmax)), N);
+}
+
+// Causality of the retarded block S in tile X (residuals.jl:455-492):
+// lambda_max(Im S) / (||S||_2 + 1e-14) when positive, else 0; Inf for a
+// non-finite block. A Cholesky factorization of -Im S (negative definite
+// Im S, to rounding) settles the block at 0; ||S||_2^2 is the largest
+// eigenvalue of S S^dagger (that of S^dagger S). T: register scratch.
+template <int B, int LD>
+__device__ inline double causality_from_tile(const c64* X, int N, int lane, EigWork* w, c64 (&T)[B][B]) {
+    c64 a[B][B];
+    load_herm<B, LD>(X, N, lane, 0.0, 1.0, a);  // Hermitian part of i S = -Im S
+    const BlockScale s = block_scale<B>(a, N, lane);
+    if (s.bad) return INFINITY;
+    if (s.amax == 0.0) return 0.0;
+    int ex;
+    frexp(s.amax, &ex);
+    const double sc = ldexp(1.0, -ex);
+    scale_regs<B>(a, sc);
+    if (chol_screen<B>(a, N, lane, w, 0.0)) return 0.0;
+    load_herm<B, LD>(X, N, lane, 0.0, -1.0, a);  // Im S = (S - S^dagger) / (2i)
+    scale_regs<B>(a, sc);
+    double lmin, lmax;
+    eig_core<B>(a, N, lane, w, lmin, lmax);
+    lmax = ldexp(lmax, ex);
+    if (!isfinite(lmax)) return INFINITY;
+    if (!(lmax > 0.0)) return 0.0;
+    rprod<B>(X, X, N, lane, T, true);
+    const BlockScale t = block_scale<B>(T, N, lane);
+    if (t.bad) return INFINITY;
+    int ex2;
+    frexp(t.amax, &ex2);
+    scale_regs<B>(T, ldexp(1.0, -ex2));
+    double s0, s1;
+    eig_core<B>(T, N, lane, w, s0, s1);
+    const double v = lmax / (sqrt(fmax(0.0, ldexp(s1, ex2))) + 1e-14);
+    return isfinite(v) ? v : INFINITY;
+}
+
+// Sum over the N x N part of |X_ij - T_ij|^2 (tile against registers).
+template <int B, int LD>
+__device__ inline double tile_reg_diff2(const c64* X, const c64 (&T)[B][B], int N, int lane) {
+    const int lr = lane & 7, lc = lane >> 3;
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr * B + r, j = lc * B + c;
+            if (i < N && j < N) {
+                const c64 d = csub(X[i * LD + j], T[r][c]);
+                s += d.x * d.x + d.y * d.y;
+            }
+        }
+    return wave_sum(s);
 }
 
 struct ResArgs {
@@ -247,6 +479,10 @@ struct ResArgs {
     double* out;  // [7][NE*Nk]: rD, specN, specD, kelN, kelD, psd, caus
 };
 
+// One wavefront per (e,m) block with two LDS tiles: G^R, and a working tile
+// X that successively holds S^R, D, Gamma, G Gamma, A, the candidate S<,
+// G S<, G<, G>, and the raw products; the right factor of each product
+// stays in registers.
 template <int B>
 __global__ __launch_bounds__(64) void k_residual_suite(ResArgs p) {
     constexpr int LD = 8 * B + 1;
@@ -255,47 +491,22 @@ __global__ __launch_bounds__(64) void k_residual_suite(ResArgs p) {
     extern __shared__ __align__(16) unsigned char smem[];
     c64* G = reinterpret_cast<c64*>(smem);
     c64* X = G + wstride;
-    c64* Y = X + wstride;
-    c64* E = Y + wstride;
-    JacobiPair* prs = reinterpret_cast<JacobiPair*>(E + wstride);
+    EigWork* ew = reinterpret_cast<EigWork*>(X + wstride);
     const int lane = threadIdx.x;
+    const int lr = lane & 7, lc = lane >> 3;
     const long long nblk = (long long)p.NE * p.Nk;
     const long long blk = blockIdx.x;  // e + NE m
     const int e = (int)(blk % p.NE), m = (int)(blk / p.NE);
     const int N = p.N;
     const size_t S = (size_t)nblk;
     c64 T[B][B];
-    double lmin, lmax, lbig;
     const c64* one[1];
     one[0] = p.GR;
     tile_load<W, LD>(G, wstride, one, 1, blk, 1, N, S);
     tile_load<W, LD>(X, wstride, p.famR, p.nfam, blk, 1, N, S);
     wave_sync();
-    // Causality of the total S^R (residuals.jl:455-492).
-    double caus = 0.0;
-    const bool sr_finite = tile_finite<LD>(X, N, lane);
-    for (int f = lane; f < N * N; f += 64) {
-        const int i = f / N, j = f % N;
-        const c64 a = X[i * LD + j], b = X[j * LD + i];
-        E[i * LD + j] = cmk(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));  // (S - S^dagger) / (2i)
-    }
+    const double caus = causality_from_tile<B, LD>(X, N, lane, ew, T);
     wave_sync();
-    wave_eig_extremes<LD>(E, N, lane, prs, lmin, lmax, lbig);
-    if (!sr_finite || !isfinite(lmax)) {
-        caus = INFINITY;
-    } else if (lmax > 0.0) {
-        // ||S^R||_2 = sqrt(lambda_max(S^dagger S))
-        for (int f = lane; f < N * N; f += 64) {
-            const int i = f / N, j = f % N;
-            c64 v = cmk(0.0, 0.0);
-            for (int c = 0; c < N; ++c) v = cfma(v, cconj(X[c * LD + i]), X[c * LD + j]);
-            E[i * LD + j] = v;
-        }
-        wave_sync();
-        double a0, a1, a2;
-        wave_eig_extremes<LD>(E, N, lane, prs, a0, a1, a2);
-        caus = lmax / (sqrt(fmax(0.0, a1)) + 1e-14);
-    }
     // r_D: D = eps - h - S^R in X, ||D G - I|| / (||D|| ||G|| + sqrt(N)).
     for (int f = lane; f < N * N; f += 64) {
         const int i = f / N, j = f % N;
@@ -305,83 +516,77 @@ __global__ __launch_bounds__(64) void k_residual_suite(ResArgs p) {
     }
     wave_sync();
     rprod<B>(X, G, N, lane, T, false);
-    wave_sync();
-    rstore<B>(Y, N, lane, T);
-    wave_sync();
-    const double rD = sqrt(tile_frob2<LD>(Y, N, lane, 1.0)) /
+    double dn = 0.0;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr * B + r, j = lc * B + c;
+            if (i < N && j < N) {
+                const c64 v = cmk(T[r][c].x - (i == j ? 1.0 : 0.0), T[r][c].y);
+                dn += v.x * v.x + v.y * v.y;
+            }
+        }
+    const double rD = sqrt(wave_sum(dn)) /
                       (sqrt(tile_frob2<LD>(X, N, lane, 0.0)) * sqrt(tile_frob2<LD>(G, N, lane, 0.0)) + sqrt((double)N));
-    double psd = 0.0;
-    // Gamma = i (S> - S<) in X.
-    for (int f = lane; f < N * N; f += 64) {
-        const int i = f / N, j = f % N;
-        const size_t idx = blk + S * (i + (size_t)N * j);
-        const c64 g = fam_load(p.famG, p.nfam, idx), l = fam_load(p.famL, p.nfam, idx);
-        X[i * LD + j] = cmk(-(g.y - l.y), g.x - l.x);
-    }
     wave_sync();
-    // r_A terms: ||A - G Gamma G^dagger||^2, ||A||^2.
-    rprod<B>(G, X, N, lane, T, false);
-    wave_sync();
-    rstore<B>(Y, N, lane, T);
-    wave_sync();
-    rprod<B>(Y, G, N, lane, T, true);
-    wave_sync();
-    rstore<B>(E, N, lane, T);
-    one[0] = p.A;
-    tile_load<W, LD>(Y, wstride, one, 1, blk, 1, N, S);
-    wave_sync();
-    double sn = 0.0;
-    for (int f = lane; f < N * N; f += 64) {
-        const int i = f / N, j = f % N;
-        const c64 d = csub(Y[i * LD + j], E[i * LD + j]);
-        sn += d.x * d.x + d.y * d.y;
-    }
-    const double specN = wave_sum(sn);
-    const double specD = tile_frob2<LD>(Y, N, lane, 0.0);
-    wave_sync();
-    // Positivity of Gamma and of A.
-    psd = fmax(psd, psd_term<LD>(X, E, N, lane, prs, 1.0, 0.0));
-    psd = fmax(psd, psd_term<LD>(Y, E, N, lane, prs, 1.0, 0.0));
-    // r_K terms against the unmixed candidate S<.
-    tile_load<W, LD>(X, wstride, p.cand, p.ncand, blk, 1, N, S);
-    wave_sync();
-    rprod<B>(G, X, N, lane, T, false);
-    wave_sync();
-    rstore<B>(Y, N, lane, T);
-    wave_sync();
-    rprod<B>(Y, G, N, lane, T, true);
-    wave_sync();
-    rstore<B>(E, N, lane, T);
-    one[0] = p.GL;
-    tile_load<W, LD>(Y, wstride, one, 1, blk, 1, N, S);
-    wave_sync();
-    sn = 0.0;
-    for (int f = lane; f < N * N; f += 64) {
-        const int i = f / N, j = f % N;
-        const c64 d = csub(Y[i * LD + j], E[i * LD + j]);
-        sn += d.x * d.x + d.y * d.y;
-    }
-    const double kelN = wave_sum(sn);
-    const double kelD = tile_frob2<LD>(Y, N, lane, 0.0);
-    wave_sync();
-    psd = fmax(psd, psd_term<LD>(Y, E, N, lane, prs, 0.0, -1.0));  // -i G<
-    one[0] = p.GG;
-    tile_load<W, LD>(Y, wstride, one, 1, blk, 1, N, S);
-    wave_sync();
-    psd = fmax(psd, psd_term<LD>(Y, E, N, lane, prs, 0.0, 1.0));  // i G>
-    // Raw Keldysh products before the paired map: -i G S< G^dagger, i G S> G^dagger.
-    for (int comp = 0; comp < 2; ++comp) {
-        tile_load<W, LD>(X, wstride, comp ? p.famG : p.famL, p.nfam, blk, 1, N, S);
+    // Six positivity checks; phase ph prepares X (and the r_A, r_K terms on
+    // the way), then checks the Hermitian part of f X:
+    //   0: Gamma = i (S> - S<)                          f = 1
+    //   1: A, after T = G Gamma G^dagger (r_A terms)     f = 1
+    //   2: G<, after T = G S<_cand G^dagger (r_K terms)  f = -i
+    //   3: G>                                            f = i
+    //   4: G S< G^dagger, 5: G S> G^dagger (raw)         f = -i, i
+    double psd = 0.0, specN = 0.0, specD = 0.0, kelN = 0.0, kelD = 0.0;
+#pragma unroll 1
+    for (int ph = 0; ph < 6; ++ph) {
+        if (ph == 0) {
+            for (int f = lane; f < N * N; f += 64) {
+                const int i = f / N, j = f % N;
+                const size_t idx = blk + S * (i + (size_t)N * j);
+                const c64 g = fam_load(p.famG, p.nfam, idx), l = fam_load(p.famL, p.nfam, idx);
+                X[i * LD + j] = cmk(-(g.y - l.y), g.x - l.x);
+            }
+        } else if (ph == 2) {
+            tile_load<W, LD>(X, wstride, p.cand, p.ncand, blk, 1, N, S);
+        } else if (ph == 3) {
+            one[0] = p.GG;
+            tile_load<W, LD>(X, wstride, one, 1, blk, 1, N, S);
+        } else if (ph == 4) {
+            tile_load<W, LD>(X, wstride, p.famL, p.nfam, blk, 1, N, S);
+        } else if (ph == 5) {
+            tile_load<W, LD>(X, wstride, p.famG, p.nfam, blk, 1, N, S);
+        }
         wave_sync();
-        rprod<B>(G, X, N, lane, T, false);
+        if (ph != 0 && ph != 3) {  // T = G X G^dagger
+            rprod<B>(G, X, N, lane, T, false);
+            wave_sync();
+            rstore<B>(X, N, lane, T);
+            wave_sync();
+            rprod<B>(X, G, N, lane, T, true);
+            wave_sync();
+            if (ph == 1 || ph == 2) {
+                one[0] = ph == 1 ? p.A : p.GL;
+                tile_load<W, LD>(X, wstride, one, 1, blk, 1, N, S);
+            } else {
+                rstore<B>(X, N, lane, T);
+            }
+            wave_sync();
+            if (ph == 1 || ph == 2) {
+                const double dn2 = tile_reg_diff2<B, LD>(X, T, N, lane);
+                const double xn2 = tile_frob2<LD>(X, N, lane, 0.0);
+                if (ph == 1) {
+                    specN = dn2;
+                    specD = xn2;
+                } else {
+                    kelN = dn2;
+                    kelD = xn2;
+                }
+            }
+        }
+        const double fi = (ph == 2 || ph == 4) ? -1.0 : ((ph == 3 || ph == 5) ? 1.0 : 0.0);
+        psd = fmax(psd, psd_from_tile<B, LD>(X, N, lane, ew, fi == 0.0 ? 1.0 : 0.0, fi));
         wave_sync();
-        rstore<B>(Y, N, lane, T);
-        wave_sync();
-        rprod<B>(Y, G, N, lane, T, true);
-        wave_sync();
-        rstore<B>(X, N, lane, T);
-        wave_sync();
-        psd = fmax(psd, psd_term<LD>(X, E, N, lane, prs, 0.0, comp ? 1.0 : -1.0));
     }
     if (lane == 0) {
         p.out[0 * nblk + blk] = rD;
@@ -481,7 +686,7 @@ __global__ __launch_bounds__(kSigBlocks) void k_sigma_reduce(const double* part,
 
 template <int B>
 int launch_residual(qclhip_ctx* ctx, const ResArgs& a) {
-    const size_t lds = 4 * (size_t)TileStride<B>::value * sizeof(c64) + 24 * sizeof(JacobiPair);
+    const size_t lds = 2 * (size_t)TileStride<B>::value * sizeof(c64) + sizeof(EigWork);
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
     if (lds > 65536)
