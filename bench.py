@@ -411,7 +411,8 @@ def main():
         line["scba_phases_ms"] = sc["phases_ms"]
         line["scba_config"] = ("N=24 NE=2048 Nk=64; momentum-independent (acoustic-like), separable IFR (J=3), "
                                "low-rank LO (r=4, thermal), bulk embedding, FFT Hilbert with 2L witness, "
-                               "linear mixing; FP64")
+                               "fixed-point measurement every iteration (residual suite r_D r_A r_K r_PSD r_caus; "
+                               "self-energy residuals fused into the linear mixing); FP64")
         line["scba_last"] = {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in sc["last"].items()
                              if k != "populations"}
         if "adjoint_iter_ms" in sc:

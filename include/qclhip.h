@@ -348,6 +348,18 @@ int qclhip_residual_suite(qclhip_ctx* ctx, const qclhip_grid* grid, const double
 int qclhip_selfenergy_residual(qclhip_ctx* ctx, int64_t n, const double* const* old_, const double* const* cand,
                                double* out_host);
 
+/* Linear mixing of the production loop (production_solver.jl:819-832,
+ * old_p <- (1 - alpha) old_p + alpha cand_p for every pair p) fused with the
+ * self-energy residuals of the unmixed pairs (production_iteration.jl:
+ * 134-166 over production_selfenergy_residual): ratios_host receives
+ * ||cand_t - old_t||_F / (||cand_t||_F + 1e-14) for the terms t = each pair
+ * alone (pair order), then, for each p with partner[p] > p (ascending p),
+ * the sum of pairs p and partner[p] (the embedding = plus + minus);
+ * partner[p] = -1 for none, symmetric otherwise. Inf for a non-finite term.
+ * Same traffic as the mixing alone. Deferrable (npairs + couples doubles). */
+int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
+                               const double* const* cand, double alpha, const int* partner, double* ratios_host);
+
 #ifdef __cplusplus
 }
 #endif

@@ -389,13 +389,21 @@ class ScbaOracle:
             setattr(s, name, P(state[name]))
         self._s = s
 
-    def iterate(self):
+    def iterate(self, measure: bool = False):
+        """One iteration; with measure, the row also holds the fixed-point
+        measurement (r_D, r_A, r_K, r_PSD, r_caus, r_Sigma, r_lambda)."""
         out = np.zeros(5)
         pops = np.zeros(self.prob["Nb"])
-        rc = lib().orc_scba_iteration(ct.byref(self._p), ct.byref(self._s), P(out), P(pops))
+        if measure:
+            meas = np.zeros(7)
+            rc = lib().orc_scba_iteration_measured(ct.byref(self._p), ct.byref(self._s), P(out), P(pops), P(meas))
+        else:
+            rc = lib().orc_scba_iteration(ct.byref(self._p), ct.byref(self._s), P(out), P(pops))
         _check(rc, "scba_iteration")
-        return dict(lam=out[0], Jbar=out[1], roundoff=out[2], number=out[3], holes=out[4],
-                    populations=pops)
+        row = dict(lam=out[0], Jbar=out[1], roundoff=out[2], number=out[3], holes=out[4], populations=pops)
+        if measure:
+            row.update(zip(("r_D", "r_A", "r_K", "r_PSD", "r_caus", "r_Sigma", "r_lambda"), meas))
+        return row
 
 def hermitian_eigvals(H):
     H = np.asfortranarray(H, dtype=np.complex128)

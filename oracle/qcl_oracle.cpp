@@ -947,7 +947,8 @@ void orc_seed_lesser(int NE, int Nk, int Nb, const double* GR_, const double* ep
 // the current total, fresh candidates (scattering with FFT Hilbert and its
 // round-off witness, embedding), observables from the current state, linear
 // mixing of every family, embedding total = plus + minus.
-int orc_scba_iteration(const orc_problem* p, orc_state* s, double* out, double* populations) {
+static int scba_iteration_core(const orc_problem* p, orc_state* s, double* out, double* populations,
+                               double* meas) {
     const int NE = p->NE, Nk = p->Nk, Nb = p->Nb, M = p->nmech;
     const size_t n = (size_t)NE * Nk * Nb * Nb;
     const long long ncols = (long long)Nk * Nb * Nb;
@@ -1007,6 +1008,44 @@ int orc_scba_iteration(const orc_problem* p, orc_state* s, double* out, double* 
         orc_embedding_component(NE, Nk, Nb, p->Tminus, X[k], ml.data(), mr.data(), mw.data(), mv.data(),
                                 cm[k].data());
     }
+    // Fixed-point measurement before mixing (production_iteration.jl:102-166):
+    // the residual suite against the totals of this iteration and the unmixed
+    // candidate total, the self-energy residuals of the embedding, plus,
+    // minus and every mechanism, and |lambda - 1|.
+    if (meas) {
+        std::vector<double> tot[3], candL(2 * n), ce[3];
+        const double* embp[3] = {s->embR, s->embL, s->embG};
+        for (int k = 0; k < 3; ++k) {
+            tot[k].assign(embp[k], embp[k] + 2 * n);
+            for (int i = 0; i < M; ++i) {
+                const double* x = k == 0 ? s->scatR[i] : (k == 1 ? s->scatL[i] : s->scatG[i]);
+                orc_add((long long)n, tot[k].data(), x, tot[k].data());
+            }
+            ce[k].resize(2 * n);
+            orc_add((long long)n, cp[k].data(), cm[k].data(), ce[k].data());
+        }
+        candL = ce[1];
+        for (int i = 0; i < M; ++i) orc_add((long long)n, candL.data(), cL[i].data(), candL.data());
+        rc = orc_residual_suite(NE, Nk, Nb, p->eps, p->h, s->GR, s->GL, s->GG, s->A, tot[0].data(), tot[1].data(),
+                                tot[2].data(), candL.data(), meas);
+        if (rc) return rc;
+        const double* oe[3] = {s->embR, s->embL, s->embG};
+        const double* cep[3] = {ce[0].data(), ce[1].data(), ce[2].data()};
+        const double* op[3] = {s->plusR, s->plusL, s->plusG};
+        const double* cpp[3] = {cp[0].data(), cp[1].data(), cp[2].data()};
+        const double* om[3] = {s->minusR, s->minusL, s->minusG};
+        const double* cmp[3] = {cm[0].data(), cm[1].data(), cm[2].data()};
+        double rs = orc_selfenergy_residual((long long)n, oe, cep);
+        rs = std::max(rs, orc_selfenergy_residual((long long)n, op, cpp));
+        rs = std::max(rs, orc_selfenergy_residual((long long)n, om, cmp));
+        for (int i = 0; i < M; ++i) {
+            const double* os[3] = {s->scatR[i], s->scatL[i], s->scatG[i]};
+            const double* cs[3] = {cR[i].data(), cL[i].data(), cG[i].data()};
+            rs = std::max(rs, orc_selfenergy_residual((long long)n, os, cs));
+        }
+        meas[5] = rs;
+        meas[6] = std::fabs(o3[0] - 1.0);
+    }
     // Observables of the current state (production_iteration.jl:218-236).
     double Jbar = orc_boundary_flux_bar(NE, Nk, Nb, p->eps, p->wE, p->wk, p->gs, s->plusL, s->plusG,
                                         s->GL, s->GG);
@@ -1033,6 +1072,15 @@ int orc_scba_iteration(const orc_problem* p, orc_state* s, double* out, double* 
         out[4] = o3[2];
     }
     return 0;
+}
+
+int orc_scba_iteration(const orc_problem* p, orc_state* s, double* out, double* populations) {
+    return scba_iteration_core(p, s, out, populations, nullptr);
+}
+
+int orc_scba_iteration_measured(const orc_problem* p, orc_state* s, double* out, double* populations,
+                                double* meas) {
+    return scba_iteration_core(p, s, out, populations, meas);
 }
 
 }  // extern "C"

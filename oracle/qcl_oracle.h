@@ -124,6 +124,11 @@ typedef struct {
 
 int orc_scba_iteration(const orc_problem* p, orc_state* s, double* out /* lambda, J,
                        roundoff, number, holes */, double* populations);
+/* The same iteration with the fixed-point measurement of the reference
+ * (production_iteration.jl:102-166) taken before mixing: meas = r_D, r_A,
+ * r_K, r_PSD, r_caus, r_Sigma, r_lambda. */
+int orc_scba_iteration_measured(const orc_problem* p, orc_state* s, double* out, double* populations,
+                                double* meas);
 
 /* Eigenvalues (ascending) of a Hermitian n x n block, cyclic Jacobi. */
 void orc_hermitian_eigvals(int n, const double* H, double* w);

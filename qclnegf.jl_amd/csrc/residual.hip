@@ -684,6 +684,92 @@ __global__ __launch_bounds__(kSigBlocks) void k_sigma_reduce(const double* part,
     }
 }
 
+// Linear mixing fused with the self-energy residual of the unmixed pairs: a
+// group is one pair, or two pairs whose sum is also a residual term (plus
+// and minus, whose sum is the embedding). Grid (kMixChunks, groups); each
+// block strides over a fixed element set, so the partial sums and their
+// reduction order are fixed.
+constexpr int kMixChunks = 512;
+constexpr int kMaxResPairs = 32;
+struct MixResArgs {
+    c64* old_[kMaxResPairs];
+    const c64* cand[kMaxResPairs];
+    int ga[kMaxResPairs], gb[kMaxResPairs], gt[kMaxResPairs];  // pairs and sum term of each group
+    int ngroups;
+};
+
+__global__ __launch_bounds__(256) void k_mix_residual(long long n, MixResArgs a, double alpha, double* part) {
+    __shared__ double red[6][256];
+    const int g = blockIdx.y;
+    const int pa = a.ga[g], pb = a.gb[g];
+    c64* __restrict__ oa = a.old_[pa];
+    const c64* __restrict__ ca = a.cand[pa];
+    c64* __restrict__ ob = pb >= 0 ? a.old_[pb] : nullptr;
+    const c64* __restrict__ cb = pb >= 0 ? a.cand[pb] : nullptr;
+    const double beta = 1.0 - alpha;
+    double acc[6] = {0, 0, 0, 0, 0, 0};  // |c-o|^2, |c|^2 of a, b and of the sum
+    const long long stride = (long long)kMixChunks * 256;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const c64 o1 = oa[i], c1 = ca[i];
+        const double dx = c1.x - o1.x, dy = c1.y - o1.y;
+        acc[0] += dx * dx + dy * dy;
+        acc[1] += c1.x * c1.x + c1.y * c1.y;
+        oa[i] = cmk(beta * o1.x + alpha * c1.x, beta * o1.y + alpha * c1.y);
+        if (pb >= 0) {
+            const c64 o2 = ob[i], c2 = cb[i];
+            const double ex = c2.x - o2.x, ey = c2.y - o2.y;
+            acc[2] += ex * ex + ey * ey;
+            acc[3] += c2.x * c2.x + c2.y * c2.y;
+            // the sum term: (c1 + c2) - (o1 + o2), as the stored sums would give it
+            const double sx = (c1.x + c2.x) - (o1.x + o2.x), sy = (c1.y + c2.y) - (o1.y + o2.y);
+            const double tx = c1.x + c2.x, ty = c1.y + c2.y;
+            acc[4] += sx * sx + sy * sy;
+            acc[5] += tx * tx + ty * ty;
+            ob[i] = cmk(beta * o2.x + alpha * c2.x, beta * o2.y + alpha * c2.y);
+        }
+    }
+    const int nacc = pb >= 0 ? 6 : 2;
+    for (int k = 0; k < nacc; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < nacc; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        // part[(2 t + {0,1}) * kMixChunks + chunk], t the residual term
+        part[(2 * pa + 0) * kMixChunks + blockIdx.x] = red[0][0];
+        part[(2 * pa + 1) * kMixChunks + blockIdx.x] = red[1][0];
+        if (pb >= 0) {
+            part[(2 * pb + 0) * kMixChunks + blockIdx.x] = red[2][0];
+            part[(2 * pb + 1) * kMixChunks + blockIdx.x] = red[3][0];
+            part[(2 * a.gt[g] + 0) * kMixChunks + blockIdx.x] = red[4][0];
+            part[(2 * a.gt[g] + 1) * kMixChunks + blockIdx.x] = red[5][0];
+        }
+    }
+}
+
+// One block per term: ratio ||c - o|| / (||c|| + 1e-14), Inf when non-finite.
+__global__ __launch_bounds__(kMixChunks) void k_mix_residual_reduce(const double* part, double* ratios) {
+    __shared__ double red[2][kMixChunks];
+    const int t = blockIdx.x;
+    red[0][threadIdx.x] = part[(2 * t + 0) * kMixChunks + threadIdx.x];
+    red[1][threadIdx.x] = part[(2 * t + 1) * kMixChunks + threadIdx.x];
+    __syncthreads();
+    for (int w = kMixChunks / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double d = red[0][0], c = red[1][0];
+        const double v = sqrt(d) / (sqrt(c) + 1e-14);
+        ratios[t] = (isfinite(d) && isfinite(c) && isfinite(v)) ? v : INFINITY;
+    }
+}
+
 template <int B>
 int launch_residual(qclhip_ctx* ctx, const ResArgs& a) {
     const size_t lds = 2 * (size_t)TileStride<B>::value * sizeof(c64) + sizeof(EigWork);
@@ -778,6 +864,43 @@ int qclhip_selfenergy_residual(qclhip_ctx* ctx, int64_t n, const double* const* 
     rc = defer_result(ctx, res, 1, &deferred);
     if (rc || deferred) return rc;
     QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(out_host, res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return qclhip_ctx_synchronize(ctx);
+}
+
+int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
+                               const double* const* cand, double alpha, const int* partner, double* ratios_host) {
+    if (!ctx || !old_ || !cand || !partner || !ratios_host || n < 0) return QCLHIP_ERR_ARGUMENT;
+    if (npairs < 1 || npairs > kMaxResPairs) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..32 mixing pairs");
+    if (!(alpha > 0 && alpha <= 1)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "mixing factor must lie in (0,1]");
+    MixResArgs a{};
+    int nterms = npairs;
+    a.ngroups = 0;
+    for (int p = 0; p < npairs; ++p) {
+        if (!old_[p] || !cand[p]) return QCLHIP_ERR_ARGUMENT;
+        const int q = partner[p];
+        if (q >= npairs || q == p || (q >= 0 && partner[q] != p))
+            return fail(ctx, QCLHIP_ERR_ARGUMENT, "partner must be a symmetric pairing of distinct pairs");
+        a.old_[p] = (c64*)old_[p];
+        a.cand[p] = (const c64*)cand[p];
+        if (q >= 0 && q < p) continue;  // grouped with its lower partner
+        a.ga[a.ngroups] = p;
+        a.gb[a.ngroups] = q;
+        a.gt[a.ngroups] = q >= 0 ? nterms++ : -1;
+        ++a.ngroups;
+    }
+    double* part = (double*)workspace(ctx, ((size_t)2 * nterms * kMixChunks + nterms) * sizeof(double));
+    if (!part) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    double* ratios = part + (size_t)2 * nterms * kMixChunks;
+    hipLaunchKernelGGL(k_mix_residual, dim3(kMixChunks, a.ngroups), dim3(256), 0, ctx->stream, (long long)n, a,
+                       alpha, part);
+    hipLaunchKernelGGL(k_mix_residual_reduce, dim3(nterms), dim3(kMixChunks), 0, ctx->stream, part, ratios);
+    int rc = launch_check(ctx, "mix with self-energy residual");
+    if (rc) return rc;
+    bool deferred = false;
+    rc = defer_result(ctx, ratios, nterms, &deferred);
+    if (rc || deferred) return rc;
+    QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(ratios_host, ratios, nterms * sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream));
     return qclhip_ctx_synchronize(ctx);
 }
 

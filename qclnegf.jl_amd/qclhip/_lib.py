@@ -111,6 +111,8 @@ _SIGNATURES = {
     "qclhip_residual_suite": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),
                               ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp), _dp],
     "qclhip_selfenergy_residual": [_vp, ct.c_int64, ct.POINTER(_vp), ct.POINTER(_vp), _dp],
+    "qclhip_mix_fields_residual": [_vp, ct.c_int64, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_double,
+                                   ct.POINTER(ct.c_int), _dp],
     "qclhip_batch_create": [_vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp)],
     "qclhip_batch_destroy": [_vp],
     "qclhip_batch_current_green": [_vp, _vp, ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),

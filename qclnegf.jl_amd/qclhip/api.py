@@ -515,6 +515,31 @@ def mix_fields(pairs, alpha: float):
     ctx.call("qclhip_mix_fields", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha))
 
 
+def mix_fields_residual(pairs, alpha: float, partner=None):
+    """Mix every (old, candidate) pair in one pass and return the self-energy
+    residual ratios of the unmixed pairs: one per pair, then one per couple
+    (p, partner[p]) with partner[p] > p, for the sum of the two pairs. A
+    Deferred of that many values in deferred-results mode."""
+    pairs = list(pairs)
+    if not pairs:
+        raise _lib.ArgumentError("no pairs to mix")
+    n = pairs[0][0].numel()
+    for o, c in pairs:
+        if o.dtype != torch.complex128 or c.dtype != torch.complex128 or o.numel() != n or c.numel() != n \
+                or not o.is_contiguous() or not c.is_contiguous():
+            raise _lib.ArgumentError("mixing pairs must be contiguous complex128 fields of one size")
+    partner = [-1] * len(pairs) if partner is None else list(partner)
+    nterms = len(pairs) + sum(1 for p, q in enumerate(partner) if q > p)
+    ctx = context(pairs[0][0].device.index or 0)
+    olds = (_vp * len(pairs))(*[o.data_ptr() for o, _ in pairs])
+    cands = (_vp * len(pairs))(*[c.data_ptr() for _, c in pairs])
+    out = (ct.c_double * nterms)()
+    slot0 = ctx.result_slot() if ctx.deferring else 0
+    ctx.call("qclhip_mix_fields_residual", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha),
+             (ct.c_int * len(pairs))(*partner), out)
+    return Deferred(slot0, nterms) if ctx.deferring else list(out)
+
+
 def boundary_flux_bar(grid: Grid, plusL, plusG, GL, GG) -> float:
     """_production_boundary_flux_bar (production_iteration.jl:189-201)."""
     v = ct.c_double()

@@ -39,6 +39,7 @@ class ScbaProblem:
     NLO: float = 0.0                # thermal LO occupation 1/expm1(hw/kT)
     alpha_mix: float = 0.3
     verify_roundoff: bool = True
+    measure: bool = True            # the fixed-point measurement of every iteration
 
     def __post_init__(self):
         g = self.grid
@@ -132,8 +133,15 @@ class ScbaWorkspace:
         self.minus = new_family(g)
 
 
+MEASURE_KEYS = ("r_D", "r_A", "r_K", "r_PSD", "r_caus")
+
+
 def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dict | None = None):
-    """One pass of the production loop (production_solver.jl:435-844)."""
+    """One pass of the production loop (production_solver.jl:435-844): Dyson,
+    candidates, the fixed-point measurement (production_iteration.jl:102-166:
+    residual suite before the observables, the self-energy residuals fused
+    into the mixing pass, which reads the same unmixed pairs), observables,
+    linear mixing."""
     g = p.grid
     t0 = time.perf_counter()
     fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
@@ -144,26 +152,41 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     t2 = time.perf_counter()
     _candidate_embedding(p, green, ws.plus, ws.minus)
     t3 = time.perf_counter()
+    suite = None
+    if p.measure:
+        cand_lesser = [ws.plus.L, ws.minus.L] + [ws.scat[m.name].L for m in p.mechanisms]
+        suite = api.residual_suite(g, p.h, green, fams, cand_lesser)
+    t4 = time.perf_counter()
     Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
     pops = api.state_populations(g, green.GL, p.target)
-    t4 = time.perf_counter()
+    t5 = time.perf_counter()
     pairs = []
     for m in p.mechanisms:
         pairs += list(zip(s.scattering[m.name].parts(), ws.scat[m.name].parts()))
+    base = len(pairs)
     for fam, cand in ((s.plus, ws.plus), (s.minus, ws.minus)):
         pairs += list(zip(fam.parts(), cand.parts()))
-    api.mix_fields(pairs, p.alpha_mix)
+    sigma = None
+    if p.measure:
+        # plus X pairs with minus X: their sum is the embedding term
+        partner = [-1] * base + [base + 3, base + 4, base + 5, base, base + 1, base + 2]
+        sigma = api.mix_fields_residual(pairs, p.alpha_mix, partner)
+    else:
+        api.mix_fields(pairs, p.alpha_mix)
     if g.ctx.deferring:  # captured or deferred: results are slots of the result buffer
-        row = dict(info=info, Jbar=Jbar, roundoff=roundoff, populations=pops)
-        return row
+        return dict(info=info, Jbar=Jbar, roundoff=roundoff, populations=pops, suite=suite, sigma=sigma)
     g.ctx.synchronize()
-    t5 = time.perf_counter()
+    t6 = time.perf_counter()
     if timings is not None:
         for k, v in (("dyson", t1 - t0), ("candidate_scattering", t2 - t1), ("candidate_embedding", t3 - t2),
-                     ("observables", t4 - t3), ("mixing", t5 - t4)):
+                     ("residuals", t4 - t3), ("observables", t5 - t4), ("mixing", t6 - t5)):
             timings[k] = timings.get(k, 0.0) + v
     row = dict(lam=info["lam"], number=info["number"], holes=info["holes"], Jbar=Jbar, roundoff=roundoff,
                populations=pops)
+    if p.measure:
+        row.update(suite)
+        row["r_Sigma"] = max(sigma)
+        row["r_lambda"] = abs(info["lam"] - 1.0)
     s.history.append(row)
     return row
 
@@ -196,7 +219,7 @@ class IterationGraph:
         self.p, self.s, self.ws = p, s, ws
         ctx = p.grid.ctx
         scba_iteration(p, s, ws)
-        nres = 5 + 1 + p.grid.Nb + len(p.mechanisms) + 8
+        nres = 5 + 1 + p.grid.Nb + len(p.mechanisms) + 5 + 3 * len(p.mechanisms) + 9 + 8
         self.buffer = torch.zeros(nres, dtype=torch.float64, device=ctx.device)
         self.stream = torch.cuda.Stream(ctx.device)
         torch.cuda.synchronize()
@@ -231,5 +254,9 @@ class IterationGraph:
         row = dict(lam=info[0], number=info[1], holes=info[2], Jbar=self.slots["Jbar"].read(host),
                    roundoff=max([r.read(host) for r in ro]) if isinstance(ro, list) and ro else 0.0,
                    populations=self.slots["populations"].read(host))
+        if self.slots["suite"] is not None:
+            row.update(zip(MEASURE_KEYS, self.slots["suite"].read(host)))
+            row["r_Sigma"] = max(self.slots["sigma"].read(host))
+            row["r_lambda"] = abs(info[0] - 1.0)
         self.s.history.append(row)
         return row

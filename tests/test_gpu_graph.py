@@ -53,7 +53,8 @@ def test_graph_replays_equal_eager(hip, lo_kind):
     for k in range(n_rep):
         row = graph.replay()
         ref = eager[1 + k]
-        for key in ("lam", "number", "holes", "Jbar", "roundoff"):
+        for key in ("lam", "number", "holes", "Jbar", "roundoff", "r_D", "r_A", "r_K", "r_PSD", "r_caus",
+                    "r_Sigma", "r_lambda"):
             assert row[key] == ref[key], (k, key, row[key], ref[key])
         assert np.array_equal(row["populations"], ref["populations"]), k
         # unrelated eager work and a synchronization between replays

@@ -280,13 +280,19 @@ def test_scba_iterations_gpu(hip, orc, lo_kind):
     runner = orc.ScbaOracle(full, state)
     ws = scba.ScbaWorkspace(p)
     for it in range(3):
-        ro = runner.iterate()
+        ro = runner.iterate(measure=True)
         rd = scba.scba_iteration(p, s, ws)
         assert abs(rd["lam"] - ro["lam"]) < TOL * abs(ro["lam"])
         assert abs(rd["Jbar"] - ro["Jbar"]) < TOL * max(abs(ro["Jbar"]), 1e-300) + 1e-14
         err = float(rel(rd["populations"], ro["populations"]))
         assert err < TOL
         assert rd["roundoff"] < 1e-12
+        # the fixed-point measurement (production_iteration.jl:102-166)
+        assert rd["r_D"] < 1e-13 and ro["r_D"] < 1e-13
+        for k in ("r_A", "r_K", "r_Sigma"):
+            assert abs(rd[k] - ro[k]) <= 1e-9 * ro[k] + 1e-13, (it, k, rd[k], ro[k])
+        for k in ("r_PSD", "r_caus", "r_lambda"):
+            assert abs(rd[k] - ro[k]) <= 1e-12, (it, k, rd[k], ro[k])
     for k in ("GR", "GL", "GG"):
         err = float(rel(hip.to_host(getattr(s.green, k)), state[k]))
         assert err < TOL, k

@@ -1,6 +1,7 @@
 """Residual suite and self-energy residual on the device against the oracle
 (residuals.jl:537-865): random states whose residuals are O(1) (relative bar
-1e-10: eigenvalue-based terms carry the Jacobi method's rounding), closed
+1e-10: the device's Householder/Sturm eigenvalues and the oracle's Jacobi
+ones agree to rounding), closed
 forms (G = 0, exact inverses, known negative and causal-violating
 eigenvalues), non-finite blocks (Inf), every tile class N_b = 1..48, an SCBA
 state (near-zero residuals, absolute bar) and the deferred-results mode."""
@@ -207,3 +208,33 @@ def test_deferred_matches_eager(hip, orc):
     host = buf.cpu().numpy()
     assert list(d1.read(host)) == [eager[k] for k in KEYS]
     assert d2.read(host) == sig
+
+
+def test_mix_fields_residual_vs_oracle(hip, orc):
+    """Fused mixing + self-energy residuals: mixed fields equal the plain
+    mixing; each pair's ratio and the couple (sum) ratio equal the oracle's
+    selfenergy residual of the same (old, candidate) sums."""
+    import torch
+    rng = np.random.default_rng(31)
+    shape = (9, 3, 5, 5)
+    olds = [_rand(rng, shape) for _ in range(5)]
+    cands = [o + _rand(rng, shape, 0.1 * (k + 1)) for k, o in enumerate(olds)]
+    dev_o = [hip.to_device(x) for x in olds]
+    dev_o2 = [t.clone() for t in dev_o]
+    dev_c = [hip.to_device(x) for x in cands]
+    partner = [-1, 3, 4, 1, 2]  # couples (1,3) and (2,4)
+    ratios = hip.mix_fields_residual(list(zip(dev_o, dev_c)), 0.3, partner)
+    hip.mix_fields(list(zip(dev_o2, dev_c)), 0.3)
+    for a, b in zip(dev_o, dev_o2):
+        assert torch.equal(a, b)
+    z = np.zeros(shape, np.complex128)
+    single = lambda o, c: orc.selfenergy_residual([o, z, z], [c, z, z])  # one component: R slot
+    refs = [single(o, c) for o, c in zip(olds, cands)]
+    refs += [single(olds[1] + olds[3], cands[1] + cands[3]), single(olds[2] + olds[4], cands[2] + cands[4])]
+    assert len(ratios) == 7
+    for got, ref in zip(ratios, refs):
+        assert abs(got - ref) <= 1e-12 * ref, (got, ref)
+    bad = [t.clone() for t in dev_c]
+    bad[4][0, 0, 0, 0] = float("nan")
+    r = hip.mix_fields_residual(list(zip([t.clone() for t in dev_o2], bad)), 0.3, partner)
+    assert r[4] == np.inf and r[6] == np.inf and r[0] < np.inf

@@ -83,3 +83,32 @@ def test_lo_shift_then_contract_commutes(orc):
     a = orc.contract(K, orc.apply_shift(plan, G), prob["wk"], 1.0)
     b = orc.apply_shift(plan, orc.contract(K, G, prob["wk"], 1.0))
     assert rel(a, b) < 1e-14
+
+
+def test_measured_iteration_composes_the_residual_operators(orc):
+    """production_iteration.jl:102-166: the measurement of one iteration is
+    the residual suite of the new Green state against the totals of this
+    iteration and the unmixed candidate total, the largest self-energy
+    residual over embedding, plus, minus and every mechanism, and |lambda-1|;
+    restated here from the separately pinned operators."""
+    prob = fx.make_problem(N=4, NE=48, Nk=3, lo_kind="low_rank")
+    full, state, _ = fx.oracle_state(prob)
+    M = len(prob["mechanisms"])
+    fams = [[state["embR"], state["embL"], state["embG"]]] + \
+        [[state["scatR"][i], state["scatL"][i], state["scatG"][i]] for i in range(M)]
+    new = orc.current_green(prob["eps"], prob["h"], [f[0] for f in fams], [f[1] for f in fams],
+                            [f[2] for f in fams], prob["wE"], prob["wk"], prob["gs"], full["target"])
+    green = dict(GR=new["GR"], GL=new["GL"], GG=new["GG"], A=orc.spectral(new["GR"]))
+    scat, plus, minus, emb = fx.oracle_candidates(prob, full["kernels"], green)
+    tot = [np.asfortranarray(sum(f[k] for f in fams)) for k in range(3)]
+    candL = np.asfortranarray(emb[1] + sum(c[1] for c in scat))
+    ref = orc.residual_suite(prob["eps"], prob["h"], green["GR"], green["GL"], green["GG"], green["A"], *tot, candL)
+    pairs = [(fams[0], emb), ([state["plusR"], state["plusL"], state["plusG"]], plus),
+             ([state["minusR"], state["minusL"], state["minusG"]], minus)] + \
+        [(fams[1 + i], scat[i]) for i in range(M)]
+    ref["r_Sigma"] = max(orc.selfenergy_residual(o, c) for o, c in pairs)
+    ref["r_lambda"] = abs(new["lam"] - 1.0)
+    row = orc.ScbaOracle(full, state).iterate(measure=True)
+    for k, v in ref.items():
+        assert abs(row[k] - v) <= 1e-12 * abs(v) + 1e-15, (k, row[k], v)
+    assert row["r_D"] < 1e-13 and row["r_Sigma"] > 0
