@@ -411,9 +411,13 @@ __device__ inline double psd_from_tile(const c64* X, int N, int lane, EigWork* w
 
 // Causality of the retarded block S in tile X (residuals.jl:455-492):
 // lambda_max(Im S) / (||S||_2 + 1e-14) when positive, else 0; Inf for a
-// non-finite block. A Cholesky factorization of -Im S (negative definite
-// Im S, to rounding) settles the block at 0; ||S||_2^2 is the largest
-// eigenvalue of S S^dagger (that of S^dagger S). T: register scratch.
+// non-finite block. A Cholesky factorization of -Im S + delta I with
+// delta = 32 N eps ||Im S||_F / sqrt(N) settles the block at 0: it proves
+// lambda_max(Im S) < delta + (factorization error), i.e. a ratio below
+// (32 N + N(N+1)) eps (~2e-13 at N = 24), rounding noise for a causal
+// block whose Im S is singular (couplings of low rank). ||S||_2^2 is the
+// largest eigenvalue of S S^dagger (that of S^dagger S). T: register
+// scratch.
 template <int B, int LD>
 __device__ inline double causality_from_tile(const c64* X, int N, int lane, EigWork* w, c64 (&T)[B][B]) {
     c64 a[B][B];
@@ -425,7 +429,8 @@ __device__ inline double causality_from_tile(const c64* X, int N, int lane, EigW
     frexp(s.amax, &ex);
     const double sc = ldexp(1.0, -ex);
     scale_regs<B>(a, sc);
-    if (chol_screen<B>(a, N, lane, w, 0.0)) return 0.0;
+    const double delta = 32.0 * N * 2.220446049250313e-16 * sqrt(s.fro2 / N);
+    if (chol_screen<B>(a, N, lane, w, delta * sc)) return 0.0;
     load_herm<B, LD>(X, N, lane, 0.0, -1.0, a);  // Im S = (S - S^dagger) / (2i)
     scale_regs<B>(a, sc);
     double lmin, lmax;
@@ -495,7 +500,8 @@ __global__ __launch_bounds__(64) void k_residual_suite(ResArgs p) {
     const int lane = threadIdx.x;
     const int lr = lane & 7, lc = lane >> 3;
     const long long nblk = (long long)p.NE * p.Nk;
-    const long long blk = blockIdx.x;  // e + NE m
+    // XCD-aware: consecutive energies (sharing cache lines) on one XCD's L2
+    const long long blk = xcd_tile(blockIdx.x, nblk);  // e + NE m
     const int e = (int)(blk % p.NE), m = (int)(blk / p.NE);
     const int N = p.N;
     const size_t S = (size_t)nblk;
