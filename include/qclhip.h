@@ -360,6 +360,29 @@ int qclhip_selfenergy_residual(qclhip_ctx* ctx, int64_t n, const double* const* 
 int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
                                const double* const* cand, double alpha, const int* partner, double* ratios_host);
 
+/* ---- The Poisson step closing the SCBA loop (production_solver.jl:
+ * 1044-1047). All arrays are device memory. */
+/* _electron_density_bar (src/physics/observables.jl:1-26): nbar (Nz) from
+ * G< on the grid: I_ab = sum_{m,e} wE wk G<_{e,m,a,b}, nbar_g = gs/(2 pi)
+ * Re(-i sum_ab chi_ga I_ab conj(chi_gb)); chi Nz x Nb complex column-major. */
+int qclhip_electron_density(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GL, int Nz, const double* chi,
+                            double gs, double* nbar);
+/* solve_periodic_poisson (src/numerics/reference/poisson.jl:12-65): the
+ * bordered periodic finite-volume system solved by LU with partial
+ * pivoting on one workgroup; U (N) receives the Hartree energy; out_host =
+ * zeta, r_P, neutrality (deferrable, 3 doubles). DomainError for a
+ * singular system. N <= 8192. */
+int qclhip_periodic_poisson(qclhip_ctx* ctx, int N, const double* eps_r, const double* wx, const double* ND,
+                            double lambdaP, const double* nbar, double* U, double* out_host);
+/* project_hamiltonians (src/numerics/reference/basis.jl:288-308): h (device
+ * layout (Nb,Nb,Nk), i.e. reference (Nk,Nb,Nb)) = (S + Phi^dagger diag(U)
+ * Phi) + kin_m Minv with S = H0 - F (Z - z0) and kin_m = C kappa_m^2. */
+int qclhip_project_hamiltonians(qclhip_ctx* ctx, int Nz, int Nb, int Nk, const double* Phi, const double* U,
+                                const double* S, const double* kin, const double* Minv, double* h);
+/* Outer mixing (production_solver.jl:1046-1047): U <- (1-alpha) U + alpha
+ * cand; rU_host = max |cand - U_old| (deferrable, 1 double). */
+int qclhip_mix_potential(qclhip_ctx* ctx, int N, double* U, const double* cand, double alpha, double* rU_host);
+
 #ifdef __cplusplus
 }
 #endif

@@ -430,3 +430,47 @@ def selfenergy_residual(old, cand):
     arr = lambda xs: (_dp * 3)(*[P(x) for x in xs])
     lib().orc_selfenergy_residual.restype = ct.c_double
     return lib().orc_selfenergy_residual(ct.c_longlong(o[0].size), arr(o), arr(c))
+
+
+def electron_density(GL, wE, wk, chi, gs):
+    """_electron_density_bar: nbar on the position grid (Nz,)."""
+    NE, Nk, Nb, _ = GL.shape
+    chi = np.asfortranarray(chi, dtype=np.complex128)
+    out = np.zeros(chi.shape[0])
+    rc = lib().orc_electron_density(NE, Nk, Nb, chi.shape[0], P(np.asfortranarray(GL)), P(np.asarray(wE, float)),
+                                    P(np.asarray(wk, float)), P(chi), ct.c_double(gs), P(out))
+    _check(rc, "electron_density")
+    return out
+
+
+def periodic_poisson(eps_r, wx, ND, lambda_P, nbar):
+    """solve_periodic_poisson: (U, zeta, r_P, neutrality)."""
+    N = len(nbar)
+    arr = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    eps_r, wx, ND, nbar = arr(eps_r), arr(wx), arr(ND), arr(nbar)
+    U = np.zeros(N)
+    out = np.zeros(3)
+    rc = lib().orc_periodic_poisson(N, P(eps_r), P(wx), P(ND), ct.c_double(lambda_P), P(nbar), P(U), P(out))
+    _check(rc, "periodic_poisson")
+    return U, out[0], out[1], out[2]
+
+
+def project_hamiltonians(Phi, U, S, kin, Minv):
+    """project_hamiltonians with the static part S = H0 - F (Z - z0) given:
+    h (Nk, Nb, Nb)."""
+    Phi = np.asfortranarray(Phi, dtype=np.complex128)
+    Nz, Nb = Phi.shape
+    kin = np.ascontiguousarray(kin, dtype=np.float64)
+    h = np.zeros((len(kin), Nb, Nb), np.complex128, order="F")
+    rc = lib().orc_project_hamiltonians(Nz, Nb, len(kin), P(Phi), P(np.ascontiguousarray(U, dtype=np.float64)),
+                                        P(np.asfortranarray(S, dtype=np.complex128)), P(kin),
+                                        P(np.asfortranarray(Minv, dtype=np.complex128)), P(h))
+    _check(rc, "project_hamiltonians")
+    return h
+
+
+def mix_potential(U, cand, alpha):
+    """U <- (1-alpha) U + alpha cand in place; returns max |cand - U_old|."""
+    lib().orc_mix_potential.restype = ct.c_double
+    return lib().orc_mix_potential(len(U), P(U), P(np.ascontiguousarray(cand, dtype=np.float64)),
+                                   ct.c_double(alpha))

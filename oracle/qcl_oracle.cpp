@@ -1307,3 +1307,151 @@ double orc_selfenergy_residual(long long n, const double* const* old_, const dou
 }
 
 }  // extern "C"
+
+extern "C" {
+
+/* _electron_density_bar (src/physics/observables.jl:1-26): integrate every
+ * coherence I_ab = sum_{m,e} wE_e wk_m G<_{e,m,a,b} (m outer, e inner), then
+ * nbar_g = gs/(2 pi) Re(-i sum_{b,a} chi_{g,a} I_ab conj(chi_{g,b})).
+ * chi: Nz x Nb complex, column-major. */
+int orc_electron_density(int NE, int Nk, int Nb, int Nz, const double* GL_, const double* wE, const double* wk,
+                         const double* chi_, double gs, double* nbar) {
+    const cd* GL = C(GL_);
+    const cd* chi = C(chi_);
+    std::vector<cd> I((size_t)Nb * Nb);
+    for (int b = 0; b < Nb; ++b)
+        for (int a = 0; a < Nb; ++a) {
+            cd v(0.0, 0.0);
+            for (int m = 0; m < Nk; ++m)
+                for (int e = 0; e < NE; ++e) v += wE[e] * wk[m] * GL[at4(e, m, a, b, NE, Nk, Nb)];
+            I[a + (size_t)Nb * b] = v;
+        }
+    const double pref = gs / (2.0 * M_PI);
+    for (int g = 0; g < Nz; ++g) {
+        cd v(0.0, 0.0);
+        for (int b = 0; b < Nb; ++b)
+            for (int a = 0; a < Nb; ++a)
+                v += chi[g + (size_t)Nz * a] * I[a + (size_t)Nb * b] * std::conj(chi[g + (size_t)Nz * b]);
+        nbar[g] = pref * (cd(0.0, -1.0) * v).real();
+    }
+    return 0;
+}
+
+/* solve_periodic_poisson (src/numerics/reference/poisson.jl:12-65): the
+ * cyclic finite-volume matrix L of -d/dx(eps_r du/dx) (harmonic-mean faces,
+ * dx = wx_1), b = -lambda_P (N_D - nbar), the bordered system
+ * [[L/sP, v], [v^T, 0]] [U; zeta] = [b/sP; 0] (sP the largest absolute row
+ * sum, v = 1/sqrt(N)) solved by LU with partial pivoting; out = zeta,
+ * r_P = ||L U - b|| / (||b|| + lambda_P 1e-14), neutrality
+ * |sum wx (N_D - nbar)| / sum wx N_D. Returns -2 for a singular system. */
+int orc_periodic_poisson(int N, const double* eps_r, const double* wx, const double* ND, double lambdaP,
+                         const double* nbar, double* U, double* out) {
+    const double dx = wx[0];
+    std::vector<double> face(N), L((size_t)N * N, 0.0);
+    for (int g = 0; g < N; ++g) {
+        const int gp = g == N - 1 ? 0 : g + 1;
+        face[g] = 2.0 * eps_r[g] * eps_r[gp] / (eps_r[g] + eps_r[gp]);
+    }
+    for (int g = 0; g < N; ++g) {
+        const int gm = g == 0 ? N - 1 : g - 1, gp = g == N - 1 ? 0 : g + 1;
+        L[g + (size_t)N * g] = (face[gm] + face[g]) / (dx * dx);
+        L[g + (size_t)N * gm] = -face[gm] / (dx * dx);
+        L[g + (size_t)N * gp] = -face[g] / (dx * dx);
+    }
+    double nd2d = 0.0, neut = 0.0;
+    for (int g = 0; g < N; ++g) {
+        nd2d += wx[g] * ND[g];
+        neut += wx[g] * (ND[g] - nbar[g]);
+    }
+    std::vector<double> b(N);
+    for (int g = 0; g < N; ++g) b[g] = -lambdaP * (ND[g] - nbar[g]);
+    double sP = 0.0;
+    for (int g = 0; g < N; ++g) {
+        double r = 0.0;
+        for (int j = 0; j < N; ++j) r += std::fabs(L[g + (size_t)N * j]);
+        sP = std::max(sP, r);
+    }
+    const int n = N + 1;
+    std::vector<double> M((size_t)n * n, 0.0), x(n);
+    const double v = 1.0 / std::sqrt((double)N);
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i) M[i + (size_t)n * j] = L[i + (size_t)N * j] / sP;
+    for (int i = 0; i < N; ++i) {
+        M[i + (size_t)n * N] = v;
+        M[N + (size_t)n * i] = v;
+        x[i] = b[i] / sP;
+    }
+    x[N] = 0.0;
+    // LU with partial pivoting (largest magnitude, first on ties), then the
+    // two triangular solves.
+    std::vector<int> piv(n);
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(M[i + (size_t)n * k]) > std::fabs(M[p + (size_t)n * k])) p = i;
+        piv[k] = p;
+        if (M[p + (size_t)n * k] == 0.0) return -2;
+        if (p != k)
+            for (int j = 0; j < n; ++j) std::swap(M[k + (size_t)n * j], M[p + (size_t)n * j]);
+        const double d = M[k + (size_t)n * k];
+        for (int i = k + 1; i < n; ++i) M[i + (size_t)n * k] /= d;
+        for (int j = k + 1; j < n; ++j) {
+            const double f = M[k + (size_t)n * j];
+            if (f == 0.0) continue;
+            for (int i = k + 1; i < n; ++i) M[i + (size_t)n * j] -= M[i + (size_t)n * k] * f;
+        }
+    }
+    for (int k = 0; k < n; ++k)
+        if (piv[k] != k) std::swap(x[k], x[piv[k]]);
+    for (int j = 0; j < n; ++j)
+        for (int i = j + 1; i < n; ++i) x[i] -= M[i + (size_t)n * j] * x[j];
+    for (int j = n - 1; j >= 0; --j) {
+        x[j] /= M[j + (size_t)n * j];
+        for (int i = 0; i < j; ++i) x[i] -= M[i + (size_t)n * j] * x[j];
+    }
+    for (int g = 0; g < N; ++g) U[g] = x[g];
+    double rn = 0.0, bn = 0.0;
+    for (int g = 0; g < N; ++g) {
+        double r = 0.0;
+        for (int j = 0; j < N; ++j) r += L[g + (size_t)N * j] * U[j];
+        rn += (r - b[g]) * (r - b[g]);
+        bn += b[g] * b[g];
+    }
+    out[0] = x[N];
+    out[1] = std::sqrt(rn) / (std::sqrt(bn) + lambdaP * 1e-14);
+    out[2] = std::fabs(neut) / nd2d;
+    return 0;
+}
+
+/* project_hamiltonians (src/numerics/reference/basis.jl:288-308):
+ * h_m = (S + Phi^dagger diag(U) Phi) + kin_m Minv, with S = H0 - F (Z - z0)
+ * precomputed by the caller and kin_m = C kappa_m^2. Phi: Nz x Nb complex
+ * column-major; S, Minv: Nb x Nb column-major; h: (Nk, Nb, Nb) column-major. */
+int orc_project_hamiltonians(int Nz, int Nb, int Nk, const double* Phi_, const double* U, const double* S_,
+                             const double* kin, const double* Minv_, double* h_) {
+    const cd *Phi = C(Phi_), *S = C(S_), *Minv = C(Minv_);
+    cd* h = C(h_);
+    for (int b = 0; b < Nb; ++b)
+        for (int a = 0; a < Nb; ++a) {
+            cd up(0.0, 0.0);
+            for (int g = 0; g < Nz; ++g)
+                up += (std::conj(Phi[g + (size_t)Nz * a]) * U[g]) * Phi[g + (size_t)Nz * b];
+            const cd st = S[a + (size_t)Nb * b] + up;
+            for (int m = 0; m < Nk; ++m)
+                h[m + (size_t)Nk * (a + (size_t)Nb * b)] = st + kin[m] * Minv[a + (size_t)Nb * b];
+        }
+    return 0;
+}
+
+/* Outer mixing of the Hartree energy (production_solver.jl:1046-1047):
+ * U <- (1 - alpha) U + alpha cand; returns max |cand - U_old|. */
+double orc_mix_potential(int N, double* U, const double* cand, double alpha) {
+    double r = 0.0;
+    for (int g = 0; g < N; ++g) {
+        r = std::max(r, std::fabs(cand[g] - U[g]));
+        U[g] = (1.0 - alpha) * U[g] + alpha * cand[g];
+    }
+    return r;
+}
+
+}  // extern "C"

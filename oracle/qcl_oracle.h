@@ -144,6 +144,15 @@ double orc_selfenergy_residual(long long n, const double* const* old_, const dou
 int orc_num_threads(void);
 void orc_set_num_threads(int n);
 
+/* Outer loop (the Poisson step closing the SCBA loop). */
+int orc_electron_density(int NE, int Nk, int Nb, int Nz, const double* GL, const double* wE, const double* wk,
+                         const double* chi, double gs, double* nbar);
+int orc_periodic_poisson(int N, const double* eps_r, const double* wx, const double* ND, double lambdaP,
+                         const double* nbar, double* U, double* out /* zeta, r_P, neutrality */);
+int orc_project_hamiltonians(int Nz, int Nb, int Nk, const double* Phi, const double* U, const double* S,
+                             const double* kin, const double* Minv, double* h);
+double orc_mix_potential(int N, double* U, const double* cand, double alpha);
+
 #ifdef __cplusplus
 }
 #endif

@@ -234,7 +234,9 @@ int qclhip_ctx_synchronize(qclhip_ctx* ctx) {
         return b > 0x7fffffffull ? 0x7fffffff : (int)b;
     }
     switch (st.code) {
-        case QCLHIP_ERR_DOMAIN: ctx->last_error = "domain error (number functional, lambda or Pauli capacity)"; break;
+        case QCLHIP_ERR_DOMAIN:
+            ctx->last_error = "domain error (number functional, lambda, Pauli capacity or a singular Poisson system)";
+            break;
         default: ctx->last_error = "device error"; break;
     }
     return st.code;

@@ -111,6 +111,10 @@ _SIGNATURES = {
     "qclhip_residual_suite": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),
                               ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp), _dp],
     "qclhip_selfenergy_residual": [_vp, ct.c_int64, ct.POINTER(_vp), ct.POINTER(_vp), _dp],
+    "qclhip_electron_density": [_vp, _vp, _vp, ct.c_int, _vp, ct.c_double, _vp],
+    "qclhip_periodic_poisson": [_vp, ct.c_int, _vp, _vp, _vp, ct.c_double, _vp, _vp, _dp],
+    "qclhip_project_hamiltonians": [_vp, ct.c_int, ct.c_int, ct.c_int, _vp, _vp, _vp, _vp, _vp, _vp],
+    "qclhip_mix_potential": [_vp, ct.c_int, _vp, _vp, ct.c_double, _dp],
     "qclhip_mix_fields_residual": [_vp, ct.c_int64, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_double,
                                    ct.POINTER(ct.c_int), _dp],
     "qclhip_batch_create": [_vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp)],

@@ -722,6 +722,75 @@ def selfenergy_residual(old, candidate):
     return Deferred(slot0, 1) if ctx.deferring else out.value
 
 
+# ---------------------------------------------------------------- outer loop
+def _dvec(x, device, dtype=torch.float64):
+    if isinstance(x, torch.Tensor):
+        if x.dtype != dtype or not x.is_contiguous() or x.device.type != "cuda":
+            raise _lib.ArgumentError("expected a contiguous device tensor of the right dtype")
+        return x
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device=device)
+
+
+def basis_to_device(chi, device="cuda"):
+    """Nz x Nb complex basis (numpy, any order) -> device column-major storage
+    (torch shape (Nb, Nz))."""
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(chi, np.complex128).T)).to(device)
+
+
+def electron_density(grid: Grid, GL, chi, gs: float):
+    """_electron_density_bar (observables.jl:1-26): nbar on the position grid.
+    chi: device basis from basis_to_device (Nb, Nz)."""
+    _check_field(grid, GL, "G<")
+    if chi.dtype != torch.complex128 or chi.dim() != 2 or chi.shape[0] != grid.Nb or not chi.is_contiguous():
+        raise _lib.DimensionMismatch("density basis must be a contiguous (Nb, Nz) complex128 device tensor")
+    Nz = chi.shape[1]
+    nbar = torch.empty(Nz, dtype=torch.float64, device=grid.ctx.device)
+    grid.ctx.call("qclhip_electron_density", grid.handle, _ptr(GL), Nz, _ptr(chi), ct.c_double(gs), _ptr(nbar))
+    return nbar
+
+
+def periodic_poisson(eps_r, wx, ND, lambda_P: float, nbar, U=None):
+    """solve_periodic_poisson (poisson.jl:41-65) on the device: returns
+    (U, zeta, r_P, neutrality), or (U, Deferred of 3) in deferred mode."""
+    dev = nbar.device
+    ctx = context(dev.index or 0)
+    er, w, nd = (_dvec(x, dev) for x in (eps_r, wx, ND))
+    N = nbar.numel()
+    if any(t.numel() != N for t in (er, w, nd)):
+        raise _lib.DimensionMismatch("Poisson inputs must have N_z entries")
+    U = torch.empty(N, dtype=torch.float64, device=dev) if U is None else U
+    out = (ct.c_double * 3)()
+    slot0 = ctx.result_slot() if ctx.deferring else 0
+    ctx.call("qclhip_periodic_poisson", N, _ptr(er), _ptr(w), _ptr(nd), ct.c_double(lambda_P), _ptr(nbar),
+             _ptr(U), out)
+    if ctx.deferring:
+        return U, Deferred(slot0, 3)
+    return U, out[0], out[1], out[2]
+
+
+def project_hamiltonians(Phi, U, S, kin, Minv, h=None):
+    """project_hamiltonians (basis.jl:288-308) with S = H0 - F (Z - z0) and
+    kin_m = C kappa_m^2: device h (Nb, Nb, Nk). Phi from basis_to_device; S,
+    Minv from matrix_to_device."""
+    dev = U.device
+    ctx = context(dev.index or 0)
+    Nb, Nz = Phi.shape
+    kin = _dvec(kin, dev)
+    Nk = kin.numel()
+    h = torch.empty((Nb, Nb, Nk), dtype=torch.complex128, device=dev) if h is None else h
+    ctx.call("qclhip_project_hamiltonians", Nz, Nb, Nk, _ptr(Phi), _ptr(U), _ptr(S), _ptr(kin), _ptr(Minv), _ptr(h))
+    return h
+
+
+def mix_potential(U, cand, alpha: float):
+    """U <- (1-alpha) U + alpha cand on the device; returns max |cand - U_old|."""
+    ctx = context(U.device.index or 0)
+    out = ct.c_double()
+    slot0 = ctx.result_slot() if ctx.deferring else 0
+    ctx.call("qclhip_mix_potential", U.numel(), _ptr(U), _ptr(cand), ct.c_double(alpha), ct.byref(out))
+    return Deferred(slot0, 1) if ctx.deferring else out.value
+
+
 def potential_gradient(hbar, Phi):
     """Host chain rule to a potential sampled on the real-space grid:
     dJ/dU_g = sum_m Re sum_ab conj(hbar_m[a,b]) conj(Phi[g,a]) Phi[g,b]
