@@ -193,7 +193,50 @@ def scba_bench(iters, warmup, adjoint=True):
             times.append(time.perf_counter() - t0)
         out["adjoint_iter_ms"] = (times[1] - times[0]) / 3 * 1e3
         out["adjoint_gradient_1iter_ms"] = times[0] * 1e3
+    out["outer"] = outer_bench(p, s)
     return out
+
+
+def outer_bench(p, s, Nz=512, reps=5):
+    """The Poisson step after an SCBA run at the same size: density of G<
+    (N_E N_k N_b^2 values), the bordered Poisson LU solve (N_z + 1 = 513),
+    potential mixing and the projection of N_k Hamiltonians; synthetic
+    orthonormal basis on N_z points."""
+    import numpy as np
+    import torch
+    from qclhip import api, outer
+    g = p.grid
+    rng = np.random.default_rng(5)
+    Nb, dx = g.Nb, 0.1
+    Q, _ = np.linalg.qr(rng.normal(size=(Nz, Nb)) + 1j * rng.normal(size=(Nz, Nb)))
+    ND = np.where(np.arange(Nz) % 64 < 8, 2.0, 0.0)
+    dev = lambda x: torch.as_tensor(x, device="cuda", dtype=torch.float64)
+    op = outer.OuterProblem(dev(rng.uniform(10, 13, Nz)), dev(np.full(Nz, dx)), dev(ND), 1.0,
+                            api.basis_to_device(Q / np.sqrt(dx)), api.basis_to_device(Q),
+                            api.matrix_to_device(np.diag(np.linspace(0, 0.4, Nb)) + 0j),
+                            dev(np.linspace(0, 1, g.Nk)), api.matrix_to_device(np.eye(Nb) + 0j),
+                            ND2D=float(dx * ND.sum()), Lp=Nz * dx)
+    U = torch.zeros(Nz, dtype=torch.float64, device="cuda")
+    outer.poisson_step(op, g, s.green.GL, U)
+    parts = {"density": 0.0, "poisson": 0.0, "mix_project": 0.0}
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nbar = api.electron_density(g, s.green.GL, op.chi, op.gs)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        cand, zeta, rP, neut = api.periodic_poisson(op.eps_r, op.wx, op.ND, op.lambda_P, nbar)
+        t2 = time.perf_counter()
+        api.mix_potential(U, cand, op.alpha_P)
+        api.project_hamiltonians(op.Phi, U, op.S, op.kin, op.Minv)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        parts["density"] += (t1 - t0) / reps * 1e3
+        parts["poisson"] += (t2 - t1) / reps * 1e3
+        parts["mix_project"] += (t3 - t2) / reps * 1e3
+    parts["step_ms"] = sum(parts.values())
+    parts["config"] = f"Nz={Nz} (bordered system {Nz + 1}), Nb={Nb}, Nk={g.Nk}, G< of NE={g.NE}"
+    return parts
 
 
 def graph_bench(iters=200):
@@ -338,6 +381,7 @@ def main():
         log("scba iteration: N=24 NE=2048 Nk=64, 3 mechanisms + embedding")
         sc = scba_bench(args.scba_iters, 1)
         log(f"scba {sc['iter_s']*1e3:.1f} ms/iter {sc['phases_ms']}")
+        log(f"poisson step {sc['outer']['step_ms']:.2f} ms {sc['outer']}")
         if "adjoint_iter_ms" in sc:
             log(f"adjoint {sc['adjoint_iter_ms']:.1f} ms/iter")
     gr = None
@@ -415,6 +459,7 @@ def main():
                                "self-energy residuals fused into the linear mixing); FP64")
         line["scba_last"] = {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in sc["last"].items()
                              if k != "populations"}
+        line["poisson_step"] = sc["outer"]
         if "adjoint_iter_ms" in sc:
             line["adjoint"] = {"iter_ms": sc["adjoint_iter_ms"], "gradient_with_1_iter_ms": sc["adjoint_gradient_1iter_ms"],
                                "config": "N=24 NE=2048 Nk=64, same problem as the SCBA iteration: one damped "

@@ -356,7 +356,9 @@ int qclhip_selfenergy_residual(qclhip_ctx* ctx, int64_t n, const double* const* 
  * alone (pair order), then, for each p with partner[p] > p (ascending p),
  * the sum of pairs p and partner[p] (the embedding = plus + minus);
  * partner[p] = -1 for none, symmetric otherwise. Inf for a non-finite term.
- * Same traffic as the mixing alone. Deferrable (npairs + couples doubles). */
+ * Same traffic as the mixing alone. alpha = 0 measures only (fields are
+ * not written; the residuals before an Anderson step). Deferrable
+ * (npairs + couples doubles). */
 int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
                                const double* const* cand, double alpha, const int* partner, double* ratios_host);
 
@@ -371,7 +373,7 @@ int qclhip_electron_density(qclhip_ctx* ctx, const qclhip_grid* grid, const doub
  * bordered periodic finite-volume system solved by LU with partial
  * pivoting on one workgroup; U (N) receives the Hartree energy; out_host =
  * zeta, r_P, neutrality (deferrable, 3 doubles). DomainError for a
- * singular system. N <= 8192. */
+ * singular system. N <= 4096. */
 int qclhip_periodic_poisson(qclhip_ctx* ctx, int N, const double* eps_r, const double* wx, const double* ND,
                             double lambdaP, const double* nbar, double* U, double* out_host);
 /* project_hamiltonians (src/numerics/reference/basis.jl:288-308): h (device
@@ -382,6 +384,30 @@ int qclhip_project_hamiltonians(qclhip_ctx* ctx, int Nz, int Nb, int Nk, const d
 /* Outer mixing (production_solver.jl:1046-1047): U <- (1-alpha) U + alpha
  * cand; rU_host = max |cand - U_old| (deferrable, 1 double). */
 int qclhip_mix_potential(qclhip_ctx* ctx, int N, double* U, const double* cand, double alpha, double* rU_host);
+
+/* ---- Anderson mixing (production_mixing.jl:135-317). A history entry is
+ * one field per (family, component) of the ordered families (mechanisms,
+ * plus, minus; R, <, >): nfields pointers, entry-major for histories. */
+/* _anderson_append! (:195-229) device part: saved_f <- state_f,
+ * resid_f <- cand_f - state_f, and for every field the real inner products
+ * with the nold older residuals and with itself: dots_host[f*(nold+1) + h]
+ * = Re <old_h,f, resid_f>, [f*(nold+1) + nold] = ||resid_f||^2 (the host
+ * sums them over fields in family and component order). nold <= 15. */
+int qclhip_anderson_append(qclhip_ctx* ctx, int64_t n, int nfields, const double* const* state,
+                           const double* const* cand, double* const* saved, double* const* resid, int nold,
+                           const double* const* old_resid, double* dots_host);
+/* The Anderson linear combination (:256-268): out_f = sum_h c_h (X_h,f +
+ * damping R_h,f) in history order; count <= 16. */
+int qclhip_anderson_combine(qclhip_ctx* ctx, int64_t n, int nfields, int count, const double* coef_host,
+                            double damping, const double* const* X, const double* const* R, double* const* out);
+/* _anderson_guard (:36-111): checks every (family, e, m) block of the nfam
+ * families (R, <, >); first_failure = one-based index of the first failing
+ * block in the reference's order (family, e, m fastest), 0 when all pass;
+ * code = its failure code 1..5 (non-finite, causality of Im S^R, Keldysh
+ * anti-Hermitian part, Keldysh positivity, S^R - S^A = S> - S<). */
+int qclhip_anderson_guard(qclhip_ctx* ctx, const qclhip_grid* grid, int nfam, const double* const* famR,
+                          const double* const* famL, const double* const* famG, int64_t* first_failure,
+                          int* code);
 
 #ifdef __cplusplus
 }

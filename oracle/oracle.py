@@ -405,6 +405,36 @@ class ScbaOracle:
             row.update(zip(("r_D", "r_A", "r_K", "r_PSD", "r_caus", "r_Sigma", "r_lambda"), meas))
         return row
 
+    def state_fields(self):
+        """The ordered families' fields (mechanisms R,<,>, plus, minus)."""
+        st = self.state
+        out = []
+        for i in range(len(self.prob["kernels"])):
+            out += [st["scatR"][i], st["scatL"][i], st["scatG"][i]]
+        return out + [st["plusR"], st["plusL"], st["plusG"], st["minusR"], st["minusL"], st["minusG"]]
+
+    def iterate_anderson(self, mixer):
+        """One iteration with Anderson mixing instead of the linear one: the
+        candidates are produced unmixed, then mixed by mixer (AndersonOracle)
+        with alpha_mix, and the embedding sum is refreshed."""
+        M = len(self.prob["kernels"])
+        NE, Nk, Nb = self.prob["NE"], self.prob["Nk"], self.prob["Nb"]
+        cand = [field(NE, Nk, Nb) for _ in range(3 * (M + 2))]
+        out = np.zeros(5)
+        pops = np.zeros(Nb)
+        meas = np.zeros(7)
+        rc = lib().orc_scba_iteration_candidates(ct.byref(self._p), ct.byref(self._s), P(out), P(pops), P(meas),
+                                                 _ptrs(cand))
+        _check(rc, "scba_iteration_candidates")
+        info = mixer.mix(self.state_fields(), cand, self.prob["alpha_mix"], M + 2)
+        st = self.state
+        for k, c in zip(("R", "L", "G"), range(3)):
+            st["emb" + k][...] = st["plus" + k] + st["minus" + k]
+        row = dict(lam=out[0], Jbar=out[1], roundoff=out[2], number=out[3], holes=out[4], populations=pops)
+        row.update(zip(("r_D", "r_A", "r_K", "r_PSD", "r_caus", "r_Sigma", "r_lambda"), meas))
+        row["anderson"] = info
+        return row
+
 def hermitian_eigvals(H):
     H = np.asfortranarray(H, dtype=np.complex128)
     n = H.shape[0]
@@ -474,3 +504,121 @@ def mix_potential(U, cand, alpha):
     lib().orc_mix_potential.restype = ct.c_double
     return lib().orc_mix_potential(len(U), P(U), P(np.ascontiguousarray(cand, dtype=np.float64)),
                                    ct.c_double(alpha))
+
+
+def _ptrs(arrs):
+    return (_dp * len(arrs))(*[P(a) for a in arrs])
+
+
+def anderson_inner(a, b):
+    """_anderson_inner over equal-length lists of fields."""
+    lib().orc_anderson_inner.restype = ct.c_double
+    return lib().orc_anderson_inner(ct.c_longlong(a[0].size), len(a), _ptrs(a), _ptrs(b))
+
+
+def anderson_combine(coef, damping, X, R, out):
+    """out_f = sum_h c_h (X[h][f] + damping R[h][f]); X, R lists of entries."""
+    nf = len(out)
+    flat = lambda H: [f for entry in H for f in entry]
+    lib().orc_anderson_combine(ct.c_longlong(out[0].size), nf, len(coef), P(np.asarray(coef, float)),
+                               ct.c_double(damping), _ptrs(flat(X)), _ptrs(flat(R)), _ptrs(out))
+
+
+def anderson_guard(families):
+    """_anderson_guard over (R, L, G) families: (first_failure, code)."""
+    NE, Nk, Nb, _ = families[0][0].shape
+    first, code = ct.c_longlong(), ct.c_int()
+    lib().orc_anderson_guard(NE, Nk, Nb, len(families), _ptrs([f[0] for f in families]),
+                             _ptrs([f[1] for f in families]), _ptrs([f[2] for f in families]),
+                             ct.byref(first), ct.byref(code))
+    return first.value, code.value
+
+
+def anderson_coefficients(gram, regularization):
+    """_anderson_coefficients (production_mixing.jl:160-187)."""
+    count = gram.shape[0]
+    if count == 1:
+        return np.ones(1)
+    g = gram.copy()
+    scale = max(np.max(np.abs(np.diag(g))), np.finfo(float).eps)
+    g[np.diag_indices(count)] += regularization * scale
+    lam, V = np.linalg.eigh(g / scale)
+    cutoff = max(regularization, np.sqrt(np.finfo(float).eps))
+    proj = V.T @ np.ones(count)
+    weights = np.array([proj[i] / lam[i] if lam[i] > cutoff else 0.0 for i in range(count)])
+    c = V @ weights
+    den = np.sum(c)
+    if not (np.isfinite(den) and abs(den) > np.finfo(float).eps):
+        raise ValueError("Anderson history is rank deficient")
+    c = c / den
+    if not np.all(np.isfinite(c)):
+        raise ValueError("Anderson coefficient solve produced non-finite values")
+    return c
+
+
+class AndersonOracle:
+    """_anderson_mix! (production_mixing.jl:195-317) over numpy fields: the
+    history of saved states and residuals, the Gram matrix, eviction and
+    reset, the bounded coefficients, the combination and the guard with the
+    convex fallback."""
+
+    def __init__(self, depth=5, damping=1.0, regularization=1e-12):
+        self.depth, self.damping, self.reg = depth, damping, regularization
+        self.states, self.residuals = [], []
+        self.gram = np.zeros((0, 0))
+
+    def clear(self):
+        self.states, self.residuals = [], []
+        self.gram = np.zeros((0, 0))
+
+    def append(self, state, cand):
+        prior = None if not self.residuals else self.gram[-1, -1]
+        if len(self.states) >= self.depth:
+            self.states.pop(0)
+            self.residuals.pop(0)
+            self.gram = self.gram[1:, 1:]
+        saved = [np.array(x, order="F") for x in state]
+        resid = [np.asfortranarray(c - x) for c, x in zip(cand, state)]
+        norm = anderson_inner(resid, resid)
+        eps = np.finfo(float).eps
+        reset = prior is not None and norm > (1 + 64 * eps) * max(prior, np.finfo(float).tiny)
+        if reset:
+            self.clear()
+        count = len(self.states) + 1
+        g = np.zeros((count, count))
+        if count > 1:
+            g[:-1, :-1] = self.gram
+        for i in range(count - 1):
+            g[i, -1] = g[-1, i] = anderson_inner(self.residuals[i], resid)
+        g[-1, -1] = norm
+        self.gram = g
+        self.states.append(saved)
+        self.residuals.append(resid)
+        return reset
+
+    def mix(self, state, cand, alpha, nfam):
+        """state, cand: field lists (families x R,L,G); state updated in place."""
+        reset = self.append(state, cand)
+        reason = "residual_increase" if reset else "none"
+        try:
+            c = anderson_coefficients(self.gram, self.reg)
+        except ValueError:
+            reason = "coefficient_solve"
+            c = np.r_[np.zeros(len(self.residuals) - 1), 1.0]
+        if np.sum(np.abs(c)) > 10 or not np.all(np.isfinite(c)):
+            reason = "coefficient_bound"
+            c = np.r_[np.zeros(len(self.residuals) - 1), 1.0]
+        damping = alpha * self.damping
+        anderson_combine(c, damping, self.states, self.residuals, state)
+        fams = [state[3 * f:3 * f + 3] for f in range(nfam)]
+        first, code = anderson_guard(fams)
+        accepted = first == 0 and np.count_nonzero(c) > 1
+        if first != 0:
+            x, r = self.states[-1], self.residuals[-1]
+            for out, xf, rf in zip(state, x, r):
+                out[...] = xf + damping * rf
+            self.clear()
+            reason = "causality_guard"
+            if anderson_guard(fams)[0] != 0:
+                raise ValueError("convex Anderson fallback is not admissible")
+        return dict(coefficients=c, reset_reason=reason, accepted=accepted, first_failure=first, code=code)

@@ -948,7 +948,7 @@ void orc_seed_lesser(int NE, int Nk, int Nb, const double* GR_, const double* ep
 // round-off witness, embedding), observables from the current state, linear
 // mixing of every family, embedding total = plus + minus.
 static int scba_iteration_core(const orc_problem* p, orc_state* s, double* out, double* populations,
-                               double* meas) {
+                               double* meas, double* const* cand_out) {
     const int NE = p->NE, Nk = p->Nk, Nb = p->Nb, M = p->nmech;
     const size_t n = (size_t)NE * Nk * Nb * Nb;
     const long long ncols = (long long)Nk * Nb * Nb;
@@ -1050,6 +1050,27 @@ static int scba_iteration_core(const orc_problem* p, orc_state* s, double* out, 
     double Jbar = orc_boundary_flux_bar(NE, Nk, Nb, p->eps, p->wE, p->wk, p->gs, s->plusL, s->plusG,
                                         s->GL, s->GG);
     if (populations) orc_state_populations(NE, Nk, Nb, p->wE, p->wk, p->gs, p->target, s->GL, populations);
+    if (cand_out) {
+        // Hand the unmixed candidates over (mechanisms R,<,>, then plus, minus)
+        // for an outside mixing step; the state keeps its self-energies.
+        for (int i = 0; i < M; ++i) {
+            std::copy(cR[i].begin(), cR[i].end(), cand_out[3 * i + 0]);
+            std::copy(cL[i].begin(), cL[i].end(), cand_out[3 * i + 1]);
+            std::copy(cG[i].begin(), cG[i].end(), cand_out[3 * i + 2]);
+        }
+        for (int k = 0; k < 3; ++k) {
+            std::copy(cp[k].begin(), cp[k].end(), cand_out[3 * M + k]);
+            std::copy(cm[k].begin(), cm[k].end(), cand_out[3 * M + 3 + k]);
+        }
+        if (out) {
+            out[0] = o3[0];
+            out[1] = Jbar;
+            out[2] = roundoff;
+            out[3] = o3[1];
+            out[4] = o3[2];
+        }
+        return 0;
+    }
     // Linear mixing (production_solver.jl:819-832).
     for (int i = 0; i < M; ++i) {
         orc_mix((long long)n, s->scatR[i], cR[i].data(), p->alpha_mix);
@@ -1075,12 +1096,17 @@ static int scba_iteration_core(const orc_problem* p, orc_state* s, double* out, 
 }
 
 int orc_scba_iteration(const orc_problem* p, orc_state* s, double* out, double* populations) {
-    return scba_iteration_core(p, s, out, populations, nullptr);
+    return scba_iteration_core(p, s, out, populations, nullptr, nullptr);
 }
 
 int orc_scba_iteration_measured(const orc_problem* p, orc_state* s, double* out, double* populations,
                                 double* meas) {
-    return scba_iteration_core(p, s, out, populations, meas);
+    return scba_iteration_core(p, s, out, populations, meas, nullptr);
+}
+
+int orc_scba_iteration_candidates(const orc_problem* p, orc_state* s, double* out, double* populations,
+                                  double* meas, double* const* cand_out) {
+    return scba_iteration_core(p, s, out, populations, meas, cand_out);
 }
 
 }  // extern "C"
@@ -1452,6 +1478,125 @@ double orc_mix_potential(int N, double* U, const double* cand, double alpha) {
         U[g] = (1.0 - alpha) * U[g] + alpha * cand[g];
     }
     return r;
+}
+
+}  // extern "C"
+
+namespace {
+// ||X||_2 of an n x n block: sqrt(lambda_max(X^dagger X)).
+double opnorm2(int n, const cd* X) {
+    std::vector<cd> T((size_t)n * n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            cd v(0.0, 0.0);
+            for (int c = 0; c < n; ++c) v += std::conj(X[c + (size_t)n * i]) * X[c + (size_t)n * j];
+            T[i + (size_t)n * j] = v;
+        }
+    std::vector<double> w(n);
+    herm_eigvals(n, T.data(), w.data());
+    return std::sqrt(std::max(0.0, w[n - 1]));
+}
+
+// _anderson_block_failure! (production_mixing.jl:36-80), exact norms and
+// eigenvalues: 0 admissible, else the first failing code.
+int anderson_block_failure(int n, const cd* R, const cd* L, const cd* G) {
+    const double eps = 2.220446049250313e-16, tiny = 2.2250738585072014e-308;
+    auto budget = [&](double norm) { return 64.0 * n * eps * std::max(norm, 1.0); };
+    if (!all_finite(n * n, R)) return 1;
+    std::vector<cd> H((size_t)n * n), D((size_t)n * n);
+    std::vector<double> w(n);
+    for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a)
+            H[a + (size_t)n * b] = (R[a + (size_t)n * b] - std::conj(R[b + (size_t)n * a])) / cd(0.0, 2.0);
+    const double rnorm = opnorm2(n, R);
+    herm_eigvals(n, H.data(), w.data());
+    if (!(w[n - 1] <= 64.0 * eps * std::max(rnorm, tiny))) return 2;
+    if (!all_finite(n * n, L) || !all_finite(n * n, G)) return 1;
+    const cd* blocks[2] = {L, G};
+    const cd factors[2] = {cd(0.0, -1.0), cd(0.0, 1.0)};
+    double norms[2];
+    for (int k = 0; k < 2; ++k) {
+        const cd* X = blocks[k];
+        const cd f = factors[k];
+        for (int i = 0; i < n * n; ++i) D[i] = f * X[i];
+        norms[k] = opnorm2(n, D.data());
+        const double bud = budget(norms[k]);
+        double an = 0.0;
+        for (int b = 0; b < n; ++b)
+            for (int a = 0; a < n; ++a) {
+                const cd value = f * X[a + (size_t)n * b], conjugate = std::conj(f * X[b + (size_t)n * a]);
+                an += std::norm(value - conjugate);
+                H[a + (size_t)n * b] = (value + conjugate) / 2.0;
+            }
+        if (!(std::sqrt(an) / 2.0 <= bud)) return 3;
+        herm_eigvals(n, H.data(), w.data());
+        if (!(w[0] >= -bud)) return 4;
+    }
+    double d5 = 0.0;
+    for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a)
+            d5 += std::norm(R[a + (size_t)n * b] - std::conj(R[b + (size_t)n * a]) -
+                            (G[a + (size_t)n * b] - L[a + (size_t)n * b]));
+    const double lnorm = opnorm2(n, L), gnorm = opnorm2(n, G);
+    if (!(std::sqrt(d5) <= budget(std::max(rnorm, std::max(lnorm, gnorm))))) return 5;
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+/* _anderson_inner (production_mixing.jl:135-144): sum over the fields, in
+ * order, of Re <a_f, b_f>. */
+double orc_anderson_inner(long long n, int nfields, const double* const* a_, const double* const* b_) {
+    double v = 0.0;
+    for (int f = 0; f < nfields; ++f) {
+        const cd *a = C(a_[f]), *b = C(b_[f]);
+        cd d(0.0, 0.0);
+        for (long long i = 0; i < n; ++i) d += std::conj(a[i]) * b[i];
+        v += d.real();
+    }
+    return v;
+}
+
+/* The Anderson combination (production_mixing.jl:256-268): out_f =
+ * sum_h c_h (X_h,f + damping R_h,f); X, R entry-major (count x nfields). */
+void orc_anderson_combine(long long n, int nfields, int count, const double* coef, double damping,
+                          const double* const* X_, const double* const* R_, double* const* out_) {
+    for (int f = 0; f < nfields; ++f) {
+        cd* out = C(out_[f]);
+        for (long long i = 0; i < n; ++i) {
+            cd v(0.0, 0.0);
+            for (int h = 0; h < count; ++h)
+                v += coef[h] * (C(X_[(size_t)h * nfields + f])[i] + damping * C(R_[(size_t)h * nfields + f])[i]);
+            out[i] = v;
+        }
+    }
+}
+
+/* _anderson_guard (production_mixing.jl:82-111): first failing block in the
+ * order family, e, m (one-based; 0 when all pass) and its code. */
+int orc_anderson_guard(int NE, int Nk, int Nb, int nfam, const double* const* R_, const double* const* L_,
+                       const double* const* G_, long long* first, int* code) {
+    const int n = Nb;
+    std::vector<cd> R((size_t)n * n), L((size_t)n * n), G((size_t)n * n);
+    *first = 0;
+    *code = 0;
+    long long index = 0;
+    for (int f = 0; f < nfam; ++f)
+        for (int e = 0; e < NE; ++e)
+            for (int m = 0; m < Nk; ++m) {
+                ++index;
+                load_block(C(R_[f]), e, m, NE, Nk, Nb, R.data());
+                load_block(C(L_[f]), e, m, NE, Nk, Nb, L.data());
+                load_block(C(G_[f]), e, m, NE, Nk, Nb, G.data());
+                const int c = anderson_block_failure(n, R.data(), L.data(), G.data());
+                if (c) {
+                    *first = index;
+                    *code = c;
+                    return 0;
+                }
+            }
+    return 0;
 }
 
 }  // extern "C"

@@ -153,6 +153,17 @@ int orc_project_hamiltonians(int Nz, int Nb, int Nk, const double* Phi, const do
                              const double* kin, const double* Minv, double* h);
 double orc_mix_potential(int N, double* U, const double* cand, double alpha);
 
+/* The iteration up to the mixing: candidates (mechanisms R,<,> in order,
+ * then plus R,<,>, minus R,<,>) copied into cand_out, state not mixed. */
+int orc_scba_iteration_candidates(const orc_problem* p, orc_state* s, double* out, double* populations,
+                                  double* meas, double* const* cand_out);
+/* Anderson mixing primitives (production_mixing.jl:36-317). */
+double orc_anderson_inner(long long n, int nfields, const double* const* a, const double* const* b);
+void orc_anderson_combine(long long n, int nfields, int count, const double* coef, double damping,
+                          const double* const* X, const double* const* R, double* const* out);
+int orc_anderson_guard(int NE, int Nk, int Nb, int nfam, const double* const* R, const double* const* L,
+                       const double* const* G, long long* first, int* code);
+
 #ifdef __cplusplus
 }
 #endif

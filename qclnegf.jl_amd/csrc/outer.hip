@@ -61,6 +61,7 @@ __global__ void k_density_project(int Nz, int Nb, const c64* chi, const c64* I, 
 // global scratch), LU with partial pivoting (largest magnitude, first index
 // on ties), the two triangular solves, and the residuals.
 constexpr int kPoissonThreads = 1024;
+constexpr int kMaxPoisson = 4096;
 
 __device__ inline double block_sum(double v, double* red) {
     red[threadIdx.x] = v;
@@ -132,6 +133,8 @@ __global__ __launch_bounds__(kPoissonThreads) void k_periodic_poisson(int N, con
                                                                    DeviceStatus* status) {
     __shared__ double red[kPoissonThreads];
     __shared__ int redi[kPoissonThreads];
+    __shared__ int rows[kMaxPoisson + 1], cols[kMaxPoisson + 1];
+    __shared__ int nrow, ncol;
     const int n = N + 1, t = threadIdx.x;
     const double dx2 = wx[0] * wx[0];
     // sP: the largest absolute row sum of L
@@ -193,13 +196,24 @@ __global__ __launch_bounds__(kPoissonThreads) void k_periodic_poisson(int N, con
             }
         __syncthreads();
         const double d = M[k + (size_t)n * k];
-        for (int i = k + 1 + t; i < n; i += kPoissonThreads) M[i + (size_t)n * k] /= d;
+        if (t == 0) nrow = ncol = 0;
         __syncthreads();
-        const int m = n - k - 1;
-        for (long long f = t; f < (long long)m * m; f += kPoissonThreads) {
-            const int i = k + 1 + (int)(f % m), j = k + 1 + (int)(f / m);
-            const double fk = M[k + (size_t)n * j];
-            if (fk != 0.0) M[i + (size_t)n * j] -= M[i + (size_t)n * k] * fk;
+        // The update touches only rows with a nonzero multiplier and columns
+        // with a nonzero pivot-row entry (skipped terms are exact zeros):
+        // the system is tridiagonal-cyclic with a border, so both lists stay
+        // a few entries long.
+        for (int i = k + 1 + t; i < n; i += kPoissonThreads) {
+            const double mult = M[i + (size_t)n * k] / d;
+            M[i + (size_t)n * k] = mult;
+            if (mult != 0.0) rows[atomicAdd(&nrow, 1)] = i;
+        }
+        for (int j = k + 1 + t; j < n; j += kPoissonThreads)
+            if (M[k + (size_t)n * j] != 0.0) cols[atomicAdd(&ncol, 1)] = j;
+        __syncthreads();
+        const int nr = nrow, nc = ncol;
+        for (long long f = t; f < (long long)nr * nc; f += kPoissonThreads) {
+            const int i = rows[f % nr], j = cols[f / nr];
+            M[i + (size_t)n * j] -= M[i + (size_t)n * k] * M[k + (size_t)n * j];
         }
         __syncthreads();
     }
@@ -311,7 +325,7 @@ int qclhip_electron_density(qclhip_ctx* ctx, const qclhip_grid* grid, const doub
 int qclhip_periodic_poisson(qclhip_ctx* ctx, int N, const double* eps_r, const double* wx, const double* ND,
                             double lambdaP, const double* nbar, double* U, double* out_host) {
     if (!ctx || !eps_r || !wx || !ND || !nbar || !U || !out_host) return QCLHIP_ERR_ARGUMENT;
-    if (N < 1 || N > 8192) return fail(ctx, QCLHIP_ERR_ARGUMENT, "Poisson grid size must lie in 1..8192");
+    if (N < 1 || N > kMaxPoisson) return fail(ctx, QCLHIP_ERR_ARGUMENT, "Poisson grid size must lie in 1..4096");
     if (!std::isfinite(lambdaP)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "lambda_P must be finite");
     const size_t n = (size_t)N + 1;
     const size_t bytes = (n * n + n + 8) * sizeof(double) + n * sizeof(int);

@@ -5,468 +5,12 @@
 // One wavefront per (e,m) block. Block products reuse the lane-grid product
 // of the Keldysh kernels; the positivity and causality checks need Hermitian
 // extreme eigenvalues, computed by a Householder reduction to tridiagonal
-// form in registers and Sturm multisection (eig_core below), after a
+// form in registers and Sturm multisection (eig_core, wave_eig.hpp), after a
 // Cholesky screen that settles positive blocks.
 // Per-block values are reduced afterwards in a fixed order.
-#include "dk_common.hpp"
+#include "wave_eig.hpp"
 
 namespace qclhip {
-
-template <int B>
-__device__ inline void rprod(const c64* __restrict__ L, const c64* __restrict__ R, int N, int lane,
-                             c64 (&out)[B][B], bool radj) {
-    constexpr int LD = 8 * B + 1;
-    const int lr = lane & 7, lc = lane >> 3;
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) out[r][c] = cmk(0.0, 0.0);
-    for (int q = 0; q < N; ++q) {
-        c64 lv[B], rv[B];
-#pragma unroll
-        for (int r = 0; r < B; ++r) lv[r] = L[(lr * B + r) * LD + q];
-#pragma unroll
-        for (int c = 0; c < B; ++c) rv[c] = radj ? cconj(R[(lc * B + c) * LD + q]) : R[q * LD + lc * B + c];
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int c = 0; c < B; ++c) out[r][c] = cfma(out[r][c], lv[r], rv[c]);
-    }
-}
-
-template <int B>
-__device__ inline void rstore(c64* T, int N, int lane, const c64 (&v)[B][B]) {
-    constexpr int LD = 8 * B + 1;
-    const int lr = lane & 7, lc = lane >> 3;
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const int i = lr * B + r, j = lc * B + c;
-            if (i < N && j < N) T[i * LD + j] = v[r][c];
-        }
-}
-
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += shfl_xor_d(v, off);
-    return v;
-}
-__device__ inline double wave_max(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, shfl_xor_d(v, off));
-    return v;
-}
-__device__ inline double wave_min(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmin(v, shfl_xor_d(v, off));
-    return v;
-}
-
-// Squared Frobenius norm of a tile (optionally of tile - c I).
-template <int LD>
-__device__ inline double tile_frob2(const c64* T, int N, int lane, double cdiag) {
-    double s = 0.0;
-    for (int f = lane; f < N * N; f += 64) {
-        const int i = f / N, j = f % N;
-        c64 v = T[i * LD + j];
-        if (i == j) v.x -= cdiag;
-        s += v.x * v.x + v.y * v.y;
-    }
-    return wave_sum(s);
-}
-
-// Hermitian part of f X into H: H = (f X + (f X)^dagger) / 2; f in {1, -i, i}
-// as (fr, fi).
-// LDS vectors of one wave's tridiagonal reduction and Cholesky screen.
-struct EigWork {
-    c64 u[48];
-    c64 q[48];
-    double d[48];
-    double e2[48];
-};
-
-// Sums over the 8 lanes of one row group (lane bits 3..5) and of one column
-// group (lane bits 0..2) of the 8 x 8 lane grid.
-__device__ inline double sum_over_lc(double v) {
-    v += shfl_xor_d(v, 8);
-    v += shfl_xor_d(v, 16);
-    return v + shfl_xor_d(v, 32);
-}
-__device__ inline double sum_over_lr(double v) {
-    v += shfl_xor_d(v, 1);
-    v += shfl_xor_d(v, 2);
-    return v + shfl_xor_d(v, 4);
-}
-
-// Hermitian part of f X, f = fr + i fi, from an LDS tile into the lane-grid
-// registers (rows lr*B + r, columns lc*B + c; zero outside N x N).
-template <int B, int LD>
-__device__ inline void load_herm(const c64* X, int N, int lane, double fr, double fi, c64 (&a)[B][B]) {
-    const int lr = lane & 7, lc = lane >> 3;
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const int i = lr * B + r, j = lc * B + c;
-            if (i < N && j < N) {
-                const c64 x = X[i * LD + j], y = X[j * LD + i];
-                const double ax = fr * x.x - fi * x.y, ay = fr * x.y + fi * x.x;
-                const double bx = fr * y.x - fi * y.y, by = fr * y.y + fi * y.x;
-                a[r][c] = cmk(0.5 * (ax + bx), 0.5 * (ay - by));
-            } else {
-                a[r][c] = cmk(0.0, 0.0);
-            }
-        }
-}
-
-struct BlockScale {
-    bool bad;     // a non-finite entry
-    double amax;  // largest |Re|, |Im|
-    double fro2;  // squared Frobenius norm
-};
-
-// Wave-uniform scale figures of the N x N part of a register block.
-template <int B>
-__device__ inline BlockScale block_scale(const c64 (&a)[B][B], int N, int lane) {
-    const int lr = lane & 7, lc = lane >> 3;
-    double amax = 0.0, fro2 = 0.0, bad = 0.0;
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c)
-            if (lr * B + r < N && lc * B + c < N) {
-                const c64 v = a[r][c];
-                if (!(isfinite(v.x) && isfinite(v.y))) bad = 1.0;
-                amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
-                fro2 += v.x * v.x + v.y * v.y;
-            }
-    BlockScale s;
-    s.bad = wave_max(bad) != 0.0;
-    s.amax = wave_max(amax);
-    s.fro2 = wave_sum(fro2);
-    return s;
-}
-
-template <int B>
-__device__ inline void scale_regs(c64 (&a)[B][B], double s) {
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) a[r][c] = cmk(a[r][c].x * s, a[r][c].y * s);
-}
-
-// True when the Hermitian register block a + shift I (N x N part) has a
-// Cholesky factorization a + shift I = L L^H, i.e. every pivot of the
-// right-looking elimination is positive; a is destroyed. The column of each
-// step is passed through LDS.
-template <int B>
-__device__ inline bool chol_screen(c64 (&a)[B][B], int N, int lane, EigWork* w, double shift) {
-    const int lr = lane & 7, lc = lane >> 3;
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c)
-            if (lr * B + r == lc * B + c) a[r][c].x += shift;
-    for (int k = 0; k < N; ++k) {
-        const int kg = k / B, kc = k - kg * B;
-        if (lc == kg) {
-#pragma unroll
-            for (int c = 0; c < B; ++c) {
-                if (c != kc) continue;
-#pragma unroll
-                for (int r = 0; r < B; ++r) {
-                    const int i = lr * B + r;
-                    if (i >= k && i < N) w->u[i] = a[r][c];
-                }
-            }
-        }
-        wave_sync();
-        const double piv = w->u[k].x;
-        if (!(piv > 0.0)) {
-            wave_sync();
-            return false;
-        }
-        const double rs = 1.0 / sqrt(piv);
-        c64 li[B], lj[B];
-#pragma unroll
-        for (int r = 0; r < B; ++r) {
-            const int i = lr * B + r, j = lc * B + r;
-            li[r] = (i > k && i < N) ? cscale(rs, w->u[i]) : cmk(0.0, 0.0);
-            lj[r] = (j > k && j < N) ? cscale(rs, w->u[j]) : cmk(0.0, 0.0);
-        }
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int c = 0; c < B; ++c) a[r][c] = cfms(a[r][c], li[r], cconj(lj[c]));
-        wave_sync();  // the next step rewrites u
-    }
-    return true;
-}
-
-// Number of eigenvalues below x of the real symmetric tridiagonal matrix
-// with diagonal d and squared off-diagonal e2: the negative pivots of
-// LDL^T(T - x I), a zero pivot replaced by -pivmin (Kahan's count, as in
-// LAPACK's bisection).
-__device__ inline int sturm_count(const double* d, const double* e2, int N, double x, double pivmin) {
-    double q = d[0] - x;
-    if (fabs(q) < pivmin) q = -pivmin;
-    int cnt = q < 0.0;
-    for (int i = 1; i < N; ++i) {
-        q = (d[i] - x) - e2[i - 1] / q;
-        if (fabs(q) < pivmin) q = -pivmin;
-        cnt += q < 0.0;
-    }
-    return cnt;
-}
-
-// Narrow [a, b] to the sub-interval that holds the transition of a
-// monotone predicate sampled at x_j = a + (b - a)(j + 1)/33, j = 0..31
-// (bit j of mask set where the predicate holds).
-__device__ inline void multisect(double& a, double& b, unsigned mask) {
-    const double w = b - a;
-    if (mask == 0u) {
-        a = a + w * (32 / 33.0);
-        return;
-    }
-    const int f = __builtin_ctz(mask);
-    const double nb = a + w * ((f + 1) / 33.0);
-    if (f > 0) a = a + w * (f / 33.0);
-    b = nb;
-}
-
-// Smallest and largest eigenvalue of the finite Hermitian register block a
-// (N x N part; a is destroyed; entries outside are ignored), wave-uniform.
-// a is reduced to real symmetric tridiagonal form by N-2 Householder
-// reflections applied in registers on the 8 x 8 lane grid
-// (P = I - beta u u^H with u = x + e^{i arg x_0} ||x|| e_1, A <- P A P
-// through p = beta A u, q = p - (beta/2)(u^H p) u, A <- A - u q^H - q u^H),
-// and the two extreme eigenvalues are located by 32-point Sturm
-// multisection each (half-waves), to eps * ||T|| absolute. Callers scale a
-// to max |a_ij| ~ 1 first.
-template <int B>
-__device__ inline void eig_core(c64 (&a)[B][B], int N, int lane, EigWork* w, double& lmin, double& lmax) {
-    constexpr double kEps = 2.220446049250313e-16, kTiny = 2.2250738585072014e-308;
-    const int lr = lane & 7, lc = lane >> 3;
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c)
-            if (lr * B + r >= N || lc * B + c >= N) a[r][c] = cmk(0.0, 0.0);
-    for (int k = 0; k + 2 < N; ++k) {
-        const int kg = k / B, kc = k - kg * B;  // column k: lanes lc == kg, register column kc
-        double part = 0.0;
-        if (lc == kg) {
-#pragma unroll
-            for (int c = 0; c < B; ++c) {
-                if (c != kc) continue;
-#pragma unroll
-                for (int r = 0; r < B; ++r) {
-                    const int i = lr * B + r;
-                    if (i > k && i < N) {
-                        w->u[i] = a[r][c];
-                        part += a[r][c].x * a[r][c].x + a[r][c].y * a[r][c].y;
-                    }
-                    if (i == k) w->d[k] = a[r][c].x;
-                }
-            }
-        }
-        const double nrm2 = wave_sum(part);
-        wave_sync();
-        const c64 x0 = w->u[k + 1];
-        if (lane == 0) w->e2[k] = nrm2;
-        if (nrm2 == 0.0) {  // column already reduced
-            wave_sync();
-            continue;
-        }
-        const double ax0 = sqrt(x0.x * x0.x + x0.y * x0.y), xn = sqrt(nrm2);
-        const c64 ph = ax0 > 0.0 ? cmk(x0.x / ax0, x0.y / ax0) : cmk(1.0, 0.0);
-        const c64 u1 = cmk(x0.x + ph.x * xn, x0.y + ph.y * xn);
-        const double beta = 1.0 / (nrm2 + xn * ax0);
-        c64 ur[B], uc[B], p[B];
-#pragma unroll
-        for (int r = 0; r < B; ++r) {
-            const int i = lr * B + r;
-            ur[r] = i == k + 1 ? u1 : (i > k + 1 && i < N ? w->u[i] : cmk(0.0, 0.0));
-            const int j = lc * B + r;
-            uc[r] = j == k + 1 ? u1 : (j > k + 1 && j < N ? w->u[j] : cmk(0.0, 0.0));
-        }
-#pragma unroll
-        for (int r = 0; r < B; ++r) {
-            p[r] = cmk(0.0, 0.0);
-#pragma unroll
-            for (int c = 0; c < B; ++c) p[r] = cfma(p[r], a[r][c], uc[c]);
-        }
-        double kd = 0.0;
-#pragma unroll
-        for (int r = 0; r < B; ++r) {
-            const int i = lr * B + r;
-            p[r] = cmk(beta * sum_over_lc(p[r].x), beta * sum_over_lc(p[r].y));
-            if (i <= k || i >= N) p[r] = cmk(0.0, 0.0);
-            kd += ur[r].x * p[r].x + ur[r].y * p[r].y;  // Re(conj(u_i) p_i)
-        }
-        const double K = 0.5 * beta * sum_over_lr(kd);
-#pragma unroll
-        for (int r = 0; r < B; ++r) p[r] = cmk(p[r].x - K * ur[r].x, p[r].y - K * ur[r].y);  // q rows
-        if (lc == 0) {
-#pragma unroll
-            for (int r = 0; r < B; ++r)
-                if (lr * B + r < N) w->q[lr * B + r] = p[r];
-        }
-        wave_sync();
-#pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const int j = lc * B + c;
-            const c64 qc = (j > k && j < N) ? w->q[j] : cmk(0.0, 0.0);
-#pragma unroll
-            for (int r = 0; r < B; ++r) {
-                const int i = lr * B + r;
-                if (i > k && i < N && j > k && j < N) {
-                    // a -= u_i conj(q_j) + q_i conj(u_j)
-                    a[r][c] = cfms(a[r][c], ur[r], cconj(qc));
-                    a[r][c] = cfms(a[r][c], p[r], cconj(uc[c]));
-                }
-            }
-        }
-        wave_sync();  // the next step rewrites u and q
-    }
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const int i = lr * B + r, j = lc * B + c;
-            if (i == j && i < N && i >= N - 2) w->d[i] = a[r][c].x;
-            if (i == N - 1 && j == N - 2) w->e2[N - 2] = a[r][c].x * a[r][c].x + a[r][c].y * a[r][c].y;
-        }
-    wave_sync();
-    double e2max = 0.0, glo = INFINITY, ghi = -INFINITY;
-    for (int i = lane; i < N; i += 64) {
-        const double ri = (i > 0 ? sqrt(w->e2[i - 1]) : 0.0) + (i + 1 < N ? sqrt(w->e2[i]) : 0.0);
-        glo = fmin(glo, w->d[i] - ri);
-        ghi = fmax(ghi, w->d[i] + ri);
-        if (i + 1 < N) e2max = fmax(e2max, w->e2[i]);
-    }
-    glo = wave_min(glo);
-    ghi = wave_max(ghi);
-    e2max = wave_max(e2max);
-    const double pivmin = kTiny * fmax(1.0, e2max);
-    const double tnorm = fmax(fabs(glo), fabs(ghi));
-    const double widen = 2.0 * kEps * tnorm * N + 2.0 * pivmin;
-    double a0 = glo - widen, b0 = ghi + widen, a1 = a0, b1 = b0;
-    const int half = lane >> 5, jx = lane & 31;
-    for (int round = 0; round < 24; ++round) {
-        const double t0 = fmax(2.0 * kEps * fmax(fabs(a0), fabs(b0)), kEps * tnorm);
-        const double t1 = fmax(2.0 * kEps * fmax(fabs(a1), fabs(b1)), kEps * tnorm);
-        if (b0 - a0 <= t0 && b1 - a1 <= t1) break;
-        const double lo = half ? a1 : a0, hi = half ? b1 : b0;
-        const double x = lo + (hi - lo) * ((jx + 1) / 33.0);
-        const int cnt = sturm_count(w->d, w->e2, N, x, pivmin);
-        const unsigned long long m = __ballot(half ? cnt >= N : cnt >= 1);
-        multisect(a0, b0, (unsigned)(m & 0xffffffffull));
-        multisect(a1, b1, (unsigned)(m >> 32));
-    }
-    lmin = 0.5 * (a0 + b0);
-    lmax = 0.5 * (a1 + b1);
-    wave_sync();  // d and e2 are rewritten by the next call
-}
-
-// The positivity ratio of the reference's witness (positivity_diagnostics.jl:
-// 61-96): max(0, -lambda_min - 64 n eps max(|lambda|_max, 1)) /
-// max(|lambda|_max, floatmin).
-__device__ inline double psd_ratio(double lmin, double lbig, int N) {
-    if (!(isfinite(lmin) && isfinite(lbig))) return INFINITY;
-    const double budget = 64.0 * N * 2.220446049250313e-16 * fmax(lbig, 1.0);
-    return fmax(0.0, fmax(0.0, -lmin) - budget) / fmax(lbig, 2.2250738585072014e-308);
-}
-
-// Positivity ratio of the Hermitian part H of f X (tile X intact); Inf for
-// a non-finite block (residuals.jl:413-452). Screen first: the ratio is 0
-// whenever lambda_min(H) >= -budget, and max |lambda| >= ||H||_F / sqrt(N)
-// bounds the budget from below, so a Cholesky factorization of H + delta I
-// with delta = 32 N eps max(||H||_F / sqrt(N), 1) (half that bound) settles
-// the block; its backward error (<= N(N+1) eps ||H||) stays within the other
-// half of the budget. Otherwise the extreme eigenvalues are computed.
-template <int B, int LD>
-__device__ inline double psd_from_tile(const c64* X, int N, int lane, EigWork* w, double fr, double fi) {
-    constexpr double kEps = 2.220446049250313e-16;
-    c64 a[B][B];
-    load_herm<B, LD>(X, N, lane, fr, fi, a);
-    const BlockScale s = block_scale<B>(a, N, lane);
-    if (s.bad) return INFINITY;
-    if (s.amax == 0.0) return 0.0;
-    int ex;
-    frexp(s.amax, &ex);
-    const double sc = ldexp(1.0, -ex);
-    scale_regs<B>(a, sc);
-    const double delta = 32.0 * N * kEps * fmax(sqrt(s.fro2 / N), 1.0);
-    if (chol_screen<B>(a, N, lane, w, delta * sc)) return 0.0;
-    load_herm<B, LD>(X, N, lane, fr, fi, a);
-    scale_regs<B>(a, sc);
-    double lmin, lmax;
-    eig_core<B>(a, N, lane, w, lmin, lmax);
-    lmin = ldexp(lmin, ex);
-    lmax = ldexp(lmax, ex);
-    return psd_ratio(lmin, fmax(fabs(lmin), fabs(lmax)), N);
-}
-
-// Causality of the retarded block S in tile X (residuals.jl:455-492):
-// lambda_max(Im S) / (||S||_2 + 1e-14) when positive, else 0; Inf for a
-// non-finite block. A Cholesky factorization of -Im S + delta I with
-// delta = 32 N eps ||Im S||_F / sqrt(N) settles the block at 0: it proves
-// lambda_max(Im S) < delta + (factorization error), i.e. a ratio below
-// (32 N + N(N+1)) eps (~2e-13 at N = 24), rounding noise for a causal
-// block whose Im S is singular (couplings of low rank). ||S||_2^2 is the
-// largest eigenvalue of S S^dagger (that of S^dagger S). T: register
-// scratch.
-template <int B, int LD>
-__device__ inline double causality_from_tile(const c64* X, int N, int lane, EigWork* w, c64 (&T)[B][B]) {
-    c64 a[B][B];
-    load_herm<B, LD>(X, N, lane, 0.0, 1.0, a);  // Hermitian part of i S = -Im S
-    const BlockScale s = block_scale<B>(a, N, lane);
-    if (s.bad) return INFINITY;
-    if (s.amax == 0.0) return 0.0;
-    int ex;
-    frexp(s.amax, &ex);
-    const double sc = ldexp(1.0, -ex);
-    scale_regs<B>(a, sc);
-    const double delta = 32.0 * N * 2.220446049250313e-16 * sqrt(s.fro2 / N);
-    if (chol_screen<B>(a, N, lane, w, delta * sc)) return 0.0;
-    load_herm<B, LD>(X, N, lane, 0.0, -1.0, a);  // Im S = (S - S^dagger) / (2i)
-    scale_regs<B>(a, sc);
-    double lmin, lmax;
-    eig_core<B>(a, N, lane, w, lmin, lmax);
-    lmax = ldexp(lmax, ex);
-    if (!isfinite(lmax)) return INFINITY;
-    if (!(lmax > 0.0)) return 0.0;
-    rprod<B>(X, X, N, lane, T, true);
-    const BlockScale t = block_scale<B>(T, N, lane);
-    if (t.bad) return INFINITY;
-    int ex2;
-    frexp(t.amax, &ex2);
-    scale_regs<B>(T, ldexp(1.0, -ex2));
-    double s0, s1;
-    eig_core<B>(T, N, lane, w, s0, s1);
-    const double v = lmax / (sqrt(fmax(0.0, ldexp(s1, ex2))) + 1e-14);
-    return isfinite(v) ? v : INFINITY;
-}
-
-// Sum over the N x N part of |X_ij - T_ij|^2 (tile against registers).
-template <int B, int LD>
-__device__ inline double tile_reg_diff2(const c64* X, const c64 (&T)[B][B], int N, int lane) {
-    const int lr = lane & 7, lc = lane >> 3;
-    double s = 0.0;
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const int i = lr * B + r, j = lc * B + c;
-            if (i < N && j < N) {
-                const c64 d = csub(X[i * LD + j], T[r][c]);
-                s += d.x * d.x + d.y * d.y;
-            }
-        }
-    return wave_sum(s);
-}
 
 struct ResArgs {
     int NE, Nk, N;
@@ -720,7 +264,7 @@ __global__ __launch_bounds__(256) void k_mix_residual(long long n, MixResArgs a,
         const double dx = c1.x - o1.x, dy = c1.y - o1.y;
         acc[0] += dx * dx + dy * dy;
         acc[1] += c1.x * c1.x + c1.y * c1.y;
-        oa[i] = cmk(beta * o1.x + alpha * c1.x, beta * o1.y + alpha * c1.y);
+        if (alpha != 0.0) oa[i] = cmk(beta * o1.x + alpha * c1.x, beta * o1.y + alpha * c1.y);
         if (pb >= 0) {
             const c64 o2 = ob[i], c2 = cb[i];
             const double ex = c2.x - o2.x, ey = c2.y - o2.y;
@@ -731,7 +275,7 @@ __global__ __launch_bounds__(256) void k_mix_residual(long long n, MixResArgs a,
             const double tx = c1.x + c2.x, ty = c1.y + c2.y;
             acc[4] += sx * sx + sy * sy;
             acc[5] += tx * tx + ty * ty;
-            ob[i] = cmk(beta * o2.x + alpha * c2.x, beta * o2.y + alpha * c2.y);
+            if (alpha != 0.0) ob[i] = cmk(beta * o2.x + alpha * c2.x, beta * o2.y + alpha * c2.y);
         }
     }
     const int nacc = pb >= 0 ? 6 : 2;
@@ -877,7 +421,7 @@ int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* c
                                const double* const* cand, double alpha, const int* partner, double* ratios_host) {
     if (!ctx || !old_ || !cand || !partner || !ratios_host || n < 0) return QCLHIP_ERR_ARGUMENT;
     if (npairs < 1 || npairs > kMaxResPairs) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..32 mixing pairs");
-    if (!(alpha > 0 && alpha <= 1)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "mixing factor must lie in (0,1]");
+    if (!(alpha >= 0 && alpha <= 1)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "mixing factor must lie in [0,1]");
     MixResArgs a{};
     int nterms = npairs;
     a.ngroups = 0;

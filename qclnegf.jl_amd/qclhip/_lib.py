@@ -115,6 +115,12 @@ _SIGNATURES = {
     "qclhip_periodic_poisson": [_vp, ct.c_int, _vp, _vp, _vp, ct.c_double, _vp, _vp, _dp],
     "qclhip_project_hamiltonians": [_vp, ct.c_int, ct.c_int, ct.c_int, _vp, _vp, _vp, _vp, _vp, _vp],
     "qclhip_mix_potential": [_vp, ct.c_int, _vp, _vp, ct.c_double, _dp],
+    "qclhip_anderson_append": [_vp, ct.c_int64, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),
+                               ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp), _dp],
+    "qclhip_anderson_combine": [_vp, ct.c_int64, ct.c_int, ct.c_int, _dp, ct.c_double, ct.POINTER(_vp),
+                                ct.POINTER(_vp), ct.POINTER(_vp)],
+    "qclhip_anderson_guard": [_vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),
+                              ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int)],
     "qclhip_mix_fields_residual": [_vp, ct.c_int64, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_double,
                                    ct.POINTER(ct.c_int), _dp],
     "qclhip_batch_create": [_vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp)],

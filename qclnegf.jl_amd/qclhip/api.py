@@ -722,6 +722,60 @@ def selfenergy_residual(old, candidate):
     return Deferred(slot0, 1) if ctx.deferring else out.value
 
 
+# ---------------------------------------------------------------- Anderson
+def _fields(ts, n=None):
+    ts = list(ts)
+    n = ts[0].numel() if n is None else n
+    for t in ts:
+        if t.dtype != torch.complex128 or t.numel() != n or not t.is_contiguous():
+            raise _lib.ArgumentError("Anderson fields must be contiguous complex128 tensors of one size")
+    return (_vp * len(ts))(*[t.data_ptr() for t in ts]), n
+
+
+def anderson_append(state, cand, saved, resid, old_residuals):
+    """Device part of _anderson_append! (production_mixing.jl:195-229):
+    saved <- state, resid <- cand - state; returns dots (nfields, nold+1):
+    Re <old_h, resid> per field and entry, then ||resid||^2 per field."""
+    nf = len(state)
+    ps, n = _fields(state)
+    pc, _ = _fields(cand, n)
+    psv, _ = _fields(saved, n)
+    pr, _ = _fields(resid, n)
+    nold = len(old_residuals)
+    po = _fields([f for entry in old_residuals for f in entry], n)[0] if nold else None
+    dots = np.zeros(nf * (nold + 1))
+    ctx = context(state[0].device.index or 0)
+    ctx.call("qclhip_anderson_append", ct.c_int64(n), nf, ps, pc, psv, pr, nold, po,
+             dots.ctypes.data_as(_dp))
+    return dots.reshape(nf, nold + 1)
+
+
+def anderson_combine(coef, damping: float, X, R, out):
+    """out_f = sum_h c_h (X[h][f] + damping R[h][f]) (production_mixing.jl:256-268)."""
+    nf = len(out)
+    po, n = _fields(out)
+    px, _ = _fields([f for entry in X for f in entry], n)
+    pr, _ = _fields([f for entry in R for f in entry], n)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    ctx = context(out[0].device.index or 0)
+    ctx.call("qclhip_anderson_combine", ct.c_int64(n), nf, len(c), c.ctypes.data_as(_dp), ct.c_double(damping),
+             px, pr, po)
+
+
+def anderson_guard(grid: Grid, families):
+    """_anderson_guard (production_mixing.jl:82-111) over (R, L, G) device
+    families: (first_failure one-based or 0, code)."""
+    for fam in families:
+        for t in fam:
+            _check_field(grid, t, "self-energy")
+    nf = len(families)
+    arr = lambda k: (_vp * nf)(*[f[k].data_ptr() for f in families])
+    first, code = ct.c_int64(), ct.c_int()
+    grid.ctx.call("qclhip_anderson_guard", grid.handle, nf, arr(0), arr(1), arr(2), ct.byref(first),
+                  ct.byref(code))
+    return first.value, code.value
+
+
 # ---------------------------------------------------------------- outer loop
 def _dvec(x, device, dtype=torch.float64):
     if isinstance(x, torch.Tensor):

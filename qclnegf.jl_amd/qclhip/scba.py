@@ -1,7 +1,7 @@
 """Host driver of the production SCBA loop over the device operators.
 
 Mirrors solve_scba_production (src/numerics/optimized/production_solver.jl:
-21-846) with linear mixing: per iteration the phases :dyson,
+21-846) with linear or Anderson mixing: per iteration the phases :dyson,
 :candidate_scattering, :candidate_embedding, :observables and :mixing run on
 the device in that order; only scalars (lambda, J, round-off, populations)
 come back to the host.
@@ -40,6 +40,10 @@ class ScbaProblem:
     alpha_mix: float = 0.3
     verify_roundoff: bool = True
     measure: bool = True            # the fixed-point measurement of every iteration
+    mixing: str = "linear"          # "linear" or "anderson" (production_solver.jl:808-832)
+    anderson_depth: int = 5
+    anderson_damping: float = 1.0
+    anderson_regularization: float = 1e-12
 
     def __post_init__(self):
         g = self.grid
@@ -131,6 +135,12 @@ class ScbaWorkspace:
         self.scat = {m.name: new_family(g) for m in p.mechanisms}
         self.plus = new_family(g)
         self.minus = new_family(g)
+        self.anderson = None
+        if p.mixing == "anderson":
+            from .anderson import AndersonMixer
+            self.anderson = AndersonMixer(g, p.anderson_depth, p.anderson_damping, p.anderson_regularization)
+        elif p.mixing != "linear":
+            raise ValueError("mixing must be 'linear' or 'anderson'")
 
 
 MEASURE_KEYS = ("r_D", "r_A", "r_K", "r_PSD", "r_caus")
@@ -167,9 +177,17 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     for fam, cand in ((s.plus, ws.plus), (s.minus, ws.minus)):
         pairs += list(zip(fam.parts(), cand.parts()))
     sigma = None
-    if p.measure:
-        # plus X pairs with minus X: their sum is the embedding term
-        partner = [-1] * base + [base + 3, base + 4, base + 5, base, base + 1, base + 2]
+    anderson_info = None
+    # plus X pairs with minus X: their sum is the embedding term
+    partner = [-1] * base + [base + 3, base + 4, base + 5, base, base + 1, base + 2]
+    if ws.anderson is not None:
+        if g.ctx.deferring:
+            raise ValueError("Anderson mixing synchronizes with the host and cannot be deferred or captured")
+        if p.measure:
+            sigma = api.mix_fields_residual(pairs, 0.0, partner)  # residuals only, before the update
+        anderson_info = ws.anderson.mix([o for o, _ in pairs], [c for _, c in pairs], p.alpha_mix,
+                                        len(p.mechanisms) + 2)
+    elif p.measure:
         sigma = api.mix_fields_residual(pairs, p.alpha_mix, partner)
     else:
         api.mix_fields(pairs, p.alpha_mix)
@@ -187,6 +205,8 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
         row.update(suite)
         row["r_Sigma"] = max(sigma)
         row["r_lambda"] = abs(info["lam"] - 1.0)
+    if anderson_info is not None:
+        row["anderson"] = anderson_info
     s.history.append(row)
     return row
 

@@ -90,3 +90,56 @@ def test_mix_potential_vs_oracle(hip, orc):
     ro = orc.mix_potential(U, cand, 0.4)
     assert r == ro
     assert np.array_equal(Ud.cpu().numpy(), U)
+
+
+def test_outer_step_after_scba_vs_oracle(hip, orc):
+    """Two SCBA iterations on device and oracle, then one Poisson step on
+    each: density, mixed Hartree energy, residuals and the next h agree."""
+    import torch
+    import scba_fixture as fx
+    from qclhip import scba, outer
+    prob = fx.make_problem(N=4, NE=64, Nk=3)
+    full, state, _ = fx.oracle_state(prob)
+    g = hip.Grid(prob["eps"], prob["wE"], prob["wk"], prob["Nb"])
+    mechs = [scba.Mechanism(m["name"], hip.ProductionKernel(g, m["kind"], m["a"], m["b"], m["c"], rank=m["rank"]),
+                            m["qK"], m["is_lo"]) for m in prob["mechanisms"]]
+    p = scba.ScbaProblem(g, hip.h_to_device(prob["h"]), hip.matrix_to_device(prob["Tplus"]),
+                         hip.matrix_to_device(prob["Tminus"]), mechs, full["target"], prob["Ep"], prob["hwLO"],
+                         prob["NLO"], prob["alpha_mix"])
+    dev = hip.to_device
+    s = scba.ScbaState(None, {m["name"]: scba.Family(dev(state["scatR"][i]), dev(state["scatL"][i]),
+                                                      dev(state["scatG"][i]))
+                              for i, m in enumerate(prob["mechanisms"])},
+                       scba.Family(dev(state["plusR"]), dev(state["plusL"]), dev(state["plusG"])),
+                       scba.Family(dev(state["minusR"]), dev(state["minusL"]), dev(state["minusG"])))
+    runner = orc.ScbaOracle(full, state)
+    ws = scba.ScbaWorkspace(p)
+    for _ in range(2):
+        runner.iterate()
+        scba.scba_iteration(p, s, ws)
+    rng = np.random.default_rng(12)
+    Nz, Nb, dx = 96, prob["Nb"], 0.25
+    Q, _ = np.linalg.qr(rng.normal(size=(Nz, Nb)) + 1j * rng.normal(size=(Nz, Nb)))
+    Phi, chi = Q, Q / np.sqrt(dx)
+    er = rng.uniform(10, 13, Nz)
+    ND = np.where(np.arange(Nz) < 12, 4.0, 0.0)
+    S = np.diag(np.linspace(0, 0.3, Nb)) + 0j
+    Minv = np.eye(Nb) + 0j
+    kin = np.array([0.0, 0.05, 0.2])
+    U0 = 0.01 * np.cos(2 * np.pi * np.arange(Nz) / Nz)
+    op = outer.OuterProblem(torch.as_tensor(er, device="cuda"), torch.full((Nz,), dx, device="cuda", dtype=torch.float64),
+                            torch.as_tensor(ND, device="cuda"), 1.3, hip.basis_to_device(chi),
+                            hip.basis_to_device(Phi), hip.matrix_to_device(S), torch.as_tensor(kin, device="cuda"),
+                            hip.matrix_to_device(Minv), gs=2.0, alpha_P=0.4, ND2D=float(dx * ND.sum()), Lp=Nz * dx)
+    U = torch.as_tensor(U0, device="cuda")
+    row = outer.poisson_step(op, g, s.green.GL, U)
+    n_o = orc.electron_density(state["GL"], prob["wE"], prob["wk"], chi, 2.0)
+    assert float(rel(row["nbar"], n_o)) < 1e-12
+    cand, zeta, rP, neut = orc.periodic_poisson(er, np.full(Nz, dx), ND, 1.3, n_o)
+    U_o = U0.copy()
+    rU = orc.mix_potential(U_o, cand, 0.4)
+    assert float(rel(U.cpu().numpy(), U_o)) < 1e-11
+    assert abs(row["r_U"] - rU) <= 1e-11 * rU
+    assert abs(row["zeta"] - zeta) <= 1e-11 * abs(zeta) + 1e-15 and abs(row["r_P"] - rP) <= 1e-10 * rP + 1e-14
+    h_o = orc.project_hamiltonians(Phi, U_o, S, kin, Minv)
+    assert float(rel(np.asfortranarray(row["h"].cpu().numpy().T), h_o)) < 1e-12
