@@ -249,4 +249,66 @@ function state_populations(g::Grid, Gˡ, target::Real)
     return out
 end
 
+"""`production_residual_suite` (residuals.jl:537-772): (r_D, r_A, r_K, r_PSD, r_caus)."""
+function residual_suite(g::Grid, h, Gᴿ, Gˡ, Gᵍ, A, families, candidate_lesser)
+    n = length(families)
+    fr = Ptr{Cvoid}[_ptr(f[1]) for f in families]
+    fl = Ptr{Cvoid}[_ptr(f[2]) for f in families]
+    fg = Ptr{Cvoid}[_ptr(f[3]) for f in families]
+    cl = Ptr{Cvoid}[_ptr(c) for c in candidate_lesser]
+    out = zeros(Float64, 5)
+    check(g.ctx, ccall((:qclhip_residual_suite, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
+                        Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Float64}),
+                       g.ctx.handle, g.handle, _ptr(h), _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(A), n, fr, fl, fg,
+                       length(cl), cl, out), "residual_suite")
+    return (r_D = out[1], r_A = out[2], r_K = out[3], r_PSD = out[4], r_caus = out[5])
+end
+
+"""Linear mixing fused with `production_selfenergy_residual` of the unmixed
+pairs; `partner[p]` (0-based, -1 for none) couples plus and minus. α = 0
+measures only."""
+function mix_fields_residual!(ctx::Context, olds, cands, α::Real, partner::Vector{Cint})
+    po = Ptr{Cvoid}[_ptr(o) for o in olds]
+    pc = Ptr{Cvoid}[_ptr(c) for c in cands]
+    nterms = length(olds) + count(p -> partner[p] > p - 1, eachindex(partner))
+    out = zeros(Float64, nterms)
+    check(ctx, ccall((:qclhip_mix_fields_residual, libqclhip), Cint,
+                     (Ptr{Cvoid}, Int64, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cdouble, Ptr{Cint}, Ptr{Float64}),
+                     ctx.handle, length(first(olds)), length(olds), po, pc, α, partner, out), "mix_fields_residual")
+    return out
+end
+
+"""The Poisson step (production_solver.jl:1044-1047): density, bordered
+Poisson solve, U mixing and projected Hamiltonians; all arrays on the device."""
+function electron_density!(n̄, g::Grid, Gˡ, χ, gs::Real)
+    check(g.ctx, ccall((:qclhip_electron_density, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}),
+                       g.ctx.handle, g.handle, _ptr(Gˡ), size(χ, 1), _ptr(χ), gs, _ptr(n̄)), "electron_density")
+    return n̄
+end
+
+function periodic_poisson!(U, ctx::Context, εᵣ, wˣ, Nᴰ, λ_P::Real, n̄)
+    out = zeros(Float64, 3)
+    check(ctx, ccall((:qclhip_periodic_poisson, libqclhip), Cint,
+                     (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}),
+                     ctx.handle, length(n̄), _ptr(εᵣ), _ptr(wˣ), _ptr(Nᴰ), λ_P, _ptr(n̄), _ptr(U), out), "poisson")
+    return U, out[1], out[2], out[3]   # U, ζ, r_P, neutrality
+end
+
+"""`_anderson_guard` (production_mixing.jl:82-111): (first_failure, code)."""
+function anderson_guard(g::Grid, families)
+    n = length(families)
+    fr = Ptr{Cvoid}[_ptr(f[1]) for f in families]
+    fl = Ptr{Cvoid}[_ptr(f[2]) for f in families]
+    fg = Ptr{Cvoid}[_ptr(f[3]) for f in families]
+    first = Ref{Int64}(0)
+    code = Ref{Cint}(0)
+    check(g.ctx, ccall((:qclhip_anderson_guard, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
+                        Ref{Int64}, Ref{Cint}),
+                       g.ctx.handle, g.handle, n, fr, fl, fg, first, code), "anderson_guard")
+    return first[], code[]
+end
+
 end # module
