@@ -5,7 +5,8 @@
 // the zero-padded linear convolution of wE*Gamma with the kernel
 // 1/(2 pi lag de), lag = -(NE-1)..NE-1, through length-L FFTs
 // (L = nextpow2(2NE-1)), extracting y[e + NE - 1]. The round-off witness
-// repeats the transform with 2L and reports ||L2 - L|| / (||L|| + 1e-14).
+// evaluates the transform at 2L as well and reports
+// ||L2 - L|| / (||L|| + 1e-14) (see k_hilbert).
 //
 // One workgroup per column. Each thread keeps 8 complex values in registers
 // (T = L/8 threads, 16 values above L = 8192); every Stockham pass does
@@ -25,8 +26,11 @@ struct qclhip_hilbert {
     long long ncols = 0;
     int L = 0;
     int verify = 0;
-    qclhip::c64* twiddle = nullptr;  // exp(-2 pi i idx / Lmax), idx < Lmax
-    qclhip::c64* kf[2] = {nullptr, nullptr};  // kernel spectra for L and 2L
+    qclhip::c64* twiddle = nullptr;  // exp(-2 pi i idx / L), idx < L
+    // kernel spectrum for L, and (with the witness) the odd-frequency half
+    // of the 2L spectrum: DFT_L(k_n exp(-i pi n / L))
+    qclhip::c64* kf[2] = {nullptr, nullptr};
+    qclhip::c64* phase = nullptr;  // exp(-i pi n / L), n < L (witness)
     int Lmax = 0;
 };
 
@@ -193,14 +197,14 @@ struct HilbertArgs {
     const c64* SL;
     const c64* SG;
     c64* out;           // Lambda (mode 0) or S^R = Lambda - i Gamma/2 (mode 1)
-    c64* lambda_copy;   // optional copy of Lambda for the witness
-    const c64* lambda_ref;  // witness mode: Lambda of the L transform
-    double* partial;    // witness mode: [2][ncols] sums |L2-L|^2 and |L|^2
+    double* partial;    // witness: [2][ncols] sums |L2-L|^2 and |L|^2
     const c64* tw;
     int twstride;
     const c64* kf;
-    int mode;           // 0: Lambda, 1: S^R, 2: witness only, 3: adjoint of S^R
+    int mode;           // 0: Lambda, 1: S^R, 3: adjoint of S^R
     c64* out2;          // adjoint mode: S>_bar (accumulated); out holds S<_bar
+    const c64* kodd;    // witness: odd half of the 2L kernel spectrum (NULL: no witness)
+    const c64* phase;   // witness: exp(-i pi n / L)
 };
 
 __device__ inline c64 load_gamma(const HilbertArgs& a, size_t idx) {
@@ -210,6 +214,67 @@ __device__ inline c64 load_gamma(const HilbertArgs& a, size_t idx) {
     return cmk(-(g.y - l.y), g.x - l.x);
 }
 
+// Odd-frequency half of the 2L transform of one column (see k_hilbert):
+// x_n exp(-i pi n/L) through the L transform with the odd kernel spectrum;
+// writes exp(i pi n/L) y'_n / L to the output positions of the column.
+template <int L>
+__global__ __launch_bounds__(threads_for(L)) void k_hilbert_odd(HilbertArgs a) {
+    constexpr int V = vals_per_thread(L), T = threads_for(L);
+    constexpr int NPASS = npasses(L);
+    constexpr int R0 = fwd_radix(L, 0), RL = fwd_radix(L, NPASS - 1);
+    extern __shared__ __align__(16) double lds[];
+    const int t = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * a.NE;
+    const int NE = a.NE;
+    const double invL = 1.0 / (double)L;
+    double re[V], im[V];
+#pragma unroll
+    for (int f = 0; f < V; ++f) {
+        const int i = f / R0, r = f % R0;
+        const int n = t + i * T + r * (L / R0);
+        double xr = 0.0, xi = 0.0;
+        if (n < NE) {
+            const c64 gm = load_gamma(a, base + n);
+            const double we = a.wE[n];
+            const c64 ph = a.phase[n];
+            const double gr = we * gm.x, gi = we * gm.y;
+            xr = gr * ph.x - gi * ph.y;
+            xi = gr * ph.y + gi * ph.x;
+        }
+        re[f] = xr;
+        im[f] = xi;
+    }
+    StockhamPasses<L, 0, false>::run(re, im, lds, t, a.tw, a.twstride);
+#pragma unroll
+    for (int f = 0; f < V; ++f) {
+        const int i = f / RL, r = f % RL;
+        const c64 k = a.kodd[t + i * T + r * (L / RL)];
+        const double xr = re[f], xi = im[f];
+        re[f] = k.x * xr - k.y * xi;
+        im[f] = k.x * xi + k.y * xr;
+    }
+    StockhamPasses<L, 0, true>::run(re, im, lds, t, a.tw, a.twstride);
+#pragma unroll
+    for (int f = 0; f < V; ++f) {
+        const int i = f / R0, r = f % R0;
+        const int n = t + i * T + r * (L / R0);
+        const int e = n - (NE - 1);
+        if (e < 0 || e >= NE) continue;
+        const c64 ph = a.phase[n];  // exp(+i pi n/L) = conj(ph)
+        const double yr = re[f] * invL, yi = im[f] * invL;
+        a.out[base + e] = cmk(yr * ph.x + yi * ph.y, yi * ph.x - yr * ph.y);
+    }
+}
+
+// The 2L witness: the 2L DFT of the zero-padded column splits into even
+// outputs, the L transform itself, and odd outputs, the L transform of
+// x_n exp(-i pi n/L) (k_hilbert_odd, run first; it parks
+// exp(i pi n/L) y'_n in the output field). With the odd half of the 2L
+// kernel spectrum, the 2L result at n < L is (y_n + exp(i pi n/L) y'_n)/2, so
+// Lambda2 - Lambda = (exp(i pi n/L) y'_n - y_n)/2 compares two independently
+// rounded evaluations of the same convolution, as the reference's 2L
+// transform does, at the cost of one more length-L transform instead of a
+// length-2L one.
 template <int L>
 __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
     constexpr int V = vals_per_thread(L), T = threads_for(L);
@@ -220,6 +285,9 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
     const long long col = blockIdx.x;
     const int NE = a.NE;
     const size_t base = (size_t)col * NE;
+    const double invL = 1.0 / (double)L;
+    const bool witness = a.kodd != nullptr;
+    double dn = 0.0, nn = 0.0;
     double re[V], im[V];
     // Load wE*Gamma, zero padded, in the first pass's input order.
 #pragma unroll
@@ -248,21 +316,19 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
     }
     StockhamPasses<L, 0, true>::run(re, im, lds, t, a.tw, a.twstride);
     // Outputs of the inverse sit at j + r L/R0; Lambda_e = y[e + NE - 1] / L.
-    const double invL = 1.0 / (double)L;
-    double dn = 0.0, nn = 0.0;
 #pragma unroll
     for (int f = 0; f < V; ++f) {
         const int i = f / R0, r = f % R0;
-        const int e = t + i * T + r * (L / R0) - (NE - 1);
+        const int n = t + i * T + r * (L / R0);
+        const int e = n - (NE - 1);
         if (e < 0 || e >= NE) continue;
         const c64 lam = cmk(re[f] * invL, im[f] * invL);
         const size_t gi = base + e;
-        if (a.mode == 2) {
-            const c64 ref = a.lambda_ref[gi];
-            const double dx = lam.x - ref.x, dy = lam.y - ref.y;
+        if (witness) {
+            const c64 od = a.out[gi];  // parked by k_hilbert_odd
+            const double dx = 0.5 * (lam.x + od.x) - lam.x, dy = 0.5 * (lam.y + od.y) - lam.y;
             dn += dx * dx + dy * dy;
-            nn += ref.x * ref.x + ref.y * ref.y;
-            continue;
+            nn += lam.x * lam.x + lam.y * lam.y;
         }
         if (a.mode == 3) {
             // Gamma_bar = -wE * M SR_bar + (i/2) SR_bar (M antisymmetric);
@@ -274,7 +340,6 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
             a.out2[gi] = csub(a.out2[gi], ig);
             continue;
         }
-        if (a.lambda_copy) a.lambda_copy[gi] = lam;
         if (a.mode == 1) {
             const c64 gm = load_gamma(a, gi);
             a.out[gi] = cmk(lam.x + 0.5 * gm.y, lam.y - 0.5 * gm.x);  // Lambda - 0.5 i Gamma
@@ -282,7 +347,7 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
             a.out[gi] = lam;
         }
     }
-    if (a.mode != 2) return;
+    if (!witness) return;
     // Deterministic block reduction of the witness sums (fixed tree in LDS).
     double* rd = lds;
     double* rn = lds + T;
@@ -349,7 +414,10 @@ void host_fft(std::vector<std::complex<long double>>& x) {
     }
 }
 
-std::vector<c64> kernel_spectrum(int NE, int L, double de) {
+// Spectrum of the kernel sequence k[lag + NE - 1] = 1/(2 pi lag de) on L
+// points; odd: the odd-frequency half of its 2L spectrum,
+// DFT_L(k_n exp(-i pi n/L)) (the sequence is shorter than L).
+std::vector<c64> kernel_spectrum(int NE, int L, double de, bool odd) {
     const long double pi = 3.141592653589793238462643383279502884L;
     std::vector<std::complex<long double>> k(L, 0.0L);
     for (int lag = -(NE - 1); lag <= NE - 1; ++lag) {
@@ -358,6 +426,9 @@ std::vector<c64> kernel_spectrum(int NE, int L, double de) {
         double v = 1.0 / (2.0 * M_PI * (double)lag * de);
         k[lag + NE - 1] = (long double)v;
     }
+    if (odd)
+        for (int n = 0; n < L; ++n)
+            k[n] *= std::complex<long double>(cosl(-pi * n / (long double)L), sinl(-pi * n / (long double)L));
     host_fft(k);
     std::vector<c64> out(L);
     for (int i = 0; i < L; ++i) out[i] = cmk((double)k[i].real(), (double)k[i].imag());
@@ -376,16 +447,22 @@ int launch_hilbert_len(qclhip_ctx* ctx, long long ncols, const HilbertArgs& a) {
     if (lds > 65536)
         QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_hilbert<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 (int)lds));
+    if (a.kodd) {
+        if (lds > 65536)
+            QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_hilbert_odd<L>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_hilbert_odd<L>, dim3((unsigned)ncols), dim3(T), lds, ctx->stream, a);
+    }
     hipLaunchKernelGGL(k_hilbert<L>, dim3((unsigned)ncols), dim3(T), lds, ctx->stream, a);
     return launch_check(ctx, "k_hilbert");
 }
 
-int hilbert_launch(qclhip_ctx* ctx, const qclhip_hilbert* p, const HilbertArgs& a_in, int which) {
+int hilbert_launch(qclhip_ctx* ctx, const qclhip_hilbert* p, const HilbertArgs& a_in) {
     HilbertArgs a = a_in;
-    const int L = which ? 2 * p->L : p->L;
-    a.kf = p->kf[which];
+    const int L = p->L;
+    a.kf = p->kf[0];
     a.tw = p->twiddle;
-    a.twstride = p->Lmax / L;
+    a.twstride = 1;
     switch (L) {
         case 4: return launch_hilbert_len<4>(ctx, p->ncols, a);
         case 8: return launch_hilbert_len<8>(ctx, p->ncols, a);
@@ -416,26 +493,18 @@ int hilbert_run(qclhip_ctx* ctx, const qclhip_hilbert* p, const double* Gamma, c
     a.SG = (const c64*)SG;
     a.out = (c64*)out;
     a.mode = mode;
-    c64* lam = nullptr;
+    (void)n;
     double* partial = nullptr;
     if (roundoff_host) {
         if (!p->verify) return fail(ctx, QCLHIP_ERR_ARGUMENT, "the FFT plan was built without roundoff verification");
-        char* ws = (char*)workspace(ctx, (size_t)n * sizeof(c64) + (size_t)2 * p->ncols * sizeof(double) + 64);
-        if (!ws) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
-        lam = (c64*)ws;
-        partial = (double*)(ws + (size_t)n * sizeof(c64));
-        a.lambda_copy = mode == 0 ? nullptr : lam;
+        partial = (double*)workspace(ctx, (size_t)2 * p->ncols * sizeof(double) + 64);
+        if (!partial) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+        a.kodd = p->kf[1];
+        a.phase = p->phase;
+        a.partial = partial;
     }
-    int rc = hilbert_launch(ctx, p, a, 0);
+    int rc = hilbert_launch(ctx, p, a);
     if (rc || !roundoff_host) return rc;
-    HilbertArgs w = a;
-    w.mode = 2;
-    w.out = nullptr;
-    w.lambda_copy = nullptr;
-    w.lambda_ref = mode == 0 ? (const c64*)out : lam;
-    w.partial = partial;
-    rc = hilbert_launch(ctx, p, w, 1);
-    if (rc) return rc;
     double* res = partial + 2 * p->ncols;
     hipLaunchKernelGGL(k_witness_reduce, dim3(1), dim3(1024), 0, ctx->stream, partial, p->ncols, res);
     rc = launch_check(ctx, "k_witness_reduce");
@@ -465,7 +534,7 @@ int qclhip_hilbert_create(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t ncol
     if (!(grid->de > 0)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "FFT Hilbert transform requires a uniform energy grid");
     int L = 1;
     while (L < 2 * grid->NE - 1) L <<= 1;
-    const int Lmax = verify ? 2 * L : L;
+    const int Lmax = L;  // the witness runs at length L (odd-frequency half of the 2L transform)
     if ((size_t)Lmax * sizeof(double) > ctx->max_lds_per_block || Lmax > 16384)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "FFT length exceeds the LDS of one workgroup");
     auto* p = new qclhip_hilbert();
@@ -485,10 +554,18 @@ int qclhip_hilbert_create(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t ncol
     bool ok = hipMalloc(&p->twiddle, (size_t)Lmax * sizeof(c64)) == hipSuccess &&
               hipMemcpy(p->twiddle, tw.data(), (size_t)Lmax * sizeof(c64), hipMemcpyHostToDevice) == hipSuccess;
     for (int w = 0; w < (verify ? 2 : 1) && ok; ++w) {
-        const int Lw = w ? 2 * L : L;
-        std::vector<c64> kf = kernel_spectrum(grid->NE, Lw, grid->de);
-        ok = hipMalloc(&p->kf[w], (size_t)Lw * sizeof(c64)) == hipSuccess &&
-             hipMemcpy(p->kf[w], kf.data(), (size_t)Lw * sizeof(c64), hipMemcpyHostToDevice) == hipSuccess;
+        std::vector<c64> kf = kernel_spectrum(grid->NE, L, grid->de, w == 1);
+        ok = hipMalloc(&p->kf[w], (size_t)L * sizeof(c64)) == hipSuccess &&
+             hipMemcpy(p->kf[w], kf.data(), (size_t)L * sizeof(c64), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (verify && ok) {
+        std::vector<c64> ph(L);
+        for (int i = 0; i < L; ++i) {
+            long double ang = -pi * (long double)i / (long double)L;
+            ph[i] = cmk((double)cosl(ang), (double)sinl(ang));
+        }
+        ok = hipMalloc(&p->phase, (size_t)L * sizeof(c64)) == hipSuccess &&
+             hipMemcpy(p->phase, ph.data(), (size_t)L * sizeof(c64), hipMemcpyHostToDevice) == hipSuccess;
     }
     if (!ok) {
         qclhip_hilbert_destroy(p);
@@ -504,6 +581,7 @@ int qclhip_hilbert_destroy(qclhip_hilbert* p) {
     if (p->twiddle) hipFree(p->twiddle);
     if (p->kf[0]) hipFree(p->kf[0]);
     if (p->kf[1]) hipFree(p->kf[1]);
+    if (p->phase) hipFree(p->phase);
     delete p;
     return QCLHIP_OK;
 }
@@ -528,7 +606,7 @@ int qclhip_retarded_self_energy_adjoint(qclhip_ctx* ctx, const qclhip_hilbert* p
     a.out = (c64*)SLbar;
     a.out2 = (c64*)SGbar;
     a.mode = 3;
-    return hilbert_launch(ctx, plan, a, 0);
+    return hilbert_launch(ctx, plan, a);
 }
 
 int qclhip_retarded_self_energy(qclhip_ctx* ctx, const qclhip_hilbert* plan, const double* SL, const double* SG,

@@ -387,3 +387,26 @@ def test_seed_reproduces_the_target_number(orc):
     mu = orc.seed_mu(w["eps"], q, w["kbt"], target)
     GL = orc.seed_lesser(GR, w["eps"], mu, w["kbt"])
     assert abs(orc.number_functional(GL, w["wE"], w["wk"], 2) - target) < 1e-12 * target
+
+
+def test_two_l_transform_from_two_l_point_halves():
+    """The identity behind the device's 2L round-off witness: for a column of
+    length NE and the kernel of lags -(NE-1)..NE-1 (both shorter than L), the
+    2L cyclic convolution at n < L equals (y_n + exp(i pi n/L) y'_n)/2, with y
+    the L-point convolution and y' the L-point convolution of
+    x_n exp(-i pi n/L) with k_n exp(-i pi n/L)."""
+    rng = np.random.default_rng(0)
+    NE = 37
+    L = 1 << int(np.ceil(np.log2(2 * NE - 1)))
+    x = rng.normal(size=NE) + 1j * rng.normal(size=NE)
+    k = rng.normal(size=2 * NE - 1)
+    xp, kp = np.zeros(L, complex), np.zeros(L, complex)
+    xp[:NE], kp[:2 * NE - 1] = x, k
+    y2 = np.fft.ifft(np.fft.fft(np.r_[xp, np.zeros(L)]) * np.fft.fft(np.r_[kp, np.zeros(L)]))[:L]
+    ph = np.exp(-1j * np.pi * np.arange(L) / L)
+    y = np.fft.ifft(np.fft.fft(xp) * np.fft.fft(kp))
+    yo = np.fft.ifft(np.fft.fft(xp * ph) * np.fft.fft(kp * ph))
+    assert np.max(np.abs(0.5 * (y + np.conj(ph) * yo) - y2)) < 1e-13 * np.max(np.abs(y2))
+    # on the extracted range the L result itself equals the 2L one
+    e = np.arange(NE) + NE - 1
+    assert np.max(np.abs(y[e] - y2[e])) < 1e-13 * np.max(np.abs(y2))
