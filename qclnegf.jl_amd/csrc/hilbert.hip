@@ -132,13 +132,33 @@ __device__ inline void stockham_pass(double* re, double* im, int t, const c64* _
         if (NS > 1) {
             const int k = j & (NS - 1);
             const int step = (L / (NS * R)) * twstride;
+            c64 w[R];
+            if constexpr (R == 8 || R == 16) {
+                // w^1, w^2, w^4 (and w^8) from the table, the other powers as
+                // products of at most three of them (a few ulps, against 15
+                // table loads per butterfly)
+                w[1] = tw[k * step];
+                w[2] = tw[2 * k * step];
+                w[4] = tw[4 * k * step];
+                w[3] = cmul(w[1], w[2]);
+                w[5] = cmul(w[4], w[1]);
+                w[6] = cmul(w[4], w[2]);
+                w[7] = cmul(w[4], w[3]);
+                if constexpr (R == 16) {
+                    w[8] = tw[8 * k * step];
+#pragma unroll
+                    for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
+                }
+            } else {
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[r] = tw[r * k * step];
+            }
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                const c64 w = tw[r * k * step];
-                const double wi = INV ? -w.y : w.y;
+                const double wi = INV ? -w[r].y : w[r].y;
                 const double xr = re[i * R + r], xi = im[i * R + r];
-                re[i * R + r] = w.x * xr - wi * xi;
-                im[i * R + r] = w.x * xi + wi * xr;
+                re[i * R + r] = w[r].x * xr - wi * xi;
+                im[i * R + r] = w[r].x * xi + wi * xr;
             }
         }
         dft<R, INV>(re + i * R, im + i * R);
