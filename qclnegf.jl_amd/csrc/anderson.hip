@@ -106,7 +106,7 @@ struct GuardArgs {
 
 // sigma_max of the N x N block in tile X: sqrt(lambda_max(X X^dagger)).
 template <int B, int LD>
-__device__ inline double sigma_max(const c64* X, int N, int lane, EigWork* w, c64 (&T)[B][B]) {
+__device__ inline double sigma_max(const c64* X, int N, int lane, EigWork<B>* w, c64 (&T)[B][B]) {
     rprod<B>(X, X, N, lane, T, true);
     const BlockScale s = block_scale<B>(T, N, lane);
     if (s.bad) return INFINITY;
@@ -121,7 +121,7 @@ __device__ inline double sigma_max(const c64* X, int N, int lane, EigWork* w, c6
 
 // Extreme eigenvalue of the Hermitian part of f X (lowest when want_min).
 template <int B, int LD>
-__device__ inline double herm_extreme(const c64* X, int N, int lane, EigWork* w, double fr, double fi,
+__device__ inline double herm_extreme(const c64* X, int N, int lane, EigWork<B>* w, double fr, double fi,
                                       bool want_min) {
     c64 a[B][B];
     load_herm<B, LD>(X, N, lane, fr, fi, a);
@@ -140,7 +140,7 @@ __device__ inline double herm_extreme(const c64* X, int N, int lane, EigWork* w,
 // lambda_min(H) >= -delta (Cholesky of H + delta I; its backward error is
 // rounding-level next to delta).
 template <int B, int LD>
-__device__ inline bool herm_screen(const c64* X, int N, int lane, EigWork* w, double fr, double fi, double delta) {
+__device__ inline bool herm_screen(const c64* X, int N, int lane, EigWork<B>* w, double fr, double fi, double delta) {
     c64 a[B][B];
     load_herm<B, LD>(X, N, lane, fr, fi, a);
     const BlockScale s = block_scale<B>(a, N, lane);
@@ -170,7 +170,7 @@ __device__ inline bool tile_all_finite(const c64* T, int N, int lane) {
 // budget. Exact norms and eigenvalues are computed only where Frobenius
 // bounds and a Cholesky screen cannot settle the comparison.
 template <int B>
-__device__ inline int guard_block(const c64* SR, const c64* SL, const c64* SG, int N, int lane, EigWork* w,
+__device__ inline int guard_block(const c64* SR, const c64* SL, const c64* SG, int N, int lane, EigWork<B>* w,
                                   c64 (&T)[B][B]) {
     constexpr int LD = 8 * B + 1;
     constexpr double kEps = 2.220446049250313e-16, kTiny = 2.2250738585072014e-308;
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(64) void k_anderson_guard(GuardArgs p) {
     c64* SR = reinterpret_cast<c64*>(smem);
     c64* SL = SR + wstride;
     c64* SG = SL + wstride;
-    EigWork* w = reinterpret_cast<EigWork*>(SG + wstride);
+    EigWork<B>* w = reinterpret_cast<EigWork<B>*>(SG + wstride);
     const int lane = threadIdx.x;
     const long long nblk = (long long)p.NE * p.Nk;
     const long long b = xcd_tile(blockIdx.x, nblk * p.nfam);
@@ -279,7 +279,7 @@ __global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
 
 template <int B>
 int launch_guard(qclhip_ctx* ctx, const GuardArgs& a) {
-    const size_t lds = 3 * (size_t)TileStride<B>::value * sizeof(c64) + sizeof(EigWork);
+    const size_t lds = 3 * (size_t)TileStride<B>::value * sizeof(c64) + sizeof(EigWork<B>);
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
     if (lds > 65536)

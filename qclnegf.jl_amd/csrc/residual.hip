@@ -40,7 +40,7 @@ __global__ __launch_bounds__(64) void k_residual_suite(ResArgs p) {
     extern __shared__ __align__(16) unsigned char smem[];
     c64* G = reinterpret_cast<c64*>(smem);
     c64* X = G + wstride;
-    EigWork* ew = reinterpret_cast<EigWork*>(X + wstride);
+    EigWork<B>* ew = reinterpret_cast<EigWork<B>*>(X + wstride);
     const int lane = threadIdx.x;
     const int lr = lane & 7, lc = lane >> 3;
     const long long nblk = (long long)p.NE * p.Nk;
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(kMixChunks) void k_mix_residual_reduce(const double
 
 template <int B>
 int launch_residual(qclhip_ctx* ctx, const ResArgs& a) {
-    const size_t lds = 2 * (size_t)TileStride<B>::value * sizeof(c64) + sizeof(EigWork);
+    const size_t lds = 2 * (size_t)TileStride<B>::value * sizeof(c64) + sizeof(EigWork<B>);
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
     if (lds > 65536)

@@ -73,12 +73,14 @@ __device__ inline double tile_frob2(const c64* T, int N, int lane, double cdiag)
     return wave_sum(s);
 }
 
-// LDS vectors of one wave's tridiagonal reduction and Cholesky screen.
+// LDS vectors of one wave's tridiagonal reduction and Cholesky screen, sized
+// for the tile class (N <= 8B).
+template <int B>
 struct EigWork {
-    c64 u[48];
-    c64 q[48];
-    double d[48];
-    double e2[48];
+    c64 u[8 * B];
+    c64 q[8 * B];
+    double d[8 * B];
+    double e2[8 * B];
 };
 
 // Sums over the 8 lanes of one row group (lane bits 3..5) and of one column
@@ -156,7 +158,7 @@ __device__ inline void scale_regs(c64 (&a)[B][B], double s) {
 // right-looking elimination is positive; a is destroyed. The column of each
 // step is passed through LDS.
 template <int B>
-__device__ inline bool chol_screen(c64 (&a)[B][B], int N, int lane, EigWork* w, double shift) {
+__device__ inline bool chol_screen(c64 (&a)[B][B], int N, int lane, EigWork<B>* w, double shift) {
     const int lr = lane & 7, lc = lane >> 3;
 #pragma unroll
     for (int r = 0; r < B; ++r)
@@ -240,7 +242,7 @@ __device__ inline void multisect(double& a, double& b, unsigned mask) {
 // multisection each (half-waves), to eps * ||T|| absolute. Callers scale a
 // to max |a_ij| ~ 1 first.
 template <int B>
-__device__ inline void eig_core(c64 (&a)[B][B], int N, int lane, EigWork* w, double& lmin, double& lmax) {
+__device__ inline void eig_core(c64 (&a)[B][B], int N, int lane, EigWork<B>* w, double& lmin, double& lmax) {
     constexpr double kEps = 2.220446049250313e-16, kTiny = 2.2250738585072014e-308;
     const int lr = lane & 7, lc = lane >> 3;
 #pragma unroll
@@ -382,7 +384,7 @@ __device__ inline double psd_ratio(double lmin, double lbig, int N) {
 // the block; its backward error (<= N(N+1) eps ||H||) stays within the other
 // half of the budget. Otherwise the extreme eigenvalues are computed.
 template <int B, int LD>
-__device__ inline double psd_from_tile(const c64* X, int N, int lane, EigWork* w, double fr, double fi) {
+__device__ inline double psd_from_tile(const c64* X, int N, int lane, EigWork<B>* w, double fr, double fi) {
     constexpr double kEps = 2.220446049250313e-16;
     c64 a[B][B];
     load_herm<B, LD>(X, N, lane, fr, fi, a);
@@ -414,7 +416,7 @@ __device__ inline double psd_from_tile(const c64* X, int N, int lane, EigWork* w
 // largest eigenvalue of S S^dagger (that of S^dagger S). T: register
 // scratch.
 template <int B, int LD>
-__device__ inline double causality_from_tile(const c64* X, int N, int lane, EigWork* w, c64 (&T)[B][B]) {
+__device__ inline double causality_from_tile(const c64* X, int N, int lane, EigWork<B>* w, c64 (&T)[B][B]) {
     c64 a[B][B];
     load_herm<B, LD>(X, N, lane, 0.0, 1.0, a);  // Hermitian part of i S = -Im S
     const BlockScale s = block_scale<B>(a, N, lane);
