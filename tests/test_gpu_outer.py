@@ -143,3 +143,30 @@ def test_outer_step_after_scba_vs_oracle(hip, orc):
     assert abs(row["zeta"] - zeta) <= 1e-11 * abs(zeta) + 1e-15 and abs(row["r_P"] - rP) <= 1e-10 * rP + 1e-14
     h_o = orc.project_hamiltonians(Phi, U_o, S, kin, Minv)
     assert float(rel(np.asfortranarray(row["h"].cpu().numpy().T), h_o)) < 1e-12
+
+
+def test_interchange_problem_runs_like_the_in_memory_one(hip, orc, tmp_path):
+    """A problem written to disk and read back runs the SCBA iteration of the
+    oracle on the in-memory problem it was written from (1e-12)."""
+    import torch
+    import scba_fixture as fx
+    from qclhip import interchange as ic, scba
+    prob = fx.make_problem(N=4, NE=64, Nk=3)
+    full, state, _ = fx.oracle_state(prob)
+    arrays = {k: prob[k] for k in ("eps", "wE", "wk", "h", "Tplus", "Tminus")}
+    scalars = dict(gs=2, target=full["target"], Ep=prob["Ep"], hwLO=prob["hwLO"], NLO=prob["NLO"],
+                   kbt=prob["kbt"], alpha_mix=prob["alpha_mix"])
+    ic.save_problem(tmp_path, arrays, scalars, prob["mechanisms"])
+    p, op = ic.device_problem(ic.load_problem(tmp_path))
+    assert op is None
+    dev = hip.to_device
+    mk = lambda: scba.ScbaState(None, {m["name"]: scba.Family(dev(state["scatR"][i]), dev(state["scatL"][i]),
+                                                               dev(state["scatG"][i]))
+                                       for i, m in enumerate(prob["mechanisms"])},
+                                scba.Family(dev(state["plusR"]), dev(state["plusL"]), dev(state["plusG"])),
+                                scba.Family(dev(state["minusR"]), dev(state["minusL"]), dev(state["minusG"])))
+    s1 = mk()
+    r1 = scba.scba_iteration(p, s1, scba.ScbaWorkspace(p))
+    ro = orc.ScbaOracle(full, state).iterate()
+    assert abs(r1["lam"] - ro["lam"]) < 1e-12 * abs(ro["lam"])
+    assert float(rel(r1["populations"], ro["populations"])) < 1e-12
