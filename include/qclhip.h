@@ -158,6 +158,16 @@ int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
                          int nfam, const double* const* famR, const double* const* famL,
                          const double* const* famG, double target, double* GR, double* GL,
                          double* GG, double* A, double* scale, double* out_host);
+/* qclhip_current_green that also writes the summed families (the total
+ * self-energy S^R, S<, S> of the iteration, _total_family_production,
+ * production_operators.jl:1447-1473) to totR, totL, totG: in the FP64
+ * vector path the kernels store the sums they load (one extra write per
+ * component); otherwise a summation pass fills them. For the later passes
+ * of an iteration (the residual suite) that would sum the families again. */
+int qclhip_current_green_totals(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
+                                const double* const* famR, const double* const* famL, const double* const* famG,
+                                double target, double* GR, double* GL, double* GG, double* A, double* scale,
+                                double* out_host, double* totR, double* totL, double* totG);
 
 /* A3: number functional N[G] = Re[-i g_s/(2 pi) sum wE wk tr G] with the
  * realness gate (synchronizes). */

@@ -52,6 +52,11 @@ struct DKArgs {
     int refine;
     double refine_tol;
     unsigned* block_flags;
+    // Optional by-products of the FP64 vector path: the summed families
+    // (total self-energy) as loaded, for later passes of the iteration.
+    c64* totR;
+    c64* totL;
+    c64* totG;
 };
 
 // out[r][c] (lane block) = Gs[rows] * Xs[:, cols]  or  Xs[rows] * Gs[cols]^dagger

@@ -65,6 +65,10 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
     if (threadIdx.x == 0) s_ok = 0;
     if (p.nfamR) tile_load<W, LD>(reinterpret_cast<c64*>(smem), wstride, p.famR, p.nfamR, blk0, nvalid, N, S);
     __syncthreads();
+    if (p.totR && p.nfamR) {
+        const unsigned vmask = nvalid >= 32 ? 0xffffffffu : ((1u << nvalid) - 1u);
+        tile_store<W, LD>(reinterpret_cast<const c64*>(smem), wstride, p.totR, blk0, vmask, N, S);
+    }
     bool ok = wave < nvalid;
     double s = 0.0, rs = 0.0;
     c64 A[B][B];
@@ -370,6 +374,11 @@ __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) 
                     tile_load<W, LD>(X0, wstride, p.famG, p.nfamG, blk0, nvalid, N, S);
                 }
                 __syncthreads();
+            }
+            c64* tot = comp ? p.totG : p.totL;
+            if (tot) {
+                tile_store<W, LD>(X0, wstride, tot, blk0, mask, N, S);
+                __syncthreads();  // the products overwrite the tiles
             }
             if (active) keldysh_products<B>(Gs, Xs, N, lane, p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr);
             __syncthreads();
@@ -995,10 +1004,10 @@ int qclhip_retarded_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double
                                 nullptr);
 }
 
-int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
-                         const double* const* famR, const double* const* famL, const double* const* famG,
-                         double target, double* GR, double* GL, double* GG, double* A, double* scale,
-                         double* out_host) {
+static int current_green_impl(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
+                              const double* const* famR, const double* const* famL, const double* const* famG,
+                              double target, double* GR, double* GL, double* GG, double* A, double* scale,
+                              double* out_host, double* const* totals) {
     if (!ctx || !grid || !h || !famR || !famL || !famG || !GR || !GL || !GG) return QCLHIP_ERR_ARGUMENT;
     if (nfam < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "at least the embedding family is required");
     if (!(std::isfinite(target) && target > 0))
@@ -1055,6 +1064,30 @@ int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
     a.traces = traces;
     a.status = ctx->status;
     a.keldysh = 1;
+    if (totals) {
+        const bool in_kernels = fused && grid->precision == QCLHIP_PRECISION_FP64 &&
+                                grid->keldysh_engine == QCLHIP_ENGINE_VECTOR && a.N <= 48;
+        if (in_kernels) {
+            a.totR = (c64*)totals[0];
+            a.totL = (c64*)totals[1];
+            a.totG = (c64*)totals[2];
+        } else {
+            const c64* const* src[3] = {a.famR, a.famL, a.famG};
+            for (int k = 0; k < 3; ++k) {
+                SumArgs sa;
+                sa.nx = 1;
+                sa.x[0] = src[k][0];
+                if (fused) {
+                    sa.nx = nfam;
+                    for (int f = 0; f < nfam; ++f) sa.x[f] = src[k][f];
+                }
+                hipLaunchKernelGGL(k_sum_fields, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
+                                   n, sa, 0, (c64*)totals[k]);
+            }
+            int rc0 = launch_check(ctx, "k_sum_fields");
+            if (rc0) return rc0;
+        }
+    }
     int rc = dyson_keldysh_dispatch(ctx, grid, a, true);
     if (rc) return rc;
     hipLaunchKernelGGL(k_number_pair, dim3(1), dim3(kReduceThreads), 0, ctx->stream, traces, 2, grid->NE,
@@ -1082,6 +1115,23 @@ int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
         out_host[4] = pc->c;
     }
     return QCLHIP_OK;
+}
+
+int qclhip_current_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
+                         const double* const* famR, const double* const* famL, const double* const* famG,
+                         double target, double* GR, double* GL, double* GG, double* A, double* scale,
+                         double* out_host) {
+    return current_green_impl(ctx, grid, h, nfam, famR, famL, famG, target, GR, GL, GG, A, scale, out_host,
+                              nullptr);
+}
+
+int qclhip_current_green_totals(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
+                                const double* const* famR, const double* const* famL, const double* const* famG,
+                                double target, double* GR, double* GL, double* GG, double* A, double* scale,
+                                double* out_host, double* totR, double* totL, double* totG) {
+    if (!totR || !totL || !totG) return QCLHIP_ERR_ARGUMENT;
+    double* const tot[3] = {totR, totL, totG};
+    return current_green_impl(ctx, grid, h, nfam, famR, famL, famG, target, GR, GL, GG, A, scale, out_host, tot);
 }
 
 int qclhip_number_functional(qclhip_ctx* ctx, const qclhip_grid* grid, const double* G, double* value_host) {

@@ -381,13 +381,14 @@ class GreenState:
 
 
 def current_green_production(grid: Grid, h, families, target: float, state: GreenState | None = None,
-                             spectral: bool = True, return_scalars: bool = True):
+                             spectral: bool = True, return_scalars: bool = True, totals=None):
     """_current_green_production (production_state.jl:1-74).
 
     families: list of (SR, SL, SG) device triples, the embedding total first,
     then the scattering mechanisms in mechanism order. Returns
     (GreenState, info) where info holds lambda, number, holes and the paired
-    coefficients (a, c) when return_scalars (the call then synchronizes)."""
+    coefficients (a, c) when return_scalars (the call then synchronizes).
+    totals: optional (SR, SL, SG) fields that receive the summed families."""
     nf = len(families)
     for fam in families:
         for t in fam:
@@ -399,10 +400,15 @@ def current_green_production(grid: Grid, h, families, target: float, state: Gree
     arr = lambda k: (_vp * nf)(*[f[k].data_ptr() for f in families])
     out = (ct.c_double * 5)()
     slot0 = grid.ctx.result_slot() if grid.ctx.deferring else 0
-    grid.ctx.call("qclhip_current_green", grid.handle, _ptr(h), nf, arr(0), arr(1), arr(2),
-                  ct.c_double(target), _ptr(state.GR), _ptr(state.GL), _ptr(state.GG),
-                  _ptr(state.A if spectral else None), _ptr(state.scale),
-                  out if return_scalars else None)
+    common = (grid.handle, _ptr(h), nf, arr(0), arr(1), arr(2), ct.c_double(target), _ptr(state.GR),
+              _ptr(state.GL), _ptr(state.GG), _ptr(state.A if spectral else None), _ptr(state.scale),
+              out if return_scalars else None)
+    if totals is None:
+        grid.ctx.call("qclhip_current_green", *common)
+    else:
+        for t in totals:
+            _check_field(grid, t, "total")
+        grid.ctx.call("qclhip_current_green_totals", *common, *(_ptr(t) for t in totals))
     if return_scalars and grid.ctx.deferring:
         return state, Deferred(slot0, 5)
     info = dict(lam=out[0], number=out[1], holes=out[2], a=out[3], c=out[4]) if return_scalars else None

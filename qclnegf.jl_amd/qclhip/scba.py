@@ -135,6 +135,9 @@ class ScbaWorkspace:
         self.scat = {m.name: new_family(g) for m in p.mechanisms}
         self.plus = new_family(g)
         self.minus = new_family(g)
+        # the total self-energy of the iteration, written by the GF solve for
+        # the residual suite (which would otherwise sum the families again)
+        self.totals = new_family(g) if p.measure else None
         self.anderson = None
         if p.mixing == "anderson":
             from .anderson import AndersonMixer
@@ -155,7 +158,8 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     g = p.grid
     t0 = time.perf_counter()
     fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
-    green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=True)
+    green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=True,
+                                               totals=ws.totals.parts() if ws.totals is not None else None)
     s.green = green
     t1 = time.perf_counter()
     roundoff = _candidate_scattering(p, green, ws.scat)
@@ -165,7 +169,7 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     suite = None
     if p.measure:
         cand_lesser = [ws.plus.L, ws.minus.L] + [ws.scat[m.name].L for m in p.mechanisms]
-        suite = api.residual_suite(g, p.h, green, fams, cand_lesser)
+        suite = api.residual_suite(g, p.h, green, [ws.totals.parts()], cand_lesser)
     t4 = time.perf_counter()
     Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
     pops = api.state_populations(g, green.GL, p.target)

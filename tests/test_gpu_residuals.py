@@ -238,3 +238,29 @@ def test_mix_fields_residual_vs_oracle(hip, orc):
     bad[4][0, 0, 0, 0] = float("nan")
     r = hip.mix_fields_residual(list(zip([t.clone() for t in dev_o2], bad)), 0.3, partner)
     assert r[4] == np.inf and r[6] == np.inf and r[0] < np.inf
+
+
+@pytest.mark.parametrize("engine", ["vector", "matrix"])
+def test_current_green_totals(hip, engine):
+    """qclhip_current_green_totals: the Green functions equal those of
+    qclhip_current_green bitwise, and the totals equal the family sums
+    bitwise (kernel by-product in the FP64 vector path, a summation pass
+    otherwise)."""
+    import torch
+    w = fx.syn.gf_workload(N=9, NE=40, Nk=3)
+    g = hip.Grid(w["eps"], w["wE"], w["wk"], 9)
+    g.set_engine(engine)
+    inj = fx.syn.injection_family(w)
+    fams = [fx.syn.diag_fields_device(inj), fx.syn.diag_fields_device(w)]
+    fams.append(tuple(0.37 * x for x in fams[1]))  # a third physical family
+    h = hip.h_to_device(w["h"])
+    st1, i1 = hip.current_green_production(g, h, fams, 1.0)
+    tot = tuple(torch.empty_like(st1.GR) for _ in range(3))
+    st2, i2 = hip.current_green_production(g, h, fams, 1.0, totals=tot)
+    for k in ("GR", "GL", "GG", "A"):
+        assert torch.equal(getattr(st1, k), getattr(st2, k)), k
+    assert i1 == i2
+    for k in range(3):
+        ref = torch.empty_like(tot[k])
+        hip.sum_fields([f[k] for f in fams], ref)
+        assert torch.equal(tot[k], ref), k
