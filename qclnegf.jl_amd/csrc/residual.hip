@@ -259,14 +259,14 @@ __global__ __launch_bounds__(256) void k_mix_residual(long long n, MixResArgs a,
     const double beta = 1.0 - alpha;
     double acc[6] = {0, 0, 0, 0, 0, 0};  // |c-o|^2, |c|^2 of a, b and of the sum
     const long long stride = (long long)kMixChunks * 256;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-        const c64 o1 = oa[i], c1 = ca[i];
+    // Two elements per iteration, all loads first (more bytes in flight);
+    // the sums still run over i, i + stride, ... in order.
+    auto step = [&](long long i, c64 o1, c64 c1, c64 o2, c64 c2) {
         const double dx = c1.x - o1.x, dy = c1.y - o1.y;
         acc[0] += dx * dx + dy * dy;
         acc[1] += c1.x * c1.x + c1.y * c1.y;
         if (alpha != 0.0) oa[i] = cmk(beta * o1.x + alpha * c1.x, beta * o1.y + alpha * c1.y);
         if (pb >= 0) {
-            const c64 o2 = ob[i], c2 = cb[i];
             const double ex = c2.x - o2.x, ey = c2.y - o2.y;
             acc[2] += ex * ex + ey * ey;
             acc[3] += c2.x * c2.x + c2.y * c2.y;
@@ -277,6 +277,17 @@ __global__ __launch_bounds__(256) void k_mix_residual(long long n, MixResArgs a,
             acc[5] += tx * tx + ty * ty;
             if (alpha != 0.0) ob[i] = cmk(beta * o2.x + alpha * c2.x, beta * o2.y + alpha * c2.y);
         }
+    };
+    const c64 z = cmk(0.0, 0.0);
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += 2 * stride) {
+        const long long j = i + stride;
+        const bool hj = j < n;
+        const c64 o1 = oa[i], c1 = ca[i];
+        const c64 o2 = pb >= 0 ? ob[i] : z, c2 = pb >= 0 ? cb[i] : z;
+        const c64 p1 = hj ? oa[j] : z, q1 = hj ? ca[j] : z;
+        const c64 p2 = (hj && pb >= 0) ? ob[j] : z, q2 = (hj && pb >= 0) ? cb[j] : z;
+        step(i, o1, c1, o2, c2);
+        if (hj) step(j, p1, q1, p2, q2);
     }
     const int nacc = pb >= 0 ? 6 : 2;
     for (int k = 0; k < nacc; ++k) red[k][threadIdx.x] = acc[k];

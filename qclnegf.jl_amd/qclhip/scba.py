@@ -138,6 +138,7 @@ class ScbaWorkspace:
         # the total self-energy of the iteration, written by the GF solve for
         # the residual suite (which would otherwise sum the families again)
         self.totals = new_family(g) if p.measure else None
+        self.cand_lesser = api.zeros_field(g) if p.measure else None
         self.anderson = None
         if p.mixing == "anderson":
             from .anderson import AndersonMixer
@@ -168,8 +169,10 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     t3 = time.perf_counter()
     suite = None
     if p.measure:
-        cand_lesser = [ws.plus.L, ws.minus.L] + [ws.scat[m.name].L for m in p.mechanisms]
-        suite = api.residual_suite(g, p.h, green, [ws.totals.parts()], cand_lesser)
+        # the unmixed candidate total S<, summed once in the reference's order
+        # ((plus + minus) + mechanism 1 + ...)
+        api.sum_fields([ws.plus.L, ws.minus.L] + [ws.scat[m.name].L for m in p.mechanisms], ws.cand_lesser)
+        suite = api.residual_suite(g, p.h, green, [ws.totals.parts()], [ws.cand_lesser])
     t4 = time.perf_counter()
     Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
     pops = api.state_populations(g, green.GL, p.target)
