@@ -108,9 +108,10 @@ __device__ inline void dft(double* re, double* im) {
 // inverse runs the radices in reverse order so that the forward outputs are
 // already in the inverse's input order.
 constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x >> 1); }
-// 16 values per thread (radix-16 passes) up to L = 4096, 8 (radix 8, 1024
-// threads) at 8192 and 16 above.
-constexpr int vals_per_thread(int L) { return L < 16 ? L : (L == 8192 ? 8 : 16); }
+// 16 values per thread (radix-16 passes) up to L = 2048, 8 (radix 8; 512 and
+// 1024 threads) at 4096 and 8192, where 16 would hold occupancy at two waves
+// per SIMD, and 16 above.
+constexpr int vals_per_thread(int L) { return L < 16 ? L : ((L == 4096 || L == 8192) ? 8 : 16); }
 constexpr int threads_for(int L) { return L / vals_per_thread(L); }
 constexpr int base_log(int L) { return ilog2c(vals_per_thread(L)); }
 constexpr int npasses(int L) { return (ilog2c(L) + base_log(L) - 1) / base_log(L); }
@@ -132,33 +133,40 @@ __device__ inline void stockham_pass(double* re, double* im, int t, const c64* _
         if (NS > 1) {
             const int k = j & (NS - 1);
             const int step = (L / (NS * R)) * twstride;
-            c64 w[R];
             if constexpr (R == 8 || R == 16) {
-                // w^1, w^2, w^4 (and w^8) from the table, the other powers as
-                // products of at most three of them (a few ulps, against 15
-                // table loads per butterfly)
-                w[1] = tw[k * step];
-                w[2] = tw[2 * k * step];
-                w[4] = tw[4 * k * step];
-                w[3] = cmul(w[1], w[2]);
-                w[5] = cmul(w[4], w[1]);
-                w[6] = cmul(w[4], w[2]);
-                w[7] = cmul(w[4], w[3]);
-                if constexpr (R == 16) {
-                    w[8] = tw[8 * k * step];
+                // w^1, w^2, w^4 (and w^8) from the table; each other power is
+                // formed where it is applied, as the product of the base powers
+                // of its binary digits (at most three products, a few ulps),
+                // against 15 table loads per butterfly
+                c64 b[4];
+                b[0] = tw[k * step];
+                b[1] = tw[2 * k * step];
+                b[2] = tw[4 * k * step];
+                b[3] = R == 16 ? tw[8 * k * step] : cmk(1.0, 0.0);
 #pragma unroll
-                    for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
+                for (int r = 1; r < R; ++r) {
+                    c64 w = cmk(1.0, 0.0);
+                    bool first = true;
+#pragma unroll
+                    for (int bit = 0; bit < 4; ++bit)
+                        if (r & (1 << bit)) {
+                            w = first ? b[bit] : cmul(w, b[bit]);
+                            first = false;
+                        }
+                    const double wi = INV ? -w.y : w.y;
+                    const double xr = re[i * R + r], xi = im[i * R + r];
+                    re[i * R + r] = w.x * xr - wi * xi;
+                    im[i * R + r] = w.x * xi + wi * xr;
                 }
             } else {
 #pragma unroll
-                for (int r = 1; r < R; ++r) w[r] = tw[r * k * step];
-            }
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const double wi = INV ? -w[r].y : w[r].y;
-                const double xr = re[i * R + r], xi = im[i * R + r];
-                re[i * R + r] = w[r].x * xr - wi * xi;
-                im[i * R + r] = w[r].x * xi + wi * xr;
+                for (int r = 1; r < R; ++r) {
+                    const c64 w = tw[r * k * step];
+                    const double wi = INV ? -w.y : w.y;
+                    const double xr = re[i * R + r], xi = im[i * R + r];
+                    re[i * R + r] = w.x * xr - wi * xi;
+                    im[i * R + r] = w.x * xi + wi * xr;
+                }
             }
         }
         dft<R, INV>(re + i * R, im + i * R);

@@ -157,26 +157,33 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     into the mixing pass, which reads the same unmixed pairs), observables,
     linear mixing."""
     g = p.grid
-    t0 = time.perf_counter()
+
+    def stamp():
+        # with timings, each phase ends when its device work does
+        if timings is not None and not g.ctx.deferring:
+            g.ctx.synchronize()
+        return time.perf_counter()
+
+    t0 = stamp()
     fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
     green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=True,
                                                totals=ws.totals.parts() if ws.totals is not None else None)
     s.green = green
-    t1 = time.perf_counter()
+    t1 = stamp()
     roundoff = _candidate_scattering(p, green, ws.scat)
-    t2 = time.perf_counter()
+    t2 = stamp()
     _candidate_embedding(p, green, ws.plus, ws.minus)
-    t3 = time.perf_counter()
+    t3 = stamp()
     suite = None
     if p.measure:
         # the unmixed candidate total S<, summed once in the reference's order
         # ((plus + minus) + mechanism 1 + ...)
         api.sum_fields([ws.plus.L, ws.minus.L] + [ws.scat[m.name].L for m in p.mechanisms], ws.cand_lesser)
         suite = api.residual_suite(g, p.h, green, [ws.totals.parts()], [ws.cand_lesser])
-    t4 = time.perf_counter()
+    t4 = stamp()
     Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
     pops = api.state_populations(g, green.GL, p.target)
-    t5 = time.perf_counter()
+    t5 = stamp()
     pairs = []
     for m in p.mechanisms:
         pairs += list(zip(s.scattering[m.name].parts(), ws.scat[m.name].parts()))
