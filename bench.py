@@ -55,7 +55,7 @@ VARIANTS = (("fp64", "vector"), ("fp64", "matrix"), ("fp32_refine", "vector"), (
 CURRENT_TOL = 1e-8
 
 
-def gf_bench(steps, warmup, variants=VARIANTS):
+def gf_bench(steps, warmup, variants=VARIANTS, autotune=True):
     """GF-solve step (current_green: Dyson, Keldysh, number functionals,
     paired normalization) on the synthetic batch driven by an injecting
     family, for each (precision, engine) variant: step time, the Dyson/Keldysh
@@ -117,7 +117,7 @@ def gf_bench(steps, warmup, variants=VARIANTS):
                          fp64_fallback_fraction=fallback)
         del st, g
         torch.cuda.empty_cache()
-    tune = api.Grid(w["eps"], w["wE"], w["wk"], N).autotune()
+    tune = api.Grid(w["eps"], w["wE"], w["wk"], N).autotune() if autotune else None
     log(f"  autotune: {tune}")
     ref = out["fp64_vector"]["J"] if "fp64_vector" in out else None
     for v in out.values():
@@ -352,6 +352,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--variants", default=",".join(f"{p}_{e}" if p != "fp32_products" else p for p, e in VARIANTS),
+                    help="comma-separated GF-solve variants to run (profiling runs select one)")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (the reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -371,7 +374,13 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 3 x 1 GiB device-to-device copies")
     log("gf batch: N=24 NE=2048 Nk=64")
-    gf = gf_bench(args.steps, args.warmup)
+    wanted = set(args.variants.split(","))
+    sel = [(p, e) for p, e in VARIANTS if (f"{p}_{e}" if p != "fp32_products" else p) in wanted]
+    if not sel or "fp64_vector" not in wanted:
+        print("--variants must include fp64_vector (the reference arithmetic the others are checked against)",
+              file=sys.stderr)
+        sys.exit(2)
+    gf = gf_bench(args.steps, args.warmup, sel, autotune=not args.no_autotune)
     for name, v in gf["variants"].items():
         log(f"{name}: step {v['step_s']*1e3:.3f} ms ({v['solves_per_s']/1e6:.2f} M solves/s), Dyson+Keldysh "
             f"{v['kernel_ms']:.3f} ms, J {v['J']:.12g}, rel. current error {v['current_rel_err']}, "

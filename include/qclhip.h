@@ -395,6 +395,16 @@ int qclhip_project_hamiltonians(qclhip_ctx* ctx, int Nz, int Nb, int Nk, const d
  * cand; rU_host = max |cand - U_old| (deferrable, 1 double). */
 int qclhip_mix_potential(qclhip_ctx* ctx, int N, double* U, const double* cand, double alpha, double* rU_host);
 
+/* The same fused pass for a fixed-point iteration's change: mixes every
+ * pair and returns sums_host = (sum_p ||cand_p - old_p||^2,
+ * sum_p ||cand_p||^2) of the unmixed pairs (fixed-order reductions). No
+ * counterpart in the reference: the adjoint's stopping criterion. */
+int qclhip_mix_fields_change(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_, const double* const* cand,
+                             double alpha, double* sums_host);
+/* X[0..n) <- 0 (complex) on the context's stream, by a kernel (usable under
+ * graph capture, where hipMemsetAsync nodes are avoided). */
+int qclhip_fill_zero(qclhip_ctx* ctx, int64_t n, double* X);
+
 /* ---- Anderson mixing (production_mixing.jl:135-317). A history entry is
  * one field per (family, component) of the ordered families (mechanisms,
  * plus, minus; R, <, >): nfields pointers, entry-major for histories. */

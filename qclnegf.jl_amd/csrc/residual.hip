@@ -8,6 +8,9 @@
 // form in registers and Sturm multisection (eig_core, wave_eig.hpp), after a
 // Cholesky screen that settles positive blocks.
 // Per-block values are reduced afterwards in a fixed order.
+#include <algorithm>
+#include <vector>
+
 #include "wave_eig.hpp"
 
 namespace qclhip {
@@ -310,8 +313,9 @@ __global__ __launch_bounds__(256) void k_mix_residual(long long n, MixResArgs a,
     }
 }
 
-// One block per term: ratio ||c - o|| / (||c|| + 1e-14), Inf when non-finite.
-__global__ __launch_bounds__(kMixChunks) void k_mix_residual_reduce(const double* part, double* ratios) {
+// One block per term: ratio ||c - o|| / (||c|| + 1e-14), Inf when non-finite;
+// with raw != NULL also the sums themselves (raw[2t], raw[2t+1]).
+__global__ __launch_bounds__(kMixChunks) void k_mix_residual_reduce(const double* part, double* ratios, double* raw) {
     __shared__ double red[2][kMixChunks];
     const int t = blockIdx.x;
     red[0][threadIdx.x] = part[(2 * t + 0) * kMixChunks + threadIdx.x];
@@ -328,7 +332,16 @@ __global__ __launch_bounds__(kMixChunks) void k_mix_residual_reduce(const double
         const double d = red[0][0], c = red[1][0];
         const double v = sqrt(d) / (sqrt(c) + 1e-14);
         ratios[t] = (isfinite(d) && isfinite(c) && isfinite(v)) ? v : INFINITY;
+        if (raw) {
+            raw[2 * t] = d;
+            raw[2 * t + 1] = c;
+        }
     }
+}
+
+__global__ void k_fill_zero(long long n, c64* __restrict__ x) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] = cmk(0.0, 0.0);
 }
 
 template <int B>
@@ -428,9 +441,10 @@ int qclhip_selfenergy_residual(qclhip_ctx* ctx, int64_t n, const double* const* 
     return qclhip_ctx_synchronize(ctx);
 }
 
-int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
-                               const double* const* cand, double alpha, const int* partner, double* ratios_host) {
-    if (!ctx || !old_ || !cand || !partner || !ratios_host || n < 0) return QCLHIP_ERR_ARGUMENT;
+static int mix_fields_residual_impl(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
+                                    const double* const* cand, double alpha, const int* partner, double* ratios_host,
+                                    double* raw_host) {
+    if (!ctx || !old_ || !cand || !partner || !(ratios_host || raw_host) || n < 0) return QCLHIP_ERR_ARGUMENT;
     if (npairs < 1 || npairs > kMaxResPairs) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..32 mixing pairs");
     if (!(alpha >= 0 && alpha <= 1)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "mixing factor must lie in [0,1]");
     MixResArgs a{};
@@ -449,20 +463,58 @@ int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* c
         a.gt[a.ngroups] = q >= 0 ? nterms++ : -1;
         ++a.ngroups;
     }
-    double* part = (double*)workspace(ctx, ((size_t)2 * nterms * kMixChunks + nterms) * sizeof(double));
+    double* part = (double*)workspace(ctx, ((size_t)2 * nterms * kMixChunks + 3 * (size_t)nterms) * sizeof(double));
     if (!part) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
     double* ratios = part + (size_t)2 * nterms * kMixChunks;
+    double* raw = ratios + nterms;
     hipLaunchKernelGGL(k_mix_residual, dim3(kMixChunks, a.ngroups), dim3(256), 0, ctx->stream, (long long)n, a,
                        alpha, part);
-    hipLaunchKernelGGL(k_mix_residual_reduce, dim3(nterms), dim3(kMixChunks), 0, ctx->stream, part, ratios);
+    hipLaunchKernelGGL(k_mix_residual_reduce, dim3(nterms), dim3(kMixChunks), 0, ctx->stream, part, ratios,
+                       raw_host ? raw : nullptr);
     int rc = launch_check(ctx, "mix with self-energy residual");
     if (rc) return rc;
+    if (raw_host) {
+        QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(raw_host, raw, 2 * nterms * sizeof(double), hipMemcpyDeviceToHost,
+                                           ctx->stream));
+        return qclhip_ctx_synchronize(ctx);
+    }
     bool deferred = false;
     rc = defer_result(ctx, ratios, nterms, &deferred);
     if (rc || deferred) return rc;
     QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(ratios_host, ratios, nterms * sizeof(double), hipMemcpyDeviceToHost,
                                        ctx->stream));
     return qclhip_ctx_synchronize(ctx);
+}
+
+int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
+                               const double* const* cand, double alpha, const int* partner, double* ratios_host) {
+    if (!ratios_host) return QCLHIP_ERR_ARGUMENT;
+    return mix_fields_residual_impl(ctx, n, npairs, old_, cand, alpha, partner, ratios_host, nullptr);
+}
+
+int qclhip_mix_fields_change(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_, const double* const* cand,
+                             double alpha, double* sums_host) {
+    if (!ctx || !sums_host || npairs < 1 || npairs > kMaxResPairs) return QCLHIP_ERR_ARGUMENT;
+    std::vector<int> partner(npairs, -1);
+    std::vector<double> raw(2 * (size_t)npairs);
+    int rc = mix_fields_residual_impl(ctx, n, npairs, old_, cand, alpha, partner.data(), nullptr, raw.data());
+    if (rc) return rc;
+    double d = 0.0, c = 0.0;
+    for (int p = 0; p < npairs; ++p) {  // pair order
+        d += raw[2 * p];
+        c += raw[2 * p + 1];
+    }
+    sums_host[0] = d;
+    sums_host[1] = c;
+    return QCLHIP_OK;
+}
+
+int qclhip_fill_zero(qclhip_ctx* ctx, int64_t n, double* X) {
+    if (!ctx || !X || n < 0) return QCLHIP_ERR_ARGUMENT;
+    long long blocks = std::min<long long>(((long long)n + 255) / 256, (long long)std::max(ctx->sm_count, 1) * 32);
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::max<long long>(1, blocks)), dim3(256), 0, ctx->stream,
+                       (long long)n, (c64*)X);
+    return launch_check(ctx, "k_fill_zero");
 }
 
 }  // extern "C"

@@ -123,6 +123,8 @@ _SIGNATURES = {
                                 ct.POINTER(_vp), ct.POINTER(_vp)],
     "qclhip_anderson_guard": [_vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),
                               ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int)],
+    "qclhip_mix_fields_change": [_vp, ct.c_int64, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_double, _dp],
+    "qclhip_fill_zero": [_vp, ct.c_int64, _vp],
     "qclhip_mix_fields_residual": [_vp, ct.c_int64, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_double,
                                    ct.POINTER(ct.c_int), _dp],
     "qclhip_batch_create": [_vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp)],

@@ -20,7 +20,6 @@ Gradient convention: Zbar = dJ/dRe Z + i dJ/dIm Z (dJ = Re sum conj(Zbar) dZ).
 from __future__ import annotations
 
 import numpy as np
-import torch
 
 from . import api
 from .scba import Family, ScbaProblem, ScbaState, new_family, scba_iteration, ScbaWorkspace
@@ -67,8 +66,8 @@ class ScbaAdjoint:
         tmpL, tmpG, Z = api.empty_field(g), api.empty_field(g), api.empty_field(g)
         for m in p.mechanisms:
             f = mu.scat[m.name]
-            tmpL.copy_(f.L)
-            tmpG.copy_(f.G)
+            api.copy_field(f.L, tmpL)
+            api.copy_field(f.G, tmpG)
             api.retarded_self_energy_adjoint(p.hilbert, f.R, tmpL, tmpG)
             K = self.kadj[m.name]
             if m.is_lo:
@@ -101,15 +100,15 @@ class ScbaAdjoint:
         api.antihermitian_part(g, dplusL)
         api.antihermitian_part(g, dplusG)
         mu = _Families(p)
-        new = _Families(p)
         GRb, GLb, GGb = api.empty_field(g), api.empty_field(g), api.empty_field(g)
         sig = (api.empty_field(g), api.empty_field(g), api.empty_field(g))
+        plus_new = (api.empty_field(g), api.empty_field(g))  # sig + the direct dJ/dplus
         it, change = 0, float("inf")
         history = []
         for it in range(1, max_iterations + 1):
-            GRb.zero_()
-            GLb.copy_(JL)
-            GGb.copy_(JG)
+            api.fill_zero(GRb)
+            api.copy_field(JL, GLb)
+            api.copy_field(JG, GGb)
             self.candidates_vjp(mu, (GRb, GLb, GGb))
             api.current_green_adjoint(g, fams, p.target, info, green, GLb, GGb, GRbar=GRb, want_hbar=False,
                                       out=sig)
@@ -118,30 +117,23 @@ class ScbaAdjoint:
             # neutral mode (eigenvalue 1) that the forward never excites.
             api.antihermitian_part(g, sig[1])
             api.antihermitian_part(g, sig[2])
-            for f in new.all():
-                f.R.copy_(sig[0])
-                f.L.copy_(sig[1])
-                f.G.copy_(sig[2])
-            new.plus.L.add_(dplusL)
-            new.plus.G.add_(dplusG)
-            num = 0.0
-            den = 0.0
-            for a, b in zip(mu.all(), new.all()):
-                for x, y in zip(a.parts(), b.parts()):
-                    num += float(torch.linalg.vector_norm(y - x)) ** 2
-                    den += float(torch.linalg.vector_norm(y)) ** 2
+            api.sum_fields([sig[1], dplusL], plus_new[0])
+            api.sum_fields([sig[2], dplusG], plus_new[1])
+            # every family's new value is sig (plus: sig + dJ/dplus); mix and
+            # measure the change ||new - mu|| / ||new|| in one pass
             pairs = []
-            for a, b in zip(mu.all(), new.all()):
-                pairs += list(zip(a.parts(), b.parts()))
-            api.mix_fields(pairs, alpha)
+            for f in mu.all():
+                newL, newG = (plus_new if f is mu.plus else (sig[1], sig[2]))
+                pairs += [(f.R, sig[0]), (f.L, newL), (f.G, newG)]
+            num, den = api.mix_fields_change(pairs, alpha)
             change = (num / max(den, 1e-300)) ** 0.5
             if it == 1 or it % 100 == 0:
                 history.append((it, change))
             if change <= tol:
                 break
-        GRb.zero_()
-        GLb.copy_(JL)
-        GGb.copy_(JG)
+        api.fill_zero(GRb)
+        api.copy_field(JL, GLb)
+        api.copy_field(JG, GGb)
         self.candidates_vjp(mu, (GRb, GLb, GGb))
         _, hbar = api.current_green_adjoint(g, fams, p.target, info, green, GLb, GGb, GRbar=GRb, want_hbar=True,
                                             out=sig)

@@ -144,8 +144,21 @@ def empty_field(grid, device=None):
 
 
 def zeros_field(grid, device=None):
-    return torch.zeros(field_shape(grid.NE, grid.Nk, grid.Nb), dtype=torch.complex128,
-                       device=device or grid.ctx.device)
+    t = torch.empty(field_shape(grid.NE, grid.Nk, grid.Nb), dtype=torch.complex128,
+                    device=device or grid.ctx.device)
+    return fill_zero(t)
+
+
+def fill_zero(t):
+    """t <- 0 by the library's fill kernel (on the context's stream)."""
+    ctx = context(t.device.index or 0)
+    ctx.call("qclhip_fill_zero", ct.c_int64(t.numel()), _ptr(t))
+    return t
+
+
+def copy_field(src, dst):
+    """dst <- src by the library's field kernel."""
+    return sum_fields([src], dst)
 
 
 def to_device(x, device="cuda"):
@@ -544,6 +557,19 @@ def mix_fields_residual(pairs, alpha: float, partner=None):
     ctx.call("qclhip_mix_fields_residual", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha),
              (ct.c_int * len(pairs))(*partner), out)
     return Deferred(slot0, nterms) if ctx.deferring else list(out)
+
+
+def mix_fields_change(pairs, alpha: float):
+    """Mix every (old, candidate) pair in one pass; returns
+    (sum ||c - o||^2, sum ||c||^2) over the unmixed pairs."""
+    pairs = list(pairs)
+    n = pairs[0][0].numel()
+    ctx = context(pairs[0][0].device.index or 0)
+    olds = (_vp * len(pairs))(*[o.data_ptr() for o, _ in pairs])
+    cands = (_vp * len(pairs))(*[c.data_ptr() for _, c in pairs])
+    out = (ct.c_double * 2)()
+    ctx.call("qclhip_mix_fields_change", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha), out)
+    return out[0], out[1]
 
 
 def boundary_flux_bar(grid: Grid, plusL, plusG, GL, GG) -> float:
