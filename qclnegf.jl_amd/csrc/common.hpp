@@ -11,6 +11,7 @@
 #include <rocblas/rocblas.h>
 
 #include <cstdint>
+#include <type_traits>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -42,6 +43,16 @@ __device__ inline c64 cfmac(c64 c, c64 a, c64 b) {
 }
 __device__ inline c64 cconj(c64 a) { return cmk(a.x, -a.y); }
 __device__ inline double cabs1(c64 a) { return fabs(a.x) + fabs(a.y); }
+// Calls f(std::integral_constant<int, I>) for I = First..Last-1: the index
+// is a compile-time constant in the body, so a comparison with a runtime
+// value never turns a register array access into a dynamic index.
+template <int First, int Last, typename F>
+__device__ inline void static_for(F&& f) {
+    if constexpr (First < Last) {
+        f(std::integral_constant<int, First>{});
+        static_for<First + 1, Last>(f);
+    }
+}
 // 1/z by Smith's algorithm (no overflow for |z| near the double range).
 __device__ inline c64 crecip(c64 z) {
     if (fabs(z.x) >= fabs(z.y)) {
@@ -50,6 +61,16 @@ __device__ inline c64 crecip(c64 z) {
     }
     double r = z.x / z.y, d = z.y + z.x * r;
     return cmk(r / d, -1.0 / d);
+}
+// The same quotient with selects instead of branches and two divisions
+// (b/a and 1/d, shared by both components): for wave-uniform pivots.
+__device__ inline c64 crecip_sel(c64 z) {
+    const bool sw = fabs(z.y) > fabs(z.x);
+    const double a = sw ? z.y : z.x, b = sw ? z.x : z.y;
+    const double r = b / a;
+    const double t = 1.0 / fma(b, r, a);
+    const double u = r * t;
+    return sw ? cmk(u, -t) : cmk(t, -u);
 }
 
 // Device-side status word shared by all kernels of one context:

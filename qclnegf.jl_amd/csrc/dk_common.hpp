@@ -160,10 +160,10 @@ __device__ inline void dpp_argmax_step(double& mag, int& idx) {
     int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
     int oi = __builtin_amdgcn_update_dpp(0, idx, CTRL, 0xF, 0xF, false);
     double om = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-    if (om > mag || (om == mag && oi < idx)) {
-        mag = om;
-        idx = oi;
-    }
+    // Larger magnitude, ties to the lower row; selects, no divergent branch.
+    const bool take = (om > mag) | ((om == mag) & (oi < idx));
+    mag = take ? om : mag;
+    idx = take ? oi : idx;
 }
 
 __device__ inline void argmax8(double& mag, int& idx) {

@@ -34,9 +34,13 @@ template <int B, int W>
 struct DyShape {
     static constexpr int NP = 8 * B;
     static constexpr int LD = NP + 1;
-    static constexpr size_t per_wave =
-        (size_t)TileStride<B>::value * sizeof(c64) + 2 * NP * sizeof(c64) + 2 * NP * sizeof(int);
+    // The pivot column and row and the two permutation vectors live in the
+    // wave's own tile during the elimination (the tile is dead between the
+    // load of D and the store of G), so a wave needs only its tile:
+    // 4 workgroups of 4 waves per CU at N <= 24.
+    static constexpr size_t per_wave = (size_t)TileStride<B>::value * sizeof(c64);
     static constexpr size_t bytes = per_wave * W;
+    static_assert(2 * NP * sizeof(c64) + 2 * NP * sizeof(int) <= per_wave, "elimination buffers exceed the tile");
 };
 
 // One tile (W consecutive energies of one momentum) of the FP64 Dyson step;
@@ -50,8 +54,7 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
     const long long nblk = (long long)p.NE * p.Nk;
     const int wstride = TileStride<B>::value;
     c64* Ms = reinterpret_cast<c64*>(smem) + (size_t)wave * wstride;
-    unsigned char* tail = smem + (size_t)W * wstride * sizeof(c64) + (size_t)wave * (2 * NP * sizeof(c64) + 2 * NP * sizeof(int));
-    c64* colbuf = reinterpret_cast<c64*>(tail);
+    c64* colbuf = Ms;
     c64* rowbuf = colbuf + NP;
     int* pivrow = reinterpret_cast<int*>(rowbuf + NP);
     int* steprow = pivrow + NP;
@@ -106,33 +109,36 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
         s = readlane_d(s, 0);
         ok = s > 0.0;
     }
+    // Every wave has read its tile (and the totals store has read all of
+    // them) before the elimination buffers overwrite it.
+    __syncthreads();
     if (ok) {
         rs = 1.0 / s;
 #pragma unroll
         for (int r = 0; r < B; ++r)
 #pragma unroll
             for (int c = 0; c < B; ++c) A[r][c] = cmk(A[r][c].x * rs, A[r][c].y * rs);
+        // Padding rows and columns (index >= N) hold zeros and stay zero: a
+        // padding row never wins a pivot search against a nonzero entry and
+        // ties go to the lower (real) row, so the steps need no bounds
+        // checks. piv, pr and ck are wave-uniform and the lane conditions are
+        // applied with selects: every step is straight-line code.
         unsigned used = 0;  // bit r: own row lr*B+r already pivoted
-        for (int ck = 0; ck < 8 && ok; ++ck) {
-            if (ck * B >= N) break;
+        const int nck = (N + B - 1) / B;
+        for (int ck = 0; ck < nck && ok; ++ck) {
+            const bool is_c = lc == ck;
 #pragma unroll
             for (int kk = 0; kk < B; ++kk) {
                 const int k = ck * B + kk;
-                if (k >= N || !ok) break;
+                if (k >= N) break;
                 double mag = -1.0;
-                int idx = 0x7fffffff;
-                if (lc == ck) {
+                int idx = lr * B;
 #pragma unroll
-                    for (int r = 0; r < B; ++r) {
-                        const int i = lr * B + r;
-                        if (i < N) {
-                            colbuf[i] = A[r][kk];
-                            if (!((used >> r) & 1u)) {
-                                const double mg = cabs1(A[r][kk]);
-                                if (mg > mag) { mag = mg; idx = i; }
-                            }
-                        }
-                    }
+                for (int r = 0; r < B; ++r) {
+                    const double mg = ((used >> r) & 1u) ? -1.0 : cabs1(A[r][kk]);
+                    const bool gt = mg > mag;
+                    mag = gt ? mg : mag;
+                    idx = gt ? lr * B + r : idx;
                 }
                 argmax8(mag, idx);
                 const int piv = __builtin_amdgcn_readlane(idx, 8 * ck);
@@ -142,17 +148,18 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
                     break;
                 }
                 const int pg = piv / B, pr = piv - pg * B;
-                if (lr == pg) {
+                const bool in_pg = lr == pg;
+                if (is_c) {
 #pragma unroll
-                    for (int r = 0; r < B; ++r)
-                        if (r == pr) {
-#pragma unroll
-                            for (int c = 0; c < B; ++c) {
-                                const int j = lc * B + c;
-                                if (j < N) rowbuf[j] = A[r][c];
-                            }
-                        }
+                    for (int r = 0; r < B; ++r) colbuf[lr * B + r] = A[r][kk];
                 }
+                static_for<0, B>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    if (pr == r && in_pg) {
+#pragma unroll
+                        for (int c = 0; c < B; ++c) rowbuf[lc * B + c] = A[r][c];
+                    }
+                });
                 if (lane == 0) {
                     pivrow[k] = piv;
                     steprow[piv] = k;
@@ -161,39 +168,35 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
                 const c64 pv = colbuf[piv];
                 c64 f[B], R[B];
 #pragma unroll
-                for (int r = 0; r < B; ++r) {
-                    const int i = lr * B + r;
-                    f[r] = (i < N) ? colbuf[i] : cmk(0.0, 0.0);
-                }
+                for (int r = 0; r < B; ++r) f[r] = colbuf[lr * B + r];
 #pragma unroll
-                for (int c = 0; c < B; ++c) {
-                    const int j = lc * B + c;
-                    R[c] = (j < N) ? rowbuf[j] : cmk(0.0, 0.0);
-                }
-                const c64 inv = crecip(pv);
+                for (int c = 0; c < B; ++c) R[c] = rowbuf[lc * B + c];
+                const c64 inv = crecip_sel(pv);
 #pragma unroll
                 for (int c = 0; c < B; ++c) R[c] = cmul(R[c], inv);
+                // Column k of the pivot column group becomes -f/pivot: zero
+                // it before the update and let R[kk] = 1/pivot there, which
+                // is also the new pivot entry.
+                // (Selections are written as products with 0/1 weights, exact
+                // for finite entries, so that no select of two array loads is
+                // formed: such a select becomes a dynamic register index.)
+                const double zc = is_c ? 0.0 : 1.0, oc = 1.0 - zc;
+                R[kk] = cmk(fma(inv.x, oc, R[kk].x * zc), fma(inv.y, oc, R[kk].y * zc));
+#pragma unroll
+                for (int r = 0; r < B; ++r) A[r][kk] = cmk(A[r][kk].x * zc, A[r][kk].y * zc);
+                // Pivot row <- scaled pivot row: zero it and let f = -1 there.
+#pragma unroll
+                for (int r = 0; r < B; ++r) {
+                    const double zr = (in_pg && pr == r) ? 0.0 : 1.0;
+                    f[r] = cmk(fma(f[r].x, zr, zr - 1.0), f[r].y * zr);
+#pragma unroll
+                    for (int c = 0; c < B; ++c) A[r][c] = cmk(A[r][c].x * zr, A[r][c].y * zr);
+                }
 #pragma unroll
                 for (int r = 0; r < B; ++r)
 #pragma unroll
                     for (int c = 0; c < B; ++c) A[r][c] = cfms(A[r][c], f[r], R[c]);
-                // Pivot row <- scaled pivot row; pivot column <- -f/pivot; pivot <- 1/pivot.
-                if (lr == pg) {
-#pragma unroll
-                    for (int r = 0; r < B; ++r)
-                        if (r == pr) {
-#pragma unroll
-                            for (int c = 0; c < B; ++c) A[r][c] = R[c];
-                            used |= 1u << r;
-                        }
-                }
-                if (lc == ck) {
-#pragma unroll
-                    for (int r = 0; r < B; ++r) {
-                        const c64 tt = cmul(f[r], inv);
-                        A[r][kk] = (lr == pg && r == pr) ? inv : cmk(-tt.x, -tt.y);
-                    }
-                }
+                used |= in_pg ? (1u << pr) : 0u;
                 wave_sync();
             }
         }
