@@ -62,13 +62,22 @@ __device__ inline c64 crecip(c64 z) {
     double r = z.x / z.y, d = z.y + z.x * r;
     return cmk(r / d, -1.0 / d);
 }
-// The same quotient with selects instead of branches and two divisions
-// (b/a and 1/d, shared by both components): for wave-uniform pivots.
+// 1/a from the hardware reciprocal refined by two Newton steps (within an
+// ulp or two for normal a; no IEEE division sequence).
+__device__ inline double rcp_nr(double a) {
+    double x = __builtin_amdgcn_rcp(a);
+    double e = fma(-a, x, 1.0);
+    x = fma(x, e, x);
+    e = fma(-a, x, 1.0);
+    return fma(x, e, x);
+}
+// The same quotient with selects instead of branches and two refined
+// reciprocals (1/a and 1/d): for wave-uniform pivots of normal magnitude.
 __device__ inline c64 crecip_sel(c64 z) {
     const bool sw = fabs(z.y) > fabs(z.x);
     const double a = sw ? z.y : z.x, b = sw ? z.x : z.y;
-    const double r = b / a;
-    const double t = 1.0 / fma(b, r, a);
+    const double r = b * rcp_nr(a);
+    const double t = rcp_nr(fma(b, r, a));
     const double u = r * t;
     return sw ? cmk(u, -t) : cmk(t, -u);
 }

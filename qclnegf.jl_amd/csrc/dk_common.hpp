@@ -156,9 +156,11 @@ __device__ inline void tile_store(const c64* lds, int wstride, c64* dst, long lo
 template <int CTRL>
 __device__ inline void dpp_argmax_step(double& mag, int& idx) {
     long long bits = __double_as_longlong(mag);
-    int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
-    int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
-    int oi = __builtin_amdgcn_update_dpp(0, idx, CTRL, 0xF, 0xF, false);
+    // Every lane reads a lane of its own row (quad_perm, row_half_mirror), so
+    // the old value of the destination is never used.
+    int lo = __builtin_amdgcn_mov_dpp((int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, true);
+    int hi = __builtin_amdgcn_mov_dpp((int)(bits >> 32), CTRL, 0xF, 0xF, true);
+    int oi = __builtin_amdgcn_mov_dpp(idx, CTRL, 0xF, 0xF, true);
     double om = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
     // Larger magnitude, ties to the lower row; selects, no divergent branch.
     const bool take = (om > mag) | ((om == mag) & (oi < idx));

@@ -123,7 +123,11 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
         // ties go to the lower (real) row, so the steps need no bounds
         // checks. piv, pr and ck are wave-uniform and the lane conditions are
         // applied with selects: every step is straight-line code.
-        unsigned used = 0;  // bit r: own row lr*B+r already pivoted
+        // pen[r]: 0 for an own row not yet pivoted, -inf once it has been, so
+        // that its candidate magnitude |Re|+|Im| + pen never wins.
+        double pen[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) pen[r] = 0.0;
         const int nck = (N + B - 1) / B;
         for (int ck = 0; ck < nck && ok; ++ck) {
             const bool is_c = lc == ck;
@@ -135,7 +139,7 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
                 int idx = lr * B;
 #pragma unroll
                 for (int r = 0; r < B; ++r) {
-                    const double mg = ((used >> r) & 1u) ? -1.0 : cabs1(A[r][kk]);
+                    const double mg = cabs1(A[r][kk]) + pen[r];
                     const bool gt = mg > mag;
                     mag = gt ? mg : mag;
                     idx = gt ? lr * B + r : idx;
@@ -187,7 +191,9 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
                 // Pivot row <- scaled pivot row: zero it and let f = -1 there.
 #pragma unroll
                 for (int r = 0; r < B; ++r) {
-                    const double zr = (in_pg && pr == r) ? 0.0 : 1.0;
+                    const bool is_p = in_pg && pr == r;
+                    const double zr = is_p ? 0.0 : 1.0;
+                    pen[r] = is_p ? -INFINITY : pen[r];
                     f[r] = cmk(fma(f[r].x, zr, zr - 1.0), f[r].y * zr);
 #pragma unroll
                     for (int c = 0; c < B; ++c) A[r][c] = cmk(A[r][c].x * zr, A[r][c].y * zr);
@@ -196,7 +202,6 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
                 for (int r = 0; r < B; ++r)
 #pragma unroll
                     for (int c = 0; c < B; ++c) A[r][c] = cfms(A[r][c], f[r], R[c]);
-                used |= in_pg ? (1u << pr) : 0u;
                 wave_sync();
             }
         }
