@@ -120,7 +120,11 @@ __global__ void k_thin_reduce(long long M, int K, int J, const c64* __restrict__
         for (int j = 0; j < kMaxRank; ++j)
             if (j < J) acc[j] = cfma(acc[j], a, sb[j + J * q]);
     }
-    for (int j = 0; j < J; ++j) C[row + M * j] = acc[j];
+    // Unrolled with a guard: a runtime-bounded loop over acc would index the
+    // register array dynamically and move it to scratch.
+#pragma unroll
+    for (int j = 0; j < kMaxRank; ++j)
+        if (j < J) C[row + M * j] = acc[j];
 }
 
 // S[row + M q] = f * sum_j C[row + M j] * Bm[q + K j] + beta * S  (q < K).
@@ -166,7 +170,9 @@ __global__ void k_lowrank_partial(int NE, long long Nc, int r, const c64* __rest
         for (int i = 0; i < kMaxRank; ++i)
             if (i < r) acc[i] = cfma(acc[i], g, RAw[i + (long long)r * q]);
     }
-    for (int i = 0; i < r; ++i) partial[((long long)blockIdx.y * r + i) * NE + e] = acc[i];
+#pragma unroll
+    for (int i = 0; i < kMaxRank; ++i)
+        if (i < r) partial[((long long)blockIdx.y * r + i) * NE + e] = acc[i];
 }
 
 // red[e + NE i] = sum over chunks, in chunk order.
