@@ -71,6 +71,15 @@ __device__ inline double rcp_nr(double a) {
     e = fma(-a, x, 1.0);
     return fma(x, e, x);
 }
+// 1/sqrt(a) for a normal positive a: the hardware estimate refined by two
+// Newton steps.
+__device__ inline double rsqrt_nr(double a) {
+    double y = __builtin_amdgcn_rsq(a);
+    double h = 0.5 * a * y;
+    y = fma(y, fma(-h, y, 0.5), y);
+    h = 0.5 * a * y;
+    return fma(y, fma(-h, y, 0.5), y);
+}
 // The same quotient with selects instead of branches and two refined
 // reciprocals (1/a and 1/d): for wave-uniform pivots of normal magnitude.
 __device__ inline c64 crecip_sel(c64 z) {

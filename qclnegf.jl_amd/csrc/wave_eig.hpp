@@ -44,20 +44,34 @@ __device__ inline void rstore(c64* T, int N, int lane, const c64 (&v)[B][B]) {
         }
 }
 
+// One DPP lane permute of a double (every lane reads a lane of its own row).
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Wave reductions: DPP within each row of 16 lanes (xor 1, xor 2, then the
+// half-row and row mirrors, after which every lane of a row holds the row's
+// value), then the four rows read back in a fixed order. Every lane gets
+// the same result; no LDS traffic.
+template <typename Op>
+__device__ inline double wave_reduce(double v, Op op) {
+    v = op(v, dpp_d<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = op(v, dpp_d<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = op(v, dpp_d<0x141>(v));  // row_half_mirror
+    v = op(v, dpp_d<0x140>(v));  // row_mirror
+    return op(op(readlane_d(v, 0), readlane_d(v, 16)), op(readlane_d(v, 32), readlane_d(v, 48)));
+}
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += shfl_xor_d(v, off);
-    return v;
+    return wave_reduce(v, [](double a, double b) { return a + b; });
 }
 __device__ inline double wave_max(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, shfl_xor_d(v, off));
-    return v;
+    return wave_reduce(v, [](double a, double b) { return fmax(a, b); });
 }
 __device__ inline double wave_min(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmin(v, shfl_xor_d(v, off));
-    return v;
+    return wave_reduce(v, [](double a, double b) { return fmin(a, b); });
 }
 
 // Squared Frobenius norm of a tile (optionally of tile - c I).
@@ -165,38 +179,44 @@ __device__ inline bool chol_screen(c64 (&a)[B][B], int N, int lane, EigWork<B>* 
 #pragma unroll
         for (int c = 0; c < B; ++c)
             if (lr * B + r == lc * B + c) a[r][c].x += shift;
-    for (int k = 0; k < N; ++k) {
-        const int kg = k / B, kc = k - kg * B;
-        if (lc == kg) {
+    // Steps k = ck*B + kk with kk unrolled, so the pivot column is a fixed
+    // register column of the lanes lc == ck; row and column masks are 0/1
+    // weights. Padding rows and columns (index >= N) hold zeros off the
+    // diagonal and never change.
+    c64* u = w->u;
+    const int nck = (N + B - 1) / B;
+    for (int ck = 0; ck < nck; ++ck) {
+        const bool is_c = lc == ck;
 #pragma unroll
-            for (int c = 0; c < B; ++c) {
-                if (c != kc) continue;
+        for (int kk = 0; kk < B; ++kk) {
+            const int k = ck * B + kk;
+            if (k >= N) break;
+            if (is_c) {
 #pragma unroll
-                for (int r = 0; r < B; ++r) {
-                    const int i = lr * B + r;
-                    if (i >= k && i < N) w->u[i] = a[r][c];
-                }
+                for (int r = 0; r < B; ++r) u[lr * B + r] = a[r][kk];
             }
-        }
-        wave_sync();
-        const double piv = w->u[k].x;
-        if (!(piv > 0.0)) {
             wave_sync();
-            return false;
+            const double piv = u[k].x;
+            if (!(piv > 0.0)) {
+                wave_sync();
+                return false;
+            }
+            const double rs = rsqrt_nr(piv);
+            c64 li[B], lj[B];
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                const double wi = lr * B + r > k ? rs : 0.0;
+                const double wj = lc * B + r > k ? rs : 0.0;
+                const c64 ui = u[lr * B + r], uj = u[lc * B + r];
+                li[r] = cmk(wi * ui.x, wi * ui.y);
+                lj[r] = cmk(wj * uj.x, -(wj * uj.y));
+            }
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int c = 0; c < B; ++c) a[r][c] = cfms(a[r][c], li[r], lj[c]);
+            wave_sync();  // the next step rewrites u
         }
-        const double rs = 1.0 / sqrt(piv);
-        c64 li[B], lj[B];
-#pragma unroll
-        for (int r = 0; r < B; ++r) {
-            const int i = lr * B + r, j = lc * B + r;
-            li[r] = (i > k && i < N) ? cscale(rs, w->u[i]) : cmk(0.0, 0.0);
-            lj[r] = (j > k && j < N) ? cscale(rs, w->u[j]) : cmk(0.0, 0.0);
-        }
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int c = 0; c < B; ++c) a[r][c] = cfms(a[r][c], li[r], cconj(lj[c]));
-        wave_sync();  // the next step rewrites u
     }
     return true;
 }
