@@ -372,6 +372,19 @@ int qclhip_selfenergy_residual(qclhip_ctx* ctx, int64_t n, const double* const* 
 int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
                                const double* const* cand, double alpha, const int* partner, double* ratios_host);
 
+/* The same mixing pass that also writes the totals the next GF solve sums
+ * (production_solver.jl:819-832 followed by _total_family_production,
+ * production_operators.jl:1447-1518): pairs are grouped into ntot totals
+ * (tot_count[t] consecutive entries of tot_members, each a pair index, in
+ * summation order; every pair in exactly one total, a couple inside one
+ * total); totals[t] <- the left-fold sum of the mixed values of its
+ * members, bitwise equal to summing the mixed families afterwards. Ratios
+ * as qclhip_mix_fields_residual (same term numbering). Deferrable. */
+int qclhip_mix_fields_residual_totals(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
+                                      const double* const* cand, double alpha, const int* partner, int ntot,
+                                      const int* tot_count, const int* tot_members, double* const* totals,
+                                      double* ratios_host);
+
 /* ---- The Poisson step closing the SCBA loop (production_solver.jl:
  * 1044-1047). All arrays are device memory. */
 /* _electron_density_bar (src/physics/observables.jl:1-26): nbar (Nz) from

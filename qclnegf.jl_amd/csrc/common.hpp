@@ -43,6 +43,11 @@ __device__ inline c64 cfmac(c64 c, c64 a, c64 b) {
 }
 __device__ inline c64 cconj(c64 a) { return cmk(a.x, -a.y); }
 __device__ inline double cabs1(c64 a) { return fabs(a.x) + fabs(a.y); }
+// Linear mixing (1 - alpha) o + alpha c (beta = 1 - alpha) with one explicit
+// rounding order, so that every mixing kernel produces the same bits.
+__device__ inline c64 mix1(c64 o, c64 c, double alpha, double beta) {
+    return cmk(fma(alpha, c.x, beta * o.x), fma(alpha, c.y, beta * o.y));
+}
 // Calls f(std::integral_constant<int, I>) for I = First..Last-1: the index
 // is a compile-time constant in the body, so a comparison with a runtime
 // value never turns a register array access into a dynamic index.

@@ -750,7 +750,7 @@ __global__ void k_mix(long long n, c64* __restrict__ old_, const c64* __restrict
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         c64 o = old_[i], c = cand[i];
-        old_[i] = cmk(beta * o.x + alpha * c.x, beta * o.y + alpha * c.y);
+        old_[i] = mix1(o, c, alpha, beta);
     }
 }
 
@@ -768,7 +768,7 @@ __global__ void k_mix_many(long long n, MixArgs a, double alpha) {
          i += (long long)gridDim.x * blockDim.x) {
         for (int p = 0; p < a.npairs; ++p) {
             const c64 o = a.old_[p][i], c = a.cand[p][i];
-            a.old_[p][i] = cmk(beta * o.x + alpha * c.x, beta * o.y + alpha * c.y);
+            a.old_[p][i] = mix1(o, c, alpha, beta);
         }
     }
 }

@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_mix_residual(long long n, MixResArgs a,
         const double dx = c1.x - o1.x, dy = c1.y - o1.y;
         acc[0] += dx * dx + dy * dy;
         acc[1] += c1.x * c1.x + c1.y * c1.y;
-        if (alpha != 0.0) oa[i] = cmk(beta * o1.x + alpha * c1.x, beta * o1.y + alpha * c1.y);
+        if (alpha != 0.0) oa[i] = mix1(o1, c1, alpha, beta);
         if (pb >= 0) {
             const double ex = c2.x - o2.x, ey = c2.y - o2.y;
             acc[2] += ex * ex + ey * ey;
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void k_mix_residual(long long n, MixResArgs a,
             const double tx = c1.x + c2.x, ty = c1.y + c2.y;
             acc[4] += sx * sx + sy * sy;
             acc[5] += tx * tx + ty * ty;
-            if (alpha != 0.0) ob[i] = cmk(beta * o2.x + alpha * c2.x, beta * o2.y + alpha * c2.y);
+            if (alpha != 0.0) ob[i] = mix1(o2, c2, alpha, beta);
         }
     };
     const c64 z = cmk(0.0, 0.0);
@@ -315,6 +315,90 @@ __global__ __launch_bounds__(256) void k_mix_residual(long long n, MixResArgs a,
 
 // One block per term: ratio ||c - o|| / (||c|| + 1e-14), Inf when non-finite;
 // with raw != NULL also the sums themselves (raw[2t], raw[2t+1]).
+// The same mixing pass grouped by total: group t mixes its member pairs (in
+// summation order) and writes totals[t] = the left-fold sum of their mixed
+// values, i.e. what the family sum of the next GF solve would load
+// (fam[0] + fam[1] + ...), so that solve reads one family. A couple (the
+// plus/minus embedding pair) lies inside one group. Residual partials as in
+// k_mix_residual, accumulated over the same elements in the same order.
+constexpr int kMaxTotals = 4;
+struct MixTotArgs {
+    c64* old_[kMaxResPairs];
+    const c64* cand[kMaxResPairs];
+    int mem[kMaxTotals][kMaxFamilies];  // member pairs of each total
+    int nmem[kMaxTotals];
+    int ca[kMaxTotals], cb[kMaxTotals], ct[kMaxTotals];  // couple: member positions and term, -1 none
+    c64* tot[kMaxTotals];
+};
+
+__global__ __launch_bounds__(256) void k_mix_totals(long long n, MixTotArgs a, double alpha, double* part) {
+    constexpr int kAcc = 2 * kMaxFamilies + 2;
+    __shared__ double red[kAcc][256];
+    const int t = blockIdx.y;
+    const int nm = a.nmem[t], pa = a.ca[t], pb = a.cb[t];
+    c64* __restrict__ tot = a.tot[t];
+    const double beta = 1.0 - alpha;
+    double acc[kAcc];
+#pragma unroll
+    for (int k = 0; k < kAcc; ++k) acc[k] = 0.0;
+    const long long stride = (long long)kMixChunks * 256;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        c64 o[kMaxFamilies], c[kMaxFamilies];
+#pragma unroll
+        for (int f = 0; f < kMaxFamilies; ++f)  // all loads first
+            if (f < nm) {
+                o[f] = a.old_[a.mem[t][f]][i];
+                c[f] = a.cand[a.mem[t][f]][i];
+            }
+        c64 sum = cmk(0.0, 0.0), o1 = sum, c1 = sum, o2 = sum, c2 = sum;
+#pragma unroll
+        for (int f = 0; f < kMaxFamilies; ++f)
+            if (f < nm) {
+                const double dx = c[f].x - o[f].x, dy = c[f].y - o[f].y;
+                acc[2 * f] += dx * dx + dy * dy;
+                acc[2 * f + 1] += c[f].x * c[f].x + c[f].y * c[f].y;
+                const c64 v = alpha != 0.0 ? mix1(o[f], c[f], alpha, beta) : o[f];
+                if (alpha != 0.0) a.old_[a.mem[t][f]][i] = v;
+                sum = f == 0 ? v : cadd(sum, v);
+                if (f == pa) {
+                    o1 = o[f];
+                    c1 = c[f];
+                }
+                if (f == pb) {
+                    o2 = o[f];
+                    c2 = c[f];
+                }
+            }
+        if (pb >= 0) {
+            const double sx = (c1.x + c2.x) - (o1.x + o2.x), sy = (c1.y + c2.y) - (o1.y + o2.y);
+            const double tx = c1.x + c2.x, ty = c1.y + c2.y;
+            acc[2 * kMaxFamilies] += sx * sx + sy * sy;
+            acc[2 * kMaxFamilies + 1] += tx * tx + ty * ty;
+        }
+        tot[i] = sum;
+    }
+#pragma unroll
+    for (int k = 0; k < kAcc; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int k = 0; k < kAcc; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int f = 0; f < nm; ++f) {
+            const int p = a.mem[t][f];
+            part[(2 * p + 0) * kMixChunks + blockIdx.x] = red[2 * f][0];
+            part[(2 * p + 1) * kMixChunks + blockIdx.x] = red[2 * f + 1][0];
+        }
+        if (pb >= 0) {
+            part[(2 * a.ct[t] + 0) * kMixChunks + blockIdx.x] = red[2 * kMaxFamilies][0];
+            part[(2 * a.ct[t] + 1) * kMixChunks + blockIdx.x] = red[2 * kMaxFamilies + 1][0];
+        }
+    }
+}
+
 __global__ __launch_bounds__(kMixChunks) void k_mix_residual_reduce(const double* part, double* ratios, double* raw) {
     __shared__ double red[2][kMixChunks];
     const int t = blockIdx.x;
@@ -490,6 +574,66 @@ int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* c
                                const double* const* cand, double alpha, const int* partner, double* ratios_host) {
     if (!ratios_host) return QCLHIP_ERR_ARGUMENT;
     return mix_fields_residual_impl(ctx, n, npairs, old_, cand, alpha, partner, ratios_host, nullptr);
+}
+
+int qclhip_mix_fields_residual_totals(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
+                                      const double* const* cand, double alpha, const int* partner, int ntot,
+                                      const int* tot_count, const int* tot_members, double* const* totals,
+                                      double* ratios_host) {
+    if (!ctx || !old_ || !cand || !partner || !tot_count || !tot_members || !totals || !ratios_host || n < 0)
+        return QCLHIP_ERR_ARGUMENT;
+    if (npairs < 1 || npairs > kMaxResPairs) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..32 mixing pairs");
+    if (ntot < 1 || ntot > kMaxTotals) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..4 totals");
+    if (!(alpha >= 0 && alpha <= 1)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "mixing factor must lie in [0,1]");
+    MixTotArgs a{};
+    std::vector<int> owner(npairs, -1), pos(npairs, -1);
+    int k = 0;
+    for (int t = 0; t < ntot; ++t) {
+        if (!totals[t] || tot_count[t] < 1 || tot_count[t] > kMaxFamilies)
+            return fail(ctx, QCLHIP_ERR_ARGUMENT, "each total sums 1..8 pairs");
+        a.tot[t] = (c64*)totals[t];
+        a.nmem[t] = tot_count[t];
+        a.ca[t] = a.cb[t] = a.ct[t] = -1;
+        for (int f = 0; f < tot_count[t]; ++f, ++k) {
+            const int p = tot_members[k];
+            if (p < 0 || p >= npairs || owner[p] >= 0)
+                return fail(ctx, QCLHIP_ERR_ARGUMENT, "every pair belongs to exactly one total");
+            owner[p] = t;
+            pos[p] = f;
+            a.mem[t][f] = p;
+        }
+    }
+    int nterms = npairs;
+    for (int p = 0; p < npairs; ++p) {
+        if (!old_[p] || !cand[p]) return QCLHIP_ERR_ARGUMENT;
+        if (owner[p] < 0) return fail(ctx, QCLHIP_ERR_ARGUMENT, "every pair belongs to exactly one total");
+        const int q = partner[p];
+        if (q >= npairs || q == p || (q >= 0 && partner[q] != p))
+            return fail(ctx, QCLHIP_ERR_ARGUMENT, "partner must be a symmetric pairing of distinct pairs");
+        a.old_[p] = (c64*)old_[p];
+        a.cand[p] = (const c64*)cand[p];
+        if (q > p) {  // couple terms numbered in ascending p, after the pairs
+            const int t = owner[p];
+            if (owner[q] != t || a.cb[t] >= 0)
+                return fail(ctx, QCLHIP_ERR_ARGUMENT, "a couple lies inside one total, at most one per total");
+            a.ca[t] = pos[p];
+            a.cb[t] = pos[q];
+            a.ct[t] = nterms++;
+        }
+    }
+    double* part = (double*)workspace(ctx, ((size_t)2 * nterms * kMixChunks + (size_t)nterms) * sizeof(double));
+    if (!part) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    double* ratios = part + (size_t)2 * nterms * kMixChunks;
+    hipLaunchKernelGGL(k_mix_totals, dim3(kMixChunks, ntot), dim3(256), 0, ctx->stream, (long long)n, a, alpha, part);
+    hipLaunchKernelGGL(k_mix_residual_reduce, dim3(nterms), dim3(kMixChunks), 0, ctx->stream, part, ratios, nullptr);
+    int rc = launch_check(ctx, "mix with totals and self-energy residual");
+    if (rc) return rc;
+    bool deferred = false;
+    rc = defer_result(ctx, ratios, nterms, &deferred);
+    if (rc || deferred) return rc;
+    QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(ratios_host, ratios, nterms * sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+    return qclhip_ctx_synchronize(ctx);
 }
 
 int qclhip_mix_fields_change(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_, const double* const* cand,

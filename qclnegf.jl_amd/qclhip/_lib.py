@@ -127,6 +127,9 @@ _SIGNATURES = {
     "qclhip_fill_zero": [_vp, ct.c_int64, _vp],
     "qclhip_mix_fields_residual": [_vp, ct.c_int64, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_double,
                                    ct.POINTER(ct.c_int), _dp],
+    "qclhip_mix_fields_residual_totals": [_vp, ct.c_int64, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_double,
+                                          ct.POINTER(ct.c_int), ct.c_int, ct.POINTER(ct.c_int),
+                                          ct.POINTER(ct.c_int), ct.POINTER(_vp), _dp],
     "qclhip_batch_create": [_vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp)],
     "qclhip_batch_destroy": [_vp],
     "qclhip_batch_current_green": [_vp, _vp, ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),

@@ -534,11 +534,14 @@ def mix_fields(pairs, alpha: float):
     ctx.call("qclhip_mix_fields", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha))
 
 
-def mix_fields_residual(pairs, alpha: float, partner=None):
+def mix_fields_residual(pairs, alpha: float, partner=None, totals=None):
     """Mix every (old, candidate) pair in one pass and return the self-energy
     residual ratios of the unmixed pairs: one per pair, then one per couple
     (p, partner[p]) with partner[p] > p, for the sum of the two pairs. A
-    Deferred of that many values in deferred-results mode."""
+    Deferred of that many values in deferred-results mode.
+
+    totals: optional list of (field, [pair indices]); each field receives the
+    left-fold sum of its members' mixed values (every pair in one total)."""
     pairs = list(pairs)
     if not pairs:
         raise _lib.ArgumentError("no pairs to mix")
@@ -554,8 +557,19 @@ def mix_fields_residual(pairs, alpha: float, partner=None):
     cands = (_vp * len(pairs))(*[c.data_ptr() for _, c in pairs])
     out = (ct.c_double * nterms)()
     slot0 = ctx.result_slot() if ctx.deferring else 0
-    ctx.call("qclhip_mix_fields_residual", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha),
-             (ct.c_int * len(pairs))(*partner), out)
+    if totals is None:
+        ctx.call("qclhip_mix_fields_residual", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha),
+                 (ct.c_int * len(pairs))(*partner), out)
+    else:
+        totals = list(totals)
+        for t, _ in totals:
+            if t.dtype != torch.complex128 or t.numel() != n or not t.is_contiguous():
+                raise _lib.ArgumentError("totals must be contiguous complex128 fields of the pairs' size")
+        counts = [len(m) for _, m in totals]
+        members = [int(p) for _, m in totals for p in m]
+        ctx.call("qclhip_mix_fields_residual_totals", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha),
+                 (ct.c_int * len(pairs))(*partner), len(totals), (ct.c_int * len(counts))(*counts),
+                 (ct.c_int * len(members))(*members), (_vp * len(totals))(*[t.data_ptr() for t, _ in totals]), out)
     return Deferred(slot0, nterms) if ctx.deferring else list(out)
 
 

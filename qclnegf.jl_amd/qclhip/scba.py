@@ -135,9 +135,14 @@ class ScbaWorkspace:
         self.scat = {m.name: new_family(g) for m in p.mechanisms}
         self.plus = new_family(g)
         self.minus = new_family(g)
-        # the total self-energy of the iteration, written by the GF solve for
-        # the residual suite (which would otherwise sum the families again)
+        # the total self-energy of the iteration: written by the GF solve
+        # when it sums the families, or by the previous linear mixing pass
+        # (which mixes every family anyway); the residual suite reads it.
+        # totals_valid: it holds the sum of the current state's families, so
+        # the GF solve loads it as one family. A caller that changes the
+        # state's families between iterations clears the flag.
         self.totals = new_family(g) if p.measure else None
+        self.totals_valid = False
         self.cand_lesser = api.zeros_field(g) if p.measure else None
         self.anderson = None
         if p.mixing == "anderson":
@@ -165,9 +170,14 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
         return time.perf_counter()
 
     t0 = stamp()
-    fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
-    green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=True,
-                                               totals=ws.totals.parts() if ws.totals is not None else None)
+    if ws.totals is not None and ws.totals_valid:
+        green, info = api.current_green_production(g, p.h, [ws.totals.parts()], p.target, state=s.green,
+                                                   spectral=True)
+    else:
+        fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
+        green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=True,
+                                                   totals=ws.totals.parts() if ws.totals is not None else None)
+        ws.totals_valid = ws.totals is not None
     s.green = green
     t1 = stamp()
     roundoff = _candidate_scattering(p, green, ws.scat)
@@ -201,8 +211,14 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
             sigma = api.mix_fields_residual(pairs, 0.0, partner)  # residuals only, before the update
         anderson_info = ws.anderson.mix([o for o, _ in pairs], [c for _, c in pairs], p.alpha_mix,
                                         len(p.mechanisms) + 2)
+        ws.totals_valid = False
     elif p.measure:
-        sigma = api.mix_fields_residual(pairs, p.alpha_mix, partner)
+        # the next solve's totals, per component in the family order plus,
+        # minus, mechanisms (pairs: mechanisms first, then plus, then minus)
+        M = len(p.mechanisms)
+        members = [[base + c, base + 3 + c] + [3 * k + c for k in range(M)] for c in range(3)]
+        sigma = api.mix_fields_residual(pairs, p.alpha_mix, partner,
+                                        totals=list(zip(ws.totals.parts(), members)))
     else:
         api.mix_fields(pairs, p.alpha_mix)
     if g.ctx.deferring:  # captured or deferred: results are slots of the result buffer

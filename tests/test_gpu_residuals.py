@@ -240,6 +240,40 @@ def test_mix_fields_residual_vs_oracle(hip, orc):
     assert r[4] == np.inf and r[6] == np.inf and r[0] < np.inf
 
 
+def test_mix_fields_residual_totals(hip):
+    """The mixing pass grouped by totals: mixed fields and ratios equal the
+    ungrouped pass bitwise; each total equals the left-fold sum of its
+    members' mixed fields bitwise (what the next solve's family sum loads);
+    the argument checks reject a pair outside every total and a couple split
+    over two totals."""
+    import torch
+    rng = np.random.default_rng(37)
+    shape = (11, 3, 4, 4)
+    olds = [_rand(rng, shape) for _ in range(7)]
+    cands = [o + _rand(rng, shape, 0.1 * (k + 1)) for k, o in enumerate(olds)]
+    dev_o = [hip.to_device(x) for x in olds]
+    dev_o2 = [t.clone() for t in dev_o]
+    dev_c = [hip.to_device(x) for x in cands]
+    partner = [-1, 3, -1, 1, -1, 6, 5]  # couples (1,3) and (5,6)
+    members = [[3, 1, 0], [6, 2, 5, 4]]  # summation order deliberately not ascending
+    tots = [torch.empty_like(dev_o[0]) for _ in members]
+    r1 = hip.mix_fields_residual(list(zip(dev_o, dev_c)), 0.4, partner, totals=list(zip(tots, members)))
+    r2 = hip.mix_fields_residual(list(zip(dev_o2, dev_c)), 0.4, partner)
+    assert r1 == r2
+    for a, b in zip(dev_o, dev_o2):
+        assert torch.equal(a, b)
+    for t, m in zip(tots, members):
+        ref = dev_o[m[0]].clone()
+        for k in m[1:]:
+            ref = ref + dev_o[k]
+        assert torch.equal(t, ref)
+    with pytest.raises(hip._lib.ArgumentError):
+        hip.mix_fields_residual(list(zip(dev_o, dev_c)), 0.4, partner, totals=[(tots[0], [3, 1, 0])])
+    with pytest.raises(hip._lib.ArgumentError):
+        hip.mix_fields_residual(list(zip(dev_o, dev_c)), 0.4, partner,
+                                totals=[(tots[0], [1, 0, 2]), (tots[1], [3, 4, 5, 6])])
+
+
 @pytest.mark.parametrize("engine", ["vector", "matrix"])
 def test_current_green_totals(hip, engine):
     """qclhip_current_green_totals: the Green functions equal those of
