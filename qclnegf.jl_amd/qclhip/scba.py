@@ -143,7 +143,6 @@ class ScbaWorkspace:
         # state's families between iterations clears the flag.
         self.totals = new_family(g) if p.measure else None
         self.totals_valid = False
-        self.cand_lesser = api.zeros_field(g) if p.measure else None
         self.anderson = None
         if p.mixing == "anderson":
             from .anderson import AndersonMixer
@@ -186,10 +185,10 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     t3 = stamp()
     suite = None
     if p.measure:
-        # the unmixed candidate total S<, summed once in the reference's order
-        # ((plus + minus) + mechanism 1 + ...)
-        api.sum_fields([ws.plus.L, ws.minus.L] + [ws.scat[m.name].L for m in p.mechanisms], ws.cand_lesser)
-        suite = api.residual_suite(g, p.h, green, [ws.totals.parts()], [ws.cand_lesser])
+        # the unmixed candidate total S<, summed by the suite as it loads the
+        # candidates, in the reference's order ((plus + minus) + mechanism 1 + ...)
+        suite = api.residual_suite(g, p.h, green, [ws.totals.parts()],
+                                   [ws.plus.L, ws.minus.L] + [ws.scat[m.name].L for m in p.mechanisms])
     t4 = stamp()
     Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
     pops = api.state_populations(g, green.GL, p.target)
