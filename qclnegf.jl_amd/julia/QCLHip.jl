@@ -267,15 +267,28 @@ end
 
 """Linear mixing fused with `production_selfenergy_residual` of the unmixed
 pairs; `partner[p]` (0-based, -1 for none) couples plus and minus. α = 0
-measures only."""
-function mix_fields_residual!(ctx::Context, olds, cands, α::Real, partner::Vector{Cint})
+measures only. With `totals = [(T, members), ...]` (members: 0-based pair
+indices in summation order, every pair in one total) each `T` receives the
+`_total_family_production` sum of its members' mixed values."""
+function mix_fields_residual!(ctx::Context, olds, cands, α::Real, partner::Vector{Cint}; totals = nothing)
     po = Ptr{Cvoid}[_ptr(o) for o in olds]
     pc = Ptr{Cvoid}[_ptr(c) for c in cands]
     nterms = length(olds) + count(p -> partner[p] > p - 1, eachindex(partner))
     out = zeros(Float64, nterms)
-    check(ctx, ccall((:qclhip_mix_fields_residual, libqclhip), Cint,
-                     (Ptr{Cvoid}, Int64, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cdouble, Ptr{Cint}, Ptr{Float64}),
-                     ctx.handle, length(first(olds)), length(olds), po, pc, α, partner, out), "mix_fields_residual")
+    if totals === nothing
+        check(ctx, ccall((:qclhip_mix_fields_residual, libqclhip), Cint,
+                         (Ptr{Cvoid}, Int64, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cdouble, Ptr{Cint}, Ptr{Float64}),
+                         ctx.handle, length(first(olds)), length(olds), po, pc, α, partner, out), "mix_fields_residual")
+    else
+        counts = Cint[length(m) for (_, m) in totals]
+        members = Cint[p for (_, m) in totals for p in m]
+        pt = Ptr{Cvoid}[_ptr(T) for (T, _) in totals]
+        check(ctx, ccall((:qclhip_mix_fields_residual_totals, libqclhip), Cint,
+                         (Ptr{Cvoid}, Int64, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cdouble, Ptr{Cint}, Cint, Ptr{Cint},
+                          Ptr{Cint}, Ptr{Ptr{Cvoid}}, Ptr{Float64}),
+                         ctx.handle, length(first(olds)), length(olds), po, pc, α, partner, length(totals), counts,
+                         members, pt, out), "mix_fields_residual_totals")
+    end
     return out
 end
 
