@@ -218,6 +218,16 @@ inline unsigned grid_blocks(long long n, int threads, int sm_count) {
     return (unsigned)(blocks < 1 ? 1 : blocks);
 }
 
+// Grid of an (energy, column) kernel: x covers the NE energies with 256
+// threads per block, y strides over the columns (about 16 blocks per CU).
+inline dim3 ecol_grid(int NE, long long ncols, int sm_count) {
+    const long long gx = ((long long)NE + 255) / 256;
+    long long gy = (long long)(sm_count > 0 ? sm_count : 32) * 16 / (gx > 0 ? gx : 1);
+    gy = gy < 1 ? 1 : (gy > ncols ? ncols : gy);
+    gy = gy > 65535 ? 65535 : gy;
+    return dim3((unsigned)(gx < 1 ? 1 : gx), (unsigned)(gy < 1 ? 1 : gy));
+}
+
 }  // namespace qclhip
 
 #define QCLHIP_TRY_HIP(ctx, expr)                                              \

@@ -54,19 +54,23 @@ __global__ void k_mi_reduce(int NE, int Nk, int N2, const double* __restrict__ w
     }
 }
 
+// S[e + NE (m + Nk q)] = f w_m Y[e + NE q] + beta S; thread = energy, block
+// rows stride over the columns (m, q).
 __global__ void k_mi_broadcast(int NE, int Nk, int N2, const c64* __restrict__ Y, double f, double beta,
                                const double* __restrict__ wout, c64* __restrict__ S) {
-    const long long n = (long long)NE * Nk * N2;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const int e = (int)(idx % NE);
-        const long long q = idx / ((long long)NE * Nk);
-        c64 y = Y[e + (long long)NE * q];
-        const double fw = wout ? f * wout[(idx / NE) % Nk] : f;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NE) return;
+    const long long ncols = (long long)Nk * N2;
+    for (long long col = blockIdx.y; col < ncols; col += gridDim.y) {
+        const long long q = col / Nk;  // uniform over the block
+        const int m = (int)(col - q * Nk);
+        const long long idx = col * NE + e;
+        const c64 y = Y[e + (long long)NE * q];
+        const double fw = wout ? f * wout[m] : f;
         c64 v = cmk(fw * y.x, fw * y.y);
         if (beta != 0.0) {
-            c64 s = S[idx];
-            v = cmk(beta * s.x + v.x, beta * s.y + v.y);
+            const c64 o = S[idx];
+            v = cmk(beta * o.x + v.x, beta * o.y + v.y);
         }
         S[idx] = v;
     }
@@ -185,17 +189,22 @@ __global__ void k_lowrank_sum(int NE, int r, int nchunk, const c64* __restrict__
     red[e + (long long)NE * i] = v;
 }
 
-// S2[e + NE q] = f sum_i red[e + NE i] left[q + Nc i] + beta S2.
+// S2[e + NE q] = f sum_i red[e + NE i] left[q + Nc i] + beta S2; thread =
+// energy (its r reduced values held in registers), block rows stride over q.
 __global__ void k_lowrank_expand(int NE, long long Nc, int r, const c64* __restrict__ red,
                                  const c64* __restrict__ left, double f, double beta, c64* __restrict__ S) {
-    const long long n = (long long)NE * Nc;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const int e = (int)(idx % NE);
-        const long long q = idx / NE;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NE) return;
+    c64 rv[kMaxRank];
+#pragma unroll
+    for (int i = 0; i < kMaxRank; ++i) rv[i] = i < r ? red[e + (long long)NE * i] : cmk(0.0, 0.0);
+    for (long long q = blockIdx.y; q < Nc; q += gridDim.y) {
         c64 v = cmk(0.0, 0.0);
-        for (int i = 0; i < r; ++i) v = cfma(v, red[e + (long long)NE * i], left[q + Nc * i]);
+#pragma unroll
+        for (int i = 0; i < kMaxRank; ++i)
+            if (i < r) v = cfma(v, rv[i], left[q + Nc * i]);
         v = cmk(f * v.x, f * v.y);
+        const long long idx = q * NE + e;
         if (beta != 0.0) {
             const c64 o = S[idx];
             v = cmk(beta * o.x + v.x, beta * o.y + v.y);
@@ -259,7 +268,7 @@ int contract_core(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double 
             // S2 (NE x Nc) = f red left^T + beta S2, streamed over q.
             const size_t lds = 0;
             (void)lds;
-            hipLaunchKernelGGL(k_lowrank_expand, dim3(grid_blocks(NE * Nc, 256, ctx->sm_count)), dim3(256), 0,
+            hipLaunchKernelGGL(k_lowrank_expand, ecol_grid((int)NE, Nc, ctx->sm_count), dim3(256), 0,
                                ctx->stream, (int)NE, Nc, (int)r, red, K->a, f, beta, S);
             return launch_check(ctx, "low-rank contraction");
         }
@@ -273,7 +282,7 @@ int contract_core(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double 
             if (st) return st;
             st = gemm_nt(ctx, NE, N2, N2, 1.0, X, NE, K->a, N2, 0.0, Y, NE);
             if (st) return st;
-            hipLaunchKernelGGL(k_mi_broadcast, dim3(grid_blocks(NE * Nk * N2, 256, ctx->sm_count)), dim3(256), 0,
+            hipLaunchKernelGGL(k_mi_broadcast, ecol_grid((int)NE, (long long)Nk * N2, ctx->sm_count), dim3(256), 0,
                                ctx->stream, (int)NE, (int)Nk, (int)N2, Y, f, beta,
                                K->weights_out ? (const double*)g->wk : nullptr, S);
             return launch_check(ctx, "k_mi_broadcast");

@@ -7,62 +7,65 @@
 namespace qclhip {
 
 // apply_energy_shift! (production_cache.jl:300-325): column-wise two-point
-// interpolation along energy; consecutive threads = consecutive energies.
-__global__ void k_apply_shift(int NE, long long n, const int* __restrict__ left, const int* __restrict__ right,
+// interpolation along energy. Thread = energy (its interpolation entry read
+// once), block rows stride over the columns; consecutive threads touch
+// consecutive energies.
+__global__ void k_apply_shift(int NE, long long ncols, const int* __restrict__ left, const int* __restrict__ right,
                               const double* __restrict__ wr, const unsigned char* __restrict__ valid,
                               const c64* __restrict__ X, c64* __restrict__ Y) {
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const int e = (int)(idx % NE);
-        if (!valid[e]) {
-            Y[idx] = cmk(0.0, 0.0);
-            continue;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NE) return;
+    const bool ok = valid[e];
+    const int l = left[e], r = right[e];
+    const double t = wr[e];
+    for (long long col = blockIdx.y; col < ncols; col += gridDim.y) {
+        const long long base = col * NE;
+        c64 y = cmk(0.0, 0.0);
+        if (ok) {
+            const c64 xl = X[base + l];
+            if (l == r || t == 0.0) {
+                y = xl;
+            } else {
+                const c64 xr = X[base + r];
+                y = cmk((1.0 - t) * xl.x + t * xr.x, (1.0 - t) * xl.y + t * xr.y);
+            }
         }
-        const long long base = idx - e;
-        const int l = left[e], r = right[e];
-        const double t = wr[e];
-        c64 xl = X[base + l];
-        if (l == r || t == 0.0) {
-            Y[idx] = xl;
-        } else {
-            c64 xr = X[base + r];
-            Y[idx] = cmk((1.0 - t) * xl.x + t * xr.x, (1.0 - t) * xl.y + t * xr.y);
-        }
+        Y[base + e] = y;
     }
 }
 
 // Thermal LO combination with the shifts applied after the contraction:
 // out = ca * (S_a Y) + cb * (S_b Y), exact since the shift acts on energy
-// and the kernel on (m,a,b) (production_operators.jl:1107-1162).
-__global__ void k_lo_combine(int NE, long long n, const c64* __restrict__ Y, double ca, const int* la,
+// and the kernel on (m,a,b) (production_operators.jl:1107-1162). Layout of
+// k_apply_shift.
+__global__ void k_lo_combine(int NE, long long ncols, const c64* __restrict__ Y, double ca, const int* la,
                              const int* ra, const double* wa, const unsigned char* va, double cb, const int* lb,
                              const int* rb, const double* wb, const unsigned char* vb, c64* __restrict__ out) {
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const int e = (int)(idx % NE);
-        const long long base = idx - e;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NE) return;
+    const bool oka = va[e], okb = vb[e];
+    const int l1 = la[e], r1 = ra[e], l2 = lb[e], r2 = rb[e];
+    const double t1 = wa[e], t2 = wb[e];
+    for (long long col = blockIdx.y; col < ncols; col += gridDim.y) {
+        const long long base = col * NE;
         c64 sa = cmk(0.0, 0.0), sb = cmk(0.0, 0.0);
-        if (va[e]) {
-            const int l = la[e], r = ra[e];
-            const double t = wa[e];
-            c64 yl = Y[base + l];
-            if (l == r || t == 0.0) sa = yl;
+        if (oka) {
+            const c64 yl = Y[base + l1];
+            if (l1 == r1 || t1 == 0.0) sa = yl;
             else {
-                c64 yr = Y[base + r];
-                sa = cmk((1.0 - t) * yl.x + t * yr.x, (1.0 - t) * yl.y + t * yr.y);
+                const c64 yr = Y[base + r1];
+                sa = cmk((1.0 - t1) * yl.x + t1 * yr.x, (1.0 - t1) * yl.y + t1 * yr.y);
             }
         }
-        if (vb[e]) {
-            const int l = lb[e], r = rb[e];
-            const double t = wb[e];
-            c64 yl = Y[base + l];
-            if (l == r || t == 0.0) sb = yl;
+        if (okb) {
+            const c64 yl = Y[base + l2];
+            if (l2 == r2 || t2 == 0.0) sb = yl;
             else {
-                c64 yr = Y[base + r];
-                sb = cmk((1.0 - t) * yl.x + t * yr.x, (1.0 - t) * yl.y + t * yr.y);
+                const c64 yr = Y[base + r2];
+                sb = cmk((1.0 - t2) * yl.x + t2 * yr.x, (1.0 - t2) * yl.y + t2 * yr.y);
             }
         }
-        out[idx] = cmk(ca * sa.x + cb * sb.x, ca * sa.y + cb * sb.y);
+        out[base + e] = cmk(ca * sa.x + cb * sb.x, ca * sa.y + cb * sb.y);
     }
 }
 
@@ -157,7 +160,10 @@ int gamma_field(qclhip_ctx* ctx, long long n, const double* SL, const double* SG
 
 int lo_combine(qclhip_ctx* ctx, int NE, long long n, const c64* Y, double ca, const qclhip_shift* a, double cb,
                const qclhip_shift* b, c64* out) {
-    hipLaunchKernelGGL(k_lo_combine, dim3(grid_blocks(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, NE, n, Y,
+    if (NE < 1 || n % NE) return fail(ctx, QCLHIP_ERR_DIMENSION, "LO combination: field size not a multiple of NE");
+    const long long ncols = n / NE;
+    if (ncols == 0) return QCLHIP_OK;
+    hipLaunchKernelGGL(k_lo_combine, ecol_grid(NE, ncols, ctx->sm_count), dim3(256), 0, ctx->stream, NE, ncols, Y,
                        ca, a->left, a->right, a->wr, a->valid, cb, b->left, b->right, b->wr, b->valid, out);
     return launch_check(ctx, "k_lo_combine");
 }
@@ -201,10 +207,10 @@ int qclhip_antihermitian_part(qclhip_ctx* ctx, const qclhip_grid* grid, double* 
 int qclhip_apply_shift(qclhip_ctx* ctx, const qclhip_shift* shift, int64_t ncols, const double* X, double* Y) {
     if (!ctx || !shift || !X || !Y || ncols < 0) return QCLHIP_ERR_ARGUMENT;
     if (X == Y) return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
-    const long long n = (long long)shift->NE * ncols;
-    if (n == 0) return QCLHIP_OK;
-    hipLaunchKernelGGL(k_apply_shift, dim3(grid_blocks(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
-                       shift->NE, n, shift->left, shift->right, shift->wr, shift->valid, (const c64*)X, (c64*)Y);
+    if (ncols == 0 || shift->NE == 0) return QCLHIP_OK;
+    hipLaunchKernelGGL(k_apply_shift, ecol_grid(shift->NE, ncols, ctx->sm_count), dim3(256), 0, ctx->stream,
+                       shift->NE, (long long)ncols, shift->left, shift->right, shift->wr, shift->valid,
+                       (const c64*)X, (c64*)Y);
     return launch_check(ctx, "k_apply_shift");
 }
 
