@@ -391,7 +391,9 @@ __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) 
             if (active) keldysh_products<B>(Gs, Xs, N, lane, p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr);
             __syncthreads();
             tile_store<W, LD>(X0, wstride, dst, blk0, mask, N, S);
-            __syncthreads();
+            // before S> replaces the tile; after comp 1 the next tile's
+            // barrier at the top of the loop does this
+            if (comp == 0) __syncthreads();
         }
     }
 }
