@@ -11,8 +11,9 @@
 // izamax choice of zgetrf, production_operators.jl:20-41, and of the owned
 // small inverse, :84-139). Rows are never swapped: the pivot row is
 // remembered and the permutation is undone when the inverse is written to
-// LDS. Per step the pivot column and the scaled pivot row travel through
-// LDS; every lane then does B*B complex FMAs on its own registers.
+// LDS. Per step the pivot column and the pivot row travel through LDS;
+// every lane scales the row by the pivot's reciprocal and does B*B complex
+// FMAs on its own registers.
 //
 // Keldysh: with G^R in LDS, T = G^R S and G = T G^R^dagger are formed by the
 // same 8x8 lane grid, each lane accumulating its B x B block.
