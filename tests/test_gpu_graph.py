@@ -97,3 +97,31 @@ def test_deferred_results_match_eager(hip):
     assert list(vals) == [info["lam"], info["number"], info["holes"], info["a"], info["c"]]
     assert dJ.read(host) == J and dn.read(host) == num
     assert np.array_equal(dp.read(host), pops)
+
+
+def test_mixing_totals_equal_family_sums(hip):
+    """The totals written by the mixing pass and loaded as one family by the
+    next GF solve give bitwise the iterations that sum the families at every
+    solve (the workspace's cache cleared before each iteration)."""
+    import torch
+    from qclhip import scba
+    prob = fx.make_problem(N=5, NE=48, Nk=3)
+    full, state, green = fx.oracle_state(prob)
+    g = hip.Grid(prob["eps"], prob["wE"], prob["wk"], prob["Nb"])
+    mechs = [scba.Mechanism(m["name"], hip.ProductionKernel(g, m["kind"], m["a"], m["b"], m["c"], rank=m["rank"]),
+                            m["qK"], m["is_lo"]) for m in prob["mechanisms"]]
+    p = scba.ScbaProblem(g, hip.h_to_device(prob["h"]), hip.matrix_to_device(prob["Tplus"]),
+                         hip.matrix_to_device(prob["Tminus"]), mechs, full["target"], prob["Ep"], prob["hwLO"],
+                         prob["NLO"], prob["alpha_mix"])
+    s_c, s_f = _device_state(hip, prob, state), _device_state(hip, prob, state)
+    ws_c, ws_f = scba.ScbaWorkspace(p), scba.ScbaWorkspace(p)
+    for it in range(4):
+        ws_f.totals_valid = False  # sum the families at this solve
+        rc, rf = scba.scba_iteration(p, s_c, ws_c), scba.scba_iteration(p, s_f, ws_f)
+        assert ws_c.totals_valid
+        for key in ("lam", "number", "Jbar", "r_D", "r_A", "r_K", "r_PSD", "r_caus", "r_Sigma"):
+            assert rc[key] == rf[key], (it, key)
+        for a, b in zip(ws_c.totals.parts(), ws_f.totals.parts()):
+            assert torch.equal(a, b), it
+    for k in ("GR", "GL", "GG"):
+        assert torch.equal(getattr(s_c.green, k), getattr(s_f.green, k)), k
