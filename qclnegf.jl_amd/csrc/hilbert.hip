@@ -233,7 +233,15 @@ struct HilbertArgs {
     c64* out2;          // adjoint mode: S>_bar (accumulated); out holds S<_bar
     const c64* kodd;    // witness: odd half of the 2L kernel spectrum (NULL: no witness)
     const c64* phase;   // witness: exp(-i pi n / L)
+    int stash;          // mode 1: Gamma kept in LDS behind the FFT buffer for the output
 };
+
+// LDS of the FFT exchange (and of the witness reduction), in doubles,
+// rounded up to a 16-byte multiple for the Gamma stash behind it.
+template <int L>
+constexpr int fft_lds_len() {
+    return (((L + L / 32) > 2 * threads_for(L) ? (L + L / 32) : 2 * threads_for(L)) + 1) / 2 * 2;
+}
 
 __device__ inline c64 load_gamma(const HilbertArgs& a, size_t idx) {
     if (a.gamma) return a.gamma[idx];
@@ -315,6 +323,7 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
     const size_t base = (size_t)col * NE;
     const double invL = 1.0 / (double)L;
     const bool witness = a.kodd != nullptr;
+    c64* gst = reinterpret_cast<c64*>(lds + fft_lds_len<L>());  // Gamma stash (a.stash)
     double dn = 0.0, nn = 0.0;
     double re[V], im[V];
     // Load wE*Gamma, zero padded, in the first pass's input order.
@@ -325,6 +334,7 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
         double xr = 0.0, xi = 0.0;
         if (e < NE) {
             const c64 gm = load_gamma(a, base + e);
+            if (a.stash) gst[e] = gm;  // read back at the output, after the passes' barriers
             const double we = a.mode == 3 ? 1.0 : a.wE[e];  // the adjoint weights the output
             xr = we * gm.x;
             xi = we * gm.y;
@@ -369,7 +379,7 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
             continue;
         }
         if (a.mode == 1) {
-            const c64 gm = load_gamma(a, gi);
+            const c64 gm = a.stash ? gst[e] : load_gamma(a, gi);
             a.out[gi] = cmk(lam.x + 0.5 * gm.y, lam.y - 0.5 * gm.x);  // Lambda - 0.5 i Gamma
         } else {
             a.out[gi] = lam;
@@ -469,19 +479,25 @@ std::vector<c64> kernel_spectrum(int NE, int L, double de, bool odd) {
 template <int L>
 int launch_hilbert_len(qclhip_ctx* ctx, long long ncols, const HilbertArgs& a) {
     constexpr int T = threads_for(L);
-    const size_t lds = (size_t)((L + L / 32) > 2 * T ? (L + L / 32) : 2 * T) * sizeof(double);
+    const size_t lds = (size_t)fft_lds_len<L>() * sizeof(double);
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "FFT length exceeds the LDS of one workgroup");
-    if (lds > 65536)
+    // S^R output: keep Gamma in LDS instead of reading S< and S> again,
+    // while two workgroups still fit on a CU (L <= 4096).
+    HilbertArgs b = a;
+    const size_t lds_main = lds + (size_t)a.NE * sizeof(c64);
+    b.stash = a.mode == 1 && lds_main <= 80 * 1024 && lds_main <= ctx->max_lds_per_block;
+    const size_t lds_k = b.stash ? lds_main : lds;
+    if (lds_k > 65536)
         QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_hilbert<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                (int)lds));
+                                                (int)lds_k));
     if (a.kodd) {
         if (lds > 65536)
             QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_hilbert_odd<L>,
                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_hilbert_odd<L>, dim3((unsigned)ncols), dim3(T), lds, ctx->stream, a);
     }
-    hipLaunchKernelGGL(k_hilbert<L>, dim3((unsigned)ncols), dim3(T), lds, ctx->stream, a);
+    hipLaunchKernelGGL(k_hilbert<L>, dim3((unsigned)ncols), dim3(T), lds_k, ctx->stream, b);
     return launch_check(ctx, "k_hilbert");
 }
 
