@@ -229,6 +229,31 @@ __global__ void k_ifr_radial(int NE, int Nk, int J, const c64* __restrict__ C1, 
 
 }  // namespace
 
+// Thermal LO step of a low-rank kernel (rank <= kMaxRank): both energy
+// shifts act on the NE x r reduced coefficients red = K_right[w G] before
+// the expansion, S = qK left (ca S_a red + cb S_b red)^T, exact since the
+// shifts act on energy and the expansion on (m,a,b); the full-size
+// intermediate K[G] and its two shifted reads are not formed.
+static int lo_lowrank(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double qK, double ca,
+                      const qclhip_shift* a, double cb, const qclhip_shift* b, c64* S) {
+    const qclhip_grid* g = K->grid;
+    const long long NE = g->NE, Nc = (long long)g->Nk * g->Nb * g->Nb, r = K->rank;
+    const long long nchunk = (Nc + kQChunk - 1) / kQChunk;
+    c64* red = (c64*)workspace(ctx, (size_t)(2 * NE * r + nchunk * r * NE) * sizeof(c64));
+    if (!red) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    c64* coef = red + NE * r;
+    c64* partial = coef + NE * r;
+    hipLaunchKernelGGL(k_lowrank_partial, dim3((unsigned)((NE + 255) / 256), (unsigned)nchunk), dim3(256), 0,
+                       ctx->stream, (int)NE, Nc, (int)r, G, K->b, partial);
+    hipLaunchKernelGGL(k_lowrank_sum, dim3((unsigned)((NE * r + 255) / 256)), dim3(256), 0, ctx->stream, (int)NE,
+                       (int)r, (int)nchunk, partial, red);
+    int st = lo_combine(ctx, (int)NE, NE * r, red, ca, a, cb, b, coef);
+    if (st) return st;
+    hipLaunchKernelGGL(k_lowrank_expand, ecol_grid((int)NE, Nc, ctx->sm_count), dim3(256), 0, ctx->stream, (int)NE,
+                       Nc, (int)r, coef, K->a, qK, 0.0, S);
+    return launch_check(ctx, "low-rank LO contraction");
+}
+
 // Device scratch that contract_core takes from the context workspace.
 size_t core_scratch_bytes(const qclhip_kernel* K) {
     const qclhip_grid* g = K->grid;
@@ -478,6 +503,11 @@ int qclhip_lo_contraction(qclhip_ctx* ctx, const qclhip_kernel* kernel, const do
     if (!ctx || !kernel || !GL || !GG || !plus || !minus || !SL || !SG) return QCLHIP_ERR_ARGUMENT;
     const qclhip_grid* g = kernel->grid;
     if (plus->NE != g->NE || minus->NE != g->NE) return fail(ctx, QCLHIP_ERR_DIMENSION, "shift plan has the wrong energy size");
+    if (kernel->kind == QCLHIP_KERNEL_LOW_RANK && kernel->rank <= kMaxRank) {
+        int st = lo_lowrank(ctx, kernel, (const c64*)GL, qK, NLO + 1.0, plus, NLO, minus, (c64*)SL);
+        if (st) return st;
+        return lo_lowrank(ctx, kernel, (const c64*)GG, qK, NLO + 1.0, minus, NLO, plus, (c64*)SG);
+    }
     const long long n = (long long)g->NE * g->Nk * g->Nb * g->Nb;
     // Y = qK K[w G] once per component, then both shifts of Y: the shifts act
     // on energy and K on (m,a,b), so S K[G] = K[S G]. Y sits behind the
