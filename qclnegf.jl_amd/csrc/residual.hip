@@ -31,58 +31,105 @@ struct ResArgs {
     double* out;  // [7][NE*Nk]: rD, specN, specD, kelN, kelD, psd, caus
 };
 
-// One wavefront per (e,m) block with two LDS tiles: G^R, and a working tile
-// X that successively holds S^R, D, Gamma, G Gamma, A, the candidate S<,
-// G S<, G<, G>, and the raw products; the right factor of each product
-// stays in registers.
+// Gamma = i (S> - S<) of the summed families, cooperatively into the W
+// per-wave tiles (the thread mapping of tile_load).
+template <int W, int LD>
+__device__ inline void tile_load_gamma(c64* lds, int wstride, const c64* const* famG, const c64* const* famL,
+                                       int nfam, long long blk0, int nvalid, int N, size_t S) {
+    const int total = N * N * W;
+    int f = threadIdx.x;
+    const int w = f % W;
+    int ij = f / W;
+    int i = ij % N, j = ij / N;
+    for (; f < total; f += 64 * W) {
+        if (w < nvalid) {
+            const size_t idx = blk0 + w + S * (i + (size_t)N * j);
+            const c64 g = fam_load(famG, nfam, idx), l = fam_load(famL, nfam, idx);
+            lds[w * wstride + i * LD + j] = cmk(-(g.y - l.y), g.x - l.x);
+        }
+        i += 64;
+        while (i >= N) {
+            i -= N;
+            ++j;
+        }
+    }
+}
+
+// LDS of one wave of the residual suite: two tiles and the eigen work vectors.
 template <int B>
-__global__ __launch_bounds__(64) void k_residual_suite(ResArgs p) {
+constexpr size_t res_wave_bytes() {
+    return 2 * (size_t)TileStride<B>::value * sizeof(c64) + sizeof(EigWork<B>);
+}
+
+// Waves (consecutive blocks) per workgroup: as many as keep two workgroups
+// within the 160 KiB of LDS of a CU, at most 4 (64-byte load segments).
+template <int B>
+constexpr int res_waves() {
+    return 4 * res_wave_bytes<B>() <= 81920 ? 4 : (2 * res_wave_bytes<B>() <= 81920 ? 2 : 1);
+}
+
+// W waves per workgroup, one (e,m) block per wave, blocks blk0 .. blk0+W-1.
+// Every field is moved into LDS cooperatively (consecutive threads =
+// consecutive blocks, 16W-byte segments) between workgroup barriers; all
+// other work is wave-local. Each wave owns two LDS tiles: G^R, and a working
+// tile X that successively holds S^R, D, Gamma, G Gamma, A, the candidate
+// S<, G S<, G<, G>, and the raw products; the right factor of each product
+// stays in registers. Waves past the last block only join the barriers.
+template <int B, int W>
+__global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
     constexpr int LD = 8 * B + 1;
-    constexpr int W = 1;
     const int wstride = TileStride<B>::value;
     extern __shared__ __align__(16) unsigned char smem[];
-    c64* G = reinterpret_cast<c64*>(smem);
-    c64* X = G + wstride;
-    EigWork<B>* ew = reinterpret_cast<EigWork<B>*>(X + wstride);
-    const int lane = threadIdx.x;
+    c64* Gall = reinterpret_cast<c64*>(smem);
+    c64* Xall = Gall + W * wstride;
+    EigWork<B>* ewall = reinterpret_cast<EigWork<B>*>(Xall + W * wstride);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lr = lane & 7, lc = lane >> 3;
     const long long nblk = (long long)p.NE * p.Nk;
-    // XCD-aware: consecutive energies (sharing cache lines) on one XCD's L2
-    const long long blk = xcd_tile(blockIdx.x, nblk);  // e + NE m
+    // XCD-aware: consecutive tiles (sharing cache lines) on one XCD's L2
+    const long long blk0 = (long long)W * xcd_tile(blockIdx.x, (nblk + W - 1) / W);
+    const int nvalid = (int)min((long long)W, nblk - blk0);
+    const bool valid = W == 1 || wave < nvalid;  // a one-wave tile is never empty
+    const long long blk = blk0 + (valid ? wave : 0);  // e + NE m
     const int e = (int)(blk % p.NE), m = (int)(blk / p.NE);
     const int N = p.N;
     const size_t S = (size_t)nblk;
+    c64* G = Gall + wave * wstride;
+    c64* X = Xall + wave * wstride;
+    EigWork<B>* ew = ewall + wave;
     c64 T[B][B];
     const c64* one[1];
     one[0] = p.GR;
-    tile_load<W, LD>(G, wstride, one, 1, blk, 1, N, S);
-    tile_load<W, LD>(X, wstride, p.famR, p.nfam, blk, 1, N, S);
-    wave_sync();
-    const double caus = causality_from_tile<B, LD>(X, N, lane, ew, T);
-    wave_sync();
-    // r_D: D = eps - h - S^R in X, ||D G - I|| / (||D|| ||G|| + sqrt(N)).
-    for (int f = lane; f < N * N; f += 64) {
-        const int i = f / N, j = f % N;
-        const c64 hv = p.h[(size_t)m + (size_t)p.Nk * (i + (size_t)N * j)];
-        const c64 sv = X[i * LD + j];
-        X[i * LD + j] = cmk((i == j ? p.eps[e] : 0.0) - hv.x - sv.x, -hv.y - sv.y);
-    }
-    wave_sync();
-    rprod<B>(X, G, N, lane, T, false);
-    double dn = 0.0;
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const int i = lr * B + r, j = lc * B + c;
-            if (i < N && j < N) {
-                const c64 v = cmk(T[r][c].x - (i == j ? 1.0 : 0.0), T[r][c].y);
-                dn += v.x * v.x + v.y * v.y;
-            }
+    tile_load<W, LD>(Gall, wstride, one, 1, blk0, nvalid, N, S);
+    tile_load<W, LD>(Xall, wstride, p.famR, p.nfam, blk0, nvalid, N, S);
+    __syncthreads();
+    double caus = 0.0, rD = 0.0;
+    if (valid) {
+        caus = causality_from_tile<B, LD>(X, N, lane, ew, T);
+        wave_sync();
+        // r_D: D = eps - h - S^R in X, ||D G - I|| / (||D|| ||G|| + sqrt(N)).
+        for (int f = lane; f < N * N; f += 64) {
+            const int i = f / N, j = f % N;
+            const c64 hv = p.h[(size_t)m + (size_t)p.Nk * (i + (size_t)N * j)];
+            const c64 sv = X[i * LD + j];
+            X[i * LD + j] = cmk((i == j ? p.eps[e] : 0.0) - hv.x - sv.x, -hv.y - sv.y);
         }
-    const double rD = sqrt(wave_sum(dn)) /
-                      (sqrt(tile_frob2<LD>(X, N, lane, 0.0)) * sqrt(tile_frob2<LD>(G, N, lane, 0.0)) + sqrt((double)N));
-    wave_sync();
+        wave_sync();
+        rprod<B>(X, G, N, lane, T, false);
+        double dn = 0.0;
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int i = lr * B + r, j = lc * B + c;
+                if (i < N && j < N) {
+                    const c64 v = cmk(T[r][c].x - (i == j ? 1.0 : 0.0), T[r][c].y);
+                    dn += v.x * v.x + v.y * v.y;
+                }
+            }
+        rD = sqrt(wave_sum(dn)) /
+             (sqrt(tile_frob2<LD>(X, N, lane, 0.0)) * sqrt(tile_frob2<LD>(G, N, lane, 0.0)) + sqrt((double)N));
+    }
     // Six positivity checks; phase ph prepares X (and the r_A, r_K terms on
     // the way), then checks the Hermitian part of f X:
     //   0: Gamma = i (S> - S<)                          f = 1
@@ -93,39 +140,40 @@ __global__ __launch_bounds__(64) void k_residual_suite(ResArgs p) {
     double psd = 0.0, specN = 0.0, specD = 0.0, kelN = 0.0, kelD = 0.0;
 #pragma unroll 1
     for (int ph = 0; ph < 6; ++ph) {
-        if (ph == 0) {
-            for (int f = lane; f < N * N; f += 64) {
-                const int i = f / N, j = f % N;
-                const size_t idx = blk + S * (i + (size_t)N * j);
-                const c64 g = fam_load(p.famG, p.nfam, idx), l = fam_load(p.famL, p.nfam, idx);
-                X[i * LD + j] = cmk(-(g.y - l.y), g.x - l.x);
+        if (ph != 1) {  // phase 1 works on the Gamma of phase 0
+            __syncthreads();  // every wave is done with its X
+            if (ph == 0) {
+                tile_load_gamma<W, LD>(Xall, wstride, p.famG, p.famL, p.nfam, blk0, nvalid, N, S);
+            } else if (ph == 2) {
+                tile_load<W, LD>(Xall, wstride, p.cand, p.ncand, blk0, nvalid, N, S);
+            } else if (ph == 3) {
+                one[0] = p.GG;
+                tile_load<W, LD>(Xall, wstride, one, 1, blk0, nvalid, N, S);
+            } else if (ph == 4) {
+                tile_load<W, LD>(Xall, wstride, p.famL, p.nfam, blk0, nvalid, N, S);
+            } else {
+                tile_load<W, LD>(Xall, wstride, p.famG, p.nfam, blk0, nvalid, N, S);
             }
-        } else if (ph == 2) {
-            tile_load<W, LD>(X, wstride, p.cand, p.ncand, blk, 1, N, S);
-        } else if (ph == 3) {
-            one[0] = p.GG;
-            tile_load<W, LD>(X, wstride, one, 1, blk, 1, N, S);
-        } else if (ph == 4) {
-            tile_load<W, LD>(X, wstride, p.famL, p.nfam, blk, 1, N, S);
-        } else if (ph == 5) {
-            tile_load<W, LD>(X, wstride, p.famG, p.nfam, blk, 1, N, S);
+            __syncthreads();
         }
-        wave_sync();
-        if (ph != 0 && ph != 3) {  // T = G X G^dagger
+        if (valid && ph != 0 && ph != 3) {  // T = G X G^dagger
             rprod<B>(G, X, N, lane, T, false);
             wave_sync();
             rstore<B>(X, N, lane, T);
             wave_sync();
             rprod<B>(X, G, N, lane, T, true);
             wave_sync();
-            if (ph == 1 || ph == 2) {
-                one[0] = ph == 1 ? p.A : p.GL;
-                tile_load<W, LD>(X, wstride, one, 1, blk, 1, N, S);
-            } else {
+            if (ph >= 4) {
                 rstore<B>(X, N, lane, T);
+                wave_sync();
             }
-            wave_sync();
-            if (ph == 1 || ph == 2) {
+        }
+        if (ph == 1 || ph == 2) {
+            __syncthreads();
+            one[0] = ph == 1 ? p.A : p.GL;
+            tile_load<W, LD>(Xall, wstride, one, 1, blk0, nvalid, N, S);
+            __syncthreads();
+            if (valid) {
                 const double dn2 = tile_reg_diff2<B, LD>(X, T, N, lane);
                 const double xn2 = tile_frob2<LD>(X, N, lane, 0.0);
                 if (ph == 1) {
@@ -137,11 +185,13 @@ __global__ __launch_bounds__(64) void k_residual_suite(ResArgs p) {
                 }
             }
         }
-        const double fi = (ph == 2 || ph == 4) ? -1.0 : ((ph == 3 || ph == 5) ? 1.0 : 0.0);
-        psd = fmax(psd, psd_from_tile<B, LD>(X, N, lane, ew, fi == 0.0 ? 1.0 : 0.0, fi));
-        wave_sync();
+        if (valid) {
+            const double fi = (ph == 2 || ph == 4) ? -1.0 : ((ph == 3 || ph == 5) ? 1.0 : 0.0);
+            psd = fmax(psd, psd_from_tile<B, LD>(X, N, lane, ew, fi == 0.0 ? 1.0 : 0.0, fi));
+            wave_sync();
+        }
     }
-    if (lane == 0) {
+    if (valid && lane == 0) {
         p.out[0 * nblk + blk] = rD;
         p.out[1 * nblk + blk] = specN;
         p.out[2 * nblk + blk] = specD;
@@ -430,13 +480,15 @@ __global__ void k_fill_zero(long long n, c64* __restrict__ x) {
 
 template <int B>
 int launch_residual(qclhip_ctx* ctx, const ResArgs& a) {
-    const size_t lds = 2 * (size_t)TileStride<B>::value * sizeof(c64) + sizeof(EigWork<B>);
+    constexpr int W = res_waves<B>();
+    const size_t lds = W * res_wave_bytes<B>();
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
     if (lds > 65536)
-        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_residual_suite<B>,
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_residual_suite<B, W>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_residual_suite<B>), dim3((unsigned)((long long)a.NE * a.Nk)), dim3(64), lds, ctx->stream, a);
+    const long long ntile = ((long long)a.NE * a.Nk + W - 1) / W;
+    hipLaunchKernelGGL((k_residual_suite<B, W>), dim3((unsigned)ntile), dim3(64 * W), lds, ctx->stream, a);
     return launch_check(ctx, "k_residual_suite");
 }
 
