@@ -304,14 +304,15 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert_odd(HilbertArgs a) {
 
 // The 2L witness: the 2L DFT of the zero-padded column splits into even
 // outputs, the L transform itself, and odd outputs, the L transform of
-// x_n exp(-i pi n/L) (k_hilbert_odd, run first; it parks
-// exp(i pi n/L) y'_n in the output field). With the odd half of the 2L
+// x_n exp(-i pi n/L) (k_hilbert_odd, run first, or this kernel itself when
+// Gamma is stashed in LDS; either parks exp(i pi n/L) y'_n in the output
+// field). With the odd half of the 2L
 // kernel spectrum, the 2L result at n < L is (y_n + exp(i pi n/L) y'_n)/2, so
 // Lambda2 - Lambda = (exp(i pi n/L) y'_n - y_n)/2 compares two independently
 // rounded evaluations of the same convolution, as the reference's 2L
 // transform does, at the cost of one more length-L transform instead of a
 // length-2L one.
-template <int L>
+template <int L, bool FUSE>
 __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
     constexpr int V = vals_per_thread(L), T = threads_for(L);
     constexpr int NPASS = npasses(L);
@@ -326,21 +327,80 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
     c64* gst = reinterpret_cast<c64*>(lds + fft_lds_len<L>());  // Gamma stash (a.stash)
     double dn = 0.0, nn = 0.0;
     double re[V], im[V];
-    // Load wE*Gamma, zero padded, in the first pass's input order.
+    if constexpr (FUSE) {
+        // Fused witness (stash and witness both on): Gamma is read once, into the stash, and the odd half
+        // runs first (the computation of k_hilbert_odd), parking its outputs
+        // in the output field; each thread reads back only positions it wrote.
 #pragma unroll
-    for (int f = 0; f < V; ++f) {
-        const int i = f / R0, r = f % R0;
-        const int e = t + i * T + r * (L / R0);
-        double xr = 0.0, xi = 0.0;
-        if (e < NE) {
-            const c64 gm = load_gamma(a, base + e);
-            if (a.stash) gst[e] = gm;  // read back at the output, after the passes' barriers
-            const double we = a.mode == 3 ? 1.0 : a.wE[e];  // the adjoint weights the output
-            xr = we * gm.x;
-            xi = we * gm.y;
+        for (int f = 0; f < V; ++f) {
+            const int i = f / R0, r = f % R0;
+            const int n = t + i * T + r * (L / R0);
+            double xr = 0.0, xi = 0.0;
+            if (n < NE) {
+                const c64 gm = load_gamma(a, base + n);
+                gst[n] = gm;
+                const double we = a.wE[n];
+                const c64 ph = a.phase[n];
+                const double gr = we * gm.x, gi = we * gm.y;
+                xr = gr * ph.x - gi * ph.y;
+                xi = gr * ph.y + gi * ph.x;
+            }
+            re[f] = xr;
+            im[f] = xi;
         }
-        re[f] = xr;
-        im[f] = xi;
+        StockhamPasses<L, 0, false>::run(re, im, lds, t, a.tw, a.twstride);
+#pragma unroll
+        for (int f = 0; f < V; ++f) {
+            const int i = f / RL, r = f % RL;
+            const c64 k = a.kodd[t + i * T + r * (L / RL)];
+            const double xr = re[f], xi = im[f];
+            re[f] = k.x * xr - k.y * xi;
+            im[f] = k.x * xi + k.y * xr;
+        }
+        StockhamPasses<L, 0, true>::run(re, im, lds, t, a.tw, a.twstride);
+#pragma unroll
+        for (int f = 0; f < V; ++f) {
+            const int i = f / R0, r = f % R0;
+            const int n = t + i * T + r * (L / R0);
+            const int e = n - (NE - 1);
+            if (e < 0 || e >= NE) continue;
+            const c64 ph = a.phase[n];
+            const double yr = re[f] * invL, yi = im[f] * invL;
+            a.out[base + e] = cmk(yr * ph.x + yi * ph.y, yi * ph.x - yr * ph.y);
+        }
+        // The main transform's input from the stash (written before the
+        // passes' barriers).
+#pragma unroll
+        for (int f = 0; f < V; ++f) {
+            const int i = f / R0, r = f % R0;
+            const int e = t + i * T + r * (L / R0);
+            double xr = 0.0, xi = 0.0;
+            if (e < NE) {
+                const c64 gm = gst[e];
+                const double we = a.wE[e];
+                xr = we * gm.x;
+                xi = we * gm.y;
+            }
+            re[f] = xr;
+            im[f] = xi;
+        }
+    } else {
+        // Load wE*Gamma, zero padded, in the first pass's input order.
+#pragma unroll
+        for (int f = 0; f < V; ++f) {
+            const int i = f / R0, r = f % R0;
+            const int e = t + i * T + r * (L / R0);
+            double xr = 0.0, xi = 0.0;
+            if (e < NE) {
+                const c64 gm = load_gamma(a, base + e);
+                if (a.stash) gst[e] = gm;  // read back at the output, after the passes' barriers
+                const double we = a.mode == 3 ? 1.0 : a.wE[e];  // the adjoint weights the output
+                xr = we * gm.x;
+                xi = we * gm.y;
+            }
+            re[f] = xr;
+            im[f] = xi;
+        }
     }
     StockhamPasses<L, 0, false>::run(re, im, lds, t, a.tw, a.twstride);
     // Kernel spectrum, at the forward outputs j + r L/RL.
@@ -363,7 +423,7 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
         const c64 lam = cmk(re[f] * invL, im[f] * invL);
         const size_t gi = base + e;
         if (witness) {
-            const c64 od = a.out[gi];  // parked by k_hilbert_odd
+            const c64 od = a.out[gi];  // parked by k_hilbert_odd or above
             const double dx = 0.5 * (lam.x + od.x) - lam.x, dy = 0.5 * (lam.y + od.y) - lam.y;
             dn += dx * dx + dy * dy;
             nn += lam.x * lam.x + lam.y * lam.y;
@@ -488,16 +548,27 @@ int launch_hilbert_len(qclhip_ctx* ctx, long long ncols, const HilbertArgs& a) {
     const size_t lds_main = lds + (size_t)a.NE * sizeof(c64);
     b.stash = a.mode == 1 && lds_main <= 80 * 1024 && lds_main <= ctx->max_lds_per_block;
     const size_t lds_k = b.stash ? lds_main : lds;
+    // At L = 4096 (8 values per thread) the fused kernel needs no more
+    // registers than the plain one; at 16 values per thread it would cost occupancy.
+    if constexpr (L == 4096) {
+        if (a.kodd && b.stash) {  // one kernel runs the odd half and the transform
+            if (lds_k > 65536)
+                QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_hilbert<L, true>,
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_k));
+            hipLaunchKernelGGL((k_hilbert<L, true>), dim3((unsigned)ncols), dim3(T), lds_k, ctx->stream, b);
+            return launch_check(ctx, "k_hilbert");
+        }
+    }
     if (lds_k > 65536)
-        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_hilbert<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                (int)lds_k));
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_hilbert<L, false>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_k));
     if (a.kodd) {
         if (lds > 65536)
             QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_hilbert_odd<L>,
                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_hilbert_odd<L>, dim3((unsigned)ncols), dim3(T), lds, ctx->stream, a);
     }
-    hipLaunchKernelGGL(k_hilbert<L>, dim3((unsigned)ncols), dim3(T), lds_k, ctx->stream, b);
+    hipLaunchKernelGGL((k_hilbert<L, false>), dim3((unsigned)ncols), dim3(T), lds_k, ctx->stream, b);
     return launch_check(ctx, "k_hilbert");
 }
 

@@ -210,8 +210,9 @@ def test_fft_hilbert_gpu(hip, orc, NE):
     SL, SG = _random_fields(rng, NE, Nk, Nb, 2, scale=1.0)
     SRo, _, ro = orc.retarded_self_energy_fft(SL, SG, eps, wE)
     SR, _, r2 = hip.retarded_self_energy_fft(plan, hip.to_device(SL), hip.to_device(SG), return_roundoff=True)
-    err = float(rel(hip.to_host(SR), SRo) < TOL and abs(r2 - ro))
-    assert err < 1e-13
+    err = float(rel(hip.to_host(SR), SRo))
+    assert err < TOL
+    assert abs(r2 - ro) < 1e-13
 
 
 def test_embedding_gpu(hip, orc):
