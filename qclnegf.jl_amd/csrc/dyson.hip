@@ -792,7 +792,10 @@ constexpr int tile_waves(size_t per_wave) {
 
 template <int B>
 int launch_dyson(qclhip_ctx* ctx, const DKArgs& a) {
-    constexpr int W = kDysonTileWaves;
+    // 8 energies per workgroup (128-byte segments: the tile access pattern
+    // moves 4.2 TB/s there against 2.7 TB/s with 64-byte segments) while two
+    // workgroups still fit in the LDS of a CU; else kDysonTileWaves.
+    constexpr int W = DyShape<B, 8>::per_wave * 8 <= 80 * 1024 ? 8 : kDysonTileWaves;
     const size_t lds = DyShape<B, W>::bytes;
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
