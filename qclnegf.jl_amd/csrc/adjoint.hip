@@ -365,7 +365,9 @@ int shift_transpose(qclhip_ctx* ctx, const qclhip_shift* s, long long ncols, con
 template <int B>
 int launch_green_adjoint(qclhip_ctx* ctx, const GAArgs& a) {
     constexpr size_t per_wave = 4 * (size_t)TileStride<B>::value * sizeof(c64);
-    constexpr int W = per_wave * 2 <= 80 * 1024 ? 2 : 1;
+    // 4 energies (64-byte segments, one workgroup per CU) where they fit, else 2
+    // (two workgroups per CU) or 1: the same waves per CU either way.
+    constexpr int W = per_wave * 4 <= 160 * 1024 ? 4 : (per_wave * 2 <= 80 * 1024 ? 2 : 1);
     const size_t lds = per_wave * W;
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
