@@ -422,8 +422,8 @@ def main():
     achieved = gf["flops"] / k_s / 1e12
     mixed = best["precision"] == "fp32_products"
     peak = PEAK_FP32_MATRIX_TFLOPS if mixed else PEAK_FP64_TFLOPS
-    kernels = {"fp64_vector": "k_dyson<3,4> + k_keldysh<3,4>",
-               "fp64_matrix": "k_dyson<3,4> + k_keldysh_mfma<double,3,4>",
+    kernels = {"fp64_vector": "k_dyson<3,8> + k_keldysh<3,4>",
+               "fp64_matrix": "k_dyson<3,8> + k_keldysh_mfma<double,3,4>",
                "fp32_refine_vector": "k_dyson_mixed<3,4> + k_dyson<3,4> (flagged tiles) + k_keldysh<3,4>",
                "fp32_refine_matrix": "k_dyson_mixed<3,4> + k_dyson<3,4> (flagged tiles) + "
                                      "k_keldysh_mfma<double,3,4>",
