@@ -46,11 +46,13 @@ def test_scalar_lorentzian_gpu(hip, orc):
     assert err < 2e-13
 
 
+@pytest.mark.parametrize("NE", [24, 21])
 @pytest.mark.parametrize("Nb", [1, 2, 3, 5, 8, 12, 16, 20, 24, 31, 32, 40, 48])
-def test_dyson_keldysh_dense_sigma(hip, orc, Nb):
-    """Dense (non-diagonal) self-energies at every tile size of the kernel."""
-    rng = np.random.default_rng(100 + Nb)
-    NE, Nk = 24, 3
+def test_dyson_keldysh_dense_sigma(hip, orc, Nb, NE):
+    """Dense (non-diagonal) self-energies at every tile size of the kernel;
+    NE = 21 leaves a partial last tile of 8 (Dyson) and 4 (Keldysh) energies."""
+    rng = np.random.default_rng(100 + Nb + NE)
+    Nk = 3
     w = fx.syn.gf_workload(N=Nb, NE=NE, Nk=Nk, seed=Nb)
     SR, SL, SG = fx.syn.diag_fields_host(w)
     dR, dL, dG = _random_fields(rng, NE, Nk, Nb, 3, scale=0.01)
@@ -170,16 +172,20 @@ def test_contractions_gpu(hip, orc, kind):
     assert err < TOL
 
 
-def test_lo_contraction_gpu(hip, orc):
+@pytest.mark.parametrize("lo_kind", ["dense", "low_rank"])
+def test_lo_contraction_gpu(hip, orc, lo_kind):
+    """Both LO paths: one contraction per component then the shifts (dense),
+    and the shifts on the reduced coefficients before the expansion (low rank)."""
     rng = np.random.default_rng(5)
-    prob = fx.make_problem(N=4, NE=64, Nk=3, lo_kind="dense")
+    prob = fx.make_problem(N=4, NE=64, Nk=3, lo_kind=lo_kind)
     mech = prob["mechanisms"][2]
     GL, GG = _random_fields(rng, 64, 3, 4, 2, scale=1.0)
     eps = prob["eps"]
-    ref = orc.lo_contraction(orc.make_kernel("dense", mech["a"]), GL, GG, prob["wk"], 0.9, prob["NLO"],
+    ko = orc.make_kernel(lo_kind, mech["a"], mech["b"], mech["c"], rank=mech["rank"])
+    ref = orc.lo_contraction(ko, GL, GG, prob["wk"], 0.9, prob["NLO"],
                              orc.shift_plan(eps, prob["hwLO"]), orc.shift_plan(eps, -prob["hwLO"]))
     g = _grid(hip, eps, prob["wE"], prob["wk"], 4)
-    kd = hip.ProductionKernel(g, "dense", mech["a"])
+    kd = hip.ProductionKernel(g, lo_kind, mech["a"], mech["b"], mech["c"], rank=mech["rank"])
     SL, SG = hip.production_lo_contraction(kd, hip.to_device(GL), hip.to_device(GG), 0.9, prob["NLO"],
                                            hip.EnergyShiftPlan(g, prob["hwLO"]),
                                            hip.EnergyShiftPlan(g, -prob["hwLO"]))
