@@ -146,6 +146,17 @@ int qclhip_dyson_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
                          double eta, double* GR, double* GL, double* GG, double* scale,
                          double* traces);
 
+/* The same launch pair on fields whose (a,b) planes lie `plane` elements
+ * apart (element (e,m,a,b) at e + NE m + plane (a + Nb b), plane >= NE*Nk):
+ * the layout of a Julia view into a larger array, as the reference's
+ * loops over G[e, m, a, b] accept (production_operators.jl:1337-1385,
+ * production_state.jl:14-48). A plane stride off a power of two avoids the
+ * HBM bank conflicts of NE*Nk = 2^k. FP64 vector engine only. */
+int qclhip_dyson_keldysh_strided(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t plane,
+                                 const double* h, const double* sigR, const double* sigL,
+                                 const double* sigG, double eta, double* GR, double* GL,
+                                 double* GG, double* scale, double* traces);
+
 /* ---- A2: _current_green_production (production_state.jl:1-74) ----------
  * total = fam[0] + fam[1] + ... (embedding first, then mechanisms in order;
  * production_operators.jl:1447-1473); Dyson; Keldysh; number/holes

@@ -57,6 +57,10 @@ struct DKArgs {
     c64* totR;
     c64* totL;
     c64* totG;
+    // Elements between the (a,b) planes of every field (element (e,m,a,b) at
+    // e + NE m + plane (a + N b)); 0: NE*Nk, the contiguous layout. Honoured
+    // by the FP64 vector Dyson and Keldysh kernels.
+    long long plane;
 };
 
 // out[r][c] (lane block) = Gs[rows] * Xs[:, cols]  or  Xs[rows] * Gs[cols]^dagger

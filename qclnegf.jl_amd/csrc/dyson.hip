@@ -60,7 +60,7 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
     int* pivrow = reinterpret_cast<int*>(rowbuf + NP);
     int* steprow = pivrow + NP;
     const int N = p.N;
-    const size_t S = (size_t)nblk;
+    const size_t S = p.plane ? (size_t)p.plane : (size_t)nblk;
     const int lr = lane & 7, lc = lane >> 3;
     const int e0 = tc.e0, m = tc.m, nvalid = tc.nvalid;
     const long long blk0 = tc.blk0;
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) 
     c64* Gs = G0 + (size_t)wave * wstride;
     c64* Xs = X0 + (size_t)wave * wstride;
     const int N = p.N;
-    const size_t S = (size_t)nblk;
+    const size_t S = p.plane ? (size_t)p.plane : (size_t)nblk;
     const int lr = lane & 7, lc = lane >> 3;
     (void)NP;
     // Persistent loop: G^R and S< of the next tile, and S> of the current
@@ -978,9 +978,9 @@ extern "C" {
 
 int qclhip_max_basis(void) { return 48; }
 
-int qclhip_dyson_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* sigR,
-                         const double* sigL, const double* sigG, double eta, double* GR, double* GL,
-                         double* GG, double* scale, double* traces) {
+static int dyson_keldysh_impl(qclhip_ctx* ctx, const qclhip_grid* grid, long long plane, const double* h,
+                              const double* sigR, const double* sigL, const double* sigG, double eta, double* GR,
+                              double* GL, double* GG, double* scale, double* traces) {
     if (!ctx || !grid || !h || !GR) return QCLHIP_ERR_ARGUMENT;
     const bool keldysh = sigL || sigG || GL || GG;
     if (keldysh && !(sigL && sigG && GL && GG))
@@ -1009,7 +1009,29 @@ int qclhip_dyson_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double*
     a.traces = (c64*)traces;
     a.status = ctx->status;
     a.keldysh = keldysh ? 1 : 0;
-    return dyson_keldysh_dispatch(ctx, grid, a, keldysh);
+    if (plane == 0) return dyson_keldysh_dispatch(ctx, grid, a, keldysh);
+    if (grid->precision != QCLHIP_PRECISION_FP64 || grid->keldysh_engine == QCLHIP_ENGINE_MATRIX)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "the plane-strided layout is implemented by the FP64 vector engine");
+    if (a.N > 48) return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
+    a.plane = plane;
+    int rc = launch_dyson_fp64(ctx, a);
+    if (rc || !keldysh) return rc;
+    return launch_keldysh_valu(ctx, a);
+}
+
+int qclhip_dyson_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* sigR,
+                         const double* sigL, const double* sigG, double eta, double* GR, double* GL,
+                         double* GG, double* scale, double* traces) {
+    return dyson_keldysh_impl(ctx, grid, 0, h, sigR, sigL, sigG, eta, GR, GL, GG, scale, traces);
+}
+
+int qclhip_dyson_keldysh_strided(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t plane, const double* h,
+                                 const double* sigR, const double* sigL, const double* sigG, double eta,
+                                 double* GR, double* GL, double* GG, double* scale, double* traces) {
+    if (!ctx || !grid) return QCLHIP_ERR_ARGUMENT;
+    if (plane < (int64_t)grid->NE * grid->Nk)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "the plane stride must be at least NE*Nk");
+    return dyson_keldysh_impl(ctx, grid, (long long)plane, h, sigR, sigL, sigG, eta, GR, GL, GG, scale, traces);
 }
 
 int qclhip_retarded_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* sigR,

@@ -75,6 +75,8 @@ _SIGNATURES = {
     "qclhip_grid_autotune": [_vp, _vp, _dp],
     "qclhip_retarded_green": [_vp, _vp, _vp, _vp, ct.c_double, _vp, _vp],
     "qclhip_dyson_keldysh": [_vp, _vp, _vp, _vp, _vp, _vp, ct.c_double, _vp, _vp, _vp, _vp, _vp],
+    "qclhip_dyson_keldysh_strided": [_vp, _vp, ct.c_int64, _vp, _vp, _vp, _vp, ct.c_double, _vp, _vp, _vp, _vp,
+                                     _vp],
     "qclhip_current_green": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),
                              ct.c_double, _vp, _vp, _vp, _vp, _vp, _dp],
     "qclhip_current_green_totals": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),

@@ -383,6 +383,29 @@ def dyson_keldysh(grid: Grid, h, sigR, sigL, sigG, GR=None, GL=None, GG=None, sc
     return GR, GL, GG
 
 
+def strided_field(grid: Grid, plane: int):
+    """A zeroed field whose (a,b) planes lie `plane` elements apart: the
+    (Nb, Nb, Nk, NE) view of (Nb*Nb, plane) device storage."""
+    if plane < grid.NE * grid.Nk:
+        raise _lib.ArgumentError("the plane stride must be at least NE*Nk")
+    store = torch.zeros((grid.Nb * grid.Nb, plane), dtype=torch.complex128, device=grid.ctx.device)
+    return store[:, :grid.NE * grid.Nk].view(field_shape(grid.NE, grid.Nk, grid.Nb))
+
+
+def dyson_keldysh_strided(grid: Grid, plane: int, h, sigR, sigL, sigG, GR, GL, GG, scale=None, traces=None):
+    """The Dyson + Keldysh launch pair of dyson_keldysh on plane-strided fields
+    (views from strided_field with the same plane)."""
+    want = (grid.Nb * plane, plane, grid.NE, 1)
+    for t, n in ((sigR, "sigmaR"), (sigL, "sigmaL"), (sigG, "sigmaG"), (GR, "GR"), (GL, "GL"), (GG, "GG")):
+        if (t.dtype != torch.complex128 or t.device != grid.ctx.device
+                or tuple(t.shape) != field_shape(grid.NE, grid.Nk, grid.Nb) or t.stride() != want):
+            raise _lib.DimensionMismatch(f"{n}: expected a complex128 field with strides {want}")
+    grid.ctx.call("qclhip_dyson_keldysh_strided", grid.handle, ct.c_int64(plane), _ptr(h), _ptr(sigR),
+                  _ptr(sigL), _ptr(sigG), ct.c_double(0.0), _ptr(GR), _ptr(GL), _ptr(GG), _ptr(scale),
+                  _ptr(traces))
+    return GR, GL, GG
+
+
 @dataclass
 class GreenState:
     """GreenState (types.jl:733-740) with device fields."""

@@ -80,6 +80,41 @@ def test_dyson_keldysh_dense_sigma(hip, orc, Nb, NE):
     assert err < TOL
 
 
+@pytest.mark.parametrize("Nb", [5, 24])
+def test_dyson_keldysh_strided_equals_contiguous(hip, Nb):
+    """The launch pair on fields with padded planes gives the contiguous
+    results bit for bit and leaves the padding untouched; NE = 21 leaves
+    partial last tiles."""
+    import torch
+    rng = np.random.default_rng(40 + Nb)
+    NE, Nk = 21, 3
+    w = fx.syn.gf_workload(N=Nb, NE=NE, Nk=Nk, seed=Nb)
+    SR, SL, SG = fx.syn.diag_fields_host(w)
+    dR, dL, dG = _random_fields(rng, NE, Nk, Nb, 3, scale=0.01)
+    sig = [hip.to_device(np.asfortranarray(x + d)) for x, d in ((SR, dR), (SL, dL), (SG, dG))]
+    g = _grid(hip, w["eps"], w["wE"], w["wk"], Nb)
+    h = hip.h_to_device(w["h"])
+    GR, GL, GG = hip.dyson_keldysh(g, h, *sig, check_status=True)
+    plane = NE * Nk + 37
+    fields = [hip.strided_field(g, plane) for _ in range(6)]
+    for f, x in zip(fields[:3], sig):
+        f.copy_(x)
+    for f in fields[3:]:
+        f._base[:, NE * Nk:] = 1.5  # padding the kernels must not touch
+    hip.dyson_keldysh_strided(g, plane, h, *fields)
+    g.ctx.synchronize()
+    for got, ref in zip(fields[3:], (GR, GL, GG)):
+        assert torch.equal(got, ref)
+        assert bool((got._base[:, NE * Nk:] == 1.5).all())
+    import ctypes
+    from qclhip import ArgumentError
+    with pytest.raises(ArgumentError):
+        hip.strided_field(g, NE * Nk - 1)
+    with pytest.raises(ArgumentError):  # the library's own check
+        g.ctx.call("qclhip_dyson_keldysh_strided", g.handle, ctypes.c_int64(NE * Nk - 1), *[None] * 4,
+                   ctypes.c_double(0.0), *[None] * 5)
+
+
 def test_retarded_green_scale_and_seed_eta(hip, orc):
     w = fx.syn.gf_workload(N=6, NE=32, Nk=2, seed=4)
     g = _grid(hip, w["eps"], w["wE"], w["wk"], 6)
