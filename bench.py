@@ -117,6 +117,30 @@ def gf_bench(steps, warmup, variants=VARIANTS, autotune=True):
                          fp64_fallback_fraction=fallback)
         del st, g
         torch.cuda.empty_cache()
+    # The same FP64 vector launch pair on fields whose planes carry a 256-byte
+    # gap (plane stride NE*Nk + 16 elements, off the power of two of this size).
+    padded = None
+    if "fp64_vector" in out:
+        g = api.Grid(w["eps"], w["wE"], w["wk"], N)
+        pad = 16
+        plane = nblk + pad
+        fields = [api.strided_field(g, plane) for _ in range(6)]
+        for f, x in zip(fields[:3], (SR, SL, SG)):
+            f.copy_(x)
+        traces = torch.empty(4 * nblk, dtype=torch.float64, device="cuda")
+        api.dyson_keldysh_strided(g, plane, h, *fields, traces=traces)
+        reps = max(5, steps)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            api.dyson_keldysh_strided(g, plane, h, *fields, traces=traces)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        g.ctx.synchronize()
+        padded = dict(plane_pad_bytes=pad * 16, kernel_ms=e0.elapsed_time(e1) / reps)
+        log(f"  padded planes: Dyson+Keldysh {padded['kernel_ms']:.3f} ms")
+        del fields, traces, g
+        torch.cuda.empty_cache()
     tune = api.Grid(w["eps"], w["wE"], w["wk"], N).autotune() if autotune else None
     log(f"  autotune: {tune}")
     ref = out["fp64_vector"]["J"] if "fp64_vector" in out else None
@@ -124,7 +148,8 @@ def gf_bench(steps, warmup, variants=VARIANTS, autotune=True):
         v["current_rel_err"] = abs(v["J"] - ref) / abs(ref) if ref else None
     flops = nblk * 40.0 * N ** 3          # 8N^3 inverse + 2 x 16N^3 Keldysh
     bytes_ = nblk * 16.0 * N * N * 6      # read S^R, S<, S>; write G^R, G<, G>
-    return dict(N=N, NE=NE, Nk=Nk, nblk=nblk, flops=flops, bytes=bytes_, variants=out, autotune=tune)
+    return dict(N=N, NE=NE, Nk=Nk, nblk=nblk, flops=flops, bytes=bytes_, variants=out, autotune=tune,
+                padded=padded)
 
 
 def scba_problem(N=24, NE=2048, Nk=64):
@@ -458,6 +483,14 @@ def main():
         "variants": {k: {kk: vv for kk, vv in v.items() if kk != "precision"} for k, v in gf["variants"].items()},
         "autotune": gf["autotune"],
     }
+    if gf.get("padded"):
+        pm = gf["padded"]["kernel_ms"]
+        line["pair_padded_planes"] = {
+            "layout": "planes NE*Nk + 16 elements apart (qclhip_dyson_keldysh_strided); value above: contiguous",
+            "kernel": kernels["fp64_vector"], "kernel_ms": pm,
+            "achieved_tflops": gf["flops"] / (pm * 1e-3) / 1e12,
+            "frac": gf["flops"] / (pm * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+            "speedup_vs_contiguous": gf["variants"]["fp64_vector"]["kernel_ms"] / pm}
     if sc is not None:
         line["scba_iter_per_s"] = 1.0 / sc["iter_s"]
         line["scba_ms_per_iter"] = sc["iter_s"] * 1e3
