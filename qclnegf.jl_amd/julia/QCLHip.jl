@@ -105,6 +105,31 @@ function retarded_green(g::Grid, h::ROCArray{ComplexF64,3}, Σᴿ; η::Real = 0.
 end
 
 """
+    dyson_keldysh_strided!(g, h, Σᴿ, Σˡ, Σᵍ, Gᴿ, Gˡ, Gᵍ)
+
+The Dyson + Keldysh launch pair (raw G< and G> products) on fields stored as
+`plane × Nb²` matrices with `plane ≥ NE*Nk`: element `(e, m, a, b)` of a field
+is entry `(e + NE*(m-1), a + Nb*(b-1))` (one-based), the layout of a
+`reshape(view(F, 1:NE*Nk, :), NE, Nk, Nb, Nb)` into a larger array.
+"""
+function dyson_keldysh_strided!(g::Grid, h::ROCArray{ComplexF64,3}, Σᴿ::ROCArray{ComplexF64,2},
+                                Σˡ::ROCArray{ComplexF64,2}, Σᵍ::ROCArray{ComplexF64,2},
+                                Gᴿ::ROCArray{ComplexF64,2}, Gˡ::ROCArray{ComplexF64,2},
+                                Gᵍ::ROCArray{ComplexF64,2})
+    plane = size(Σᴿ, 1)
+    for F in (Σᴿ, Σˡ, Σᵍ, Gᴿ, Gˡ, Gᵍ)
+        size(F) == (plane, g.Nb * g.Nb) || throw(DimensionMismatch("fields must be plane × Nb² with one plane"))
+    end
+    rc = ccall((:qclhip_dyson_keldysh_strided, libqclhip), Cint,
+               (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble,
+                Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+               g.ctx.handle, g.handle, plane, _ptr(h), _ptr(Σᴿ), _ptr(Σˡ), _ptr(Σᵍ), 0.0,
+               _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), C_NULL, C_NULL)
+    check(g.ctx, rc, "dyson_keldysh_strided")
+    return Gᴿ, Gˡ, Gᵍ
+end
+
+"""
     current_green!(g, h, families, target, Gᴿ, Gˡ, Gᵍ, A, scale)
 
 `_current_green_production`: `families` is a vector of `(Σᴿ, Σˡ, Σᵍ)` device
