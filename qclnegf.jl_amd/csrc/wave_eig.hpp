@@ -74,16 +74,25 @@ __device__ inline double wave_min(double v) {
     return wave_reduce(v, [](double a, double b) { return fmin(a, b); });
 }
 
-// Squared Frobenius norm of a tile (optionally of tile - c I).
+// Squared Frobenius norm of a tile (optionally of tile - c I); each lane
+// sums the positions of its register block on the 8 x 8 lane grid.
 template <int LD>
 __device__ inline double tile_frob2(const c64* T, int N, int lane, double cdiag) {
+    static_assert((LD - 1) % 8 == 0, "tiles of the lane grid have LD = 8B + 1");
+    constexpr int B = (LD - 1) / 8;
+    const int lr = lane & 7, lc = lane >> 3;
     double s = 0.0;
-    for (int f = lane; f < N * N; f += 64) {
-        const int i = f / N, j = f % N;
-        c64 v = T[i * LD + j];
-        if (i == j) v.x -= cdiag;
-        s += v.x * v.x + v.y * v.y;
-    }
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr * B + r, j = lc * B + c;
+            if (i < N && j < N) {
+                c64 v = T[i * LD + j];
+                if (i == j) v.x -= cdiag;
+                s += v.x * v.x + v.y * v.y;
+            }
+        }
     return wave_sum(s);
 }
 
