@@ -107,13 +107,20 @@ __global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
     if (valid) {
         caus = causality_from_tile<B, LD>(X, N, lane, ew, T);
         wave_sync();
-        // r_D: D = eps - h - S^R in X, ||D G - I|| / (||D|| ||G|| + sqrt(N)).
-        for (int f = lane; f < N * N; f += 64) {
-            const int i = f / N, j = f % N;
-            const c64 hv = p.h[(size_t)m + (size_t)p.Nk * (i + (size_t)N * j)];
-            const c64 sv = X[i * LD + j];
-            X[i * LD + j] = cmk((i == j ? p.eps[e] : 0.0) - hv.x - sv.x, -hv.y - sv.y);
-        }
+        // r_D: D = eps - h - S^R in X, ||D G - I|| / (||D|| ||G|| + sqrt(N));
+        // each lane rewrites its register-block positions (all h loads in flight).
+        const double ee = p.eps[e];
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int i = lr * B + r, j = lc * B + c;
+                if (i < N && j < N) {
+                    const c64 hv = p.h[(size_t)m + (size_t)p.Nk * (i + (size_t)N * j)];
+                    const c64 sv = X[i * LD + j];
+                    X[i * LD + j] = cmk((i == j ? ee : 0.0) - hv.x - sv.x, -hv.y - sv.y);
+                }
+            }
         wave_sync();
         rprod<B>(X, G, N, lane, T, false);
         double dn = 0.0;
