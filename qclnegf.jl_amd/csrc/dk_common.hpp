@@ -61,6 +61,9 @@ struct DKArgs {
     // e + NE m + plane (a + N b)); 0: NE*Nk, the contiguous layout. Honoured
     // by the FP64 vector Dyson and Keldysh kernels.
     long long plane;
+    // Optional: A = i (G^R - G^R^dagger), written by the FP64 vector Keldysh
+    // kernel from its G^R tile.
+    c64* A;
 };
 
 // out[r][c] (lane block) = Gs[rows] * Xs[:, cols]  or  Xs[rows] * Gs[cols]^dagger
@@ -145,6 +148,31 @@ __device__ inline void tile_store(const c64* lds, int wstride, c64* dst, long lo
             const size_t idx = blk0 + w + S * (i + (size_t)N * j);
             const c64 v = lds[w * wstride + i * LD + j];
             dst[idx] = ACC ? cadd(dst[idx], v) : v;
+        }
+        i += 64;
+        while (i >= N) {
+            i -= N;
+            ++j;
+        }
+    }
+}
+
+// A = i (G - G^dagger) of the W per-wave G tiles, stored with the thread
+// mapping of tile_store (the arithmetic of k_spectral).
+template <int W, int LD>
+__device__ inline void tile_store_spectral(const c64* lds, int wstride, c64* dst, long long blk0, unsigned wave_mask,
+                                           int N, size_t S) {
+    const int total = N * N * W;
+    int f = threadIdx.x;
+    const int w = f % W;
+    int ij = f / W;
+    int i = ij % N, j = ij / N;
+    const bool on = (wave_mask >> w) & 1u;
+    for (; f < total; f += 64 * W) {
+        if (on) {
+            const c64 g = lds[w * wstride + i * LD + j], gt = lds[w * wstride + j * LD + i];
+            const double dx = g.x - gt.x, dy = g.y + gt.y;  // G - conj(G^T)
+            dst[blk0 + w + S * (i + (size_t)N * j)] = cmk(-dy, dx);
         }
         i += 64;
         while (i >= N) {

@@ -369,6 +369,7 @@ __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) 
         }
         __syncthreads();
         if constexpr (kPrefetch) tm.fetch(rS, p.famG, p.nfamG, blk0, nvalid, S);
+        if (p.A) tile_store_spectral<W, LD>(G0, wstride, p.A, blk0, mask, N, S);  // G0 stays read-only
         for (int comp = 0; comp < 2; ++comp) {
             c64* dst = comp ? p.GG : p.GL;
             if (comp == 1) {
@@ -1124,13 +1125,17 @@ static int current_green_impl(qclhip_ctx* ctx, const qclhip_grid* grid, const do
             if (rc0) return rc0;
         }
     }
+    // A from the Keldysh kernel's G^R tiles on the FP64 vector path, else k_spectral.
+    const bool a_in_kernel = A && grid->precision == QCLHIP_PRECISION_FP64 &&
+                             grid->keldysh_engine == QCLHIP_ENGINE_VECTOR && a.N <= 48;
+    if (a_in_kernel) a.A = (c64*)A;
     int rc = dyson_keldysh_dispatch(ctx, grid, a, true);
     if (rc) return rc;
     hipLaunchKernelGGL(k_number_pair, dim3(1), dim3(kReduceThreads), 0, ctx->stream, traces, 2, grid->NE,
                        grid->Nk, grid->wE, grid->wk, grid->gs, target, 1, realness_limit(grid), ctx->coef, ctx->status);
     hipLaunchKernelGGL(k_normalize_pair, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, n,
                        (c64*)GL, (c64*)GG, (const PairCoefficients*)ctx->coef, 0.0, 0.0);
-    if (A)
+    if (A && !a_in_kernel)
         hipLaunchKernelGGL(k_spectral, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, nblk,
                            grid->Nb, (const c64*)GR, (c64*)A);
     rc = launch_check(ctx, "current_green");
