@@ -80,11 +80,12 @@ def gf_bench(steps, warmup, variants=VARIANTS, autotune=True):
         name = f"{precision}_{engine}" if precision != "fp32_products" else precision
         log(f"  variant {name}")
         g = api.Grid(w["eps"], w["wE"], w["wk"], N).set_engine(engine).set_precision(precision)
-        st, info = api.current_green_production(g, h, fams, 1.0, spectral=False)
+        st, info = api.current_green_production(g, h, fams, 1.0, spectral=True)
         target = 0.9 * info["number"]  # exercises the paired map (a = 0.9)
 
         def step():
-            api.current_green_production(g, h, fams, target, state=st, spectral=False, return_scalars=False)
+            # the full output of _current_green_production: G^R, G<, G>, A
+            api.current_green_production(g, h, fams, target, state=st, spectral=True, return_scalars=False)
 
         for _ in range(warmup):
             step()
@@ -334,8 +335,9 @@ def batch_bench(reps=3, ninst=16, Nk=16, N_range=(12, 48), NE_range=(1024, 4096)
                 shapes=f"N in U{{{N_range[0]}..{N_range[1]}}}, NE in U{{{NE_range[0]}..{NE_range[1]}}}")
 
 
-def cpu_baseline(seconds=15.0):
-    """Oracle GF solves on a bounded sample (all NE energies, the first k momenta)."""
+def cpu_baseline(seconds=15.0, repeats=5):
+    """Oracle GF solves on a bounded sample (all NE energies, the first k
+    momenta): one warm-up run, then the median of `repeats` timed runs."""
     import numpy as np
 
     from oracle import oracle as orc
@@ -357,13 +359,17 @@ def cpu_baseline(seconds=15.0):
         orc.current_green(w["eps"], h, [PR, SR], [PL, SL], [PG, SG], w["wE"], w["wk"][:k], 2, 1e-3)
         return time.perf_counter() - t0
 
-    t1 = run(1)
-    k = int(max(1, min(w["Nk"], round(seconds / max(t1, 1e-3)))))
-    tk = run(k) if k > 1 else t1
+    t1 = run(1)  # warm-up (and the per-momentum cost)
+    k = int(max(1, min(w["Nk"], round(seconds / repeats / max(t1, 1e-3)))))
+    times = sorted(run(k) for _ in range(repeats))
+    tk = times[len(times) // 2]
     blocks = w["NE"] * k
     return dict(value=blocks / tk, unit="GF solves/s", cores=threads, kind="port",
-                sample=f"N=24 NE=2048, first {k} of 64 momenta ({blocks} blocks), {tk:.1f} s, "
-                       f"C++ oracle of the production Dyson/Keldysh/normalization, OpenMP {threads} threads")
+                sample=f"N=24 NE=2048, first {k} of 64 momenta ({blocks} blocks); median of {repeats} runs after "
+                       f"a warm-up ({tk:.2f} s; runs {', '.join(f'{t:.2f}' for t in times)} s); C++ oracle of the "
+                       f"production Dyson/Keldysh/normalization with A, OpenMP {threads} threads; the reference's "
+                       f"per-block cond(D) SVD (production_operators.jl:1380) is not computed, so the reference "
+                       f"CPU path would be slower")
 
 
 def main():

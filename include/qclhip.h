@@ -78,7 +78,10 @@ int qclhip_ctx_result_slot(const qclhip_ctx* ctx);
  * default stream) between begin and end becomes one executable graph;
  * launch replays it on the stream. No allocation may happen during capture:
  * run the captured sequence once eagerly first (it sizes the scratch), and
- * call synchronizing entries only in deferred-results mode. */
+ * call synchronizing entries only in deferred-results mode. A graph holds
+ * the context's scratch as it was at capture: launch returns
+ * QCLHIP_ERR_ARGUMENT when an eager call has since reallocated that scratch
+ * (capture again), or when the context is not the capturing one. */
 typedef struct qclhip_graph qclhip_graph;
 int qclhip_graph_begin(qclhip_ctx* ctx);
 int qclhip_graph_end(qclhip_ctx* ctx, qclhip_graph** out);
@@ -179,6 +182,18 @@ int qclhip_current_green_totals(qclhip_ctx* ctx, const qclhip_grid* grid, const 
                                 const double* const* famR, const double* const* famL, const double* const* famG,
                                 double target, double* GR, double* GL, double* GG, double* A, double* scale,
                                 double* out_host, double* totR, double* totL, double* totG);
+
+/* qclhip_current_green with occupation_normalization = :scalar_lesser, the
+ * reference's legacy comparator branch (production_state.jl:54-61; option
+ * accepted at algorithm_options.jl:29,63): lambda = N[G<raw]/target must be
+ * finite and positive (DomainError otherwise), G< = G<raw / lambda, and
+ * G> = G^R - G^R^dagger + G< as _green_state_production computes it without
+ * a given G> (production_operators.jl:1387-1415); no paired map and no
+ * capacity gates. out_host = (lambda, number, 0, 1, 0). */
+int qclhip_current_green_scalar_lesser(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
+                                       const double* const* famR, const double* const* famL,
+                                       const double* const* famG, double target, double* GR, double* GL,
+                                       double* GG, double* A, double* scale, double* out_host);
 
 /* A3: number functional N[G] = Re[-i g_s/(2 pi) sum wE wk tr G] with the
  * realness gate (synchronizes). */

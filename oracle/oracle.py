@@ -173,6 +173,25 @@ def current_green(eps, h, famR, famL, famG, wE, wk, gs, target):
     return dict(GR=GR, GL=GL, GG=GG, A=A, scale=scale, lam=out3[0], number=out3[1], holes=out3[2])
 
 
+def current_green_scalar_lesser(eps, h, famR, famL, famG, wE, wk, gs, target):
+    """The occupation_normalization = :scalar_lesser branch of
+    _current_green_production (production_state.jl:54-61), composed from the
+    oracle's Dyson, Keldysh product and number functional: G< = G<raw/lambda,
+    G> = G^R - G^R^dagger + G< (production_operators.jl:1387-1415)."""
+    SR = sum(famR[1:], famR[0].copy())
+    SL = sum(famL[1:], famL[0].copy())
+    GR, scale = retarded_green(eps, h, np.asfortranarray(SR))
+    GL = keldysh(GR, np.asfortranarray(SL))
+    number = number_functional(GL, wE, wk, gs)
+    lam = number / target
+    if not (np.isfinite(lam) and lam > 0):
+        raise ValueError("kinetic eigenvalue lambda must be positive")
+    GL = np.asfortranarray(GL / lam)
+    A = spectral(GR)
+    GG = np.asfortranarray(GR - np.conj(np.transpose(GR, (0, 1, 3, 2))) + GL)
+    return dict(GR=GR, GL=GL, GG=GG, A=A, scale=scale, lam=lam, number=number)
+
+
 def shift_plan(eps, delta):
     NE = len(eps)
     left = np.zeros(NE, np.int32)

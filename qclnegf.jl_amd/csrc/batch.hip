@@ -123,7 +123,10 @@ int qclhip_batch_current_green(qclhip_ctx* ctx, qclhip_batch* b, const double* c
         return QCLHIP_ERR_ARGUMENT;
     if (nfam < 1 || nfam > kMaxFamilies)
         return fail(ctx, QCLHIP_ERR_ARGUMENT, "the batched solve takes 1..8 self-energy families per instance");
-    // The staging buffers may still feed the previous call's copies.
+    // The staging buffers may still feed the previous call's copies; the
+    // wait below cannot be captured.
+    if (ctx->capturing)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "the batched solve synchronizes and cannot run during graph capture");
     QCLHIP_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < b->ninst; ++i) {
         const qclhip_grid* g = b->grids[i];

@@ -148,6 +148,9 @@ struct qclhip_ctx {
     double* host_scalars = nullptr;                 // pinned, 64 doubles
     void* workspace = nullptr;                      // device scratch, grows on demand
     size_t workspace_bytes = 0;
+    // Incremented whenever the scratch is reallocated: a captured graph holds
+    // the scratch address of its capture and must not replay after a change.
+    unsigned long long workspace_gen = 0;
     int sm_count = 0;
     size_t max_lds_per_block = 65536;
     std::string last_error;

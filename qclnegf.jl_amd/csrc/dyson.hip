@@ -604,7 +604,10 @@ __device__ inline void number_pair_body(const c64* traces, int ncomp, int NE, in
     out.lambda = out.number / target;
     out.a = 1.0;
     out.c = 0.0;
-    if (pair) {
+    if (pair == 2) {
+        // :scalar_lesser (production_state.jl:54-61): only lambda > 0 is required.
+        if (!(isfinite(out.lambda) && out.lambda > 0)) report_status(st, QCLHIP_ERR_DOMAIN, 0);
+    } else if (pair) {
         if (!(isfinite(out.lambda) && out.lambda > 0)) report_status(st, QCLHIP_ERR_DOMAIN, 0);
         double number = out.number, holes = out.holes;
         if (!(isfinite(number) && number > 0) || !(isfinite(holes) && holes >= 0) ||
@@ -666,6 +669,17 @@ __global__ void k_normalize_pair(long long n, c64* __restrict__ GL, c64* __restr
         c64 l = GL[i], g = GG[i];
         GL[i] = cmk(a * l.x - c * g.x, a * l.y - c * g.y);
         GG[i] = cmk((1.0 - c) * g.x - (1.0 - a) * l.x, (1.0 - c) * g.y - (1.0 - a) * l.y);
+    }
+}
+
+// G< <- G< / lambda, the legacy scalar normalization (production_state.jl:54-61;
+// a complex divided by a real divides both parts).
+__global__ void k_scalar_lesser(long long n, c64* __restrict__ GL, const PairCoefficients* coef) {
+    const double lam = coef->lambda;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const c64 l = GL[i];
+        GL[i] = cmk(l.x / lam, l.y / lam);
     }
 }
 
@@ -1044,7 +1058,7 @@ int qclhip_retarded_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double
 static int current_green_impl(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
                               const double* const* famR, const double* const* famL, const double* const* famG,
                               double target, double* GR, double* GL, double* GG, double* A, double* scale,
-                              double* out_host, double* const* totals) {
+                              double* out_host, double* const* totals, bool scalar_lesser = false) {
     if (!ctx || !grid || !h || !famR || !famL || !famG || !GR || !GL || !GG) return QCLHIP_ERR_ARGUMENT;
     if (nfam < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "at least the embedding family is required");
     if (!(std::isfinite(target) && target > 0))
@@ -1126,16 +1140,29 @@ static int current_green_impl(qclhip_ctx* ctx, const qclhip_grid* grid, const do
         }
     }
     // A from the Keldysh kernel's G^R tiles on the FP64 vector path, else k_spectral.
-    const bool a_in_kernel = A && grid->precision == QCLHIP_PRECISION_FP64 &&
+    const bool a_in_kernel = A && !scalar_lesser && grid->precision == QCLHIP_PRECISION_FP64 &&
                              grid->keldysh_engine == QCLHIP_ENGINE_VECTOR && a.N <= 48;
     if (a_in_kernel) a.A = (c64*)A;
     int rc = dyson_keldysh_dispatch(ctx, grid, a, true);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_number_pair, dim3(1), dim3(kReduceThreads), 0, ctx->stream, traces, 2, grid->NE,
-                       grid->Nk, grid->wE, grid->wk, grid->gs, target, 1, realness_limit(grid), ctx->coef, ctx->status);
-    hipLaunchKernelGGL(k_normalize_pair, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, n,
-                       (c64*)GL, (c64*)GG, (const PairCoefficients*)ctx->coef, 0.0, 0.0);
-    if (A && !a_in_kernel)
+    if (scalar_lesser) {
+        // number of G<raw only, lambda = number/target > 0, G< /= lambda, then
+        // _green_state_production without a given G>: G> = G^R - G^R^dagger + G<.
+        hipLaunchKernelGGL(k_number_pair, dim3(1), dim3(kReduceThreads), 0, ctx->stream, traces, 1, grid->NE,
+                           grid->Nk, grid->wE, grid->wk, grid->gs, target, 2, realness_limit(grid), ctx->coef,
+                           ctx->status);
+        hipLaunchKernelGGL(k_scalar_lesser, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, n,
+                           (c64*)GL, (const PairCoefficients*)ctx->coef);
+        hipLaunchKernelGGL(k_green_state, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, nblk,
+                           grid->Nb, (const c64*)GR, (const c64*)GL, (c64*)GG, (c64*)A);
+    } else {
+        hipLaunchKernelGGL(k_number_pair, dim3(1), dim3(kReduceThreads), 0, ctx->stream, traces, 2, grid->NE,
+                           grid->Nk, grid->wE, grid->wk, grid->gs, target, 1, realness_limit(grid), ctx->coef,
+                           ctx->status);
+        hipLaunchKernelGGL(k_normalize_pair, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, n,
+                           (c64*)GL, (c64*)GG, (const PairCoefficients*)ctx->coef, 0.0, 0.0);
+    }
+    if (A && !a_in_kernel && !scalar_lesser)
         hipLaunchKernelGGL(k_spectral, dim3(grid_for(n, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, nblk,
                            grid->Nb, (const c64*)GR, (c64*)A);
     rc = launch_check(ctx, "current_green");
@@ -1173,6 +1200,14 @@ int qclhip_current_green_totals(qclhip_ctx* ctx, const qclhip_grid* grid, const 
     if (!totR || !totL || !totG) return QCLHIP_ERR_ARGUMENT;
     double* const tot[3] = {totR, totL, totG};
     return current_green_impl(ctx, grid, h, nfam, famR, famL, famG, target, GR, GL, GG, A, scale, out_host, tot);
+}
+
+int qclhip_current_green_scalar_lesser(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
+                                       const double* const* famR, const double* const* famL,
+                                       const double* const* famG, double target, double* GR, double* GL,
+                                       double* GG, double* A, double* scale, double* out_host) {
+    return current_green_impl(ctx, grid, h, nfam, famR, famL, famG, target, GR, GL, GG, A, scale, out_host,
+                              nullptr, true);
 }
 
 int qclhip_number_functional(qclhip_ctx* ctx, const qclhip_grid* grid, const double* G, double* value_host) {

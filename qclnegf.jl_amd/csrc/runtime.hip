@@ -29,6 +29,7 @@ void* workspace(qclhip_ctx* ctx, size_t bytes) {
         ctx->workspace_bytes = 0;
     }
     size_t want = std::max(bytes, (size_t)1 << 20);
+    ++ctx->workspace_gen;
     if (hipMalloc(&ctx->workspace, want) != hipSuccess) {
         ctx->workspace = nullptr;
         return nullptr;
@@ -173,6 +174,10 @@ int qclhip_ctx_result_slot(const qclhip_ctx* ctx) { return ctx ? ctx->defer_next
 struct qclhip_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // The capturing context and its scratch generation at capture: the graph's
+    // kernels hold that scratch address.
+    const qclhip_ctx* ctx = nullptr;
+    unsigned long long workspace_gen = 0;
 };
 
 int qclhip_graph_begin(qclhip_ctx* ctx) {
@@ -197,12 +202,19 @@ int qclhip_graph_end(qclhip_ctx* ctx, qclhip_graph** out) {
         delete g;
         return hip_fail(ctx, err, "graph capture");
     }
+    g->ctx = ctx;
+    g->workspace_gen = ctx->workspace_gen;
     *out = g;
     return QCLHIP_OK;
 }
 
 int qclhip_graph_launch(qclhip_ctx* ctx, const qclhip_graph* g) {
     if (!ctx || !g) return QCLHIP_ERR_ARGUMENT;
+    if (g->ctx != ctx) return fail(ctx, QCLHIP_ERR_ARGUMENT, "the graph was captured on another context");
+    if (g->workspace_gen != ctx->workspace_gen)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT,
+                    "the context's scratch was reallocated after the graph was captured (an eager call needed "
+                    "more); capture the graph again");
     QCLHIP_TRY_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
     return QCLHIP_OK;
 }

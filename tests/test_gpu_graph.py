@@ -125,3 +125,66 @@ def test_mixing_totals_equal_family_sums(hip):
             assert torch.equal(a, b), it
     for k in ("GR", "GL", "GG"):
         assert torch.equal(getattr(s_c.green, k), getattr(s_f.green, k)), k
+
+
+def test_replay_refused_after_scratch_grows(hip):
+    """A graph keeps the scratch address of its capture: after an eager call
+    on the same context reallocates the scratch (a Poisson solve needs far more
+    than a number functional), launch is refused with ArgumentError instead of
+    replaying into freed memory; a new capture works again."""
+    import ctypes as ct
+    import torch
+    from qclhip import _lib
+    ctx = hip.Context(0)  # its own scratch, sized only by this test
+    w = fx.syn.gf_workload(N=4, NE=32, Nk=2)
+    g = hip.Grid(w["eps"], w["wE"], w["wk"], 4, ctx=ctx)
+    SR, SL, SG = fx.syn.diag_fields_device(w)
+    st, _ = hip.current_green_production(g, hip.h_to_device(w["h"]), [(SR, SL, SG)], 1.0, spectral=False)
+    ref = hip.number_functional(g, st.GL)  # eager: sizes the scratch
+    side = torch.cuda.Stream()
+    base = torch.cuda.current_stream()
+    buf = torch.zeros(4, dtype=torch.float64, device="cuda")
+
+    def capture():
+        torch.cuda.synchronize()
+        ctx.set_stream(side)
+        try:
+            ctx.defer(buf)
+            ctx.graph_begin()
+            try:
+                hip.number_functional(g, st.GL)
+            finally:
+                graph = ctx.graph_end()
+                ctx.defer(None)
+        finally:
+            ctx.set_stream(base)
+        return graph
+
+    def replay(graph):
+        buf.zero_()
+        torch.cuda.synchronize()
+        ctx.set_stream(side)
+        try:
+            graph.launch()
+            ctx.synchronize()
+        finally:
+            ctx.set_stream(base)
+        return float(buf[0].item())
+
+    graph = capture()
+    assert replay(graph) == ref
+    # grow the scratch: the bordered Poisson system of N_z = 1024 needs ~8 MB
+    Nz = 1024
+    dev = lambda x: torch.as_tensor(x, device="cuda", dtype=torch.float64)
+    nbar = dev(np.full(Nz, 1.0))
+    U = torch.empty(Nz, dtype=torch.float64, device="cuda")
+    out = (ct.c_double * 3)()
+    args = (dev(np.full(Nz, 12.0)), dev(np.full(Nz, 0.1)), dev(np.full(Nz, 1.0)))
+    ctx.call("qclhip_periodic_poisson", Nz, *(ct.c_void_p(t.data_ptr()) for t in args), ct.c_double(1.0),
+             ct.c_void_p(nbar.data_ptr()), ct.c_void_p(U.data_ptr()), out)
+    with pytest.raises(_lib.ArgumentError, match="reallocated"):
+        replay(graph)
+    graph = capture()
+    assert replay(graph) == ref
+    del graph, g
+    ctx.close()

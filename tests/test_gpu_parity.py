@@ -158,6 +158,64 @@ def test_current_green_vs_oracle(hip, orc):
         assert abs(info[k] - ref[k]) <= TOL * abs(ref[k]), k
 
 
+@pytest.mark.parametrize("NE", [48, 45])
+@pytest.mark.parametrize("Nb", [5, 20, 31, 48])
+def test_current_green_spectral_shapes(hip, orc, Nb, NE):
+    """current_green's A (stored by the Keldysh kernel from its G^R tiles)
+    and the other outputs against the oracle on both Keldysh code paths
+    (register prefetch for N <= 24, none above) and with partial last tiles
+    (NE = 45); two dense families summed on load."""
+    rng = np.random.default_rng(700 + Nb + NE)
+    Nk = 2
+    w = fx.syn.gf_workload(N=Nb, NE=NE, Nk=Nk, seed=Nb + 3)
+    fams_h = []
+    for src in (fx.syn.injection_family(w), w):
+        R, L, G = fx.syn.diag_fields_host(src)
+        dR, dL, dG = _random_fields(rng, NE, Nk, Nb, 3, scale=0.002)
+        fams_h.append((np.asfortranarray(R + dR), np.asfortranarray(L + dL - np.conj(dL.transpose(0, 1, 3, 2))),
+                       np.asfortranarray(G + dG - np.conj(dG.transpose(0, 1, 3, 2)))))
+    probe = orc.current_green(w["eps"], w["h"], *zip(*fams_h), w["wE"], w["wk"], 2, 1.0)
+    for target in (0.9 * probe["number"], 1.05 * probe["number"]):  # both branches of the paired map
+        ref = orc.current_green(w["eps"], w["h"], *zip(*fams_h), w["wE"], w["wk"], 2, target)
+        g = _grid(hip, w["eps"], w["wE"], w["wk"], Nb)
+        fams = [tuple(hip.to_device(x) for x in f) for f in fams_h]
+        st, info = hip.current_green_production(g, hip.h_to_device(w["h"]), fams, target)
+        for k in ("GR", "GL", "GG", "A"):
+            err = float(rel(hip.to_host(getattr(st, k)), ref[k]))
+            assert err < TOL, (k, err)
+        A_sep = hip.to_host(hip.spectral_function(g, st.GR))
+        assert np.array_equal(hip.to_host(st.A), A_sep)  # the arithmetic of the spectral kernel
+        for k in ("lam", "number", "holes"):
+            assert abs(info[k] - ref[k]) <= TOL * abs(ref[k]), k
+
+
+@pytest.mark.parametrize("Nb", [5, 31])
+def test_current_green_scalar_lesser(hip, orc, Nb):
+    """occupation_normalization = :scalar_lesser (production_state.jl:54-61):
+    G< = G<raw/lambda, G> = G^R - G^A + G<, against the oracle's composition;
+    lambda <= 0 raises DomainError."""
+    w = fx.syn.gf_workload(N=Nb, NE=45, Nk=2, seed=Nb + 11)
+    fams_h = [fx.syn.diag_fields_host(fx.syn.injection_family(w)), fx.syn.diag_fields_host(w)]
+    probe = orc.current_green(w["eps"], w["h"], *zip(*fams_h), w["wE"], w["wk"], 2, 1.0)
+    target = 0.8 * probe["number"]
+    ref = orc.current_green_scalar_lesser(w["eps"], w["h"], *zip(*fams_h), w["wE"], w["wk"], 2, target)
+    g = _grid(hip, w["eps"], w["wE"], w["wk"], Nb)
+    fams = [tuple(hip.to_device(x) for x in f) for f in fams_h]
+    h = hip.h_to_device(w["h"])
+    st, info = hip.current_green_production(g, h, fams, target, occupation_normalization="scalar_lesser")
+    for k in ("GR", "GL", "GG", "A"):
+        err = float(rel(hip.to_host(getattr(st, k)), ref[k]))
+        assert err < TOL, (k, err)
+    assert abs(info["lam"] - ref["lam"]) <= TOL * ref["lam"]
+    assert abs(info["number"] - ref["number"]) <= TOL * abs(ref["number"])
+    from qclhip import DomainError, ArgumentError
+    neg = [tuple(hip.to_device(-x) if k else hip.to_device(x) for k, x in enumerate(f)) for f in fams_h]
+    with pytest.raises(DomainError):  # S< and S> negated: N[G<raw] < 0
+        hip.current_green_production(g, h, neg, target, occupation_normalization="scalar_lesser")
+    with pytest.raises(ArgumentError):
+        hip.current_green_production(g, h, fams, target, occupation_normalization="scalar")
+
+
 def test_number_functional_gpu(hip, orc):
     rng = np.random.default_rng(3)
     NE, Nk, Nb = 40, 3, 4
