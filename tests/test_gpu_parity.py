@@ -441,3 +441,38 @@ def test_gf_batch_full_size_sampled(hip, orc):
             assert err < TOL
             err = float(rel(GLd[:, :, m, e].T, GL_o[i, j]))
             assert err < TOL
+
+
+def test_current_equals_landauer_transmission_gpu(hip):
+    """The device current functional on the rotated three-site chain equals
+    the closed-form Landauer current (physics/cavity_current_and_spectral_
+    oracles.jl:5-36), independently of the oracle: Dyson, both Keldysh
+    products and the boundary flux all run on the device."""
+    from test_oracle_anchors import landauer_chain
+    for fLR, bound in ((None, None), ((0.4, 0.4), 3e-14)):
+        c = landauer_chain(fLR=fLR)
+        g = _grid(hip, c["eps"], c["wE"], c["wk"], 3)
+        GR, GL, GG = hip.dyson_keldysh(g, hip.h_to_device(c["h"]), *(hip.to_device(c[k]) for k in ("SR", "SL", "SG")))
+        J = hip.boundary_flux_bar(g, hip.to_device(c["plusL"]), hip.to_device(c["plusG"]), GL, GG)
+        if bound is None:
+            assert abs(J - c["J"]) <= 2e-13 * abs(c["J"]), (J, c["J"])
+        else:
+            assert abs(J) < bound
+
+
+@pytest.mark.parametrize("n", [481, 1921])
+def test_hilbert_sign_and_scale_dawson_gpu(hip, n):
+    """The device FFT Hilbert transform of Gaussian columns equals Dawson's
+    integral plus the trapezoid rule's omitted-node term (closed form, no
+    oracle), with the 2L round-off witness below 1e-12."""
+    from test_oracle_anchors import dawson_columns
+    x, w, G, exact = dawson_columns(n)
+    ncol = G.shape[2]
+    G = np.asfortranarray(G.reshape(n, ncol, 1, 1, order="F"))
+    exact = exact.reshape(n, ncol, 1, 1, order="F")
+    g = _grid(hip, x, w, np.ones(ncol), 1)
+    plan = hip.FFTHilbertPlan(g, ncol, verify_roundoff=True)
+    L, r = hip.production_fft_hilbert_transform(plan, hip.to_device(G), return_roundoff=True)
+    scale = np.max(np.abs(exact))
+    assert np.max(np.abs(hip.to_host(L) - exact)) < 1e-13 * scale
+    assert r < 1e-12

@@ -410,3 +410,106 @@ def test_two_l_transform_from_two_l_point_halves():
     # on the extracted range the L result itself equals the 2L one
     e = np.arange(NE) + NE - 1
     assert np.max(np.abs(y[e] - y2[e])) < 1e-13 * np.max(np.abs(y2))
+
+
+def landauer_chain(NE=41, Nk=3, seed=5, fLR=None):
+    """Three-site chain with leads on sites 1 and 3 (the closed form of
+    physics/cavity_current_and_spectral_oracles.jl:5-36), on an energy x
+    momentum grid, in a basis rotated by a random unitary U so that every
+    self-energy and Green block is dense. Returns the fields, the plus family
+    (the left lead alone) and the closed-form current
+    Jbar = g_s/(2 pi) sum_{e,m} wE wk (-T_em (fL_e - fR_e)),
+    T = gL gR |t1 t2 / det D|^2, det D = a b c - t1^2 c - t2^2 a with
+    a = E - l1 + i gL/2, b = E - l2, c = E - l3 + i gR/2 (levels shifted by
+    the momentum's kinetic energy). The per-energy term follows from G< and
+    G> = G^R S^{<,>} G^A: Re tr(S>_+ G< - S<_+ G>) = gL gR |G^R_13|^2 (fR - fL)."""
+    rng = np.random.default_rng(seed)
+    levels = np.array([0.1, -0.05, 0.2])
+    t1, t2, gL, gR = 0.25, 0.32, 0.07, 0.11
+    eps = np.linspace(-0.6, 0.7, NE)
+    wE = rng.uniform(0.5, 1.5, NE) * (eps[1] - eps[0])
+    kin = np.array([0.0, 0.03, 0.11])[:Nk]
+    wk = rng.uniform(0.2, 1.0, Nk)
+    if fLR is None:
+        fL = 1 / (np.exp((eps - 0.15) / 0.05) + 1)
+        fR = 1 / (np.exp((eps + 0.05) / 0.05) + 1)
+    else:
+        fL, fR = np.full(NE, fLR[0]), np.full(NE, fLR[1])
+    U, _ = np.linalg.qr(rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3)))
+    rot = lambda X: U @ X @ U.conj().T
+    h = np.zeros((Nk, 3, 3), np.complex128, order="F")
+    for m in range(Nk):
+        h[m] = rot(np.diag(levels + kin[m]) + np.diag([t1, t2], 1) + np.diag([t1, t2], -1))
+    SR, SL, SG, PL, PG = (np.zeros((NE, Nk, 3, 3), np.complex128, order="F") for _ in range(5))
+    J = 0.0
+    for e in range(NE):
+        sl = np.diag([1j * gL * fL[e], 0, 1j * gR * fR[e]])
+        sg = np.diag([-1j * gL * (1 - fL[e]), 0, -1j * gR * (1 - fR[e])])
+        pl = np.diag([1j * gL * fL[e], 0, 0])
+        pg = np.diag([-1j * gL * (1 - fL[e]), 0, 0])
+        for m in range(Nk):
+            SR[e, m] = rot(np.diag([-0.5j * gL, 0, -0.5j * gR]))
+            SL[e, m], SG[e, m], PL[e, m], PG[e, m] = rot(sl), rot(sg), rot(pl), rot(pg)
+            E = eps[e] - kin[m]
+            a, b, c = E - levels[0] + 0.5j * gL, E - levels[1], E - levels[2] + 0.5j * gR
+            T = gL * gR * abs(t1 * t2 / (a * b * c - t1 ** 2 * c - t2 ** 2 * a)) ** 2
+            assert 0 <= T <= 1 + 2e-14
+            J += wE[e] * wk[m] * (-T * (fL[e] - fR[e]))
+    return dict(eps=eps, wE=wE, wk=wk, h=h, SR=SR, SL=SL, SG=SG, plusL=PL, plusG=PG, J=2 * J / (2 * math.pi))
+
+
+def test_current_equals_landauer_transmission(orc):
+    # physics/cavity_current_and_spectral_oracles.jl:5-36 through the production
+    # current functional (production_iteration.jl:174-201) with g_s = 2: a sign
+    # flip, a lost g_s/(2 pi) or a transposed trace operand (the rotated blocks
+    # are dense) breaks the equality.
+    c = landauer_chain()
+    GR, _ = orc.retarded_green(c["eps"], c["h"], c["SR"])
+    GL, GG = orc.keldysh(GR, c["SL"]), orc.keldysh(GR, c["SG"])
+    J = orc.boundary_flux_bar(c["eps"], c["wE"], c["wk"], 2, c["plusL"], c["plusG"], GL, GG)
+    assert abs(c["J"]) > 1e-3
+    assert abs(J - c["J"]) <= 2e-13 * abs(c["J"])
+    # equal reservoir occupations: zero current to rounding
+    c0 = landauer_chain(fLR=(0.4, 0.4))
+    GR0, _ = orc.retarded_green(c0["eps"], c0["h"], c0["SR"])
+    J0 = orc.boundary_flux_bar(c0["eps"], c0["wE"], c0["wk"], 2, c0["plusL"], c0["plusG"],
+                               orc.keldysh(GR0, c0["SL"]), orc.keldysh(GR0, c0["SG"]))
+    assert abs(J0) < 3e-14
+
+
+def dawson_columns(n=961, half=12.0):
+    """Gaussian columns c e^{-((x-x0)/s)^2} on a uniform grid with trapezoid
+    weights, and the closed form of the discrete principal-value map
+    Lambda_i = sum_{j != i} w_j Gamma_j / (2 pi (x_i - x_j)):
+    (1/(2 pi)) PV int e^{-u^2}/(x - u) du = D(x)/sqrt(pi) (Dawson's integral,
+    independent of the width s), plus the omitted-node term h Gamma'(x_i)/(2 pi)
+    of the trapezoid rule on the regular part; the remaining error is spectral."""
+    from scipy.special import dawsn
+    x = np.linspace(-half, half, n)
+    hstep = x[1] - x[0]
+    w = np.full(n, hstep)
+    w[[0, -1]] /= 2
+    params = [(1.0, 0.0, 1.0), (0.5 - 0.25j, 0.7, 1.3), (-2.0j, -1.1, 0.8)]
+    G = np.zeros((n, 1, len(params), 1), np.complex128, order="F")
+    exact = np.zeros_like(G)
+    for k, (c, x0, s) in enumerate(params):
+        u = (x - x0) / s
+        G[:, 0, k, 0] = c * np.exp(-u ** 2)
+        exact[:, 0, k, 0] = c * (dawsn(u) / math.sqrt(math.pi) + hstep * (-2 * u / s) * np.exp(-u ** 2) / (2 * math.pi))
+    return x, w, G, exact
+
+
+@pytest.mark.parametrize("n", [481, 961, 1921])
+def test_hilbert_sign_and_scale_dawson(orc, n):
+    # Sign and scale of the Hilbert kernel 1/(2 pi (eps_e - eps_e')) of
+    # fft_hilbert_transform (production_operators.jl:735-809) and of the direct
+    # sum (reference/kernels.jl:467-510): a flipped sign or a lost 2 pi misses
+    # Dawson's integral by O(1) at every grid spacing.
+    x, w, G, exact = dawson_columns(n)
+    L, r = orc.fft_hilbert(G, x, w)
+    D = orc.direct_hilbert(G, x, w)
+    scale = np.max(np.abs(exact))
+    assert np.max(np.abs(L - exact)) < 1e-13 * scale
+    # the O(NE^2) direct sum accumulates rounding of order n eps
+    assert np.max(np.abs(D - exact)) < 1e-15 * n * scale
+    assert r < 1e-12
