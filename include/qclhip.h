@@ -183,6 +183,21 @@ int qclhip_current_green_totals(qclhip_ctx* ctx, const qclhip_grid* grid, const 
                                 double target, double* GR, double* GL, double* GG, double* A, double* scale,
                                 double* out_host, double* totR, double* totL, double* totG);
 
+/* qclhip_current_green that allocates nothing. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as qclhip_current_green (bitwise),
+ * except that the block traces live in a fixed 16 MiB device array of the
+ * library (one per device, held until the process ends) instead of the
+ * context's growable scratch: NE*Nk <= 524,288 blocks and nfam <= 8, else
+ * QCLHIP_ERR_ARGUMENT. Every call on one device shares that array, so calls
+ * that overlap on one device (two streams, two contexts, two threads) race;
+ * serialize them. Nothing in it needs to start at zero. */
+int qclhip_current_green_noalloc(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
+                                 const double* const* famR, const double* const* famL, const double* const* famG,
+                                 double target, double* GR, double* GL, double* GG, double* A, double* scale,
+                                 double* out_host);
+
 /* qclhip_current_green with occupation_normalization = :scalar_lesser, the
  * reference's legacy comparator branch (production_state.jl:54-61; option
  * accepted at algorithm_options.jl:29,63): lambda = N[G<raw]/target must be

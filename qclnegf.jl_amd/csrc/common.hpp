@@ -200,6 +200,13 @@ struct qclhip_shift {
 
 namespace qclhip {
 
+// Fixed device scratch of the allocation-free entry points (16 MiB per device).
+constexpr size_t kStaticScratchBytes = (size_t)16 << 20;
+// Looks up the device's copy of that scratch once, with `device` current.
+int static_scratch_setup(int device);
+// Its device address (nullptr before set-up).
+void* static_scratch(int device);
+
 int fail(qclhip_ctx* ctx, int code, const std::string& msg);
 int hip_fail(qclhip_ctx* ctx, hipError_t err, const char* what);
 // Device scratch of at least `bytes`, owned by the context.

@@ -19,6 +19,7 @@
 // same 8x8 lane grid, each lane accumulating its B x B block.
 #include <climits>
 #include <cstddef>
+#include <mutex>
 
 #include "dk_common.hpp"
 
@@ -987,6 +988,41 @@ int launch_pair_batch(qclhip_ctx* ctx, const BatchScalars* d, int ninst, long lo
 
 }  // namespace qclhip
 
+// ---- fixed scratch of the allocation-free current_green ------------------
+// current_green's only scratch (with at most kMaxFamilies families, summed in
+// the kernels' loads) is 2 NE Nk complex block traces, every one written by
+// the Keldysh kernel before the number functionals read it, so nothing here
+// needs clearing between calls. Each device holds its own copy of this array
+// for the life of the process; in host code its name is not a device address,
+// so the address is looked up per device (hipGetSymbolAddress) and only that
+// address is handed to kernels.
+static __device__ __align__(256) unsigned char scratch[qclhip::kStaticScratchBytes];
+
+namespace qclhip {
+namespace {
+constexpr int kMaxStaticDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxStaticDevices] = {};
+}  // namespace
+
+int static_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxStaticDevices) return QCLHIP_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return QCLHIP_OK;
+    void* ptr = nullptr;
+    // the calling context has made `device` current
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return QCLHIP_ERR_RUNTIME;
+    g_scratch_ptr[device] = ptr;
+    return QCLHIP_OK;
+}
+
+void* static_scratch(int device) {
+    if (device < 0 || device >= kMaxStaticDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+}  // namespace qclhip
+
 using namespace qclhip;
 
 extern "C" {
@@ -1058,7 +1094,8 @@ int qclhip_retarded_green(qclhip_ctx* ctx, const qclhip_grid* grid, const double
 static int current_green_impl(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
                               const double* const* famR, const double* const* famL, const double* const* famG,
                               double target, double* GR, double* GL, double* GG, double* A, double* scale,
-                              double* out_host, double* const* totals, bool scalar_lesser = false) {
+                              double* out_host, double* const* totals, bool scalar_lesser = false,
+                              c64* traces_ext = nullptr) {
     if (!ctx || !grid || !h || !famR || !famL || !famG || !GR || !GL || !GG) return QCLHIP_ERR_ARGUMENT;
     if (nfam < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "at least the embedding family is required");
     if (!(std::isfinite(target) && target > 0))
@@ -1069,9 +1106,15 @@ static int current_green_impl(qclhip_ctx* ctx, const qclhip_grid* grid, const do
     const bool fused = nfam <= kMaxFamilies;
     size_t tr_bytes = (size_t)2 * nblk * sizeof(c64);
     size_t tot_bytes = fused ? 0 : (size_t)3 * n * sizeof(c64);
-    char* ws = (char*)workspace(ctx, tr_bytes + tot_bytes);
-    if (!ws) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
-    c64* traces = (c64*)ws;
+    char* ws = nullptr;
+    if (traces_ext) {
+        // caller-supplied trace storage: nothing is allocated here
+        if (!fused) return fail(ctx, QCLHIP_ERR_ARGUMENT, "the allocation-free call sums at most 8 families");
+    } else {
+        ws = (char*)workspace(ctx, tr_bytes + tot_bytes);
+        if (!ws) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    }
+    c64* traces = traces_ext ? traces_ext : (c64*)ws;
     DKArgs a{};
     a.NE = grid->NE;
     a.Nk = grid->Nk;
@@ -1200,6 +1243,24 @@ int qclhip_current_green_totals(qclhip_ctx* ctx, const qclhip_grid* grid, const 
     if (!totR || !totL || !totG) return QCLHIP_ERR_ARGUMENT;
     double* const tot[3] = {totR, totL, totG};
     return current_green_impl(ctx, grid, h, nfam, famR, famL, famG, target, GR, GL, GG, A, scale, out_host, tot);
+}
+
+int qclhip_current_green_noalloc(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
+                                 const double* const* famR, const double* const* famL, const double* const* famG,
+                                 double target, double* GR, double* GL, double* GG, double* A, double* scale,
+                                 double* out_host) {
+    if (!ctx || !grid) return QCLHIP_ERR_ARGUMENT;
+    if (nfam > kMaxFamilies) return fail(ctx, QCLHIP_ERR_ARGUMENT, "the allocation-free call sums at most 8 families");
+    const size_t need = (size_t)2 * grid->NE * grid->Nk * sizeof(c64);
+    if (need > kStaticScratchBytes)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "the allocation-free call takes at most NE*Nk = " +
+                                                  std::to_string(kStaticScratchBytes / (2 * sizeof(c64))) +
+                                                  " blocks (" + std::to_string(grid->NE) + " x " +
+                                                  std::to_string(grid->Nk) + " given)");
+    c64* traces = (c64*)static_scratch(ctx->device);
+    if (!traces) return fail(ctx, QCLHIP_ERR_RUNTIME, "the fixed scratch of this device was not set up");
+    return current_green_impl(ctx, grid, h, nfam, famR, famL, famG, target, GR, GL, GG, A, scale, out_host,
+                              nullptr, false, traces);
 }
 
 int qclhip_current_green_scalar_lesser(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,

@@ -100,6 +100,9 @@ int qclhip_ctx_create(int device, void* stream, qclhip_ctx** out) {
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return QCLHIP_ERR_RUNTIME;
     if (device < 0 || device >= count) return QCLHIP_ERR_ARGUMENT;
     if (hipSetDevice(device) != hipSuccess) return QCLHIP_ERR_RUNTIME;
+    // the fixed scratch of the allocation-free entry points, once per device
+    int src = static_scratch_setup(device);
+    if (src != QCLHIP_OK) return src;
     auto* ctx = new qclhip_ctx();
     ctx->device = device;
     hipDeviceProp_t prop;

@@ -81,6 +81,8 @@ _SIGNATURES = {
                              ct.c_double, _vp, _vp, _vp, _vp, _vp, _dp],
     "qclhip_current_green_totals": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),
                                     ct.c_double, _vp, _vp, _vp, _vp, _vp, _dp, _vp, _vp, _vp],
+    "qclhip_current_green_noalloc": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),
+                                     ct.c_double, _vp, _vp, _vp, _vp, _vp, _dp],
     "qclhip_current_green_scalar_lesser": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),
                                            ct.POINTER(_vp), ct.c_double, _vp, _vp, _vp, _vp, _vp, _dp],
     "qclhip_number_functional": [_vp, _vp, _vp, _dp],

@@ -418,7 +418,7 @@ class GreenState:
 
 def current_green_production(grid: Grid, h, families, target: float, state: GreenState | None = None,
                              spectral: bool = True, return_scalars: bool = True, totals=None,
-                             occupation_normalization: str = "paired_convex"):
+                             occupation_normalization: str = "paired_convex", allocation_free: bool = False):
     """_current_green_production (production_state.jl:1-74).
 
     families: list of (SR, SL, SG) device triples, the embedding total first,
@@ -428,12 +428,16 @@ def current_green_production(grid: Grid, h, families, target: float, state: Gree
     totals: optional (SR, SL, SG) fields that receive the summed families.
     occupation_normalization: "paired_convex" (default) or "scalar_lesser",
     the reference's legacy branch G< = G<raw/lambda, G> = G^R - G^A + G<
-    (production_state.jl:54-61; the options of algorithm_options.jl:63)."""
+    (production_state.jl:54-61; the options of algorithm_options.jl:63).
+    allocation_free: use qclhip_current_green_noalloc (a fixed device scratch,
+    no device allocation; NE*Nk <= 524,288, at most 8 families)."""
     if occupation_normalization not in ("paired_convex", "scalar_lesser"):
         raise _lib.ArgumentError("occupation_normalization must be :paired_convex or :scalar_lesser")
     scalar = occupation_normalization == "scalar_lesser"
     if scalar and totals is not None:
         raise _lib.ArgumentError("totals are written by the paired_convex path only")
+    if allocation_free and (scalar or totals is not None):
+        raise _lib.ArgumentError("the allocation-free call is the plain paired_convex current_green")
     nf = len(families)
     for fam in families:
         for t in fam:
@@ -448,7 +452,9 @@ def current_green_production(grid: Grid, h, families, target: float, state: Gree
     common = (grid.handle, _ptr(h), nf, arr(0), arr(1), arr(2), ct.c_double(target), _ptr(state.GR),
               _ptr(state.GL), _ptr(state.GG), _ptr(state.A if spectral else None), _ptr(state.scale),
               out if return_scalars else None)
-    if scalar:
+    if allocation_free:
+        grid.ctx.call("qclhip_current_green_noalloc", *common)
+    elif scalar:
         grid.ctx.call("qclhip_current_green_scalar_lesser", *common)
     elif totals is None:
         grid.ctx.call("qclhip_current_green", *common)
