@@ -425,6 +425,42 @@ int qclhip_mix_fields_residual_totals(qclhip_ctx* ctx, int64_t n, int npairs, do
                                       const double* const* cand, double alpha, const int* partner, int ntot,
                                       const int* tot_count, const int* tot_members, double* const* totals,
                                       double* ratios_host);
+/* The same pass one family at a time, for an iteration that never holds
+ * every candidate at once (the family totals of _total_family_production,
+ * production_operators.jl:1447-1518, built across calls): tot_flags[t] is a
+ * combination of QCLHIP_TOTAL_ACCUMULATE (the left fold starts from the
+ * current value of totals[t], so consecutive calls continue one fold,
+ * bitwise equal to a single pass over all families) and
+ * QCLHIP_TOTAL_CANDIDATE (totals[t] sums the members' CANDIDATE values; such
+ * a total neither mixes nor measures its members, which must each belong to
+ * a mixing total as well: the candidate S< sum of the residual suite).
+ * Totals must not alias a pair's fields. Ratios as
+ * qclhip_mix_fields_residual_totals. Deferrable. */
+#define QCLHIP_TOTAL_ACCUMULATE 1
+#define QCLHIP_TOTAL_CANDIDATE 2
+int qclhip_mix_fields_residual_accumulate(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
+                                          const double* const* cand, double alpha, const int* partner, int ntot,
+                                          const int* tot_count, const int* tot_members, const int* tot_flags,
+                                          double* const* totals, double* ratios_host);
+
+/* The residual suite in two passes, for an iteration that measures the state
+ * before its candidates exist (production_residual_suite, residuals.jl:
+ * 537-772, split by the fields each term reads):
+ * qclhip_residual_state: every term but r_K (r_D, r_A, r_PSD over the six
+ * positivity checks, r_caus) from the Green functions and the total
+ * self-energy; A may be NULL, in which case A = i(G^R - G^R^dagger) is
+ * formed from the G^R tile (bitwise the spectral function). out_host = the
+ * five results with r_K = 0.
+ * qclhip_residual_keldysh: r_K = ||G< - G^R S<_cand G^R^dagger|| / (||G<|| +
+ * 1e-14) alone (:643-664), the candidate families summed in list order;
+ * out_host = the five results with only r_K set. Both deferrable (5 doubles);
+ * each term equals the one of qclhip_residual_suite. */
+int qclhip_residual_state(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,
+                          const double* GL, const double* GG, const double* A, int nfam,
+                          const double* const* famR, const double* const* famL, const double* const* famG,
+                          double* out_host);
+int qclhip_residual_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR, const double* GL,
+                            int ncand, const double* const* candL, double* out_host);
 
 /* ---- The Poisson step closing the SCBA loop (production_solver.jl:
  * 1044-1047). All arrays are device memory. */

@@ -28,8 +28,23 @@ struct ResArgs {
     const c64* famG[kMaxFamilies];
     const c64* cand[kMaxFamilies];
     int nfam, ncand;
+    // 0: the whole suite; 1: every term but r_K (no candidate; the state
+    // pass, before the candidates exist)
+    int mode;
     double* out;  // [7][NE*Nk]: rD, specN, specD, kelN, kelD, psd, caus
 };
+
+// A = i (G - G^dagger) of the wave's own G^R tile into X, the arithmetic of
+// k_spectral (bitwise the stored spectral function).
+template <int LD>
+__device__ inline void tile_spectral(const c64* G, c64* X, int N, int lane) {
+    for (int q = lane; q < N * N; q += 64) {
+        const int i = q % N, j = q / N;
+        const c64 g = G[i * LD + j], gt = G[j * LD + i];
+        const double dx = g.x - gt.x, dy = g.y + gt.y;
+        X[i * LD + j] = cmk(-dy, dx);
+    }
+}
 
 // Gamma = i (S> - S<) of the summed families, cooperatively into the W
 // per-wave tiles (the thread mapping of tile_load).
@@ -152,7 +167,7 @@ __global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
             if (ph == 0) {
                 tile_load_gamma<W, LD>(Xall, wstride, p.famG, p.famL, p.nfam, blk0, nvalid, N, S);
             } else if (ph == 2) {
-                tile_load<W, LD>(Xall, wstride, p.cand, p.ncand, blk0, nvalid, N, S);
+                if (p.mode != 1) tile_load<W, LD>(Xall, wstride, p.cand, p.ncand, blk0, nvalid, N, S);
             } else if (ph == 3) {
                 one[0] = p.GG;
                 tile_load<W, LD>(Xall, wstride, one, 1, blk0, nvalid, N, S);
@@ -163,7 +178,8 @@ __global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
             }
             __syncthreads();
         }
-        if (valid && ph != 0 && ph != 3) {  // T = G X G^dagger
+        const bool no_cand = p.mode == 1 && ph == 2;  // the state pass skips G S<_cand G^dagger
+        if (valid && ph != 0 && ph != 3 && !no_cand) {  // T = G X G^dagger
             rprod<B>(G, X, N, lane, T, false);
             wave_sync();
             rstore<B>(X, N, lane, T);
@@ -177,10 +193,14 @@ __global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
         }
         if (ph == 1 || ph == 2) {
             __syncthreads();
-            one[0] = ph == 1 ? p.A : p.GL;
-            tile_load<W, LD>(Xall, wstride, one, 1, blk0, nvalid, N, S);
+            if (ph == 1 && !p.A) {  // A from the G^R tile (not stored by the caller)
+                if (valid) tile_spectral<LD>(G, X, N, lane);
+            } else {
+                one[0] = ph == 1 ? p.A : p.GL;
+                tile_load<W, LD>(Xall, wstride, one, 1, blk0, nvalid, N, S);
+            }
             __syncthreads();
-            if (valid) {
+            if (valid && !no_cand) {
                 const double dn2 = tile_reg_diff2<B, LD>(X, T, N, lane);
                 const double xn2 = tile_frob2<LD>(X, N, lane, 0.0);
                 if (ph == 1) {
@@ -209,15 +229,65 @@ __global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
     }
 }
 
+// r_K alone (residuals.jl:643-664): ||G< - G^R S<_cand G^R^dagger||^2 and
+// ||G<||^2 per block into rows 3 and 4, the candidate families summed on
+// load. Two LDS tiles per wave (G^R, then the candidate and G<).
+template <int B, int W>
+__global__ __launch_bounds__(64 * W) void k_residual_keldysh(ResArgs p) {
+    constexpr int LD = 8 * B + 1;
+    const int wstride = TileStride<B>::value;
+    extern __shared__ __align__(16) unsigned char smem[];
+    c64* Gall = reinterpret_cast<c64*>(smem);
+    c64* Xall = Gall + W * wstride;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long long nblk = (long long)p.NE * p.Nk;
+    const long long blk0 = (long long)W * xcd_tile(blockIdx.x, (nblk + W - 1) / W);
+    const int nvalid = (int)min((long long)W, nblk - blk0);
+    const bool valid = W == 1 || wave < nvalid;
+    const long long blk = blk0 + (valid ? wave : 0);
+    const int N = p.N;
+    const size_t S = (size_t)nblk;
+    c64* G = Gall + wave * wstride;
+    c64* X = Xall + wave * wstride;
+    c64 T[B][B];
+    const c64* one[1];
+    one[0] = p.GR;
+    tile_load<W, LD>(Gall, wstride, one, 1, blk0, nvalid, N, S);
+    tile_load<W, LD>(Xall, wstride, p.cand, p.ncand, blk0, nvalid, N, S);
+    __syncthreads();
+    if (valid) {
+        rprod<B>(G, X, N, lane, T, false);
+        wave_sync();
+        rstore<B>(X, N, lane, T);
+        wave_sync();
+        rprod<B>(X, G, N, lane, T, true);
+    }
+    __syncthreads();  // every wave is done with its X
+    one[0] = p.GL;
+    tile_load<W, LD>(Xall, wstride, one, 1, blk0, nvalid, N, S);
+    __syncthreads();
+    if (valid) {
+        const double dn2 = tile_reg_diff2<B, LD>(X, T, N, lane);
+        const double xn2 = tile_frob2<LD>(X, N, lane, 0.0);
+        if (lane == 0) {
+            p.out[3 * nblk + blk] = dn2;
+            p.out[4 * nblk + blk] = xn2;
+        }
+    }
+}
+
 // Fixed-order reduction of the per-block values: sums for rows 1-4, maxima
-// for rows 0, 5, 6; a non-finite value makes its residual Inf.
-__global__ __launch_bounds__(1024) void k_residual_reduce(long long nblk, const double* v, double* out) {
+// for rows 0, 5, 6; a non-finite value makes its residual Inf. Rows outside
+// the mask `rows` are not read (their results are 0).
+__global__ __launch_bounds__(1024) void k_residual_reduce(long long nblk, const double* v, double* out,
+                                                          unsigned rows) {
     __shared__ double red[7][1024];
     const int t = threadIdx.x;
     const long long per = (nblk + 1023) / 1024;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     for (long long i = t * per; i < min(nblk, (t + 1) * per); ++i)
         for (int k = 0; k < 7; ++k) {
+            if (!((rows >> k) & 1u)) continue;
             const double x = v[k * nblk + i];
             const bool is_sum = k >= 1 && k <= 4;
             if (!isfinite(x))
@@ -237,8 +307,8 @@ __global__ __launch_bounds__(1024) void k_residual_reduce(long long nblk, const 
     }
     if (t == 0) {
         out[0] = red[0][0];
-        out[1] = sqrt(red[1][0]) / (sqrt(red[2][0]) + 1e-14);
-        out[2] = sqrt(red[3][0]) / (sqrt(red[4][0]) + 1e-14);
+        out[1] = ((rows >> 1) & 1u) ? sqrt(red[1][0]) / (sqrt(red[2][0]) + 1e-14) : 0.0;
+        out[2] = ((rows >> 3) & 1u) ? sqrt(red[3][0]) / (sqrt(red[4][0]) + 1e-14) : 0.0;
         out[3] = red[5][0];
         out[4] = red[6][0];
         for (int k = 1; k <= 2; ++k)
@@ -386,13 +456,20 @@ struct MixTotArgs {
     int nmem[kMaxTotals];
     int ca[kMaxTotals], cb[kMaxTotals], ct[kMaxTotals];  // couple: member positions and term, -1 none
     c64* tot[kMaxTotals];
+    // kTotAccumulate: the fold starts from the total's current value;
+    // kTotCandidate: the total sums the members' candidate values and
+    // neither mixes them nor measures them (another total owns the pairs)
+    int flags[kMaxTotals];
 };
+constexpr int kTotAccumulate = 1;
+constexpr int kTotCandidate = 2;
 
 __global__ __launch_bounds__(256) void k_mix_totals(long long n, MixTotArgs a, double alpha, double* part) {
     constexpr int kAcc = 2 * kMaxFamilies + 2;
     __shared__ double red[kAcc][256];
     const int t = blockIdx.y;
     const int nm = a.nmem[t], pa = a.ca[t], pb = a.cb[t];
+    const bool accumulate = a.flags[t] & kTotAccumulate, cand_only = a.flags[t] & kTotCandidate;
     c64* __restrict__ tot = a.tot[t];
     const double beta = 1.0 - alpha;
     double acc[kAcc];
@@ -404,10 +481,18 @@ __global__ __launch_bounds__(256) void k_mix_totals(long long n, MixTotArgs a, d
 #pragma unroll
         for (int f = 0; f < kMaxFamilies; ++f)  // all loads first
             if (f < nm) {
-                o[f] = a.old_[a.mem[t][f]][i];
+                if (!cand_only) o[f] = a.old_[a.mem[t][f]][i];
                 c[f] = a.cand[a.mem[t][f]][i];
             }
+        const c64 prior = accumulate ? tot[i] : cmk(0.0, 0.0);
         c64 sum = cmk(0.0, 0.0), o1 = sum, c1 = sum, o2 = sum, c2 = sum;
+        if (cand_only) {
+#pragma unroll
+            for (int f = 0; f < kMaxFamilies; ++f)
+                if (f < nm) sum = (f == 0 && !accumulate) ? c[f] : cadd(f == 0 ? prior : sum, c[f]);
+            tot[i] = sum;
+            continue;
+        }
 #pragma unroll
         for (int f = 0; f < kMaxFamilies; ++f)
             if (f < nm) {
@@ -416,7 +501,7 @@ __global__ __launch_bounds__(256) void k_mix_totals(long long n, MixTotArgs a, d
                 acc[2 * f + 1] += c[f].x * c[f].x + c[f].y * c[f].y;
                 const c64 v = alpha != 0.0 ? mix1(o[f], c[f], alpha, beta) : o[f];
                 if (alpha != 0.0) a.old_[a.mem[t][f]][i] = v;
-                sum = f == 0 ? v : cadd(sum, v);
+                sum = (f == 0 && !accumulate) ? v : cadd(f == 0 ? prior : sum, v);
                 if (f == pa) {
                     o1 = o[f];
                     c1 = c[f];
@@ -443,7 +528,7 @@ __global__ __launch_bounds__(256) void k_mix_totals(long long n, MixTotArgs a, d
             for (int k = 0; k < kAcc; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && !cand_only) {
         for (int f = 0; f < nm; ++f) {
             const int p = a.mem[t][f];
             part[(2 * p + 0) * kMixChunks + blockIdx.x] = red[2 * f][0];
@@ -499,61 +584,46 @@ int launch_residual(qclhip_ctx* ctx, const ResArgs& a) {
     return launch_check(ctx, "k_residual_suite");
 }
 
-}  // namespace qclhip
+// The r_K kernel: two tiles per wave, four waves per workgroup (64-byte
+// segments) while two workgroups fit a CU's LDS.
+template <int B>
+int launch_residual_keldysh(qclhip_ctx* ctx, const ResArgs& a) {
+    constexpr size_t wave_bytes = 2 * (size_t)TileStride<B>::value * sizeof(c64);
+    constexpr int W = 4 * wave_bytes <= 81920 ? 4 : (2 * wave_bytes <= 81920 ? 2 : 1);
+    const size_t lds = W * wave_bytes;
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_residual_keldysh<B, W>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long long ntile = ((long long)a.NE * a.Nk + W - 1) / W;
+    hipLaunchKernelGGL((k_residual_keldysh<B, W>), dim3((unsigned)ntile), dim3(64 * W), lds, ctx->stream, a);
+    return launch_check(ctx, "k_residual_keldysh");
+}
 
-using namespace qclhip;
-
-extern "C" {
-
-int qclhip_residual_suite(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,
-                          const double* GL, const double* GG, const double* A, int nfam,
-                          const double* const* famR, const double* const* famL, const double* const* famG,
-                          int ncand, const double* const* candL, double* out_host) {
-    if (!ctx || !grid || !h || !GR || !GL || !GG || !A || !famR || !famL || !famG || !candL || !out_host)
-        return QCLHIP_ERR_ARGUMENT;
-    if (nfam < 1 || nfam > kMaxFamilies || ncand < 1 || ncand > kMaxFamilies)
-        return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..8 families and candidate families");
-    const long long nblk = (long long)grid->NE * grid->Nk;
+// Runs the suite kernel (keldysh_only = false) or the r_K kernel for the
+// basis class of N, reduces the rows of `rows` and returns the 5 results
+// (r_D, r_A, r_K, r_PSD, r_caus) to out_host or the deferred buffer.
+int residual_run(qclhip_ctx* ctx, ResArgs& a, bool keldysh_only, unsigned rows, double* out_host) {
+    const long long nblk = (long long)a.NE * a.Nk;
     double* v = (double*)workspace(ctx, (size_t)(7 * nblk + 8) * sizeof(double));
     if (!v) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
-    ResArgs a{};
-    a.NE = grid->NE;
-    a.Nk = grid->Nk;
-    a.N = grid->Nb;
-    a.eps = grid->eps;
-    a.h = (const c64*)h;
-    a.GR = (const c64*)GR;
-    a.GL = (const c64*)GL;
-    a.GG = (const c64*)GG;
-    a.A = (const c64*)A;
-    for (int f = 0; f < nfam; ++f) {
-        a.famR[f] = (const c64*)famR[f];
-        a.famL[f] = (const c64*)famL[f];
-        a.famG[f] = (const c64*)famG[f];
-    }
-    for (int f = 0; f < ncand; ++f) a.cand[f] = (const c64*)candL[f];
-    a.nfam = nfam;
-    a.ncand = ncand;
     a.out = v;
     int rc;
-    const int N = grid->Nb;
-    if (N <= 8)
-        rc = launch_residual<1>(ctx, a);
-    else if (N <= 16)
-        rc = launch_residual<2>(ctx, a);
-    else if (N <= 24)
-        rc = launch_residual<3>(ctx, a);
-    else if (N <= 32)
-        rc = launch_residual<4>(ctx, a);
-    else if (N <= 40)
-        rc = launch_residual<5>(ctx, a);
-    else if (N <= 48)
-        rc = launch_residual<6>(ctx, a);
-    else
-        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
+    const int N = a.N;
+    const int B = N <= 8 ? 1 : (N <= 16 ? 2 : (N <= 24 ? 3 : (N <= 32 ? 4 : (N <= 40 ? 5 : (N <= 48 ? 6 : 0)))));
+    switch (B) {
+        case 1: rc = keldysh_only ? launch_residual_keldysh<1>(ctx, a) : launch_residual<1>(ctx, a); break;
+        case 2: rc = keldysh_only ? launch_residual_keldysh<2>(ctx, a) : launch_residual<2>(ctx, a); break;
+        case 3: rc = keldysh_only ? launch_residual_keldysh<3>(ctx, a) : launch_residual<3>(ctx, a); break;
+        case 4: rc = keldysh_only ? launch_residual_keldysh<4>(ctx, a) : launch_residual<4>(ctx, a); break;
+        case 5: rc = keldysh_only ? launch_residual_keldysh<5>(ctx, a) : launch_residual<5>(ctx, a); break;
+        case 6: rc = keldysh_only ? launch_residual_keldysh<6>(ctx, a) : launch_residual<6>(ctx, a); break;
+        default: return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
+    }
     if (rc) return rc;
     double* res = v + 7 * nblk;
-    hipLaunchKernelGGL(k_residual_reduce, dim3(1), dim3(1024), 0, ctx->stream, nblk, v, res);
+    hipLaunchKernelGGL(k_residual_reduce, dim3(1), dim3(1024), 0, ctx->stream, nblk, v, res, rows);
     rc = launch_check(ctx, "k_residual_reduce");
     if (rc) return rc;
     bool deferred = false;
@@ -561,6 +631,83 @@ int qclhip_residual_suite(qclhip_ctx* ctx, const qclhip_grid* grid, const double
     if (rc || deferred) return rc;
     QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(out_host, res, 5 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     return qclhip_ctx_synchronize(ctx);
+}
+
+}  // namespace qclhip
+
+using namespace qclhip;
+
+extern "C" {
+
+static int residual_args(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,
+                         const double* GL, const double* GG, const double* A, int nfam, const double* const* famR,
+                         const double* const* famL, const double* const* famG, int ncand,
+                         const double* const* candL, ResArgs* a) {
+    if (nfam < 0 || nfam > kMaxFamilies || ncand < 0 || ncand > kMaxFamilies)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..8 families and candidate families");
+    *a = ResArgs{};
+    a->NE = grid->NE;
+    a->Nk = grid->Nk;
+    a->N = grid->Nb;
+    a->eps = grid->eps;
+    a->h = (const c64*)h;
+    a->GR = (const c64*)GR;
+    a->GL = (const c64*)GL;
+    a->GG = (const c64*)GG;
+    a->A = (const c64*)A;
+    for (int f = 0; f < nfam; ++f) {
+        if (!famR[f] || !famL[f] || !famG[f]) return fail(ctx, QCLHIP_ERR_ARGUMENT, "null family field");
+        a->famR[f] = (const c64*)famR[f];
+        a->famL[f] = (const c64*)famL[f];
+        a->famG[f] = (const c64*)famG[f];
+    }
+    for (int f = 0; f < ncand; ++f) {
+        if (!candL[f]) return fail(ctx, QCLHIP_ERR_ARGUMENT, "null candidate field");
+        a->cand[f] = (const c64*)candL[f];
+    }
+    a->nfam = nfam;
+    a->ncand = ncand;
+    return QCLHIP_OK;
+}
+
+int qclhip_residual_suite(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,
+                          const double* GL, const double* GG, const double* A, int nfam,
+                          const double* const* famR, const double* const* famL, const double* const* famG,
+                          int ncand, const double* const* candL, double* out_host) {
+    if (!ctx || !grid || !h || !GR || !GL || !GG || !A || !famR || !famL || !famG || !candL || !out_host)
+        return QCLHIP_ERR_ARGUMENT;
+    if (nfam < 1 || ncand < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..8 families and candidate families");
+    ResArgs a;
+    int rc = residual_args(ctx, grid, h, GR, GL, GG, A, nfam, famR, famL, famG, ncand, candL, &a);
+    if (rc) return rc;
+    a.mode = 0;
+    return residual_run(ctx, a, false, 0x7fu, out_host);
+}
+
+int qclhip_residual_state(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,
+                          const double* GL, const double* GG, const double* A, int nfam,
+                          const double* const* famR, const double* const* famL, const double* const* famG,
+                          double* out_host) {
+    if (!ctx || !grid || !h || !GR || !GL || !GG || !famR || !famL || !famG || !out_host)
+        return QCLHIP_ERR_ARGUMENT;
+    if (nfam < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..8 families");
+    ResArgs a;
+    int rc = residual_args(ctx, grid, h, GR, GL, GG, A, nfam, famR, famL, famG, 0, nullptr, &a);
+    if (rc) return rc;
+    a.mode = 1;
+    return residual_run(ctx, a, false, 0x67u, out_host);  // rows 0, 1, 2, 5, 6
+}
+
+int qclhip_residual_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR, const double* GL,
+                            int ncand, const double* const* candL, double* out_host) {
+    if (!ctx || !grid || !GR || !GL || !candL || !out_host) return QCLHIP_ERR_ARGUMENT;
+    if (ncand < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..8 candidate families");
+    ResArgs a;
+    int rc = residual_args(ctx, grid, nullptr, GR, GL, nullptr, nullptr, 0, nullptr, nullptr, nullptr, ncand,
+                           candL, &a);
+    if (rc) return rc;
+    a.mode = 2;
+    return residual_run(ctx, a, true, 0x18u, out_host);  // rows 3, 4
 }
 
 int qclhip_selfenergy_residual(qclhip_ctx* ctx, int64_t n, const double* const* old_, const double* const* cand,
@@ -635,10 +782,9 @@ int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* c
     return mix_fields_residual_impl(ctx, n, npairs, old_, cand, alpha, partner, ratios_host, nullptr);
 }
 
-int qclhip_mix_fields_residual_totals(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
-                                      const double* const* cand, double alpha, const int* partner, int ntot,
-                                      const int* tot_count, const int* tot_members, double* const* totals,
-                                      double* ratios_host) {
+static int mix_totals_impl(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_, const double* const* cand,
+                           double alpha, const int* partner, int ntot, const int* tot_count, const int* tot_members,
+                           const int* tot_flags, double* const* totals, double* ratios_host) {
     if (!ctx || !old_ || !cand || !partner || !tot_count || !tot_members || !totals || !ratios_host || n < 0)
         return QCLHIP_ERR_ARGUMENT;
     if (npairs < 1 || npairs > kMaxResPairs) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..32 mixing pairs");
@@ -653,13 +799,20 @@ int qclhip_mix_fields_residual_totals(qclhip_ctx* ctx, int64_t n, int npairs, do
         a.tot[t] = (c64*)totals[t];
         a.nmem[t] = tot_count[t];
         a.ca[t] = a.cb[t] = a.ct[t] = -1;
+        a.flags[t] = tot_flags ? tot_flags[t] : 0;
+        if (a.flags[t] & ~(kTotAccumulate | kTotCandidate))
+            return fail(ctx, QCLHIP_ERR_ARGUMENT, "unknown total flag");
+        for (int u = 0; u < t; ++u)
+            if (a.tot[u] == a.tot[t]) return fail(ctx, QCLHIP_ERR_ARGUMENT, "totals must be distinct fields");
         for (int f = 0; f < tot_count[t]; ++f, ++k) {
             const int p = tot_members[k];
-            if (p < 0 || p >= npairs || owner[p] >= 0)
-                return fail(ctx, QCLHIP_ERR_ARGUMENT, "every pair belongs to exactly one total");
+            if (p < 0 || p >= npairs) return fail(ctx, QCLHIP_ERR_ARGUMENT, "total member out of range");
+            a.mem[t][f] = p;
+            if (a.flags[t] & kTotCandidate) continue;  // reads candidates only; owns no pair
+            if (owner[p] >= 0)
+                return fail(ctx, QCLHIP_ERR_ARGUMENT, "every pair belongs to exactly one mixing total");
             owner[p] = t;
             pos[p] = f;
-            a.mem[t][f] = p;
         }
     }
     int nterms = npairs;
@@ -683,6 +836,10 @@ int qclhip_mix_fields_residual_totals(qclhip_ctx* ctx, int64_t n, int npairs, do
     double* part = (double*)workspace(ctx, ((size_t)2 * nterms * kMixChunks + (size_t)nterms) * sizeof(double));
     if (!part) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
     double* ratios = part + (size_t)2 * nterms * kMixChunks;
+    for (int t = 0; t < ntot; ++t)
+        for (int p = 0; p < npairs; ++p)
+            if ((const void*)a.tot[t] == (const void*)old_[p] || (const void*)a.tot[t] == (const void*)cand[p])
+                return fail(ctx, QCLHIP_ERR_ARGUMENT, "a total must not alias a mixing pair");
     hipLaunchKernelGGL(k_mix_totals, dim3(kMixChunks, ntot), dim3(256), 0, ctx->stream, (long long)n, a, alpha, part);
     hipLaunchKernelGGL(k_mix_residual_reduce, dim3(nterms), dim3(kMixChunks), 0, ctx->stream, part, ratios, nullptr);
     int rc = launch_check(ctx, "mix with totals and self-energy residual");
@@ -693,6 +850,23 @@ int qclhip_mix_fields_residual_totals(qclhip_ctx* ctx, int64_t n, int npairs, do
     QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(ratios_host, ratios, nterms * sizeof(double), hipMemcpyDeviceToHost,
                                        ctx->stream));
     return qclhip_ctx_synchronize(ctx);
+}
+
+int qclhip_mix_fields_residual_totals(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
+                                      const double* const* cand, double alpha, const int* partner, int ntot,
+                                      const int* tot_count, const int* tot_members, double* const* totals,
+                                      double* ratios_host) {
+    return mix_totals_impl(ctx, n, npairs, old_, cand, alpha, partner, ntot, tot_count, tot_members, nullptr,
+                           totals, ratios_host);
+}
+
+int qclhip_mix_fields_residual_accumulate(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_,
+                                          const double* const* cand, double alpha, const int* partner, int ntot,
+                                          const int* tot_count, const int* tot_members, const int* tot_flags,
+                                          double* const* totals, double* ratios_host) {
+    if (!tot_flags) return QCLHIP_ERR_ARGUMENT;
+    return mix_totals_impl(ctx, n, npairs, old_, cand, alpha, partner, ntot, tot_count, tot_members, tot_flags,
+                           totals, ratios_host);
 }
 
 int qclhip_mix_fields_change(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_, const double* const* cand,

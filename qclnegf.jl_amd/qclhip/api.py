@@ -580,8 +580,12 @@ def mix_fields_residual(pairs, alpha: float, partner=None, totals=None):
     (p, partner[p]) with partner[p] > p, for the sum of the two pairs. A
     Deferred of that many values in deferred-results mode.
 
-    totals: optional list of (field, [pair indices]); each field receives the
-    left-fold sum of its members' mixed values (every pair in one total)."""
+    totals: optional list of (field, [pair indices]) or (field, [pair
+    indices], flags); each field receives the left-fold sum of its members'
+    mixed values (every pair in one total). flags: "accumulate" continues
+    the fold from the field's current value; "candidate" sums the members'
+    candidate values instead (the pairs then also belong to a mixing total);
+    both may be given as "accumulate,candidate"."""
     pairs = list(pairs)
     if not pairs:
         raise _lib.ArgumentError("no pairs to mix")
@@ -601,15 +605,24 @@ def mix_fields_residual(pairs, alpha: float, partner=None, totals=None):
         ctx.call("qclhip_mix_fields_residual", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha),
                  (ct.c_int * len(pairs))(*partner), out)
     else:
-        totals = list(totals)
-        for t, _ in totals:
+        totals = [tuple(t) if len(t) == 3 else (t[0], t[1], "") for t in totals]
+        for t, _, _ in totals:
             if t.dtype != torch.complex128 or t.numel() != n or not t.is_contiguous():
                 raise _lib.ArgumentError("totals must be contiguous complex128 fields of the pairs' size")
-        counts = [len(m) for _, m in totals]
-        members = [int(p) for _, m in totals for p in m]
-        ctx.call("qclhip_mix_fields_residual_totals", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha),
-                 (ct.c_int * len(pairs))(*partner), len(totals), (ct.c_int * len(counts))(*counts),
-                 (ct.c_int * len(members))(*members), (_vp * len(totals))(*[t.data_ptr() for t, _ in totals]), out)
+        counts = [len(m) for _, m, _ in totals]
+        members = [int(p) for _, m, _ in totals for p in m]
+        flag_bits = {"": 0, "accumulate": 1, "candidate": 2}
+        try:
+            flags = [sum(flag_bits[f.strip()] for f in fl.split(",")) for _, _, fl in totals]
+        except KeyError:
+            raise _lib.ArgumentError("total flags are 'accumulate' and/or 'candidate'") from None
+        common = (ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha), (ct.c_int * len(pairs))(*partner),
+                  len(totals), (ct.c_int * len(counts))(*counts), (ct.c_int * len(members))(*members))
+        tptr = (_vp * len(totals))(*[t.data_ptr() for t, _, _ in totals])
+        if any(flags):
+            ctx.call("qclhip_mix_fields_residual_accumulate", *common, (ct.c_int * len(flags))(*flags), tptr, out)
+        else:
+            ctx.call("qclhip_mix_fields_residual_totals", *common, tptr, out)
     return Deferred(slot0, nterms) if ctx.deferring else list(out)
 
 
@@ -788,6 +801,41 @@ def residual_suite(grid: Grid, h, green, families, candidate_lesser):
     if grid.ctx.deferring:
         return Deferred(slot0, 5)
     return dict(r_D=out[0], r_A=out[1], r_K=out[2], r_PSD=out[3], r_caus=out[4])
+
+
+def residual_state(grid: Grid, h, green, families):
+    """Every term of production_residual_suite (residuals.jl:537-772) but r_K:
+    r_D, r_A, r_PSD, r_caus from the Green state (A may be None: formed from
+    G^R) and the total self-energy, before the candidates exist. Returns the
+    dict of the five keys with r_K = 0, or a Deferred of 5 values."""
+    for fam in families:
+        for t in fam:
+            _check_field(grid, t, "self-energy")
+    nf = len(families)
+    arr = lambda k: (_vp * nf)(*[f[k].data_ptr() for f in families])
+    out = (ct.c_double * 5)()
+    slot0 = grid.ctx.result_slot() if grid.ctx.deferring else 0
+    grid.ctx.call("qclhip_residual_state", grid.handle, _ptr(h), _ptr(green.GR), _ptr(green.GL), _ptr(green.GG),
+                  _ptr(green.A), nf, arr(0), arr(1), arr(2), out)
+    if grid.ctx.deferring:
+        return Deferred(slot0, 5)
+    return dict(r_D=out[0], r_A=out[1], r_K=out[2], r_PSD=out[3], r_caus=out[4])
+
+
+def residual_keldysh(grid: Grid, green, candidate_lesser) -> float:
+    """r_K of production_residual_suite (residuals.jl:643-664) alone, from
+    G^R, G< and the unmixed candidate S< (a list of fields summed in order).
+    Returns r_K, or a Deferred of 5 values (r_K in position 2)."""
+    cands = list(candidate_lesser)
+    for t in cands:
+        _check_field(grid, t, "candidate")
+    carr = (_vp * len(cands))(*[t.data_ptr() for t in cands])
+    out = (ct.c_double * 5)()
+    slot0 = grid.ctx.result_slot() if grid.ctx.deferring else 0
+    grid.ctx.call("qclhip_residual_keldysh", grid.handle, _ptr(green.GR), _ptr(green.GL), len(cands), carr, out)
+    if grid.ctx.deferring:
+        return Deferred(slot0, 5)
+    return out[2]
 
 
 def selfenergy_residual(old, candidate):

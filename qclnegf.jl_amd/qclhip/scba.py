@@ -240,6 +240,112 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     return row
 
 
+class LeanWorkspace:
+    """Buffers of the memory-lean iteration: three candidate fields (one
+    family at a time), the running candidate S< sum of the residual suite and
+    the family totals. With M mechanisms the iteration then holds
+    3 (G^R, G<, G>) + 3(M + 2) (state) + 3 (totals) + 3 + 1 fields instead of
+    the reference's 3M + 15 candidate + 4 green + 3M + 9 stored ones
+    (production_cache.jl:570-584): 31 fields at M = 5."""
+
+    def __init__(self, p: ScbaProblem):
+        g = p.grid
+        if p.mixing != "linear":
+            raise ValueError("the lean iteration mixes linearly, one family at a time")
+        self.tmp = [api.empty_field(g) for _ in range(3)]
+        self.csum = api.empty_field(g)
+        self.totals = new_family(g)
+        self.totals_valid = False
+
+
+def scba_iteration_lean(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, timings: dict | None = None):
+    """One pass of the production loop (production_solver.jl:435-844) that
+    never holds every candidate: the fixed-point measurement of the state
+    (every residual but r_K) and the observables run right after the GF
+    solve; then each family's candidate is built (embedding plus/minus one
+    component at a time, then every mechanism), measured and mixed at once
+    into the state, into the totals of the next solve and into the candidate
+    S< sum; r_K runs last on that sum. Every field and scalar equals the
+    standard iteration's bit for bit (same kernels, same folds); the Green
+    state keeps no A (formed from G^R where the residuals need it)."""
+    g = p.grid
+    if g.ctx.deferring:
+        raise ValueError("the lean iteration runs eagerly")
+    if not p.measure:
+        raise ValueError("the lean iteration always measures")
+
+    def stamp():
+        if timings is not None:
+            g.ctx.synchronize()
+        return time.perf_counter()
+
+    t0 = stamp()
+    if s.green is not None:
+        s.green.A = None  # not kept
+    if ws.totals_valid:
+        green, info = api.current_green_production(g, p.h, [ws.totals.parts()], p.target, state=s.green,
+                                                   spectral=False)
+    else:
+        fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
+        green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=False,
+                                                   totals=ws.totals.parts())
+    s.green = green
+    t1 = stamp()
+    suite = api.residual_state(g, p.h, green, [ws.totals.parts()])
+    Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
+    pops = api.state_populations(g, green.GL, p.target)
+    t2 = stamp()
+    ratios = []
+    totals = ws.totals.parts()
+    tmp = ws.tmp
+    for c, X in enumerate((green.GR, green.GL, green.GG)):  # embedding, one component at a time
+        api.embedding_component_production(g, p.Tplus, X, p.plusE, S=tmp[0])
+        api.embedding_component_production(g, p.Tminus, X, p.minusE, S=tmp[1])
+        tot = [(totals[c], [0, 1])]
+        if c == 1:
+            tot.append((ws.csum, [0, 1], "candidate"))
+        pairs = [(s.plus.parts()[c], tmp[0]), (s.minus.parts()[c], tmp[1])]
+        ratios += api.mix_fields_residual(pairs, p.alpha_mix, [1, 0], totals=tot)
+    t3 = stamp()
+    roundoff = 0.0
+    cand = Family(tmp[0], tmp[1], tmp[2])
+    for mech in p.mechanisms:
+        if mech.is_lo:
+            api.production_lo_contraction(mech.kernel, green.GL, green.GG, mech.qK, p.NLO, p.plusLO,
+                                          p.minusLO, SL=cand.L, SG=cand.G)
+        else:
+            api.production_static_contraction(mech.kernel, green.GL, mech.qK, S=cand.L)
+            api.production_static_contraction(mech.kernel, green.GG, mech.qK, S=cand.G)
+        if p.verify_roundoff:
+            _, _, r = api.retarded_self_energy_fft(p.hilbert, cand.L, cand.G, return_roundoff=True, SR=cand.R)
+            roundoff = max(roundoff, r)
+        else:
+            api.retarded_self_energy_fft(p.hilbert, cand.L, cand.G, SR=cand.R)
+        fam = s.scattering[mech.name]
+        pairs = list(zip(fam.parts(), cand.parts()))
+        tot = [(totals[0], [0], "accumulate"), (totals[1], [1], "accumulate"), (totals[2], [2], "accumulate"),
+               (ws.csum, [1], "accumulate,candidate")]
+        ratios += api.mix_fields_residual(pairs, p.alpha_mix, [-1, -1, -1], totals=tot)
+    t4 = stamp()
+    r_K = api.residual_keldysh(g, green, [ws.csum])
+    ws.totals_valid = True
+    g.ctx.synchronize()
+    t5 = time.perf_counter()
+    if timings is not None:
+        for k, v in (("dyson", t1 - t0), ("residuals_state_observables", t2 - t1),
+                     ("candidate_embedding_mix", t3 - t2), ("candidate_scattering_mix", t4 - t3),
+                     ("residual_keldysh", t5 - t4)):
+            timings[k] = timings.get(k, 0.0) + v
+    row = dict(lam=info["lam"], number=info["number"], holes=info["holes"], Jbar=Jbar, roundoff=roundoff,
+               populations=pops)
+    row.update(suite)
+    row["r_K"] = r_K
+    row["r_Sigma"] = max(ratios)
+    row["r_lambda"] = abs(info["lam"] - 1.0)
+    s.history.append(row)
+    return row
+
+
 def solve_scba_production(p: ScbaProblem, s: ScbaState, max_iterations: int, tol_population: float = 0.0):
     """Run up to max_iterations passes; stops early when the relative
     population change falls below tol_population (when > 0)."""

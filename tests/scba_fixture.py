@@ -42,6 +42,21 @@ def make_problem(N=4, NE=64, Nk=3, seed=17, lo_kind="low_rank", alpha_mix=0.3):
     return prob
 
 
+def make_problem5(N=6, NE=128, Nk=4, seed=29, alpha_mix=0.3):
+    """The five mechanisms of the full-iteration workload in their device
+    representations: acoustic and alloy momentum-independent, IFR separable,
+    LO low-rank and thermal, ionized impurity low-rank."""
+    prob = make_problem(N=N, NE=NE, Nk=Nk, seed=seed, alpha_mix=alpha_mix)
+    rng = np.random.default_rng(seed + 2)
+    mech = [m for m in prob["mechanisms"] if m["name"] in ("acoustic", "IFR", "LO")]
+    mech.insert(1, dict(name="alloy", kind="momentum_independent", a=syn.covariance_block(rng, N, 3, 0.04),
+                        b=None, c=None, rank=0, qK=1.0, is_lo=False))
+    l, r = syn.low_rank_factors(rng, N, Nk, r=2, scale=0.04)
+    mech.append(dict(name="impurity", kind="low_rank", a=l, b=r, c=None, rank=2, qK=1.0, is_lo=False))
+    prob["mechanisms"] = mech
+    return prob
+
+
 def oracle_kernels(prob):
     return [orc.make_kernel(m["kind"], m["a"], m["b"], m["c"], rank=m["rank"]) for m in prob["mechanisms"]]
 
