@@ -223,6 +223,115 @@ def scba_bench(iters, warmup, adjoint=True):
     return out
 
 
+# Algorithmic HBM traffic of each phase of the lean iteration, in fields
+# (one field = 16 N^2 NE Nk bytes): GF solve reads the three totals and writes
+# G^R, G<, G>; the state residual pass reads G^R, S^R, S<, S> (Gamma), G<,
+# G>, S<, S> (raw products) and the observables read S<_+, S>_+, G<, G>;
+# each embedding component reads its field twice (plus, minus) and writes
+# two candidates, then mixing reads 2 old + 2 candidates and writes 2 old
+# and the total (and the candidate S< sum for S<); each mechanism reads G<
+# and G> and writes S<, S> (contraction), reads S<, S> and writes S^R
+# (Hilbert), then mixing reads 3 old + 3 candidates, writes 3 old, reads and
+# writes 3 totals and the candidate sum; r_K reads G^R, the sum and G<.
+def lean_phase_fields(M):
+    return {"dyson": 6, "residuals_state_observables": 12, "candidate_embedding_mix": 34,
+            "candidate_scattering_mix": 24 * M, "residual_keldysh": 3}
+
+
+def scba_full_bench(iters=2, warmup=1):
+    """The full SCBA iteration of a six-layer GaAs/Al0.15Ga0.85As period at
+    N=32, NE=4096, Nk=128 (524,288 blocks; one field 8.59 GB): five
+    mechanisms (LO thermal low rank, acoustic and alloy momentum independent,
+    ionized impurity low rank, IFR separable), the embedding, FFT Hilbert at
+    L=8192 with the 2L witness, the state residuals, r_K and r_Sigma every
+    iteration, the memory-lean order (31 fields resident); the GF solve in
+    FP32 with FP64 refinement, mixing, normalization, observables and
+    Poisson in FP64. Reports iter/s, per-phase ms and HBM rates, the
+    full-size invariants and the FP32-vs-FP64 error of every observable."""
+    import numpy as np
+    import torch
+
+    from qclhip import api, outer, scba, synthetic as syn
+
+    torch.cuda.empty_cache()
+    free0, total = torch.cuda.mem_get_info()
+    t0 = time.perf_counter()
+    w = syn.two_well_period()
+    p, s, qK, mu = syn.period_scba_problem(w, precision="fp32_refine")
+    ws = scba.LeanWorkspace(p)
+    g = p.grid
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    resident = (total - torch.cuda.mem_get_info()[0]) / 2**30
+    log(f"  period problem ready in {setup_s:.1f} s, mu {mu:.6f}, qK {qK}, "
+        f"device memory in use {resident:.1f} GiB of {total / 2**30:.1f}")
+    for _ in range(warmup):
+        row = scba.scba_iteration_lean(p, s, ws)
+        log(f"  warm-up iteration: lam {row['lam']:.9f} J {row['Jbar']:.9e} r_Sigma {row['r_Sigma']:.3e}")
+    timings = {}
+    rows = []
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        rows.append(scba.scba_iteration_lean(p, s, ws, timings))
+        log(f"  iteration: lam {rows[-1]['lam']:.9f} J {rows[-1]['Jbar']:.9e} r_Sigma {rows[-1]['r_Sigma']:.3e}")
+    dt = (time.perf_counter() - t0) / iters
+    field = 16.0 * w["N"] ** 2 * w["NE"] * w["Nk"]
+    phases = {}
+    for k, nf in lean_phase_fields(len(p.mechanisms)).items():
+        ms = 1e3 * timings[k] / iters
+        phases[k] = dict(ms=ms, fields=nf, GBs=nf * field / (ms * 1e-3) / 1e9,
+                         hbm_frac=nf * field / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS)
+    nblk = w["NE"] * w["Nk"]
+    phases["dyson"]["tflops"] = nblk * 40.0 * w["N"] ** 3 / (phases["dyson"]["ms"] * 1e-3) / 1e12
+    last = rows[-1]
+    number = api.number_functional(g, s.green.GL)
+    # FP32+refine against FP64 on the same state (the totals of the last mix)
+    chi = api.basis_to_device(w["phi"])
+    obs = {}
+    for prec in ("fp64", "fp32_refine"):
+        g.set_precision(prec)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        green, info = api.current_green_production(g, p.h, [ws.totals.parts()], p.target, state=s.green,
+                                                   spectral=False)
+        torch.cuda.synchronize()
+        gf_ms = (time.perf_counter() - t1) * 1e3
+        J = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
+        pops = api.state_populations(g, green.GL, p.target)
+        dens = api.electron_density(g, green.GL, chi, 2.0).cpu().numpy()
+        obs[prec] = dict(J=J, lam=info["lam"], pops=pops, dens=dens, gf_ms=gf_ms,
+                         fallbacks=g.refine_fallbacks() if prec != "fp64" else 0)
+    a, b = obs["fp64"], obs["fp32_refine"]
+    errors = dict(J=abs(b["J"] - a["J"]) / abs(a["J"]), lam=abs(b["lam"] - a["lam"]) / abs(a["lam"]),
+                  populations=float(np.linalg.norm(b["pops"] - a["pops"]) / np.linalg.norm(a["pops"])),
+                  density=float(np.linalg.norm(b["dens"] - a["dens"]) / np.linalg.norm(a["dens"])))
+    # Poisson step at N_z = 512 from the FP32+refine G<
+    dev = lambda x: torch.as_tensor(x, device="cuda", dtype=torch.float64)
+    op = outer.OuterProblem(dev(w["eps_r"]), dev(w["wx"]), dev(w["ND"]), w["lambda_P"], chi,
+                            api.basis_to_device(w["phi"] * math.sqrt(w["dz"])), api.matrix_to_device(w["S"]),
+                            dev(w["kin"]), api.matrix_to_device(w["Minv"]), ND2D=w["ND2D"], Lp=w["Lp"])
+    U = torch.zeros(w["Nz"], dtype=torch.float64, device="cuda")
+    outer.poisson_step(op, g, s.green.GL, U)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prow = outer.poisson_step(op, g, s.green.GL, U)
+    torch.cuda.synchronize()
+    poisson_ms = (time.perf_counter() - t1) * 1e3
+    out = dict(iter_s=dt, iter_per_s=1.0 / dt, phases=phases, setup_s=setup_s,
+               memory_GiB=dict(before=(total - free0) / 2**30, resident=resident, device=total / 2**30,
+                               field=field / 2**30), fields_resident=31,
+               invariants=dict(number_rel=abs(number - p.target) / p.target, roundoff_witness=last["roundoff"],
+                               r_D=last["r_D"], r_caus=last["r_caus"], r_PSD=last["r_PSD"]),
+               fp32_vs_fp64=errors, gf_ms=dict(fp64=a["gf_ms"], fp32_refine=b["gf_ms"]),
+               fp64_fallbacks=int(b["fallbacks"]),
+               last={k: v for k, v in last.items() if k != "populations"},
+               poisson=dict(ms=poisson_ms, r_P=prow.get("r_P"), neutrality=prow.get("neutrality")),
+               qK=qK, mu=mu, thicknesses_nm=[float(x) for x in w["thick_nm"]])
+    del p, s, ws, op, chi
+    torch.cuda.empty_cache()
+    return out
+
+
 def outer_bench(p, s, Nz=512, reps=5):
     """The Poisson step after an SCBA run at the same size: density of G<
     (N_E N_k N_b^2 values), the bordered Poisson LU solve (N_z + 1 = 513),
@@ -379,6 +488,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scba-iters", type=int, default=3)
     ap.add_argument("--no-scba", action="store_true")
+    ap.add_argument("--no-full", action="store_true", help="skip the full-iteration period workload (N=32)")
+    ap.add_argument("--full-iters", type=int, default=2)
+    ap.add_argument("--only-full", action="store_true", help="run only the full-iteration workload (profiling)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -404,6 +516,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy calibration: 3 x 1 GiB device-to-device copies")
+    if args.only_full:
+        log("full SCBA iteration: GaAs/Al0.15Ga0.85As period, N=32 NE=4096 Nk=128, five mechanisms")
+        full = scba_full_bench(args.full_iters, 1)
+        print(json.dumps({"scba_full": full}, default=float), flush=True)
+        return
     log("gf batch: N=24 NE=2048 Nk=64")
     wanted = set(args.variants.split(","))
     sel = [(p, e) for p, e in VARIANTS if (f"{p}_{e}" if p != "fp32_products" else p) in wanted]
@@ -424,6 +541,12 @@ def main():
         log(f"poisson step {sc['outer']['step_ms']:.2f} ms {sc['outer']}")
         if "adjoint_iter_ms" in sc:
             log(f"adjoint {sc['adjoint_iter_ms']:.1f} ms/iter")
+    full = None
+    if not args.no_full:
+        log("full SCBA iteration: GaAs/Al0.15Ga0.85As period, N=32 NE=4096 Nk=128, five mechanisms")
+        full = scba_full_bench(args.full_iters, 1)
+        log(f"full iteration {full['iter_s']*1e3:.1f} ms/iter, phases "
+            f"{ {k: round(v['ms'], 2) for k, v in full['phases'].items()} }, fp32 vs fp64 {full['fp32_vs_fp64']}")
     gr = None
     if not args.no_graph:
         log("graph replay of small SCBA iterations")
@@ -512,6 +635,15 @@ def main():
             line["adjoint"] = {"iter_ms": sc["adjoint_iter_ms"], "gradient_with_1_iter_ms": sc["adjoint_gradient_1iter_ms"],
                                "config": "N=24 NE=2048 Nk=64, same problem as the SCBA iteration: one damped "
                                          "fixed-point step = candidate reverse + current_green reverse + mixing"}
+    if full is not None:
+        line["scba_full_iter_per_s"] = full["iter_per_s"]
+        line["scba_full"] = dict(full, config="six-layer GaAs/Al0.15Ga0.85As period (thicknesses U[2,20] nm, seeded), "
+                                               "N=32 NE=4096 Nk=128, LO (36 meV, thermal, low rank r=6), acoustic "
+                                               "and alloy (momentum independent), ionized impurity (3e16 cm^-3, "
+                                               "low rank r=4), IFR (6 interfaces, separable, Lambda=8 nm); FFT "
+                                               "Hilbert L=8192 with 2L witness; GF solve FP32+2 FP64 refinement "
+                                               "steps, FP64 mixing/normalization/Poisson (N_z=512); "
+                                               "memory-lean order")
     if gr is not None:
         line["graph"] = gr
     if bt is not None:

@@ -108,7 +108,7 @@ int gemm_nt(qclhip_ctx* ctx, long long m, long long n, long long k, double alpha
 constexpr int kMaxRank = 16;
 
 // C[row, j] = sum_q A[row + M q] * Bm[j + J q]  (q < K), J <= kMaxRank.
-__global__ void k_thin_reduce(long long M, int K, int J, const c64* __restrict__ A, const c64* __restrict__ Bm,
+__global__ __launch_bounds__(1024) void k_thin_reduce(long long M, int K, int J, const c64* __restrict__ A, const c64* __restrict__ Bm,
                               c64* __restrict__ C) {
     extern __shared__ c64 sb[];
     for (int idx = threadIdx.x; idx < J * K; idx += blockDim.x) sb[idx] = Bm[idx];
@@ -132,7 +132,7 @@ __global__ void k_thin_reduce(long long M, int K, int J, const c64* __restrict__
 }
 
 // S[row + M q] = f * sum_j C[row + M j] * Bm[q + K j] + beta * S  (q < K).
-__global__ void k_thin_expand(long long M, int K, int J, const c64* __restrict__ C, const c64* __restrict__ Bm,
+__global__ __launch_bounds__(1024) void k_thin_expand(long long M, int K, int J, const c64* __restrict__ C, const c64* __restrict__ Bm,
                               double f, double beta, c64* __restrict__ S) {
     extern __shared__ c64 sb[];
     for (int idx = threadIdx.x; idx < J * K; idx += blockDim.x) sb[idx] = Bm[idx];
@@ -314,18 +314,28 @@ int contract_core(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double 
         }
         case QCLHIP_KERNEL_SEPARABLE_IFR: {
             const long long J = K->rank, M = NE * Nk;
-            if (J > kMaxRank || (size_t)J * N2 * sizeof(c64) > 60 * 1024)
+            // The J x N^2 factor sits in LDS; above 64 KiB (e.g. six interfaces
+            // at N = 32: 96 KiB) one workgroup fills a CU, so it takes 1024
+            // threads (16 waves) to keep the strided field reads in flight.
+            const size_t lds = (size_t)J * N2 * sizeof(c64);
+            if (J > kMaxRank || lds > ctx->max_lds_per_block)
                 return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "separable kernel rank too large for this build");
+            const int threads = lds > 65536 ? 1024 : 256;
+            if (lds > 65536) {
+                QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_thin_reduce,
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_thin_expand,
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            }
             c64* C1 = (c64*)workspace(ctx, (size_t)2 * M * J * sizeof(c64));
             if (!C1) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
             c64* C2 = C1 + M * J;
-            const size_t lds = (size_t)J * N2 * sizeof(c64);
-            const unsigned blocks = (unsigned)((M + 255) / 256);
-            hipLaunchKernelGGL(k_thin_reduce, dim3(blocks), dim3(256), lds, ctx->stream, M, (int)N2, (int)J, G,
+            const unsigned blocks = (unsigned)((M + threads - 1) / threads);
+            hipLaunchKernelGGL(k_thin_reduce, dim3(blocks), dim3(threads), lds, ctx->stream, M, (int)N2, (int)J, G,
                                K->b, C1);
             hipLaunchKernelGGL(k_ifr_radial, dim3((unsigned)((M * J + 255) / 256)), dim3(256), 0, ctx->stream,
                                (int)NE, (int)Nk, (int)J, C1, K->c, C2);
-            hipLaunchKernelGGL(k_thin_expand, dim3(blocks), dim3(256), lds, ctx->stream, M, (int)N2, (int)J, C2,
+            hipLaunchKernelGGL(k_thin_expand, dim3(blocks), dim3(threads), lds, ctx->stream, M, (int)N2, (int)J, C2,
                                K->a, f, beta, S);
             return launch_check(ctx, "separable contraction");
         }

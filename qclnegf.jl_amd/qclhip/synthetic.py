@@ -199,3 +199,178 @@ def dense_covariance_kernel(rng, N, Nk, nvert=2, scale=0.03):
         for qp in range(N * N):
             F[q * Nk:(q + 1) * Nk, qp * Nk:(qp + 1) * Nk] = R * B[q, qp]
     return F
+
+
+# ---- one GaAs/Al(x)Ga(1-x)As period: the full-iteration workload ----------
+
+EPS0 = 8.8541878128e-12
+# Poisson coupling e^2/(eps0 E0 L0) of the scale system (scaling.jl:10-24)
+LAMBDA_P = QE ** 2 / (EPS0 * E0_EV * QE * L0_M)
+
+
+def two_well_period(N=32, NE=4096, Nk=128, Nz=512, x_al=0.15, Ep_eV=0.05, T_K=77.0, doping_cm3=3e16,
+                    seed=SEED + 30):
+    """A six-layer Al(x)Ga(1-x)As / GaAs period (barrier first, alternating)
+    with seeded thicknesses U[2, 20] nm, on Nz grid points, in the solver's
+    units. Material inputs are explicit (the reference's Layer takes the band
+    edge and mass directly, types.jl:11-66): conduction-band offset
+    0.835 x eV, masses 0.067 (GaAs) and 0.067 + 0.083 x m0, permittivities
+    12.9 and 12.9 - 2.84 x; the bias drops Ep_eV over the period; the last
+    well is doped at doping_cm3.
+
+    Basis: the N lowest eigenstates phi of the period's finite-difference
+    BenDaniel-Duke Hamiltonian (hard walls), so H0 = diag(E); the in-plane
+    mass matrix Minv_ab = sum_z phi_a phi_b m_ref/m(z); h_m = H0 + Cbar
+    kappa_m^2 Minv; the coupling across the period boundary
+    T+_ab = -t phi_a(z_last) phi_b(z_first) (t the boundary hopping), T- = T+^H.
+
+    Scattering kernels in the device representations, their shapes from the
+    basis (magnitudes are calibrated later through qK):
+      acoustic, momentum independent: sum_z dz V_z (x) conj(V_z), V_z[a,c] = phi_a phi_c;
+      alloy, momentum independent: the same over the barrier points;
+      IFR, separable: one rank per interface z_j, u_j = phi(z_j), radial
+        exp(-((kappa_m - kappa_m') Lambda / 2)^2), Lambda = 8 nm;
+      LO, low rank: r projectors u_i u_i^H of smooth envelopes
+        (u_i[a] = sum_z phi_a g_i dz), Froehlich-like weights 1/(1 + (kappa/q_i)^2);
+      impurity, low rank: the same with envelopes exp(-|z - z_d|/l_i) about the doped layer.
+    Returns a dict with the grids, h (Nk,N,N), Tplus, Tminus, the kernel
+    factors, the Poisson inputs (eps_r, wx, ND, lambda_P, chi, Phi, S, kin,
+    Minv, ND2D, Lp) and the layer thicknesses."""
+    rng = np.random.default_rng(seed)
+    thick_nm = rng.uniform(2.0, 20.0, 6)
+    barrier = np.array([True, False, True, False, True, False])
+    Lp = thick_nm.sum() * 1e-9 / L0_M
+    dz = Lp / Nz
+    z = (np.arange(Nz) + 0.5) * dz
+    edges = np.concatenate([[0.0], np.cumsum(thick_nm) * 1e-9 / L0_M])
+    layer = np.minimum(np.searchsorted(edges, z, side="right") - 1, 5)
+    in_barrier = barrier[layer]
+    dEc = 0.835 * x_al / E0_EV
+    mass = np.where(in_barrier, (0.067 + 0.083 * x_al) / 0.067, 1.0)  # in m_ref
+    V = np.where(in_barrier, dEc, 0.0) - (Ep_eV / E0_EV) * z / Lp
+    m_half = 2.0 / (1.0 / mass[:-1] + 1.0 / mass[1:])
+    off = -CBAR / (m_half * dz * dz)
+    diag = V.copy()
+    diag[:-1] += CBAR / (m_half * dz * dz)
+    diag[1:] += CBAR / (m_half * dz * dz)
+    diag[0] += CBAR / (mass[0] * dz * dz)    # hard walls
+    diag[-1] += CBAR / (mass[-1] * dz * dz)
+    H = np.diag(diag) + np.diag(off, 1) + np.diag(off, -1)
+    E, vec = np.linalg.eigh(H)
+    phi = vec[:, :N] / math.sqrt(dz)  # continuum normalization: sum phi^2 dz = 1
+    Ebasis = E[:N]
+    H0 = np.diag(Ebasis).astype(np.complex128)
+    Minv = ((phi * (1.0 / mass)[:, None]).T @ phi * dz).astype(np.complex128)
+    eps, wE = energy_grid(NE)
+    kappa, wk = momentum_grid(Nk, kmax_for_energy(0.3))
+    kin = CBAR * kappa ** 2
+    h = np.zeros((Nk, N, N), np.complex128, order="F")
+    for m in range(Nk):
+        h[m] = H0 + kin[m] * Minv
+    t_edge = CBAR / (mass[0] * dz * dz)
+    Tplus = (-t_edge * dz * np.outer(phi[-1], phi[0])).astype(np.complex128)
+    # kernels
+    Vz = np.einsum("za,zc->zac", phi, phi).reshape(Nz, N * N)  # V_z[a,c], (a,c) with a fastest
+
+    def mi_block(weight):
+        # B[(a,b),(c,d)] = sum_z w_z V_z[a,c] V_z[b,d] (real basis)
+        M = (Vz * weight[:, None]).T @ Vz  # M[(a,c),(b,d)]
+        return np.ascontiguousarray(M.reshape(N, N, N, N).transpose(0, 2, 1, 3)).reshape(N * N, N * N).astype(
+            np.complex128)
+
+    acoustic = mi_block(np.full(Nz, dz))
+    alloy = mi_block(dz * in_barrier.astype(float) * x_al * (1 - x_al))
+    zint = edges[1:]  # six interfaces per period (the last is the period boundary)
+    J = len(zint)
+    idx = np.minimum((zint / dz).astype(int), Nz - 1)
+    ifr_left = np.zeros((N * N, J), np.complex128)
+    ifr_right = np.zeros((J, N * N), np.complex128)
+    for j, iz in enumerate(idx):
+        u = phi[iz]
+        P = np.outer(u, u)  # P[a,b]
+        ifr_left[:, j] = P.reshape(-1, order="F")
+        ifr_right[j, :] = P.reshape(-1, order="F")
+    Lam = 8e-9 / L0_M
+    ifr_radial = np.exp(-(((kappa[:, None] - kappa[None, :]) * Lam / 2) ** 2)).astype(np.complex128)
+
+    def low_rank(envelopes, weights):
+        r = len(envelopes)
+        Nc = Nk * N * N
+        left = np.zeros((Nc, r), np.complex128)
+        radj = np.zeros((r, Nc), np.complex128)
+        for i, (g, wm) in enumerate(zip(envelopes, weights)):
+            u = phi.T @ g * dz
+            u /= max(np.linalg.norm(u), 1e-300)
+            P = np.outer(u, u).reshape(-1, order="F")  # (a,b), a fastest
+            col = (wm[:, None] * P[None, :]).reshape(-1, order="F")  # (m,(a,b)), m fastest
+            left[:, i] = col
+            radj[i, :] = np.conj(col)
+        return left, radj
+
+    zc = z / Lp
+    lo_env = [np.exp(-((zc - c) / 0.18) ** 2) for c in (0.1, 0.3, 0.5, 0.7, 0.9, 0.5)]
+    lo_env[-1] = np.ones(Nz)
+    qs = [0.5, 1.0, 2.0, 1.0, 0.5, 3.0]
+    lo_w = [1.0 / (1.0 + (kappa / q) ** 2) for q in qs]
+    lo_left, lo_radj = low_rank(lo_env, lo_w)
+    zd = 0.5 * (edges[5] + edges[6])
+    imp_env = [np.exp(-np.abs(z - zd) / l) for l in (0.2, 0.5, 1.0, 2.0)]
+    imp_w = [np.exp(-kappa * l) for l in (0.2, 0.5, 1.0, 2.0)]
+    imp_left, imp_radj = low_rank(imp_env, imp_w)
+    eps_r = np.where(in_barrier, 12.9 - 2.84 * x_al, 12.9)
+    ND = np.where(layer == 5, doping_cm3 * 1e6 * L0_M ** 3, 0.0)
+    return dict(N=N, NE=NE, Nk=Nk, Nz=Nz, eps=eps, wE=wE, kappa=kappa, wk=wk, kbt=kbt_scaled(T_K), h=h, H0=H0,
+                Minv=Minv, kin=kin, Tplus=Tplus, Tminus=Tplus.conj().T.copy(), thick_nm=thick_nm, x_al=x_al,
+                Ep=Ep_eV / E0_EV, Ebasis=Ebasis, phi=phi, dz=dz, z=z, Lp=Lp,
+                kernels=dict(acoustic=("momentum_independent", acoustic, None, None, 0),
+                             alloy=("momentum_independent", alloy, None, None, 0),
+                             IFR=("separable_ifr", ifr_left, ifr_right, ifr_radial, J),
+                             LO=("low_rank", lo_left, lo_radj, None, lo_left.shape[1]),
+                             impurity=("low_rank", imp_left, imp_radj, None, imp_left.shape[1])),
+                eps_r=eps_r, wx=np.full(Nz, dz), ND=ND, lambda_P=LAMBDA_P, ND2D=float(dz * ND.sum()),
+                S=H0.copy())  # the bias is inside H0: h = S + Phi^H diag(U) Phi + kin Minv
+
+
+# Scattering strength of each mechanism of the period workload: qK is chosen
+# so that the mechanism's S< from the seed state peaks at this level (eV).
+PERIOD_LEVELS_EV = dict(acoustic=1e-3, alloy=3e-4, IFR=1e-3, LO=2e-3, impurity=1e-3)
+PERIOD_MECHANISMS = ("LO", "acoustic", "impurity", "IFR", "alloy")  # the reference's mechanism order
+HW_LO_EV = 0.036
+
+
+def calibrated_qk(contract_peak, w):
+    """qK per mechanism from contract_peak(name) = max |K[G<_seed]| at qK = 1."""
+    return {name: (PERIOD_LEVELS_EV[name] / E0_EV) / max(contract_peak(name), 1e-300) for name in PERIOD_MECHANISMS}
+
+
+def period_scba_problem(w, precision="fp64", eta_seed=0.05, alpha_mix=0.3, verify_roundoff=True):
+    """The SCBA problem of a two_well_period dict on the device: five
+    mechanisms in the device representations with calibrated qK, the seed at
+    the chemical potential that makes the number equal the sheet doping
+    N_D^2D (neutrality; seed.jl:151-197), no stored A. Returns (problem,
+    state, qK dict, mu)."""
+    import torch
+
+    from . import api, scba
+    g = api.Grid(w["eps"], w["wE"], w["wk"], w["N"])
+    g.set_precision(precision)
+    h = api.h_to_device(w["h"])
+    seed, mu = api.seed_green_production(g, h, eta_seed, w["kbt"], w["ND2D"], spectral=False)
+    kernels = {name: api.ProductionKernel(g, kind, a, b, c, rank=r)
+               for name, (kind, a, b, c, r) in w["kernels"].items()}
+
+    def peak(name):
+        S = api.production_static_contraction(kernels[name], seed.GL, 1.0)
+        v = float(torch.abs(S).max().item())
+        del S
+        return v
+
+    qK = calibrated_qk(peak, w)
+    hw = HW_LO_EV / E0_EV
+    mechs = [scba.Mechanism(name, kernels[name], qK[name], name == "LO") for name in PERIOD_MECHANISMS]
+    p = scba.ScbaProblem(g, h, api.matrix_to_device(w["Tplus"]), api.matrix_to_device(w["Tminus"]), mechs,
+                         w["ND2D"], Ep=w["Ep"], hwLO=hw, NLO=1.0 / math.expm1(hw / w["kbt"]), alpha_mix=alpha_mix,
+                         verify_roundoff=verify_roundoff)
+    s = scba.initial_state(p, seed)
+    s.green.A = None
+    return p, s, qK, mu

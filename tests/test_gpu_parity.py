@@ -265,6 +265,25 @@ def test_contractions_gpu(hip, orc, kind):
     assert err < TOL
 
 
+@pytest.mark.parametrize("N,J", [(32, 6), (24, 16), (8, 16)])
+def test_separable_large_factor_gpu(hip, orc, N, J):
+    """Separable (IFR) kernels whose J x N^2 factor exceeds 64 KiB of LDS
+    (N=32, six interfaces: 96 KiB; N=24, J=16: 144 KiB) run on 1024-thread
+    workgroups; a small one on 256 threads; all against the oracle, with beta."""
+    rng = np.random.default_rng(60 + N + J)
+    NE, Nk = 20, 4
+    w = fx.syn.gf_workload(N=N, NE=NE, Nk=Nk, seed=N)
+    left, right, radial = fx.syn.separable_factors(rng, N, Nk, J=J, scale=0.05)
+    G = _random_fields(rng, NE, Nk, N, 1, scale=1.0)[0]
+    S0 = _random_fields(rng, NE, Nk, N, 1, scale=1.0)[0]
+    ko = orc.make_kernel("separable_ifr", left, right, radial, rank=J)
+    g = _grid(hip, w["eps"], w["wE"], w["wk"], N)
+    kd = hip.ProductionKernel(g, "separable_ifr", left, right, radial, rank=J)
+    S = hip.production_static_contraction(kd, hip.to_device(G), qK=0.7, alpha=1.1, beta=0.5, S=hip.to_device(S0))
+    ref = orc.contract(ko, G, w["wk"], 0.7, alpha=1.1, beta=0.5, S=S0.copy(order="F"))
+    assert float(rel(hip.to_host(S), ref)) < TOL
+
+
 @pytest.mark.parametrize("lo_kind", ["dense", "low_rank"])
 def test_lo_contraction_gpu(hip, orc, lo_kind):
     """Both LO paths: one contraction per component then the shifts (dense),
