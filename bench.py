@@ -332,6 +332,77 @@ def scba_full_bench(iters=2, warmup=1):
     return out
 
 
+def gradient_bench(forward_tol=1e-10, adjoint_tol=1e-8, max_forward=1000, max_adjoint=250, Nz=512):
+    """The differentiation workload at N=24, NE=2048, Nk=64 (the SCBA
+    benchmark problem): the forward loop converged (relative change of J
+    below forward_tol), then the fixed-point adjoint dJ/dh converged
+    (relative change of the adjoint state below adjoint_tol; the implicit
+    adjoint needs only the converged state, so no iterate is checkpointed),
+    carried to dJ/dU on an Nz-point potential grid (project_hamiltonians'
+    chain rule with a synthetic orthonormal basis) and to six layer
+    thicknesses shaping that potential (central differences of the host
+    assembly). The forward mixes linearly with alpha = 0.6 and without the
+    per-iteration measurement (neither moves the fixed point, hence the
+    gradient); the adjoint's linear fixed point is accelerated by Anderson
+    mixing over a window of 4. Both stop at their tolerance or iteration
+    cap; the achieved changes are reported."""
+    import numpy as np
+    import torch
+
+    from qclhip import adjoint as adj
+    from qclhip import api
+
+    p, s = scba_problem()
+    p.alpha_mix = 0.6  # the fixed point does not depend on the damping
+    p.measure = False  # neither the fixed-point measurement nor the witness moves the fixed point
+    p.verify_roundoff = False
+    g = p.grid
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fwd = adj.converge(p, s, tol=forward_tol, max_iterations=max_forward)
+    g.ctx.synchronize()
+    t_fwd = time.perf_counter() - t0
+    J = s.history[-1]["Jbar"]
+    dJ_last = abs(J - s.history[-2]["Jbar"]) / abs(J)
+    log(f"  forward: {fwd} iterations in {t_fwd:.1f} s, J {J:.12e}, last relative change {dJ_last:.2e}")
+    A = adj.ScbaAdjoint(p)
+    t0 = time.perf_counter()
+    hbar, meta = A.gradient(s, tol=adjoint_tol, max_iterations=max_adjoint, acceleration="anderson", depth=4)
+    g.ctx.synchronize()
+    t_adj = time.perf_counter() - t0
+    log(f"  adjoint: {meta['iterations']} iterations in {t_adj:.1f} s, change {meta['change']:.2e}")
+    rng = np.random.default_rng(91)
+    Q, _ = np.linalg.qr(rng.normal(size=(Nz, g.Nb)) + 1j * rng.normal(size=(Nz, g.Nb)))
+    t0 = time.perf_counter()
+    gU = api.potential_gradient(hbar, Q)
+    t_U = time.perf_counter() - t0
+    z = np.linspace(0.0, 6.0, Nz)
+    L0 = np.array([0.9, 1.3, 0.8, 1.1, 0.7, 1.2])
+    edges = lambda L: np.concatenate([[0.0], np.cumsum(L)])
+    sig = lambda x: 1.0 / (1.0 + np.exp(-x / 0.05))
+    h_host = api.h_to_host(p.h)
+
+    def assemble(L):
+        e = edges(L)
+        U = sum(0.08 * (sig(z - e[j]) - sig(z - e[j + 1])) for j in (0, 2, 4))
+        V = (Q.conj().T * U[None, :]) @ Q
+        return h_host + V[None, :, :]
+
+    t0 = time.perf_counter()
+    gL = adj.thickness_gradient(hbar, assemble, L0, step=1e-4)
+    t_L = time.perf_counter() - t0
+    hb = hbar.cpu().numpy()
+    return dict(forward_iterations=fwd, forward_s=t_fwd, forward_last_rel_change=dJ_last, J=J,
+                adjoint_iterations=meta["iterations"], adjoint_s=t_adj, adjoint_change=meta["change"],
+                adjoint_ms_per_iter=1e3 * t_adj / max(meta["iterations"], 1),
+                hbar_norm=float(np.linalg.norm(hb)), dJdU_norm=float(np.linalg.norm(gU)), dJdU_s=t_U,
+                dJdU_sample=[float(x) for x in gU[::64]], dJdL=[float(x) for x in gL], dJdL_s=t_L,
+                config="N=24 NE=2048 Nk=64 SCBA benchmark problem (3 mechanisms + embedding), forward linear "
+                       "mixing alpha 0.6, adjoint Anderson-accelerated (window 4); "
+                       f"potential on Nz={Nz} points (synthetic orthonormal basis), six layer thicknesses "
+                       "shaping a barrier profile on that grid; fixed-point adjoint (no checkpoints needed)")
+
+
 def outer_bench(p, s, Nz=512, reps=5):
     """The Poisson step after an SCBA run at the same size: density of G<
     (N_E N_k N_b^2 values), the bordered Poisson LU solve (N_z + 1 = 513),
@@ -491,6 +562,8 @@ def main():
     ap.add_argument("--no-full", action="store_true", help="skip the full-iteration period workload (N=32)")
     ap.add_argument("--full-iters", type=int, default=2)
     ap.add_argument("--only-full", action="store_true", help="run only the full-iteration workload (profiling)")
+    ap.add_argument("--no-gradient", action="store_true", help="skip the converged-gradient leg")
+    ap.add_argument("--only-gradient", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -516,6 +589,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy calibration: 3 x 1 GiB device-to-device copies")
+    if args.only_gradient:
+        log("converged gradient: N=24 NE=2048 Nk=64")
+        print(json.dumps({"gradient": gradient_bench()}, default=float), flush=True)
+        return
     if args.only_full:
         log("full SCBA iteration: GaAs/Al0.15Ga0.85As period, N=32 NE=4096 Nk=128, five mechanisms")
         full = scba_full_bench(args.full_iters, 1)
@@ -547,6 +624,10 @@ def main():
         full = scba_full_bench(args.full_iters, 1)
         log(f"full iteration {full['iter_s']*1e3:.1f} ms/iter, phases "
             f"{ {k: round(v['ms'], 2) for k, v in full['phases'].items()} }, fp32 vs fp64 {full['fp32_vs_fp64']}")
+    grad = None
+    if not args.no_gradient:
+        log("converged gradient: N=24 NE=2048 Nk=64")
+        grad = gradient_bench()
     gr = None
     if not args.no_graph:
         log("graph replay of small SCBA iterations")
@@ -635,6 +716,8 @@ def main():
             line["adjoint"] = {"iter_ms": sc["adjoint_iter_ms"], "gradient_with_1_iter_ms": sc["adjoint_gradient_1iter_ms"],
                                "config": "N=24 NE=2048 Nk=64, same problem as the SCBA iteration: one damped "
                                          "fixed-point step = candidate reverse + current_green reverse + mixing"}
+    if grad is not None:
+        line["gradient"] = grad
     if full is not None:
         line["scba_full_iter_per_s"] = full["iter_per_s"]
         line["scba_full"] = dict(full, config="six-layer GaAs/Al0.15Ga0.85As period (thicknesses U[2,20] nm, seeded), "

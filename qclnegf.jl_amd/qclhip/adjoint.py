@@ -86,11 +86,22 @@ class ScbaAdjoint:
             api.embedding_adjoint(g, T, plan, f.L, GLb)
             api.embedding_adjoint(g, T, plan, f.G, GGb)
 
-    def gradient(self, s: ScbaState, tol: float = 1e-13, max_iterations: int = 5000, alpha: float | None = None):
+    def gradient(self, s: ScbaState, tol: float = 1e-13, max_iterations: int = 5000, alpha: float | None = None,
+                 acceleration: str = "linear", depth: int = 5):
         """dJ/dh (device tensor (Nb,Nb,Nk), the reference layout of h) at the
-        converged state s, plus the adjoint state and iteration count."""
+        converged state s, plus the adjoint state and iteration count.
+        acceleration: "linear" (the damped iteration at the forward's rate)
+        or "anderson" (Anderson acceleration of the same linear fixed point:
+        the forward's mixer without its self-energy guard, restarts or
+        coefficient bound)."""
         p, g = self.p, self.p.grid
         alpha = p.alpha_mix if alpha is None else alpha
+        if acceleration not in ("linear", "anderson"):
+            raise ValueError("acceleration must be 'linear' or 'anderson'")
+        mixer = None
+        if acceleration == "anderson":
+            from .anderson import AndersonMixer
+            mixer = AndersonMixer(g, depth=depth)
         fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
         green, info = api.current_green_production(g, p.h, fams, p.target, spectral=False)
         # dJ/dG (constant) and the direct dJ/dplus.
@@ -125,7 +136,12 @@ class ScbaAdjoint:
             for f in mu.all():
                 newL, newG = (plus_new if f is mu.plus else (sig[1], sig[2]))
                 pairs += [(f.R, sig[0]), (f.L, newL), (f.G, newG)]
-            num, den = api.mix_fields_change(pairs, alpha)
+            if mixer is None:
+                num, den = api.mix_fields_change(pairs, alpha)
+            else:
+                num, den = api.mix_fields_change(pairs, 0.0)  # measure only
+                mixer.mix([o for o, _ in pairs], [c for _, c in pairs], alpha, len(pairs) // 3, guard=False,
+                          linear=True)
             change = (num / max(den, 1e-300)) ** 0.5
             if it == 1 or it % 100 == 0:
                 history.append((it, change))
@@ -138,3 +154,23 @@ class ScbaAdjoint:
         _, hbar = api.current_green_adjoint(g, fams, p.target, info, green, GLb, GGb, GRbar=GRb, want_hbar=True,
                                             out=sig)
         return hbar, dict(iterations=it, change=change, info=info, history=history)
+
+
+def thickness_gradient(hbar, assemble, thick, step: float = 1e-3):
+    """dJ/dL_i of layer thicknesses by the chain rule through h:
+    dJ/dL_i = Re sum_m sum_ab conj(hbar_m[a,b]) dh_m[a,b]/dL_i, with dh/dL_i
+    from central differences of the host assembly h(L) (assemble: thickness
+    vector -> h of shape (Nk, Nb, Nb), e.g. synthetic.period_h on a fixed
+    basis). Covers what J sees through h; inputs the assembly keeps fixed
+    (basis, kernels, T+-) do not move. hbar: the device gradient of
+    ScbaAdjoint.gradient (reference layout (Nb, Nb, Nk))."""
+    hb = hbar.cpu().numpy().transpose(2, 1, 0)  # (Nk, Nb, Nb)
+    thick = np.asarray(thick, dtype=np.float64)
+    out = np.zeros(len(thick))
+    for i in range(len(thick)):
+        d = np.zeros(len(thick))
+        d[i] = step
+        dh = (assemble(thick + d) - assemble(thick - d)) / (2.0 * step)
+        out[i] = float(np.real(np.sum(np.conj(hb) * dh)))
+    return out
+

@@ -74,7 +74,7 @@ class AndersonMixer:
     def _entry(self, free, like):
         return free.pop() if free else [torch.empty_like(t) for t in like]
 
-    def append(self, state, cand) -> bool:
+    def append(self, state, cand, reset_on_increase: bool = True) -> bool:
         prior = None if not self.residuals else float(self.gram[-1, -1])
         if len(self.states) >= self.depth:
             self.free_states.append(self.states.pop(0))
@@ -86,7 +86,7 @@ class AndersonMixer:
         # sum over the fields in family and component order
         inner = [sum(float(dots[f, h]) for f in range(dots.shape[0])) for h in range(dots.shape[1])]
         norm = inner[-1]
-        reset = prior is not None and norm > (1 + 64 * _EPS) * max(prior, np.finfo(float).tiny)
+        reset = reset_on_increase and prior is not None and norm > (1 + 64 * _EPS) * max(prior, np.finfo(float).tiny)
         if reset:
             self.clear()
             inner = [norm]
@@ -101,23 +101,28 @@ class AndersonMixer:
         self.residuals.append(resid)
         return reset
 
-    def mix(self, state, cand, alpha: float, nfam: int) -> dict:
-        """Replace the state fields (in place) by the accelerated update."""
-        reset = self.append(state, cand)
+    def mix(self, state, cand, alpha: float, nfam: int, guard: bool = True, linear: bool = False) -> dict:
+        """Replace the state fields (in place) by the accelerated update.
+        guard=False skips the self-energy causality guard (for iterates that
+        are not self-energies, such as the adjoint state). linear=True is for
+        a linear fixed point (the adjoint): no restart when the residual
+        grows and no bound on the coefficients, i.e. plain Anderson
+        acceleration over the sliding window, a Krylov-type solver there."""
+        reset = self.append(state, cand, reset_on_increase=not linear)
         reason = "residual_increase" if reset else "none"
         try:
             coef = anderson_coefficients(self.gram, self.regularization)
         except ValueError:
             reason = "coefficient_solve"
             coef = np.r_[np.zeros(len(self.residuals) - 1), 1.0]
-        if np.sum(np.abs(coef)) > 10 or not np.all(np.isfinite(coef)):
+        if (not linear and np.sum(np.abs(coef)) > 10) or not np.all(np.isfinite(coef)):
             reason = "coefficient_bound"
             coef = np.r_[np.zeros(len(self.residuals) - 1), 1.0]
         depth = len(self.states)
         damping = alpha * self.damping
         api.anderson_combine(coef, damping, self.states, self.residuals, state)
         fams = [tuple(state[3 * f:3 * f + 3]) for f in range(nfam)]
-        first, code = api.anderson_guard(self.grid, fams)
+        first, code = api.anderson_guard(self.grid, fams) if guard else (0, 0)
         accepted = first == 0 and int(np.count_nonzero(coef)) > 1
         if first != 0:
             api.anderson_combine([1.0], damping, [self.states[-1]], [self.residuals[-1]], state)

@@ -177,6 +177,11 @@ def h_to_device(h, device="cuda"):
     return torch.from_numpy(np.ascontiguousarray(np.asarray(h, np.complex128).T)).to(device)
 
 
+def h_to_host(t):
+    """device h (Nb,Nb,Nk) -> numpy (Nk,Nb,Nb)."""
+    return np.ascontiguousarray(t.cpu().numpy().T)
+
+
 def matrix_to_device(T, device="cuda"):
     """Column-major Nb x Nb complex matrix on the device (torch shape (Nb,Nb) = T^T)."""
     return torch.from_numpy(np.ascontiguousarray(np.asarray(T, np.complex128).T)).to(device)

@@ -208,6 +208,57 @@ EPS0 = 8.8541878128e-12
 LAMBDA_P = QE ** 2 / (EPS0 * E0_EV * QE * L0_M)
 
 
+def period_profile(thick_nm, Ldom, Nz, x_al=0.15, Ep_eV=0.05):
+    """Band edge, inverse mass and material fractions of a six-layer period
+    (barrier first, alternating) on Nz cells of a fixed domain [0, Ldom]
+    (solver units): each cell takes its barrier fraction (the overlap of the
+    cell with the barrier layers), so the profile moves continuously with
+    the thicknesses; cells beyond the last layer are well material.
+    Returns (z, dz, V, inv_mass, barrier_fraction, doped_fraction, edges)."""
+    dz = Ldom / Nz
+    z = (np.arange(Nz) + 0.5) * dz
+    edges = np.concatenate([[0.0], np.cumsum(thick_nm) * 1e-9 / L0_M])
+    lo, hi = z - dz / 2, z + dz / 2
+    overlap = lambda a, b: np.clip(np.minimum(hi, b) - np.maximum(lo, a), 0.0, None) / dz
+    frac = sum(overlap(edges[j], edges[j + 1]) for j in (0, 2, 4))
+    doped = overlap(edges[5], edges[6])
+    dEc = 0.835 * x_al / E0_EV
+    mb = (0.067 + 0.083 * x_al) / 0.067  # barrier mass in m_ref
+    inv_mass = frac / mb + (1.0 - frac)
+    V = frac * dEc - (Ep_eV / E0_EV) * z / Ldom
+    return z, dz, V, inv_mass, frac, doped, edges
+
+
+def period_fd_hamiltonian(V, inv_mass, dz):
+    """Finite-difference BenDaniel-Duke Hamiltonian -d/dz (Cbar/m) d/dz + V
+    with hard walls (1/m at half points: the mean of the neighbours')."""
+    inv_half = 0.5 * (inv_mass[:-1] + inv_mass[1:])
+    off = -CBAR * inv_half / (dz * dz)
+    diag = V.copy()
+    diag[:-1] += CBAR * inv_half / (dz * dz)
+    diag[1:] += CBAR * inv_half / (dz * dz)
+    diag[0] += CBAR * inv_mass[0] / (dz * dz)
+    diag[-1] += CBAR * inv_mass[-1] / (dz * dz)
+    return np.diag(diag) + np.diag(off, 1) + np.diag(off, -1)
+
+
+def period_h(w, thick_nm, U=None):
+    """h_m (Nk, N, N) of the period with thicknesses thick_nm projected on the
+    FIXED basis of w (its eigenstates at w's own thicknesses):
+    h_m = Phi^H (H_fd + diag(U)) Phi + Cbar kappa_m^2 Phi^H diag(1/m) Phi,
+    Phi = phi sqrt(dz) orthonormal (project_hamiltonians, basis.jl:288-308).
+    At w's thicknesses and U = 0 this is w's h."""
+    z, dz, V, inv_mass, _, _, _ = period_profile(thick_nm, w["Lp"], w["Nz"], w["x_al"], w["Ep"] * E0_EV)
+    H = period_fd_hamiltonian(V + (0.0 if U is None else U), inv_mass, dz)
+    P = w["phi"] * math.sqrt(dz)
+    S = P.T @ H @ P
+    Mi = (P * inv_mass[:, None]).T @ P
+    h = np.zeros((w["Nk"], w["N"], w["N"]), np.complex128, order="F")
+    for m in range(w["Nk"]):
+        h[m] = S + w["kin"][m] * Mi
+    return h
+
+
 def two_well_period(N=32, NE=4096, Nk=128, Nz=512, x_al=0.15, Ep_eV=0.05, T_K=77.0, doping_cm3=3e16,
                     seed=SEED + 30):
     """A six-layer Al(x)Ga(1-x)As / GaAs period (barrier first, alternating)
@@ -216,18 +267,20 @@ def two_well_period(N=32, NE=4096, Nk=128, Nz=512, x_al=0.15, Ep_eV=0.05, T_K=77
     edge and mass directly, types.jl:11-66): conduction-band offset
     0.835 x eV, masses 0.067 (GaAs) and 0.067 + 0.083 x m0, permittivities
     12.9 and 12.9 - 2.84 x; the bias drops Ep_eV over the period; the last
-    well is doped at doping_cm3.
+    well is doped at doping_cm3. Cells straddling an interface take their
+    material fractions (period_profile).
 
     Basis: the N lowest eigenstates phi of the period's finite-difference
-    BenDaniel-Duke Hamiltonian (hard walls), so H0 = diag(E); the in-plane
-    mass matrix Minv_ab = sum_z phi_a phi_b m_ref/m(z); h_m = H0 + Cbar
-    kappa_m^2 Minv; the coupling across the period boundary
-    T+_ab = -t phi_a(z_last) phi_b(z_first) (t the boundary hopping), T- = T+^H.
+    BenDaniel-Duke Hamiltonian (hard walls); S = H0 = Phi^H H Phi (diagonal
+    to rounding), the in-plane mass matrix Minv = Phi^H diag(m_ref/m) Phi and
+    h_m = H0 + Cbar kappa_m^2 Minv (Phi = phi sqrt(dz)); the coupling across
+    the period boundary T+_ab = -t phi_a(z_last) phi_b(z_first) dz (t the
+    boundary hopping), T- = T+^H.
 
     Scattering kernels in the device representations, their shapes from the
     basis (magnitudes are calibrated later through qK):
       acoustic, momentum independent: sum_z dz V_z (x) conj(V_z), V_z[a,c] = phi_a phi_c;
-      alloy, momentum independent: the same over the barrier points;
+      alloy, momentum independent: the same weighted by the barrier fraction;
       IFR, separable: one rank per interface z_j, u_j = phi(z_j), radial
         exp(-((kappa_m - kappa_m') Lambda / 2)^2), Lambda = 8 nm;
       LO, low rank: r projectors u_i u_i^H of smooth envelopes
@@ -238,37 +291,24 @@ def two_well_period(N=32, NE=4096, Nk=128, Nz=512, x_al=0.15, Ep_eV=0.05, T_K=77
     Minv, ND2D, Lp) and the layer thicknesses."""
     rng = np.random.default_rng(seed)
     thick_nm = rng.uniform(2.0, 20.0, 6)
-    barrier = np.array([True, False, True, False, True, False])
     Lp = thick_nm.sum() * 1e-9 / L0_M
-    dz = Lp / Nz
-    z = (np.arange(Nz) + 0.5) * dz
-    edges = np.concatenate([[0.0], np.cumsum(thick_nm) * 1e-9 / L0_M])
-    layer = np.minimum(np.searchsorted(edges, z, side="right") - 1, 5)
-    in_barrier = barrier[layer]
-    dEc = 0.835 * x_al / E0_EV
-    mass = np.where(in_barrier, (0.067 + 0.083 * x_al) / 0.067, 1.0)  # in m_ref
-    V = np.where(in_barrier, dEc, 0.0) - (Ep_eV / E0_EV) * z / Lp
-    m_half = 2.0 / (1.0 / mass[:-1] + 1.0 / mass[1:])
-    off = -CBAR / (m_half * dz * dz)
-    diag = V.copy()
-    diag[:-1] += CBAR / (m_half * dz * dz)
-    diag[1:] += CBAR / (m_half * dz * dz)
-    diag[0] += CBAR / (mass[0] * dz * dz)    # hard walls
-    diag[-1] += CBAR / (mass[-1] * dz * dz)
-    H = np.diag(diag) + np.diag(off, 1) + np.diag(off, -1)
+    z, dz, V, inv_mass, frac, doped, edges = period_profile(thick_nm, Lp, Nz, x_al, Ep_eV)
+    H = period_fd_hamiltonian(V, inv_mass, dz)
     E, vec = np.linalg.eigh(H)
     phi = vec[:, :N] / math.sqrt(dz)  # continuum normalization: sum phi^2 dz = 1
     Ebasis = E[:N]
-    H0 = np.diag(Ebasis).astype(np.complex128)
-    Minv = ((phi * (1.0 / mass)[:, None]).T @ phi * dz).astype(np.complex128)
+    P = phi * math.sqrt(dz)
+    H0 = (P.T @ H @ P).astype(np.complex128)
+    Minv = ((P * inv_mass[:, None]).T @ P).astype(np.complex128)
     eps, wE = energy_grid(NE)
     kappa, wk = momentum_grid(Nk, kmax_for_energy(0.3))
     kin = CBAR * kappa ** 2
     h = np.zeros((Nk, N, N), np.complex128, order="F")
     for m in range(Nk):
         h[m] = H0 + kin[m] * Minv
-    t_edge = CBAR / (mass[0] * dz * dz)
+    t_edge = CBAR * inv_mass[0] / (dz * dz)
     Tplus = (-t_edge * dz * np.outer(phi[-1], phi[0])).astype(np.complex128)
+    in_barrier = frac
     # kernels
     Vz = np.einsum("za,zc->zac", phi, phi).reshape(Nz, N * N)  # V_z[a,c], (a,c) with a fastest
 
@@ -279,7 +319,7 @@ def two_well_period(N=32, NE=4096, Nk=128, Nz=512, x_al=0.15, Ep_eV=0.05, T_K=77
             np.complex128)
 
     acoustic = mi_block(np.full(Nz, dz))
-    alloy = mi_block(dz * in_barrier.astype(float) * x_al * (1 - x_al))
+    alloy = mi_block(dz * in_barrier * x_al * (1 - x_al))
     zint = edges[1:]  # six interfaces per period (the last is the period boundary)
     J = len(zint)
     idx = np.minimum((zint / dz).astype(int), Nz - 1)
@@ -317,8 +357,8 @@ def two_well_period(N=32, NE=4096, Nk=128, Nz=512, x_al=0.15, Ep_eV=0.05, T_K=77
     imp_env = [np.exp(-np.abs(z - zd) / l) for l in (0.2, 0.5, 1.0, 2.0)]
     imp_w = [np.exp(-kappa * l) for l in (0.2, 0.5, 1.0, 2.0)]
     imp_left, imp_radj = low_rank(imp_env, imp_w)
-    eps_r = np.where(in_barrier, 12.9 - 2.84 * x_al, 12.9)
-    ND = np.where(layer == 5, doping_cm3 * 1e6 * L0_M ** 3, 0.0)
+    eps_r = 12.9 - 2.84 * x_al * frac
+    ND = doped * doping_cm3 * 1e6 * L0_M ** 3
     return dict(N=N, NE=NE, Nk=Nk, Nz=Nz, eps=eps, wE=wE, kappa=kappa, wk=wk, kbt=kbt_scaled(T_K), h=h, H0=H0,
                 Minv=Minv, kin=kin, Tplus=Tplus, Tminus=Tplus.conj().T.copy(), thick_nm=thick_nm, x_al=x_al,
                 Ep=Ep_eV / E0_EV, Ebasis=Ebasis, phi=phi, dz=dz, z=z, Lp=Lp,
