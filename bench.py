@@ -332,7 +332,7 @@ def scba_full_bench(iters=2, warmup=1):
     return out
 
 
-def gradient_bench(forward_tol=1e-10, adjoint_tol=1e-8, max_forward=1000, max_adjoint=250, Nz=512):
+def gradient_bench(forward_tol=1e-10, adjoint_tol=1e-8, max_forward=400, max_adjoint=120, Nz=512):
     """The differentiation workload at N=24, NE=2048, Nk=64 (the SCBA
     benchmark problem): the forward loop converged (relative change of J
     below forward_tol), then the fixed-point adjoint dJ/dh converged
@@ -473,6 +473,111 @@ def graph_bench(iters=200):
     return out
 
 
+def _instance_problem(N, NE, Nk, seed):
+    """One instance of the uneven sweep: the synthetic banded Hamiltonian of
+    the GF workload, two mechanisms (momentum independent; low-rank thermal
+    LO), the bulk embedding, seed at the workload's mu, lean workspace."""
+    import numpy as np
+
+    from qclhip import api, scba, synthetic as syn
+
+    w = syn.gf_workload(N=N, NE=NE, Nk=Nk, seed=seed)
+    rng = np.random.default_rng(seed + 7)
+    g = api.Grid(w["eps"], w["wE"], w["wk"], N)
+    mechs = [scba.Mechanism("acoustic", api.ProductionKernel(g, "momentum_independent",
+                                                             syn.covariance_block(rng, N, 2, 0.04)), 1.0)]
+    lo_l, lo_r = syn.low_rank_factors(rng, N, Nk, r=4, scale=0.02)
+    mechs.append(scba.Mechanism("LO", api.ProductionKernel(g, "low_rank", lo_l, lo_r, rank=4), 1.0, True))
+    T = 0.05 * (rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N)))
+    h = api.h_to_device(w["h"])
+    kbt, hw = w["kbt"], 0.36
+    GR0, _ = api.retarded_green_production(g, h, None, 0.05)
+    q = api.seed_number_weights(g, GR0)
+    del GR0
+    target = float(np.sum(q * api.fermi(w["eps"], w["mu"], kbt)))
+    seed_green, _ = api.seed_green_production(g, h, 0.05, kbt, target, spectral=False)
+    p = scba.ScbaProblem(g, h, api.matrix_to_device(T), api.matrix_to_device(T.conj().T), mechs, target, Ep=0.5,
+                         hwLO=hw, NLO=1.0 / math.expm1(hw / kbt), alpha_mix=0.3)
+    s = scba.initial_state(p, seed_green)
+    return p, s, scba.LeanWorkspace(p)
+
+
+def batch_scba_bench(ninst=64, Nk=64, N_range=(12, 48), NE_range=(1024, 4096), replays=2, wave_GiB=200.0):
+    """The uneven sweep as batched SCBA iterations: ninst instances (N, NE
+    uniform in the ranges, Nk momenta, seeded), streamed in waves whose lean
+    state fits wave_GiB of HBM (22 fields per instance: G^R, G<, G>, four
+    families, totals, three candidate fields and the candidate S< sum). Per
+    wave: every instance's first lean iteration, the batch autotuner (launch
+    mode of the batched GF solve), one wave-iteration captured as a HIP
+    graph, `replays` timed replays, and the same instances iterated one at a
+    time (eager lean iterations) for comparison. Batched results equal the
+    one-at-a-time ones bit for bit (tests/test_gpu_batch_scba.py)."""
+    import gc
+
+    import numpy as np
+    import torch
+
+    from qclhip import api, scba, synthetic as syn
+
+    rng = np.random.default_rng(syn.SEED + 64)
+    shapes = [(int(rng.integers(N_range[0], N_range[1] + 1)), int(rng.integers(NE_range[0], NE_range[1] + 1)))
+              for _ in range(ninst)]
+    fields = 22
+    waves, cur, cur_b = [], [], 0.0
+    for i, (N, NE) in enumerate(shapes):
+        b = fields * 16.0 * N * N * NE * Nk / 2**30
+        if cur and cur_b + b > wave_GiB:
+            waves.append(cur)
+            cur, cur_b = [], 0.0
+        cur.append(i)
+        cur_b += b
+    waves.append(cur)
+    def run_wave(wv, members):
+        insts = [_instance_problem(shapes[i][0], shapes[i][1], Nk, syn.SEED + 100 + i) for i in members]
+        ps, ss, ws = [x[0] for x in insts], [x[1] for x in insts], [x[2] for x in insts]
+        for p, s, w in zip(ps, ss, ws):
+            scba.scba_iteration_lean(p, s, w)
+        lb = scba.LeanBatch(ps, ss, ws)
+        if len(members) > 1:
+            tune = lb.autotune()
+        else:  # one instance: its own launches (register prefetch), nothing to share
+            api.batch_set_mode(lb.batch, "per_instance")
+            tune = dict(mode="per_instance")
+        lb.capture()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(replays):
+            lb.replay()
+        tb = (time.perf_counter() - t0) / replays
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for p, s, w in zip(ps, ss, ws):
+            scba.scba_iteration_lean(p, s, w)
+        ts = time.perf_counter() - t0
+        return tb, ts, dict(wave=wv, instances=len(members), **tune)
+
+    t_batch = t_single = 0.0
+    solves = 0
+    tunes = []
+    t_all = time.perf_counter()
+    for wv, members in enumerate(waves):
+        tb, ts, tune = run_wave(wv, members)
+        gc.collect()
+        torch.cuda.empty_cache()
+        t_batch += tb
+        t_single += ts
+        tunes.append(tune)
+        solves += sum(shapes[i][1] * Nk for i in members)
+        log(f"  wave {wv}: {len(members)} instances, batched {tb * 1e3:.1f} ms, one at a time {ts * 1e3:.1f} ms, "
+            f"GF {tune}")
+    return dict(instances=ninst, waves=len(waves), Nk=Nk,
+                shapes=f"N in U{{{N_range[0]}..{N_range[1]}}}, NE in U{{{NE_range[0]}..{NE_range[1]}}} (seeded)",
+                batched_ms_per_iteration=t_batch * 1e3, one_by_one_ms_per_iteration=t_single * 1e3,
+                speedup=t_single / t_batch, blocks_per_iteration=solves,
+                batched_solves_per_s=solves / t_batch, autotune=tunes, total_s=time.perf_counter() - t_all,
+                per_instance_error_vs_fp64="0 (batched = one-at-a-time FP64 iteration bit for bit)")
+
+
 def batch_bench(reps=3, ninst=16, Nk=16, N_range=(12, 48), NE_range=(1024, 4096)):
     """Uneven sweep: ninst instances with N and NE uniform in the given ranges,
     Nk momenta, two families each; one batched current_green call against
@@ -564,6 +669,9 @@ def main():
     ap.add_argument("--only-full", action="store_true", help="run only the full-iteration workload (profiling)")
     ap.add_argument("--no-gradient", action="store_true", help="skip the converged-gradient leg")
     ap.add_argument("--only-gradient", action="store_true")
+    ap.add_argument("--no-batch-scba", action="store_true", help="skip the batched SCBA sweep")
+    ap.add_argument("--batch-instances", type=int, default=24)
+    ap.add_argument("--only-batch-scba", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -589,6 +697,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy calibration: 3 x 1 GiB device-to-device copies")
+    if args.only_batch_scba:
+        log(f"batched SCBA sweep: {args.batch_instances} uneven instances, Nk=64")
+        print(json.dumps({"batch_scba": batch_scba_bench(args.batch_instances)}, default=float), flush=True)
+        return
     if args.only_gradient:
         log("converged gradient: N=24 NE=2048 Nk=64")
         print(json.dumps({"gradient": gradient_bench()}, default=float), flush=True)
@@ -624,6 +736,12 @@ def main():
         full = scba_full_bench(args.full_iters, 1)
         log(f"full iteration {full['iter_s']*1e3:.1f} ms/iter, phases "
             f"{ {k: round(v['ms'], 2) for k, v in full['phases'].items()} }, fp32 vs fp64 {full['fp32_vs_fp64']}")
+    bsc = None
+    if not args.no_batch_scba:
+        log(f"batched SCBA sweep: {args.batch_instances} uneven instances, Nk=64")
+        bsc = batch_scba_bench(args.batch_instances)
+        log(f"batched SCBA: {bsc['batched_ms_per_iteration']:.1f} ms/iteration batched, "
+            f"{bsc['one_by_one_ms_per_iteration']:.1f} one at a time")
     grad = None
     if not args.no_gradient:
         log("converged gradient: N=24 NE=2048 Nk=64")
@@ -718,6 +836,8 @@ def main():
                                          "fixed-point step = candidate reverse + current_green reverse + mixing"}
     if grad is not None:
         line["gradient"] = grad
+    if bsc is not None:
+        line["batch_scba"] = bsc
     if full is not None:
         line["scba_full_iter_per_s"] = full["iter_per_s"]
         line["scba_full"] = dict(full, config="six-layer GaAs/Al0.15Ga0.85As period (thicknesses U[2,20] nm, seeded), "

@@ -327,12 +327,34 @@ int qclhip_batch_destroy(qclhip_batch* batch);
  * famR/famL/famG[i*nfam + f] (HOST arrays of device pointers, nfam <= 8, the
  * same count for all instances), target_host[i], outputs GR/GL/GG[i];
  * out_host[5i..5i+4] = (lambda, number, holes, a, c) when not NULL (the call
- * then synchronizes). */
+ * then synchronizes; in deferred-results mode the 5*ninst values go to the
+ * result buffer instead). The instance tables are staged to the device when
+ * the fields or targets differ from the previous call (a host wait); a call
+ * that repeats them launches only, so it may be captured in a graph. */
 int qclhip_batch_current_green(qclhip_ctx* ctx, qclhip_batch* batch, const double* const* h,
                                int nfam, const double* const* famR, const double* const* famL,
                                const double* const* famG, const double* target_host,
                                double* const* GR, double* const* GL, double* const* GG,
                                double* out_host);
+/* Launch granularity of the batched solve: PERSISTENT runs one launch per
+ * kernel and tile class over all instances (the tile work of small
+ * instances shares the device); PER_INSTANCE runs each instance's own
+ * Dyson and Keldysh launches (the single-instance kernels, with register
+ * prefetch) followed by the batched number functionals and paired maps.
+ * Results are bitwise the same. */
+#define QCLHIP_BATCH_PERSISTENT 0
+#define QCLHIP_BATCH_PER_INSTANCE 1
+int qclhip_batch_set_mode(qclhip_ctx* ctx, qclhip_batch* batch, int mode);
+int qclhip_batch_get_mode(const qclhip_batch* batch);
+/* Autotuner of the batch: runs qclhip_batch_current_green on these
+ * arguments in both modes (a warm call and two timed calls each, HIP
+ * events on the context's stream), keeps the faster mode, and writes the
+ * two times (persistent, per instance) in ms to ms_host (may be NULL).
+ * Synchronizes; not during capture or in deferred-results mode. */
+int qclhip_batch_autotune(qclhip_ctx* ctx, qclhip_batch* batch, const double* const* h, int nfam,
+                          const double* const* famR, const double* const* famL, const double* const* famG,
+                          const double* target_host, double* const* GR, double* const* GL, double* const* GG,
+                          double* ms_host);
 
 /* ---- A19: adjoint (vector-Jacobian products) ----------------------------
  * No counterpart in the reference, which lists the adjoint as missing

@@ -11,6 +11,7 @@
 // launch each.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "dk_common.hpp"
@@ -34,7 +35,30 @@ struct qclhip_batch {
     c64* traces = nullptr;
     std::vector<long long> trace_off;
     long long maxn = 0;
+    // The instance tables last staged to the device: a call with the same
+    // fields and targets launches without restaging (no host wait), which is
+    // what lets a captured graph contain the batched solve.
+    std::vector<DKArgs> last_args;
+    std::vector<BatchScalars> last_scal;
+    bool staged = false;
+    // QCLHIP_BATCH_PERSISTENT: one launch per kernel and tile class over all
+    // instances; QCLHIP_BATCH_PER_INSTANCE: the single-instance launches
+    // (register prefetch) per instance, then the batched scalars.
+    int mode = QCLHIP_BATCH_PERSISTENT;
 };
+
+namespace {
+// (lambda, number, holes, a, c) of every instance into the deferred buffer.
+__global__ void k_copy_coefficients_batch(const PairCoefficients* c, int n, double* dst) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        dst[5 * i + 0] = c[i].lambda;
+        dst[5 * i + 1] = c[i].number;
+        dst[5 * i + 2] = c[i].holes;
+        dst[5 * i + 3] = c[i].a;
+        dst[5 * i + 4] = c[i].c;
+    }
+}
+}  // namespace
 
 static int tile_class(int N) { return (N + 7) / 8; }
 
@@ -123,11 +147,8 @@ int qclhip_batch_current_green(qclhip_ctx* ctx, qclhip_batch* b, const double* c
         return QCLHIP_ERR_ARGUMENT;
     if (nfam < 1 || nfam > kMaxFamilies)
         return fail(ctx, QCLHIP_ERR_ARGUMENT, "the batched solve takes 1..8 self-energy families per instance");
-    // The staging buffers may still feed the previous call's copies; the
-    // wait below cannot be captured.
-    if (ctx->capturing)
-        return fail(ctx, QCLHIP_ERR_ARGUMENT, "the batched solve synchronizes and cannot run during graph capture");
-    QCLHIP_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<DKArgs> args(b->ninst);
+    std::vector<BatchScalars> scal(b->ninst);
     for (int i = 0; i < b->ninst; ++i) {
         const qclhip_grid* g = b->grids[i];
         if (!h[i] || !GR[i] || !GL[i] || !GG[i]) return fail(ctx, QCLHIP_ERR_ARGUMENT, "null field in batch");
@@ -155,7 +176,7 @@ int qclhip_batch_current_green(qclhip_ctx* ctx, qclhip_batch* b, const double* c
         a.traces = b->traces + b->trace_off[i];
         a.status = ctx->status;
         a.keldysh = 1;
-        b->h_args[i] = a;
+        args[i] = a;
         BatchScalars sc{};
         sc.traces = a.traces;
         sc.NE = g->NE;
@@ -169,26 +190,59 @@ int qclhip_batch_current_green(qclhip_ctx* ctx, qclhip_batch* b, const double* c
         sc.GL = a.GL;
         sc.GG = a.GG;
         sc.n = (long long)g->NE * g->Nk * g->Nb * g->Nb;
-        b->h_scal[i] = sc;
+        scal[i] = sc;
     }
-    QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(b->d_args, b->h_args, b->ninst * sizeof(DKArgs), hipMemcpyHostToDevice,
-                                       ctx->stream));
-    QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(b->d_scal, b->h_scal, b->ninst * sizeof(BatchScalars),
-                                       hipMemcpyHostToDevice, ctx->stream));
-    for (int k = 0; k < 2; ++k) {
-        for (int B = 1; B <= 6; ++B) {
-            if (b->cls_inst[B].empty()) continue;
-            BatchClass c;
-            c.insts = b->d_args;
-            c.inst = b->d_inst[B];
-            c.begin = b->d_begin[k][B];
-            c.n = (int)b->cls_inst[B].size();
-            int rc = launch_dk_batch(ctx, B, k == 1, c, b->cls_begin[k][B].back());
+    const bool same = b->staged && std::memcmp(args.data(), b->last_args.data(), b->ninst * sizeof(DKArgs)) == 0 &&
+                      std::memcmp(scal.data(), b->last_scal.data(), b->ninst * sizeof(BatchScalars)) == 0;
+    if (!same) {
+        // The staging buffers may still feed the previous call's copies; the
+        // wait below cannot be captured.
+        if (ctx->capturing)
+            return fail(ctx, QCLHIP_ERR_ARGUMENT,
+                        "the batched solve restages its instance tables (new fields or targets) and cannot do so "
+                        "during graph capture; run it once with these arguments before capturing");
+        QCLHIP_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        std::memcpy(b->h_args, args.data(), b->ninst * sizeof(DKArgs));
+        std::memcpy(b->h_scal, scal.data(), b->ninst * sizeof(BatchScalars));
+        QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(b->d_args, b->h_args, b->ninst * sizeof(DKArgs), hipMemcpyHostToDevice,
+                                           ctx->stream));
+        QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(b->d_scal, b->h_scal, b->ninst * sizeof(BatchScalars),
+                                           hipMemcpyHostToDevice, ctx->stream));
+        b->last_args = args;
+        b->last_scal = scal;
+        b->staged = true;
+    }
+    if (b->mode == QCLHIP_BATCH_PER_INSTANCE) {
+        for (int i = 0; i < b->ninst; ++i) {
+            int rc = launch_dyson_fp64(ctx, args[i]);
             if (rc) return rc;
+            rc = launch_keldysh_valu(ctx, args[i]);
+            if (rc) return rc;
+        }
+    } else {
+        for (int k = 0; k < 2; ++k) {
+            for (int B = 1; B <= 6; ++B) {
+                if (b->cls_inst[B].empty()) continue;
+                BatchClass c;
+                c.insts = b->d_args;
+                c.inst = b->d_inst[B];
+                c.begin = b->d_begin[k][B];
+                c.n = (int)b->cls_inst[B].size();
+                int rc = launch_dk_batch(ctx, B, k == 1, c, b->cls_begin[k][B].back());
+                if (rc) return rc;
+            }
         }
     }
     int rc = launch_pair_batch(ctx, b->d_scal, b->ninst, b->maxn);
     if (rc) return rc;
+    if (out_host && ctx->defer_dev) {
+        if (ctx->defer_next + 5 * b->ninst > ctx->defer_cap)
+            return fail(ctx, QCLHIP_ERR_ARGUMENT, "deferred-result buffer is full");
+        hipLaunchKernelGGL(k_copy_coefficients_batch, dim3(1), dim3(256), 0, ctx->stream, b->d_coef, b->ninst,
+                           ctx->defer_dev + ctx->defer_next);
+        ctx->defer_next += 5 * b->ninst;
+        return launch_check(ctx, "k_copy_coefficients_batch");
+    }
     if (out_host) {
         std::vector<PairCoefficients> pc(b->ninst);
         QCLHIP_TRY_HIP(ctx, hipMemcpyAsync(pc.data(), b->d_coef, b->ninst * sizeof(PairCoefficients),
@@ -202,6 +256,55 @@ int qclhip_batch_current_green(qclhip_ctx* ctx, qclhip_batch* b, const double* c
             out_host[5 * i + 3] = pc[i].a;
             out_host[5 * i + 4] = pc[i].c;
         }
+    }
+    return QCLHIP_OK;
+}
+
+int qclhip_batch_set_mode(qclhip_ctx* ctx, qclhip_batch* b, int mode) {
+    if (!ctx || !b) return QCLHIP_ERR_ARGUMENT;
+    if (mode != QCLHIP_BATCH_PERSISTENT && mode != QCLHIP_BATCH_PER_INSTANCE)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "mode must be QCLHIP_BATCH_PERSISTENT or QCLHIP_BATCH_PER_INSTANCE");
+    b->mode = mode;
+    return QCLHIP_OK;
+}
+
+int qclhip_batch_get_mode(const qclhip_batch* b) { return b ? b->mode : QCLHIP_ERR_ARGUMENT; }
+
+int qclhip_batch_autotune(qclhip_ctx* ctx, qclhip_batch* b, const double* const* h, int nfam,
+                          const double* const* famR, const double* const* famL, const double* const* famG,
+                          const double* target_host, double* const* GR, double* const* GL, double* const* GG,
+                          double* ms_host) {
+    if (!ctx || !b) return QCLHIP_ERR_ARGUMENT;
+    if (ctx->capturing) return fail(ctx, QCLHIP_ERR_ARGUMENT, "the autotuner times on the stream; not during capture");
+    double* saved_defer = ctx->defer_dev;
+    if (saved_defer) return fail(ctx, QCLHIP_ERR_ARGUMENT, "the autotuner does not run in deferred-results mode");
+    hipEvent_t e0, e1;
+    QCLHIP_TRY_HIP(ctx, hipEventCreate(&e0));
+    QCLHIP_TRY_HIP(ctx, hipEventCreate(&e1));
+    const int modes[2] = {QCLHIP_BATCH_PERSISTENT, QCLHIP_BATCH_PER_INSTANCE};
+    float ms[2] = {0.f, 0.f};
+    int rc = QCLHIP_OK;
+    for (int k = 0; k < 2 && rc == QCLHIP_OK; ++k) {
+        b->mode = modes[k];
+        rc = qclhip_batch_current_green(ctx, b, h, nfam, famR, famL, famG, target_host, GR, GL, GG, nullptr);
+        if (rc) break;
+        hipEventRecord(e0, ctx->stream);
+        for (int r = 0; r < 2 && rc == QCLHIP_OK; ++r)
+            rc = qclhip_batch_current_green(ctx, b, h, nfam, famR, famL, famG, target_host, GR, GL, GG, nullptr);
+        hipEventRecord(e1, ctx->stream);
+        if (rc) break;
+        rc = qclhip_ctx_synchronize(ctx);
+        if (rc) break;
+        hipEventElapsedTime(&ms[k], e0, e1);
+        ms[k] /= 2.f;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    b->mode = ms[1] < ms[0] ? QCLHIP_BATCH_PER_INSTANCE : QCLHIP_BATCH_PERSISTENT;
+    if (ms_host) {
+        ms_host[0] = ms[0];
+        ms_host[1] = ms[1];
     }
     return QCLHIP_OK;
 }

@@ -1015,10 +1015,7 @@ class Batch:
             pass
 
 
-def batch_current_green(batch: Batch, hs, families, targets, states=None):
-    """current_green_production for every instance of the batch. families[i]
-    is instance i's list of (SR, SL, SG) triples (the same count for all).
-    Returns (list of GreenState without A, list of info dicts)."""
+def _batch_args(batch: Batch, hs, families, targets, states):
     n = len(batch.grids)
     nf = len(families[0])
     if any(len(f) != nf for f in families):
@@ -1027,15 +1024,48 @@ def batch_current_green(batch: Batch, hs, families, targets, states=None):
         for fam in fams:
             for t in fam:
                 _check_field(g, t, "self-energy")
-    if states is None:
-        states = [GreenState(empty_field(g), empty_field(g), empty_field(g), None, None) for g in batch.grids]
     fam = lambda k: (_vp * (n * nf))(*[families[i][f][k].data_ptr() for i in range(n) for f in range(nf)])
     ptrs = lambda xs: (_vp * n)(*[x.data_ptr() for x in xs])
     tg = np.ascontiguousarray(targets, dtype=np.float64)
+    return (ptrs(hs), nf, fam(0), fam(1), fam(2), tg.ctypes.data_as(_dp), ptrs([s.GR for s in states]),
+            ptrs([s.GL for s in states]), ptrs([s.GG for s in states])), tg
+
+
+def batch_set_mode(batch: Batch, mode: str):
+    """'persistent' (one launch per kernel and tile class) or 'per_instance'."""
+    modes = {"persistent": 0, "per_instance": 1}
+    if mode not in modes:
+        raise _lib.ArgumentError("batch mode must be 'persistent' or 'per_instance'")
+    batch.ctx.call("qclhip_batch_set_mode", batch.handle, modes[mode])
+
+
+def batch_mode(batch: Batch) -> str:
+    return ("persistent", "per_instance")[int(batch.ctx.lib.qclhip_batch_get_mode(batch.handle))]
+
+
+def batch_autotune(batch: Batch, hs, families, targets, states) -> dict:
+    """qclhip_batch_autotune: times both launch modes on these arguments and
+    keeps the faster; returns the times and the choice."""
+    args, tg = _batch_args(batch, hs, families, targets, states)
+    ms = (ct.c_double * 2)()
+    batch.ctx.call("qclhip_batch_autotune", batch.handle, *args, ms)
+    return dict(persistent_ms=ms[0], per_instance_ms=ms[1], mode=batch_mode(batch))
+
+
+def batch_current_green(batch: Batch, hs, families, targets, states=None):
+    """current_green_production for every instance of the batch. families[i]
+    is instance i's list of (SR, SL, SG) triples (the same count for all).
+    Returns (list of GreenState without A, list of info dicts), or (states,
+    Deferred of 5 n values) in deferred-results mode."""
+    if states is None:
+        states = [GreenState(empty_field(g), empty_field(g), empty_field(g), None, None) for g in batch.grids]
+    n = len(batch.grids)
+    args, tg = _batch_args(batch, hs, families, targets, states)
     out = (ct.c_double * (5 * n))()
-    batch.ctx.call("qclhip_batch_current_green", batch.handle, ptrs(hs), nf, fam(0), fam(1), fam(2),
-                   tg.ctypes.data_as(_dp), ptrs([s.GR for s in states]), ptrs([s.GL for s in states]),
-                   ptrs([s.GG for s in states]), out)
+    slot0 = batch.ctx.result_slot() if batch.ctx.deferring else 0
+    batch.ctx.call("qclhip_batch_current_green", batch.handle, *args, out)
+    if batch.ctx.deferring:
+        return states, Deferred(slot0, 5 * n)
     infos = [dict(lam=out[5 * i], number=out[5 * i + 1], holes=out[5 * i + 2], a=out[5 * i + 3], c=out[5 * i + 4])
              for i in range(n)]
     return states, infos

@@ -258,6 +258,80 @@ class LeanWorkspace:
         self.totals_valid = False
 
 
+def _lean_tail(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, green, info, stamp=None):
+    """The lean iteration after its GF solve: the state residuals and
+    observables, then each family's candidate built, measured and mixed at
+    once (embedding plus/minus one component at a time, then every
+    mechanism), then r_K. Values are host numbers, or Deferred slots in
+    deferred-results mode."""
+    g = p.grid
+    stamp = stamp or (lambda: 0.0)
+    suite = api.residual_state(g, p.h, green, [ws.totals.parts()])
+    Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
+    pops = api.state_populations(g, green.GL, p.target)
+    t2 = stamp()
+    ratios = []
+    totals = ws.totals.parts()
+    tmp = ws.tmp
+    for c, X in enumerate((green.GR, green.GL, green.GG)):  # embedding, one component at a time
+        api.embedding_component_production(g, p.Tplus, X, p.plusE, S=tmp[0])
+        api.embedding_component_production(g, p.Tminus, X, p.minusE, S=tmp[1])
+        tot = [(totals[c], [0, 1])]
+        if c == 1:
+            tot.append((ws.csum, [0, 1], "candidate"))
+        pairs = [(s.plus.parts()[c], tmp[0]), (s.minus.parts()[c], tmp[1])]
+        ratios.append(api.mix_fields_residual(pairs, p.alpha_mix, [1, 0], totals=tot))
+    t3 = stamp()
+    roundoff = []
+    cand = Family(tmp[0], tmp[1], tmp[2])
+    for mech in p.mechanisms:
+        if mech.is_lo:
+            api.production_lo_contraction(mech.kernel, green.GL, green.GG, mech.qK, p.NLO, p.plusLO,
+                                          p.minusLO, SL=cand.L, SG=cand.G)
+        else:
+            api.production_static_contraction(mech.kernel, green.GL, mech.qK, S=cand.L)
+            api.production_static_contraction(mech.kernel, green.GG, mech.qK, S=cand.G)
+        if p.verify_roundoff:
+            _, _, r = api.retarded_self_energy_fft(p.hilbert, cand.L, cand.G, return_roundoff=True, SR=cand.R)
+            roundoff.append(r)
+        else:
+            api.retarded_self_energy_fft(p.hilbert, cand.L, cand.G, SR=cand.R)
+        fam = s.scattering[mech.name]
+        pairs = list(zip(fam.parts(), cand.parts()))
+        tot = [(totals[0], [0], "accumulate"), (totals[1], [1], "accumulate"), (totals[2], [2], "accumulate"),
+               (ws.csum, [1], "accumulate,candidate")]
+        ratios.append(api.mix_fields_residual(pairs, p.alpha_mix, [-1, -1, -1], totals=tot))
+    t4 = stamp()
+    r_K = api.residual_keldysh(g, green, [ws.csum])
+    ws.totals_valid = True
+    return dict(info=info, suite=suite, Jbar=Jbar, populations=pops, ratios=ratios, roundoff=roundoff, r_K=r_K,
+                stamps=(t2, t3, t4))
+
+
+def _lean_row(res, host=None) -> dict:
+    """The iteration row from _lean_tail's values (host reads the deferred
+    ones from the copied result buffer)."""
+    rd = lambda x: x.read(host) if isinstance(x, api.Deferred) else x
+    info = res["info"]
+    if isinstance(info, api.Deferred):
+        v = info.read(host)
+        info = dict(lam=float(v[0]), number=float(v[1]), holes=float(v[2]), a=float(v[3]), c=float(v[4]))
+    suite = res["suite"]
+    if isinstance(suite, api.Deferred):
+        suite = dict(zip(MEASURE_KEYS, (float(x) for x in suite.read(host))))
+    r_K = res["r_K"]
+    r_K = float(r_K.read(host)[2]) if isinstance(r_K, api.Deferred) else r_K
+    ratios = [float(x) for r in res["ratios"] for x in np.atleast_1d(rd(r))]
+    row = dict(lam=info["lam"], number=info["number"], holes=info["holes"], Jbar=float(rd(res["Jbar"])),
+               roundoff=max([float(rd(r)) for r in res["roundoff"]], default=0.0),
+               populations=np.asarray(rd(res["populations"]), dtype=np.float64))
+    row.update(suite)
+    row["r_K"] = r_K
+    row["r_Sigma"] = max(ratios)
+    row["r_lambda"] = abs(info["lam"] - 1.0)
+    return row
+
+
 def scba_iteration_lean(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, timings: dict | None = None):
     """One pass of the production loop (production_solver.jl:435-844) that
     never holds every candidate: the fixed-point measurement of the state
@@ -267,15 +341,14 @@ def scba_iteration_lean(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, timings
     into the state, into the totals of the next solve and into the candidate
     S< sum; r_K runs last on that sum. Every field and scalar equals the
     standard iteration's bit for bit (same kernels, same folds); the Green
-    state keeps no A (formed from G^R where the residuals need it)."""
+    state keeps no A (formed from G^R where the residuals need it). In
+    deferred-results mode it returns the _lean_tail values (for capture)."""
     g = p.grid
-    if g.ctx.deferring:
-        raise ValueError("the lean iteration runs eagerly")
     if not p.measure:
         raise ValueError("the lean iteration always measures")
 
     def stamp():
-        if timings is not None:
+        if timings is not None and not g.ctx.deferring:
             g.ctx.synchronize()
         return time.perf_counter()
 
@@ -291,59 +364,105 @@ def scba_iteration_lean(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, timings
                                                    totals=ws.totals.parts())
     s.green = green
     t1 = stamp()
-    suite = api.residual_state(g, p.h, green, [ws.totals.parts()])
-    Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
-    pops = api.state_populations(g, green.GL, p.target)
-    t2 = stamp()
-    ratios = []
-    totals = ws.totals.parts()
-    tmp = ws.tmp
-    for c, X in enumerate((green.GR, green.GL, green.GG)):  # embedding, one component at a time
-        api.embedding_component_production(g, p.Tplus, X, p.plusE, S=tmp[0])
-        api.embedding_component_production(g, p.Tminus, X, p.minusE, S=tmp[1])
-        tot = [(totals[c], [0, 1])]
-        if c == 1:
-            tot.append((ws.csum, [0, 1], "candidate"))
-        pairs = [(s.plus.parts()[c], tmp[0]), (s.minus.parts()[c], tmp[1])]
-        ratios += api.mix_fields_residual(pairs, p.alpha_mix, [1, 0], totals=tot)
-    t3 = stamp()
-    roundoff = 0.0
-    cand = Family(tmp[0], tmp[1], tmp[2])
-    for mech in p.mechanisms:
-        if mech.is_lo:
-            api.production_lo_contraction(mech.kernel, green.GL, green.GG, mech.qK, p.NLO, p.plusLO,
-                                          p.minusLO, SL=cand.L, SG=cand.G)
-        else:
-            api.production_static_contraction(mech.kernel, green.GL, mech.qK, S=cand.L)
-            api.production_static_contraction(mech.kernel, green.GG, mech.qK, S=cand.G)
-        if p.verify_roundoff:
-            _, _, r = api.retarded_self_energy_fft(p.hilbert, cand.L, cand.G, return_roundoff=True, SR=cand.R)
-            roundoff = max(roundoff, r)
-        else:
-            api.retarded_self_energy_fft(p.hilbert, cand.L, cand.G, SR=cand.R)
-        fam = s.scattering[mech.name]
-        pairs = list(zip(fam.parts(), cand.parts()))
-        tot = [(totals[0], [0], "accumulate"), (totals[1], [1], "accumulate"), (totals[2], [2], "accumulate"),
-               (ws.csum, [1], "accumulate,candidate")]
-        ratios += api.mix_fields_residual(pairs, p.alpha_mix, [-1, -1, -1], totals=tot)
-    t4 = stamp()
-    r_K = api.residual_keldysh(g, green, [ws.csum])
-    ws.totals_valid = True
+    res = _lean_tail(p, s, ws, green, info, stamp)
+    if g.ctx.deferring:
+        return res
     g.ctx.synchronize()
     t5 = time.perf_counter()
     if timings is not None:
+        t2, t3, t4 = res["stamps"]
         for k, v in (("dyson", t1 - t0), ("residuals_state_observables", t2 - t1),
                      ("candidate_embedding_mix", t3 - t2), ("candidate_scattering_mix", t4 - t3),
                      ("residual_keldysh", t5 - t4)):
             timings[k] = timings.get(k, 0.0) + v
-    row = dict(lam=info["lam"], number=info["number"], holes=info["holes"], Jbar=Jbar, roundoff=roundoff,
-               populations=pops)
-    row.update(suite)
-    row["r_K"] = r_K
-    row["r_Sigma"] = max(ratios)
-    row["r_lambda"] = abs(info["lam"] - 1.0)
+    row = _lean_row(res)
     s.history.append(row)
     return row
+
+
+class LeanBatch:
+    """A wave of independent instances (bias points or structures of uneven
+    N_b and N_E) iterated together: ONE batched GF solve over all of them
+    (qclhip_batch_current_green: one persistent launch per kernel and tile
+    class, or the per-instance launches when the batch autotuner finds them
+    faster), then each instance's lean tail. capture() records a whole
+    wave-iteration as one HIP graph (deferred results), replay() runs it.
+    No counterpart in the reference, which runs one solve_production per
+    instance (src/QCLNEGF.jl:982-983). Every instance must have run one lean
+    iteration first (its totals and Green state exist)."""
+
+    def __init__(self, problems, states, workspaces):
+        self.ps, self.ss, self.wss = list(problems), list(states), list(workspaces)
+        if not all(ws.totals_valid for ws in self.wss):
+            raise ValueError("run one lean iteration per instance before batching")
+        self.ctx = self.ps[0].grid.ctx
+        self.batch = api.Batch([p.grid for p in self.ps])
+        self.graph = None
+
+    def _gf_args(self):
+        return ([p.h for p in self.ps], [[ws.totals.parts()] for ws in self.wss], [p.target for p in self.ps],
+                [s.green for s in self.ss])
+
+    def autotune(self):
+        hs, fams, tg, greens = self._gf_args()
+        return api.batch_autotune(self.batch, hs, fams, tg, greens)
+
+    def _iterate(self):
+        hs, fams, tg, greens = self._gf_args()
+        _, infos = api.batch_current_green(self.batch, hs, fams, tg, states=greens)
+        out = []
+        for i, (p, s, ws) in enumerate(zip(self.ps, self.ss, self.wss)):
+            info = api.Deferred(infos.slot + 5 * i, 5) if isinstance(infos, api.Deferred) else infos[i]
+            out.append(_lean_tail(p, s, ws, s.green, info))
+        return out
+
+    def iterate(self):
+        """One eager batched iteration; returns the instances' rows."""
+        res = self._iterate()
+        self.ctx.synchronize()
+        rows = [_lean_row(r) for r in res]
+        for s, r in zip(self.ss, rows):
+            s.history.append(r)
+        return rows
+
+    def capture(self):
+        """Runs one eager batched iteration (stages the instance tables and
+        sizes the scratch), then captures the next as one graph."""
+        self.iterate()
+        nres = sum(5 + 5 + 1 + p.grid.Nb + 9 + 4 * len(p.mechanisms) + 5 for p in self.ps)
+        self.buffer = torch.zeros(nres + 8, dtype=torch.float64, device=self.ctx.device)
+        self.stream = torch.cuda.Stream(self.ctx.device)
+        torch.cuda.synchronize()
+        base = torch.cuda.current_stream(self.ctx.device)
+        with torch.cuda.stream(self.stream):
+            self.ctx.set_stream(self.stream)
+            try:
+                self.ctx.defer(self.buffer)
+                self.ctx.graph_begin()
+                try:
+                    self.slots = self._iterate()
+                finally:
+                    self.graph = self.ctx.graph_end()
+                    self.ctx.defer(None)
+            finally:
+                self.ctx.set_stream(base)
+        return self
+
+    def replay(self):
+        base = torch.cuda.current_stream(self.ctx.device)
+        self.stream.wait_stream(base)
+        self.ctx.set_stream(self.stream)
+        try:
+            self.graph.launch()
+            self.ctx.synchronize()
+        finally:
+            self.ctx.set_stream(base)
+        base.wait_stream(self.stream)
+        host = self.buffer.cpu().numpy()
+        rows = [_lean_row(r, host) for r in self.slots]
+        for s, r in zip(self.ss, rows):
+            s.history.append(r)
+        return rows
 
 
 def solve_scba_production(p: ScbaProblem, s: ScbaState, max_iterations: int, tol_population: float = 0.0):
