@@ -79,7 +79,9 @@ def gf_bench(steps, warmup, variants=VARIANTS, autotune=True):
     for precision, engine in variants:
         name = f"{precision}_{engine}" if precision != "fp32_products" else precision
         log(f"  variant {name}")
-        g = api.Grid(w["eps"], w["wE"], w["wk"], N).set_engine(engine).set_precision(precision)
+        g = api.Grid(w["eps"], w["wE"], w["wk"], N)
+        tiles = g.autotune() if autotune else None  # tile shapes of the FP64 vector kernels (cached per shape)
+        g.set_engine(engine).set_precision(precision)
         st, info = api.current_green_production(g, h, fams, 1.0, spectral=True)
         target = 0.9 * info["number"]  # exercises the paired map (a = 0.9)
 
@@ -114,6 +116,8 @@ def gf_bench(steps, warmup, variants=VARIANTS, autotune=True):
         g.ctx.synchronize()
         del GR2, GL2, GG2, traces
         out[name] = dict(precision=precision, engine=engine, step_s=dt, kernel_ms=e0.elapsed_time(e1) / reps,
+                         tiles=None if tiles is None else dict(dyson_waves=tiles["dyson_waves"],
+                                                               keldysh_waves=tiles["keldysh_waves"]),
                          number_check=abs(num - target) / target, J=J, solves_per_s=nblk / dt,
                          fp64_fallback_fraction=fallback)
         del st, g

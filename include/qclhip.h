@@ -122,12 +122,23 @@ int qclhip_grid_set_precision(qclhip_ctx* ctx, qclhip_grid* grid, int precision,
 #define QCLHIP_ENGINE_VECTOR 0
 #define QCLHIP_ENGINE_MATRIX 1
 int qclhip_grid_set_engine(qclhip_ctx* ctx, qclhip_grid* grid, int keldysh_engine);
-/* Autotuner: times both FP64 Keldysh engines on synthetic blocks of the
- * grid's shape (a sub-grid of at most 512 energies) and selects the faster
- * for the grid; the choice is cached in the context per (Nb, NE, Nk).
- * ms_host (may be NULL) receives the two times (vector, matrix) in ms.
+/* Autotuner: times both FP64 Keldysh engines and the tile shapes of the FP64
+ * vector kernels (energies per workgroup: 4 or 8 for the Dyson kernel, 2 or
+ * 4 for the Keldysh kernel, where the LDS of a CU holds two workgroups) on
+ * synthetic blocks of the grid's shape (a sub-grid of at most 2048 energies)
+ * and selects the fastest of each for the grid; the choice is cached in the
+ * context per (Nb, NE, Nk). ms_host (may be NULL) receives the two engine
+ * times (vector, matrix) in ms. Results do not depend on the choice.
  * Allocates temporary device memory; not during graph capture. */
 int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host);
+/* The autotuner's record for the grid's shape: ms_host[6] = (vector engine,
+ * matrix engine, Dyson W=4, Dyson W=8, Keldysh W=2, Keldysh W=4; 0 where the
+ * shape is not possible), choice_host[3] = (engine, Dyson W, Keldysh W).
+ * ArgumentError when the shape has not been tuned. */
+int qclhip_grid_tuning(qclhip_ctx* ctx, const qclhip_grid* grid, double* ms_host, int* choice_host);
+/* Sets the tile shapes by hand (0: the built-in rule); ArgumentError for a
+ * shape that is not compiled or does not fit the LDS for the basis size. */
+int qclhip_grid_set_tiles(qclhip_ctx* ctx, qclhip_grid* grid, int dyson_waves, int keldysh_waves);
 /* Blocks of the grid's last mixed-precision Dyson call that the guard sent
  * to the FP64 re-solve (0 for an FP64 grid). Synchronizes. */
 int qclhip_grid_refine_fallbacks(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t* count_host);

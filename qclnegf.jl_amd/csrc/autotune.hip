@@ -1,7 +1,7 @@
 // Autotuner of the GF-solve launch configuration. For a grid shape it times
 // the FP64 Keldysh engines (vector ALUs, matrix cores) on synthetic blocks of
 // that shape and keeps the faster one for the grid, caching the choice per
-// (Nb, NE, Nk) in the context. Timing uses a sub-grid of at most 512 energies
+// (Nb, NE, Nk) in the context. Timing uses a sub-grid of at most 2048 energies
 // at the grid's Nk and Nb: the kernels' throughput per block depends on the
 // block size and occupancy, not on the number of tiles once the device is
 // full.
@@ -58,13 +58,15 @@ int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host) {
     for (const auto& t : ctx->tune_cache)
         if (t.Nb == grid->Nb && t.NE == grid->NE && t.Nk == grid->Nk) {
             grid->keldysh_engine = t.engine;
+            grid->dyson_waves = t.dyson_waves;
+            grid->keldysh_waves = t.keldysh_waves;
             if (ms_host) {
                 ms_host[0] = t.ms_vector;
                 ms_host[1] = t.ms_matrix;
             }
             return QCLHIP_OK;
         }
-    const int NE = std::min(grid->NE, 512), Nk = grid->Nk, N = grid->Nb;
+    const int NE = std::min(grid->NE, 2048), Nk = grid->Nk, N = grid->Nb;
     const long long nblk = (long long)NE * Nk, n = nblk * N * N;
     c64 *h = nullptr, *f = nullptr;
     double* eps = nullptr;
@@ -123,17 +125,74 @@ int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host) {
         if (!rc && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms[engine], e0, e1);
         ms[engine] /= reps;
     }
+    // Tile shapes of the FP64 vector kernels: energies per workgroup of the
+    // Dyson kernel (4 or 8) and of the Keldysh kernel (2 or 4), each timed
+    // where it is compiled and fits the LDS of a CU (0 ms: not possible).
+    auto timed = [&](int which, int W, float* out) {
+        DKArgs b = a;
+        b.wd = which == 0 ? W : 0;
+        b.wk = which == 1 ? W : 0;
+        auto run = [&]() { return which == 0 ? launch_dyson_fp64(ctx, b) : launch_keldysh_valu(ctx, b); };
+        int r = run();
+        const int reps = 3;
+        hipEventRecord(e0, ctx->stream);
+        for (int k = 0; k < reps && !r; ++k) r = run();
+        hipEventRecord(e1, ctx->stream);
+        if (!r && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(out, e0, e1);
+        *out /= reps;
+        return r;
+    };
+    float msd[2] = {0.f, 0.f}, msk[2] = {0.f, 0.f};
+    const int dw[2] = {kDysonTileWaves, 8}, kw[2] = {2, 4};
+    for (int k = 0; k < 2 && !rc; ++k)
+        if (dyson_waves_possible(N, dw[k])) rc = timed(0, dw[k], &msd[k]);
+    for (int k = 0; k < 2 && !rc; ++k)
+        if (keldysh_waves_possible(N, kw[k])) rc = timed(1, kw[k], &msk[k]);
     if (!rc) rc = qclhip_ctx_synchronize(ctx);
     cleanup();
     if (rc) return rc;
     const int engine = ms[1] < ms[0] ? QCLHIP_ENGINE_MATRIX : QCLHIP_ENGINE_VECTOR;
     grid->keldysh_engine = engine;
-    ctx->tune_cache.push_back({grid->Nb, grid->NE, grid->Nk, engine, ms[0], ms[1]});
+    auto pick = [](const float* t, const int* w) {
+        if (t[0] > 0.f && t[1] > 0.f) return t[1] < t[0] ? w[1] : w[0];
+        if (t[0] <= 0.f && t[1] <= 0.f) return 0;  // neither shape exists: the built-in rule
+        return t[1] > 0.f ? w[1] : w[0];
+    };
+    grid->dyson_waves = pick(msd, dw);
+    grid->keldysh_waves = pick(msk, kw);
+    qclhip_ctx::TuneEntry te{grid->Nb, grid->NE, grid->Nk, engine, grid->dyson_waves, grid->keldysh_waves,
+                             ms[0], ms[1], {msd[0], msd[1]}, {msk[0], msk[1]}};
+    ctx->tune_cache.push_back(te);
     if (ms_host) {
         ms_host[0] = ms[0];
         ms_host[1] = ms[1];
     }
     return QCLHIP_OK;
+}
+
+int qclhip_grid_set_tiles(qclhip_ctx* ctx, qclhip_grid* grid, int dyson_waves, int keldysh_waves) {
+    if (!ctx || !grid) return QCLHIP_ERR_ARGUMENT;
+    if ((dyson_waves && !dyson_waves_possible(grid->Nb, dyson_waves)) ||
+        (keldysh_waves && !keldysh_waves_possible(grid->Nb, keldysh_waves)))
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "tile shape not available for this basis size");
+    grid->dyson_waves = dyson_waves;
+    grid->keldysh_waves = keldysh_waves;
+    return QCLHIP_OK;
+}
+
+int qclhip_grid_tuning(qclhip_ctx* ctx, const qclhip_grid* grid, double* ms_host, int* choice_host) {
+    if (!ctx || !grid || !ms_host || !choice_host) return QCLHIP_ERR_ARGUMENT;
+    for (const auto& t : ctx->tune_cache)
+        if (t.Nb == grid->Nb && t.NE == grid->NE && t.Nk == grid->Nk) {
+            const double v[6] = {t.ms_vector, t.ms_matrix, t.ms_dyson[0], t.ms_dyson[1], t.ms_keldysh[0],
+                                 t.ms_keldysh[1]};
+            for (int k = 0; k < 6; ++k) ms_host[k] = v[k];
+            choice_host[0] = t.engine;
+            choice_host[1] = t.dyson_waves;
+            choice_host[2] = t.keldysh_waves;
+            return QCLHIP_OK;
+        }
+    return fail(ctx, QCLHIP_ERR_ARGUMENT, "this grid shape has not been autotuned");
 }
 
 }  // extern "C"

@@ -163,8 +163,9 @@ struct qclhip_ctx {
     // Autotune cache: (Nb, NE, Nk) -> chosen Keldysh engine and the two
     // measured times (ms).
     struct TuneEntry {
-        int Nb, NE, Nk, engine;
+        int Nb, NE, Nk, engine, dyson_waves, keldysh_waves;
         float ms_vector, ms_matrix;
+        float ms_dyson[2], ms_keldysh[2];  // Dyson W = 4, 8; Keldysh W = 2, 4 (0: not possible)
     };
     std::vector<TuneEntry> tune_cache;
 };
@@ -183,6 +184,9 @@ struct qclhip_grid {
     double refine_tol = 1e-10;
     int keldysh_engine = QCLHIP_ENGINE_VECTOR;
     unsigned* block_flags = nullptr;  // device, one per (e,m) block (mixed precision)
+    // Energies per workgroup of the FP64 vector Dyson / Keldysh kernels (0:
+    // built-in rule), set by qclhip_grid_autotune.
+    int dyson_waves = 0, keldysh_waves = 0;
 };
 
 struct qclhip_shift {

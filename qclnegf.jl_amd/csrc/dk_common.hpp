@@ -64,6 +64,10 @@ struct DKArgs {
     // Optional: A = i (G^R - G^R^dagger), written by the FP64 vector Keldysh
     // kernel from its G^R tile.
     c64* A;
+    // Energies per workgroup of the FP64 vector Dyson and Keldysh kernels
+    // (0: the built-in rule; else one of the compiled shapes, chosen by the
+    // autotuner).
+    int wd, wk;
 };
 
 // out[r][c] (lane block) = Gs[rows] * Xs[:, cols]  or  Xs[rows] * Gs[cols]^dagger
@@ -293,6 +297,9 @@ int launch_dyson_fp64(qclhip_ctx* ctx, const DKArgs& a);
 int launch_keldysh_valu(qclhip_ctx* ctx, const DKArgs& a);
 int launch_keldysh_mfma(qclhip_ctx* ctx, const DKArgs& a, bool fp32);
 int launch_mixed_dyson(qclhip_ctx* ctx, const DKArgs& a);
+// Tile shapes the FP64 vector kernels can take for basis size N (dyson.hip).
+bool dyson_waves_possible(int N, int W);
+bool keldysh_waves_possible(int N, int W);
 // out[e, col] (+)= alpha * (S^T Y)[e, col] over ncols columns (adjoint.hip).
 int shift_transpose(qclhip_ctx* ctx, const qclhip_shift* s, long long ncols, const c64* Y, double alpha,
                     int accumulate, c64* out);

@@ -806,12 +806,8 @@ constexpr int tile_waves(size_t per_wave) {
     return per_wave * 4 <= 80 * 1024 ? 4 : (per_wave * 2 <= 80 * 1024 ? 2 : 1);
 }
 
-template <int B>
-int launch_dyson(qclhip_ctx* ctx, const DKArgs& a) {
-    // 8 energies per workgroup (128-byte segments: the tile access pattern
-    // moves 4.2 TB/s there against 2.7 TB/s with 64-byte segments) while two
-    // workgroups still fit in the LDS of a CU; else kDysonTileWaves.
-    constexpr int W = DyShape<B, 8>::per_wave * 8 <= 80 * 1024 ? 8 : kDysonTileWaves;
+template <int B, int W>
+int launch_dyson_w(qclhip_ctx* ctx, const DKArgs& a) {
     const size_t lds = DyShape<B, W>::bytes;
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
@@ -825,10 +821,25 @@ int launch_dyson(qclhip_ctx* ctx, const DKArgs& a) {
     return launch_check(ctx, "k_dyson");
 }
 
+// Largest Dyson tile: 8 energies per workgroup (128-byte segments: the tile
+// access pattern moves 4.2 TB/s there against 2.7 TB/s with 64-byte
+// segments) while two workgroups still fit in the LDS of a CU.
 template <int B>
-int launch_keldysh(qclhip_ctx* ctx, const DKArgs& a) {
+constexpr bool dyson_w8_fits() {
+    return DyShape<B, 8>::per_wave * 8 <= 80 * 1024;
+}
+
+template <int B>
+int launch_dyson(qclhip_ctx* ctx, const DKArgs& a) {
+    // a.wd: the autotuner's choice (4 or 8), else 8 where it fits
+    if (a.wd == 4) return launch_dyson_w<B, kDysonTileWaves>(ctx, a);
+    if constexpr (dyson_w8_fits<B>()) return launch_dyson_w<B, 8>(ctx, a);
+    return launch_dyson_w<B, kDysonTileWaves>(ctx, a);
+}
+
+template <int B, int W>
+int launch_keldysh_w(qclhip_ctx* ctx, const DKArgs& a) {
     constexpr size_t per_wave = 2 * (size_t)TileStride<B>::value * sizeof(c64);
-    constexpr int W = tile_waves<B>(per_wave);
     const size_t lds = per_wave * W;
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
@@ -841,6 +852,17 @@ int launch_keldysh(qclhip_ctx* ctx, const DKArgs& a) {
     if (rc) return rc;
     hipLaunchKernelGGL((k_keldysh<B, W>), dim3(grid), dim3(64 * W), lds, ctx->stream, a, tiles);
     return launch_check(ctx, "k_keldysh");
+}
+
+template <int B>
+int launch_keldysh(qclhip_ctx* ctx, const DKArgs& a) {
+    // a.wk: the autotuner's choice (2 or 4 energies per workgroup), else 4
+    // (64-byte segments) when two workgroups fit in a CU's LDS
+    constexpr int W = tile_waves<B>(2 * (size_t)TileStride<B>::value * sizeof(c64));
+    if constexpr (W >= 2) {
+        if (a.wk == 2) return launch_keldysh_w<B, 2>(ctx, a);
+    }
+    return launch_keldysh_w<B, W>(ctx, a);
 }
 
 template <int B>
@@ -880,6 +902,36 @@ int launch_dyson_fp64(qclhip_ctx* ctx, const DKArgs& a) {
     return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
 }
 
+// Whether a tile shape exists (is compiled and fits the LDS) for basis size N.
+bool dyson_waves_possible(int N, int W) {
+    if (W == kDysonTileWaves) return true;
+    if (W != 8) return false;
+    switch ((N + 7) / 8) {
+        case 1: return dyson_w8_fits<1>();
+        case 2: return dyson_w8_fits<2>();
+        case 3: return dyson_w8_fits<3>();
+        case 4: return dyson_w8_fits<4>();
+        case 5: return dyson_w8_fits<5>();
+        case 6: return dyson_w8_fits<6>();
+        default: return false;
+    }
+}
+
+bool keldysh_waves_possible(int N, int W) {
+    auto widest = [](int B) {
+        switch (B) {
+            case 1: return tile_waves<1>(2 * (size_t)TileStride<1>::value * sizeof(c64));
+            case 2: return tile_waves<2>(2 * (size_t)TileStride<2>::value * sizeof(c64));
+            case 3: return tile_waves<3>(2 * (size_t)TileStride<3>::value * sizeof(c64));
+            case 4: return tile_waves<4>(2 * (size_t)TileStride<4>::value * sizeof(c64));
+            case 5: return tile_waves<5>(2 * (size_t)TileStride<5>::value * sizeof(c64));
+            default: return tile_waves<6>(2 * (size_t)TileStride<6>::value * sizeof(c64));
+        }
+    };
+    const int w = widest((N + 7) / 8);
+    return W == w || (W == 2 && w >= 2);
+}
+
 int launch_keldysh_valu(qclhip_ctx* ctx, const DKArgs& a) {
     const int N = a.N;
     if (N <= 8) return launch_keldysh<1>(ctx, a);
@@ -903,6 +955,8 @@ double realness_limit(const qclhip_grid* grid) {
 
 int dyson_keldysh_dispatch(qclhip_ctx* ctx, const qclhip_grid* grid, DKArgs a, bool keldysh) {
     if (a.N > 48) return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
+    a.wd = grid->dyson_waves;
+    a.wk = grid->keldysh_waves;
     int rc;
     if (grid->precision != QCLHIP_PRECISION_FP64) {
         a.block_flags = grid->block_flags;  // one flag per block, owned by the grid
@@ -1065,6 +1119,8 @@ static int dyson_keldysh_impl(qclhip_ctx* ctx, const qclhip_grid* grid, long lon
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "the plane-strided layout is implemented by the FP64 vector engine");
     if (a.N > 48) return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
     a.plane = plane;
+    a.wd = grid->dyson_waves;
+    a.wk = grid->keldysh_waves;
     int rc = launch_dyson_fp64(ctx, a);
     if (rc || !keldysh) return rc;
     return launch_keldysh_valu(ctx, a);

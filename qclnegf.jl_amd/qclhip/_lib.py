@@ -143,6 +143,8 @@ _SIGNATURES = {
                               ct.POINTER(_vp), _dp],
     "qclhip_residual_keldysh": [_vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), _dp],
     "qclhip_batch_create": [_vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp)],
+    "qclhip_grid_tuning": [_vp, _vp, _dp, ct.POINTER(ct.c_int)],
+    "qclhip_grid_set_tiles": [_vp, _vp, ct.c_int, ct.c_int],
     "qclhip_batch_set_mode": [_vp, _vp, ct.c_int],
     "qclhip_batch_get_mode": [_vp],
     "qclhip_batch_autotune": [_vp, _vp, ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),

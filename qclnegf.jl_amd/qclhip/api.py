@@ -247,12 +247,24 @@ class Grid:
         return self
 
     def autotune(self):
-        """Pick the faster FP64 Keldysh engine for this grid's shape (cached
-        per shape in the context); returns the two times (ms)."""
+        """Pick the faster FP64 Keldysh engine and the tile shapes (energies
+        per workgroup) of the FP64 vector Dyson and Keldysh kernels for this
+        grid's shape (cached per shape in the context); returns the times
+        (ms) and the choices."""
         ms = (ct.c_double * 2)()
         self.ctx.call("qclhip_grid_autotune", self.handle, ms)
-        self.engine = "matrix" if ms[1] < ms[0] else "vector"
-        return dict(vector_ms=ms[0], matrix_ms=ms[1], engine=self.engine)
+        t = (ct.c_double * 6)()
+        ch = (ct.c_int * 3)()
+        self.ctx.call("qclhip_grid_tuning", self.handle, t, ch)
+        self.engine = "matrix" if ch[0] == 1 else "vector"
+        return dict(vector_ms=t[0], matrix_ms=t[1], engine=self.engine, dyson_w4_ms=t[2], dyson_w8_ms=t[3],
+                    dyson_waves=ch[1], keldysh_w2_ms=t[4], keldysh_w4_ms=t[5], keldysh_waves=ch[2])
+
+    def set_tiles(self, dyson_waves: int = 0, keldysh_waves: int = 0):
+        """Energies per workgroup of the FP64 vector Dyson (4, 8) and Keldysh
+        (2, 4) kernels; 0 restores the built-in rule."""
+        self.ctx.call("qclhip_grid_set_tiles", self.handle, dyson_waves, keldysh_waves)
+        return self
 
     def refine_fallbacks(self) -> int:
         """Blocks of the last mixed-precision Dyson call re-solved in FP64."""

@@ -195,3 +195,30 @@ def test_autotune_selects_engine_and_keeps_parity(hip, orc):
     GR, GL, GG, tr = _run_dk(hip, g, w, SR, SL, SG)
     err = float(rel(GL, GL_o))
     assert err < TOL
+
+
+@pytest.mark.parametrize("N", [8, 24, 32])
+def test_tile_shapes_bitwise_and_autotuned(hip, N):
+    """Every tile shape of the FP64 vector kernels (Dyson 4 or 8 energies
+    per workgroup, Keldysh 2 or 4) gives the same bits; the autotuner's
+    record lists the timed shapes and its choice is one of them."""
+    import torch
+    from qclhip import ArgumentError
+    w, SR, SL, SG = _dense_case(N, NE=45, Nk=3)
+    g = hip.Grid(w["eps"], w["wE"], w["wk"], N)
+    ref = _run_dk(hip, g, w, SR, SL, SG)
+    ran = 0
+    for wd in (4, 8):
+        for wk in (2, 4):
+            try:
+                g.set_tiles(wd, wk)
+            except ArgumentError:  # not compiled or over the LDS of a CU at this N
+                continue
+            out = _run_dk(hip, g, w, SR, SL, SG)
+            ran += 1
+            for a, b in zip(ref, out):
+                assert np.array_equal(np.asarray(a), np.asarray(b)), (wd, wk)
+    assert ran >= 1
+    t = hip.Grid(w["eps"], w["wE"], w["wk"], N).autotune()
+    assert t["dyson_waves"] in (4, 8) and t["keldysh_waves"] in (2, 4)
+    assert t["dyson_w4_ms"] > 0 and t["keldysh_w2_ms"] > 0
