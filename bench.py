@@ -407,6 +407,53 @@ def gradient_bench(forward_tol=1e-10, adjoint_tol=1e-8, max_forward=400, max_adj
                        "shaping a barrier profile on that grid; fixed-point adjoint (no checkpoints needed)")
 
 
+def dense_contraction_bench(N=24, NE=2048, Nk=64, reps=2):
+    """The dense kernel representation (Kflat, Nc x Nc with Nc = Nk N^2 =
+    36,864: 21.7 GB at N=24, NE=2048, Nk=64) on the hand-written FP64
+    matrix-core GEMM: S = qK K[w G] over all energies, 8 NE Nc^2 flops per
+    contraction, HIP events on the context's stream. Kflat is random
+    (built on the device); four columns of S are checked against an
+    independent product."""
+    import numpy as np
+    import torch
+
+    from qclhip import api, synthetic as syn
+
+    torch.cuda.empty_cache()
+    w = syn.gf_workload(N=N, NE=NE, Nk=Nk)
+    g = api.Grid(w["eps"], w["wE"], w["wk"], N)
+    Nc = Nk * N * N
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    Kt = torch.randn((Nc, Nc), dtype=torch.complex128, device="cuda", generator=gen) * (1.0 / Nc)
+    K = api.ProductionKernel.dense_from_device(g, Kt)
+    G = torch.randn(api.field_shape(NE, Nk, N), dtype=torch.complex128, device="cuda", generator=gen)
+    S = api.production_static_contraction(K, G, 1.0)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        api.production_static_contraction(K, G, 1.0, S=S)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    flops = 8.0 * NE * Nc * Nc
+    # check: S2[:, q] = sum_q' G2[:, q'] Kflat[q, q'] w(q'), G2 = G as NE x Nc
+    G2 = G.reshape(Nc, NE)
+    wq = torch.as_tensor(np.tile(w["wk"], N * N), device="cuda", dtype=torch.float64)
+    rng = np.random.default_rng(5)
+    err = 0.0
+    S2 = S.reshape(Nc, NE)
+    for q in rng.integers(0, Nc, 4):
+        ref = (Kt[:, q] * wq) @ G2  # sum over q' of Kflat[q, q'] w(q') G2[:, q']
+        err = max(err, float(torch.linalg.norm(S2[q] - ref) / torch.linalg.norm(ref)))
+    del Kt, K, G, S, S2, G2
+    torch.cuda.empty_cache()
+    return dict(ms=ms, tflops=flops / (ms * 1e-3) / 1e12, frac_fp64=flops / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                kflat_GB=16.0 * Nc * Nc / 1e9, sampled_rel_err=err,
+                config=f"N={N} NE={NE} Nk={Nk} (Nc={Nc}), k_zgemm_nt: 64x64 tiles, 16-deep stages, four FP64 "
+                       "16x16x4 MFMAs per complex step")
+
+
 def outer_bench(p, s, Nz=512, reps=5):
     """The Poisson step after an SCBA run at the same size: density of G<
     (N_E N_k N_b^2 values), the bordered Poisson LU solve (N_z + 1 = 513),
@@ -672,6 +719,8 @@ def main():
     ap.add_argument("--full-iters", type=int, default=2)
     ap.add_argument("--only-full", action="store_true", help="run only the full-iteration workload (profiling)")
     ap.add_argument("--no-gradient", action="store_true", help="skip the converged-gradient leg")
+    ap.add_argument("--no-dense", action="store_true", help="skip the dense-kernel contraction leg")
+    ap.add_argument("--only-dense", action="store_true")
     ap.add_argument("--only-gradient", action="store_true")
     ap.add_argument("--no-batch-scba", action="store_true", help="skip the batched SCBA sweep")
     ap.add_argument("--batch-instances", type=int, default=24)
@@ -701,6 +750,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy calibration: 3 x 1 GiB device-to-device copies")
+    if args.only_dense:
+        print(json.dumps({"dense_contraction": dense_contraction_bench()}, default=float), flush=True)
+        return
     if args.only_batch_scba:
         log(f"batched SCBA sweep: {args.batch_instances} uneven instances, Nk=64")
         print(json.dumps({"batch_scba": batch_scba_bench(args.batch_instances)}, default=float), flush=True)
@@ -740,6 +792,11 @@ def main():
         full = scba_full_bench(args.full_iters, 1)
         log(f"full iteration {full['iter_s']*1e3:.1f} ms/iter, phases "
             f"{ {k: round(v['ms'], 2) for k, v in full['phases'].items()} }, fp32 vs fp64 {full['fp32_vs_fp64']}")
+    dense = None
+    if not args.no_dense:
+        log("dense kernel contraction: N=24 NE=2048 Nk=64 (Kflat 21.7 GB)")
+        dense = dense_contraction_bench()
+        log(f"dense contraction {dense['ms']:.1f} ms, {dense['tflops']:.1f} TFLOP/s")
     bsc = None
     if not args.no_batch_scba:
         log(f"batched SCBA sweep: {args.batch_instances} uneven instances, Nk=64")
@@ -842,6 +899,8 @@ def main():
         line["gradient"] = grad
     if bsc is not None:
         line["batch_scba"] = bsc
+    if dense is not None:
+        line["dense_contraction"] = dense
     if full is not None:
         line["scba_full_iter_per_s"] = full["iter_per_s"]
         line["scba_full"] = dict(full, config="six-layer GaAs/Al0.15Ga0.85As period (thicknesses U[2,20] nm, seeded), "

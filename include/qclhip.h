@@ -20,8 +20,7 @@
  *  - Return codes: 0 ok; >0 one-based linear index e + NE*m + 1 of a singular
  *    Dyson block (Julia: SingularException); QCLHIP_ERR_ARGUMENT
  *    (ArgumentError), QCLHIP_ERR_DIMENSION (DimensionMismatch),
- *    QCLHIP_ERR_DOMAIN (DomainError), QCLHIP_ERR_RUNTIME (HIP/rocBLAS
- *    failure), QCLHIP_ERR_UNSUPPORTED (a size this build does not handle).
+ *    QCLHIP_ERR_DOMAIN (DomainError), QCLHIP_ERR_RUNTIME (HIP failure), QCLHIP_ERR_UNSUPPORTED (a size this build does not handle).
  *  - Output fields must not alias inputs unless stated (the reference rejects
  *    aliasing with ArgumentError, production_cache.jl:308-309).
  *
@@ -52,15 +51,15 @@ typedef struct qclhip_batch qclhip_batch;
 
 /* ---- context ------------------------------------------------------------
  * One context per host thread and device; enqueues on the caller's stream
- * (a hipStream_t; NULL is the default stream) and owns the rocBLAS handle,
- * device status and scratch. Replaces the per-worker workspaces of the
+ * (a hipStream_t; NULL is the default stream) and owns the device status
+ * and scratch. Replaces the per-worker workspaces of the
  * reference (production_cache.jl:63-117). */
 int qclhip_ctx_create(int device, void* stream, qclhip_ctx** out);
 int qclhip_ctx_destroy(qclhip_ctx* ctx);
 /* Waits for the stream, returns (and clears) the latched device status. */
 int qclhip_ctx_synchronize(qclhip_ctx* ctx);
 const char* qclhip_ctx_last_error(const qclhip_ctx* ctx);
-/* Moves the context (and its rocBLAS handle) to another stream. */
+/* Moves the context to another stream. */
 int qclhip_ctx_set_stream(qclhip_ctx* ctx, void* stream);
 /* Deferred results: while dev_buffer != NULL, the entries that return host
  * scalars (current_green's out_host, number_functional, boundary_flux,
@@ -262,6 +261,12 @@ int qclhip_apply_shift(qclhip_ctx* ctx, const qclhip_shift* shift, int64_t ncols
 int qclhip_kernel_create(qclhip_ctx* ctx, const qclhip_grid* grid, int kind, int rank,
                          const double* a_host, const double* b_host, const double* c_host,
                          qclhip_kernel** out);
+/* The dense representation from a DEVICE matrix Kflat (Nc x Nc,
+ * column-major complex): the weights are folded into the library's own
+ * device copy on the stream (at config sizes Kflat is tens of GB, which a
+ * host round trip would not carry). */
+int qclhip_kernel_create_dense_device(qclhip_ctx* ctx, const qclhip_grid* grid, const double* a_dev,
+                                      qclhip_kernel** out);
 int qclhip_kernel_destroy(qclhip_kernel* kernel);
 /* S <- alpha*qK*K[wk G] + beta*S; beta == 0 does not read S. */
 int qclhip_contract(qclhip_ctx* ctx, const qclhip_kernel* kernel, const double* G,

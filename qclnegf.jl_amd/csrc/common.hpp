@@ -8,7 +8,6 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <cstdint>
 #include <type_traits>
@@ -141,7 +140,6 @@ struct qclhip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool owns_stream = false;
-    rocblas_handle blas = nullptr;
     qclhip::DeviceStatus* status = nullptr;         // device
     qclhip::DeviceStatus* status_host = nullptr;    // pinned
     qclhip::PairCoefficients* coef = nullptr;       // device

@@ -3,7 +3,8 @@
 //
 // With G stored energy-fastest, G viewed as the column-major matrix
 // G2 (NE x Nc), Nc = Nk*Nb^2 (column q = m + Nk*(a + Nb*b)), every
-// representation is a short chain of plain GEMMs, which go to rocBLAS:
+// representation is a short chain of plain GEMMs, which run on the FP64
+// matrix cores (k_zgemm_nt, below):
 //   dense:      S2 = f * G2 * (K diag(w))^T                 (NE x Nc)
 //   low rank:   S2 = f * (G2 * (R^+ diag(w))^T) * left^T
 //   momentum independent: X(e,(c,d)) = sum_m' wk_m' G(e,m',c,d) (kernel),
@@ -76,29 +77,97 @@ __global__ void k_mi_broadcast(int NE, int Nk, int N2, const c64* __restrict__ Y
     }
 }
 
-inline rocblas_double_complex rc(double re, double im = 0.0) {
-    rocblas_double_complex z(re, im);
-    return z;
-}
+// ---- complex FP64 GEMM on the matrix cores -------------------------------
+// C (m x n, column-major, ld ldc) = alpha A (m x k, ld lda) B^T (B n x k, ld
+// ldb) + beta C. A 256-thread workgroup owns a 64 x 64 tile of C; its four
+// waves own 32 x 32 quadrants, i.e. 2 x 2 MFMA tiles of 16 x 16 with a real
+// and an imaginary accumulator each. K advances in stages of 16: the A and B
+// panels (64 x 16 complex each) are loaded coalesced (column-major: 64
+// consecutive rows of one column per pass) into LDS, then each wave issues,
+// per 4-deep step and MFMA tile, the four real 16x16x4 FP64 MFMAs of the
+// complex product (re += ar br - ai bi, im += ar bi + ai br). The 16x16x4
+// FP64 operand layout: lane l holds A[l%16][l/16] and B[l/16][l%16]; result
+// element v of lane l is row (l/16) + 4v, column l%16.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kGemmTile = 64;
+constexpr int kGemmStage = 16;
 
-inline const rocblas_double_complex* B_(const c64* p) { return reinterpret_cast<const rocblas_double_complex*>(p); }
-inline rocblas_double_complex* B_(c64* p) { return reinterpret_cast<rocblas_double_complex*>(p); }
-
-int blas_check(qclhip_ctx* ctx, rocblas_status st, const char* what) {
-    if (st != rocblas_status_success)
-        return fail(ctx, QCLHIP_ERR_RUNTIME, std::string(what) + ": rocBLAS status " + std::to_string((int)st));
-    return QCLHIP_OK;
+__global__ __launch_bounds__(256) void k_zgemm_nt(long long m, long long n, long long k, double alpha,
+                                                  const c64* __restrict__ A, long long lda,
+                                                  const c64* __restrict__ Bm, long long ldb, double beta,
+                                                  c64* __restrict__ C, long long ldc) {
+    __shared__ c64 As[kGemmStage][kGemmTile + 1];
+    __shared__ c64 Bs[kGemmStage][kGemmTile + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const long long i0 = (long long)blockIdx.x * kGemmTile, j0 = (long long)blockIdx.y * kGemmTile;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    f64x4 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) accR[t][u][v] = accI[t][u][v] = 0.0;
+    for (long long p0 = 0; p0 < k; p0 += kGemmStage) {
+        // 64 x 16 complex per panel: four elements per thread, 64 rows of one
+        // column per group of 64 threads (1 KiB contiguous)
+#pragma unroll
+        for (int r = 0; r < (kGemmTile * kGemmStage) / 256; ++r) {
+            const int e = tid + 256 * r;
+            const int ii = e % kGemmTile, pp = e / kGemmTile;
+            const long long gi = i0 + ii, gj = j0 + ii, gp = p0 + pp;
+            As[pp][ii] = (gi < m && gp < k) ? A[gi + lda * gp] : cmk(0.0, 0.0);
+            Bs[pp][ii] = (gj < n && gp < k) ? Bm[gj + ldb * gp] : cmk(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ps = 0; ps < kGemmStage; ps += 4) {
+            c64 a[2], b[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) a[t] = As[ps + lk][wi + 16 * t + li];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) b[u] = Bs[ps + lk][wj + 16 * u + li];
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    accR[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t].x, b[u].x, accR[t][u], 0, 0, 0);
+                    accR[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[t].y, b[u].y, accR[t][u], 0, 0, 0);
+                    accI[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t].x, b[u].y, accI[t][u], 0, 0, 0);
+                    accI[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t].y, b[u].x, accI[t][u], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long i = i0 + wi + 16 * t + lk + 4 * v, j = j0 + wj + 16 * u + li;
+                if (i < m && j < n) {
+                    c64 out = cmk(alpha * accR[t][u][v], alpha * accI[t][u][v]);
+                    if (beta != 0.0) {
+                        const c64 o = C[i + ldc * j];
+                        out = cmk(fma(beta, o.x, out.x), fma(beta, o.y, out.y));
+                    }
+                    C[i + ldc * j] = out;
+                }
+            }
 }
 
 // C (m x n, ld m) = alpha * A (m x k, ld lda) * B^T (B is n x k, ld ldb) + beta * C
 int gemm_nt(qclhip_ctx* ctx, long long m, long long n, long long k, double alpha, const c64* A, long long lda,
             const c64* Bm, long long ldb, double beta, c64* C, long long ldc) {
-    rocblas_double_complex al = rc(alpha), be = rc(beta);
-    return blas_check(ctx,
-                      rocblas_zgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)m,
-                                    (rocblas_int)n, (rocblas_int)k, &al, B_(A), (rocblas_int)lda, B_(Bm),
-                                    (rocblas_int)ldb, &be, B_(C), (rocblas_int)ldc),
-                      "rocblas_zgemm");
+    if (m <= 0 || n <= 0) return QCLHIP_OK;
+    const long long gx = (m + kGemmTile - 1) / kGemmTile, gy = (n + kGemmTile - 1) / kGemmTile;
+    if (gy > 65535) return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "contraction too wide for one launch");
+    // blockIdx.x over rows: the workgroups that share a B panel run together
+    hipLaunchKernelGGL(k_zgemm_nt, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, m, n, k, alpha, A, lda,
+                       Bm, ldb, beta, C, ldc);
+    return launch_check(ctx, "k_zgemm_nt");
 }
 
 // Streaming kernels for the thin factors (rank J or r of a few), where a
@@ -406,6 +475,46 @@ int qclhip_kernel_create(qclhip_ctx* ctx, const qclhip_grid* grid, int kind, int
     if (!ok) {
         qclhip_kernel_destroy(K);
         return fail(ctx, QCLHIP_ERR_RUNTIME, "kernel allocation failed");
+    }
+    *out = K;
+    return QCLHIP_OK;
+}
+
+namespace {
+// dst[q + Nc qp] = src[q + Nc qp] * wk[qp mod Nk] (the weights folded into the
+// dense kernel's columns).
+__global__ void k_fold_weights(long long Nc, int Nk, const double* __restrict__ wk, const qclhip::c64* __restrict__ src,
+                               qclhip::c64* __restrict__ dst) {
+    const long long n = Nc * Nc;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double w = wk[(i / Nc) % Nk];
+        const qclhip::c64 v = src[i];
+        dst[i] = qclhip::cmk(v.x * w, v.y * w);
+    }
+}
+}  // namespace
+
+int qclhip_kernel_create_dense_device(qclhip_ctx* ctx, const qclhip_grid* grid, const double* a_dev,
+                                      qclhip_kernel** out) {
+    if (!ctx || !grid || !out || !a_dev) return QCLHIP_ERR_ARGUMENT;
+    *out = nullptr;
+    if (ctx->capturing) return fail(ctx, QCLHIP_ERR_ARGUMENT, "kernel creation allocates; not during capture");
+    const long long Nk = grid->Nk, Nc = Nk * (long long)grid->Nb * grid->Nb;
+    auto* K = new qclhip_kernel();
+    K->kind = QCLHIP_KERNEL_DENSE;
+    K->grid = grid;
+    K->Nk = (int)Nk;
+    K->Nb = grid->Nb;
+    if (hipMalloc(&K->a, (size_t)Nc * Nc * sizeof(c64)) != hipSuccess) {
+        qclhip_kernel_destroy(K);
+        return fail(ctx, QCLHIP_ERR_RUNTIME, "kernel allocation failed");
+    }
+    hipLaunchKernelGGL(k_fold_weights, dim3(grid_blocks(Nc * Nc, 256, ctx->sm_count)), dim3(256), 0, ctx->stream, Nc,
+                       (int)Nk, grid->wk, (const c64*)a_dev, K->a);
+    int rc = launch_check(ctx, "k_fold_weights");
+    if (rc) {
+        qclhip_kernel_destroy(K);
+        return rc;
     }
     *out = K;
     return QCLHIP_OK;

@@ -120,10 +120,7 @@ int qclhip_ctx_create(int device, void* stream, qclhip_ctx** out) {
     // stream), so work stays ordered with the caller's own kernels and copies.
     ctx->stream = (hipStream_t)stream;
     ctx->owns_stream = false;
-    bool ok = rocblas_create_handle(&ctx->blas) == rocblas_status_success &&
-              rocblas_set_stream(ctx->blas, ctx->stream) == rocblas_status_success &&
-              rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host) == rocblas_status_success;
-    ok = ok && hipMalloc(&ctx->status, sizeof(DeviceStatus)) == hipSuccess;
+    bool ok = hipMalloc(&ctx->status, sizeof(DeviceStatus)) == hipSuccess;
     ok = ok && hipMalloc(&ctx->coef, sizeof(PairCoefficients)) == hipSuccess;
     ok = ok && hipHostMalloc(&ctx->status_host, sizeof(DeviceStatus), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc(&ctx->host_scalars, 64 * sizeof(double), hipHostMallocDefault) == hipSuccess;
@@ -142,7 +139,6 @@ int qclhip_ctx_create(int device, void* stream, qclhip_ctx** out) {
 int qclhip_ctx_destroy(qclhip_ctx* ctx) {
     if (!ctx) return QCLHIP_OK;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->workspace) hipFree(ctx->workspace);
     if (ctx->status) hipFree(ctx->status);
     if (ctx->coef) hipFree(ctx->coef);
@@ -159,8 +155,6 @@ int qclhip_ctx_set_stream(qclhip_ctx* ctx, void* stream) {
     if (!ctx) return QCLHIP_ERR_ARGUMENT;
     if (ctx->capturing) return fail(ctx, QCLHIP_ERR_ARGUMENT, "cannot change the stream during graph capture");
     ctx->stream = (hipStream_t)stream;
-    if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
-        return fail(ctx, QCLHIP_ERR_RUNTIME, "rocblas_set_stream failed");
     return QCLHIP_OK;
 }
 

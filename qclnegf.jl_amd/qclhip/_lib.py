@@ -144,6 +144,7 @@ _SIGNATURES = {
     "qclhip_residual_keldysh": [_vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), _dp],
     "qclhip_batch_create": [_vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp)],
     "qclhip_grid_tuning": [_vp, _vp, _dp, ct.POINTER(ct.c_int)],
+    "qclhip_kernel_create_dense_device": [_vp, _vp, _vp, ct.POINTER(_vp)],
     "qclhip_grid_set_tiles": [_vp, _vp, ct.c_int, ct.c_int],
     "qclhip_batch_set_mode": [_vp, _vp, ct.c_int],
     "qclhip_batch_get_mode": [_vp],

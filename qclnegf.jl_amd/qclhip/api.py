@@ -331,6 +331,19 @@ class ProductionKernel:
                       keep[0][1], keep[1][1], keep[2][1], ct.byref(h))
         self.handle = h
 
+    @classmethod
+    def dense_from_device(cls, grid: Grid, Kt) -> "ProductionKernel":
+        """The dense representation from device memory: Kt is a contiguous
+        complex128 tensor of shape (Nc, Nc) with Kt[q', q] = Kflat[q, q'] (the
+        memory of the column-major Kflat); qclhip_kernel_create_dense_device
+        folds the weights into the library's own copy."""
+        Nc = grid.Nk * grid.Nb * grid.Nb
+        if Kt.dtype != torch.complex128 or tuple(Kt.shape) != (Nc, Nc) or not Kt.is_contiguous():
+            raise _lib.DimensionMismatch("Kflat must be a contiguous complex128 (Nc, Nc) device tensor")
+        h = _vp()
+        grid.ctx.call("qclhip_kernel_create_dense_device", grid.handle, _ptr(Kt), ct.byref(h))
+        return cls(grid, "dense", _handle=h)
+
     def adjoint(self) -> "ProductionKernel":
         """(K diag(wk))^dagger in the same representation (qclhip_kernel_adjoint)."""
         h = _vp()

@@ -495,3 +495,42 @@ def test_hilbert_sign_and_scale_dawson_gpu(hip, n):
     scale = np.max(np.abs(exact))
     assert np.max(np.abs(hip.to_host(L) - exact)) < 1e-13 * scale
     assert r < 1e-12
+
+
+@pytest.mark.parametrize("N,Nk,NE", [(5, 7, 70), (3, 2, 200), (8, 3, 33)])
+def test_matrix_core_gemm_dense_odd_sizes(hip, orc, N, Nk, NE):
+    """The dense contraction on the FP64 matrix-core GEMM at sizes that leave
+    partial 64 x 64 tiles and partial 16-deep stages (Nc = Nk N^2 = 175, 18,
+    192), with beta, against the oracle; the device-built kernel equals the
+    host-built one bit for bit."""
+    import torch
+    rng = np.random.default_rng(80 + N + Nk)
+    w = fx.syn.gf_workload(N=N, NE=NE, Nk=Nk, seed=N)
+    Nc = Nk * N * N
+    F = np.asfortranarray(rng.normal(size=(Nc, Nc)) + 1j * rng.normal(size=(Nc, Nc))) * 0.1
+    G = _random_fields(rng, NE, Nk, N, 1, scale=1.0)[0]
+    S0 = _random_fields(rng, NE, Nk, N, 1, scale=1.0)[0]
+    g = _grid(hip, w["eps"], w["wE"], w["wk"], N)
+    kd = hip.ProductionKernel(g, "dense", F)
+    kdev = hip.ProductionKernel.dense_from_device(g, torch.from_numpy(np.ascontiguousarray(F.T)).cuda())
+    ko = orc.make_kernel("dense", F)
+    S = hip.production_static_contraction(kd, hip.to_device(G), qK=0.7, alpha=1.2, beta=0.4, S=hip.to_device(S0))
+    S2 = hip.production_static_contraction(kdev, hip.to_device(G), qK=0.7, alpha=1.2, beta=0.4, S=hip.to_device(S0))
+    ref = orc.contract(ko, G, w["wk"], 0.7, alpha=1.2, beta=0.4, S=S0.copy(order="F"))
+    assert float(rel(hip.to_host(S), ref)) < TOL
+    assert torch.equal(S, S2)
+
+
+def test_matrix_core_gemm_low_rank_wide(hip, orc):
+    """A low-rank kernel of rank 20 (above the streaming kernels' 16) runs
+    as two matrix-core GEMMs (NE x r x Nc, then NE x Nc x r)."""
+    rng = np.random.default_rng(91)
+    N, Nk, NE, r = 4, 5, 90, 20
+    w = fx.syn.gf_workload(N=N, NE=NE, Nk=Nk, seed=3)
+    l, rr = fx.syn.low_rank_factors(rng, N, Nk, r=r, scale=0.05)
+    G = _random_fields(rng, NE, Nk, N, 1, scale=1.0)[0]
+    g = _grid(hip, w["eps"], w["wE"], w["wk"], N)
+    kd = hip.ProductionKernel(g, "low_rank", l, rr, rank=r)
+    S = hip.production_static_contraction(kd, hip.to_device(G), qK=0.9)
+    ref = orc.contract(orc.make_kernel("low_rank", l, rr, rank=r), G, w["wk"], 0.9)
+    assert float(rel(hip.to_host(S), ref)) < TOL
