@@ -18,14 +18,23 @@ The module mirrors the production operators it replaces:
 `_embedding_component_production` (:886-922), `apply_energy_shift!`
 (production_cache.jl:300-325), `_mix_family_production!` (:1475-1491),
 `_production_boundary_flux_bar` / `_production_state_populations`
-(production_iteration.jl:174-215) and the seed helpers (seed.jl:28-128).
+(production_iteration.jl:174-215) and the seed helpers (seed.jl:28-128),
+plus what has no reference counterpart: the split residual suite, totals
+folded across calls, the batched solve over independent instances, graphs
+with deferred results, the autotuner, the allocation-free `current_green`
+and the reverse operators, which give a fixed-point adjoint of the device
+SCBA loop and a ChainRulesCore `rrule` for its converged current.
 """
 module QCLHip
 
 using LinearAlgebra
+using Libdl
 using AMDGPU
+using ChainRulesCore: ChainRulesCore, NoTangent
 
 const libqclhip = joinpath(@__DIR__, "..", "lib", "libqclhip.so")
+# entry points chosen at run time are resolved by name
+_entry(name::Symbol) = Libdl.dlsym(Libdl.dlopen(libqclhip), name)
 
 const OK = 0
 const ERR_ARGUMENT = -1
@@ -134,21 +143,33 @@ end
 
 `_current_green_production`: `families` is a vector of `(Σᴿ, Σˡ, Σᵍ)` device
 triples, embedding first, then the mechanisms in `problem.kernels.enabled`
-order. Returns `(λ, number, holes)`.
+order. Returns `(λ, number, holes)`; `current_green_coefficients!` takes the
+same arguments and returns all five results `(λ, number, holes, a, c)`.
 """
-function current_green!(g::Grid, h, families, target::Real, Gᴿ, Gˡ, Gᵍ, A, scale)
+current_green!(args...; kw...) = Tuple(current_green_coefficients!(args...; kw...)[1:3])
+
+function current_green_coefficients!(g::Grid, h, families, target::Real, Gᴿ, Gˡ, Gᵍ, A, scale;
+                        occupation_normalization::Symbol = :paired_convex, allocation_free::Bool = false)
+    occupation_normalization in (:paired_convex, :scalar_lesser) ||
+        throw(ArgumentError("occupation_normalization must be :paired_convex or :scalar_lesser"))
     n = length(families)
     fr = Ptr{Cvoid}[_ptr(f[1]) for f in families]
     fl = Ptr{Cvoid}[_ptr(f[2]) for f in families]
     fg = Ptr{Cvoid}[_ptr(f[3]) for f in families]
     out = zeros(Float64, 5)
-    rc = ccall((:qclhip_current_green, libqclhip), Cint,
+    # :scalar_lesser is production_state.jl:54-61; the allocation-free entry
+    # takes the block traces from the library's fixed per-device scratch
+    entry = occupation_normalization === :scalar_lesser ? :qclhip_current_green_scalar_lesser :
+            allocation_free ? :qclhip_current_green_noalloc : :qclhip_current_green
+    (occupation_normalization === :scalar_lesser && allocation_free) &&
+        throw(ArgumentError("the allocation-free call is the :paired_convex current_green"))
+    rc = ccall(_entry(entry), Cint,
                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
                 Cdouble, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}),
                g.ctx.handle, g.handle, _ptr(h), n, fr, fl, fg, target, _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(A),
                _ptr(scale), out)
     check(g.ctx, rc, "current_green")
-    return out[1], out[2], out[3]
+    return out
 end
 
 """Device `EnergyShiftPlan` (production_cache.jl:258-288)."""
@@ -347,6 +368,468 @@ function anderson_guard(g::Grid, families)
                         Ref{Int64}, Ref{Cint}),
                        g.ctx.handle, g.handle, n, fr, fl, fg, first, code), "anderson_guard")
     return first[], code[]
+end
+
+# ---------------------------------------------------------------------------
+# Remaining operators of the C-ABI
+
+"""`_number_functional_production` (production_operators.jl:1417-1445)."""
+function number_functional(g::Grid, G)
+    v = Ref{Float64}(0.0)
+    check(g.ctx, ccall((:qclhip_number_functional, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Float64}),
+                       g.ctx.handle, g.handle, _ptr(G), v), "number_functional")
+    return v[]
+end
+
+"""`_green_state_production` without a given G> (production_operators.jl:1387-1415):
+G> = Gᴿ - Gᴿ' + G<, A = i(Gᴿ - Gᴿ'); either output may be `nothing`."""
+function green_state!(Gᵍ, A, g::Grid, Gᴿ, Gˡ)
+    check(g.ctx, ccall((:qclhip_green_state, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                       g.ctx.handle, g.handle, _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(A)), "green_state")
+    return Gᵍ, A
+end
+
+_fermi(ε, μ, kBT) = (x = (ε - μ) / kBT; x > 40 ? 0.0 : (x < -40 ? 1.0 : 1 / (exp(x) + 1)))
+
+"""Bisection of Σₑ qₑ f(εₑ; μ) = target on [ε₁ - 50kBT, ε_N + 50kBT]
+(seed.jl:175-182), down to adjacent floating-point numbers."""
+function seed_chemical_potential(ε::Vector{Float64}, q::Vector{Float64}, kBT::Real, target::Real)
+    number(μ) = sum(q[e] * _fermi(ε[e], μ, kBT) for e in eachindex(ε)) - target
+    lo, hi = ε[1] - 50kBT, ε[end] + 50kBT
+    flo, fhi = number(lo), number(hi)
+    flo * fhi <= 0 || throw(ArgumentError("seed chemical-potential root is not bracketed"))
+    flo == 0 && return lo
+    fhi == 0 && return hi
+    for _ = 1:4000
+        mid = lo + (hi - lo) / 2
+        (mid <= lo || mid >= hi) && break
+        fm = number(mid)
+        fm == 0 && return mid
+        if (fm < 0) == (flo < 0)
+            lo, flo = mid, fm
+        else
+            hi, fhi = mid, fm
+        end
+    end
+    return abs(flo) <= abs(fhi) ? lo : hi
+end
+
+"""`_seed_green_production` (seed.jl:151-197): Dyson with η and Σ = 0, the
+spectral number weights on the device, the chemical potential by bisection
+on [ε₁ - 50kBT, ε_N + 50kBT] on the host, G< = -f (Gᴿ - Gᴿ') on the device.
+Returns (Gᴿ, G<, G>, μ)."""
+function seed_green(g::Grid, h, ε::Vector{Float64}, η::Real, kBT::Real, target::Real)
+    Gᴿ, _ = retarded_green(g, h, nothing; η = η)
+    q = zeros(Float64, g.NE)
+    check(g.ctx, ccall((:qclhip_seed_weights, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}),
+                       g.ctx.handle, g.handle, _ptr(Gᴿ), q), "seed_weights")
+    μ = seed_chemical_potential(ε, q, kBT, target)
+    Gˡ = similar(Gᴿ)
+    check(g.ctx, ccall((:qclhip_seed_lesser, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Ptr{Cvoid}),
+                       g.ctx.handle, g.handle, _ptr(Gᴿ), μ, kBT, _ptr(Gˡ)), "seed_lesser")
+    Gᵍ = similar(Gᴿ)
+    green_state!(Gᵍ, nothing, g, Gᴿ, Gˡ)
+    synchronize(g.ctx)
+    return Gᴿ, Gˡ, Gᵍ, μ
+end
+
+"""GF-solve precision: `:fp64`, `:fp32_refine` or `:fp32_products` (see qclhip.h)."""
+function set_precision!(g::Grid, precision::Symbol; refine_steps::Integer = 2, refine_tol::Real = 0.0)
+    code = Dict(:fp64 => 0, :fp32_refine => 1, :fp32_products => 2)
+    haskey(code, precision) || throw(ArgumentError("unknown precision $precision"))
+    check(g.ctx, ccall((:qclhip_grid_set_precision, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cdouble),
+                       g.ctx.handle, g.handle, code[precision], refine_steps, refine_tol), "set_precision")
+    return g
+end
+
+set_engine!(g::Grid, engine::Symbol) = (check(g.ctx, ccall((:qclhip_grid_set_engine, libqclhip), Cint,
+    (Ptr{Cvoid}, Ptr{Cvoid}, Cint), g.ctx.handle, g.handle, engine === :matrix ? 1 : 0), "set_engine"); g)
+
+set_tiles!(g::Grid, dyson_waves::Integer, keldysh_waves::Integer) = (check(g.ctx, ccall((:qclhip_grid_set_tiles,
+    libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint), g.ctx.handle, g.handle, dyson_waves, keldysh_waves),
+    "set_tiles"); g)
+
+"""The autotuner (engine and tile shapes, cached per shape in the context);
+returns its record."""
+function autotune!(g::Grid)
+    ms = zeros(Float64, 2)
+    check(g.ctx, ccall((:qclhip_grid_autotune, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}),
+                       g.ctx.handle, g.handle, ms), "autotune")
+    t = zeros(Float64, 6)
+    ch = zeros(Cint, 3)
+    check(g.ctx, ccall((:qclhip_grid_tuning, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Cint}),
+                       g.ctx.handle, g.handle, t, ch), "grid_tuning")
+    return (engine = ch[1] == 1 ? :matrix : :vector, dyson_waves = Int(ch[2]), keldysh_waves = Int(ch[3]), ms = t)
+end
+
+# -- deferred results and graphs ---------------------------------------------
+defer_results!(ctx::Context, buf::Union{Nothing,ROCArray{Float64,1}}) =
+    check(ctx, ccall((:qclhip_ctx_defer_results, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint), ctx.handle,
+                     _ptr(buf), buf === nothing ? 0 : length(buf)), "defer_results")
+
+mutable struct Graph
+    ctx::Context
+    handle::Ptr{Cvoid}
+end
+
+graph_begin(ctx::Context) = check(ctx, ccall((:qclhip_graph_begin, libqclhip), Cint, (Ptr{Cvoid},), ctx.handle),
+                                  "graph_begin")
+
+function graph_end(ctx::Context)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ctx, ccall((:qclhip_graph_end, libqclhip), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), ctx.handle, h), "graph_end")
+    gr = Graph(ctx, h[])
+    finalizer(x -> ccall((:qclhip_graph_destroy, libqclhip), Cint, (Ptr{Cvoid},), x.handle), gr)
+    return gr
+end
+
+launch(gr::Graph) = check(gr.ctx, ccall((:qclhip_graph_launch, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}),
+                                        gr.ctx.handle, gr.handle), "graph_launch")
+
+# -- batched GF solve over independent instances ------------------------------
+mutable struct Batch
+    ctx::Context
+    handle::Ptr{Cvoid}
+    grids::Vector{Grid}
+end
+
+function Batch(grids::Vector{Grid})
+    ctx = first(grids).ctx
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    pg = Ptr{Cvoid}[g.handle for g in grids]
+    check(ctx, ccall((:qclhip_batch_create, libqclhip), Cint, (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ref{Ptr{Cvoid}}),
+                     ctx.handle, length(grids), pg, h), "batch_create")
+    b = Batch(ctx, h[], grids)
+    finalizer(x -> ccall((:qclhip_batch_destroy, libqclhip), Cint, (Ptr{Cvoid},), x.handle), b)
+    return b
+end
+
+"""current_green for every instance: `hs[i]`, `families[i]` (same count for
+all), `targets[i]`, outputs `Gᴿs[i]`, `Gˡs[i]`, `Gᵍs[i]`; returns a 5 × n
+matrix of (λ, number, holes, a, c)."""
+function batch_current_green!(b::Batch, hs, families, targets::Vector{Float64}, Gᴿs, Gˡs, Gᵍs)
+    n, nf = length(b.grids), length(first(families))
+    fam(k) = Ptr{Cvoid}[_ptr(families[i][f][k]) for i = 1:n for f = 1:nf]
+    out = zeros(Float64, 5, n)
+    check(b.ctx, ccall((:qclhip_batch_current_green, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
+                        Ptr{Ptr{Cvoid}}, Ptr{Float64}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Float64}),
+                       b.ctx.handle, b.handle, Ptr{Cvoid}[_ptr(h) for h in hs], nf, fam(1), fam(2), fam(3), targets,
+                       Ptr{Cvoid}[_ptr(x) for x in Gᴿs], Ptr{Cvoid}[_ptr(x) for x in Gˡs],
+                       Ptr{Cvoid}[_ptr(x) for x in Gᵍs], out), "batch_current_green")
+    return out
+end
+
+"""Launch granularity of the batched solve: `:persistent` or `:per_instance`."""
+set_mode!(b::Batch, mode::Symbol) = check(b.ctx, ccall((:qclhip_batch_set_mode, libqclhip), Cint,
+    (Ptr{Cvoid}, Ptr{Cvoid}, Cint), b.ctx.handle, b.handle, mode === :per_instance ? 1 : 0), "batch_set_mode")
+
+# -- split residual suite -----------------------------------------------------
+"""Every residual but r_K, before the candidates exist; `A` may be `nothing`."""
+function residual_state(g::Grid, h, Gᴿ, Gˡ, Gᵍ, A, families)
+    n = length(families)
+    out = zeros(Float64, 5)
+    check(g.ctx, ccall((:qclhip_residual_state, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
+                        Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Float64}),
+                       g.ctx.handle, g.handle, _ptr(h), _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(A), n,
+                       Ptr{Cvoid}[_ptr(f[1]) for f in families], Ptr{Cvoid}[_ptr(f[2]) for f in families],
+                       Ptr{Cvoid}[_ptr(f[3]) for f in families], out), "residual_state")
+    return (r_D = out[1], r_A = out[2], r_PSD = out[4], r_caus = out[5])
+end
+
+function residual_keldysh(g::Grid, Gᴿ, Gˡ, candidate_lesser)
+    out = zeros(Float64, 5)
+    check(g.ctx, ccall((:qclhip_residual_keldysh, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Float64}),
+                       g.ctx.handle, g.handle, _ptr(Gᴿ), _ptr(Gˡ), length(candidate_lesser),
+                       Ptr{Cvoid}[_ptr(c) for c in candidate_lesser], out), "residual_keldysh")
+    return out[3]
+end
+
+"""`mix_fields_residual!` with totals folded across calls: `totals =
+[(T, members, flags), ...]`, flags a combination of 1 (accumulate: continue
+the fold from T) and 2 (sum the members' candidate values)."""
+function mix_fields_residual_accumulate!(ctx::Context, olds, cands, α::Real, partner::Vector{Cint}, totals)
+    nterms = length(olds) + count(p -> partner[p] > p - 1, eachindex(partner))
+    out = zeros(Float64, nterms)
+    counts = Cint[length(m) for (_, m, _) in totals]
+    members = Cint[p for (_, m, _) in totals for p in m]
+    flags = Cint[f for (_, _, f) in totals]
+    check(ctx, ccall((:qclhip_mix_fields_residual_accumulate, libqclhip), Cint,
+                     (Ptr{Cvoid}, Int64, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cdouble, Ptr{Cint}, Cint, Ptr{Cint},
+                      Ptr{Cint}, Ptr{Cint}, Ptr{Ptr{Cvoid}}, Ptr{Float64}),
+                     ctx.handle, length(first(olds)), length(olds), Ptr{Cvoid}[_ptr(o) for o in olds],
+                     Ptr{Cvoid}[_ptr(c) for c in cands], α, partner, length(totals), counts, members, flags,
+                     Ptr{Cvoid}[_ptr(T) for (T, _, _) in totals], out), "mix_fields_residual_accumulate")
+    return out
+end
+
+sum_fields!(out, ctx::Context, xs) = check(ctx, ccall((:qclhip_sum_fields, libqclhip), Cint,
+    (Ptr{Cvoid}, Int64, Cint, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}), ctx.handle, length(out), length(xs),
+    Ptr{Cvoid}[_ptr(x) for x in xs], _ptr(out)), "sum_fields")
+
+fill_zero!(X, ctx::Context) = check(ctx, ccall((:qclhip_fill_zero, libqclhip), Cint, (Ptr{Cvoid}, Int64, Ptr{Cvoid}),
+                                               ctx.handle, length(X), _ptr(X)), "fill_zero")
+
+# ---------------------------------------------------------------------------
+# Adjoint (no reference counterpart; algorithm_registry.jl:226-249 lists it
+# as missing). Gradient convention, as ChainRulesCore's for a real function
+# of complex arguments: Z̄ = ∂J/∂Re Z + i ∂J/∂Im Z.
+
+"""Reverse of `current_green!`: the gradients of the total self-energy from
+those of (Gᴿ, G<, G>) and, when `h̄ !== nothing`, of h."""
+function current_green_adjoint!(Σ̄ᴿ, Σ̄ˡ, Σ̄ᵍ, h̄, g::Grid, families, target::Real, coef::Vector{Float64}, Gᴿ, Gˡ, Gᵍ,
+                                Ḡᴿ, Ḡˡ, Ḡᵍ)
+    n = length(families)
+    check(g.ctx, ccall((:qclhip_current_green_adjoint, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cdouble, Ptr{Float64},
+                        Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                        Ptr{Cvoid}, Ptr{Cvoid}),
+                       g.ctx.handle, g.handle, n, Ptr{Cvoid}[_ptr(f[2]) for f in families],
+                       Ptr{Cvoid}[_ptr(f[3]) for f in families], target, coef, _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ),
+                       _ptr(Ḡᴿ), _ptr(Ḡˡ), _ptr(Ḡᵍ), _ptr(Σ̄ᴿ), _ptr(Σ̄ˡ), _ptr(Σ̄ᵍ), _ptr(h̄)), "current_green_adjoint")
+    return h̄
+end
+
+boundary_flux_adjoint!(g::Grid, J̄::Real, plusˡ, plusᵍ, Gˡ, Gᵍ, p̄ˡ, p̄ᵍ, Ḡˡ, Ḡᵍ) =
+    check(g.ctx, ccall((:qclhip_boundary_flux_adjoint, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                        Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                       g.ctx.handle, g.handle, J̄, _ptr(plusˡ), _ptr(plusᵍ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(p̄ˡ), _ptr(p̄ᵍ),
+                       _ptr(Ḡˡ), _ptr(Ḡᵍ)), "boundary_flux_adjoint")
+
+shift_transpose!(X, g::Grid, plan::ShiftPlan, Y, α::Real; accumulate::Bool = false) =
+    check(g.ctx, ccall((:qclhip_shift_transpose, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint, Ptr{Cvoid}),
+                       g.ctx.handle, g.handle, plan.handle, _ptr(Y), α, accumulate, _ptr(X)), "shift_transpose")
+
+"""(K diag(wᵏ))† in the same representation."""
+function kernel_adjoint(op::HipProductionKernel)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(op.grid.ctx, ccall((:qclhip_kernel_adjoint, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                             op.grid.ctx.handle, op.handle, h), "kernel_adjoint")
+    k = HipProductionKernel(op.grid, h[])
+    finalizer(x -> ccall((:qclhip_kernel_destroy, libqclhip), Cint, (Ptr{Cvoid},), x.handle), k)
+    return k
+end
+
+retarded_self_energy_adjoint!(Σ̄ˡ, Σ̄ᵍ, plan::HilbertPlan, Σ̄ᴿ) =
+    check(plan.grid.ctx, ccall((:qclhip_retarded_self_energy_adjoint, libqclhip), Cint,
+                               (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                               plan.grid.ctx.handle, plan.handle, _ptr(Σ̄ᴿ), _ptr(Σ̄ˡ), _ptr(Σ̄ᵍ)),
+          "retarded_self_energy_adjoint")
+
+antihermitian_part!(X, g::Grid) = check(g.ctx, ccall((:qclhip_antihermitian_part, libqclhip), Cint,
+    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}), g.ctx.handle, g.handle, _ptr(X)), "antihermitian_part")
+
+embedding_adjoint!(X̄, g::Grid, T, plan::ShiftPlan, Σ̄) =
+    check(g.ctx, ccall((:qclhip_embedding_adjoint, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                       g.ctx.handle, g.handle, _ptr(T), plan.handle, _ptr(Σ̄), _ptr(X̄)), "embedding_adjoint")
+
+"""Mix every (old, new) pair and return (Σ‖new - old‖², Σ‖new‖²); α = 0 measures only."""
+function mix_fields_change!(ctx::Context, olds, news, α::Real)
+    out = zeros(Float64, 2)
+    check(ctx, ccall((:qclhip_mix_fields_change, libqclhip), Cint,
+                     (Ptr{Cvoid}, Int64, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cdouble, Ptr{Float64}),
+                     ctx.handle, length(first(olds)), length(olds), Ptr{Cvoid}[_ptr(o) for o in olds],
+                     Ptr{Cvoid}[_ptr(c) for c in news], α, out), "mix_fields_change")
+    return out[1], out[2]
+end
+
+# ---------------------------------------------------------------------------
+# The production SCBA loop on the device and its fixed-point adjoint
+# (production_solver.jl:435-844 with linear mixing), for the rrule below.
+
+struct HipMechanism
+    kernel::HipProductionKernel
+    adjoint::HipProductionKernel
+    qᴷ::Float64
+    is_lo::Bool
+end
+
+HipMechanism(k::HipProductionKernel, qᴷ::Real, is_lo::Bool) = HipMechanism(k, kernel_adjoint(k), qᴷ, is_lo)
+
+"""Everything the loop needs on the device: grid, T± (device Nb × Nb),
+mechanisms in `problem.kernels.enabled` order, shift plans for ±Eᵖ and
+±ħωᴸᴼ, the Hilbert plan, target N_D²ᴰ, thermal Nᴸᴼ and the mixing α."""
+struct HipScbaProblem
+    grid::Grid
+    T₊::ROCArray{ComplexF64,2}
+    T₋::ROCArray{ComplexF64,2}
+    mechanisms::Vector{HipMechanism}
+    plusE::ShiftPlan
+    minusE::ShiftPlan
+    plusLO::ShiftPlan
+    minusLO::ShiftPlan
+    hilbert::HilbertPlan
+    target::Float64
+    Nᴸᴼ::Float64
+    α::Float64
+end
+
+_field(g::Grid) = ROCArray{ComplexF64}(undef, g.NE, g.Nk, g.Nb, g.Nb)
+_zfield(g::Grid) = (X = _field(g); fill_zero!(X, g.ctx); X)
+
+"""The SCBA state: one (Σᴿ, Σ<, Σ>) triple per family (plus, minus, then
+the mechanisms) and the Green functions of the last solve."""
+mutable struct HipScbaState
+    families::Vector{NTuple{3,ROCArray{ComplexF64,4}}}
+    Gᴿ::ROCArray{ComplexF64,4}
+    Gˡ::ROCArray{ComplexF64,4}
+    Gᵍ::ROCArray{ComplexF64,4}
+    coef::Vector{Float64}
+end
+
+"""Fresh candidates of every family from (Gᴿ, G<, G>)
+(_scattering_candidate_production, _embedding_self_energy_production)."""
+function candidates!(out, p::HipScbaProblem, Gᴿ, Gˡ, Gᵍ)
+    g = p.grid
+    for (X, c) in ((Gᴿ, 1), (Gˡ, 2), (Gᵍ, 3))
+        embedding_component!(out[1][c], g, p.T₊, X, p.plusE)
+        embedding_component!(out[2][c], g, p.T₋, X, p.minusE)
+    end
+    for (i, m) in enumerate(p.mechanisms)
+        R, L, G = out[2+i]
+        if m.is_lo
+            lo_contraction!(L, G, m.kernel, Gˡ, Gᵍ, m.qᴷ, p.Nᴸᴼ, p.plusLO, p.minusLO)
+        else
+            production_static_contraction!(L, m.kernel, Gˡ, m.qᴷ)
+            production_static_contraction!(G, m.kernel, Gᵍ, m.qᴷ)
+        end
+        retarded_self_energy!(R, p.hilbert, L, G)
+    end
+    return out
+end
+
+"""One production iteration with linear mixing; returns (λ, J̄)."""
+function scba_iteration!(s::HipScbaState, p::HipScbaProblem, h, scratch)
+    g = p.grid
+    λ, number, holes = current_green!(g, h, s.families, p.target, s.Gᴿ, s.Gˡ, s.Gᵍ, nothing, nothing)
+    J̄ = boundary_flux_bar(g, s.families[1][2], s.families[1][3], s.Gˡ, s.Gᵍ)
+    candidates!(scratch, p, s.Gᴿ, s.Gˡ, s.Gᵍ)
+    for (f, c) in zip(s.families, scratch), k = 1:3
+        mix!(f[k], c[k], p.α, g.ctx)
+    end
+    return λ, J̄
+end
+
+"""Iterate to |ΔJ̄| ≤ tol |J̄|; returns (J̄, iterations)."""
+function converge!(s::HipScbaState, p::HipScbaProblem, h; tol = 1e-12, max_iterations = 5000)
+    g = p.grid
+    scratch = [(_field(g), _field(g), _field(g)) for _ in s.families]
+    prev, J̄ = NaN, NaN
+    it = 0
+    for outer it = 1:max_iterations
+        _, J̄ = scba_iteration!(s, p, h, scratch)
+        it >= 5 && abs(J̄ - prev) <= tol * abs(J̄) && break
+        prev = J̄
+    end
+    return J̄, it
+end
+
+"""dJ̄/dh at a converged state by the damped fixed-point adjoint
+μ ← (1-α)μ + α[(∂F/∂s)† μ + ∂J̄/∂s] on the anti-Hermitian tangent space of
+Σ< and Σ> (the mirror of qclhip/adjoint.py); h̄ has the layout of h."""
+function adjoint_gradient(s::HipScbaState, p::HipScbaProblem, h; tol = 1e-13, max_iterations = 5000)
+    g, ctx = p.grid, p.grid.ctx
+    # (λ, number, holes, a, c) of the solve at the converged state
+    coef = current_green_coefficients!(g, h, s.families, p.target, s.Gᴿ, s.Gˡ, s.Gᵍ, nothing, nothing)
+    plus = s.families[1]
+    Jˡ, Jᵍ, p̄ˡ, p̄ᵍ = (_zfield(g) for _ = 1:4)
+    boundary_flux_adjoint!(g, 1.0, plus[2], plus[3], s.Gˡ, s.Gᵍ, p̄ˡ, p̄ᵍ, Jˡ, Jᵍ)
+    antihermitian_part!(p̄ˡ, g)
+    antihermitian_part!(p̄ᵍ, g)
+    μ = [(_zfield(g), _zfield(g), _zfield(g)) for _ in s.families]
+    Ḡᴿ, Ḡˡ, Ḡᵍ = _field(g), _field(g), _field(g)
+    σ = (_field(g), _field(g), _field(g))
+    pnew = (_field(g), _field(g))
+    tmpL, tmpG, Z = _field(g), _field(g), _field(g)
+    function reverse_candidates!()
+        fill_zero!(Ḡᴿ, ctx)
+        copyto!(Ḡˡ, Jˡ)
+        copyto!(Ḡᵍ, Jᵍ)
+        for (i, m) in enumerate(p.mechanisms)
+            R, L, G = μ[2+i]
+            copyto!(tmpL, L)
+            copyto!(tmpG, G)
+            retarded_self_energy_adjoint!(tmpL, tmpG, p.hilbert, R)
+            if m.is_lo
+                shift_transpose!(Z, g, p.plusLO, tmpL, p.Nᴸᴼ + 1)
+                shift_transpose!(Z, g, p.minusLO, tmpL, p.Nᴸᴼ; accumulate = true)
+                production_static_contraction!(Ḡˡ, m.adjoint, Z, m.qᴷ; β = 1)
+                shift_transpose!(Z, g, p.minusLO, tmpG, p.Nᴸᴼ + 1)
+                shift_transpose!(Z, g, p.plusLO, tmpG, p.Nᴸᴼ; accumulate = true)
+                production_static_contraction!(Ḡᵍ, m.adjoint, Z, m.qᴷ; β = 1)
+            else
+                production_static_contraction!(Ḡˡ, m.adjoint, tmpL, m.qᴷ; β = 1)
+                production_static_contraction!(Ḡᵍ, m.adjoint, tmpG, m.qᴷ; β = 1)
+            end
+        end
+        for (T, plan, f) in ((p.T₊, p.plusE, μ[1]), (p.T₋, p.minusE, μ[2]))
+            embedding_adjoint!(Ḡᴿ, g, T, plan, f[1])
+            embedding_adjoint!(Ḡˡ, g, T, plan, f[2])
+            embedding_adjoint!(Ḡᵍ, g, T, plan, f[3])
+        end
+    end
+    for it = 1:max_iterations
+        reverse_candidates!()
+        current_green_adjoint!(σ[1], σ[2], σ[3], nothing, g, s.families, p.target, coef, s.Gᴿ, s.Gˡ, s.Gᵍ, Ḡᴿ, Ḡˡ, Ḡᵍ)
+        antihermitian_part!(σ[2], g)
+        antihermitian_part!(σ[3], g)
+        sum_fields!(pnew[1], ctx, (σ[2], p̄ˡ))
+        sum_fields!(pnew[2], ctx, (σ[3], p̄ᵍ))
+        olds, news = ROCArray{ComplexF64,4}[], ROCArray{ComplexF64,4}[]
+        for (k, f) in enumerate(μ)
+            nl, ng = k == 1 ? pnew : (σ[2], σ[3])
+            append!(olds, f)
+            append!(news, (σ[1], nl, ng))
+        end
+        num, den = mix_fields_change!(ctx, olds, news, p.α)
+        sqrt(num / max(den, 1e-300)) <= tol && break
+    end
+    reverse_candidates!()
+    h̄ = ROCArray{ComplexF64}(undef, g.Nk, g.Nb, g.Nb)
+    current_green_adjoint!(σ[1], σ[2], σ[3], h̄, g, s.families, p.target, coef, s.Gᴿ, s.Gˡ, s.Gᵍ, Ḡᴿ, Ḡˡ, Ḡᵍ)
+    return h̄
+end
+
+"""The converged boundary flux J̄(h) of the device SCBA loop started from
+`state` (which it updates in place)."""
+converged_current(p::HipScbaProblem, s::HipScbaState, h) = first(converge!(s, p, h))
+
+"""ChainRulesCore rrule of `converged_current`: the forward converges the
+loop; the pullback runs the fixed-point adjoint at that state and returns
+J̄̄ · dJ̄/dh (the implicit adjoint needs only the converged state, so no
+iterate is checkpointed)."""
+function ChainRulesCore.rrule(::typeof(converged_current), p::HipScbaProblem, s::HipScbaState, h)
+    J̄ = converged_current(p, s, h)
+    function converged_current_pullback(Δ)
+        h̄ = adjoint_gradient(s, p, h)
+        return NoTangent(), NoTangent(), NoTangent(), h̄ .* ChainRulesCore.unthunk(Δ)
+    end
+    return J̄, converged_current_pullback
+end
+
+"""dJ̄/dU on a real-space grid from h̄ (basis.jl:300: ∂h_m/∂U_g = Φ[g,:]* Φ[g,:]ᵀ):
+dJ̄/dU_g = Σ_m Re Σ_ab conj(h̄_m[a,b]) conj(Φ[g,a]) Φ[g,b]."""
+function potential_gradient(h̄, Φ::Matrix{ComplexF64})
+    H = dropdims(sum(Array(h̄); dims = 1); dims = 1)  # Σ_m h̄_m, (Nb, Nb)
+    return [real(sum(conj(H[a, b]) * conj(Φ[g, a]) * Φ[g, b] for a in axes(H, 1), b in axes(H, 2)))
+            for g in axes(Φ, 1)]
+end
+
+"""dJ̄/dL_i of layer thicknesses by the chain rule through central
+differences of a host assembly L ↦ h(L) (Nk, Nb, Nb)."""
+function thickness_gradient(h̄, assemble, L::Vector{Float64}; step = 1e-3)
+    hb = Array(h̄)
+    return [begin
+        d = zeros(length(L)); d[i] = step
+        real(sum(conj.(hb) .* ((assemble(L .+ d) .- assemble(L .- d)) ./ (2step))))
+    end for i in eachindex(L)]
 end
 
 end # module
