@@ -454,6 +454,51 @@ def dense_contraction_bench(N=24, NE=2048, Nk=64, reps=2):
                        "16x16x4 MFMAs per complex step")
 
 
+def long_axis_hilbert_bench(NE=30721, Nk=32, N=10, reps=3):
+    """S^R = H[Gamma] - i Gamma/2 with the 2L round-off witness on an energy
+    axis of NE = 30721 nodes (L = 65536, the four-step transform through
+    HBM) over Nk N^2 columns; algorithmic bytes: read S< and S>, write S^R.
+    Beside it, both transform paths at L = 16384 (NE = 8192), the longest
+    length the one-workgroup path takes."""
+    import torch
+
+    from qclhip import api
+
+    def run(NE_, path):
+        torch.cuda.empty_cache()
+        eps = torch.linspace(-0.3, 0.6, NE_, dtype=torch.float64).numpy()
+        wE = eps.copy()
+        wE[:] = eps[1] - eps[0]
+        wE[[0, -1]] /= 2
+        g = api.Grid(eps, wE, torch.ones(Nk, dtype=torch.float64).numpy(), N)
+        plan = api.FFTHilbertPlan(g, verify_roundoff=True)
+        plan.set_path(path)
+        gen = torch.Generator(device="cuda").manual_seed(NE_)
+        SL = torch.randn(api.field_shape(NE_, Nk, N), dtype=torch.complex128, device="cuda", generator=gen)
+        SG = torch.randn(api.field_shape(NE_, Nk, N), dtype=torch.complex128, device="cuda", generator=gen)
+        SR = torch.empty_like(SL)
+        api.retarded_self_energy_fft(plan, SL, SG, return_roundoff=True, SR=SR)
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            _, _, r = api.retarded_self_energy_fft(plan, SL, SG, return_roundoff=True, SR=SR)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = 3.0 * SL.numel() * 16
+        del SL, SG, SR, plan
+        torch.cuda.empty_cache()
+        return dict(ms=ms, GBs=nbytes / (ms * 1e-3) / 1e9, frac_hbm=nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                    witness=r, field_GB=nbytes / 3 / 1e9)
+
+    out = dict(config=f"NE={NE} (L=65536) Nk={Nk} N={N}: {Nk * N * N} columns, S^R with the 2L witness; "
+                      f"k_fs_columns<16> + k_fs_rows<4096> + k_fs_output<16>",
+               long_axis=run(NE, "hbm"))
+    out["L16384"] = {path: run(8192, path) for path in ("lds", "hbm")}
+    return out
+
+
 def outer_bench(p, s, Nz=512, reps=5):
     """The Poisson step after an SCBA run at the same size: density of G<
     (N_E N_k N_b^2 values), the bordered Poisson LU solve (N_z + 1 = 513),
@@ -721,6 +766,8 @@ def main():
     ap.add_argument("--no-gradient", action="store_true", help="skip the converged-gradient leg")
     ap.add_argument("--no-dense", action="store_true", help="skip the dense-kernel contraction leg")
     ap.add_argument("--only-dense", action="store_true")
+    ap.add_argument("--no-long-axis", action="store_true", help="skip the NE = 30721 Hilbert leg")
+    ap.add_argument("--only-long-axis", action="store_true")
     ap.add_argument("--only-gradient", action="store_true")
     ap.add_argument("--no-batch-scba", action="store_true", help="skip the batched SCBA sweep")
     ap.add_argument("--batch-instances", type=int, default=24)
@@ -752,6 +799,9 @@ def main():
         log("copy calibration: 3 x 1 GiB device-to-device copies")
     if args.only_dense:
         print(json.dumps({"dense_contraction": dense_contraction_bench()}, default=float), flush=True)
+        return
+    if args.only_long_axis:
+        print(json.dumps({"long_axis_hilbert": long_axis_hilbert_bench()}, default=float), flush=True)
         return
     if args.only_batch_scba:
         log(f"batched SCBA sweep: {args.batch_instances} uneven instances, Nk=64")
@@ -797,6 +847,11 @@ def main():
         log("dense kernel contraction: N=24 NE=2048 Nk=64 (Kflat 21.7 GB)")
         dense = dense_contraction_bench()
         log(f"dense contraction {dense['ms']:.1f} ms, {dense['tflops']:.1f} TFLOP/s")
+    longax = None
+    if not args.no_long_axis:
+        log("long energy axis: S^R with the witness at NE=30721 (L=65536), Nk=32, N=10")
+        longax = long_axis_hilbert_bench()
+        log(f"long axis {longax['long_axis']['ms']:.2f} ms, {longax['long_axis']['GBs']:.0f} GB/s")
     bsc = None
     if not args.no_batch_scba:
         log(f"batched SCBA sweep: {args.batch_instances} uneven instances, Nk=64")
@@ -901,6 +956,8 @@ def main():
         line["batch_scba"] = bsc
     if dense is not None:
         line["dense_contraction"] = dense
+    if longax is not None:
+        line["long_axis_hilbert"] = longax
     if full is not None:
         line["scba_full_iter_per_s"] = full["iter_per_s"]
         line["scba_full"] = dict(full, config="six-layer GaAs/Al0.15Ga0.85As period (thicknesses U[2,20] nm, seeded), "

@@ -282,10 +282,22 @@ int qclhip_lo_contraction(qclhip_ctx* ctx, const qclhip_kernel* kernel, const do
 
 /* ---- A9/A10: FFT Hilbert transform (fft_parallel.jl:27-291;
  * production_operators.jl:735-884). Plan for ncols columns of length NE on a
- * uniform grid; L = nextpow2(2NE-1); verify adds the 2L round-off witness. */
+ * uniform grid; L = nextpow2(2NE-1) <= 262144 (NE <= 131072); verify adds
+ * the 2L round-off witness. Up to L = 16384 a column's transform runs in one
+ * workgroup's LDS; above, as a four-step transform through HBM scratch
+ * (columns in chunks of at most 1 GiB of scratch). */
 int qclhip_hilbert_create(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t ncols,
                           int verify, qclhip_hilbert** out);
 int qclhip_hilbert_destroy(qclhip_hilbert* plan);
+/* Which implementation the plan runs: AUTO (by L, the default), LDS (L <=
+ * 16384) or HBM (the four-step transform, L >= 4096). Results agree to
+ * rounding; no counterpart in the reference (its FFTW plan has no such
+ * choice). get_path reports the resolved choice (LDS or HBM). */
+#define QCLHIP_HILBERT_PATH_AUTO 0
+#define QCLHIP_HILBERT_PATH_LDS 1
+#define QCLHIP_HILBERT_PATH_HBM 2
+int qclhip_hilbert_set_path(qclhip_ctx* ctx, qclhip_hilbert* plan, int path);
+int qclhip_hilbert_get_path(qclhip_ctx* ctx, const qclhip_hilbert* plan, int* path_host);
 /* Lambda_e = sum_{e' != e} wE_e' Gamma_e' / (2 pi (eps_e - eps_e')).
  * roundoff_host (may be NULL; requires verify) = ||L2 - L|| / (||L|| + 1e-14). */
 int qclhip_fft_hilbert(qclhip_ctx* ctx, const qclhip_hilbert* plan, const double* Gamma,

@@ -360,8 +360,6 @@ int contract_core(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double 
             hipLaunchKernelGGL(k_lowrank_sum, dim3((unsigned)((NE * r + 255) / 256)), dim3(256), 0, ctx->stream,
                                (int)NE, (int)r, (int)nchunk, partial, red);
             // S2 (NE x Nc) = f red left^T + beta S2, streamed over q.
-            const size_t lds = 0;
-            (void)lds;
             hipLaunchKernelGGL(k_lowrank_expand, ecol_grid((int)NE, Nc, ctx->sm_count), dim3(256), 0,
                                ctx->stream, (int)NE, Nc, (int)r, red, K->a, f, beta, S);
             return launch_check(ctx, "low-rank contraction");

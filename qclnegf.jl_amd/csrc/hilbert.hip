@@ -32,6 +32,11 @@ struct qclhip_hilbert {
     qclhip::c64* kf[2] = {nullptr, nullptr};
     qclhip::c64* phase = nullptr;  // exp(-i pi n / L), n < L (witness)
     int Lmax = 0;
+    // Four-step path through HBM (L = L1 * L2, see k_fs_columns): the two
+    // spectra above in row order, kp[k1 * L2 + k2] = kf[k1 + L1 * k2].
+    qclhip::c64* kp[2] = {nullptr, nullptr};
+    int L1 = 0, L2 = 0;
+    int path = 0;  // QCLHIP_HILBERT_PATH_*; AUTO resolves by L
 };
 
 namespace qclhip {
@@ -489,6 +494,243 @@ __global__ __launch_bounds__(1024) void k_witness_reduce(const double* partial, 
     if (t == 0) out[0] = sqrt(red[0][0]) / (sqrt(red[1][0]) + 1e-14);
 }
 
+// ---- energy axes beyond one workgroup's LDS: a four-step FFT through HBM ---
+// L = L1 * L2 (L1 <= 16, L2 a one-workgroup length), n = L2 n1 + n2 and
+// k = k1 + L1 k2, so that
+//   X[k1 + L1 k2] = sum_n2 w_L2^(n2 k2) w_L^(n2 k1) sum_n1 w_L1^(n1 k1) x[L2 n1 + n2]
+// (w_M = exp(-2 pi i / M)). The convolution runs in three passes over a
+// column-chunk scratch of L1 x L2 rows:
+//  k_fs_columns: one thread per (column, n2) loads x[L2 n1 + n2], n1 < L1
+//    (consecutive threads read consecutive n2, so the loads coalesce), runs
+//    the L1-point DFT in registers, applies w_L^(n2 k1) and writes row k1;
+//  k_fs_rows: one workgroup per row runs the L2-point transform with the
+//    Stockham plan of k_hilbert, multiplies by the kernel spectrum (stored
+//    in the same row order, so no transpose is ever made), runs the inverse
+//    L2-point transform and applies w_L^(-n2 k1);
+//  k_fs_output: the inverse L1-point DFT in registers gives y[L2 n1 + n2];
+//    the outputs y[e + NE - 1] / L take the modes of k_hilbert.
+// With the witness every pass also carries the odd half of the 2L transform
+// (input x_n exp(-i pi n/L), the odd kernel spectrum), as k_hilbert_odd.
+struct FourStepArgs {
+    HilbertArgs h;       // NE, wE, inputs, outputs, mode, witness tables and partials
+    c64* work;           // [nc][L1][L2] rows, then the odd half's (witness)
+    const c64* kp;       // kernel spectrum in row order
+    const c64* kpodd;    // odd half of the 2L spectrum in row order (NULL: no witness)
+    long long col0, nc;  // the chunk's first column and its column count
+    long long part0;     // the chunk's first witness partial
+    long long nparts;    // witness partials of the whole call (one per output workgroup)
+    int L;
+};
+
+constexpr int kFsThreads = 256;
+constexpr int kMaxHilbertLength = 262144;  // L1 = 16 rows of 16384
+constexpr int kMinFourStepLength = 4096;
+
+// w_L^(n2 k1), k1 < L1, from w^(n2), w^(2 n2), w^(4 n2), w^(8 n2) (the loads
+// of consecutive n2 coalesce, where tw[n2 k1] would stride by k1) with at
+// most three products, as stockham_pass forms its powers.
+template <int L1>
+__device__ inline void fs_twiddles(const c64* __restrict__ tw, int n2, c64* w) {
+    c64 b[4];
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) b[bit] = (1 << bit) < L1 ? tw[(1 << bit) * n2] : cmk(1.0, 0.0);
+    w[0] = cmk(1.0, 0.0);
+#pragma unroll
+    for (int k1 = 1; k1 < L1; ++k1) {
+        c64 x = cmk(1.0, 0.0);
+        bool first = true;
+#pragma unroll
+        for (int bit = 0; bit < 4; ++bit)
+            if (k1 & (1 << bit)) {
+                x = first ? b[bit] : cmul(x, b[bit]);
+                first = false;
+            }
+        w[k1] = x;
+    }
+}
+
+template <int L1>
+__global__ __launch_bounds__(kFsThreads) void k_fs_columns(FourStepArgs f) {
+    const int L = f.L, L2 = L / L1, NE = f.h.NE;
+    const long long gid = (long long)blockIdx.x * kFsThreads + threadIdx.x;
+    if (gid >= f.nc * L2) return;
+    const long long c = gid / L2;
+    const int n2 = (int)(gid - c * L2);
+    const size_t base = (size_t)(f.col0 + c) * NE;
+    const bool witness = f.kpodd != nullptr;
+    double re[L1], im[L1], ro[L1], io[L1];
+#pragma unroll
+    for (int n1 = 0; n1 < L1; ++n1) {
+        const int n = n2 + L2 * n1;
+        double xr = 0.0, xi = 0.0;
+        if (n < NE) {
+            const c64 gm = load_gamma(f.h, base + n);
+            const double we = f.h.mode == 3 ? 1.0 : f.h.wE[n];  // the adjoint weights the output
+            xr = we * gm.x;
+            xi = we * gm.y;
+        }
+        re[n1] = xr;
+        im[n1] = xi;
+        if (witness) {
+            double yr = 0.0, yi = 0.0;
+            if (n < NE) {
+                const c64 ph = f.h.phase[n];
+                yr = xr * ph.x - xi * ph.y;
+                yi = xr * ph.y + xi * ph.x;
+            }
+            ro[n1] = yr;
+            io[n1] = yi;
+        }
+    }
+    dft<L1, false>(re, im);
+    c64 w[L1];
+    fs_twiddles<L1>(f.h.tw, n2, w);  // 8 n2 < L / 2
+    c64* row = f.work + (size_t)c * L + n2;
+#pragma unroll
+    for (int k1 = 0; k1 < L1; ++k1)
+        row[(size_t)k1 * L2] = cmk(w[k1].x * re[k1] - w[k1].y * im[k1], w[k1].x * im[k1] + w[k1].y * re[k1]);
+    if (!witness) return;
+    dft<L1, false>(ro, io);
+    c64* rowo = row + (size_t)f.nc * L;
+#pragma unroll
+    for (int k1 = 0; k1 < L1; ++k1)
+        rowo[(size_t)k1 * L2] = cmk(w[k1].x * ro[k1] - w[k1].y * io[k1], w[k1].x * io[k1] + w[k1].y * ro[k1]);
+}
+
+// At most 128 VGPRs: two 512-thread workgroups per CU at L2 = 4096, so that
+// one row's loads overlap another's transforms.
+template <int L2>
+__global__ __launch_bounds__(threads_for(L2)) __attribute__((amdgpu_waves_per_eu(L2 == 4096 ? 4 : 1))) void k_fs_rows(
+    FourStepArgs f) {
+    constexpr int V = vals_per_thread(L2), T = threads_for(L2);
+    constexpr int NPASS = npasses(L2);
+    constexpr int R0 = fwd_radix(L2, 0), RL = fwd_radix(L2, NPASS - 1);
+    extern __shared__ __align__(16) double lds[];
+    const int t = threadIdx.x;
+    const int L = f.L, L1 = L / L2;
+    const long long rowi = blockIdx.x;  // c * L1 + k1
+    const int k1 = (int)(rowi % L1);
+    const bool odd = blockIdx.y == 1;
+    c64* row = f.work + (odd ? (size_t)f.nc * L : 0) + (size_t)rowi * L2;
+    const c64* kp = (odd ? f.kpodd : f.kp) + (size_t)k1 * L2;
+    double re[V], im[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int i = v / R0, r = v % R0;
+        const c64 x = row[t + i * T + r * (L2 / R0)];
+        re[v] = x.x;
+        im[v] = x.y;
+    }
+    // the table holds w_L; stride L1 gives w_L2
+    StockhamPasses<L2, 0, false>::run(re, im, lds, t, f.h.tw, L1);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int i = v / RL, r = v % RL;
+        const c64 k = kp[t + i * T + r * (L2 / RL)];
+        const double xr = re[v], xi = im[v];
+        re[v] = k.x * xr - k.y * xi;
+        im[v] = k.x * xi + k.y * xr;
+    }
+    StockhamPasses<L2, 0, true>::run(re, im, lds, t, f.h.tw, L1);
+    // each thread writes back the positions it read, times w_L^(-n2 k1):
+    // w_L^(k1 (t + i T)) (one gather per i) times the workgroup-uniform
+    // w_L^(k1 r L2/R0)
+#pragma unroll
+    for (int i = 0; i < V / R0; ++i) {
+        const c64 a = f.h.tw[k1 * (t + i * T)];
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int v = i * R0 + r;
+            const int n2 = t + i * T + r * (L2 / R0);
+            const c64 w = r == 0 ? a : cmul(a, f.h.tw[k1 * r * (L2 / R0)]);  // conjugated below
+            row[n2] = cmk(w.x * re[v] + w.y * im[v], w.x * im[v] - w.y * re[v]);
+        }
+    }
+}
+
+template <int L1>
+__global__ __launch_bounds__(kFsThreads) void k_fs_output(FourStepArgs f) {
+    __shared__ double rd[kFsThreads], rn[kFsThreads];
+    const int L = f.L, L2 = L / L1, NE = f.h.NE;
+    const int t = threadIdx.x;
+    const long long gid = (long long)blockIdx.x * kFsThreads + t;
+    const bool witness = f.kpodd != nullptr;
+    const double invL = 1.0 / (double)L;
+    double dn = 0.0, nn = 0.0;
+    if (gid < f.nc * L2) {
+        const long long c = gid / L2;
+        const int n2 = (int)(gid - c * L2);
+        const size_t base = (size_t)(f.col0 + c) * NE;
+        const c64* row = f.work + (size_t)c * L + n2;
+        double re[L1], im[L1], ro[L1], io[L1];
+#pragma unroll
+        for (int k1 = 0; k1 < L1; ++k1) {
+            const c64 x = row[(size_t)k1 * L2];
+            re[k1] = x.x;
+            im[k1] = x.y;
+        }
+        dft<L1, true>(re, im);
+        if (witness) {
+            const c64* rowo = row + (size_t)f.nc * L;
+#pragma unroll
+            for (int k1 = 0; k1 < L1; ++k1) {
+                const c64 x = rowo[(size_t)k1 * L2];
+                ro[k1] = x.x;
+                io[k1] = x.y;
+            }
+            dft<L1, true>(ro, io);
+        }
+#pragma unroll
+        for (int n1 = 0; n1 < L1; ++n1) {
+            const int n = n2 + L2 * n1;
+            const int e = n - (NE - 1);
+            if (e < 0 || e >= NE) continue;
+            const c64 lam = cmk(re[n1] * invL, im[n1] * invL);
+            const size_t gi = base + e;
+            if (witness) {
+                const c64 ph = f.h.phase[n];  // exp(+i pi n/L) = conj(ph)
+                const double yr = ro[n1] * invL, yi = io[n1] * invL;
+                const c64 od = cmk(yr * ph.x + yi * ph.y, yi * ph.x - yr * ph.y);
+                const double dx = 0.5 * (lam.x + od.x) - lam.x, dy = 0.5 * (lam.y + od.y) - lam.y;
+                dn += dx * dx + dy * dy;
+                nn += lam.x * lam.x + lam.y * lam.y;
+            }
+            if (f.h.mode == 3) {
+                const c64 rb = f.h.gamma[gi];
+                const double we = f.h.wE[e];
+                const c64 ig = cmk(we * lam.y - 0.5 * rb.x, -we * lam.x - 0.5 * rb.y);
+                f.h.out[gi] = cadd(f.h.out[gi], ig);
+                f.h.out2[gi] = csub(f.h.out2[gi], ig);
+            } else if (f.h.mode == 1) {
+                const c64 gm = load_gamma(f.h, gi);
+                f.h.out[gi] = cmk(lam.x + 0.5 * gm.y, lam.y - 0.5 * gm.x);  // Lambda - 0.5 i Gamma
+            } else {
+                f.h.out[gi] = lam;
+            }
+        }
+    }
+    if (!witness) return;  // uniform over the workgroup
+    rd[t] = dn;
+    rn[t] = nn;
+    __syncthreads();
+#pragma unroll
+    for (int width = kFsThreads / 2; width > 0; width >>= 1) {
+        if (t < width) {
+            rd[t] += rd[t + width];
+            rn[t] += rn[t + width];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        f.h.partial[f.part0 + blockIdx.x] = rd[0];
+        f.h.partial[f.nparts + f.part0 + blockIdx.x] = rn[0];
+    }
+}
+
+// Row length of the four-step split: L / L2 <= 16 and L2 a length the
+// one-workgroup transform is tuned for (4096 while L1 <= 16 allows it).
+int fourstep_rows(int L) { return L <= 8192 ? L / 2 : (L <= 65536 ? 4096 : L / 16); }
+
 // Host FFT in long double for the plan tables.
 void host_fft(std::vector<std::complex<long double>>& x) {
     const size_t L = x.size();
@@ -572,6 +814,88 @@ int launch_hilbert_len(qclhip_ctx* ctx, long long ncols, const HilbertArgs& a) {
     return launch_check(ctx, "k_hilbert");
 }
 
+template <int L1>
+int launch_fs_columns_output(qclhip_ctx* ctx, const FourStepArgs& f, bool output, unsigned blocks) {
+    if (output)
+        hipLaunchKernelGGL(k_fs_output<L1>, dim3(blocks), dim3(kFsThreads), 0, ctx->stream, f);
+    else
+        hipLaunchKernelGGL(k_fs_columns<L1>, dim3(blocks), dim3(kFsThreads), 0, ctx->stream, f);
+    return launch_check(ctx, output ? "k_fs_output" : "k_fs_columns");
+}
+
+int fs_columns_output(qclhip_ctx* ctx, const FourStepArgs& f, int L1, bool output, unsigned blocks) {
+    switch (L1) {
+        case 2: return launch_fs_columns_output<2>(ctx, f, output, blocks);
+        case 4: return launch_fs_columns_output<4>(ctx, f, output, blocks);
+        case 8: return launch_fs_columns_output<8>(ctx, f, output, blocks);
+        case 16: return launch_fs_columns_output<16>(ctx, f, output, blocks);
+        default: return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "four-step split outside L1 = 2..16");
+    }
+}
+
+template <int L2>
+int launch_fs_rows(qclhip_ctx* ctx, const FourStepArgs& f, unsigned rows) {
+    const size_t lds = (size_t)fft_lds_len<L2>() * sizeof(double);
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "FFT row length exceeds the LDS of one workgroup");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_fs_rows<L2>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_fs_rows<L2>, dim3(rows, f.kpodd ? 2 : 1), dim3(threads_for(L2)), lds, ctx->stream, f);
+    return launch_check(ctx, "k_fs_rows");
+}
+
+int fs_rows(qclhip_ctx* ctx, const FourStepArgs& f, int L2, unsigned rows) {
+    switch (L2) {
+        case 2048: return launch_fs_rows<2048>(ctx, f, rows);
+        case 4096: return launch_fs_rows<4096>(ctx, f, rows);
+        case 8192: return launch_fs_rows<8192>(ctx, f, rows);
+        case 16384: return launch_fs_rows<16384>(ctx, f, rows);
+        default: return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "four-step row length outside 2048..16384");
+    }
+}
+
+// Columns per four-step chunk: the chunk's rows (and the odd half's) within
+// kFourStepChunkBytes of scratch.
+constexpr size_t kFourStepChunkBytes = size_t(128) << 20;
+
+long long fourstep_chunk(const qclhip_hilbert* p, bool witness) {
+    const size_t per_col = (size_t)p->L * sizeof(c64) * (witness ? 2 : 1);
+    return std::max<long long>(1, std::min<long long>(p->ncols, (long long)(kFourStepChunkBytes / per_col)));
+}
+
+bool fourstep_path(const qclhip_hilbert* p) {
+    return p->path == QCLHIP_HILBERT_PATH_HBM || (p->path == QCLHIP_HILBERT_PATH_AUTO && p->L > 16384);
+}
+
+// The four-step transform of every column, chunk by chunk, in the scratch
+// `work` (fourstep_chunk columns of rows, times two with the witness).
+int hilbert_fourstep(qclhip_ctx* ctx, const qclhip_hilbert* p, const HilbertArgs& a, c64* work) {
+    FourStepArgs f{};
+    f.h = a;
+    f.h.tw = p->twiddle;
+    f.work = work;
+    f.kp = p->kp[0];
+    f.kpodd = a.kodd ? p->kp[1] : nullptr;
+    f.L = p->L;
+    const int L1 = p->L1, L2 = p->L2;
+    const long long chunk = fourstep_chunk(p, a.kodd != nullptr);
+    f.nparts = p->ncols * L2 / kFsThreads;
+    for (long long c0 = 0; c0 < p->ncols; c0 += chunk) {
+        f.col0 = c0;
+        f.nc = std::min(chunk, p->ncols - c0);
+        f.part0 = c0 * L2 / kFsThreads;
+        const unsigned blocks = (unsigned)(f.nc * L2 / kFsThreads);  // L2 >= 2048
+        int rc = fs_columns_output(ctx, f, L1, false, blocks);
+        if (rc) return rc;
+        rc = fs_rows(ctx, f, L2, (unsigned)(f.nc * L1));
+        if (rc) return rc;
+        rc = fs_columns_output(ctx, f, L1, true, blocks);
+        if (rc) return rc;
+    }
+    return QCLHIP_OK;
+}
+
 int hilbert_launch(qclhip_ctx* ctx, const qclhip_hilbert* p, const HilbertArgs& a_in) {
     HilbertArgs a = a_in;
     const int L = p->L;
@@ -597,8 +921,7 @@ int hilbert_launch(qclhip_ctx* ctx, const qclhip_hilbert* p, const HilbertArgs& 
 }
 
 int hilbert_run(qclhip_ctx* ctx, const qclhip_hilbert* p, const double* Gamma, const double* SL,
-                const double* SG, double* out, int mode, double* roundoff_host) {
-    const long long n = (long long)p->NE * p->ncols;
+                const double* SG, double* out, int mode, double* roundoff_host, double* out2 = nullptr) {
     HilbertArgs a{};
     a.NE = p->NE;
     a.ncols = p->ncols;
@@ -607,21 +930,32 @@ int hilbert_run(qclhip_ctx* ctx, const qclhip_hilbert* p, const double* Gamma, c
     a.SL = (const c64*)SL;
     a.SG = (const c64*)SG;
     a.out = (c64*)out;
+    a.out2 = (c64*)out2;
     a.mode = mode;
-    (void)n;
-    double* partial = nullptr;
-    if (roundoff_host) {
-        if (!p->verify) return fail(ctx, QCLHIP_ERR_ARGUMENT, "the FFT plan was built without roundoff verification");
-        partial = (double*)workspace(ctx, (size_t)2 * p->ncols * sizeof(double) + 64);
-        if (!partial) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    if (roundoff_host && !p->verify)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "the FFT plan was built without roundoff verification");
+    const bool fs = fourstep_path(p);
+    const bool witness = roundoff_host != nullptr;
+    // One scratch request: the witness partials (one per column, or one per
+    // output workgroup of the four-step path) and the four-step rows.
+    const long long nparts = fs ? p->ncols * p->L2 / kFsThreads : p->ncols;
+    const size_t part_bytes = witness ? ((size_t)(2 * nparts + 8) * sizeof(double) + 255) / 256 * 256 : 0;
+    const size_t work_bytes = fs ? (size_t)fourstep_chunk(p, witness) * p->L * sizeof(c64) * (witness ? 2 : 1) : 0;
+    char* scratch = nullptr;
+    if (part_bytes + work_bytes) {
+        scratch = (char*)workspace(ctx, part_bytes + work_bytes);
+        if (!scratch) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    }
+    double* partial = witness ? (double*)scratch : nullptr;
+    if (witness) {
         a.kodd = p->kf[1];
         a.phase = p->phase;
         a.partial = partial;
     }
-    int rc = hilbert_launch(ctx, p, a);
-    if (rc || !roundoff_host) return rc;
-    double* res = partial + 2 * p->ncols;
-    hipLaunchKernelGGL(k_witness_reduce, dim3(1), dim3(1024), 0, ctx->stream, partial, p->ncols, res);
+    int rc = fs ? hilbert_fourstep(ctx, p, a, (c64*)(scratch + part_bytes)) : hilbert_launch(ctx, p, a);
+    if (rc || !witness) return rc;
+    double* res = partial + 2 * nparts;
+    hipLaunchKernelGGL(k_witness_reduce, dim3(1), dim3(1024), 0, ctx->stream, partial, nparts, res);
     rc = launch_check(ctx, "k_witness_reduce");
     if (rc) return rc;
     bool deferred = false;
@@ -650,8 +984,8 @@ int qclhip_hilbert_create(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t ncol
     int L = 1;
     while (L < 2 * grid->NE - 1) L <<= 1;
     const int Lmax = L;  // the witness runs at length L (odd-frequency half of the 2L transform)
-    if ((size_t)Lmax * sizeof(double) > ctx->max_lds_per_block || Lmax > 16384)
-        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "FFT length exceeds the LDS of one workgroup");
+    if (Lmax > kMaxHilbertLength)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "FFT length above 262144 (NE above 131072)");
     auto* p = new qclhip_hilbert();
     p->grid = grid;
     p->NE = grid->NE;
@@ -668,10 +1002,22 @@ int qclhip_hilbert_create(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t ncol
     }
     bool ok = hipMalloc(&p->twiddle, (size_t)Lmax * sizeof(c64)) == hipSuccess &&
               hipMemcpy(p->twiddle, tw.data(), (size_t)Lmax * sizeof(c64), hipMemcpyHostToDevice) == hipSuccess;
+    if (L >= kMinFourStepLength) {
+        p->L2 = fourstep_rows(L);
+        p->L1 = L / p->L2;
+    }
     for (int w = 0; w < (verify ? 2 : 1) && ok; ++w) {
         std::vector<c64> kf = kernel_spectrum(grid->NE, L, grid->de, w == 1);
         ok = hipMalloc(&p->kf[w], (size_t)L * sizeof(c64)) == hipSuccess &&
              hipMemcpy(p->kf[w], kf.data(), (size_t)L * sizeof(c64), hipMemcpyHostToDevice) == hipSuccess;
+        if (ok && p->L2) {
+            // row order of the four-step path: kp[k1 * L2 + k2] = kf[k1 + L1 * k2]
+            std::vector<c64> kp((size_t)L);
+            for (int k1 = 0; k1 < p->L1; ++k1)
+                for (int k2 = 0; k2 < p->L2; ++k2) kp[(size_t)k1 * p->L2 + k2] = kf[(size_t)k1 + (size_t)p->L1 * k2];
+            ok = hipMalloc(&p->kp[w], (size_t)L * sizeof(c64)) == hipSuccess &&
+                 hipMemcpy(p->kp[w], kp.data(), (size_t)L * sizeof(c64), hipMemcpyHostToDevice) == hipSuccess;
+        }
     }
     if (verify && ok) {
         std::vector<c64> ph(L);
@@ -697,7 +1043,27 @@ int qclhip_hilbert_destroy(qclhip_hilbert* p) {
     if (p->kf[0]) hipFree(p->kf[0]);
     if (p->kf[1]) hipFree(p->kf[1]);
     if (p->phase) hipFree(p->phase);
+    if (p->kp[0]) hipFree(p->kp[0]);
+    if (p->kp[1]) hipFree(p->kp[1]);
     delete p;
+    return QCLHIP_OK;
+}
+
+int qclhip_hilbert_set_path(qclhip_ctx* ctx, qclhip_hilbert* plan, int path) {
+    if (!ctx || !plan) return QCLHIP_ERR_ARGUMENT;
+    if (path == QCLHIP_HILBERT_PATH_LDS && plan->L > 16384)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "the one-workgroup transform takes L <= 16384");
+    if (path == QCLHIP_HILBERT_PATH_HBM && !plan->L2)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "the four-step transform takes L >= 4096");
+    if (path < QCLHIP_HILBERT_PATH_AUTO || path > QCLHIP_HILBERT_PATH_HBM)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "unknown Hilbert transform path");
+    plan->path = path;
+    return QCLHIP_OK;
+}
+
+int qclhip_hilbert_get_path(qclhip_ctx* ctx, const qclhip_hilbert* plan, int* path_host) {
+    if (!ctx || !plan || !path_host) return QCLHIP_ERR_ARGUMENT;
+    *path_host = fourstep_path(plan) ? QCLHIP_HILBERT_PATH_HBM : QCLHIP_HILBERT_PATH_LDS;
     return QCLHIP_OK;
 }
 
@@ -713,15 +1079,7 @@ int qclhip_retarded_self_energy_adjoint(qclhip_ctx* ctx, const qclhip_hilbert* p
     if (!ctx || !plan || !SRbar || !SLbar || !SGbar) return QCLHIP_ERR_ARGUMENT;
     if (SLbar == SGbar || SRbar == SLbar || SRbar == SGbar)
         return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
-    HilbertArgs a{};
-    a.NE = plan->NE;
-    a.ncols = plan->ncols;
-    a.wE = plan->grid->wE;
-    a.gamma = (const c64*)SRbar;
-    a.out = (c64*)SLbar;
-    a.out2 = (c64*)SGbar;
-    a.mode = 3;
-    return hilbert_launch(ctx, plan, a);
+    return hilbert_run(ctx, plan, SRbar, nullptr, nullptr, SLbar, 3, nullptr, SGbar);
 }
 
 int qclhip_retarded_self_energy(qclhip_ctx* ctx, const qclhip_hilbert* plan, const double* SL, const double* SG,

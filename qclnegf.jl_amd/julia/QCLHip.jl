@@ -255,6 +255,16 @@ function HilbertPlan(g::Grid; verify_roundoff::Bool = true)
     return p
 end
 
+"""Implementation of the plan's transform: `:auto` (by L), `:lds` (one
+workgroup per column, L ≤ 16384) or `:hbm` (four-step through HBM, L ≥ 4096)."""
+function set_path!(plan::HilbertPlan, path::Symbol)
+    code = Dict(:auto => 0, :lds => 1, :hbm => 2)
+    haskey(code, path) || throw(ArgumentError("unknown Hilbert path $path"))
+    check(plan.grid.ctx, ccall((:qclhip_hilbert_set_path, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint),
+                               plan.grid.ctx.handle, plan.handle, code[path]), "hilbert_set_path")
+    return plan
+end
+
 """`_retarded_self_energy_fft_production`: returns `(Σᴿ, roundoff)`."""
 function retarded_self_energy!(Σᴿ, plan::HilbertPlan, Σˡ, Σᵍ)
     r = Ref{Float64}(0.0)

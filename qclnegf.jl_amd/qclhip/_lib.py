@@ -98,6 +98,8 @@ _SIGNATURES = {
     "qclhip_lo_contraction": [_vp, _vp, _vp, _vp, ct.c_double, ct.c_double, _vp, _vp, _vp, _vp],
     "qclhip_hilbert_create": [_vp, _vp, _i64, ct.c_int, ct.POINTER(_vp)],
     "qclhip_hilbert_destroy": [_vp],
+    "qclhip_hilbert_set_path": [_vp, _vp, ct.c_int],
+    "qclhip_hilbert_get_path": [_vp, _vp, ct.POINTER(ct.c_int)],
     "qclhip_fft_hilbert": [_vp, _vp, _vp, _vp, _dp],
     "qclhip_retarded_self_energy": [_vp, _vp, _vp, _vp, _vp, _vp, _dp],
     "qclhip_embedding_component": [_vp, _vp, _vp, _vp, _vp, _vp],

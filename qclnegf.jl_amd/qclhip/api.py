@@ -371,6 +371,20 @@ class FFTHilbertPlan:
                       ct.byref(h))
         self.handle = h
 
+    PATHS = {"auto": 0, "lds": 1, "hbm": 2}
+
+    def set_path(self, path: str):
+        """'auto' (by L), 'lds' (one workgroup per column, L <= 16384) or
+        'hbm' (the four-step transform through HBM scratch, L >= 4096)."""
+        self.grid.ctx.call("qclhip_hilbert_set_path", self.handle, self.PATHS[path])
+        return self
+
+    @property
+    def path(self) -> str:
+        v = ct.c_int(0)
+        self.grid.ctx.call("qclhip_hilbert_get_path", self.handle, ct.byref(v))
+        return {1: "lds", 2: "hbm"}[v.value]
+
     def __del__(self):
         try:
             if self.handle:
