@@ -117,11 +117,15 @@ int qclhip_grid_destroy(qclhip_grid* grid);
 int qclhip_grid_set_precision(qclhip_ctx* ctx, qclhip_grid* grid, int precision,
                               int refine_steps, double refine_tol);
 /* Engine of the FP64 Keldysh products: vector ALUs (default) or the FP64
- * matrix cores. Results agree to rounding. */
+ * matrix cores (results agree to rounding), or FUSED: the FP64 Dyson step
+ * and the vector Keldysh products in one kernel per tile, G^R kept in LDS
+ * between them (bitwise the vector engine's results; at FP32 precisions the
+ * vector engine runs instead). */
 #define QCLHIP_ENGINE_VECTOR 0
 #define QCLHIP_ENGINE_MATRIX 1
+#define QCLHIP_ENGINE_FUSED 2
 int qclhip_grid_set_engine(qclhip_ctx* ctx, qclhip_grid* grid, int keldysh_engine);
-/* Autotuner: times both FP64 Keldysh engines and the tile shapes of the FP64
+/* Autotuner: times both FP64 Keldysh engines, the fused kernel and the tile shapes of the FP64
  * vector kernels (energies per workgroup: 4 or 8 for the Dyson kernel, 2 or
  * 4 for the Keldysh kernel, where the LDS of a CU holds two workgroups) on
  * synthetic blocks of the grid's shape (a sub-grid of at most 2048 energies)
@@ -130,9 +134,10 @@ int qclhip_grid_set_engine(qclhip_ctx* ctx, qclhip_grid* grid, int keldysh_engin
  * times (vector, matrix) in ms. Results do not depend on the choice.
  * Allocates temporary device memory; not during graph capture. */
 int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host);
-/* The autotuner's record for the grid's shape: ms_host[6] = (vector engine,
- * matrix engine, Dyson W=4, Dyson W=8, Keldysh W=2, Keldysh W=4; 0 where the
- * shape is not possible), choice_host[3] = (engine, Dyson W, Keldysh W).
+/* The autotuner's record for the grid's shape: ms_host[7] = (vector engine,
+ * matrix engine, Dyson W=4, Dyson W=8, Keldysh W=2, Keldysh W=4, fused
+ * Dyson + Keldysh; 0 where the shape is not possible), choice_host[3] =
+ * (engine, Dyson W, Keldysh W); the engine is the fastest launch set.
  * ArgumentError when the shape has not been tuned. */
 int qclhip_grid_tuning(qclhip_ctx* ctx, const qclhip_grid* grid, double* ms_host, int* choice_host);
 /* Sets the tile shapes by hand (0: the built-in rule); ArgumentError for a

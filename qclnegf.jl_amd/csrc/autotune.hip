@@ -1,7 +1,8 @@
 // Autotuner of the GF-solve launch configuration. For a grid shape it times
-// the FP64 Keldysh engines (vector ALUs, matrix cores) on synthetic blocks of
-// that shape and keeps the faster one for the grid, caching the choice per
-// (Nb, NE, Nk) in the context. Timing uses a sub-grid of at most 2048 energies
+// the FP64 Keldysh engines (vector ALUs, matrix cores), the tile shapes of
+// the vector kernels and the fused Dyson + Keldysh kernel on synthetic blocks
+// of that shape and keeps the fastest launch set for the grid, caching the
+// choice per (Nb, NE, Nk) in the context. Timing uses a sub-grid of at most 2048 energies
 // at the grid's Nk and Nb: the kernels' throughput per block depends on the
 // block size and occupancy, not on the number of tiles once the device is
 // full.
@@ -148,20 +149,42 @@ int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host) {
         if (dyson_waves_possible(N, dw[k])) rc = timed(0, dw[k], &msd[k]);
     for (int k = 0; k < 2 && !rc; ++k)
         if (keldysh_waves_possible(N, kw[k])) rc = timed(1, kw[k], &msk[k]);
+    // The fused Dyson + Keldysh kernel, against the two launches.
+    float msf = 0.f;
+    if (!rc) {
+        rc = launch_fused_fp64(ctx, a);  // warm-up
+        const int reps = 3;
+        hipEventRecord(e0, ctx->stream);
+        for (int k = 0; k < reps && !rc; ++k) rc = launch_fused_fp64(ctx, a);
+        hipEventRecord(e1, ctx->stream);
+        if (!rc && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&msf, e0, e1);
+        msf /= reps;
+    }
     if (!rc) rc = qclhip_ctx_synchronize(ctx);
     cleanup();
     if (rc) return rc;
-    const int engine = ms[1] < ms[0] ? QCLHIP_ENGINE_MATRIX : QCLHIP_ENGINE_VECTOR;
-    grid->keldysh_engine = engine;
     auto pick = [](const float* t, const int* w) {
         if (t[0] > 0.f && t[1] > 0.f) return t[1] < t[0] ? w[1] : w[0];
         if (t[0] <= 0.f && t[1] <= 0.f) return 0;  // neither shape exists: the built-in rule
         return t[1] > 0.f ? w[1] : w[0];
     };
+    auto best = [](const float* t, float fallback) {
+        float b = fallback;
+        for (int k = 0; k < 2; ++k)
+            if (t[k] > 0.f && t[k] < b) b = t[k];
+        return b;
+    };
     grid->dyson_waves = pick(msd, dw);
     grid->keldysh_waves = pick(msk, kw);
+    // Whole launch sets: the best Dyson tile with either Keldysh engine, or
+    // the fused kernel.
+    const float dy = best(msd, 3.0e38f);
+    const float t_vec = dy + best(msk, ms[0]), t_mat = dy + ms[1];
+    int engine = t_mat < t_vec ? QCLHIP_ENGINE_MATRIX : QCLHIP_ENGINE_VECTOR;
+    if (msf > 0.f && msf < std::min(t_vec, t_mat)) engine = QCLHIP_ENGINE_FUSED;
+    grid->keldysh_engine = engine;
     qclhip_ctx::TuneEntry te{grid->Nb, grid->NE, grid->Nk, engine, grid->dyson_waves, grid->keldysh_waves,
-                             ms[0], ms[1], {msd[0], msd[1]}, {msk[0], msk[1]}};
+                             ms[0], ms[1], {msd[0], msd[1]}, {msk[0], msk[1]}, msf};
     ctx->tune_cache.push_back(te);
     if (ms_host) {
         ms_host[0] = ms[0];
@@ -184,9 +207,9 @@ int qclhip_grid_tuning(qclhip_ctx* ctx, const qclhip_grid* grid, double* ms_host
     if (!ctx || !grid || !ms_host || !choice_host) return QCLHIP_ERR_ARGUMENT;
     for (const auto& t : ctx->tune_cache)
         if (t.Nb == grid->Nb && t.NE == grid->NE && t.Nk == grid->Nk) {
-            const double v[6] = {t.ms_vector, t.ms_matrix, t.ms_dyson[0], t.ms_dyson[1], t.ms_keldysh[0],
-                                 t.ms_keldysh[1]};
-            for (int k = 0; k < 6; ++k) ms_host[k] = v[k];
+            const double v[7] = {t.ms_vector, t.ms_matrix, t.ms_dyson[0], t.ms_dyson[1], t.ms_keldysh[0],
+                                 t.ms_keldysh[1], t.ms_fused};
+            for (int k = 0; k < 7; ++k) ms_host[k] = v[k];
             choice_host[0] = t.engine;
             choice_host[1] = t.dyson_waves;
             choice_host[2] = t.keldysh_waves;

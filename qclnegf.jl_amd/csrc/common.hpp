@@ -164,6 +164,7 @@ struct qclhip_ctx {
         int Nb, NE, Nk, engine, dyson_waves, keldysh_waves;
         float ms_vector, ms_matrix;
         float ms_dyson[2], ms_keldysh[2];  // Dyson W = 4, 8; Keldysh W = 2, 4 (0: not possible)
+        float ms_fused;                    // the fused Dyson + Keldysh kernel
     };
     std::vector<TuneEntry> tune_cache;
 };

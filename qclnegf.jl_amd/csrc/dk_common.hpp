@@ -295,6 +295,7 @@ int persistent_grid(qclhip_ctx* ctx, const void* kernel, int threads, size_t lds
 // Launchers (dyson.hip, mixed.hip).
 int launch_dyson_fp64(qclhip_ctx* ctx, const DKArgs& a);
 int launch_keldysh_valu(qclhip_ctx* ctx, const DKArgs& a);
+int launch_fused_fp64(qclhip_ctx* ctx, const DKArgs& a);  // k_dyson_keldysh
 int launch_keldysh_mfma(qclhip_ctx* ctx, const DKArgs& a, bool fp32);
 int launch_mixed_dyson(qclhip_ctx* ctx, const DKArgs& a);
 // Tile shapes the FP64 vector kernels can take for basis size N (dyson.hip).

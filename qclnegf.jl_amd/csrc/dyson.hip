@@ -46,16 +46,21 @@ struct DyShape {
 };
 
 // One tile (W consecutive energies of one momentum) of the FP64 Dyson step;
-// called by the single-instance and the batched kernels.
+// called by the single-instance, batched and fused kernels. The self-energy
+// tiles are loaded into in_tiles (whose wave tiles then hold the elimination
+// buffers) and the inverse is written to out_tiles (the same LDS for k_dyson;
+// the G^R tiles that the fused kernel's products read), then stored.
 template <int B, int W>
-__device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned char* smem, unsigned& s_ok) {
+__device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, c64* in_tiles, c64* out_tiles,
+                                  unsigned& s_ok) {
     constexpr int NP = DyShape<B, W>::NP;
     constexpr int LD = DyShape<B, W>::LD;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const long long nblk = (long long)p.NE * p.Nk;
     const int wstride = TileStride<B>::value;
-    c64* Ms = reinterpret_cast<c64*>(smem) + (size_t)wave * wstride;
+    c64* Ms = in_tiles + (size_t)wave * wstride;
+    c64* Os = out_tiles + (size_t)wave * wstride;
     c64* colbuf = Ms;
     c64* rowbuf = colbuf + NP;
     int* pivrow = reinterpret_cast<int*>(rowbuf + NP);
@@ -68,11 +73,11 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
     const int e = e0 + wave;
     const long long blk = blk0 + wave;
     if (threadIdx.x == 0) s_ok = 0;
-    if (p.nfamR) tile_load<W, LD>(reinterpret_cast<c64*>(smem), wstride, p.famR, p.nfamR, blk0, nvalid, N, S);
+    if (p.nfamR) tile_load<W, LD>(in_tiles, wstride, p.famR, p.nfamR, blk0, nvalid, N, S);
     __syncthreads();
     if (p.totR && p.nfamR) {
         const unsigned vmask = nvalid >= 32 ? 0xffffffffu : ((1u << nvalid) - 1u);
-        tile_store<W, LD>(reinterpret_cast<const c64*>(smem), wstride, p.totR, blk0, vmask, N, S);
+        tile_store<W, LD>(in_tiles, wstride, p.totR, blk0, vmask, N, S);
     }
     bool ok = wave < nvalid;
     double s = 0.0, rs = 0.0;
@@ -223,7 +228,7 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
 #pragma unroll
             for (int c = 0; c < B; ++c) {
                 const int i = lr * B + r, j = lc * B + c;
-                if (i < N && j < N) Ms[si[r] * LD + pj[c]] = cmk(A[r][c].x * rs, A[r][c].y * rs);
+                if (i < N && j < N) Os[si[r] * LD + pj[c]] = cmk(A[r][c].x * rs, A[r][c].y * rs);
             }
         if (lane == 0) {
             atomicOr(&s_ok, 1u << wave);
@@ -231,7 +236,7 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, unsigned
         }
     }
     __syncthreads();
-    tile_store<W, LD>(reinterpret_cast<const c64*>(smem), wstride, p.GR, blk0, s_ok, N, S);
+    tile_store<W, LD>(out_tiles, wstride, p.GR, blk0, s_ok, N, S);
 }
 
 // One tile per workgroup. With block_flags set (the FP64 re-solve of the
@@ -246,7 +251,8 @@ __global__ __launch_bounds__(64 * W) void k_dyson(DKArgs p, long long ntiles) {
         for (int w = 0; w < tc.nvalid; ++w) any |= p.block_flags[tc.blk0 + w];
         if (!any) return;
     }
-    dyson_tile<B, W>(p, tc, smem, s_ok);
+    c64* tiles = reinterpret_cast<c64*>(smem);
+    dyson_tile<B, W>(p, tc, tiles, tiles, s_ok);
 }
 
 __device__ inline int batch_find(const BatchClass& c, long long g) {
@@ -276,7 +282,8 @@ __global__ __launch_bounds__(64 * W) void k_dyson_batch(BatchClass c) {
     tc.m = (int)(t / nte);
     tc.nvalid = min(W, p.NE - tc.e0);
     tc.blk0 = tc.e0 + (long long)p.NE * tc.m;
-    dyson_tile<B, W>(p, tc, smem, s_ok);
+    c64* tiles = reinterpret_cast<c64*>(smem);
+    dyson_tile<B, W>(p, tc, tiles, tiles, s_ok);
 }
 
 // Keldysh products (A2): G<raw = G^R S< G^R^dagger and G>raw likewise, plus
@@ -398,6 +405,57 @@ __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) 
             // barrier at the top of the loop does this
             if (comp == 0) __syncthreads();
         }
+    }
+}
+
+// Dyson and Keldysh in one pass (QCLHIP_ENGINE_FUSED), one tile per
+// workgroup: the self-energy tile goes into the X tiles, the elimination
+// leaves G^R in the G tiles (stored once, as k_dyson stores it), and the
+// Keldysh products of k_keldysh read it from there, so G^R is never read
+// back from HBM. Same arithmetic as k_dyson + k_keldysh: the results are
+// bitwise those of the two launches. The two LDS tiles per wave hold the
+// elimination to two waves per SIMD at N <= 24, where k_dyson runs four;
+// the autotuner weighs that against the saved G^R traffic per shape.
+template <int B, int W>
+__global__ __launch_bounds__(64 * W) void k_dyson_keldysh(DKArgs p, long long ntiles) {
+    constexpr int LD = DKShape<B>::LD;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ unsigned s_ok;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const long long nblk = (long long)p.NE * p.Nk;
+    const int wstride = TileStride<B>::value;
+    c64* G0 = reinterpret_cast<c64*>(smem);
+    c64* X0 = G0 + (size_t)W * wstride;
+    c64* Gs = G0 + (size_t)wave * wstride;
+    c64* Xs = X0 + (size_t)wave * wstride;
+    const int N = p.N;
+    const size_t S = p.plane ? (size_t)p.plane : (size_t)nblk;
+    const TileCoord tc = tile_coord<W>(blockIdx.x, ntiles, p.NE);
+    dyson_tile<B, W>(p, tc, X0, G0, s_ok);  // ends with a barrier and the G^R store from G0: X0 is free
+    const int nvalid = tc.nvalid;
+    const long long blk0 = tc.blk0;
+    const long long blk = blk0 + wave;
+    const bool active = wave < nvalid;
+    const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
+    tile_load<W, LD>(X0, wstride, p.famL, p.nfamL, blk0, nvalid, N, S);
+    __syncthreads();
+    if (p.A) tile_store_spectral<W, LD>(G0, wstride, p.A, blk0, mask, N, S);  // G0 stays read-only
+    for (int comp = 0; comp < 2; ++comp) {
+        c64* dst = comp ? p.GG : p.GL;
+        if (comp == 1) {
+            tile_load<W, LD>(X0, wstride, p.famG, p.nfamG, blk0, nvalid, N, S);
+            __syncthreads();
+        }
+        c64* tot = comp ? p.totG : p.totL;
+        if (tot) {
+            tile_store<W, LD>(X0, wstride, tot, blk0, mask, N, S);
+            __syncthreads();  // the products overwrite the tiles
+        }
+        if (active) keldysh_products<B>(Gs, Xs, N, lane, p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr);
+        __syncthreads();
+        tile_store<W, LD>(X0, wstride, dst, blk0, mask, N, S);
+        if (comp == 0) __syncthreads();  // before S> replaces the tile
     }
 }
 
@@ -865,6 +923,36 @@ int launch_keldysh(qclhip_ctx* ctx, const DKArgs& a) {
     return launch_keldysh_w<B, W>(ctx, a);
 }
 
+// The fused kernel: the Keldysh kernel's two tiles per wave and its
+// energies per workgroup (4 while two workgroups fit a CU's LDS), one tile
+// per workgroup as k_dyson.
+template <int B>
+int launch_fused(qclhip_ctx* ctx, const DKArgs& a) {
+    constexpr size_t per_wave = 2 * (size_t)TileStride<B>::value * sizeof(c64);
+    constexpr int W = tile_waves<B>(per_wave);
+    static_assert(DyShape<B, W>::per_wave <= per_wave / 2, "elimination buffers exceed the X tile");
+    const size_t lds = per_wave * W;
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_dyson_keldysh<B, W>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
+    hipLaunchKernelGGL((k_dyson_keldysh<B, W>), dim3((unsigned)tiles), dim3(64 * W), lds, ctx->stream, a, tiles);
+    return launch_check(ctx, "k_dyson_keldysh");
+}
+
+int launch_fused_fp64(qclhip_ctx* ctx, const DKArgs& a) {
+    const int N = a.N;
+    if (N <= 8) return launch_fused<1>(ctx, a);
+    if (N <= 16) return launch_fused<2>(ctx, a);
+    if (N <= 24) return launch_fused<3>(ctx, a);
+    if (N <= 32) return launch_fused<4>(ctx, a);
+    if (N <= 40) return launch_fused<5>(ctx, a);
+    if (N <= 48) return launch_fused<6>(ctx, a);
+    return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
+}
+
 template <int B>
 int launch_sw(qclhip_ctx* ctx, const SWArgs& a) {
     constexpr size_t per_wave = (size_t)TileStride<B>::value * sizeof(c64);
@@ -968,6 +1056,7 @@ int dyson_keldysh_dispatch(qclhip_ctx* ctx, const qclhip_grid* grid, DKArgs a, b
         if (grid->precision == QCLHIP_PRECISION_FP32_PRODUCTS) return launch_keldysh_mfma(ctx, a, true);
     } else {
         a.block_flags = nullptr;
+        if (keldysh && grid->keldysh_engine == QCLHIP_ENGINE_FUSED) return launch_fused_fp64(ctx, a);
         rc = launch_dyson_fp64(ctx, a);
         if (rc || !keldysh) return rc;
     }
@@ -1121,6 +1210,7 @@ static int dyson_keldysh_impl(qclhip_ctx* ctx, const qclhip_grid* grid, long lon
     a.plane = plane;
     a.wd = grid->dyson_waves;
     a.wk = grid->keldysh_waves;
+    if (keldysh && grid->keldysh_engine == QCLHIP_ENGINE_FUSED) return launch_fused_fp64(ctx, a);
     int rc = launch_dyson_fp64(ctx, a);
     if (rc || !keldysh) return rc;
     return launch_keldysh_valu(ctx, a);
@@ -1216,7 +1306,7 @@ static int current_green_impl(qclhip_ctx* ctx, const qclhip_grid* grid, const do
     a.keldysh = 1;
     if (totals) {
         const bool in_kernels = fused && grid->precision == QCLHIP_PRECISION_FP64 &&
-                                grid->keldysh_engine == QCLHIP_ENGINE_VECTOR && a.N <= 48;
+                                grid->keldysh_engine != QCLHIP_ENGINE_MATRIX && a.N <= 48;
         if (in_kernels) {
             a.totR = (c64*)totals[0];
             a.totL = (c64*)totals[1];
@@ -1240,7 +1330,7 @@ static int current_green_impl(qclhip_ctx* ctx, const qclhip_grid* grid, const do
     }
     // A from the Keldysh kernel's G^R tiles on the FP64 vector path, else k_spectral.
     const bool a_in_kernel = A && !scalar_lesser && grid->precision == QCLHIP_PRECISION_FP64 &&
-                             grid->keldysh_engine == QCLHIP_ENGINE_VECTOR && a.N <= 48;
+                             grid->keldysh_engine != QCLHIP_ENGINE_MATRIX && a.N <= 48;
     if (a_in_kernel) a.A = (c64*)A;
     int rc = dyson_keldysh_dispatch(ctx, grid, a, true);
     if (rc) return rc;

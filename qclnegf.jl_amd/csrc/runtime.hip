@@ -332,8 +332,9 @@ int qclhip_grid_set_precision(qclhip_ctx* ctx, qclhip_grid* g, int precision, in
 
 int qclhip_grid_set_engine(qclhip_ctx* ctx, qclhip_grid* g, int keldysh_engine) {
     if (!ctx || !g) return QCLHIP_ERR_ARGUMENT;
-    if (keldysh_engine != QCLHIP_ENGINE_VECTOR && keldysh_engine != QCLHIP_ENGINE_MATRIX)
-        return fail(ctx, QCLHIP_ERR_ARGUMENT, "engine must be QCLHIP_ENGINE_VECTOR or _MATRIX");
+    if (keldysh_engine != QCLHIP_ENGINE_VECTOR && keldysh_engine != QCLHIP_ENGINE_MATRIX &&
+        keldysh_engine != QCLHIP_ENGINE_FUSED)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "engine must be QCLHIP_ENGINE_VECTOR, _MATRIX or _FUSED");
     g->keldysh_engine = keldysh_engine;
     return QCLHIP_OK;
 }

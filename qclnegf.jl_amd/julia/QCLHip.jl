@@ -454,8 +454,15 @@ function set_precision!(g::Grid, precision::Symbol; refine_steps::Integer = 2, r
     return g
 end
 
-set_engine!(g::Grid, engine::Symbol) = (check(g.ctx, ccall((:qclhip_grid_set_engine, libqclhip), Cint,
-    (Ptr{Cvoid}, Ptr{Cvoid}, Cint), g.ctx.handle, g.handle, engine === :matrix ? 1 : 0), "set_engine"); g)
+"""Keldysh engine: `:vector`, `:matrix` or `:fused` (Dyson and the vector
+products in one kernel)."""
+function set_engine!(g::Grid, engine::Symbol)
+    code = Dict(:vector => 0, :matrix => 1, :fused => 2)
+    haskey(code, engine) || throw(ArgumentError("unknown engine $engine"))
+    check(g.ctx, ccall((:qclhip_grid_set_engine, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint), g.ctx.handle,
+                       g.handle, code[engine]), "set_engine")
+    return g
+end
 
 set_tiles!(g::Grid, dyson_waves::Integer, keldysh_waves::Integer) = (check(g.ctx, ccall((:qclhip_grid_set_tiles,
     libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint), g.ctx.handle, g.handle, dyson_waves, keldysh_waves),
@@ -467,11 +474,12 @@ function autotune!(g::Grid)
     ms = zeros(Float64, 2)
     check(g.ctx, ccall((:qclhip_grid_autotune, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}),
                        g.ctx.handle, g.handle, ms), "autotune")
-    t = zeros(Float64, 6)
+    t = zeros(Float64, 7)
     ch = zeros(Cint, 3)
     check(g.ctx, ccall((:qclhip_grid_tuning, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Cint}),
                        g.ctx.handle, g.handle, t, ch), "grid_tuning")
-    return (engine = ch[1] == 1 ? :matrix : :vector, dyson_waves = Int(ch[2]), keldysh_waves = Int(ch[3]), ms = t)
+    return (engine = (:vector, :matrix, :fused)[ch[1] + 1], dyson_waves = Int(ch[2]), keldysh_waves = Int(ch[3]),
+            ms = t)
 end
 
 # -- deferred results and graphs ---------------------------------------------

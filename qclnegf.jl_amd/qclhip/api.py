@@ -222,6 +222,7 @@ class Grid:
     PRECISION_FP32_PRODUCTS = 2
     ENGINE_VECTOR = 0
     ENGINE_MATRIX = 1
+    ENGINE_FUSED = 2
 
     def set_precision(self, precision: str = "fp64", refine_steps: int = 2, refine_tol: float = 0.0):
         """GF-solve precision of this grid: "fp64" (the reference's),
@@ -238,8 +239,9 @@ class Grid:
         return self
 
     def set_engine(self, engine: str = "vector"):
-        """Engine of the FP64 Keldysh products: "vector" or "matrix" cores."""
-        code = {"vector": self.ENGINE_VECTOR, "matrix": self.ENGINE_MATRIX}.get(engine)
+        """Engine of the FP64 Keldysh products: "vector" or "matrix" cores, or
+        "fused" (the FP64 Dyson step and the vector products in one kernel)."""
+        code = {"vector": self.ENGINE_VECTOR, "matrix": self.ENGINE_MATRIX, "fused": self.ENGINE_FUSED}.get(engine)
         if code is None:
             raise _lib.ArgumentError(f"unknown engine {engine!r}")
         self.ctx.call("qclhip_grid_set_engine", self.handle, code)
@@ -253,12 +255,12 @@ class Grid:
         (ms) and the choices."""
         ms = (ct.c_double * 2)()
         self.ctx.call("qclhip_grid_autotune", self.handle, ms)
-        t = (ct.c_double * 6)()
+        t = (ct.c_double * 7)()
         ch = (ct.c_int * 3)()
         self.ctx.call("qclhip_grid_tuning", self.handle, t, ch)
-        self.engine = "matrix" if ch[0] == 1 else "vector"
+        self.engine = {0: "vector", 1: "matrix", 2: "fused"}[ch[0]]
         return dict(vector_ms=t[0], matrix_ms=t[1], engine=self.engine, dyson_w4_ms=t[2], dyson_w8_ms=t[3],
-                    dyson_waves=ch[1], keldysh_w2_ms=t[4], keldysh_w4_ms=t[5], keldysh_waves=ch[2])
+                    dyson_waves=ch[1], keldysh_w2_ms=t[4], keldysh_w4_ms=t[5], keldysh_waves=ch[2], fused_ms=t[6])
 
     def set_tiles(self, dyson_waves: int = 0, keldysh_waves: int = 0):
         """Energies per workgroup of the FP64 vector Dyson (4, 8) and Keldysh

@@ -187,7 +187,8 @@ def test_autotune_selects_engine_and_keeps_parity(hip, orc):
     w, SR, SL, SG = _dense_case(24, NE=64, Nk=4)
     g = hip.Grid(w["eps"], w["wE"], w["wk"], 24)
     t = g.autotune()
-    assert t["vector_ms"] > 0 and t["matrix_ms"] > 0 and t["engine"] in ("vector", "matrix")
+    assert t["vector_ms"] > 0 and t["matrix_ms"] > 0 and t["fused_ms"] > 0
+    assert t["engine"] in ("vector", "matrix", "fused")
     g2 = hip.Grid(w["eps"], w["wE"], w["wk"], 24)
     assert g2.autotune() == t  # cached per shape in the context
     GR_o, _ = orc.retarded_green(w["eps"], w["h"], SR)

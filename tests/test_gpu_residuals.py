@@ -274,7 +274,7 @@ def test_mix_fields_residual_totals(hip):
                                 totals=[(tots[0], [1, 0, 2]), (tots[1], [3, 4, 5, 6])])
 
 
-@pytest.mark.parametrize("engine", ["vector", "matrix"])
+@pytest.mark.parametrize("engine", ["vector", "matrix", "fused"])
 def test_current_green_totals(hip, engine):
     """qclhip_current_green_totals: the Green functions equal those of
     qclhip_current_green bitwise, and the totals equal the family sums
