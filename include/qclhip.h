@@ -134,12 +134,21 @@ int qclhip_grid_set_engine(qclhip_ctx* ctx, qclhip_grid* grid, int keldysh_engin
  * times (vector, matrix) in ms. Results do not depend on the choice.
  * Allocates temporary device memory; not during graph capture. */
 int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host);
-/* The autotuner's record for the grid's shape: ms_host[7] = (vector engine,
+/* The autotuner's record for the grid's shape: ms_host[8] = (vector engine,
  * matrix engine, Dyson W=4, Dyson W=8, Keldysh W=2, Keldysh W=4, fused
- * Dyson + Keldysh; 0 where the shape is not possible), choice_host[3] =
- * (engine, Dyson W, Keldysh W); the engine is the fastest launch set.
+ * Dyson + Keldysh, vector Keldysh at the chosen waves per block; 0 where the
+ * shape is not possible), choice_host[4] = (engine, Dyson W, Keldysh W,
+ * Keldysh waves per block); the engine is the fastest launch set.
  * ArgumentError when the shape has not been tuned. */
 int qclhip_grid_tuning(qclhip_ctx* ctx, const qclhip_grid* grid, double* ms_host, int* choice_host);
+/* Waves per block of the FP64 vector Keldysh kernel (0: the built-in rule,
+ * one): S > 1 splits a block's rows over S waves that share its LDS tiles,
+ * for N > 24 where two tiles per wave would leave one wave per SIMD.
+ * Compiled: 2 or 4 for N in 25..32, 5 for 33..40, 2, 3 or 6 for 41..48;
+ * ArgumentError otherwise. Results are bitwise those of one wave per block
+ * (same per-element arithmetic and trace order). No counterpart in the
+ * reference. */
+int qclhip_grid_set_keldysh_split(qclhip_ctx* ctx, qclhip_grid* grid, int waves_per_block);
 /* Sets the tile shapes by hand (0: the built-in rule); ArgumentError for a
  * shape that is not compiled or does not fit the LDS for the basis size. */
 int qclhip_grid_set_tiles(qclhip_ctx* ctx, qclhip_grid* grid, int dyson_waves, int keldysh_waves);

@@ -61,6 +61,7 @@ int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host) {
             grid->keldysh_engine = t.engine;
             grid->dyson_waves = t.dyson_waves;
             grid->keldysh_waves = t.keldysh_waves;
+            grid->keldysh_split = t.keldysh_split;
             if (ms_host) {
                 ms_host[0] = t.ms_vector;
                 ms_host[1] = t.ms_matrix;
@@ -149,6 +150,34 @@ int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host) {
         if (dyson_waves_possible(N, dw[k])) rc = timed(0, dw[k], &msd[k]);
     for (int k = 0; k < 2 && !rc; ++k)
         if (keldysh_waves_possible(N, kw[k])) rc = timed(1, kw[k], &msk[k]);
+    // Waves per block of the vector Keldysh kernel (rows of a block split
+    // over 1..6 waves where compiled), at the chosen energies per workgroup.
+    int best_ks = 1;
+    float ms_ks = 0.f;
+    {
+        const int kw_best = [&]() {
+            if (msk[0] > 0.f && msk[1] > 0.f) return msk[1] < msk[0] ? kw[1] : kw[0];
+            return msk[1] > 0.f ? kw[1] : (msk[0] > 0.f ? kw[0] : 0);
+        }();
+        for (int ks = 1; ks <= 6 && !rc; ++ks) {
+            if (!keldysh_split_possible(N, ks)) continue;
+            DKArgs b = a;
+            b.wk = kw_best;
+            b.ks = ks;
+            rc = launch_keldysh_valu(ctx, b);  // warm-up
+            const int reps = 3;
+            hipEventRecord(e0, ctx->stream);
+            for (int k = 0; k < reps && !rc; ++k) rc = launch_keldysh_valu(ctx, b);
+            hipEventRecord(e1, ctx->stream);
+            float t = 0.f;
+            if (!rc && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&t, e0, e1);
+            t /= reps;
+            if (ms_ks == 0.f || t < ms_ks) {
+                ms_ks = t;
+                best_ks = ks;
+            }
+        }
+    }
     // The fused Dyson + Keldysh kernel, against the two launches.
     float msf = 0.f;
     if (!rc) {
@@ -179,12 +208,24 @@ int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host) {
     // Whole launch sets: the best Dyson tile with either Keldysh engine, or
     // the fused kernel.
     const float dy = best(msd, 3.0e38f);
-    const float t_vec = dy + best(msk, ms[0]), t_mat = dy + ms[1];
+    const float t_vec = dy + std::min(best(msk, ms[0]), ms_ks > 0.f ? ms_ks : 3.0e38f), t_mat = dy + ms[1];
     int engine = t_mat < t_vec ? QCLHIP_ENGINE_MATRIX : QCLHIP_ENGINE_VECTOR;
     if (msf > 0.f && msf < std::min(t_vec, t_mat)) engine = QCLHIP_ENGINE_FUSED;
     grid->keldysh_engine = engine;
-    qclhip_ctx::TuneEntry te{grid->Nb, grid->NE, grid->Nk, engine, grid->dyson_waves, grid->keldysh_waves,
-                             ms[0], ms[1], {msd[0], msd[1]}, {msk[0], msk[1]}, msf};
+    grid->keldysh_split = best_ks;
+    qclhip_ctx::TuneEntry te{grid->Nb,
+                             grid->NE,
+                             grid->Nk,
+                             engine,
+                             grid->dyson_waves,
+                             grid->keldysh_waves,
+                             grid->keldysh_split,
+                             ms[0],
+                             ms[1],
+                             {msd[0], msd[1]},
+                             {msk[0], msk[1]},
+                             msf,
+                             ms_ks};
     ctx->tune_cache.push_back(te);
     if (ms_host) {
         ms_host[0] = ms[0];
@@ -203,16 +244,25 @@ int qclhip_grid_set_tiles(qclhip_ctx* ctx, qclhip_grid* grid, int dyson_waves, i
     return QCLHIP_OK;
 }
 
+int qclhip_grid_set_keldysh_split(qclhip_ctx* ctx, qclhip_grid* grid, int waves_per_block) {
+    if (!ctx || !grid) return QCLHIP_ERR_ARGUMENT;
+    if (waves_per_block && !keldysh_split_possible(grid->Nb, waves_per_block))
+        return fail(ctx, QCLHIP_ERR_ARGUMENT, "waves per block not available for this basis size");
+    grid->keldysh_split = waves_per_block;
+    return QCLHIP_OK;
+}
+
 int qclhip_grid_tuning(qclhip_ctx* ctx, const qclhip_grid* grid, double* ms_host, int* choice_host) {
     if (!ctx || !grid || !ms_host || !choice_host) return QCLHIP_ERR_ARGUMENT;
     for (const auto& t : ctx->tune_cache)
         if (t.Nb == grid->Nb && t.NE == grid->NE && t.Nk == grid->Nk) {
-            const double v[7] = {t.ms_vector, t.ms_matrix, t.ms_dyson[0], t.ms_dyson[1], t.ms_keldysh[0],
-                                 t.ms_keldysh[1], t.ms_fused};
-            for (int k = 0; k < 7; ++k) ms_host[k] = v[k];
+            const double v[8] = {t.ms_vector, t.ms_matrix, t.ms_dyson[0], t.ms_dyson[1], t.ms_keldysh[0],
+                                 t.ms_keldysh[1], t.ms_fused, t.ms_split};
+            for (int k = 0; k < 8; ++k) ms_host[k] = v[k];
             choice_host[0] = t.engine;
             choice_host[1] = t.dyson_waves;
             choice_host[2] = t.keldysh_waves;
+            choice_host[3] = t.keldysh_split;
             return QCLHIP_OK;
         }
     return fail(ctx, QCLHIP_ERR_ARGUMENT, "this grid shape has not been autotuned");

@@ -161,10 +161,11 @@ struct qclhip_ctx {
     // Autotune cache: (Nb, NE, Nk) -> chosen Keldysh engine and the two
     // measured times (ms).
     struct TuneEntry {
-        int Nb, NE, Nk, engine, dyson_waves, keldysh_waves;
+        int Nb, NE, Nk, engine, dyson_waves, keldysh_waves, keldysh_split;
         float ms_vector, ms_matrix;
         float ms_dyson[2], ms_keldysh[2];  // Dyson W = 4, 8; Keldysh W = 2, 4 (0: not possible)
         float ms_fused;                    // the fused Dyson + Keldysh kernel
+        float ms_split;                    // the vector Keldysh kernel at the chosen waves per block
     };
     std::vector<TuneEntry> tune_cache;
 };
@@ -186,6 +187,7 @@ struct qclhip_grid {
     // Energies per workgroup of the FP64 vector Dyson / Keldysh kernels (0:
     // built-in rule), set by qclhip_grid_autotune.
     int dyson_waves = 0, keldysh_waves = 0;
+    int keldysh_split = 0;  // waves per block of the FP64 vector Keldysh kernel (0: the rule)
 };
 
 struct qclhip_shift {

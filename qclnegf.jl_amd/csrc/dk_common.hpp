@@ -66,8 +66,9 @@ struct DKArgs {
     c64* A;
     // Energies per workgroup of the FP64 vector Dyson and Keldysh kernels
     // (0: the built-in rule; else one of the compiled shapes, chosen by the
-    // autotuner).
-    int wd, wk;
+    // autotuner), and waves per block of the vector Keldysh kernel (0: the
+    // rule; S > 1 splits a block's rows over S waves, for N > 24).
+    int wd, wk, ks;
 };
 
 // out[r][c] (lane block) = Gs[rows] * Xs[:, cols]  or  Xs[rows] * Gs[cols]^dagger
@@ -119,8 +120,9 @@ __device__ inline c64 fam_load(const c64* const* fam, int nfam, size_t idx) {
 // one momentum. Thread f handles element (i,j) of energy w with
 // f = w + W*(i + N*j): consecutive threads touch consecutive energies, so one
 // instruction moves contiguous 16*W-byte segments. Wave w's matrix sits at
-// lds + w*wstride, element (i,j) at i*LD + j.
-template <int W, int LD>
+// lds + w*wstride, element (i,j) at i*LD + j. T: threads of the workgroup
+// (64 W, or 64 S W when S waves share a block).
+template <int W, int LD, int T = 64 * W>
 __device__ inline void tile_load(c64* lds, int wstride, const c64* const* fam, int nfam, long long blk0,
                                  int nvalid, int N, size_t S) {
     const int total = N * N * W;
@@ -128,9 +130,9 @@ __device__ inline void tile_load(c64* lds, int wstride, const c64* const* fam, i
     const int w = f % W;
     int ij = f / W;
     int i = ij % N, j = ij / N;
-    for (; f < total; f += 64 * W) {
+    for (; f < total; f += T) {
         if (w < nvalid) lds[w * wstride + i * LD + j] = fam_load(fam, nfam, blk0 + w + S * (i + (size_t)N * j));
-        i += 64;
+        i += T / W;
         while (i >= N) {
             i -= N;
             ++j;
@@ -138,7 +140,7 @@ __device__ inline void tile_load(c64* lds, int wstride, const c64* const* fam, i
     }
 }
 
-template <int W, int LD, bool ACC = false>
+template <int W, int LD, bool ACC = false, int T = 64 * W>
 __device__ inline void tile_store(const c64* lds, int wstride, c64* dst, long long blk0, unsigned wave_mask, int N,
                                   size_t S) {
     const int total = N * N * W;
@@ -147,13 +149,13 @@ __device__ inline void tile_store(const c64* lds, int wstride, c64* dst, long lo
     int ij = f / W;
     int i = ij % N, j = ij / N;
     const bool on = (wave_mask >> w) & 1u;
-    for (; f < total; f += 64 * W) {
+    for (; f < total; f += T) {
         if (on) {
             const size_t idx = blk0 + w + S * (i + (size_t)N * j);
             const c64 v = lds[w * wstride + i * LD + j];
             dst[idx] = ACC ? cadd(dst[idx], v) : v;
         }
-        i += 64;
+        i += T / W;
         while (i >= N) {
             i -= N;
             ++j;
@@ -163,7 +165,7 @@ __device__ inline void tile_store(const c64* lds, int wstride, c64* dst, long lo
 
 // A = i (G - G^dagger) of the W per-wave G tiles, stored with the thread
 // mapping of tile_store (the arithmetic of k_spectral).
-template <int W, int LD>
+template <int W, int LD, int T = 64 * W>
 __device__ inline void tile_store_spectral(const c64* lds, int wstride, c64* dst, long long blk0, unsigned wave_mask,
                                            int N, size_t S) {
     const int total = N * N * W;
@@ -172,13 +174,13 @@ __device__ inline void tile_store_spectral(const c64* lds, int wstride, c64* dst
     int ij = f / W;
     int i = ij % N, j = ij / N;
     const bool on = (wave_mask >> w) & 1u;
-    for (; f < total; f += 64 * W) {
+    for (; f < total; f += T) {
         if (on) {
             const c64 g = lds[w * wstride + i * LD + j], gt = lds[w * wstride + j * LD + i];
             const double dx = g.x - gt.x, dy = g.y + gt.y;  // G - conj(G^T)
             dst[blk0 + w + S * (i + (size_t)N * j)] = cmk(-dy, dx);
         }
-        i += 64;
+        i += T / W;
         while (i >= N) {
             i -= N;
             ++j;
@@ -301,6 +303,7 @@ int launch_mixed_dyson(qclhip_ctx* ctx, const DKArgs& a);
 // Tile shapes the FP64 vector kernels can take for basis size N (dyson.hip).
 bool dyson_waves_possible(int N, int W);
 bool keldysh_waves_possible(int N, int W);
+bool keldysh_split_possible(int N, int S);  // waves per block of the vector Keldysh kernel
 // out[e, col] (+)= alpha * (S^T Y)[e, col] over ncols columns (adjoint.hip).
 int shift_transpose(qclhip_ctx* ctx, const qclhip_shift* s, long long ncols, const c64* Y, double alpha,
                     int accumulate, c64* out);

@@ -325,6 +325,144 @@ __device__ inline void keldysh_products(c64* Gs, c64* Xs, int N, int lane, c64* 
     }
 }
 
+// Keldysh products of one block shared by SPLIT waves, for the large blocks
+// whose two LDS tiles per wave would hold a CU to one wave per SIMD. Wave s
+// of the block computes rows [s NP/SPLIT, (s+1) NP/SPLIT) of T = G X and of
+// T G^dagger, each lane a (B/SPLIT) x B register block, with block_product's
+// per-element arithmetic (FMAs in q order), so G< and G> are bitwise those
+// of keldysh_products. The block trace gathers the diagonal through diag and
+// one lane sums it as keldysh_products does: per diagonal group of B entries
+// in order, then the 64-leaf xor-butterfly tree over the lanes of the 8x8
+// grid (group g sits at lane 9g, every other leaf is +0), so the traces are
+// bitwise equal too. Every thread of the workgroup calls it (workgroup
+// barriers); active marks the waves with a valid block.
+template <int B, int SPLIT, bool ADJ>
+__device__ inline void block_product_rows(const c64* __restrict__ L, const c64* __restrict__ R, int N, int s, int lr,
+                                          int lc, c64 (&out)[B / SPLIT][B]) {
+    constexpr int LD = DKShape<B>::LD;
+    constexpr int BR = B / SPLIT;
+    const int row0 = s * 8 * BR + lr * BR;
+#pragma unroll
+    for (int r = 0; r < BR; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) out[r][c] = cmk(0.0, 0.0);
+#pragma unroll 2
+    for (int q = 0; q < N; ++q) {
+        c64 lv[BR], rv[B];
+#pragma unroll
+        for (int r = 0; r < BR; ++r) lv[r] = L[(row0 + r) * LD + q];
+#pragma unroll
+        for (int c = 0; c < B; ++c) rv[c] = ADJ ? R[(lc * B + c) * LD + q] : R[q * LD + lc * B + c];
+#pragma unroll
+        for (int r = 0; r < BR; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c)
+                out[r][c] = ADJ ? cfmac(out[r][c], lv[r], rv[c]) : cfma(out[r][c], lv[r], rv[c]);
+    }
+}
+
+template <int B, int SPLIT>
+__device__ inline void keldysh_products_split(c64* Gs, c64* Xs, c64* diag, int N, int lane, int s, bool active,
+                                              c64* trace) {
+    static_assert(B % SPLIT == 0, "the rows of a block split evenly over its waves");
+    constexpr int LD = DKShape<B>::LD;
+    constexpr int BR = B / SPLIT;
+    const int lr = lane & 7, lc = lane >> 3;
+    const int row0 = s * 8 * BR + lr * BR;
+    c64 T[BR][B];
+    if (active) block_product_rows<B, SPLIT, false>(Gs, Xs, N, s, lr, lc, T);
+    __syncthreads();  // every wave of the block has read X before T replaces it
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < BR; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) Xs[(row0 + r) * LD + lc * B + c] = T[r][c];
+    }
+    wave_sync();  // T G^dagger reads only this wave's rows of T
+    if (active) block_product_rows<B, SPLIT, true>(Xs, Gs, N, s, lr, lc, T);
+    wave_sync();
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < BR; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int i = row0 + r, j = lc * B + c;
+                Xs[i * LD + j] = T[r][c];
+                if (diag && i == j) diag[i] = T[r][c];
+            }
+    }
+    __syncthreads();
+    if (trace && active && s == 0 && lane == 0) {
+        c64 leaf[64];
+#pragma unroll
+        for (int l = 0; l < 64; ++l) leaf[l] = cmk(0.0, 0.0);
+#pragma unroll
+        for (int g = 0; g < 8; ++g) {
+            c64 t = cmk(0.0, 0.0);
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+                if (g * B + r < N) t = cadd(t, diag[g * B + r]);
+            leaf[9 * g] = t;
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+            for (int l = 0; l < 64; l += 2 * off) leaf[l] = cadd(leaf[l], leaf[l + off]);
+        *trace = leaf[0];
+    }
+}
+
+// The Keldysh products with SPLIT waves per block (W blocks per workgroup),
+// no register prefetch; otherwise the k_keldysh loop.
+template <int B, int W, int SPLIT>
+__global__ __launch_bounds__(64 * SPLIT * W) void k_keldysh_split(DKArgs p, long long ntiles) {
+    constexpr int LD = DKShape<B>::LD;
+    constexpr int T = 64 * SPLIT * W;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ c64 diag[W][8 * B];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int b = wave / SPLIT, s = wave % SPLIT;
+    const long long nblk = (long long)p.NE * p.Nk;
+    const int wstride = TileStride<B>::value;
+    c64* G0 = reinterpret_cast<c64*>(smem);
+    c64* X0 = G0 + (size_t)W * wstride;
+    c64* Gs = G0 + (size_t)b * wstride;
+    c64* Xs = X0 + (size_t)b * wstride;
+    const int N = p.N;
+    const size_t S = p.plane ? (size_t)p.plane : (size_t)nblk;
+    const c64* gr_fam[1] = {p.GR};
+    for (long long d = blockIdx.x; d < ntiles; d += gridDim.x) {
+        const TileCoord tc = tile_coord<W>(d, ntiles, p.NE);
+        const int nvalid = tc.nvalid;
+        const long long blk0 = tc.blk0;
+        const long long blk = blk0 + b;
+        const bool active = b < nvalid;
+        const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
+        __syncthreads();  // the previous tile's LDS is free
+        tile_load<W, LD, T>(G0, wstride, gr_fam, 1, blk0, nvalid, N, S);
+        tile_load<W, LD, T>(X0, wstride, p.famL, p.nfamL, blk0, nvalid, N, S);
+        __syncthreads();
+        if (p.A) tile_store_spectral<W, LD, T>(G0, wstride, p.A, blk0, mask, N, S);  // G0 stays read-only
+        for (int comp = 0; comp < 2; ++comp) {
+            c64* dst = comp ? p.GG : p.GL;
+            if (comp == 1) {
+                tile_load<W, LD, T>(X0, wstride, p.famG, p.nfamG, blk0, nvalid, N, S);
+                __syncthreads();
+            }
+            c64* tot = comp ? p.totG : p.totL;
+            if (tot) {
+                tile_store<W, LD, false, T>(X0, wstride, tot, blk0, mask, N, S);
+                __syncthreads();  // the products overwrite the tiles
+            }
+            keldysh_products_split<B, SPLIT>(Gs, Xs, diag[b], N, lane, s, active,
+                                             p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr);
+            tile_store<W, LD, false, T>(X0, wstride, dst, blk0, mask, N, S);
+            if (comp == 0) __syncthreads();  // before S> replaces the tile
+        }
+    }
+}
+
 template <int B, int W>
 __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) {
     constexpr int NP = DKShape<B>::NP;
@@ -520,13 +658,17 @@ struct SWArgs {
     int accumulate;  // 1: out += result
 };
 
-template <int B, int W>
-__global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
+// SPLIT waves per block share its tile (keldysh_products_split) for the
+// large blocks; the results are bitwise those of one wave per block.
+template <int B, int W, int SPLIT = 1>
+__global__ __launch_bounds__(64 * SPLIT * W) void k_sandwich(SWArgs p) {
     constexpr int NP = DKShape<B>::NP;
     constexpr int LD = DKShape<B>::LD;
+    constexpr int TH = 64 * SPLIT * W;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const int blkw = wave / SPLIT, part = wave % SPLIT;
     const int nte = (p.NE + W - 1) / W;
     const long long t = xcd_tile(blockIdx.x, gridDim.x);
     const int e0 = (int)(t % nte) * W;
@@ -537,11 +679,11 @@ __global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
     const int wstride = TileStride<B>::value;
     c64* Ts = reinterpret_cast<c64*>(smem);          // the constant T, shared
     c64* X0 = Ts + wstride;
-    c64* Xs = X0 + (size_t)wave * wstride;
+    c64* Xs = X0 + (size_t)blkw * wstride;
     const int N = p.N;
     const size_t S = (size_t)nblk;
     const int lr = lane & 7, lc = lane >> 3;
-    for (int f = threadIdx.x; f < N * N; f += 64 * W) {
+    for (int f = threadIdx.x; f < N * N; f += TH) {
         if (p.tadj)
             Ts[(f / N) * LD + f % N] = cconj(p.T[f]);
         else
@@ -563,7 +705,7 @@ __global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
             r = p.right[e];
             th = p.wr[e];
         }
-        for (; f < total; f += 64 * W) {
+        for (; f < total; f += TH) {
             if (w < nvalid) {
                 c64 v = cmk(0.0, 0.0);
                 if (valid) {
@@ -577,7 +719,7 @@ __global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
                 }
                 X0[w * wstride + i * LD + j] = v;
             }
-            i += 64;
+            i += TH / W;
             while (i >= N) {
                 i -= N;
                 ++j;
@@ -585,7 +727,9 @@ __global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
         }
     }
     __syncthreads();
-    if (wave < nvalid) {
+    if constexpr (SPLIT > 1) {
+        keldysh_products_split<B, SPLIT>(Ts, Xs, nullptr, N, lane, part, blkw < nvalid, nullptr);
+    } else if (wave < nvalid) {
         c64 Tm[B][B];
         block_product<B, false>(Ts, Xs, N, lr, lc, Tm);
         wave_sync();
@@ -604,9 +748,9 @@ __global__ __launch_bounds__(64 * W) void k_sandwich(SWArgs p) {
     __syncthreads();
     const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
     if (p.accumulate)
-        tile_store<W, LD, true>(X0, wstride, p.out, e0 + mbase, mask, N, S);
+        tile_store<W, LD, true, TH>(X0, wstride, p.out, e0 + mbase, mask, N, S);
     else
-        tile_store<W, LD>(X0, wstride, p.out, e0 + mbase, mask, N, S);
+        tile_store<W, LD, false, TH>(X0, wstride, p.out, e0 + mbase, mask, N, S);
 }
 
 // Deterministic reduction of the weighted block traces in the reference's
@@ -912,11 +1056,60 @@ int launch_keldysh_w(qclhip_ctx* ctx, const DKArgs& a) {
     return launch_check(ctx, "k_keldysh");
 }
 
+// Waves per block when the autotuner has not chosen (index B): one wave
+// per block up to N = 24, where two LDS tiles per wave still leave two waves
+// per SIMD; above, one row of the 8x8 lane grid per wave (measured at
+// NE = 2048 or 1024, Nk = 64: 9.4 -> 6.2 ms at N = 32, 9.1 -> 7.2 ms at
+// N = 40, 19.7 -> 9.3 ms at N = 48).
+constexpr int kDefaultKeldyshSplit[7] = {1, 1, 1, 1, 4, 5, 6};
+
+template <int B, int W, int SPLIT>
+int launch_keldysh_split_w(qclhip_ctx* ctx, const DKArgs& a) {
+    constexpr size_t per_block = 2 * (size_t)TileStride<B>::value * sizeof(c64);
+    const size_t lds = per_block * W;
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_keldysh_split<B, W, SPLIT>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
+    unsigned grid = 0;
+    int rc = persistent_grid(ctx, (const void*)k_keldysh_split<B, W, SPLIT>, 64 * SPLIT * W, lds, tiles, &grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_keldysh_split<B, W, SPLIT>), dim3(grid), dim3(64 * SPLIT * W), lds, ctx->stream, a, tiles);
+    return launch_check(ctx, "k_keldysh_split");
+}
+
+// Waves per block of the vector Keldysh kernel compiled for each tile size:
+// B = 4 (N <= 32): 2 or 4; B = 5: 5; B = 6: 2, 3 or 6 (the rows of a block
+// split evenly).
+constexpr bool split_compiled(int B, int S) {
+    return S == 1 || (B == 4 && (S == 2 || S == 4)) || (B == 5 && S == 5) || (B == 6 && (S == 2 || S == 3 || S == 6));
+}
+
+template <int B, int W>
+int launch_keldysh_split(qclhip_ctx* ctx, const DKArgs& a, int ks) {
+    if constexpr (B == 4) {
+        if (ks == 2) return launch_keldysh_split_w<B, W, 2>(ctx, a);
+        if (ks == 4) return launch_keldysh_split_w<B, W, 4>(ctx, a);
+    } else if constexpr (B == 5) {
+        if (ks == 5) return launch_keldysh_split_w<B, W, 5>(ctx, a);
+    } else if constexpr (B == 6) {
+        if (ks == 2) return launch_keldysh_split_w<B, W, 2>(ctx, a);
+        if (ks == 3) return launch_keldysh_split_w<B, W, 3>(ctx, a);
+        if (ks == 6) return launch_keldysh_split_w<B, W, 6>(ctx, a);
+    }
+    return fail(ctx, QCLHIP_ERR_ARGUMENT, "waves per block not available for this basis size");
+}
+
 template <int B>
 int launch_keldysh(qclhip_ctx* ctx, const DKArgs& a) {
     // a.wk: the autotuner's choice (2 or 4 energies per workgroup), else 4
     // (64-byte segments) when two workgroups fit in a CU's LDS
     constexpr int W = tile_waves<B>(2 * (size_t)TileStride<B>::value * sizeof(c64));
+    // a.ks: waves per block (the autotuner's choice, else the rule below)
+    const int ks = a.ks ? a.ks : kDefaultKeldyshSplit[B];
+    if (ks > 1) return launch_keldysh_split<B, W>(ctx, a, ks);
     if constexpr (W >= 2) {
         if (a.wk == 2) return launch_keldysh_w<B, 2>(ctx, a);
     }
@@ -953,18 +1146,21 @@ int launch_fused_fp64(qclhip_ctx* ctx, const DKArgs& a) {
     return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
 }
 
+// The sandwich takes the Keldysh kernel's waves per block (the same
+// products on the same tiles).
 template <int B>
 int launch_sw(qclhip_ctx* ctx, const SWArgs& a) {
     constexpr size_t per_wave = (size_t)TileStride<B>::value * sizeof(c64);
     constexpr int W = tile_waves<B>(2 * per_wave);
+    constexpr int SPLIT = kDefaultKeldyshSplit[B];
     const size_t lds = per_wave * (W + 1);
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
     if (lds > 65536)
-        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_sandwich<B, W>,
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_sandwich<B, W, SPLIT>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
-    hipLaunchKernelGGL((k_sandwich<B, W>), dim3((unsigned)tiles), dim3(64 * W), lds, ctx->stream, a);
+    hipLaunchKernelGGL((k_sandwich<B, W, SPLIT>), dim3((unsigned)tiles), dim3(64 * SPLIT * W), lds, ctx->stream, a);
     return launch_check(ctx, "k_sandwich");
 }
 
@@ -1020,6 +1216,11 @@ bool keldysh_waves_possible(int N, int W) {
     return W == w || (W == 2 && w >= 2);
 }
 
+bool keldysh_split_possible(int N, int S) {
+    const int B = (N + 7) / 8;
+    return B >= 1 && B <= 6 && split_compiled(B, S);
+}
+
 int launch_keldysh_valu(qclhip_ctx* ctx, const DKArgs& a) {
     const int N = a.N;
     if (N <= 8) return launch_keldysh<1>(ctx, a);
@@ -1045,6 +1246,7 @@ int dyson_keldysh_dispatch(qclhip_ctx* ctx, const qclhip_grid* grid, DKArgs a, b
     if (a.N > 48) return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "Nb > 48 is not supported by this build");
     a.wd = grid->dyson_waves;
     a.wk = grid->keldysh_waves;
+    a.ks = grid->keldysh_split;
     int rc;
     if (grid->precision != QCLHIP_PRECISION_FP64) {
         a.block_flags = grid->block_flags;  // one flag per block, owned by the grid
@@ -1210,6 +1412,7 @@ static int dyson_keldysh_impl(qclhip_ctx* ctx, const qclhip_grid* grid, long lon
     a.plane = plane;
     a.wd = grid->dyson_waves;
     a.wk = grid->keldysh_waves;
+    a.ks = grid->keldysh_split;
     if (keldysh && grid->keldysh_engine == QCLHIP_ENGINE_FUSED) return launch_fused_fp64(ctx, a);
     int rc = launch_dyson_fp64(ctx, a);
     if (rc || !keldysh) return rc;

@@ -464,6 +464,9 @@ function set_engine!(g::Grid, engine::Symbol)
     return g
 end
 
+set_keldysh_split!(g::Grid, waves_per_block::Integer) = (check(g.ctx, ccall((:qclhip_grid_set_keldysh_split,
+    libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint), g.ctx.handle, g.handle, waves_per_block), "set_keldysh_split"); g)
+
 set_tiles!(g::Grid, dyson_waves::Integer, keldysh_waves::Integer) = (check(g.ctx, ccall((:qclhip_grid_set_tiles,
     libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint), g.ctx.handle, g.handle, dyson_waves, keldysh_waves),
     "set_tiles"); g)
@@ -474,12 +477,12 @@ function autotune!(g::Grid)
     ms = zeros(Float64, 2)
     check(g.ctx, ccall((:qclhip_grid_autotune, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}),
                        g.ctx.handle, g.handle, ms), "autotune")
-    t = zeros(Float64, 7)
-    ch = zeros(Cint, 3)
+    t = zeros(Float64, 8)
+    ch = zeros(Cint, 4)
     check(g.ctx, ccall((:qclhip_grid_tuning, libqclhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Cint}),
                        g.ctx.handle, g.handle, t, ch), "grid_tuning")
     return (engine = (:vector, :matrix, :fused)[ch[1] + 1], dyson_waves = Int(ch[2]), keldysh_waves = Int(ch[3]),
-            ms = t)
+            keldysh_split = Int(ch[4]), ms = t)
 end
 
 # -- deferred results and graphs ---------------------------------------------

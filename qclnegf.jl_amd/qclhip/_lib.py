@@ -146,6 +146,7 @@ _SIGNATURES = {
     "qclhip_residual_keldysh": [_vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), _dp],
     "qclhip_batch_create": [_vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp)],
     "qclhip_grid_tuning": [_vp, _vp, _dp, ct.POINTER(ct.c_int)],
+    "qclhip_grid_set_keldysh_split": [_vp, _vp, ct.c_int],
     "qclhip_kernel_create_dense_device": [_vp, _vp, _vp, ct.POINTER(_vp)],
     "qclhip_grid_set_tiles": [_vp, _vp, ct.c_int, ct.c_int],
     "qclhip_batch_set_mode": [_vp, _vp, ct.c_int],

@@ -255,12 +255,20 @@ class Grid:
         (ms) and the choices."""
         ms = (ct.c_double * 2)()
         self.ctx.call("qclhip_grid_autotune", self.handle, ms)
-        t = (ct.c_double * 7)()
-        ch = (ct.c_int * 3)()
+        t = (ct.c_double * 8)()
+        ch = (ct.c_int * 4)()
         self.ctx.call("qclhip_grid_tuning", self.handle, t, ch)
         self.engine = {0: "vector", 1: "matrix", 2: "fused"}[ch[0]]
         return dict(vector_ms=t[0], matrix_ms=t[1], engine=self.engine, dyson_w4_ms=t[2], dyson_w8_ms=t[3],
-                    dyson_waves=ch[1], keldysh_w2_ms=t[4], keldysh_w4_ms=t[5], keldysh_waves=ch[2], fused_ms=t[6])
+                    dyson_waves=ch[1], keldysh_w2_ms=t[4], keldysh_w4_ms=t[5], keldysh_waves=ch[2], fused_ms=t[6],
+                    keldysh_split=ch[3], keldysh_split_ms=t[7])
+
+    def set_keldysh_split(self, waves_per_block: int = 0):
+        """Waves per block of the FP64 vector Keldysh kernel (0: the rule;
+        2/4 at N 25..32, 5 at 33..40, 2/3/6 at 41..48); bitwise the same
+        results."""
+        self.ctx.call("qclhip_grid_set_keldysh_split", self.handle, waves_per_block)
+        return self
 
     def set_tiles(self, dyson_waves: int = 0, keldysh_waves: int = 0):
         """Energies per workgroup of the FP64 vector Dyson (4, 8) and Keldysh

@@ -223,3 +223,49 @@ def test_tile_shapes_bitwise_and_autotuned(hip, N):
     t = hip.Grid(w["eps"], w["wE"], w["wk"], N).autotune()
     assert t["dyson_waves"] in (4, 8) and t["keldysh_waves"] in (2, 4)
     assert t["dyson_w4_ms"] > 0 and t["keldysh_w2_ms"] > 0
+
+
+@pytest.mark.parametrize("N", [25, 32, 37, 40, 44, 48])
+def test_keldysh_split_bitwise(hip, N):
+    """The vector Keldysh kernel with a block's rows split over S waves
+    gives the one-wave results bit for bit: G<, G>, the block traces (the
+    same summation tree) and, through current_green, A and the totals."""
+    import torch
+    from qclhip import ArgumentError
+    w, SR, SL, SG = _dense_case(N, NE=45, Nk=3)
+    g = hip.Grid(w["eps"], w["wE"], w["wk"], N)
+    ref = _run_dk(hip, g, w, SR, SL, SG)
+    # physical families (the realness gates of current_green hold)
+    w2 = fx.syn.gf_workload(N=N, NE=40, Nk=3)
+    g2 = hip.Grid(w2["eps"], w2["wE"], w2["wk"], N)
+    fams = [fx.syn.diag_fields_device(fx.syn.injection_family(w2)), fx.syn.diag_fields_device(w2)]
+    h = hip.h_to_device(w2["h"])
+
+    def cg():
+        tot = tuple(torch.empty(hip.field_shape(w2["NE"], w2["Nk"], N), dtype=torch.complex128, device="cuda")
+                    for _ in range(3))
+        st, info = hip.current_green_production(g2, h, fams, 1.0, totals=tot)
+        return [st.GR, st.GL, st.GG, st.A, *tot], info
+
+    cref, iref = cg()
+    ran = []
+    for ks in (2, 3, 4, 5, 6):
+        try:
+            g.set_keldysh_split(ks)
+        except ArgumentError:  # not compiled for this basis size
+            continue
+        g2.set_keldysh_split(ks)
+        ran.append(ks)
+        out = _run_dk(hip, g, w, SR, SL, SG)
+        for a, b in zip(ref, out):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), ks
+        cout, iout = cg()
+        for a, b in zip(cref, cout):
+            assert torch.equal(a, b), ks
+        assert iout == iref
+    assert ran
+    g.set_keldysh_split(0)
+    with pytest.raises(ArgumentError):
+        g.set_keldysh_split(7)
+    t = hip.Grid(w["eps"], w["wE"], w["wk"], N).autotune()
+    assert t["keldysh_split"] in [1] + ran and t["keldysh_split_ms"] > 0

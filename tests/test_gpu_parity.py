@@ -333,9 +333,12 @@ def test_fft_hilbert_gpu(hip, orc, NE):
     assert abs(r2 - ro) < 1e-13
 
 
-def test_embedding_gpu(hip, orc):
-    rng = np.random.default_rng(12)
-    NE, Nk, Nb = 60, 2, 5
+@pytest.mark.parametrize("Nb", [5, 24, 32, 40, 48])
+def test_embedding_gpu(hip, orc, Nb):
+    """Every tile size of k_sandwich, including the large blocks whose rows
+    are split over several waves (N > 24); NE = 61 leaves partial tiles."""
+    rng = np.random.default_rng(12 + Nb)
+    NE, Nk = 61, 2
     eps = np.linspace(-1, 2, NE)
     X = _random_fields(rng, NE, Nk, Nb, 1, scale=1.0)[0]
     T = rng.normal(size=(Nb, Nb)) + 1j * rng.normal(size=(Nb, Nb))
