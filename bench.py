@@ -695,6 +695,7 @@ def batch_bench(reps=3, ninst=16, Nk=16, N_range=(12, 48), NE_range=(1024, 4096)
     batch = api.Batch(grids)
     targets = [1.0] * ninst
     states, _ = api.batch_current_green(batch, hs, fams, targets)
+    tune = api.batch_autotune(batch, hs, fams, targets, states)  # persistent vs per-instance launches
     nblk = sum(NE * Nk for _, NE in shapes)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -711,7 +712,7 @@ def batch_bench(reps=3, ninst=16, Nk=16, N_range=(12, 48), NE_range=(1024, 4096)
     grids[0].ctx.synchronize()
     torch.cuda.synchronize()
     ts = (time.perf_counter() - t0) / reps
-    return dict(instances=ninst, Nk=Nk, solves=nblk, batch_ms=tb * 1e3, batch_solves_per_s=nblk / tb,
+    return dict(instances=ninst, Nk=Nk, solves=nblk, batch_ms=tb * 1e3, batch_solves_per_s=nblk / tb, launch=tune,
                 one_by_one_ms=ts * 1e3, one_by_one_solves_per_s=nblk / ts,
                 shapes=f"N in U{{{N_range[0]}..{N_range[1]}}}, NE in U{{{NE_range[0]}..{NE_range[1]}}}")
 

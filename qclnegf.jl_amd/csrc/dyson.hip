@@ -598,19 +598,23 @@ __global__ __launch_bounds__(64 * W) void k_dyson_keldysh(DKArgs p, long long nt
 }
 
 // Batched Keldysh products: a persistent grid over the tiles of every
-// instance of one tile class (each tile finds its instance; no prefetch).
-template <int B, int W>
-__global__ __launch_bounds__(64 * W) void k_keldysh_batch(BatchClass cl) {
+// instance of one tile class (each tile finds its instance; no prefetch);
+// SPLIT waves per block for the large blocks, as k_keldysh_split.
+template <int B, int W, int SPLIT = 1>
+__global__ __launch_bounds__(64 * SPLIT * W) void k_keldysh_batch(BatchClass cl) {
     constexpr int LD = DKShape<B>::LD;
+    constexpr int TH = 64 * SPLIT * W;
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ c64 diag[W][8 * B];
     const long long total = cl.begin[cl.n];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const int bw = wave / SPLIT, part = wave % SPLIT;
     const int wstride = TileStride<B>::value;
     c64* G0 = reinterpret_cast<c64*>(smem);
     c64* X0 = G0 + (size_t)W * wstride;
-    c64* Gs = G0 + (size_t)wave * wstride;
-    c64* Xs = X0 + (size_t)wave * wstride;
+    c64* Gs = G0 + (size_t)bw * wstride;
+    c64* Xs = X0 + (size_t)bw * wstride;
     for (long long d = blockIdx.x; d < total; d += gridDim.x) {
         const long long g = xcd_tile(d, total);
         const int jj = batch_find(cl, g);
@@ -622,21 +626,26 @@ __global__ __launch_bounds__(64 * W) void k_keldysh_batch(BatchClass cl) {
         const int nvalid = min(W, p.NE - e0);
         const long long nblk = (long long)p.NE * p.Nk;
         const long long blk0 = e0 + (long long)p.NE * m;
-        const long long blk = blk0 + wave;
+        const long long blk = blk0 + bw;
         const int N = p.N;
         const size_t S = (size_t)nblk;
-        const bool active = wave < nvalid;
+        const bool active = bw < nvalid;
         const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
         const c64* gr_fam[1] = {p.GR};
         __syncthreads();  // the previous tile's LDS is free
-        tile_load<W, LD>(G0, wstride, gr_fam, 1, blk0, nvalid, N, S);
+        tile_load<W, LD, TH>(G0, wstride, gr_fam, 1, blk0, nvalid, N, S);
         for (int comp = 0; comp < 2; ++comp) {
-            tile_load<W, LD>(X0, wstride, comp ? p.famG : p.famL, comp ? p.nfamG : p.nfamL, blk0, nvalid, N, S);
+            tile_load<W, LD, TH>(X0, wstride, comp ? p.famG : p.famL, comp ? p.nfamG : p.nfamL, blk0, nvalid, N,
+                                 S);
             __syncthreads();
-            if (active)
-                keldysh_products<B>(Gs, Xs, N, lane, p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr);
-            __syncthreads();
-            tile_store<W, LD>(X0, wstride, comp ? p.GG : p.GL, blk0, mask, N, S);
+            c64* tr = p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr;
+            if constexpr (SPLIT > 1) {
+                keldysh_products_split<B, SPLIT>(Gs, Xs, diag[bw], N, lane, part, active, tr);
+            } else {
+                if (active) keldysh_products<B>(Gs, Xs, N, lane, tr);
+                __syncthreads();
+            }
+            tile_store<W, LD, false, TH>(X0, wstride, comp ? p.GG : p.GL, blk0, mask, N, S);
             __syncthreads();
         }
     }
@@ -1287,16 +1296,17 @@ constexpr int keldysh_waves() {
 template <int B>
 int launch_keldysh_batch_b(qclhip_ctx* ctx, const BatchClass& c, long long total) {
     constexpr int W = keldysh_waves<B>();
+    constexpr int SPLIT = kDefaultKeldyshSplit[B];
     const size_t lds = 2 * (size_t)TileStride<B>::value * sizeof(c64) * W;
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
     if (lds > 65536)
-        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_keldysh_batch<B, W>,
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_keldysh_batch<B, W, SPLIT>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     unsigned grid = 0;
-    int rc = persistent_grid(ctx, (const void*)k_keldysh_batch<B, W>, 64 * W, lds, total, &grid);
+    int rc = persistent_grid(ctx, (const void*)k_keldysh_batch<B, W, SPLIT>, 64 * SPLIT * W, lds, total, &grid);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_keldysh_batch<B, W>), dim3(grid), dim3(64 * W), lds, ctx->stream, c);
+    hipLaunchKernelGGL((k_keldysh_batch<B, W, SPLIT>), dim3(grid), dim3(64 * SPLIT * W), lds, ctx->stream, c);
     return launch_check(ctx, "k_keldysh_batch");
 }
 
