@@ -152,7 +152,7 @@ int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host) {
         if (keldysh_waves_possible(N, kw[k])) rc = timed(1, kw[k], &msk[k]);
     // Waves per block of the vector Keldysh kernel (rows of a block split
     // over 1..6 waves where compiled), at the chosen energies per workgroup.
-    int best_ks = 1;
+    int best_ks = 1, best_ks_w = 0;
     float ms_ks = 0.f;
     {
         const int kw_best = [&]() {
@@ -161,20 +161,27 @@ int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host) {
         }();
         for (int ks = 1; ks <= 6 && !rc; ++ks) {
             if (!keldysh_split_possible(N, ks)) continue;
-            DKArgs b = a;
-            b.wk = kw_best;
-            b.ks = ks;
-            rc = launch_keldysh_valu(ctx, b);  // warm-up
-            const int reps = 3;
-            hipEventRecord(e0, ctx->stream);
-            for (int k = 0; k < reps && !rc; ++k) rc = launch_keldysh_valu(ctx, b);
-            hipEventRecord(e1, ctx->stream);
-            float t = 0.f;
-            if (!rc && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&t, e0, e1);
-            t /= reps;
-            if (ms_ks == 0.f || t < ms_ks) {
-                ms_ks = t;
-                best_ks = ks;
+            // split kernels also at twice the energies per workgroup
+            const int widths[2] = {kw_best, ks > 1 ? 2 * (kw_best ? kw_best : 1) : 0};
+            for (int iw = 0; iw < 2 && !rc; ++iw) {
+                const int wk = widths[iw];
+                if (iw == 1 && (wk == 0 || !keldysh_waves_possible(N, wk))) continue;
+                DKArgs b = a;
+                b.wk = wk;
+                b.ks = ks;
+                rc = launch_keldysh_valu(ctx, b);  // warm-up
+                const int reps = 3;
+                hipEventRecord(e0, ctx->stream);
+                for (int k = 0; k < reps && !rc; ++k) rc = launch_keldysh_valu(ctx, b);
+                hipEventRecord(e1, ctx->stream);
+                float t = 0.f;
+                if (!rc && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&t, e0, e1);
+                t /= reps;
+                if (ms_ks == 0.f || t < ms_ks) {
+                    ms_ks = t;
+                    best_ks = ks;
+                    best_ks_w = wk;
+                }
             }
         }
     }
@@ -213,6 +220,7 @@ int qclhip_grid_autotune(qclhip_ctx* ctx, qclhip_grid* grid, double* ms_host) {
     if (msf > 0.f && msf < std::min(t_vec, t_mat)) engine = QCLHIP_ENGINE_FUSED;
     grid->keldysh_engine = engine;
     grid->keldysh_split = best_ks;
+    if (best_ks > 1) grid->keldysh_waves = best_ks_w;  // the split kernel's energies per workgroup
     qclhip_ctx::TuneEntry te{grid->Nb,
                              grid->NE,
                              grid->Nk,

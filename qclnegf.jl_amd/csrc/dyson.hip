@@ -1111,6 +1111,16 @@ int launch_keldysh_split(qclhip_ctx* ctx, const DKArgs& a, int ks) {
     return fail(ctx, QCLHIP_ERR_ARGUMENT, "waves per block not available for this basis size");
 }
 
+// Twice the energies per workgroup for the split kernels (N > 24), where one
+// workgroup per CU still holds several waves per SIMD: 64-byte segments at
+// N <= 32 (4 energies), 32-byte ones above. Fits when the two tiles of every
+// block stay within the LDS of a CU.
+template <int B>
+constexpr bool split_wide_fits() {
+    constexpr int W = tile_waves<B>(2 * (size_t)TileStride<B>::value * sizeof(c64));
+    return B >= 4 && 2 * (size_t)TileStride<B>::value * sizeof(c64) * (2 * W) <= 160 * 1024;
+}
+
 template <int B>
 int launch_keldysh(qclhip_ctx* ctx, const DKArgs& a) {
     // a.wk: the autotuner's choice (2 or 4 energies per workgroup), else 4
@@ -1118,7 +1128,12 @@ int launch_keldysh(qclhip_ctx* ctx, const DKArgs& a) {
     constexpr int W = tile_waves<B>(2 * (size_t)TileStride<B>::value * sizeof(c64));
     // a.ks: waves per block (the autotuner's choice, else the rule below)
     const int ks = a.ks ? a.ks : kDefaultKeldyshSplit[B];
-    if (ks > 1) return launch_keldysh_split<B, W>(ctx, a, ks);
+    if (ks > 1) {
+        if constexpr (split_wide_fits<B>()) {
+            if (a.wk == 2 * W) return launch_keldysh_split<B, 2 * W>(ctx, a, ks);
+        }
+        return launch_keldysh_split<B, W>(ctx, a, ks);
+    }
     if constexpr (W >= 2) {
         if (a.wk == 2) return launch_keldysh_w<B, 2>(ctx, a);
     }
@@ -1160,6 +1175,8 @@ int launch_fused_fp64(qclhip_ctx* ctx, const DKArgs& a) {
 template <int B>
 int launch_sw(qclhip_ctx* ctx, const SWArgs& a) {
     constexpr size_t per_wave = (size_t)TileStride<B>::value * sizeof(c64);
+    // (twice the energies per workgroup measured slower at N = 32: 12.6 ms
+    // against 9.9 ms at NE = 4096, Nk = 128)
     constexpr int W = tile_waves<B>(2 * per_wave);
     constexpr int SPLIT = kDefaultKeldyshSplit[B];
     const size_t lds = per_wave * (W + 1);
@@ -1222,7 +1239,14 @@ bool keldysh_waves_possible(int N, int W) {
         }
     };
     const int w = widest((N + 7) / 8);
-    return W == w || (W == 2 && w >= 2);
+    bool wide = false;  // the split kernels' twice-as-wide tile
+    switch ((N + 7) / 8) {
+        case 4: wide = split_wide_fits<4>(); break;
+        case 5: wide = split_wide_fits<5>(); break;
+        case 6: wide = split_wide_fits<6>(); break;
+        default: break;
+    }
+    return W == w || (W == 2 && w >= 2) || (wide && W == 2 * w);
 }
 
 bool keldysh_split_possible(int N, int S) {

@@ -259,6 +259,15 @@ def test_keldysh_split_bitwise(hip, N):
         out = _run_dk(hip, g, w, SR, SL, SG)
         for a, b in zip(ref, out):
             assert np.array_equal(np.asarray(a), np.asarray(b)), ks
+        for wk in (1, 2, 4):  # energies per workgroup of the split kernel
+            try:
+                g.set_tiles(0, wk)
+            except ArgumentError:
+                continue
+            out = _run_dk(hip, g, w, SR, SL, SG)
+            for a, b in zip(ref, out):
+                assert np.array_equal(np.asarray(a), np.asarray(b)), (ks, wk)
+        g.set_tiles(0, 0)
         cout, iout = cg()
         for a, b in zip(cref, cout):
             assert torch.equal(a, b), ks
