@@ -45,8 +45,10 @@ def _ip(a, b):
     return float(torch.sum(torch.conj(a) * b).real)
 
 
-@pytest.mark.parametrize("NE", [8193, 12000, 30721])
+@pytest.mark.parametrize("NE", [8193, 12000, 30721, 40000, 70000])
 def test_long_axis_vs_oracle(hip, orc, NE):
+    """Every row length of the four-step split: L2 = 4096 (L = 32768,
+    65536), 8192 (L = 131072) and 16384 (L = 262144)."""
     rng = np.random.default_rng(NE)
     Nk, Nb = 3, 1
     eps, wE = _uniform(NE)
@@ -68,12 +70,12 @@ def test_long_axis_vs_oracle(hip, orc, NE):
         assert rel(hip.to_host(lam), orc.direct_hilbert(Gam, eps, wE)) < 1e-12
 
 
-def test_long_axis_dawson(hip):
-    """Gaussian columns on NE = 30721 nodes (L = 65536): Dawson's integral
-    plus the trapezoid rule's omitted-node term, the closed form that fixes
-    the kernel's sign and 1/(2 pi) scale."""
+@pytest.mark.parametrize("n", [30721, 70001])
+def test_long_axis_dawson(hip, n):
+    """Gaussian columns on 30721 and 70001 nodes (L = 65536, 262144):
+    Dawson's integral plus the trapezoid rule's omitted-node term, the
+    closed form that fixes the kernel's sign and 1/(2 pi) scale."""
     from test_oracle_anchors import dawson_columns
-    n = 30721
     x, w, G, exact = dawson_columns(n)
     ncol = G.shape[2]
     G = np.asfortranarray(G.reshape(n, ncol, 1, 1, order="F"))
