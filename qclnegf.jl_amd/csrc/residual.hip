@@ -32,6 +32,10 @@ struct ResArgs {
     // pass, before the candidates exist)
     int mode;
     double* out;  // [7][NE*Nk]: rD, specN, specD, kelN, kelD, psd, caus
+    // Blocks whose Cholesky screens did not settle: (blk << 8) | screen bits,
+    // appended through the counter (zeroed before the first pass).
+    unsigned long long* flagged;
+    unsigned* nflagged;
 };
 
 // A = i (G - G^dagger) of the wave's own G^R tile into X, the arithmetic of
@@ -83,16 +87,64 @@ constexpr int res_waves() {
     return 4 * res_wave_bytes<B>() <= 81920 ? 4 : (2 * res_wave_bytes<B>() <= 81920 ? 2 : 1);
 }
 
+// Screens of the suite (bit positions in the per-block fallback word).
+enum ResScreen : unsigned {
+    kScrCaus = 1u << 0,   // lambda_max(Im S^R)
+    kScrGamma = 1u << 1,  // Gamma
+    kScrA = 1u << 2,      // A
+    kScrLess = 1u << 3,   // -i G<
+    kScrGrt = 1u << 4,    // i G>
+    kScrRawL = 1u << 5,   // -i G^R S< G^R^dagger
+    kScrRawG = 1u << 6    // i G^R S> G^R^dagger
+};
+
+// Register prefetch of Gamma = i (S> - S<) of the summed families with the
+// thread mapping of TileMap (the value of tile_load_gamma).
+template <int B, int W>
+__device__ inline void fetch_gamma(const TileMap<B, W>& tm, c64 (&r)[TileMap<B, W>::EPT], const c64* const* famG,
+                                   const c64* const* famL, int nfam, long long blk0, int nvalid, size_t S) {
+    const bool on = tm.w < nvalid;
+#pragma unroll
+    for (int k = 0; k < TileMap<B, W>::EPT; ++k)
+        if (on && k < tm.n) {
+            const size_t idx = blk0 + tm.w + S * (size_t)tm.gidx[k];
+            const c64 g = fam_load(famG, nfam, idx), l = fam_load(famL, nfam, idx);
+            r[k] = cmk(-(g.y - l.y), g.x - l.x);
+        }
+}
+
+// T = G X G^dagger of one wave's tiles (X is overwritten by G X); with
+// STORE the result also replaces X.
+template <int B, bool STORE>
+__device__ inline void res_sandwich(const c64* G, c64* X, int N, int lane, c64 (&T)[B][B]) {
+    rprod<B>(G, X, N, lane, T, false);
+    wave_sync();
+    rstore<B>(X, N, lane, T);
+    wave_sync();
+    rprod<B>(X, G, N, lane, T, true);
+    wave_sync();
+    if (STORE) {
+        rstore<B>(X, N, lane, T);
+        wave_sync();
+    }
+}
+
 // W waves per workgroup, one (e,m) block per wave, blocks blk0 .. blk0+W-1.
-// Every field is moved into LDS cooperatively (consecutive threads =
-// consecutive blocks, 16W-byte segments) between workgroup barriers; all
-// other work is wave-local. Each wave owns two LDS tiles: G^R, and a working
-// tile X that successively holds S^R, D, Gamma, G Gamma, A, the candidate
-// S<, G S<, G<, G>, and the raw products; the right factor of each product
-// stays in registers. Waves past the last block only join the barriers.
+// Each wave owns two LDS tiles: G^R, and a working tile X that successively
+// holds S^R, D, Gamma, G Gamma, A, the candidate S<, G S<, G<, G>, and the
+// raw products; the right factor of each product stays in registers. Fields
+// reach X through a register prefetch: the cooperative loads of the next
+// field (consecutive threads = consecutive blocks, 16W-byte segments) are
+// issued before the current phase's products and screens run, and written
+// to LDS between two workgroup barriers once X is free. The positivity and
+// causality checks run their Cholesky screens here; a block whose screen
+// does not settle is appended to a list with one bit per unsettled check,
+// and k_residual_fallback computes its eigenvalues afterwards. Waves past the
+// last block only join the barriers.
 template <int B, int W>
 __global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
     constexpr int LD = 8 * B + 1;
+    using TM = TileMap<B, W>;
     const int wstride = TileStride<B>::value;
     extern __shared__ __align__(16) unsigned char smem[];
     c64* Gall = reinterpret_cast<c64*>(smem);
@@ -113,14 +165,57 @@ __global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
     c64* X = Xall + wave * wstride;
     EigWork<B>* ew = ewall + wave;
     c64 T[B][B];
-    const c64* one[1];
-    one[0] = p.GR;
+    // Register prefetch up to N = 32; larger tiles would spill, and their
+    // fields are loaded straight into LDS once X is free.
+    constexpr bool kPrefetch = B <= 4;
+    TM tm;
+    tm.init(N);
+    c64 P[kPrefetch ? TM::EPT : 1];  // the next field of X, in flight
+    const c64* one[1] = {p.GR};
+    const c64* fA[1] = {p.A};
+    const c64* fL[1] = {p.GL};
+    const c64* fGg[1] = {p.GG};
     tile_load<W, LD>(Gall, wstride, one, 1, blk0, nvalid, N, S);
     tile_load<W, LD>(Xall, wstride, p.famR, p.nfam, blk0, nvalid, N, S);
     __syncthreads();
+    // The next field of X: fetched into registers now, or remembered and
+    // loaded by put().
+    const c64* const* pend = nullptr;
+    int pend_n = 0;
+    bool pend_gamma = true;
+    auto fetch = [&](const c64* const* fam, int nfam) {
+        pend = fam;
+        pend_n = nfam;
+        pend_gamma = false;
+        if constexpr (kPrefetch) tm.fetch(P, fam, nfam, blk0, nvalid, S);
+    };
+    if constexpr (kPrefetch) fetch_gamma<B, W>(tm, P, p.famG, p.famL, p.nfam, blk0, nvalid, S);
+    // Moves the next field into the X tiles once every wave is done with them.
+    auto put = [&]() {
+        __syncthreads();
+        if constexpr (kPrefetch) {
+            tm.put(Xall, wstride, P, nvalid);
+        } else {
+            if (pend_gamma)
+                tile_load_gamma<W, LD>(Xall, wstride, p.famG, p.famL, p.nfam, blk0, nvalid, N, S);
+            else
+                tile_load<W, LD>(Xall, wstride, pend, pend_n, blk0, nvalid, N, S);
+        }
+        __syncthreads();
+    };
+    unsigned unsettled = 0;
+    auto screen = [&](unsigned bit, double fr, double fi) {
+        bool u = false;
+        const double v = psd_screen_tile<B, LD>(X, N, lane, ew, fr, fi, u);
+        if (u) unsettled |= bit;
+        wave_sync();
+        return v;
+    };
     double caus = 0.0, rD = 0.0;
     if (valid) {
-        caus = causality_from_tile<B, LD>(X, N, lane, ew, T);
+        bool u = false;
+        caus = causality_screen_tile<B, LD>(X, N, lane, ew, u);
+        if (u) unsettled |= kScrCaus;
         wave_sync();
         // r_D: D = eps - h - S^R in X, ||D G - I|| / (||D|| ||G|| + sqrt(N));
         // each lane rewrites its register-block positions (all h loads in flight).
@@ -152,71 +247,63 @@ __global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
         rD = sqrt(wave_sum(dn)) /
              (sqrt(tile_frob2<LD>(X, N, lane, 0.0)) * sqrt(tile_frob2<LD>(G, N, lane, 0.0)) + sqrt((double)N));
     }
-    // Six positivity checks; phase ph prepares X (and the r_A, r_K terms on
-    // the way), then checks the Hermitian part of f X:
-    //   0: Gamma = i (S> - S<)                          f = 1
-    //   1: A, after T = G Gamma G^dagger (r_A terms)     f = 1
-    //   2: G<, after T = G S<_cand G^dagger (r_K terms)  f = -i
-    //   3: G>                                            f = i
-    //   4: G S< G^dagger, 5: G S> G^dagger (raw)         f = -i, i
+    // Gamma = i (S> - S<): its screen, then T = G Gamma G^dagger (r_A).
+    put();
+    if (p.A) fetch(fA, 1);
     double psd = 0.0, specN = 0.0, specD = 0.0, kelN = 0.0, kelD = 0.0;
-#pragma unroll 1
-    for (int ph = 0; ph < 6; ++ph) {
-        if (ph != 1) {  // phase 1 works on the Gamma of phase 0
-            __syncthreads();  // every wave is done with its X
-            if (ph == 0) {
-                tile_load_gamma<W, LD>(Xall, wstride, p.famG, p.famL, p.nfam, blk0, nvalid, N, S);
-            } else if (ph == 2) {
-                if (p.mode != 1) tile_load<W, LD>(Xall, wstride, p.cand, p.ncand, blk0, nvalid, N, S);
-            } else if (ph == 3) {
-                one[0] = p.GG;
-                tile_load<W, LD>(Xall, wstride, one, 1, blk0, nvalid, N, S);
-            } else if (ph == 4) {
-                tile_load<W, LD>(Xall, wstride, p.famL, p.nfam, blk0, nvalid, N, S);
-            } else {
-                tile_load<W, LD>(Xall, wstride, p.famG, p.nfam, blk0, nvalid, N, S);
-            }
-            __syncthreads();
+    if (valid) {
+        psd = fmax(psd, screen(kScrGamma, 1.0, 0.0));
+        res_sandwich<B, false>(G, X, N, lane, T);
+    }
+    // A (stored, or formed from the G^R tile): the r_A terms and its screen.
+    if (p.A) {
+        put();
+    } else {
+        __syncthreads();
+        if (valid) tile_spectral<LD>(G, X, N, lane);
+        __syncthreads();
+    }
+    const bool cand = p.mode != 1;  // the state pass has no candidate
+    if (cand)
+        fetch(p.cand, p.ncand);
+    else
+        fetch(fL, 1);
+    if (valid) {
+        specN = tile_reg_diff2<B, LD>(X, T, N, lane);
+        specD = tile_frob2<LD>(X, N, lane, 0.0);
+        psd = fmax(psd, screen(kScrA, 1.0, 0.0));
+    }
+    // The candidate S<: T = G S<_cand G^dagger (r_K).
+    if (cand) {
+        put();
+        fetch(fL, 1);
+        if (valid) res_sandwich<B, false>(G, X, N, lane, T);
+    }
+    // G<: the r_K terms and its screen.
+    put();
+    fetch(fGg, 1);
+    if (valid) {
+        if (cand) {
+            kelN = tile_reg_diff2<B, LD>(X, T, N, lane);
+            kelD = tile_frob2<LD>(X, N, lane, 0.0);
         }
-        const bool no_cand = p.mode == 1 && ph == 2;  // the state pass skips G S<_cand G^dagger
-        if (valid && ph != 0 && ph != 3 && !no_cand) {  // T = G X G^dagger
-            rprod<B>(G, X, N, lane, T, false);
-            wave_sync();
-            rstore<B>(X, N, lane, T);
-            wave_sync();
-            rprod<B>(X, G, N, lane, T, true);
-            wave_sync();
-            if (ph >= 4) {
-                rstore<B>(X, N, lane, T);
-                wave_sync();
-            }
-        }
-        if (ph == 1 || ph == 2) {
-            __syncthreads();
-            if (ph == 1 && !p.A) {  // A from the G^R tile (not stored by the caller)
-                if (valid) tile_spectral<LD>(G, X, N, lane);
-            } else {
-                one[0] = ph == 1 ? p.A : p.GL;
-                tile_load<W, LD>(Xall, wstride, one, 1, blk0, nvalid, N, S);
-            }
-            __syncthreads();
-            if (valid && !no_cand) {
-                const double dn2 = tile_reg_diff2<B, LD>(X, T, N, lane);
-                const double xn2 = tile_frob2<LD>(X, N, lane, 0.0);
-                if (ph == 1) {
-                    specN = dn2;
-                    specD = xn2;
-                } else {
-                    kelN = dn2;
-                    kelD = xn2;
-                }
-            }
-        }
-        if (valid) {
-            const double fi = (ph == 2 || ph == 4) ? -1.0 : ((ph == 3 || ph == 5) ? 1.0 : 0.0);
-            psd = fmax(psd, psd_from_tile<B, LD>(X, N, lane, ew, fi == 0.0 ? 1.0 : 0.0, fi));
-            wave_sync();
-        }
+        psd = fmax(psd, screen(kScrLess, 0.0, -1.0));
+    }
+    // G>.
+    put();
+    fetch(p.famL, p.nfam);
+    if (valid) psd = fmax(psd, screen(kScrGrt, 0.0, 1.0));
+    // The raw products G S< G^dagger and G S> G^dagger.
+    put();
+    fetch(p.famG, p.nfam);
+    if (valid) {
+        res_sandwich<B, true>(G, X, N, lane, T);
+        psd = fmax(psd, screen(kScrRawL, 0.0, -1.0));
+    }
+    put();
+    if (valid) {
+        res_sandwich<B, true>(G, X, N, lane, T);
+        psd = fmax(psd, screen(kScrRawG, 0.0, 1.0));
     }
     if (valid && lane == 0) {
         p.out[0 * nblk + blk] = rD;
@@ -226,6 +313,88 @@ __global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
         p.out[4 * nblk + blk] = kelD;
         p.out[5 * nblk + blk] = psd;
         p.out[6 * nblk + blk] = caus;
+        if (unsettled) {
+            const unsigned slot = atomicAdd(p.nflagged, 1u);
+            p.flagged[slot] = ((unsigned long long)blk << 8) | unsettled;
+        }
+    }
+}
+
+// Second pass of the suite: the listed blocks whose Cholesky screens did not
+// settle. One wave per block (grid-stride over the list); each unsettled
+// check rebuilds its matrix as the first pass did (same loads, same product
+// arithmetic) and runs the full check (psd_from_tile, causality_from_tile:
+// Householder + Sturm eigenvalues), raising the block's r_PSD entry and
+// setting its r_caus entry. The list holds each block at most once.
+template <int B>
+__global__ __launch_bounds__(64) void k_residual_fallback(ResArgs p) {
+    constexpr int LD = 8 * B + 1;
+    const int wstride = TileStride<B>::value;
+    extern __shared__ __align__(16) unsigned char smem[];
+    c64* G = reinterpret_cast<c64*>(smem);
+    c64* X = G + wstride;
+    EigWork<B>* ew = reinterpret_cast<EigWork<B>*>(X + wstride);
+    const int lane = threadIdx.x;
+    const long long nblk = (long long)p.NE * p.Nk;
+    const int N = p.N;
+    const size_t S = (size_t)nblk;
+    const unsigned count = *p.nflagged;
+    c64 T[B][B];
+    const c64* one[1];
+    for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {
+        const unsigned long long entry = p.flagged[i];
+        const long long blk = (long long)(entry >> 8);
+        const unsigned bits = (unsigned)(entry & 0xffu);
+        double psd = 0.0;
+        if ((bits & (kScrRawL | kScrRawG)) || ((bits & kScrA) && !p.A)) {
+            one[0] = p.GR;
+            tile_load<1, LD>(G, wstride, one, 1, blk, 1, N, S);
+        }
+        if (bits & kScrCaus) {
+            tile_load<1, LD>(X, wstride, p.famR, p.nfam, blk, 1, N, S);
+            wave_sync();
+            const double c = causality_from_tile<B, LD>(X, N, lane, ew, T);
+            if (lane == 0) p.out[6 * nblk + blk] = c;
+        }
+        wave_sync();
+        if (bits & kScrGamma) {
+            tile_load_gamma<1, LD>(X, wstride, p.famG, p.famL, p.nfam, blk, 1, N, S);
+            wave_sync();
+            psd = fmax(psd, psd_from_tile<B, LD>(X, N, lane, ew, 1.0, 0.0));
+            wave_sync();
+        }
+        if (bits & kScrA) {
+            if (p.A) {
+                one[0] = p.A;
+                tile_load<1, LD>(X, wstride, one, 1, blk, 1, N, S);
+            } else {
+                wave_sync();
+                tile_spectral<LD>(G, X, N, lane);
+            }
+            wave_sync();
+            psd = fmax(psd, psd_from_tile<B, LD>(X, N, lane, ew, 1.0, 0.0));
+            wave_sync();
+        }
+        if (bits & (kScrLess | kScrGrt)) {
+            for (int k = 0; k < 2; ++k) {
+                if (!(bits & (k ? kScrGrt : kScrLess))) continue;
+                one[0] = k ? p.GG : p.GL;
+                tile_load<1, LD>(X, wstride, one, 1, blk, 1, N, S);
+                wave_sync();
+                psd = fmax(psd, psd_from_tile<B, LD>(X, N, lane, ew, 0.0, k ? 1.0 : -1.0));
+                wave_sync();
+            }
+        }
+        for (int k = 0; k < 2; ++k) {
+            if (!(bits & (k ? kScrRawG : kScrRawL))) continue;
+            tile_load<1, LD>(X, wstride, k ? p.famG : p.famL, p.nfam, blk, 1, N, S);
+            wave_sync();
+            res_sandwich<B, true>(G, X, N, lane, T);
+            psd = fmax(psd, psd_from_tile<B, LD>(X, N, lane, ew, 0.0, k ? 1.0 : -1.0));
+            wave_sync();
+        }
+        if (lane == 0) p.out[5 * nblk + blk] = fmax(p.out[5 * nblk + blk], psd);
+        wave_sync();
     }
 }
 
@@ -570,17 +739,31 @@ __global__ void k_fill_zero(long long n, c64* __restrict__ x) {
         x[i] = cmk(0.0, 0.0);
 }
 
+__global__ void k_zero_counter(unsigned* c) {
+    if (threadIdx.x == 0) *c = 0u;
+}
+
+// The screening pass over every block, then the eigenvalue pass over the
+// blocks it listed (a fixed grid that reads the list length on the device).
 template <int B>
 int launch_residual(qclhip_ctx* ctx, const ResArgs& a) {
     constexpr int W = res_waves<B>();
     const size_t lds = W * res_wave_bytes<B>();
+    const size_t lds_fb = res_wave_bytes<B>();
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
     if (lds > 65536)
         QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_residual_suite<B, W>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const long long ntile = ((long long)a.NE * a.Nk + W - 1) / W;
+    if (lds_fb > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_residual_fallback<B>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fb));
+    const long long nblk = (long long)a.NE * a.Nk;
+    const long long ntile = (nblk + W - 1) / W;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, ctx->stream, a.nflagged);
     hipLaunchKernelGGL((k_residual_suite<B, W>), dim3((unsigned)ntile), dim3(64 * W), lds, ctx->stream, a);
+    const unsigned fb_grid = (unsigned)std::min<long long>(nblk, 1024);
+    hipLaunchKernelGGL((k_residual_fallback<B>), dim3(fb_grid), dim3(64), lds_fb, ctx->stream, a);
     return launch_check(ctx, "k_residual_suite");
 }
 
@@ -606,9 +789,14 @@ int launch_residual_keldysh(qclhip_ctx* ctx, const ResArgs& a) {
 // (r_D, r_A, r_K, r_PSD, r_caus) to out_host or the deferred buffer.
 int residual_run(qclhip_ctx* ctx, ResArgs& a, bool keldysh_only, unsigned rows, double* out_host) {
     const long long nblk = (long long)a.NE * a.Nk;
-    double* v = (double*)workspace(ctx, (size_t)(7 * nblk + 8) * sizeof(double));
+    // per-block values, the 5 results (8 slots), the list counter (8
+    // slots) and the list of unsettled blocks
+    double* v = (double*)workspace(ctx, (size_t)(7 * nblk + 16) * sizeof(double) +
+                                            (keldysh_only ? 0 : (size_t)nblk * sizeof(unsigned long long)));
     if (!v) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
     a.out = v;
+    a.nflagged = reinterpret_cast<unsigned*>(v + 7 * nblk + 8);
+    a.flagged = reinterpret_cast<unsigned long long*>(v + 7 * nblk + 16);
     int rc;
     const int N = a.N;
     const int B = N <= 8 ? 1 : (N <= 16 ? 2 : (N <= 24 ? 3 : (N <= 32 ? 4 : (N <= 40 ? 5 : (N <= 48 ? 6 : 0)))));

@@ -435,6 +435,46 @@ __device__ inline double psd_from_tile(const c64* X, int N, int lane, EigWork<B>
     return psd_ratio(lmin, fmax(fabs(lmin), fabs(lmax)), N);
 }
 
+// Screen-only forms of psd_from_tile and causality_from_tile for the first
+// pass of the residual suite: the same value wherever the finiteness check,
+// the zero block or the Cholesky screen settles the block; otherwise
+// `unsettled` is set and 0 returned, and a second pass computes the
+// eigenvalues (the full functions). Without the eigenvalue code the pass
+// needs far fewer registers.
+template <int B, int LD>
+__device__ inline double psd_screen_tile(const c64* X, int N, int lane, EigWork<B>* w, double fr, double fi,
+                                         bool& unsettled) {
+    constexpr double kEps = 2.220446049250313e-16;
+    c64 a[B][B];
+    load_herm<B, LD>(X, N, lane, fr, fi, a);
+    const BlockScale s = block_scale<B>(a, N, lane);
+    if (s.bad) return INFINITY;
+    if (s.amax == 0.0) return 0.0;
+    int ex;
+    frexp(s.amax, &ex);
+    const double sc = ldexp(1.0, -ex);
+    scale_regs<B>(a, sc);
+    const double delta = 32.0 * N * kEps * fmax(sqrt(s.fro2 / N), 1.0);
+    if (!chol_screen<B>(a, N, lane, w, delta * sc)) unsettled = true;
+    return 0.0;
+}
+
+template <int B, int LD>
+__device__ inline double causality_screen_tile(const c64* X, int N, int lane, EigWork<B>* w, bool& unsettled) {
+    c64 a[B][B];
+    load_herm<B, LD>(X, N, lane, 0.0, 1.0, a);  // Hermitian part of i S = -Im S
+    const BlockScale s = block_scale<B>(a, N, lane);
+    if (s.bad) return INFINITY;
+    if (s.amax == 0.0) return 0.0;
+    int ex;
+    frexp(s.amax, &ex);
+    const double sc = ldexp(1.0, -ex);
+    scale_regs<B>(a, sc);
+    const double delta = 32.0 * N * 2.220446049250313e-16 * sqrt(s.fro2 / N);
+    if (!chol_screen<B>(a, N, lane, w, delta * sc)) unsettled = true;
+    return 0.0;
+}
+
 // Causality of the retarded block S in tile X (residuals.jl:455-492):
 // lambda_max(Im S) / (||S||_2 + 1e-14) when positive, else 0; Inf for a
 // non-finite block. A Cholesky factorization of -Im S + delta I with

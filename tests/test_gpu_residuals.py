@@ -72,6 +72,35 @@ def test_random_states_vs_oracle(hip, orc, Nb):
         assert abs(got[k] - ref[k]) <= 1e-10 * ref[k], (k, got[k], ref[k])
 
 
+@pytest.mark.parametrize("Nb", [5, 24, 32])
+def test_many_unsettled_blocks_vs_oracle(hip, orc, Nb):
+    # every block of a random state fails its Cholesky screens: the list of
+    # unsettled blocks outgrows the eigenvalue pass's grid (1024 waves)
+    NE = 1201 if Nb == 5 else 520
+    s = _random_state(NE=NE, Nk=2, Nb=Nb, seed=300 + Nb, nfam=1, ncand=1)
+    ref = _oracle(orc, s)
+    got = _gpu(hip, s)
+    for k in KEYS:
+        assert abs(got[k] - ref[k]) <= 1e-10 * ref[k], (k, got[k], ref[k])
+
+
+def test_unsettled_blocks_among_settled(hip, orc):
+    # a consistent state with a few blocks of A given a negative eigenvalue:
+    # settled and unsettled blocks in the same tiles, the maximum from one
+    s, _ = _scba_state(fx.make_problem(N=10, NE=48, Nk=3))
+    rng = np.random.default_rng(9)
+    Q, _ = np.linalg.qr(rng.normal(size=(10, 10)) + 1j * rng.normal(size=(10, 10)))
+    for (e, m, neg) in ((3, 0, -0.2), (17, 1, -0.05), (40, 2, -0.4), (41, 2, -0.1)):
+        d = np.linspace(0.1, 1.0, 10)
+        d[0] = neg
+        s["A"][e, m] = Q @ np.diag(d) @ Q.conj().T
+    ref = _oracle(orc, s)
+    got = _gpu(hip, s)
+    assert ref["r_PSD"] > 0.3
+    for k in ("r_PSD", "r_caus", "r_A"):
+        assert abs(got[k] - ref[k]) <= 1e-10 * ref[k] + 1e-13, (k, got[k], ref[k])
+
+
 def test_single_family_and_many_candidates(hip, orc):
     s = _random_state(NE=9, Nk=2, Nb=6, seed=5, nfam=1, ncand=8)
     ref = _oracle(orc, s)
