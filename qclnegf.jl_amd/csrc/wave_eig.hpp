@@ -435,19 +435,113 @@ __device__ inline double psd_from_tile(const c64* X, int N, int lane, EigWork<B>
     return psd_ratio(lmin, fmax(fabs(lmin), fabs(lmax)), N);
 }
 
+// The screen on the cyclic lane grid: lane (lr, lc) holds rows lr + 8r and
+// columns lc + 8c (r, c < B). Column k = 8 kc + kl lives in register column
+// kc of the lanes lc == kl, and once k >= 8 kc every row and column of
+// register index below kc is eliminated on every lane, so step k updates
+// only the (B - kc)^2 trailing registers: a third of the blocked screen's
+// FMAs for B = 3. Each entry sees the same operations in the same order as
+// in chol_screen (skipped updates there subtract exact zeros).
+template <int B, int LD>
+__device__ inline void load_herm_cyc(const c64* X, int N, int lane, double fr, double fi, c64 (&a)[B][B]) {
+    const int lr = lane & 7, lc = lane >> 3;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = lr + 8 * r, j = lc + 8 * c;
+            if (i < N && j < N) {
+                const c64 x = X[i * LD + j], y = X[j * LD + i];
+                const double ax = fr * x.x - fi * x.y, ay = fr * x.y + fi * x.x;
+                const double bx = fr * y.x - fi * y.y, by = fr * y.y + fi * y.x;
+                a[r][c] = cmk(0.5 * (ax + bx), 0.5 * (ay - by));
+            } else {
+                a[r][c] = cmk(0.0, 0.0);
+            }
+        }
+}
+
+template <int B>
+__device__ inline BlockScale block_scale_cyc(const c64 (&a)[B][B], int N, int lane) {
+    const int lr = lane & 7, lc = lane >> 3;
+    double amax = 0.0, fro2 = 0.0, bad = 0.0;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+            if (lr + 8 * r < N && lc + 8 * c < N) {
+                const c64 v = a[r][c];
+                if (!(isfinite(v.x) && isfinite(v.y))) bad = 1.0;
+                amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
+                fro2 += v.x * v.x + v.y * v.y;
+            }
+    BlockScale s;
+    s.bad = wave_max(bad) != 0.0;
+    s.amax = wave_max(amax);
+    s.fro2 = wave_sum(fro2);
+    return s;
+}
+
+template <int B>
+__device__ inline bool chol_screen_cyc(c64 (&a)[B][B], int N, int lane, EigWork<B>* w, double shift) {
+    const int lr = lane & 7, lc = lane >> 3;
+    if (lr == lc) {
+#pragma unroll
+        for (int r = 0; r < B; ++r) a[r][r].x += shift;
+    }
+    c64* u = w->u;
+    bool ok = true;
+    static_for<0, B>([&](auto kcc) {
+        constexpr int kc = decltype(kcc)::value;
+        for (int kl = 0; kl < 8 && ok; ++kl) {
+            const int k = 8 * kc + kl;
+            if (k >= N) break;
+            if (lc == kl) {
+#pragma unroll
+                for (int r = kc; r < B; ++r) u[lr + 8 * r] = a[r][kc];
+            }
+            wave_sync();
+            const double piv = u[k].x;
+            if (!(piv > 0.0)) {
+                wave_sync();
+                ok = false;
+                break;
+            }
+            const double rs = rsqrt_nr(piv);
+            c64 li[B], lj[B];
+#pragma unroll
+            for (int r = kc; r < B; ++r) {
+                const double wi = lr + 8 * r > k ? rs : 0.0;
+                const double wj = lc + 8 * r > k ? rs : 0.0;
+                const c64 ui = u[lr + 8 * r], uj = u[lc + 8 * r];
+                li[r] = cmk(wi * ui.x, wi * ui.y);
+                lj[r] = cmk(wj * uj.x, -(wj * uj.y));
+            }
+#pragma unroll
+            for (int r = kc; r < B; ++r)
+#pragma unroll
+                for (int c = kc; c < B; ++c) a[r][c] = cfms(a[r][c], li[r], lj[c]);
+            wave_sync();  // the next step rewrites u
+        }
+    });
+    return ok;
+}
+
 // Screen-only forms of psd_from_tile and causality_from_tile for the first
-// pass of the residual suite: the same value wherever the finiteness check,
-// the zero block or the Cholesky screen settles the block; otherwise
-// `unsettled` is set and 0 returned, and a second pass computes the
-// eigenvalues (the full functions). Without the eigenvalue code the pass
-// needs far fewer registers.
+// pass of the residual suite: the value 0 (or Inf for a non-finite block)
+// wherever the finiteness check, the zero block or the Cholesky screen
+// settles the block; otherwise `unsettled` is set and 0 returned, and a
+// second pass computes the eigenvalues (the full functions, which repeat the
+// screen on the blocked grid). A screen that passes proves the ratio 0 in
+// either layout, so the outcome does not depend on which grid screened.
+// Without the eigenvalue code the pass needs far fewer registers.
 template <int B, int LD>
 __device__ inline double psd_screen_tile(const c64* X, int N, int lane, EigWork<B>* w, double fr, double fi,
                                          bool& unsettled) {
     constexpr double kEps = 2.220446049250313e-16;
     c64 a[B][B];
-    load_herm<B, LD>(X, N, lane, fr, fi, a);
-    const BlockScale s = block_scale<B>(a, N, lane);
+    load_herm_cyc<B, LD>(X, N, lane, fr, fi, a);
+    const BlockScale s = block_scale_cyc<B>(a, N, lane);
     if (s.bad) return INFINITY;
     if (s.amax == 0.0) return 0.0;
     int ex;
@@ -455,15 +549,15 @@ __device__ inline double psd_screen_tile(const c64* X, int N, int lane, EigWork<
     const double sc = ldexp(1.0, -ex);
     scale_regs<B>(a, sc);
     const double delta = 32.0 * N * kEps * fmax(sqrt(s.fro2 / N), 1.0);
-    if (!chol_screen<B>(a, N, lane, w, delta * sc)) unsettled = true;
+    if (!chol_screen_cyc<B>(a, N, lane, w, delta * sc)) unsettled = true;
     return 0.0;
 }
 
 template <int B, int LD>
 __device__ inline double causality_screen_tile(const c64* X, int N, int lane, EigWork<B>* w, bool& unsettled) {
     c64 a[B][B];
-    load_herm<B, LD>(X, N, lane, 0.0, 1.0, a);  // Hermitian part of i S = -Im S
-    const BlockScale s = block_scale<B>(a, N, lane);
+    load_herm_cyc<B, LD>(X, N, lane, 0.0, 1.0, a);  // Hermitian part of i S = -Im S
+    const BlockScale s = block_scale_cyc<B>(a, N, lane);
     if (s.bad) return INFINITY;
     if (s.amax == 0.0) return 0.0;
     int ex;
@@ -471,7 +565,7 @@ __device__ inline double causality_screen_tile(const c64* X, int N, int lane, Ei
     const double sc = ldexp(1.0, -ex);
     scale_regs<B>(a, sc);
     const double delta = 32.0 * N * 2.220446049250313e-16 * sqrt(s.fro2 / N);
-    if (!chol_screen<B>(a, N, lane, w, delta * sc)) unsettled = true;
+    if (!chol_screen_cyc<B>(a, N, lane, w, delta * sc)) unsettled = true;
     return 0.0;
 }
 
