@@ -81,10 +81,12 @@ constexpr size_t res_wave_bytes() {
 }
 
 // Waves (consecutive blocks) per workgroup: as many as keep two workgroups
-// within the 160 KiB of LDS of a CU, at most 4 (64-byte load segments).
+// within the 160 KiB of LDS of a CU, at most 4 (64-byte load segments);
+// four in one workgroup per CU where two would not fit.
 template <int B>
 constexpr int res_waves() {
-    return 4 * res_wave_bytes<B>() <= 81920 ? 4 : (2 * res_wave_bytes<B>() <= 81920 ? 2 : 1);
+    return 4 * res_wave_bytes<B>() <= 81920 ? 4
+           : (4 * res_wave_bytes<B>() <= 163840 ? 4 : (2 * res_wave_bytes<B>() <= 81920 ? 2 : 1));
 }
 
 // Screens of the suite (bit positions in the per-block fallback word).
