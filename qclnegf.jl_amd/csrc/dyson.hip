@@ -195,20 +195,21 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, c64* in_
                 R[kk] = cmk(fma(inv.x, oc, R[kk].x * zc), fma(inv.y, oc, R[kk].y * zc));
 #pragma unroll
                 for (int r = 0; r < B; ++r) A[r][kk] = cmk(A[r][kk].x * zc, A[r][kk].y * zc);
-                // Pivot row <- scaled pivot row: zero it and let f = -1 there.
 #pragma unroll
-                for (int r = 0; r < B; ++r) {
-                    const bool is_p = in_pg && pr == r;
-                    const double zr = is_p ? 0.0 : 1.0;
-                    pen[r] = is_p ? -INFINITY : pen[r];
-                    f[r] = cmk(fma(f[r].x, zr, zr - 1.0), f[r].y * zr);
-#pragma unroll
-                    for (int c = 0; c < B; ++c) A[r][c] = cmk(A[r][c].x * zr, A[r][c].y * zr);
-                }
+                for (int r = 0; r < B; ++r) pen[r] = (in_pg && pr == r) ? -INFINITY : pen[r];
 #pragma unroll
                 for (int r = 0; r < B; ++r)
 #pragma unroll
                     for (int c = 0; c < B; ++c) A[r][c] = cfms(A[r][c], f[r], R[c]);
+                // The pivot row becomes the scaled pivot row (pr is
+                // wave-uniform: one register row, on the lanes of group pg).
+                static_for<0, B>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    if (pr == r && in_pg) {
+#pragma unroll
+                        for (int c = 0; c < B; ++c) A[r][c] = R[c];
+                    }
+                });
                 wave_sync();
             }
         }
