@@ -448,10 +448,14 @@ int qclhip_embedding_adjoint(qclhip_ctx* ctx, const qclhip_grid* grid, const dou
  * out_host = r_D, r_A, r_K, r_PSD, r_caus of one SCBA iteration, from the
  * Green functions (GR, GL, GG, A), h (Nk,Nb,Nb), the families of the total
  * self-energy (summed in list order) and the unmixed candidate S< (families
- * summed likewise). One wavefront per (e,m) block; Hermitian eigenvalues by a
- * parallel-ordered Jacobi method; per-block values reduced in a fixed order
- * (sums for r_A and r_K, maxima otherwise; a non-finite value gives Inf).
- * Deferrable (5 doubles). */
+ * summed likewise). A may be NULL: A = i(G^R - G^R^dagger) is then formed
+ * from the G^R tile (bitwise the spectral function), for callers that do not
+ * store it. One wavefront per (e,m) block; the positivity and causality
+ * checks run Cholesky screens of H + delta I, and the blocks they do not
+ * settle get their extreme Hermitian eigenvalues (Householder reduction and
+ * Sturm multisection) in a second pass; per-block values reduced in a fixed
+ * order (sums for r_A and r_K, maxima otherwise; a non-finite value gives
+ * Inf). Deferrable (5 doubles). */
 int qclhip_residual_suite(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,
                           const double* GL, const double* GG, const double* A, int nfam,
                           const double* const* famR, const double* const* famL, const double* const* famG,

@@ -864,7 +864,7 @@ int qclhip_residual_suite(qclhip_ctx* ctx, const qclhip_grid* grid, const double
                           const double* GL, const double* GG, const double* A, int nfam,
                           const double* const* famR, const double* const* famL, const double* const* famG,
                           int ncand, const double* const* candL, double* out_host) {
-    if (!ctx || !grid || !h || !GR || !GL || !GG || !A || !famR || !famL || !famG || !candL || !out_host)
+    if (!ctx || !grid || !h || !GR || !GL || !GG || !famR || !famL || !famG || !candL || !out_host)
         return QCLHIP_ERR_ARGUMENT;
     if (nfam < 1 || ncand < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..8 families and candidate families");
     ResArgs a;

@@ -833,12 +833,11 @@ def antihermitian_part(grid: Grid, X):
 def residual_suite(grid: Grid, h, green, families, candidate_lesser):
     """production_residual_suite (residuals.jl:537-772) on the device.
 
-    green: a GreenState with A filled; families: (SR, SL, SG) triples of the
+    green: a GreenState (A may be None: formed from G^R in the kernel,
+    bitwise the spectral function); families: (SR, SL, SG) triples of the
     total self-energy; candidate_lesser: the unmixed candidate S< as a list of
     fields summed in order. Returns dict r_D, r_A, r_K, r_PSD, r_caus, or a
     Deferred of 5 values in deferred-results mode."""
-    if green.A is None:
-        raise _lib.ArgumentError("the residual suite needs the spectral function A")
     for fam in families:
         for t in fam:
             _check_field(grid, t, "self-energy")

@@ -169,12 +169,17 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
         return time.perf_counter()
 
     t0 = stamp()
+    # The Green state keeps no A: the residual suite forms A = i(G^R - G^R^dagger)
+    # from its G^R tile (bitwise the stored spectral function), and nothing
+    # else in the iteration reads it (api.spectral_function gives it on demand).
+    if s.green is not None:
+        s.green.A = None
     if ws.totals is not None and ws.totals_valid:
         green, info = api.current_green_production(g, p.h, [ws.totals.parts()], p.target, state=s.green,
-                                                   spectral=True)
+                                                   spectral=False)
     else:
         fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
-        green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=True,
+        green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=False,
                                                    totals=ws.totals.parts() if ws.totals is not None else None)
         ws.totals_valid = ws.totals is not None
     s.green = green
