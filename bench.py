@@ -157,6 +157,51 @@ def gf_bench(steps, warmup, variants=VARIANTS, autotune=True):
                 padded=padded)
 
 
+def pair_only(reps):
+    """The FP64 vector Dyson + Keldysh launch pair alone on the GF batch (one
+    family), reps times, for kernel counters."""
+    import torch
+
+    from qclhip import api, synthetic as syn
+
+    w = syn.gf_workload()
+    g = api.Grid(w["eps"], w["wE"], w["wk"], w["N"])
+    h = api.h_to_device(w["h"])
+    SR, SL, SG = syn.diag_fields_device(w)
+    GR, GL, GG = (torch.empty_like(SR) for _ in range(3))
+    traces = torch.empty(4 * w["NE"] * w["Nk"], dtype=torch.float64, device="cuda")
+    for _ in range(reps):
+        api.dyson_keldysh(g, h, SR, SL, SG, GR=GR, GL=GL, GG=GG, traces=traces)
+    g.ctx.synchronize()
+    torch.cuda.synchronize()
+    return dict(reps=reps, N=w["N"], NE=w["NE"], Nk=w["Nk"])
+
+
+def residuals_only(reps):
+    """The residual suite (every term) and its state pass (every term but
+    r_K) on the state of one SCBA iteration of the benchmark problem, reps
+    times each, for kernel counters."""
+    import torch
+
+    from qclhip import api, scba
+
+    p, s = scba_problem()
+    ws = scba.ScbaWorkspace(p)
+    scba.scba_iteration(p, s, ws)
+    g = p.grid
+    cands = [ws.plus.L, ws.minus.L] + [ws.scat[m.name].L for m in p.mechanisms]
+    out = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = api.residual_suite(g, p.h, s.green, [ws.totals.parts()], cands)
+        t1 = time.perf_counter()
+        api.residual_state(g, p.h, s.green, [ws.totals.parts()])
+        t2 = time.perf_counter()
+        out.append(dict(suite_ms=1e3 * (t1 - t0), state_ms=1e3 * (t2 - t1), suite=r))
+    return out
+
+
 def scba_problem(N=24, NE=2048, Nk=64):
     """The SCBA benchmark problem: seeded synthetic Hamiltonian, three
     mechanisms (momentum-independent, separable IFR, low-rank thermal LO),
@@ -802,6 +847,10 @@ def main():
     ap.add_argument("--variants", default=",".join(f"{p}_{e}" if p != "fp32_products" else p for p, e in VARIANTS),
                     help="comma-separated GF-solve variants to run (profiling runs select one)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--only-pair", type=int, default=0, metavar="REPS",
+                    help="run only the FP64 vector Dyson + Keldysh launch pair REPS times (kernel counters)")
+    ap.add_argument("--only-residuals", type=int, default=0, metavar="REPS",
+                    help="run only the residual suite and its state pass REPS times (kernel counters)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (the reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -820,6 +869,12 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy calibration: 3 x 1 GiB device-to-device copies")
+    if args.only_pair:
+        print(json.dumps({"pair": pair_only(args.only_pair)}), flush=True)
+        return
+    if args.only_residuals:
+        print(json.dumps({"residuals": residuals_only(args.only_residuals)}, default=float), flush=True)
+        return
     if args.only_dense:
         print(json.dumps({"dense_contraction": dense_contraction_bench()}, default=float), flush=True)
         return
