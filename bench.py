@@ -202,10 +202,12 @@ def residuals_only(reps):
     return out
 
 
-def scba_problem(N=24, NE=2048, Nk=64):
+def scba_problem(N=24, NE=2048, Nk=64, strength=1.0, coupling=1.0, lo_strength=1.0):
     """The SCBA benchmark problem: seeded synthetic Hamiltonian, three
     mechanisms (momentum-independent, separable IFR, low-rank thermal LO),
-    bulk embedding, seed state at a fixed chemical potential."""
+    bulk embedding, seed state at a fixed chemical potential. strength
+    scales every mechanism's kernel factor (lo_strength the LO one once
+    more) and coupling the embedding's T."""
     import numpy as np
 
     from qclhip import api, scba, synthetic as syn
@@ -215,13 +217,14 @@ def scba_problem(N=24, NE=2048, Nk=64):
     g = api.Grid(w["eps"], w["wE"], w["wk"], N)
     mechs = [
         scba.Mechanism("acoustic", api.ProductionKernel(g, "momentum_independent",
-                                                        syn.covariance_block(rng, N, 2, 0.04)), 1.0),
+                                                        syn.covariance_block(rng, N, 2, 0.04)), strength),
     ]
     l, r_, rad = syn.separable_factors(rng, N, Nk, J=3, scale=0.02)
-    mechs.append(scba.Mechanism("IFR", api.ProductionKernel(g, "separable_ifr", l, r_, rad, rank=3), 1.0))
+    mechs.append(scba.Mechanism("IFR", api.ProductionKernel(g, "separable_ifr", l, r_, rad, rank=3), strength))
     lo_l, lo_r = syn.low_rank_factors(rng, N, Nk, r=4, scale=0.02)
-    mechs.append(scba.Mechanism("LO", api.ProductionKernel(g, "low_rank", lo_l, lo_r, rank=4), 1.0, True))
-    T = 0.05 * (rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N)))
+    mechs.append(scba.Mechanism("LO", api.ProductionKernel(g, "low_rank", lo_l, lo_r, rank=4), strength * lo_strength,
+                                 True))
+    T = 0.05 * coupling * (rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N)))
     h = api.h_to_device(w["h"])
     kbt = w["kbt"]
     hw = 0.36
@@ -401,57 +404,136 @@ def scba_full_bench(iters=2, warmup=1):
     return out
 
 
-def gradient_bench(forward_tol=1e-10, adjoint_tol=1e-8, max_forward=400, max_adjoint=120, Nz=512):
+def _clone_state(s):
+    """A device copy of an SCBA state (the Green functions and every family)."""
+    from qclhip import api, scba
+
+    cl = lambda f: scba.Family(f.R.clone(), f.L.clone(), f.G.clone())
+    g = s.green
+    green = api.GreenState(g.GR.clone(), g.GL.clone(), g.GG.clone(), None if g.A is None else g.A.clone(),
+                           g.scale.clone())
+    return scba.ScbaState(green, {k: cl(v) for k, v in s.scattering.items()}, cl(s.plus), cl(s.minus))
+
+
+def _converge_logged(p, s, tol, max_iterations, label, chunk=100):
+    """adjoint.converge in chunks of `chunk` iterations with a progress line
+    per chunk; returns the iterations done."""
+    from qclhip import adjoint as adj
+
+    done = 0
+    while done < max_iterations:
+        n = min(chunk, max_iterations - done)
+        it = adj.converge(p, s, tol=tol, max_iterations=n)
+        done += it
+        J = s.history[-1]["Jbar"]
+        log(f"    {label}: {done} iterations, J {J:.15e}, last relative change "
+            f"{abs(J - s.history[-2]['Jbar']) / abs(J):.2e}")
+        if it < n:
+            break
+    return done
+
+
+def gradient_bench(forward_tol=1e-12, adjoint_tol=1e-9, max_forward=1500, max_adjoint=600, Nz=512,
+                   checkpoints=(100, 200, 400), fd_check=False, lo_strength=32.0):
     """The differentiation workload at N=24, NE=2048, Nk=64 (the SCBA
-    benchmark problem): the forward loop iterated towards its fixed point
-    (until the relative change of J is below forward_tol or max_forward
-    iterations), then the fixed-point adjoint dJ/dh (until the relative
-    change of the adjoint state is below adjoint_tol or max_adjoint
-    iterations; the implicit adjoint needs only the final state, so no
-    iterate is checkpointed),
-    carried to dJ/dU on an Nz-point potential grid (project_hamiltonians'
-    chain rule with a synthetic orthonormal basis) and to six layer
-    thicknesses shaping that potential (central differences of the host
-    assembly). The forward mixes linearly with alpha = 0.6 and without the
-    per-iteration measurement (neither moves the fixed point, hence the
-    gradient); the adjoint's linear fixed point is accelerated by Anderson
-    mixing over a window of 4. Both stop at their tolerance or iteration
-    cap; the achieved changes are reported."""
+    benchmark problem): the forward loop converged (relative change of J
+    below forward_tol, at most max_forward iterations), then the fixed-point
+    adjoint dJ/dh (until the relative change of the adjoint state is below
+    adjoint_tol or max_adjoint iterations; the implicit adjoint needs only
+    the converged state, so no iterate is checkpointed), carried to dJ/dU on
+    an Nz-point potential grid (project_hamiltonians' chain rule with a
+    synthetic orthonormal basis) and to six layer thicknesses shaping that
+    potential (central differences of the host assembly). Both fixed points
+    iterate undamped (alpha = 1) and without the per-iteration measurement
+    (neither moves the fixed point); dJ/dU is also formed at the adjoint's
+    checkpoints and its relative change between them is reported. With
+    fd_check, dJ/dU along a random potential direction is compared with the
+    central difference of the converged current (two more forward solves)."""
+    import dataclasses
+
     import numpy as np
     import torch
 
     from qclhip import adjoint as adj
     from qclhip import api
 
-    p, s = scba_problem()
-    p.alpha_mix = 0.6  # the fixed point does not depend on the damping
+    # The LO mechanism 32 times stronger than in the SCBA benchmark: with the
+    # benchmark's weak inelastic scattering the energy distribution relaxes
+    # through a slowly oscillating mode (the current still moves by 1e-8 per
+    # iteration after 1200 undamped iterations), with this one the fixed
+    # point is reached to 1e-14 within about 800.
+    p, s = scba_problem(lo_strength=lo_strength)
+    p.alpha_mix = 1.0  # the fixed point does not depend on the damping
     p.measure = False  # neither the fixed-point measurement nor the witness moves the fixed point
     p.verify_roundoff = False
     g = p.grid
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fwd = adj.converge(p, s, tol=forward_tol, max_iterations=max_forward)
+    fwd = _converge_logged(p, s, forward_tol, max_forward, "forward", chunk=200)
     g.ctx.synchronize()
     t_fwd = time.perf_counter() - t0
     J = s.history[-1]["Jbar"]
     dJ_last = abs(J - s.history[-2]["Jbar"]) / abs(J)
     log(f"  forward: {fwd} iterations in {t_fwd:.1f} s, J {J:.12e}, last relative change {dJ_last:.2e}")
+    rng = np.random.default_rng(91)
+    Q, _ = np.linalg.qr(rng.normal(size=(Nz, g.Nb)) + 1j * rng.normal(size=(Nz, g.Nb)))
+    h_host = api.h_to_host(p.h)
+    fd = None
+    if fd_check:
+        # central differences of the converged current along a random potential
+        # direction dU at two step sizes (max |dU| 0.2 and 0.1 meV) and their
+        # Richardson extrapolation; each side restarts from the converged state
+        # and runs fd_iterations undamped iterations (the per-iteration change
+        # of J alone can stop early on the problem's slow mode)
+        fd_iterations = 600
+        dU1 = rng.normal(size=Nz)
+        dU1 /= np.max(np.abs(dU1))
+        t0 = time.perf_counter()
+        cds = {}
+        for step in (2e-4, 1e-4):
+            dV = (Q.conj().T * (step * dU1)[None, :]) @ Q
+            Js = []
+            for sgn in (1.0, -1.0):
+                p2 = dataclasses.replace(p, h=api.h_to_device(h_host + sgn * dV[None, :, :]))
+                s2 = _clone_state(s)
+                _converge_logged(p2, s2, 0.0, fd_iterations, f"forward at U {'+' if sgn > 0 else '-'} {step:g} dU",
+                                 chunk=200)
+                Js.append(s2.history[-1]["Jbar"])
+                del p2, s2
+                torch.cuda.empty_cache()
+            cds[step] = (Js[0] - Js[1]) / (2.0 * step)
+        rich = (4.0 * cds[1e-4] - cds[2e-4]) / 3.0
+        fd = dict(steps_eV=[2e-4, 1e-4], central_differences=[cds[2e-4], cds[1e-4]], richardson=rich,
+                  forward_iterations_each=fd_iterations, seconds=time.perf_counter() - t0)
+        log(f"  finite differences (per unit step): {cds[2e-4]:.12e}, {cds[1e-4]:.12e}; Richardson {rich:.12e}")
     A = adj.ScbaAdjoint(p)
     t0 = time.perf_counter()
-    hbar, meta = A.gradient(s, tol=adjoint_tol, max_iterations=max_adjoint, acceleration="anderson", depth=4)
+    hbar, meta = A.gradient(s, tol=adjoint_tol, max_iterations=max_adjoint, acceleration="linear", alpha=1.0,
+                            checkpoints=checkpoints)
     g.ctx.synchronize()
     t_adj = time.perf_counter() - t0
     log(f"  adjoint: {meta['iterations']} iterations in {t_adj:.1f} s, change {meta['change']:.2e}")
-    rng = np.random.default_rng(91)
-    Q, _ = np.linalg.qr(rng.normal(size=(Nz, g.Nb)) + 1j * rng.normal(size=(Nz, g.Nb)))
     t0 = time.perf_counter()
     gU = api.potential_gradient(hbar, Q)
     t_U = time.perf_counter() - t0
+    steps = []
+    prev = None
+    for it_c, hb in list(meta["checkpoints"]) + [(meta["iterations"], hbar)]:
+        gc = api.potential_gradient(hb, Q)
+        if prev is not None:
+            steps.append(dict(from_iteration=prev[0], to_iteration=it_c,
+                              dJdU_rel_change=float(np.linalg.norm(gc - prev[1]) / np.linalg.norm(gc))))
+        prev = (it_c, gc)
+    if fd is not None:
+        fd["adjoint"] = float(np.dot(gU, dU1))
+        fd["rel_diff"] = [abs(fd["adjoint"] - c) / abs(c) for c in fd["central_differences"]]
+        fd["rel_diff_richardson"] = abs(fd["adjoint"] - fd["richardson"]) / abs(fd["richardson"])
+        log(f"  dJ/dU . dU {fd['adjoint']:.12e}: rel. diff {fd['rel_diff']} to the central differences, "
+            f"{fd['rel_diff_richardson']:.2e} to their Richardson extrapolation")
     z = np.linspace(0.0, 6.0, Nz)
     L0 = np.array([0.9, 1.3, 0.8, 1.1, 0.7, 1.2])
     edges = lambda L: np.concatenate([[0.0], np.cumsum(L)])
     sig = lambda x: 1.0 / (1.0 + np.exp(-x / 0.05))
-    h_host = api.h_to_host(p.h)
 
     def assemble(L):
         e = edges(L)
@@ -463,15 +545,18 @@ def gradient_bench(forward_tol=1e-10, adjoint_tol=1e-8, max_forward=400, max_adj
     gL = adj.thickness_gradient(hbar, assemble, L0, step=1e-4)
     t_L = time.perf_counter() - t0
     hb = hbar.cpu().numpy()
-    return dict(forward_iterations=fwd, forward_s=t_fwd, forward_last_rel_change=dJ_last, J=J,
-                adjoint_iterations=meta["iterations"], adjoint_s=t_adj, adjoint_change=meta["change"],
-                adjoint_ms_per_iter=1e3 * t_adj / max(meta["iterations"], 1),
-                hbar_norm=float(np.linalg.norm(hb)), dJdU_norm=float(np.linalg.norm(gU)), dJdU_s=t_U,
-                dJdU_sample=[float(x) for x in gU[::64]], dJdL=[float(x) for x in gL], dJdL_s=t_L,
-                config="N=24 NE=2048 Nk=64 SCBA benchmark problem (3 mechanisms + embedding), forward linear "
-                       "mixing alpha 0.6, adjoint Anderson-accelerated (window 4); "
-                       f"potential on Nz={Nz} points (synthetic orthonormal basis), six layer thicknesses "
-                       "shaping a barrier profile on that grid; fixed-point adjoint (no checkpoints needed)")
+    out = dict(forward_iterations=fwd, forward_s=t_fwd, forward_last_rel_change=dJ_last, J=J,
+               adjoint_iterations=meta["iterations"], adjoint_s=t_adj, adjoint_change=meta["change"],
+               adjoint_ms_per_iter=1e3 * t_adj / max(meta["iterations"], 1), dJdU_convergence=steps,
+               hbar_norm=float(np.linalg.norm(hb)), dJdU_norm=float(np.linalg.norm(gU)), dJdU_s=t_U,
+               dJdU_sample=[float(x) for x in gU[::64]], dJdL=[float(x) for x in gL], dJdL_s=t_L,
+               config=f"N=24 NE=2048 Nk=64 SCBA benchmark problem (3 mechanisms + embedding) with the LO "
+                      f"mechanism {lo_strength:g} times stronger, forward and adjoint fixed points iterated undamped; "
+                      f"potential on Nz={Nz} points (synthetic orthonormal basis), six layer thicknesses "
+                      "shaping a barrier profile on that grid; fixed-point adjoint (no checkpoints needed)")
+    if fd is not None:
+        out["fd_check"] = fd
+    return out
 
 
 def dense_contraction_bench(N=24, NE=2048, Nk=64, reps=2):
@@ -886,8 +971,9 @@ def main():
         print(json.dumps({"batch_scba": batch_scba_bench(args.batch_instances)}, default=float), flush=True)
         return
     if args.only_gradient:
-        log("gradient (forward and adjoint fixed points, iteration-capped): N=24 NE=2048 Nk=64")
-        print(json.dumps({"gradient": gradient_bench()}, default=float), flush=True)
+        log("gradient (forward converged, adjoint iteration-capped) with a finite-difference check: "
+            "N=24 NE=2048 Nk=64")
+        print(json.dumps({"gradient": gradient_bench(fd_check=True)}, default=float), flush=True)
         return
     if args.only_full:
         log("full SCBA iteration: GaAs/Al0.15Ga0.85As period, N=32 NE=4096 Nk=128, five mechanisms")
@@ -938,7 +1024,7 @@ def main():
             f"{bsc['one_by_one_ms_per_iteration']:.1f} one at a time")
     grad = None
     if not args.no_gradient:
-        log("gradient (forward and adjoint fixed points, iteration-capped): N=24 NE=2048 Nk=64")
+        log("gradient (forward converged, adjoint iteration-capped): N=24 NE=2048 Nk=64")
         grad = gradient_bench()
     gr = None
     if not args.no_graph:

@@ -87,13 +87,15 @@ class ScbaAdjoint:
             api.embedding_adjoint(g, T, plan, f.G, GGb)
 
     def gradient(self, s: ScbaState, tol: float = 1e-13, max_iterations: int = 5000, alpha: float | None = None,
-                 acceleration: str = "linear", depth: int = 5):
+                 acceleration: str = "linear", depth: int = 5, checkpoints=()):
         """dJ/dh (device tensor (Nb,Nb,Nk), the reference layout of h) at the
         converged state s, plus the adjoint state and iteration count.
         acceleration: "linear" (the damped iteration at the forward's rate)
         or "anderson" (Anderson acceleration of the same linear fixed point:
         the forward's mixer without its self-energy guard, restarts or
-        coefficient bound)."""
+        coefficient bound). checkpoints: iteration counts at which the
+        gradient of the current adjoint state is also formed; they are
+        returned as meta["checkpoints"] = [(iteration, dJ/dh), ...]."""
         p, g = self.p, self.p.grid
         alpha = p.alpha_mix if alpha is None else alpha
         if acceleration not in ("linear", "anderson"):
@@ -115,7 +117,18 @@ class ScbaAdjoint:
         sig = (api.empty_field(g), api.empty_field(g), api.empty_field(g))
         plus_new = (api.empty_field(g), api.empty_field(g))  # sig + the direct dJ/dplus
         it, change = 0, float("inf")
-        history = []
+        history, snaps = [], []
+
+        def dh_of_mu():
+            # dJ/dh of the current adjoint state (dJ/dh|_G + (dF/dh)^H mu)
+            api.fill_zero(GRb)
+            api.copy_field(JL, GLb)
+            api.copy_field(JG, GGb)
+            self.candidates_vjp(mu, (GRb, GLb, GGb))
+            _, hb = api.current_green_adjoint(g, fams, p.target, info, green, GLb, GGb, GRbar=GRb, want_hbar=True,
+                                              out=sig)
+            return hb
+
         for it in range(1, max_iterations + 1):
             api.fill_zero(GRb)
             api.copy_field(JL, GLb)
@@ -145,15 +158,11 @@ class ScbaAdjoint:
             change = (num / max(den, 1e-300)) ** 0.5
             if it == 1 or it % 100 == 0:
                 history.append((it, change))
+            if it in checkpoints:
+                snaps.append((it, dh_of_mu().clone()))
             if change <= tol:
                 break
-        api.fill_zero(GRb)
-        api.copy_field(JL, GLb)
-        api.copy_field(JG, GGb)
-        self.candidates_vjp(mu, (GRb, GLb, GGb))
-        _, hbar = api.current_green_adjoint(g, fams, p.target, info, green, GLb, GGb, GRbar=GRb, want_hbar=True,
-                                            out=sig)
-        return hbar, dict(iterations=it, change=change, info=info, history=history)
+        return dh_of_mu(), dict(iterations=it, change=change, info=info, history=history, checkpoints=snaps)
 
 
 def thickness_gradient(hbar, assemble, thick, step: float = 1e-3):

@@ -118,6 +118,34 @@ def test_thickness_gradient_vs_oracle_fd(hip, orc, converged):
         assert abs(fd - gL[i]) <= 1e-5 * abs(fd), (i, fd, gL[i])
 
 
+def test_adjoint_checkpoints(hip, converged):
+    """Gradients formed at adjoint checkpoints: the one at the last iteration
+    equals the returned gradient bitwise, and earlier ones approach it."""
+    import torch
+    from qclhip import adjoint, scba
+    c = converged
+    prob = c["prob"]
+    g = hip.Grid(prob["eps"], prob["wE"], prob["wk"], prob["Nb"])
+    mechs = [scba.Mechanism(m["name"], hip.ProductionKernel(g, m["kind"], m["a"], m["b"], m["c"], rank=m["rank"]),
+                            m["qK"], m["is_lo"]) for m in prob["mechanisms"]]
+    p = scba.ScbaProblem(g, hip.h_to_device(prob["h"]), hip.matrix_to_device(prob["Tplus"]),
+                         hip.matrix_to_device(prob["Tminus"]), mechs, c["full"]["target"], prob["Ep"], prob["hwLO"],
+                         prob["NLO"], prob["alpha_mix"], verify_roundoff=False)
+    dev = lambda x: hip.to_device(x)
+    st = c["state0"]
+    s = scba.ScbaState(None, {m["name"]: scba.Family(dev(st["scatR"][i]), dev(st["scatL"][i]), dev(st["scatG"][i]))
+                              for i, m in enumerate(prob["mechanisms"])},
+                       scba.Family(dev(st["plusR"]), dev(st["plusL"]), dev(st["plusG"])),
+                       scba.Family(dev(st["minusR"]), dev(st["minusL"]), dev(st["minusG"])))
+    adjoint.converge(p, s, tol=1e-15)
+    hbar, meta = adjoint.ScbaAdjoint(p).gradient(s, tol=0.0, max_iterations=60, checkpoints=(20, 40, 60))
+    its = [it for it, _ in meta["checkpoints"]]
+    assert its == [20, 40, 60]
+    assert torch.equal(meta["checkpoints"][-1][1], hbar)
+    err = [float(torch.linalg.norm(hb - c["hbar"]) / torch.linalg.norm(c["hbar"])) for _, hb in meta["checkpoints"]]
+    assert err[2] < err[0], err
+
+
 def test_anderson_adjoint_equals_linear(hip, orc):
     """The Anderson-accelerated adjoint iteration reaches the gradient of the
     damped linear one (the same fixed point) in fewer iterations, and a
