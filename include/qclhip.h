@@ -207,6 +207,21 @@ int qclhip_current_green_totals(qclhip_ctx* ctx, const qclhip_grid* grid, const 
                                 double target, double* GR, double* GL, double* GG, double* A, double* scale,
                                 double* out_host, double* totR, double* totL, double* totG);
 
+/* No counterpart in the reference (a fusion of two of its passes, for the
+ * SCBA iteration that measures every iteration): qclhip_current_green (with
+ * the totals when totR, totL, totG are given, else NULL) whose FP64 vector
+ * Keldysh kernel also runs the residual suite's positivity screens of the raw
+ * products -i G<raw and i G>raw (residuals.jl:699-728) on its product tiles
+ * and writes one screen word per block to raw_screen (device, NE*Nk bytes),
+ * for qclhip_residual_suite_prescreened / _state_prescreened on the same
+ * Green state and totals. Returns QCLHIP_ERR_UNSUPPORTED for the matrix-core
+ * and fused engines, for FP32 products and above Nb = 32. */
+int qclhip_current_green_screened(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
+                                  const double* const* famR, const double* const* famL,
+                                  const double* const* famG, double target, double* GR, double* GL, double* GG,
+                                  double* A, double* scale, double* out_host, double* totR, double* totL,
+                                  double* totG, unsigned char* raw_screen);
+
 /* qclhip_current_green that allocates nothing. No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
@@ -529,6 +544,25 @@ int qclhip_residual_state(qclhip_ctx* ctx, const qclhip_grid* grid, const double
                           double* out_host);
 int qclhip_residual_keldysh(qclhip_ctx* ctx, const qclhip_grid* grid, const double* GR, const double* GL,
                             int ncand, const double* const* candL, double* out_host);
+
+/* qclhip_residual_suite and qclhip_residual_state on a Green state that
+ * qclhip_current_green_screened produced from the total self-energy given
+ * here (famR/famL/famG summing to the totals it loaded): the raw-product
+ * screens are taken from raw_screen instead of being formed again (same
+ * products, same arithmetic, so every result equals the plain entries'
+ * bit for bit); blocks whose raw screens did not settle get their
+ * eigenvalues from the products formed again in the second pass. No
+ * counterpart in the reference. */
+int qclhip_residual_suite_prescreened(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,
+                                      const double* GL, const double* GG, const double* A, int nfam,
+                                      const double* const* famR, const double* const* famL,
+                                      const double* const* famG, int ncand, const double* const* candL,
+                                      const unsigned char* raw_screen, double* out_host);
+int qclhip_residual_state_prescreened(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,
+                                      const double* GL, const double* GG, const double* A, int nfam,
+                                      const double* const* famR, const double* const* famL,
+                                      const double* const* famG, const unsigned char* raw_screen,
+                                      double* out_host);
 
 /* ---- The Poisson step closing the SCBA loop (production_solver.jl:
  * 1044-1047). All arrays are device memory. */

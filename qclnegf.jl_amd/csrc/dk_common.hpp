@@ -69,6 +69,11 @@ struct DKArgs {
     // autotuner), and waves per block of the vector Keldysh kernel (0: the
     // rule; S > 1 splits a block's rows over S waves, for N > 24).
     int wd, wk, ks;
+    // Optional (FP64 vector Keldysh kernels): one screen word per block, the
+    // Cholesky screens of the raw products -i G<raw and i G>raw as the
+    // residual suite runs them (kScrRawL, kScrRawG: not settled;
+    // kScrNonFinite), for the suite's prescreened mode.
+    unsigned char* raw_screen;
 };
 
 // out[r][c] (lane block) = Gs[rows] * Xs[:, cols]  or  Xs[rows] * Gs[cols]^dagger

@@ -21,7 +21,7 @@
 #include <cstddef>
 #include <mutex>
 
-#include "dk_common.hpp"
+#include "wave_eig.hpp"
 
 namespace qclhip {
 
@@ -415,8 +415,11 @@ __device__ inline void keldysh_products_split(c64* Gs, c64* Xs, c64* diag, int N
 
 // The Keldysh products with SPLIT waves per block (W blocks per workgroup),
 // no register prefetch; otherwise the k_keldysh loop.
-template <int B, int W, int SPLIT>
-__global__ __launch_bounds__(64 * SPLIT * W) void k_keldysh_split(DKArgs p, long long ntiles) {
+// With SCR the first wave of each block also runs the raw-product screens
+// (the residual suite's Cholesky screens on -i G<raw and i G>raw) and stores
+// the block's screen word in p.raw_screen.
+template <int B, int W, int SPLIT, bool SCR>
+__device__ __forceinline__ void keldysh_split_body(const DKArgs& p, long long ntiles) {
     constexpr int LD = DKShape<B>::LD;
     constexpr int T = 64 * SPLIT * W;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -430,6 +433,7 @@ __global__ __launch_bounds__(64 * SPLIT * W) void k_keldysh_split(DKArgs p, long
     c64* X0 = G0 + (size_t)W * wstride;
     c64* Gs = G0 + (size_t)b * wstride;
     c64* Xs = X0 + (size_t)b * wstride;
+    EigWork<B>* ew = reinterpret_cast<EigWork<B>*>(X0 + (size_t)W * wstride) + b;  // SCR only
     const int N = p.N;
     const size_t S = p.plane ? (size_t)p.plane : (size_t)nblk;
     const c64* gr_fam[1] = {p.GR};
@@ -439,6 +443,7 @@ __global__ __launch_bounds__(64 * SPLIT * W) void k_keldysh_split(DKArgs p, long
         const long long blk0 = tc.blk0;
         const long long blk = blk0 + b;
         const bool active = b < nvalid;
+        unsigned sbits = 0;
         const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
         __syncthreads();  // the previous tile's LDS is free
         tile_load<W, LD, T>(G0, wstride, gr_fam, 1, blk0, nvalid, N, S);
@@ -458,13 +463,38 @@ __global__ __launch_bounds__(64 * SPLIT * W) void k_keldysh_split(DKArgs p, long
             }
             keldysh_products_split<B, SPLIT>(Gs, Xs, diag[b], N, lane, s, active,
                                              p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr);
+            if constexpr (SCR) {
+                if (active && s == 0) {
+                    bool u = false;
+                    const double v = psd_screen_tile<B, LD>(Xs, N, lane, ew, 0.0, comp ? 1.0 : -1.0, u);
+                    sbits |= (u ? (comp ? kScrRawG : kScrRawL) : 0u) | (isinf(v) ? kScrNonFinite : 0u);
+                    if (comp == 1 && lane == 0) p.raw_screen[blk] = (unsigned char)sbits;
+                }
+            }
             tile_store<W, LD, false, T>(X0, wstride, dst, blk0, mask, N, S);
             if (comp == 0) __syncthreads();  // before S> replaces the tile
         }
     }
 }
 
-template <int B, int W>
+template <int B, int W, int SPLIT>
+__global__ __launch_bounds__(64 * SPLIT * W) void k_keldysh_split(DKArgs p, long long ntiles) {
+    keldysh_split_body<B, W, SPLIT, false>(p, ntiles);
+}
+
+// The screening variant, held to four waves per SIMD (the screens would
+// otherwise raise the register count above 128 and cost a wave per SIMD).
+template <int B, int W, int SPLIT>
+__global__ __launch_bounds__(64 * SPLIT * W) __attribute__((amdgpu_waves_per_eu(4))) void k_keldysh_split_scr(
+    DKArgs p, long long ntiles) {
+    keldysh_split_body<B, W, SPLIT, true>(p, ntiles);
+}
+
+// With SCR each wave also runs the raw-product screens of its block (the
+// residual suite's Cholesky screens on -i G<raw and i G>raw, on the product
+// tile before it is stored) and stores the block's screen word in
+// p.raw_screen.
+template <int B, int W, bool SCR = false>
 __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) {
     constexpr int NP = DKShape<B>::NP;
     constexpr int LD = DKShape<B>::LD;
@@ -477,6 +507,7 @@ __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) 
     c64* X0 = G0 + (size_t)W * wstride;
     c64* Gs = G0 + (size_t)wave * wstride;
     c64* Xs = X0 + (size_t)wave * wstride;
+    EigWork<B>* ew = reinterpret_cast<EigWork<B>*>(X0 + (size_t)W * wstride) + wave;  // SCR only
     const int N = p.N;
     const size_t S = p.plane ? (size_t)p.plane : (size_t)nblk;
     const int lr = lane & 7, lc = lane >> 3;
@@ -506,6 +537,7 @@ __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) 
         const long long blk = blk0 + wave;
         const bool active = wave < nvalid;
         const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
+        unsigned sbits = 0;
         __syncthreads();  // the previous tile's LDS is free
         if constexpr (kPrefetch) {
             tm.put(G0, wstride, rG, nvalid);
@@ -538,6 +570,14 @@ __global__ __launch_bounds__(64 * W) void k_keldysh(DKArgs p, long long ntiles) 
                 __syncthreads();  // the products overwrite the tiles
             }
             if (active) keldysh_products<B>(Gs, Xs, N, lane, p.traces ? p.traces + (size_t)comp * nblk + blk : nullptr);
+            if constexpr (SCR) {
+                if (active) {
+                    bool u = false;
+                    const double v = psd_screen_tile<B, LD>(Xs, N, lane, ew, 0.0, comp ? 1.0 : -1.0, u);
+                    sbits |= (u ? (comp ? kScrRawG : kScrRawL) : 0u) | (isinf(v) ? kScrNonFinite : 0u);
+                    if (comp == 1 && lane == 0) p.raw_screen[blk] = (unsigned char)sbits;
+                }
+            }
             __syncthreads();
             tile_store<W, LD>(X0, wstride, dst, blk0, mask, N, S);
             // before S> replaces the tile; after comp 1 the next tile's
@@ -1049,21 +1089,26 @@ int launch_dyson(qclhip_ctx* ctx, const DKArgs& a) {
     return launch_dyson_w<B, kDysonTileWaves>(ctx, a);
 }
 
-template <int B, int W>
-int launch_keldysh_w(qclhip_ctx* ctx, const DKArgs& a) {
-    constexpr size_t per_wave = 2 * (size_t)TileStride<B>::value * sizeof(c64);
+template <int B, int W, bool SCR>
+int launch_keldysh_ws(qclhip_ctx* ctx, const DKArgs& a) {
+    constexpr size_t per_wave = 2 * (size_t)TileStride<B>::value * sizeof(c64) + (SCR ? sizeof(EigWork<B>) : 0);
     const size_t lds = per_wave * W;
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
     if (lds > 65536)
-        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_keldysh<B, W>,
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_keldysh<B, W, SCR>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
     unsigned grid = 0;
-    int rc = persistent_grid(ctx, (const void*)k_keldysh<B, W>, 64 * W, lds, tiles, &grid);
+    int rc = persistent_grid(ctx, (const void*)k_keldysh<B, W, SCR>, 64 * W, lds, tiles, &grid);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_keldysh<B, W>), dim3(grid), dim3(64 * W), lds, ctx->stream, a, tiles);
+    hipLaunchKernelGGL((k_keldysh<B, W, SCR>), dim3(grid), dim3(64 * W), lds, ctx->stream, a, tiles);
     return launch_check(ctx, "k_keldysh");
+}
+
+template <int B, int W>
+int launch_keldysh_w(qclhip_ctx* ctx, const DKArgs& a) {
+    return a.raw_screen ? launch_keldysh_ws<B, W, true>(ctx, a) : launch_keldysh_ws<B, W, false>(ctx, a);
 }
 
 // Waves per block when the autotuner has not chosen (index B): one wave
@@ -1073,21 +1118,41 @@ int launch_keldysh_w(qclhip_ctx* ctx, const DKArgs& a) {
 // N = 40, 19.7 -> 9.3 ms at N = 48).
 constexpr int kDefaultKeldyshSplit[7] = {1, 1, 1, 1, 4, 5, 6};
 
-template <int B, int W, int SPLIT>
-int launch_keldysh_split_w(qclhip_ctx* ctx, const DKArgs& a) {
-    constexpr size_t per_block = 2 * (size_t)TileStride<B>::value * sizeof(c64);
+template <int B, int W, int SPLIT, bool SCR>
+int launch_keldysh_split_ws(qclhip_ctx* ctx, const DKArgs& a) {
+    constexpr size_t per_block = 2 * (size_t)TileStride<B>::value * sizeof(c64) + (SCR ? sizeof(EigWork<B>) : 0);
     const size_t lds = per_block * W;
+    const void* kern;
+    if constexpr (SCR)
+        kern = (const void*)k_keldysh_split_scr<B, W, SPLIT>;
+    else
+        kern = (const void*)k_keldysh_split<B, W, SPLIT>;
     if (lds > ctx->max_lds_per_block)
         return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
     if (lds > 65536)
-        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_keldysh_split<B, W, SPLIT>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
     unsigned grid = 0;
-    int rc = persistent_grid(ctx, (const void*)k_keldysh_split<B, W, SPLIT>, 64 * SPLIT * W, lds, tiles, &grid);
+    int rc = persistent_grid(ctx, kern, 64 * SPLIT * W, lds, tiles, &grid);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_keldysh_split<B, W, SPLIT>), dim3(grid), dim3(64 * SPLIT * W), lds, ctx->stream, a, tiles);
+    if constexpr (SCR)
+        hipLaunchKernelGGL((k_keldysh_split_scr<B, W, SPLIT>), dim3(grid), dim3(64 * SPLIT * W), lds, ctx->stream, a,
+                           tiles);
+    else
+        hipLaunchKernelGGL((k_keldysh_split<B, W, SPLIT>), dim3(grid), dim3(64 * SPLIT * W), lds, ctx->stream, a,
+                           tiles);
     return launch_check(ctx, "k_keldysh_split");
+}
+
+// Raw-product screens in the split kernels up to N = 32 (B = 4); above, the
+// screening variant would spill, and the caller leaves the raw products to
+// the residual suite (qclhip_current_green_screened refuses N > 32).
+template <int B, int W, int SPLIT>
+int launch_keldysh_split_w(qclhip_ctx* ctx, const DKArgs& a) {
+    if constexpr (B <= 4) {
+        if (a.raw_screen) return launch_keldysh_split_ws<B, W, SPLIT, true>(ctx, a);
+    }
+    return launch_keldysh_split_ws<B, W, SPLIT, false>(ctx, a);
 }
 
 // Waves per block of the vector Keldysh kernel compiled for each tile size:
@@ -1479,8 +1544,12 @@ static int current_green_impl(qclhip_ctx* ctx, const qclhip_grid* grid, const do
                               const double* const* famR, const double* const* famL, const double* const* famG,
                               double target, double* GR, double* GL, double* GG, double* A, double* scale,
                               double* out_host, double* const* totals, bool scalar_lesser = false,
-                              c64* traces_ext = nullptr) {
+                              c64* traces_ext = nullptr, unsigned char* raw_screen = nullptr) {
     if (!ctx || !grid || !h || !famR || !famL || !famG || !GR || !GL || !GG) return QCLHIP_ERR_ARGUMENT;
+    if (raw_screen && (grid->keldysh_engine != QCLHIP_ENGINE_VECTOR ||
+                       grid->precision == QCLHIP_PRECISION_FP32_PRODUCTS || grid->Nb > 32))
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED,
+                    "raw-product screens run in the FP64 vector Keldysh kernels at Nb <= 32 only");
     if (nfam < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "at least the embedding family is required");
     if (!(std::isfinite(target) && target > 0))
         return fail(ctx, QCLHIP_ERR_ARGUMENT, "target charge must be finite and positive");
@@ -1542,6 +1611,7 @@ static int current_green_impl(qclhip_ctx* ctx, const qclhip_grid* grid, const do
     a.traces = traces;
     a.status = ctx->status;
     a.keldysh = 1;
+    a.raw_screen = raw_screen;
     if (totals) {
         const bool in_kernels = fused && grid->precision == QCLHIP_PRECISION_FP64 &&
                                 grid->keldysh_engine != QCLHIP_ENGINE_MATRIX && a.N <= 48;
@@ -1627,6 +1697,18 @@ int qclhip_current_green_totals(qclhip_ctx* ctx, const qclhip_grid* grid, const 
     if (!totR || !totL || !totG) return QCLHIP_ERR_ARGUMENT;
     double* const tot[3] = {totR, totL, totG};
     return current_green_impl(ctx, grid, h, nfam, famR, famL, famG, target, GR, GL, GG, A, scale, out_host, tot);
+}
+
+int qclhip_current_green_screened(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,
+                                  const double* const* famR, const double* const* famL,
+                                  const double* const* famG, double target, double* GR, double* GL, double* GG,
+                                  double* A, double* scale, double* out_host, double* totR, double* totL,
+                                  double* totG, unsigned char* raw_screen) {
+    if (!raw_screen) return QCLHIP_ERR_ARGUMENT;
+    if ((totR || totL || totG) && !(totR && totL && totG)) return QCLHIP_ERR_ARGUMENT;
+    double* const tot[3] = {totR, totL, totG};
+    return current_green_impl(ctx, grid, h, nfam, famR, famL, famG, target, GR, GL, GG, A, scale, out_host,
+                              totR ? tot : nullptr, false, nullptr, raw_screen);
 }
 
 int qclhip_current_green_noalloc(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, int nfam,

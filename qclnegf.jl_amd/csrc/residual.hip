@@ -36,6 +36,9 @@ struct ResArgs {
     // appended through the counter (zeroed before the first pass).
     unsigned long long* flagged;
     unsigned* nflagged;
+    // Prescreened mode: the Keldysh kernel's screen word per block for the
+    // raw products (NULL: the suite forms and screens them itself).
+    const unsigned char* raw_screen;
 };
 
 // A = i (G - G^dagger) of the wave's own G^R tile into X, the arithmetic of
@@ -88,17 +91,6 @@ constexpr int res_waves() {
     return 4 * res_wave_bytes<B>() <= 81920 ? 4
            : (4 * res_wave_bytes<B>() <= 163840 ? 4 : (2 * res_wave_bytes<B>() <= 81920 ? 2 : 1));
 }
-
-// Screens of the suite (bit positions in the per-block fallback word).
-enum ResScreen : unsigned {
-    kScrCaus = 1u << 0,   // lambda_max(Im S^R)
-    kScrGamma = 1u << 1,  // Gamma
-    kScrA = 1u << 2,      // A
-    kScrLess = 1u << 3,   // -i G<
-    kScrGrt = 1u << 4,    // i G>
-    kScrRawL = 1u << 5,   // -i G^R S< G^R^dagger
-    kScrRawG = 1u << 6    // i G^R S> G^R^dagger
-};
 
 // Register prefetch of Gamma = i (S> - S<) of the summed families with the
 // thread mapping of TileMap (the value of tile_load_gamma).
@@ -293,19 +285,30 @@ __global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
     }
     // G>.
     put();
-    fetch(p.famL, p.nfam);
+    if (!p.raw_screen) fetch(p.famL, p.nfam);
     if (valid) psd = fmax(psd, screen(kScrGrt, 0.0, 1.0));
-    // The raw products G S< G^dagger and G S> G^dagger.
-    put();
-    fetch(p.famG, p.nfam);
-    if (valid) {
-        res_sandwich<B, true>(G, X, N, lane, T);
-        psd = fmax(psd, screen(kScrRawL, 0.0, -1.0));
-    }
-    put();
-    if (valid) {
-        res_sandwich<B, true>(G, X, N, lane, T);
-        psd = fmax(psd, screen(kScrRawG, 0.0, 1.0));
+    // The raw products G S< G^dagger and G S> G^dagger: screened here, or
+    // (prescreened mode) by the Keldysh kernel that formed them from the same
+    // G^R and total self-energy with this arithmetic, whose screen word is
+    // taken instead (a raw product with a non-finite entry gives Inf).
+    if (p.raw_screen) {
+        if (valid) {
+            const unsigned rb = p.raw_screen[blk];
+            if (rb & kScrNonFinite) psd = INFINITY;
+            unsettled |= rb & (kScrRawL | kScrRawG);
+        }
+    } else {
+        put();
+        fetch(p.famG, p.nfam);
+        if (valid) {
+            res_sandwich<B, true>(G, X, N, lane, T);
+            psd = fmax(psd, screen(kScrRawL, 0.0, -1.0));
+        }
+        put();
+        if (valid) {
+            res_sandwich<B, true>(G, X, N, lane, T);
+            psd = fmax(psd, screen(kScrRawG, 0.0, 1.0));
+        }
     }
     if (valid && lane == 0) {
         p.out[0 * nblk + blk] = rD;
@@ -872,6 +875,38 @@ int qclhip_residual_suite(qclhip_ctx* ctx, const qclhip_grid* grid, const double
     if (rc) return rc;
     a.mode = 0;
     return residual_run(ctx, a, false, 0x7fu, out_host);
+}
+
+int qclhip_residual_suite_prescreened(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,
+                                      const double* GL, const double* GG, const double* A, int nfam,
+                                      const double* const* famR, const double* const* famL,
+                                      const double* const* famG, int ncand, const double* const* candL,
+                                      const unsigned char* raw_screen, double* out_host) {
+    if (!ctx || !grid || !h || !GR || !GL || !GG || !famR || !famL || !famG || !candL || !raw_screen || !out_host)
+        return QCLHIP_ERR_ARGUMENT;
+    if (nfam < 1 || ncand < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..8 families and candidate families");
+    ResArgs a;
+    int rc = residual_args(ctx, grid, h, GR, GL, GG, A, nfam, famR, famL, famG, ncand, candL, &a);
+    if (rc) return rc;
+    a.mode = 0;
+    a.raw_screen = raw_screen;
+    return residual_run(ctx, a, false, 0x7fu, out_host);
+}
+
+int qclhip_residual_state_prescreened(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,
+                                      const double* GL, const double* GG, const double* A, int nfam,
+                                      const double* const* famR, const double* const* famL,
+                                      const double* const* famG, const unsigned char* raw_screen,
+                                      double* out_host) {
+    if (!ctx || !grid || !h || !GR || !GL || !GG || !famR || !famL || !famG || !raw_screen || !out_host)
+        return QCLHIP_ERR_ARGUMENT;
+    if (nfam < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..8 families");
+    ResArgs a;
+    int rc = residual_args(ctx, grid, h, GR, GL, GG, A, nfam, famR, famL, famG, 0, nullptr, &a);
+    if (rc) return rc;
+    a.mode = 1;
+    a.raw_screen = raw_screen;
+    return residual_run(ctx, a, false, 0x67u, out_host);  // rows 0, 1, 2, 5, 6
 }
 
 int qclhip_residual_state(qclhip_ctx* ctx, const qclhip_grid* grid, const double* h, const double* GR,

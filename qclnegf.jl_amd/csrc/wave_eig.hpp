@@ -9,6 +9,10 @@
 
 namespace qclhip {
 
+// out (lane block) = L R or L R^dagger (radj) of two LDS tiles, with the
+// per-element arithmetic of block_product (the Keldysh kernels' products:
+// FMAs in q order, cfmac for the adjoint), so a sandwich G X G^dagger formed
+// here equals the Keldysh kernel's bit for bit.
 template <int B>
 __device__ inline void rprod(const c64* __restrict__ L, const c64* __restrict__ R, int N, int lane,
                              c64 (&out)[B][B], bool radj) {
@@ -23,11 +27,12 @@ __device__ inline void rprod(const c64* __restrict__ L, const c64* __restrict__ 
 #pragma unroll
         for (int r = 0; r < B; ++r) lv[r] = L[(lr * B + r) * LD + q];
 #pragma unroll
-        for (int c = 0; c < B; ++c) rv[c] = radj ? cconj(R[(lc * B + c) * LD + q]) : R[q * LD + lc * B + c];
+        for (int c = 0; c < B; ++c) rv[c] = radj ? R[(lc * B + c) * LD + q] : R[q * LD + lc * B + c];
 #pragma unroll
         for (int r = 0; r < B; ++r)
 #pragma unroll
-            for (int c = 0; c < B; ++c) out[r][c] = cfma(out[r][c], lv[r], rv[c]);
+            for (int c = 0; c < B; ++c)
+                out[r][c] = radj ? cfmac(out[r][c], lv[r], rv[c]) : cfma(out[r][c], lv[r], rv[c]);
     }
 }
 
@@ -526,6 +531,20 @@ __device__ inline bool chol_screen_cyc(c64 (&a)[B][B], int N, int lane, EigWork<
     });
     return ok;
 }
+
+// Checks of the residual suite, as bits of a block's screen word: the
+// suite's list of unsettled blocks and the Keldysh kernels' raw-product
+// screens (with kScrNonFinite for a raw product with a non-finite entry).
+enum ResScreen : unsigned {
+    kScrCaus = 1u << 0,   // lambda_max(Im S^R)
+    kScrGamma = 1u << 1,  // Gamma
+    kScrA = 1u << 2,      // A
+    kScrLess = 1u << 3,   // -i G<
+    kScrGrt = 1u << 4,    // i G>
+    kScrRawL = 1u << 5,   // -i G^R S< G^R^dagger
+    kScrRawG = 1u << 6,   // i G^R S> G^R^dagger
+    kScrNonFinite = 1u << 7
+};
 
 // Screen-only forms of psd_from_tile and causality_from_tile for the first
 // pass of the residual suite: the value 0 (or Inf for a non-finite block)

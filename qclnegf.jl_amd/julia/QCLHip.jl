@@ -172,6 +172,29 @@ function current_green_coefficients!(g::Grid, h, families, target::Real, Gᴿ, G
     return out
 end
 
+"""`current_green_coefficients!` that also writes the totals (`totals`: the
+(Σᴿ, Σˡ, Σᵍ) fields, or `nothing`) and the raw-product screen word of every
+(e,m) block to `raw_screen` (a device `UInt8` array of NE*Nk entries), for
+`residual_suite(...; raw_screen)` / `residual_state(...; raw_screen)` on the
+same state: the measured SCBA iteration's fused path (FP64 vector engine,
+N_b <= 32). No counterpart in the reference."""
+function current_green_screened!(g::Grid, h, families, target::Real, Gᴿ, Gˡ, Gᵍ, A, scale, raw_screen;
+                                 totals = nothing)
+    n = length(families)
+    fr = Ptr{Cvoid}[_ptr(f[1]) for f in families]
+    fl = Ptr{Cvoid}[_ptr(f[2]) for f in families]
+    fg = Ptr{Cvoid}[_ptr(f[3]) for f in families]
+    tp = totals === nothing ? (C_NULL, C_NULL, C_NULL) : map(_ptr, totals)
+    out = zeros(Float64, 5)
+    check(g.ctx, ccall((:qclhip_current_green_screened, libqclhip), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
+                        Cdouble, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64},
+                        Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                       g.ctx.handle, g.handle, _ptr(h), n, fr, fl, fg, target, _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(A),
+                       _ptr(scale), out, tp[1], tp[2], tp[3], _ptr(raw_screen)), "current_green_screened")
+    return out
+end
+
 """Device `EnergyShiftPlan` (production_cache.jl:258-288)."""
 mutable struct ShiftPlan
     ctx::Context
@@ -305,19 +328,31 @@ function state_populations(g::Grid, Gˡ, target::Real)
     return out
 end
 
-"""`production_residual_suite` (residuals.jl:537-772): (r_D, r_A, r_K, r_PSD, r_caus)."""
-function residual_suite(g::Grid, h, Gᴿ, Gˡ, Gᵍ, A, families, candidate_lesser)
+"""`production_residual_suite` (residuals.jl:537-772): (r_D, r_A, r_K, r_PSD, r_caus).
+`A` may be `nothing` (formed from Gᴿ); with `raw_screen` (from
+`current_green_screened!` on the same state and totals) the raw-product
+screens are taken from it, with the same results."""
+function residual_suite(g::Grid, h, Gᴿ, Gˡ, Gᵍ, A, families, candidate_lesser; raw_screen = nothing)
     n = length(families)
     fr = Ptr{Cvoid}[_ptr(f[1]) for f in families]
     fl = Ptr{Cvoid}[_ptr(f[2]) for f in families]
     fg = Ptr{Cvoid}[_ptr(f[3]) for f in families]
     cl = Ptr{Cvoid}[_ptr(c) for c in candidate_lesser]
     out = zeros(Float64, 5)
-    check(g.ctx, ccall((:qclhip_residual_suite, libqclhip), Cint,
-                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
-                        Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Float64}),
-                       g.ctx.handle, g.handle, _ptr(h), _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(A), n, fr, fl, fg,
-                       length(cl), cl, out), "residual_suite")
+    if raw_screen === nothing
+        check(g.ctx, ccall((:qclhip_residual_suite, libqclhip), Cint,
+                           (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
+                            Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Float64}),
+                           g.ctx.handle, g.handle, _ptr(h), _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(A), n, fr, fl, fg,
+                           length(cl), cl, out), "residual_suite")
+    else
+        check(g.ctx, ccall((:qclhip_residual_suite_prescreened, libqclhip), Cint,
+                           (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
+                            Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Cvoid},
+                            Ptr{Float64}),
+                           g.ctx.handle, g.handle, _ptr(h), _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(A), n, fr, fl, fg,
+                           length(cl), cl, _ptr(raw_screen), out), "residual_suite_prescreened")
+    end
     return (r_D = out[1], r_A = out[2], r_K = out[3], r_PSD = out[4], r_caus = out[5])
 end
 
@@ -548,16 +583,27 @@ set_mode!(b::Batch, mode::Symbol) = check(b.ctx, ccall((:qclhip_batch_set_mode, 
     (Ptr{Cvoid}, Ptr{Cvoid}, Cint), b.ctx.handle, b.handle, mode === :per_instance ? 1 : 0), "batch_set_mode")
 
 # -- split residual suite -----------------------------------------------------
-"""Every residual but r_K, before the candidates exist; `A` may be `nothing`."""
-function residual_state(g::Grid, h, Gᴿ, Gˡ, Gᵍ, A, families)
+"""Every residual but r_K, before the candidates exist; `A` may be `nothing`;
+`raw_screen` as for `residual_suite`."""
+function residual_state(g::Grid, h, Gᴿ, Gˡ, Gᵍ, A, families; raw_screen = nothing)
     n = length(families)
     out = zeros(Float64, 5)
-    check(g.ctx, ccall((:qclhip_residual_state, libqclhip), Cint,
-                       (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
-                        Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Float64}),
-                       g.ctx.handle, g.handle, _ptr(h), _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(A), n,
-                       Ptr{Cvoid}[_ptr(f[1]) for f in families], Ptr{Cvoid}[_ptr(f[2]) for f in families],
-                       Ptr{Cvoid}[_ptr(f[3]) for f in families], out), "residual_state")
+    fr = Ptr{Cvoid}[_ptr(f[1]) for f in families]
+    fl = Ptr{Cvoid}[_ptr(f[2]) for f in families]
+    fg = Ptr{Cvoid}[_ptr(f[3]) for f in families]
+    if raw_screen === nothing
+        check(g.ctx, ccall((:qclhip_residual_state, libqclhip), Cint,
+                           (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
+                            Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Float64}),
+                           g.ctx.handle, g.handle, _ptr(h), _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(A), n, fr, fl, fg, out),
+              "residual_state")
+    else
+        check(g.ctx, ccall((:qclhip_residual_state_prescreened, libqclhip), Cint,
+                           (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
+                            Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Float64}),
+                           g.ctx.handle, g.handle, _ptr(h), _ptr(Gᴿ), _ptr(Gˡ), _ptr(Gᵍ), _ptr(A), n, fr, fl, fg,
+                           _ptr(raw_screen), out), "residual_state_prescreened")
+    end
     return (r_D = out[1], r_A = out[2], r_PSD = out[4], r_caus = out[5])
 end
 

@@ -81,6 +81,8 @@ _SIGNATURES = {
                              ct.c_double, _vp, _vp, _vp, _vp, _vp, _dp],
     "qclhip_current_green_totals": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),
                                     ct.c_double, _vp, _vp, _vp, _vp, _vp, _dp, _vp, _vp, _vp],
+    "qclhip_current_green_screened": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),
+                                      ct.c_double, _vp, _vp, _vp, _vp, _vp, _dp, _vp, _vp, _vp, _vp],
     "qclhip_current_green_noalloc": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.POINTER(_vp),
                                      ct.c_double, _vp, _vp, _vp, _vp, _vp, _dp],
     "qclhip_current_green_scalar_lesser": [_vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),
@@ -144,6 +146,10 @@ _SIGNATURES = {
     "qclhip_residual_state": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),
                               ct.POINTER(_vp), _dp],
     "qclhip_residual_keldysh": [_vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), _dp],
+    "qclhip_residual_suite_prescreened": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp),
+                                          ct.POINTER(_vp), ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp), _vp, _dp],
+    "qclhip_residual_state_prescreened": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp),
+                                          ct.POINTER(_vp), ct.POINTER(_vp), _vp, _dp],
     "qclhip_batch_create": [_vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp)],
     "qclhip_grid_tuning": [_vp, _vp, _dp, ct.POINTER(ct.c_int)],
     "qclhip_grid_set_keldysh_split": [_vp, _vp, ct.c_int],

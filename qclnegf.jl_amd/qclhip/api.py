@@ -472,7 +472,8 @@ class GreenState:
 
 def current_green_production(grid: Grid, h, families, target: float, state: GreenState | None = None,
                              spectral: bool = True, return_scalars: bool = True, totals=None,
-                             occupation_normalization: str = "paired_convex", allocation_free: bool = False):
+                             occupation_normalization: str = "paired_convex", allocation_free: bool = False,
+                             raw_screen=None):
     """_current_green_production (production_state.jl:1-74).
 
     families: list of (SR, SL, SG) device triples, the embedding total first,
@@ -484,7 +485,10 @@ def current_green_production(grid: Grid, h, families, target: float, state: Gree
     the reference's legacy branch G< = G<raw/lambda, G> = G^R - G^A + G<
     (production_state.jl:54-61; the options of algorithm_options.jl:63).
     allocation_free: use qclhip_current_green_noalloc (a fixed device scratch,
-    no device allocation; NE*Nk <= 524,288, at most 8 families)."""
+    no device allocation; NE*Nk <= 524,288, at most 8 families).
+    raw_screen: optional uint8 device tensor of NE*Nk entries that receives
+    the raw-product screen words of qclhip_current_green_screened (FP64
+    vector engine), for residual_suite/residual_state(raw_screen=...)."""
     if occupation_normalization not in ("paired_convex", "scalar_lesser"):
         raise _lib.ArgumentError("occupation_normalization must be :paired_convex or :scalar_lesser")
     scalar = occupation_normalization == "scalar_lesser"
@@ -492,6 +496,10 @@ def current_green_production(grid: Grid, h, families, target: float, state: Gree
         raise _lib.ArgumentError("totals are written by the paired_convex path only")
     if allocation_free and (scalar or totals is not None):
         raise _lib.ArgumentError("the allocation-free call is the plain paired_convex current_green")
+    if raw_screen is not None:
+        if scalar or allocation_free:
+            raise _lib.ArgumentError("raw-product screens come with the paired_convex current_green")
+        _check_screen(grid, raw_screen)
     nf = len(families)
     for fam in families:
         for t in fam:
@@ -506,7 +514,13 @@ def current_green_production(grid: Grid, h, families, target: float, state: Gree
     common = (grid.handle, _ptr(h), nf, arr(0), arr(1), arr(2), ct.c_double(target), _ptr(state.GR),
               _ptr(state.GL), _ptr(state.GG), _ptr(state.A if spectral else None), _ptr(state.scale),
               out if return_scalars else None)
-    if allocation_free:
+    if raw_screen is not None:
+        if totals is not None:
+            for t in totals:
+                _check_field(grid, t, "total")
+        tp = [_ptr(t) for t in totals] if totals is not None else [None, None, None]
+        grid.ctx.call("qclhip_current_green_screened", *common, *tp, _ptr(raw_screen))
+    elif allocation_free:
         grid.ctx.call("qclhip_current_green_noalloc", *common)
     elif scalar:
         grid.ctx.call("qclhip_current_green_scalar_lesser", *common)
@@ -520,6 +534,17 @@ def current_green_production(grid: Grid, h, families, target: float, state: Gree
         return state, Deferred(slot0, 5)
     info = dict(lam=out[0], number=out[1], holes=out[2], a=out[3], c=out[4]) if return_scalars else None
     return state, info
+
+
+def _check_screen(grid: Grid, raw_screen):
+    if (not isinstance(raw_screen, torch.Tensor) or raw_screen.dtype != torch.uint8 or not raw_screen.is_cuda
+            or not raw_screen.is_contiguous() or raw_screen.numel() != grid.NE * grid.Nk):
+        raise _lib.DimensionMismatch("raw_screen must be a contiguous uint8 device tensor of NE*Nk entries")
+
+
+def screen_buffer(grid: Grid):
+    """Device storage of one raw-product screen word per (e,m) block."""
+    return torch.empty(grid.NE * grid.Nk, dtype=torch.uint8, device=grid.ctx.device)
 
 
 def number_functional(grid: Grid, G) -> float:
@@ -830,14 +855,19 @@ def antihermitian_part(grid: Grid, X):
     return X
 
 
-def residual_suite(grid: Grid, h, green, families, candidate_lesser):
+def residual_suite(grid: Grid, h, green, families, candidate_lesser, raw_screen=None):
     """production_residual_suite (residuals.jl:537-772) on the device.
 
     green: a GreenState (A may be None: formed from G^R in the kernel,
     bitwise the spectral function); families: (SR, SL, SG) triples of the
     total self-energy; candidate_lesser: the unmixed candidate S< as a list of
-    fields summed in order. Returns dict r_D, r_A, r_K, r_PSD, r_caus, or a
-    Deferred of 5 values in deferred-results mode."""
+    fields summed in order. raw_screen: the screen words of the
+    current_green_production(raw_screen=...) call that produced green from
+    these families' total (qclhip_residual_suite_prescreened: the same
+    results). Returns dict r_D, r_A, r_K, r_PSD, r_caus, or a Deferred of 5
+    values in deferred-results mode."""
+    if raw_screen is not None:
+        _check_screen(grid, raw_screen)
     for fam in families:
         for t in fam:
             _check_field(grid, t, "self-energy")
@@ -849,18 +879,26 @@ def residual_suite(grid: Grid, h, green, families, candidate_lesser):
     carr = (_vp * len(cands))(*[t.data_ptr() for t in cands])
     out = (ct.c_double * 5)()
     slot0 = grid.ctx.result_slot() if grid.ctx.deferring else 0
-    grid.ctx.call("qclhip_residual_suite", grid.handle, _ptr(h), _ptr(green.GR), _ptr(green.GL), _ptr(green.GG),
-                  _ptr(green.A), nf, arr(0), arr(1), arr(2), len(cands), carr, out)
+    if raw_screen is None:
+        grid.ctx.call("qclhip_residual_suite", grid.handle, _ptr(h), _ptr(green.GR), _ptr(green.GL),
+                      _ptr(green.GG), _ptr(green.A), nf, arr(0), arr(1), arr(2), len(cands), carr, out)
+    else:
+        grid.ctx.call("qclhip_residual_suite_prescreened", grid.handle, _ptr(h), _ptr(green.GR), _ptr(green.GL),
+                      _ptr(green.GG), _ptr(green.A), nf, arr(0), arr(1), arr(2), len(cands), carr,
+                      _ptr(raw_screen), out)
     if grid.ctx.deferring:
         return Deferred(slot0, 5)
     return dict(r_D=out[0], r_A=out[1], r_K=out[2], r_PSD=out[3], r_caus=out[4])
 
 
-def residual_state(grid: Grid, h, green, families):
+def residual_state(grid: Grid, h, green, families, raw_screen=None):
     """Every term of production_residual_suite (residuals.jl:537-772) but r_K:
     r_D, r_A, r_PSD, r_caus from the Green state (A may be None: formed from
-    G^R) and the total self-energy, before the candidates exist. Returns the
-    dict of the five keys with r_K = 0, or a Deferred of 5 values."""
+    G^R) and the total self-energy, before the candidates exist; raw_screen
+    as for residual_suite. Returns the dict of the five keys with r_K = 0, or
+    a Deferred of 5 values."""
+    if raw_screen is not None:
+        _check_screen(grid, raw_screen)
     for fam in families:
         for t in fam:
             _check_field(grid, t, "self-energy")
@@ -868,8 +906,12 @@ def residual_state(grid: Grid, h, green, families):
     arr = lambda k: (_vp * nf)(*[f[k].data_ptr() for f in families])
     out = (ct.c_double * 5)()
     slot0 = grid.ctx.result_slot() if grid.ctx.deferring else 0
-    grid.ctx.call("qclhip_residual_state", grid.handle, _ptr(h), _ptr(green.GR), _ptr(green.GL), _ptr(green.GG),
-                  _ptr(green.A), nf, arr(0), arr(1), arr(2), out)
+    if raw_screen is None:
+        grid.ctx.call("qclhip_residual_state", grid.handle, _ptr(h), _ptr(green.GR), _ptr(green.GL),
+                      _ptr(green.GG), _ptr(green.A), nf, arr(0), arr(1), arr(2), out)
+    else:
+        grid.ctx.call("qclhip_residual_state_prescreened", grid.handle, _ptr(h), _ptr(green.GR), _ptr(green.GL),
+                      _ptr(green.GG), _ptr(green.A), nf, arr(0), arr(1), arr(2), _ptr(raw_screen), out)
     if grid.ctx.deferring:
         return Deferred(slot0, 5)
     return dict(r_D=out[0], r_A=out[1], r_K=out[2], r_PSD=out[3], r_caus=out[4])

@@ -143,6 +143,8 @@ class ScbaWorkspace:
         # state's families between iterations clears the flag.
         self.totals = new_family(g) if p.measure else None
         self.totals_valid = False
+        # raw-product screen words of the GF solve, read by the residual suite
+        self.raw_screen = api.screen_buffer(g) if p.measure else None
         self.anderson = None
         if p.mixing == "anderson":
             from .anderson import AndersonMixer
@@ -152,6 +154,15 @@ class ScbaWorkspace:
 
 
 MEASURE_KEYS = ("r_D", "r_A", "r_K", "r_PSD", "r_caus")
+
+
+def _screen_buffer(g, ws):
+    """The screen words the GF solve writes for the residual suite, where its
+    Keldysh products run in the FP64 vector kernels (else None: the suite
+    forms the raw products itself)."""
+    if ws.raw_screen is None or g.engine != "vector" or g.precision == "fp32_products" or g.Nb > 32:
+        return None
+    return ws.raw_screen
 
 
 def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dict | None = None):
@@ -174,13 +185,15 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     # else in the iteration reads it (api.spectral_function gives it on demand).
     if s.green is not None:
         s.green.A = None
+    rs = _screen_buffer(g, ws) if p.measure else None
     if ws.totals is not None and ws.totals_valid:
         green, info = api.current_green_production(g, p.h, [ws.totals.parts()], p.target, state=s.green,
-                                                   spectral=False)
+                                                   spectral=False, raw_screen=rs)
     else:
         fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
         green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=False,
-                                                   totals=ws.totals.parts() if ws.totals is not None else None)
+                                                   totals=ws.totals.parts() if ws.totals is not None else None,
+                                                   raw_screen=rs)
         ws.totals_valid = ws.totals is not None
     s.green = green
     t1 = stamp()
@@ -193,7 +206,8 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
         # the unmixed candidate total S<, summed by the suite as it loads the
         # candidates, in the reference's order ((plus + minus) + mechanism 1 + ...)
         suite = api.residual_suite(g, p.h, green, [ws.totals.parts()],
-                                   [ws.plus.L, ws.minus.L] + [ws.scat[m.name].L for m in p.mechanisms])
+                                   [ws.plus.L, ws.minus.L] + [ws.scat[m.name].L for m in p.mechanisms],
+                                   raw_screen=rs)
     t4 = stamp()
     Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
     pops = api.state_populations(g, green.GL, p.target)
@@ -261,9 +275,10 @@ class LeanWorkspace:
         self.csum = api.empty_field(g)
         self.totals = new_family(g)
         self.totals_valid = False
+        self.raw_screen = api.screen_buffer(g)
 
 
-def _lean_tail(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, green, info, stamp=None):
+def _lean_tail(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, green, info, stamp=None, raw_screen=None):
     """The lean iteration after its GF solve: the state residuals and
     observables, then each family's candidate built, measured and mixed at
     once (embedding plus/minus one component at a time, then every
@@ -271,7 +286,7 @@ def _lean_tail(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, green, info, sta
     deferred-results mode."""
     g = p.grid
     stamp = stamp or (lambda: 0.0)
-    suite = api.residual_state(g, p.h, green, [ws.totals.parts()])
+    suite = api.residual_state(g, p.h, green, [ws.totals.parts()], raw_screen=raw_screen)
     Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
     pops = api.state_populations(g, green.GL, p.target)
     t2 = stamp()
@@ -360,16 +375,17 @@ def scba_iteration_lean(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, timings
     t0 = stamp()
     if s.green is not None:
         s.green.A = None  # not kept
+    rs = _screen_buffer(g, ws)
     if ws.totals_valid:
         green, info = api.current_green_production(g, p.h, [ws.totals.parts()], p.target, state=s.green,
-                                                   spectral=False)
+                                                   spectral=False, raw_screen=rs)
     else:
         fams = [s.plus.parts(), s.minus.parts()] + [s.scattering[m.name].parts() for m in p.mechanisms]
         green, info = api.current_green_production(g, p.h, fams, p.target, state=s.green, spectral=False,
-                                                   totals=ws.totals.parts())
+                                                   totals=ws.totals.parts(), raw_screen=rs)
     s.green = green
     t1 = stamp()
-    res = _lean_tail(p, s, ws, green, info, stamp)
+    res = _lean_tail(p, s, ws, green, info, stamp, raw_screen=rs)
     if g.ctx.deferring:
         return res
     g.ctx.synchronize()

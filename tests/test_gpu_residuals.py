@@ -327,3 +327,95 @@ def test_current_green_totals(hip, engine):
         ref = torch.empty_like(tot[k])
         hip.sum_fields([f[k] for f in fams], ref)
         assert torch.equal(tot[k], ref), k
+
+
+def _screened_state(hip, N, NE, Nk, indefinite):
+    """A GF solve on two diagonal physical families (an injecting one and the
+    workload's), with indefinite: a random Hermitian part added to i S< of
+    the scattering family (so the raw products -i G<raw fail their screens),
+    through current_green_production(raw_screen=...)."""
+    import torch
+    syn = fx.syn
+    w = syn.gf_workload(N=N, NE=NE, Nk=Nk)
+    g = hip.Grid(w["eps"], w["wE"], w["wk"], N)
+    SR, SL, SG = syn.diag_fields_host(w)
+    if indefinite:
+        rng = np.random.default_rng(N)
+        M = rng.normal(size=SL.shape) + 1j * rng.normal(size=SL.shape)
+        M = 0.5 * (M + np.conj(np.swapaxes(M, 2, 3)))  # Hermitian blocks
+        SL = np.asfortranarray(SL - 1j * 2e-4 * M)  # S< stays anti-Hermitian
+    fams = [tuple(hip.to_device(x) for x in syn.diag_fields_host(syn.injection_family(w))),
+            (hip.to_device(SR), hip.to_device(SL), hip.to_device(SG))]
+    tot = tuple(torch.empty_like(fams[0][0]) for _ in range(3))
+    scr = hip.screen_buffer(g)
+    st, info = hip.current_green_production(g, hip.h_to_device(w["h"]), fams, 1.0, spectral=False, totals=tot,
+                                            raw_screen=scr)
+    target = 0.9 * info["number"]
+    st, info = hip.current_green_production(g, hip.h_to_device(w["h"]), fams, target, state=st, spectral=False,
+                                            totals=tot, raw_screen=scr)
+    cands = [0.3 * fams[1][1], fams[0][1]]
+    return g, hip.h_to_device(w["h"]), st, tot, cands, scr
+
+
+@pytest.mark.parametrize("N,indefinite", [(5, False), (24, False), (24, True), (32, True)])
+def test_prescreened_suite_equals_suite(hip, N, indefinite):
+    """The suite and its state pass with the raw-product screens taken from
+    the GF solve's Keldysh kernel equal the plain entries bit for bit, on
+    states whose raw screens settle and on ones whose raw screens fail (those
+    blocks get their eigenvalues from products formed again)."""
+    g, h, st, tot, cands, scr = _screened_state(hip, N, NE=37, Nk=3, indefinite=indefinite)
+    flags = scr.cpu().numpy()
+    if indefinite:
+        assert np.count_nonzero(flags & 0x20) > 0  # unsettled -i G<raw screens
+    else:
+        assert not np.any(flags)
+    plain = hip.residual_suite(g, h, st, [tot], cands)
+    pre = hip.residual_suite(g, h, st, [tot], cands, raw_screen=scr)
+    assert plain == pre, (plain, pre)
+    if indefinite:
+        assert plain["r_PSD"] > 0.0
+    plain = hip.residual_state(g, h, st, [tot])
+    pre = hip.residual_state(g, h, st, [tot], raw_screen=scr)
+    assert plain == pre, (plain, pre)
+
+
+def test_screened_scba_iteration_equals_plain(hip):
+    """SCBA iterations whose GF solve screens the raw products for the suite
+    (the default on the FP64 vector engine) equal iterations without, bit for
+    bit, in every residual and field."""
+    import torch
+    from qclhip import scba
+    from test_gpu_adjoint import _device_problem
+    prob = fx.make_problem(N=10, NE=48, Nk=3)
+    full, state, _ = fx.oracle_state(prob)
+    rows, fields = [], []
+    for use in (True, False):
+        g, p, s = _device_problem(hip, prob, full, state)
+        ws = scba.ScbaWorkspace(p)
+        if not use:
+            ws.raw_screen = None
+        r = [scba.scba_iteration(p, s, ws) for _ in range(3)]
+        rows.append(r)
+        fields.append([t.clone() for t in (s.green.GR, s.green.GL, s.green.GG)])
+    for a, b in zip(rows[0], rows[1]):
+        for k in ("r_D", "r_A", "r_K", "r_PSD", "r_caus", "r_Sigma", "lam", "Jbar"):
+            assert a[k] == b[k], k
+    for x, y in zip(*fields):
+        assert torch.equal(x, y)
+
+
+def test_screened_gf_solve_limits(hip):
+    """Raw-product screens come from the FP64 vector Keldysh kernels up to
+    Nb = 32 only; elsewhere the screened GF solve is refused (and the SCBA
+    iteration leaves the raw products to the residual suite)."""
+    from qclhip import _lib, scba
+    with pytest.raises(_lib.UnsupportedError):
+        _screened_state(hip, 40, NE=37, Nk=3, indefinite=False)
+    w = fx.syn.gf_workload(N=6, NE=16, Nk=2)
+    g = hip.Grid(w["eps"], w["wE"], w["wk"], 6)
+    fams = [fx.syn.diag_fields_device(w)]
+    for engine in ("matrix", "fused"):
+        g.set_engine(engine)
+        with pytest.raises(_lib.UnsupportedError):
+            hip.current_green_production(g, hip.h_to_device(w["h"]), fams, 1.0, raw_screen=hip.screen_buffer(g))
+        assert scba._screen_buffer(g, type("W", (), {"raw_screen": hip.screen_buffer(g)})()) is None
