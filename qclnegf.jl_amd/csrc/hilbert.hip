@@ -182,11 +182,42 @@ __device__ inline void stockham_pass(double* re, double* im, int t, const c64* _
 // the first pass off a single bank.
 __device__ constexpr int lds_slot(int x) { return x + (x >> 5); }
 
-// Redistribute through LDS (one real component at a time) from the outputs
-// of a pass (radix R, span NS) to the inputs of the next pass (radix RN).
+// At L = 8192 the registers of a 1024-thread workgroup (88-90 VGPRs) allow
+// one workgroup per CU whatever its LDS, so the exchange moves whole complex
+// values (16-byte slots, 128 KiB): one barrier pair per pass instead of two
+// and half the LDS instructions (5% of the transform; same values). The
+// slot of element x has its low three bits XORed with bits 3-5, so that the
+// stride-8 writes of the first pass and the unit-stride reads of 8
+// consecutive lanes fall on different banks.
+constexpr bool complex_exchange(int L) { return L == 8192; }
+__device__ constexpr int cslot(int x) { return x ^ ((x >> 3) & 7); }
+
+// Redistribute through LDS from the outputs of a pass (radix R, span NS) to
+// the inputs of the next pass (radix RN): one real component at a time, or
+// complex values where complex_exchange(L).
 template <int L, int R, int NS, int RN>
 __device__ inline void stockham_exchange(double* re, double* im, double* lds, int t) {
     constexpr int V = vals_per_thread(L), T = threads_for(L);
+    if constexpr (complex_exchange(L)) {
+        c64* cl = reinterpret_cast<c64*>(lds);
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < V; ++f) {
+            const int i = f / R, r = f % R;
+            const int j = t + i * T;
+            cl[cslot((j / NS) * NS * R + (j & (NS - 1)) + r * NS)] = cmk(re[f], im[f]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < V; ++f) {
+            const int i = f / RN, r = f % RN;
+            const int j = t + i * T;
+            const c64 v = cl[cslot(j + r * (L / RN))];
+            re[f] = v.x;
+            im[f] = v.y;
+        }
+        return;
+    }
 #pragma unroll
     for (int comp = 0; comp < 2; ++comp) {
         double* v = comp ? im : re;
@@ -245,6 +276,7 @@ struct HilbertArgs {
 // rounded up to a 16-byte multiple for the Gamma stash behind it.
 template <int L>
 constexpr int fft_lds_len() {
+    if (complex_exchange(L)) return 2 * L;
     return (((L + L / 32) > 2 * threads_for(L) ? (L + L / 32) : 2 * threads_for(L)) + 1) / 2 * 2;
 }
 
