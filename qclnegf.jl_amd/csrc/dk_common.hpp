@@ -217,6 +217,33 @@ __device__ inline void argmax8(double& mag, int& idx) {
     dpp_argmax_step<0x141>(mag, idx);  // row_half_mirror
 }
 
+// One DPP lane permute of a double (every lane reads a lane of its own row).
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// The pivot of the column group whose lanes are 8g .. 8g+7 (g wave-uniform):
+// each lane offers its first-largest candidate (mag, idx); the group's
+// largest magnitude comes from a DPP max (exact: the max is one of the
+// values), and the lowest lane that holds it (a ballot) gives the row: the
+// first largest |Re|+|Im| in row order, as argmax8 selects it. Returns the
+// magnitude; piv is wave-uniform. Magnitudes are never NaN (-1 and -inf
+// mark a lane without a candidate).
+__device__ inline double pivot8(double mag, int idx, int g, int& piv) {
+    double mx = fmax(mag, dpp_d<0xB1>(mag));  // quad_perm [1,0,3,2]
+    mx = fmax(mx, dpp_d<0x4E>(mx));           // quad_perm [2,3,0,1]
+    mx = fmax(mx, dpp_d<0x141>(mx));          // row_half_mirror
+    const double pmag = readlane_d(mx, 8 * g);
+    const unsigned long long hit = __ballot(mag == pmag) & (0xffull << (8 * g));
+    const int src = hit ? __ffsll((long long)hit) - 1 : 8 * g;
+    piv = __builtin_amdgcn_readlane(idx, src);
+    return pmag;
+}
+
 // Converting variants of the cooperative tile transfer: the LDS tile holds
 // elements of type V (float2 or double2); HBM fields stay complex double.
 template <typename V>

@@ -151,9 +151,8 @@ __device__ inline void dyson_tile(const DKArgs& p, const TileCoord& tc, c64* in_
                     mag = gt ? mg : mag;
                     idx = gt ? lr * B + r : idx;
                 }
-                argmax8(mag, idx);
-                const int piv = __builtin_amdgcn_readlane(idx, 8 * ck);
-                const double pmag = readlane_d(mag, 8 * ck);
+                int piv;
+                const double pmag = pivot8(mag, idx, ck, piv);
                 if (!(pmag > 0.0) || piv >= N) {
                     ok = false;
                     break;

@@ -49,14 +49,6 @@ __device__ inline void rstore(c64* T, int N, int lane, const c64 (&v)[B][B]) {
         }
 }
 
-// One DPP lane permute of a double (every lane reads a lane of its own row).
-template <int CTRL>
-__device__ inline double dpp_d(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
 // Wave reductions: DPP within each row of 16 lanes (xor 1, xor 2, then the
 // half-row and row mirrors, after which every lane of a row holds the row's
 // value), then the four rows read back in a fixed order. Every lane gets
