@@ -401,4 +401,34 @@ __device__ inline TileCoord tile_coord(long long d, long long ntiles, int NE) {
     return c;
 }
 
+// Rows [s 8 B/SPLIT, (s+1) 8 B/SPLIT) of the lane-grid product L R (ADJ:
+// L R^dagger) of LDS tiles, the share of wave s of SPLIT waves working on one
+// block: each lane a (B/SPLIT) x B register block, with block_product's
+// per-element arithmetic (FMAs in q order), so the entries are bitwise those
+// of the one-wave product.
+template <int B, int SPLIT, bool ADJ>
+__device__ inline void block_product_rows(const c64* __restrict__ L, const c64* __restrict__ R, int N, int s, int lr,
+                                          int lc, c64 (&out)[B / SPLIT][B]) {
+    constexpr int LD = DKShape<B>::LD;
+    constexpr int BR = B / SPLIT;
+    const int row0 = s * 8 * BR + lr * BR;
+#pragma unroll
+    for (int r = 0; r < BR; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) out[r][c] = cmk(0.0, 0.0);
+#pragma unroll 2
+    for (int q = 0; q < N; ++q) {
+        c64 lv[BR], rv[B];
+#pragma unroll
+        for (int r = 0; r < BR; ++r) lv[r] = L[(row0 + r) * LD + q];
+#pragma unroll
+        for (int c = 0; c < B; ++c) rv[c] = ADJ ? R[(lc * B + c) * LD + q] : R[q * LD + lc * B + c];
+#pragma unroll
+        for (int r = 0; r < BR; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c)
+                out[r][c] = ADJ ? cfmac(out[r][c], lv[r], rv[c]) : cfma(out[r][c], lv[r], rv[c]);
+    }
+}
+
 }  // namespace qclhip

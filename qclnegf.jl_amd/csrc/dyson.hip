@@ -336,31 +336,6 @@ __device__ inline void keldysh_products(c64* Gs, c64* Xs, int N, int lane, c64* 
 // grid (group g sits at lane 9g, every other leaf is +0), so the traces are
 // bitwise equal too. Every thread of the workgroup calls it (workgroup
 // barriers); active marks the waves with a valid block.
-template <int B, int SPLIT, bool ADJ>
-__device__ inline void block_product_rows(const c64* __restrict__ L, const c64* __restrict__ R, int N, int s, int lr,
-                                          int lc, c64 (&out)[B / SPLIT][B]) {
-    constexpr int LD = DKShape<B>::LD;
-    constexpr int BR = B / SPLIT;
-    const int row0 = s * 8 * BR + lr * BR;
-#pragma unroll
-    for (int r = 0; r < BR; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) out[r][c] = cmk(0.0, 0.0);
-#pragma unroll 2
-    for (int q = 0; q < N; ++q) {
-        c64 lv[BR], rv[B];
-#pragma unroll
-        for (int r = 0; r < BR; ++r) lv[r] = L[(row0 + r) * LD + q];
-#pragma unroll
-        for (int c = 0; c < B; ++c) rv[c] = ADJ ? R[(lc * B + c) * LD + q] : R[q * LD + lc * B + c];
-#pragma unroll
-        for (int r = 0; r < BR; ++r)
-#pragma unroll
-            for (int c = 0; c < B; ++c)
-                out[r][c] = ADJ ? cfmac(out[r][c], lv[r], rv[c]) : cfma(out[r][c], lv[r], rv[c]);
-    }
-}
-
 template <int B, int SPLIT>
 __device__ inline void keldysh_products_split(c64* Gs, c64* Xs, c64* diag, int N, int lane, int s, bool active,
                                               c64* trace) {

@@ -55,7 +55,7 @@ __device__ inline void tile_spectral(const c64* G, c64* X, int N, int lane) {
 
 // Gamma = i (S> - S<) of the summed families, cooperatively into the W
 // per-wave tiles (the thread mapping of tile_load).
-template <int W, int LD>
+template <int W, int LD, int TH = 64 * W>
 __device__ inline void tile_load_gamma(c64* lds, int wstride, const c64* const* famG, const c64* const* famL,
                                        int nfam, long long blk0, int nvalid, int N, size_t S) {
     const int total = N * N * W;
@@ -63,13 +63,13 @@ __device__ inline void tile_load_gamma(c64* lds, int wstride, const c64* const* 
     const int w = f % W;
     int ij = f / W;
     int i = ij % N, j = ij / N;
-    for (; f < total; f += 64 * W) {
+    for (; f < total; f += TH) {
         if (w < nvalid) {
             const size_t idx = blk0 + w + S * (i + (size_t)N * j);
             const c64 g = fam_load(famG, nfam, idx), l = fam_load(famL, nfam, idx);
             lds[w * wstride + i * LD + j] = cmk(-(g.y - l.y), g.x - l.x);
         }
-        i += 64;
+        i += TH / W;
         while (i >= N) {
             i -= N;
             ++j;
@@ -321,6 +321,210 @@ __global__ __launch_bounds__(64 * W) void k_residual_suite(ResArgs p) {
         if (unsettled) {
             const unsigned slot = atomicAdd(p.nflagged, 1u);
             p.flagged[slot] = ((unsigned long long)blk << 8) | unsettled;
+        }
+    }
+}
+
+// The suite in prescreened mode with A formed from G^R (the SCBA loop's
+// call) at N = 25..32 (B = 4), B waves per block: wave s of a block computes
+// row 8 s + lr of every block product (block_product_rows, so the entries are
+// bitwise those of the one-wave products) and forms D on those rows; the
+// block's first wave runs the screens and the norms, reading the stored
+// products from LDS in the one-wave lane order, so that every per-block value
+// is bitwise that of k_residual_suite. Sharing the two tiles of a block over
+// B waves raises the waves a CU holds from 4 (B = 4: two 17 KB tiles per
+// wave) to 16. The sandwich products are stored into X; G< goes into the G
+// tile once G^R is no longer read.
+template <int B, int W>
+__global__ __launch_bounds__(64 * B * W) void k_residual_suite_split(ResArgs p) {
+    constexpr int SPLIT = B;
+    constexpr int LD = 8 * B + 1;
+    constexpr int TH = 64 * SPLIT * W;
+    const int wstride = TileStride<B>::value;
+    extern __shared__ __align__(16) unsigned char smem[];
+    c64* Gall = reinterpret_cast<c64*>(smem);
+    c64* Xall = Gall + W * wstride;
+    EigWork<B>* ewall = reinterpret_cast<EigWork<B>*>(Xall + W * wstride);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = wave / SPLIT, s = wave % SPLIT;
+    const int lr = lane & 7, lc = lane >> 3;
+    const long long nblk = (long long)p.NE * p.Nk;
+    const long long blk0 = (long long)W * xcd_tile(blockIdx.x, (nblk + W - 1) / W);
+    const int nvalid = (int)min((long long)W, nblk - blk0);
+    const bool valid = b < nvalid;
+    const bool lead = valid && s == 0;
+    const long long blk = blk0 + (valid ? b : 0);  // e + NE m
+    const int e = (int)(blk % p.NE), m = (int)(blk / p.NE);
+    const int N = p.N;
+    const size_t S = (size_t)nblk;
+    c64* G = Gall + b * wstride;
+    c64* X = Xall + b * wstride;
+    EigWork<B>* ew = ewall + b;
+    const int i0 = 8 * s + lr;  // this lane's row of the block products
+    c64 T[1][B];
+    // the rows of T into X
+    auto store_rows = [&]() {
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int j = lc * B + c;
+            if (i0 < N && j < N) X[i0 * LD + j] = T[0][c];
+        }
+    };
+    // T = G X G^dagger on this wave's rows, stored into X (X holds G X in
+    // between); ends with every row stored
+    auto sandwich = [&]() {
+        if (valid) block_product_rows<B, SPLIT, false>(G, X, N, s, lr, lc, T);
+        __syncthreads();  // every wave of the block has read X
+        if (valid) store_rows();
+        wave_sync();  // T G^dagger reads only this wave's rows of G X
+        if (valid) block_product_rows<B, SPLIT, true>(X, G, N, s, lr, lc, T);
+        wave_sync();
+        if (valid) store_rows();
+        __syncthreads();
+    };
+    const c64* one[1] = {p.GR};
+    tile_load<W, LD, TH>(Gall, wstride, one, 1, blk0, nvalid, N, S);
+    tile_load<W, LD, TH>(Xall, wstride, p.famR, p.nfam, blk0, nvalid, N, S);
+    __syncthreads();
+    unsigned unsettled = 0;
+    double caus = 0.0, rD = 0.0, psd = 0.0, specN = 0.0, specD = 0.0, kelN = 0.0, kelD = 0.0;
+    if (lead) {
+        bool u = false;
+        caus = causality_screen_tile<B, LD>(X, N, lane, ew, u);
+        if (u) unsettled |= kScrCaus;
+    }
+    __syncthreads();  // S^R read before D replaces it
+    // r_D: D = eps - h - S^R on this wave's rows, ||D G - I|| / (||D|| ||G|| + sqrt(N))
+    if (valid) {
+        const double ee = p.eps[e];
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = i0, j = lc * B + c;
+            if (i < N && j < N) {
+                const c64 hv = p.h[(size_t)m + (size_t)p.Nk * (i + (size_t)N * j)];
+                const c64 sv = X[i * LD + j];
+                X[i * LD + j] = cmk((i == j ? ee : 0.0) - hv.x - sv.x, -hv.y - sv.y);
+            }
+        }
+    }
+    __syncthreads();
+    double fD = 0.0, fG = 0.0;
+    if (lead) {
+        fD = tile_frob2<LD>(X, N, lane, 0.0);
+        fG = tile_frob2<LD>(G, N, lane, 0.0);
+    }
+    if (valid) block_product_rows<B, SPLIT, false>(X, G, N, s, lr, lc, T);
+    __syncthreads();  // D read by every wave before D G replaces it
+    if (valid) store_rows();
+    __syncthreads();
+    if (lead) {
+        double dn = 0.0;
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int i = lr * B + r, j = lc * B + c;
+                if (i < N && j < N) {
+                    const c64 t = X[i * LD + j];
+                    const c64 v = cmk(t.x - (i == j ? 1.0 : 0.0), t.y);
+                    dn += v.x * v.x + v.y * v.y;
+                }
+            }
+        rD = sqrt(wave_sum(dn)) / (sqrt(fD) * sqrt(fG) + sqrt((double)N));
+    }
+    __syncthreads();  // before Gamma replaces X
+    // Gamma = i (S> - S<): its screen, then G Gamma G^dagger (r_A)
+    tile_load_gamma<W, LD, TH>(Xall, wstride, p.famG, p.famL, p.nfam, blk0, nvalid, N, S);
+    __syncthreads();
+    if (lead) {
+        bool u = false;
+        psd = fmax(psd, psd_screen_tile<B, LD>(X, N, lane, ew, 1.0, 0.0, u));
+        if (u) unsettled |= kScrGamma;
+        wave_sync();
+    }
+    sandwich();
+    // A = i (G - G^dagger) from the G^R tile: the r_A terms and its screen
+    if (lead) {
+        double sn = 0.0, sd = 0.0;
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int i = lr * B + r, j = lc * B + c;
+                if (i < N && j < N) {
+                    const c64 av = spectral_at<LD>(G, i, j);
+                    const c64 d = csub(av, X[i * LD + j]);
+                    sn += d.x * d.x + d.y * d.y;
+                    c64 v = av;
+                    if (i == j) v.x -= 0.0;
+                    sd += v.x * v.x + v.y * v.y;
+                }
+            }
+        specN = wave_sum(sn);
+        specD = wave_sum(sd);
+        bool u = false;
+        psd = fmax(psd, psd_screen_at<B>([&](int i, int j) { return spectral_at<LD>(G, i, j); }, N, lane, ew, 1.0,
+                                         0.0, u));
+        if (u) unsettled |= kScrA;
+        wave_sync();
+    }
+    const bool cand = p.mode != 1;  // the state pass has no candidate
+    if (cand) {
+        // the candidate S<: G S<_cand G^dagger (r_K)
+        __syncthreads();  // X read by the first wave before the candidate replaces it
+        tile_load<W, LD, TH>(Xall, wstride, p.cand, p.ncand, blk0, nvalid, N, S);
+        __syncthreads();
+        sandwich();
+    }
+    // G< into the G tile (G^R is no longer read): the r_K terms and its screen
+    __syncthreads();
+    const c64* fL[1] = {p.GL};
+    tile_load<W, LD, TH>(Gall, wstride, fL, 1, blk0, nvalid, N, S);
+    __syncthreads();
+    if (lead) {
+        if (cand) {
+            double kn = 0.0;
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int c = 0; c < B; ++c) {
+                    const int i = lr * B + r, j = lc * B + c;
+                    if (i < N && j < N) {
+                        const c64 d = csub(G[i * LD + j], X[i * LD + j]);
+                        kn += d.x * d.x + d.y * d.y;
+                    }
+                }
+            kelN = wave_sum(kn);
+            kelD = tile_frob2<LD>(G, N, lane, 0.0);
+        }
+        bool u = false;
+        psd = fmax(psd, psd_screen_tile<B, LD>(G, N, lane, ew, 0.0, -1.0, u));
+        if (u) unsettled |= kScrLess;
+        wave_sync();
+    }
+    __syncthreads();  // X read by the first wave before G> replaces it
+    const c64* fGg[1] = {p.GG};
+    tile_load<W, LD, TH>(Xall, wstride, fGg, 1, blk0, nvalid, N, S);
+    __syncthreads();
+    if (lead) {
+        bool u = false;
+        psd = fmax(psd, psd_screen_tile<B, LD>(X, N, lane, ew, 0.0, 1.0, u));
+        if (u) unsettled |= kScrGrt;
+        const unsigned rb = p.raw_screen[blk];
+        if (rb & kScrNonFinite) psd = INFINITY;
+        unsettled |= rb & (kScrRawL | kScrRawG);
+        if (lane == 0) {
+            p.out[0 * nblk + blk] = rD;
+            p.out[1 * nblk + blk] = specN;
+            p.out[2 * nblk + blk] = specD;
+            p.out[3 * nblk + blk] = kelN;
+            p.out[4 * nblk + blk] = kelD;
+            p.out[5 * nblk + blk] = psd;
+            p.out[6 * nblk + blk] = caus;
+            if (unsettled) {
+                const unsigned slot = atomicAdd(p.nflagged, 1u);
+                p.flagged[slot] = ((unsigned long long)blk << 8) | unsettled;
+            }
         }
     }
 }
@@ -748,10 +952,39 @@ __global__ void k_zero_counter(unsigned* c) {
     if (threadIdx.x == 0) *c = 0u;
 }
 
+template <int B, int W>
+int launch_residual_split(qclhip_ctx* ctx, const ResArgs& a) {
+    const size_t lds = (size_t)W * (2 * (size_t)TileStride<B>::value * sizeof(c64) + sizeof(EigWork<B>));
+    const size_t lds_fb = res_wave_bytes<B>();
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_residual_suite_split<B, W>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (lds_fb > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_residual_fallback<B>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fb));
+    const long long nblk = (long long)a.NE * a.Nk;
+    const long long ntile = (nblk + W - 1) / W;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, ctx->stream, a.nflagged);
+    hipLaunchKernelGGL((k_residual_suite_split<B, W>), dim3((unsigned)ntile), dim3(64 * B * W), lds, ctx->stream, a);
+    const unsigned fb_grid = (unsigned)std::min<long long>(nblk, 1024);
+    hipLaunchKernelGGL((k_residual_fallback<B>), dim3(fb_grid), dim3(64), lds_fb, ctx->stream, a);
+    return launch_check(ctx, "k_residual_suite_split");
+}
+
 // The screening pass over every block, then the eigenvalue pass over the
 // blocks it listed (a fixed grid that reads the list length on the device).
 template <int B>
 int launch_residual(qclhip_ctx* ctx, const ResArgs& a) {
+    // N in 25..32, prescreened, A formed from G^R: four waves per block, two
+    // blocks per workgroup (measured at N = 32, NE = 4096, Nk = 128: 74.8 ms
+    // for the state pass with the observables against 82.7 ms; four blocks
+    // per workgroup 94.9 ms; at N <= 24 the split ran slower than one wave
+    // per block: 11.4 against 8.3 ms for the full suite at N = 24)
+    if constexpr (B == 4) {
+        if (a.raw_screen && !a.A) return launch_residual_split<B, 2>(ctx, a);
+    }
     constexpr int W = res_waves<B>();
     const size_t lds = W * res_wave_bytes<B>();
     const size_t lds_fb = res_wave_bytes<B>();

@@ -439,8 +439,10 @@ __device__ inline double psd_from_tile(const c64* X, int N, int lane, EigWork<B>
 // only the (B - kc)^2 trailing registers: a third of the blocked screen's
 // FMAs for B = 3. Each entry sees the same operations in the same order as
 // in chol_screen (skipped updates there subtract exact zeros).
-template <int B, int LD>
-__device__ inline void load_herm_cyc(const c64* X, int N, int lane, double fr, double fi, c64 (&a)[B][B]) {
+// (The entries of the matrix come through at(i, j): a tile, or A formed on
+// the fly from a G^R tile.)
+template <int B, typename At>
+__device__ inline void load_herm_cyc_at(At at, int N, int lane, double fr, double fi, c64 (&a)[B][B]) {
     const int lr = lane & 7, lc = lane >> 3;
 #pragma unroll
     for (int r = 0; r < B; ++r)
@@ -448,7 +450,7 @@ __device__ inline void load_herm_cyc(const c64* X, int N, int lane, double fr, d
         for (int c = 0; c < B; ++c) {
             const int i = lr + 8 * r, j = lc + 8 * c;
             if (i < N && j < N) {
-                const c64 x = X[i * LD + j], y = X[j * LD + i];
+                const c64 x = at(i, j), y = at(j, i);
                 const double ax = fr * x.x - fi * x.y, ay = fr * x.y + fi * x.x;
                 const double bx = fr * y.x - fi * y.y, by = fr * y.y + fi * y.x;
                 a[r][c] = cmk(0.5 * (ax + bx), 0.5 * (ay - by));
@@ -456,6 +458,20 @@ __device__ inline void load_herm_cyc(const c64* X, int N, int lane, double fr, d
                 a[r][c] = cmk(0.0, 0.0);
             }
         }
+}
+
+template <int B, int LD>
+__device__ inline void load_herm_cyc(const c64* X, int N, int lane, double fr, double fi, c64 (&a)[B][B]) {
+    load_herm_cyc_at<B>([&](int i, int j) { return X[i * LD + j]; }, N, lane, fr, fi, a);
+}
+
+// A = i (G - G^dagger) at (i, j) from a G^R tile (the arithmetic of the
+// spectral kernel, so the values are bitwise those of a stored A).
+template <int LD>
+__device__ inline c64 spectral_at(const c64* G, int i, int j) {
+    const c64 g = G[i * LD + j], gt = G[j * LD + i];
+    const double dx = g.x - gt.x, dy = g.y + gt.y;
+    return cmk(-dy, dx);
 }
 
 template <int B>
@@ -546,12 +562,11 @@ enum ResScreen : unsigned {
 // screen on the blocked grid). A screen that passes proves the ratio 0 in
 // either layout, so the outcome does not depend on which grid screened.
 // Without the eigenvalue code the pass needs far fewer registers.
-template <int B, int LD>
-__device__ inline double psd_screen_tile(const c64* X, int N, int lane, EigWork<B>* w, double fr, double fi,
-                                         bool& unsettled) {
+template <int B, typename At>
+__device__ inline double psd_screen_at(At at, int N, int lane, EigWork<B>* w, double fr, double fi, bool& unsettled) {
     constexpr double kEps = 2.220446049250313e-16;
     c64 a[B][B];
-    load_herm_cyc<B, LD>(X, N, lane, fr, fi, a);
+    load_herm_cyc_at<B>(at, N, lane, fr, fi, a);
     const BlockScale s = block_scale_cyc<B>(a, N, lane);
     if (s.bad) return INFINITY;
     if (s.amax == 0.0) return 0.0;
@@ -562,6 +577,12 @@ __device__ inline double psd_screen_tile(const c64* X, int N, int lane, EigWork<
     const double delta = 32.0 * N * kEps * fmax(sqrt(s.fro2 / N), 1.0);
     if (!chol_screen_cyc<B>(a, N, lane, w, delta * sc)) unsettled = true;
     return 0.0;
+}
+
+template <int B, int LD>
+__device__ inline double psd_screen_tile(const c64* X, int N, int lane, EigWork<B>* w, double fr, double fi,
+                                         bool& unsettled) {
+    return psd_screen_at<B>([&](int i, int j) { return X[i * LD + j]; }, N, lane, w, fr, fi, unsettled);
 }
 
 template <int B, int LD>

@@ -357,12 +357,14 @@ def _screened_state(hip, N, NE, Nk, indefinite):
     return g, hip.h_to_device(w["h"]), st, tot, cands, scr
 
 
-@pytest.mark.parametrize("N,indefinite", [(5, False), (24, False), (24, True), (32, True)])
+@pytest.mark.parametrize("N,indefinite", [(5, False), (24, False), (24, True), (28, True), (32, False), (32, True)])
 def test_prescreened_suite_equals_suite(hip, N, indefinite):
     """The suite and its state pass with the raw-product screens taken from
     the GF solve's Keldysh kernel equal the plain entries bit for bit, on
     states whose raw screens settle and on ones whose raw screens fail (those
-    blocks get their eigenvalues from products formed again)."""
+    blocks get their eigenvalues from products formed again). At N = 28 and
+    32 the prescreened pass runs the four-waves-per-block kernel, whose every
+    per-block value must equal the one-wave kernel's."""
     g, h, st, tot, cands, scr = _screened_state(hip, N, NE=37, Nk=3, indefinite=indefinite)
     flags = scr.cpu().numpy()
     if indefinite:
