@@ -56,17 +56,25 @@ __global__ void k_mi_reduce(int NE, int Nk, int N2, const double* __restrict__ w
 }
 
 // S[e + NE (m + Nk q)] = f w_m Y[e + NE q] + beta S; thread = energy, block
-// rows stride over the columns (m, q).
+// row y takes a contiguous range of the columns (m, q), so that Y is read
+// once per q (Nk consecutive columns) instead of once per column.
 __global__ void k_mi_broadcast(int NE, int Nk, int N2, const c64* __restrict__ Y, double f, double beta,
                                const double* __restrict__ wout, c64* __restrict__ S) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= NE) return;
     const long long ncols = (long long)Nk * N2;
-    for (long long col = blockIdx.y; col < ncols; col += gridDim.y) {
+    const long long cchunk = (ncols + gridDim.y - 1) / gridDim.y;
+    const long long c0 = (long long)blockIdx.y * cchunk, c1 = min(ncols, c0 + cchunk);
+    long long qcur = -1;
+    c64 y = cmk(0.0, 0.0);
+    for (long long col = c0; col < c1; ++col) {
         const long long q = col / Nk;  // uniform over the block
         const int m = (int)(col - q * Nk);
         const long long idx = col * NE + e;
-        const c64 y = Y[e + (long long)NE * q];
+        if (q != qcur) {
+            y = Y[e + (long long)NE * q];
+            qcur = q;
+        }
         const double fw = wout ? f * wout[m] : f;
         c64 v = cmk(fw * y.x, fw * y.y);
         if (beta != 0.0) {
@@ -259,26 +267,40 @@ __global__ void k_lowrank_sum(int NE, int r, int nchunk, const c64* __restrict__
 }
 
 // S2[e + NE q] = f sum_i red[e + NE i] left[q + Nc i] + beta S2; thread =
-// energy (its r reduced values held in registers), block rows stride over q.
+// energy (its r reduced values held in registers); block row y takes a
+// contiguous chunk of q, whose factor entries are staged in LDS first (one
+// load pass instead of a dependent load per q).
+constexpr int kExpandChunk = 128;
 __global__ void k_lowrank_expand(int NE, long long Nc, int r, const c64* __restrict__ red,
                                  const c64* __restrict__ left, double f, double beta, c64* __restrict__ S) {
+    __shared__ c64 lf[kExpandChunk * kMaxRank];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= NE) return;
     c64 rv[kMaxRank];
 #pragma unroll
-    for (int i = 0; i < kMaxRank; ++i) rv[i] = i < r ? red[e + (long long)NE * i] : cmk(0.0, 0.0);
-    for (long long q = blockIdx.y; q < Nc; q += gridDim.y) {
-        c64 v = cmk(0.0, 0.0);
-#pragma unroll
-        for (int i = 0; i < kMaxRank; ++i)
-            if (i < r) v = cfma(v, rv[i], left[q + Nc * i]);
-        v = cmk(f * v.x, f * v.y);
-        const long long idx = q * NE + e;
-        if (beta != 0.0) {
-            const c64 o = S[idx];
-            v = cmk(beta * o.x + v.x, beta * o.y + v.y);
+    for (int i = 0; i < kMaxRank; ++i) rv[i] = (e < NE && i < r) ? red[e + (long long)NE * i] : cmk(0.0, 0.0);
+    for (long long qb = (long long)blockIdx.y * kExpandChunk; qb < Nc; qb += (long long)gridDim.y * kExpandChunk) {
+        const int nq = (int)min((long long)kExpandChunk, Nc - qb);
+        __syncthreads();  // the previous chunk's factor entries are no longer read
+        for (int t = threadIdx.x; t < nq * r; t += blockDim.x) {
+            const int qq = t % nq, i = t / nq;
+            lf[qq * kMaxRank + i] = left[qb + qq + Nc * i];
         }
-        S[idx] = v;
+        __syncthreads();
+        if (e >= NE) continue;
+        for (int qq = 0; qq < nq; ++qq) {
+            const long long q = qb + qq;
+            c64 v = cmk(0.0, 0.0);
+#pragma unroll
+            for (int i = 0; i < kMaxRank; ++i)
+                if (i < r) v = cfma(v, rv[i], lf[qq * kMaxRank + i]);
+            v = cmk(f * v.x, f * v.y);
+            const long long idx = q * NE + e;
+            if (beta != 0.0) {
+                const c64 o = S[idx];
+                v = cmk(beta * o.x + v.x, beta * o.y + v.y);
+            }
+            S[idx] = v;
+        }
     }
 }
 
@@ -318,8 +340,8 @@ static int lo_lowrank(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, dou
                        (int)r, (int)nchunk, partial, red);
     int st = lo_combine(ctx, (int)NE, NE * r, red, ca, a, cb, b, coef);
     if (st) return st;
-    hipLaunchKernelGGL(k_lowrank_expand, ecol_grid((int)NE, Nc, ctx->sm_count), dim3(256), 0, ctx->stream, (int)NE,
-                       Nc, (int)r, coef, K->a, qK, 0.0, S);
+    hipLaunchKernelGGL(k_lowrank_expand, ecol_grid((int)NE, (Nc + kExpandChunk - 1) / kExpandChunk, ctx->sm_count),
+                       dim3(256), 0, ctx->stream, (int)NE, Nc, (int)r, coef, K->a, qK, 0.0, S);
     return launch_check(ctx, "low-rank LO contraction");
 }
 
@@ -360,8 +382,8 @@ int contract_core(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double 
             hipLaunchKernelGGL(k_lowrank_sum, dim3((unsigned)((NE * r + 255) / 256)), dim3(256), 0, ctx->stream,
                                (int)NE, (int)r, (int)nchunk, partial, red);
             // S2 (NE x Nc) = f red left^T + beta S2, streamed over q.
-            hipLaunchKernelGGL(k_lowrank_expand, ecol_grid((int)NE, Nc, ctx->sm_count), dim3(256), 0,
-                               ctx->stream, (int)NE, Nc, (int)r, red, K->a, f, beta, S);
+            hipLaunchKernelGGL(k_lowrank_expand, ecol_grid((int)NE, (Nc + kExpandChunk - 1) / kExpandChunk, ctx->sm_count),
+                               dim3(256), 0, ctx->stream, (int)NE, Nc, (int)r, red, K->a, f, beta, S);
             return launch_check(ctx, "low-rank contraction");
         }
         case QCLHIP_KERNEL_MOMENTUM_INDEPENDENT: {
