@@ -60,6 +60,51 @@ __device__ inline void bstore(c64* T, int lr, int lc, const c64 (&v)[B][B]) {
         for (int c = 0; c < B; ++c) T[(lr * B + r) * LD + lc * B + c] = v[r][c];
 }
 
+// Rows [s 8 B/SPLIT, (s+1) 8 B/SPLIT) of bprod, the share of wave s of SPLIT
+// waves on one block (each entry the same q-ordered FMA chain as bprod's).
+template <int B, int SPLIT, bool LA, bool RA>
+__device__ inline void bprod_rows(const c64* __restrict__ L, const c64* __restrict__ R, int N, int s, int lr, int lc,
+                                  c64 (&out)[B / SPLIT][B], bool accumulate) {
+    constexpr int LD = 8 * B + 1;
+    constexpr int BR = B / SPLIT;
+    const int row0 = s * 8 * BR + lr * BR;
+    if (!accumulate) {
+#pragma unroll
+        for (int r = 0; r < BR; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) out[r][c] = cmk(0.0, 0.0);
+    }
+#pragma unroll 2
+    for (int q = 0; q < N; ++q) {
+        c64 lv[BR], rv[B];
+#pragma unroll
+        for (int r = 0; r < BR; ++r) {
+            const int i = row0 + r;
+            lv[r] = LA ? cconj(L[q * LD + i]) : L[i * LD + q];
+        }
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int j = lc * B + c;
+            rv[c] = RA ? cconj(R[j * LD + q]) : R[q * LD + j];
+        }
+#pragma unroll
+        for (int r = 0; r < BR; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) out[r][c] = cfma(out[r][c], lv[r], rv[c]);
+    }
+}
+
+template <int B, int SPLIT>
+__device__ inline void bstore_rows(c64* T, int s, int lr, int lc, const c64 (&v)[B / SPLIT][B]) {
+    constexpr int LD = 8 * B + 1;
+    constexpr int BR = B / SPLIT;
+    const int row0 = s * 8 * BR + lr * BR;
+#pragma unroll
+    for (int r = 0; r < BR; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) T[(row0 + r) * LD + lc * B + c] = v[r][c];
+}
+
 // Coefficients of the paired-map reverse: a, c, Nbar, Hbar, k.
 struct PairAdjoint {
     double a, c, nbar, hbar, k;
@@ -83,7 +128,7 @@ struct GAArgs {
 };
 
 // Tile of k1 A + k2 B (+ kd i k w on the diagonal) for W energies.
-template <int W, int LD>
+template <int W, int LD, int TH = 64 * W>
 __device__ inline void tile_load_combo(c64* lds, int wstride, const c64* A, double k1, const c64* Bf, double k2,
                                        double kd, const double* wE, double wkm, int e0, long long blk0, int nvalid,
                                        int N, size_t S) {
@@ -93,7 +138,7 @@ __device__ inline void tile_load_combo(c64* lds, int wstride, const c64* A, doub
     int ij = f / W;
     int i = ij % N, j = ij / N;
     const double dw = (w < nvalid) ? kd * wE[e0 + w] * wkm : 0.0;
-    for (; f < total; f += 64 * W) {
+    for (; f < total; f += TH) {
         if (w < nvalid) {
             const size_t idx = blk0 + w + S * (i + (size_t)N * j);
             const c64 x = A[idx], y = Bf[idx];
@@ -101,7 +146,7 @@ __device__ inline void tile_load_combo(c64* lds, int wstride, const c64* A, doub
             if (i == j) v.y += dw;
             lds[w * wstride + i * LD + j] = v;
         }
-        i += 64;
+        i += TH / W;
         while (i >= N) {
             i -= N;
             ++j;
@@ -202,6 +247,99 @@ __global__ __launch_bounds__(64 * W) void k_green_adjoint(GAArgs p) {
     }
     __syncthreads();
     tile_store<W, LD>(X0, wstride, p.SRbar, blk0, mask, N, S);
+}
+
+// k_green_adjoint with SPLIT waves per block sharing its four tiles (wave s
+// computes rows s 8 B/SPLIT .. of every product, each entry the same FMA
+// chain as in the one-wave kernel, so the results are bitwise equal): four
+// tiles per wave would hold a CU to four waves at N <= 24.
+template <int B, int W, int SPLIT>
+__global__ __launch_bounds__(64 * SPLIT * W) void k_green_adjoint_split(GAArgs p) {
+    constexpr int NP = 8 * B;
+    constexpr int LD = NP + 1;
+    constexpr int TH = 64 * SPLIT * W;
+    constexpr int BR = B / SPLIT;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int b = wave / SPLIT, s = wave % SPLIT;
+    const int nte = (p.NE + W - 1) / W;
+    const long long t = xcd_tile(blockIdx.x, gridDim.x);
+    const int e0 = (int)(t % nte) * W;
+    const int m = (int)(t / nte);
+    const int nvalid = min(W, p.NE - e0);
+    const long long nblk = (long long)p.NE * p.Nk;
+    const long long blk0 = e0 + (long long)p.NE * m;
+    const int wstride = TileStride<B>::value;
+    c64* G0 = reinterpret_cast<c64*>(smem);
+    c64* X0 = G0 + (size_t)W * wstride;
+    c64* Y0 = X0 + (size_t)W * wstride;
+    c64* Z0 = Y0 + (size_t)W * wstride;
+    c64* Gs = G0 + (size_t)b * wstride;
+    c64* Xs = X0 + (size_t)b * wstride;
+    c64* Ys = Y0 + (size_t)b * wstride;
+    c64* Zs = Z0 + (size_t)b * wstride;
+    const int N = p.N;
+    const size_t S = (size_t)nblk;
+    const int lr = lane & 7, lc = lane >> 3;
+    const int row0 = s * 8 * BR + lr * BR;
+    const bool active = b < nvalid;
+    const unsigned mask = (nvalid >= 32) ? 0xffffffffu : ((1u << nvalid) - 1u);
+    const PairAdjoint cf = *p.coef;
+    const c64* gr_fam[1] = {p.GR};
+    tile_load<W, LD, TH>(G0, wstride, gr_fam, 1, blk0, nvalid, N, S);
+    if (p.GRbar) {
+        const c64* grb[1] = {p.GRbar};
+        tile_load<W, LD, TH>(Z0, wstride, grb, 1, blk0, nvalid, N, S);
+    }
+    __syncthreads();
+    c64 acc[BR][B];
+#pragma unroll
+    for (int r = 0; r < BR; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int i = row0 + r, j = lc * B + c;
+            acc[r][c] = (p.GRbar && i < N && j < N) ? Zs[i * LD + j] : cmk(0.0, 0.0);
+        }
+    c64 T[BR][B];
+    for (int comp = 0; comp < 2; ++comp) {
+        __syncthreads();
+        if (comp == 0)
+            tile_load_combo<W, LD, TH>(X0, wstride, p.GLbar, cf.a, p.GGbar, -(1.0 - cf.a), cf.nbar * cf.k, p.wE,
+                                       p.wk[m], e0, blk0, nvalid, N, S);
+        else
+            tile_load_combo<W, LD, TH>(X0, wstride, p.GLbar, -cf.c, p.GGbar, 1.0 - cf.c, -cf.hbar * cf.k, p.wE,
+                                       p.wk[m], e0, blk0, nvalid, N, S);
+        tile_load<W, LD, TH>(Y0, wstride, comp ? p.famG : p.famL, p.nfam, blk0, nvalid, N, S);
+        __syncthreads();
+        if (active) bprod_rows<B, SPLIT, false, true>(Gs, Ys, N, s, lr, lc, T, false);  // G Sc^dagger
+        if (active) bstore_rows<B, SPLIT>(Zs, s, lr, lc, T);
+        __syncthreads();
+        if (active) bprod_rows<B, SPLIT, false, false>(Xs, Zs, N, s, lr, lc, acc, true);  // += Gc_bar G Sc^dagger
+        __syncthreads();
+        if (active) bprod_rows<B, SPLIT, false, false>(Gs, Ys, N, s, lr, lc, T, false);  // G Sc
+        if (active) bstore_rows<B, SPLIT>(Zs, s, lr, lc, T);
+        __syncthreads();
+        if (active) bprod_rows<B, SPLIT, true, false>(Xs, Zs, N, s, lr, lc, acc, true);  // += Gc_bar^dagger G Sc
+        __syncthreads();
+        if (active) bprod_rows<B, SPLIT, true, false>(Gs, Xs, N, s, lr, lc, T, false);  // G^dagger Gc_bar
+        if (active) bstore_rows<B, SPLIT>(Zs, s, lr, lc, T);
+        wave_sync();  // (G^dagger Gc_bar) G reads only this wave's rows
+        if (active) bprod_rows<B, SPLIT, false, false>(Zs, Gs, N, s, lr, lc, T, false);
+        if (active) bstore_rows<B, SPLIT>(Ys, s, lr, lc, T);  // every wave is past its reads of Sc
+        __syncthreads();
+        tile_store<W, LD, false, TH>(Y0, wstride, comp ? p.SGbar : p.SLbar, blk0, mask, N, S);
+    }
+    __syncthreads();
+    if (active) bstore_rows<B, SPLIT>(Zs, s, lr, lc, acc);
+    __syncthreads();
+    if (active) bprod_rows<B, SPLIT, true, false>(Gs, Zs, N, s, lr, lc, T, false);  // G^dagger G^R_bar
+    if (active) bstore_rows<B, SPLIT>(Ys, s, lr, lc, T);
+    wave_sync();
+    if (active) bprod_rows<B, SPLIT, false, true>(Ys, Gs, N, s, lr, lc, T, false);  // (.) G^dagger
+    if (active) bstore_rows<B, SPLIT>(Xs, s, lr, lc, T);
+    __syncthreads();
+    tile_store<W, LD, false, TH>(X0, wstride, p.SRbar, blk0, mask, N, S);
 }
 
 // Partial inner products for the paired-map reverse: per thread of a fixed
@@ -362,8 +500,26 @@ int shift_transpose(qclhip_ctx* ctx, const qclhip_shift* s, long long ncols, con
     return launch_check(ctx, "k_shift_transpose");
 }
 
+template <int B, int W, int SPLIT>
+int launch_green_adjoint_split(qclhip_ctx* ctx, const GAArgs& a) {
+    const size_t lds = 4 * (size_t)TileStride<B>::value * sizeof(c64) * W;
+    if (lds > ctx->max_lds_per_block)
+        return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "basis size needs more LDS than the device offers");
+    if (lds > 65536)
+        QCLHIP_TRY_HIP(ctx, hipFuncSetAttribute((const void*)k_green_adjoint_split<B, W, SPLIT>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long long tiles = (long long)((a.NE + W - 1) / W) * a.Nk;
+    hipLaunchKernelGGL((k_green_adjoint_split<B, W, SPLIT>), dim3((unsigned)tiles), dim3(64 * SPLIT * W), lds,
+                       ctx->stream, a);
+    return launch_check(ctx, "k_green_adjoint_split");
+}
+
 template <int B>
 int launch_green_adjoint(qclhip_ctx* ctx, const GAArgs& a) {
+    // N = 17..24: three waves per block, two blocks per workgroup (measured
+    // at N = 24, NE = 2048, Nk = 64: 10.97 ms for the reverse GF solve
+    // against 12.45 ms with one wave per block, 11.66 ms with four blocks)
+    if constexpr (B == 3) return launch_green_adjoint_split<B, 2, 3>(ctx, a);
     constexpr size_t per_wave = 4 * (size_t)TileStride<B>::value * sizeof(c64);
     // 4 energies (64-byte segments, one workgroup per CU) where they fit, else 2
     // (two workgroups per CU) or 1: the same waves per CU either way.
