@@ -40,13 +40,26 @@ def converge(p: ScbaProblem, s: ScbaState, tol: float = 1e-15, max_iterations: i
 
 
 class _Families:
-    """The adjoint state mu, one Family of gradients per SCBA family."""
+    """The adjoint state mu, one Family of gradients per SCBA family.
 
-    def __init__(self, p: ScbaProblem):
+    With shared=True the minus family and every mechanism's family are one
+    Family object, and plus holds that object's R: under the linear
+    iteration every family receives the same new value (sig; plus: sig +
+    dJ/dplus in L and G) from the same zero start, so these stay equal to
+    the last bit and one copy carries them all."""
+
+    def __init__(self, p: ScbaProblem, shared: bool = False):
         g = p.grid
-        self.plus = new_family(g)
-        self.minus = new_family(g)
-        self.scat = {m.name: new_family(g) for m in p.mechanisms}
+        self.shared = shared
+        if shared:
+            common = new_family(g)
+            self.plus = Family(common.R, api.zeros_field(g), api.zeros_field(g))
+            self.minus = common
+            self.scat = {m.name: common for m in p.mechanisms}
+        else:
+            self.plus = new_family(g)
+            self.minus = new_family(g)
+            self.scat = {m.name: new_family(g) for m in p.mechanisms}
 
     def all(self):
         return [self.plus, self.minus] + list(self.scat.values())
@@ -64,11 +77,15 @@ class ScbaAdjoint:
         p, g = self.p, self.p.grid
         GRb, GLb, GGb = out
         tmpL, tmpG, Z = api.empty_field(g), api.empty_field(g), api.empty_field(g)
-        for m in p.mechanisms:
+        for k, m in enumerate(p.mechanisms):
             f = mu.scat[m.name]
-            api.copy_field(f.L, tmpL)
-            api.copy_field(f.G, tmpG)
-            api.retarded_self_energy_adjoint(p.hilbert, f.R, tmpL, tmpG)
+            # S<_bar, S>_bar of the mechanism's candidate: its L, G gradients
+            # plus the reverse of S^R = H[Gamma] - i Gamma/2 (once for a
+            # shared family: the same input gives the same fields)
+            if k == 0 or not mu.shared:
+                api.copy_field(f.L, tmpL)
+                api.copy_field(f.G, tmpG)
+                api.retarded_self_energy_adjoint(p.hilbert, f.R, tmpL, tmpG)
             K = self.kadj[m.name]
             if m.is_lo:
                 # S< = qK[(N+1) S+ K[G<] + N S- K[G<]],  S> = qK[(N+1) S- K[G>] + N S+ K[G>]]
@@ -87,7 +104,7 @@ class ScbaAdjoint:
             api.embedding_adjoint(g, T, plan, f.G, GGb)
 
     def gradient(self, s: ScbaState, tol: float = 1e-13, max_iterations: int = 5000, alpha: float | None = None,
-                 acceleration: str = "linear", depth: int = 5, checkpoints=()):
+                 acceleration: str = "linear", depth: int = 5, checkpoints=(), share_families: bool = True):
         """dJ/dh (device tensor (Nb,Nb,Nk), the reference layout of h) at the
         converged state s, plus the adjoint state and iteration count.
         acceleration: "linear" (the damped iteration at the forward's rate)
@@ -95,7 +112,11 @@ class ScbaAdjoint:
         the forward's mixer without its self-energy guard, restarts or
         coefficient bound). checkpoints: iteration counts at which the
         gradient of the current adjoint state is also formed; they are
-        returned as meta["checkpoints"] = [(iteration, dJ/dh), ...]."""
+        returned as meta["checkpoints"] = [(iteration, dJ/dh), ...].
+        share_families: carry the families that the linear iteration keeps
+        equal in one copy (the same fields; the change measure sums the
+        same terms in another grouping). Anderson acceleration mixes every
+        family on its own."""
         p, g = self.p, self.p.grid
         alpha = p.alpha_mix if alpha is None else alpha
         if acceleration not in ("linear", "anderson"):
@@ -112,7 +133,7 @@ class ScbaAdjoint:
         api.boundary_flux_adjoint(g, 1.0, s.plus.L, s.plus.G, green.GL, green.GG, dplusL, dplusG, JL, JG)
         api.antihermitian_part(g, dplusL)
         api.antihermitian_part(g, dplusG)
-        mu = _Families(p)
+        mu = _Families(p, shared=share_families and mixer is None)
         GRb, GLb, GGb = api.empty_field(g), api.empty_field(g), api.empty_field(g)
         sig = (api.empty_field(g), api.empty_field(g), api.empty_field(g))
         plus_new = (api.empty_field(g), api.empty_field(g))  # sig + the direct dJ/dplus
@@ -145,16 +166,25 @@ class ScbaAdjoint:
             api.sum_fields([sig[2], dplusG], plus_new[1])
             # every family's new value is sig (plus: sig + dJ/dplus); mix and
             # measure the change ||new - mu|| / ||new|| in one pass
-            pairs = []
-            for f in mu.all():
-                newL, newG = (plus_new if f is mu.plus else (sig[1], sig[2]))
-                pairs += [(f.R, sig[0]), (f.L, newL), (f.G, newG)]
-            if mixer is None:
-                num, den = api.mix_fields_change(pairs, alpha)
+            if mu.shared:
+                # R is carried M + 2 times, the common L and G M + 1 times
+                M = len(p.mechanisms)
+                nR, dR = api.mix_fields_change([(mu.minus.R, sig[0])], alpha)
+                nC, dC = api.mix_fields_change([(mu.minus.L, sig[1]), (mu.minus.G, sig[2])], alpha)
+                nP, dP = api.mix_fields_change([(mu.plus.L, plus_new[0]), (mu.plus.G, plus_new[1])], alpha)
+                num = (M + 2) * nR + (M + 1) * nC + nP
+                den = (M + 2) * dR + (M + 1) * dC + dP
             else:
-                num, den = api.mix_fields_change(pairs, 0.0)  # measure only
-                mixer.mix([o for o, _ in pairs], [c for _, c in pairs], alpha, len(pairs) // 3, guard=False,
-                          linear=True)
+                pairs = []
+                for f in mu.all():
+                    newL, newG = (plus_new if f is mu.plus else (sig[1], sig[2]))
+                    pairs += [(f.R, sig[0]), (f.L, newL), (f.G, newG)]
+                if mixer is None:
+                    num, den = api.mix_fields_change(pairs, alpha)
+                else:
+                    num, den = api.mix_fields_change(pairs, 0.0)  # measure only
+                    mixer.mix([o for o, _ in pairs], [c for _, c in pairs], alpha, len(pairs) // 3, guard=False,
+                              linear=True)
             change = (num / max(den, 1e-300)) ** 0.5
             if it == 1 or it % 100 == 0:
                 history.append((it, change))

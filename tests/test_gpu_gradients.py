@@ -146,6 +146,39 @@ def test_adjoint_checkpoints(hip, converged):
     assert err[2] < err[0], err
 
 
+def test_shared_adjoint_families_equal_separate(hip, converged):
+    """The linear adjoint iteration carries the families it keeps equal (minus
+    and every mechanism's; plus's R) in one copy: after a fixed number of
+    iterations the gradient, and the gradients at checkpoints, equal those of
+    the iteration that carries every family on its own, bit for bit; the
+    change measure (the same terms summed in another grouping) agrees to
+    rounding."""
+    import torch
+    from qclhip import adjoint, scba
+    c = converged
+    prob = c["prob"]
+    g = hip.Grid(prob["eps"], prob["wE"], prob["wk"], prob["Nb"])
+    mechs = [scba.Mechanism(m["name"], hip.ProductionKernel(g, m["kind"], m["a"], m["b"], m["c"], rank=m["rank"]),
+                            m["qK"], m["is_lo"]) for m in prob["mechanisms"]]
+    p = scba.ScbaProblem(g, hip.h_to_device(prob["h"]), hip.matrix_to_device(prob["Tplus"]),
+                         hip.matrix_to_device(prob["Tminus"]), mechs, c["full"]["target"], prob["Ep"], prob["hwLO"],
+                         prob["NLO"], prob["alpha_mix"], verify_roundoff=False)
+    dev = lambda x: hip.to_device(x)
+    st = c["state0"]
+    s = scba.ScbaState(None, {m["name"]: scba.Family(dev(st["scatR"][i]), dev(st["scatL"][i]), dev(st["scatG"][i]))
+                              for i, m in enumerate(prob["mechanisms"])},
+                       scba.Family(dev(st["plusR"]), dev(st["plusL"]), dev(st["plusG"])),
+                       scba.Family(dev(st["minusR"]), dev(st["minusL"]), dev(st["minusG"])))
+    adjoint.converge(p, s, tol=1e-15)
+    adj = adjoint.ScbaAdjoint(p)
+    runs = [adj.gradient(s, tol=0.0, max_iterations=40, checkpoints=(15,), share_families=sh) for sh in (True, False)]
+    (h1, m1), (h2, m2) = runs
+    assert torch.equal(h1, h2)
+    assert torch.equal(m1["checkpoints"][0][1], m2["checkpoints"][0][1])
+    assert m1["iterations"] == m2["iterations"] == 40
+    assert abs(m1["change"] - m2["change"]) <= 1e-12 * m2["change"]
+
+
 def test_anderson_adjoint_equals_linear(hip, orc):
     """The Anderson-accelerated adjoint iteration reaches the gradient of the
     damped linear one (the same fixed point) in fewer iterations, and a
