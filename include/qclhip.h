@@ -300,6 +300,16 @@ int qclhip_kernel_destroy(qclhip_kernel* kernel);
 /* S <- alpha*qK*K[wk G] + beta*S; beta == 0 does not read S. */
 int qclhip_contract(qclhip_ctx* ctx, const qclhip_kernel* kernel, const double* G,
                     double qK, double alpha, double beta, double* S);
+/* The momentum-independent branch of _production_static_contraction!
+ * (production_operators.jl:535-595) without its broadcast over momenta: Y
+ * (NE x Nb^2, one momentum's columns) <- alpha*qK*K[wk G], which every
+ * momentum of the full result holds bit for bit. Forward-form kernels of
+ * kind QCLHIP_KERNEL_MOMENTUM_INDEPENDENT only (ARGUMENT otherwise). */
+int qclhip_contract_compact(qclhip_ctx* ctx, const qclhip_kernel* kernel, const double* G,
+                            double qK, double alpha, double* Y);
+/* The broadcast of that branch: S(e, m, a, b) = Y(e, a, b) for every
+ * momentum m of the grid (Y: NE x Nb^2). */
+int qclhip_broadcast_momenta(qclhip_ctx* ctx, const qclhip_grid* grid, const double* Y, double* S);
 
 /* ---- A8: _production_lo_contraction, thermal N_LO (production_operators.jl:
  * 1089-1162): S< = (N+1) K[S+ G<] + N K[S- G<], S> = (N+1) K[S- G>] + N K[S+ G>]
@@ -326,6 +336,11 @@ int qclhip_hilbert_destroy(qclhip_hilbert* plan);
 #define QCLHIP_HILBERT_PATH_LDS 1
 #define QCLHIP_HILBERT_PATH_HBM 2
 int qclhip_hilbert_set_path(qclhip_ctx* ctx, qclhip_hilbert* plan, int path);
+/* Each column of the plan stands for mult identical columns in the round-off
+ * witness's norms (a momentum-independent self-energy transformed once for
+ * all Nk momenta, mult = Nk): the witness equals that of the full field's
+ * transform (fft_parallel.jl:247-291) up to rounding. Default 1. */
+int qclhip_hilbert_set_multiplicity(qclhip_ctx* ctx, qclhip_hilbert* plan, int64_t mult);
 int qclhip_hilbert_get_path(qclhip_ctx* ctx, const qclhip_hilbert* plan, int* path_host);
 /* Lambda_e = sum_{e' != e} wE_e' Gamma_e' / (2 pi (eps_e - eps_e')).
  * roundoff_host (may be NULL; requires verify) = ||L2 - L|| / (||L|| + 1e-14). */

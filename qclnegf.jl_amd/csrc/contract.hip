@@ -357,6 +357,29 @@ size_t core_scratch_bytes(const qclhip_kernel* K) {
     }
 }
 
+// The momentum-independent representation: X = sum_m' w_m' G(., m', .),
+// Y = X block^T, then S(e, m, q) = f Y(e, q) + beta S for nk_out momenta:
+// nk_out = Nk writes the field, nk_out = 1 the single column set that every
+// momentum holds in the forward form (the values of each momentum's columns
+// bit for bit).
+int mi_contract(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double f, double beta, c64* S, int nk_out) {
+    const qclhip_grid* g = K->grid;
+    const long long NE = g->NE, Nk = g->Nk, N2 = (long long)g->Nb * g->Nb;
+    c64* X = (c64*)workspace(ctx, (size_t)2 * NE * N2 * sizeof(c64));
+    if (!X) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
+    c64* Y = X + NE * N2;
+    hipLaunchKernelGGL(k_mi_reduce, dim3(grid_blocks(NE * N2, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
+                       (int)NE, (int)Nk, (int)N2, K->weights_out ? (const double*)nullptr : g->wk, G, X);
+    int st = launch_check(ctx, "k_mi_reduce");
+    if (st) return st;
+    st = gemm_nt(ctx, NE, N2, N2, 1.0, X, NE, K->a, N2, 0.0, Y, NE);
+    if (st) return st;
+    hipLaunchKernelGGL(k_mi_broadcast, ecol_grid((int)NE, (long long)nk_out * N2, ctx->sm_count), dim3(256), 0,
+                       ctx->stream, (int)NE, nk_out, (int)N2, Y, f, beta,
+                       K->weights_out ? (const double*)g->wk : nullptr, S);
+    return launch_check(ctx, "k_mi_broadcast");
+}
+
 // S <- f K[w G] + beta S for the given representation.
 int contract_core(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double f, double beta, c64* S) {
     const qclhip_grid* g = K->grid;
@@ -386,21 +409,8 @@ int contract_core(qclhip_ctx* ctx, const qclhip_kernel* K, const c64* G, double 
                                dim3(256), 0, ctx->stream, (int)NE, Nc, (int)r, red, K->a, f, beta, S);
             return launch_check(ctx, "low-rank contraction");
         }
-        case QCLHIP_KERNEL_MOMENTUM_INDEPENDENT: {
-            c64* X = (c64*)workspace(ctx, (size_t)2 * NE * N2 * sizeof(c64));
-            if (!X) return fail(ctx, QCLHIP_ERR_RUNTIME, "scratch allocation failed");
-            c64* Y = X + NE * N2;
-            hipLaunchKernelGGL(k_mi_reduce, dim3(grid_blocks(NE * N2, 256, ctx->sm_count)), dim3(256), 0, ctx->stream,
-                               (int)NE, (int)Nk, (int)N2, K->weights_out ? (const double*)nullptr : g->wk, G, X);
-            int st = launch_check(ctx, "k_mi_reduce");
-            if (st) return st;
-            st = gemm_nt(ctx, NE, N2, N2, 1.0, X, NE, K->a, N2, 0.0, Y, NE);
-            if (st) return st;
-            hipLaunchKernelGGL(k_mi_broadcast, ecol_grid((int)NE, (long long)Nk * N2, ctx->sm_count), dim3(256), 0,
-                               ctx->stream, (int)NE, (int)Nk, (int)N2, Y, f, beta,
-                               K->weights_out ? (const double*)g->wk : nullptr, S);
-            return launch_check(ctx, "k_mi_broadcast");
-        }
+        case QCLHIP_KERNEL_MOMENTUM_INDEPENDENT:
+            return mi_contract(ctx, K, G, f, beta, S, (int)Nk);
         case QCLHIP_KERNEL_SEPARABLE_IFR: {
             const long long J = K->rank, M = NE * Nk;
             // The J x N^2 factor sits in LDS; above 64 KiB (e.g. six interfaces
@@ -634,6 +644,27 @@ int qclhip_contract(qclhip_ctx* ctx, const qclhip_kernel* kernel, const double* 
     if (!ctx || !kernel || !G || !S) return QCLHIP_ERR_ARGUMENT;
     if (G == S) return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
     return contract_core(ctx, kernel, (const c64*)G, alpha * qK, beta, (c64*)S);
+}
+
+int qclhip_contract_compact(qclhip_ctx* ctx, const qclhip_kernel* kernel, const double* G, double qK, double alpha,
+                            double* Y) {
+    if (!ctx || !kernel || !G || !Y) return QCLHIP_ERR_ARGUMENT;
+    if (G == Y) return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
+    if (kernel->kind != QCLHIP_KERNEL_MOMENTUM_INDEPENDENT || kernel->weights_out)
+        return fail(ctx, QCLHIP_ERR_ARGUMENT,
+                    "the compact contraction takes a momentum-independent kernel in its forward form");
+    return mi_contract(ctx, kernel, (const c64*)G, alpha * qK, 0.0, (c64*)Y, 1);
+}
+
+int qclhip_broadcast_momenta(qclhip_ctx* ctx, const qclhip_grid* grid, const double* Y, double* S) {
+    if (!ctx || !grid || !Y || !S) return QCLHIP_ERR_ARGUMENT;
+    if (Y == S) return fail(ctx, QCLHIP_ERR_ARGUMENT, "in-place aliasing is not supported");
+    const long long N2 = (long long)grid->Nb * grid->Nb;
+    // S = 1.0 * Y + 0: every momentum's columns are Y bit for bit.
+    hipLaunchKernelGGL(k_mi_broadcast, ecol_grid(grid->NE, (long long)grid->Nk * N2, ctx->sm_count), dim3(256), 0,
+                       ctx->stream, grid->NE, grid->Nk, (int)N2, (const c64*)Y, 1.0, 0.0, (const double*)nullptr,
+                       (c64*)S);
+    return launch_check(ctx, "k_mi_broadcast");
 }
 
 int qclhip_lo_contraction(qclhip_ctx* ctx, const qclhip_kernel* kernel, const double* GL, const double* GG,

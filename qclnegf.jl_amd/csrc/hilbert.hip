@@ -37,6 +37,9 @@ struct qclhip_hilbert {
     qclhip::c64* kp[2] = {nullptr, nullptr};
     int L1 = 0, L2 = 0;
     int path = 0;  // QCLHIP_HILBERT_PATH_*; AUTO resolves by L
+    // Identical columns each column stands for in the witness's norms (a
+    // momentum-independent self-energy transformed once for every momentum)
+    double col_mult = 1.0;
 };
 
 namespace qclhip {
@@ -504,7 +507,8 @@ __global__ __launch_bounds__(threads_for(L)) void k_hilbert(HilbertArgs a) {
     }
 }
 
-__global__ __launch_bounds__(1024) void k_witness_reduce(const double* partial, long long ncols, double* out) {
+__global__ __launch_bounds__(1024) void k_witness_reduce(const double* partial, long long ncols, double mult,
+                                                        double* out) {
     __shared__ double red[2][1024];
     const int t = threadIdx.x;
     const long long per = (ncols + 1023) / 1024;
@@ -523,7 +527,7 @@ __global__ __launch_bounds__(1024) void k_witness_reduce(const double* partial, 
         }
         __syncthreads();
     }
-    if (t == 0) out[0] = sqrt(red[0][0]) / (sqrt(red[1][0]) + 1e-14);
+    if (t == 0) out[0] = sqrt(mult * red[0][0]) / (sqrt(mult * red[1][0]) + 1e-14);
 }
 
 // ---- energy axes beyond one workgroup's LDS: a four-step FFT through HBM ---
@@ -987,7 +991,7 @@ int hilbert_run(qclhip_ctx* ctx, const qclhip_hilbert* p, const double* Gamma, c
     int rc = fs ? hilbert_fourstep(ctx, p, a, (c64*)(scratch + part_bytes)) : hilbert_launch(ctx, p, a);
     if (rc || !witness) return rc;
     double* res = partial + 2 * nparts;
-    hipLaunchKernelGGL(k_witness_reduce, dim3(1), dim3(1024), 0, ctx->stream, partial, nparts, res);
+    hipLaunchKernelGGL(k_witness_reduce, dim3(1), dim3(1024), 0, ctx->stream, partial, nparts, p->col_mult, res);
     rc = launch_check(ctx, "k_witness_reduce");
     if (rc) return rc;
     bool deferred = false;
@@ -1078,6 +1082,13 @@ int qclhip_hilbert_destroy(qclhip_hilbert* p) {
     if (p->kp[0]) hipFree(p->kp[0]);
     if (p->kp[1]) hipFree(p->kp[1]);
     delete p;
+    return QCLHIP_OK;
+}
+
+int qclhip_hilbert_set_multiplicity(qclhip_ctx* ctx, qclhip_hilbert* plan, int64_t mult) {
+    if (!ctx || !plan) return QCLHIP_ERR_ARGUMENT;
+    if (mult < 1) return fail(ctx, QCLHIP_ERR_ARGUMENT, "the column multiplicity must be positive");
+    plan->col_mult = (double)mult;
     return QCLHIP_OK;
 }
 

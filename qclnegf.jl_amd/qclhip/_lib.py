@@ -97,10 +97,13 @@ _SIGNATURES = {
     "qclhip_kernel_create": [_vp, _vp, ct.c_int, ct.c_int, _dp, _dp, _dp, ct.POINTER(_vp)],
     "qclhip_kernel_destroy": [_vp],
     "qclhip_contract": [_vp, _vp, _vp, ct.c_double, ct.c_double, ct.c_double, _vp],
+    "qclhip_contract_compact": [_vp, _vp, _vp, ct.c_double, ct.c_double, _vp],
+    "qclhip_broadcast_momenta": [_vp, _vp, _vp, _vp],
     "qclhip_lo_contraction": [_vp, _vp, _vp, _vp, ct.c_double, ct.c_double, _vp, _vp, _vp, _vp],
     "qclhip_hilbert_create": [_vp, _vp, _i64, ct.c_int, ct.POINTER(_vp)],
     "qclhip_hilbert_destroy": [_vp],
     "qclhip_hilbert_set_path": [_vp, _vp, ct.c_int],
+    "qclhip_hilbert_set_multiplicity": [_vp, _vp, ct.c_int64],
     "qclhip_hilbert_get_path": [_vp, _vp, ct.POINTER(ct.c_int)],
     "qclhip_fft_hilbert": [_vp, _vp, _vp, _vp, _dp],
     "qclhip_retarded_self_energy": [_vp, _vp, _vp, _vp, _vp, _vp, _dp],

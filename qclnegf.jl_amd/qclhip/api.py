@@ -383,6 +383,13 @@ class FFTHilbertPlan:
 
     PATHS = {"auto": 0, "lds": 1, "hbm": 2}
 
+    def set_multiplicity(self, mult: int):
+        """Each column stands for mult identical columns in the round-off
+        witness (a momentum-independent self-energy transformed once for
+        all Nk momenta)."""
+        self.grid.ctx.call("qclhip_hilbert_set_multiplicity", self.handle, ct.c_int64(int(mult)))
+        return self
+
     def set_path(self, path: str):
         """'auto' (by L), 'lds' (one workgroup per column, L <= 16384) or
         'hbm' (the four-step transform through HBM scratch, L >= 4096)."""
@@ -582,6 +589,43 @@ def production_static_contraction(kernel: ProductionKernel, G, qK: float = 1.0, 
     S = empty_field(kernel.grid) if S is None else S
     kernel.grid.ctx.call("qclhip_contract", kernel.handle, _ptr(G), ct.c_double(qK), ct.c_double(alpha),
                          ct.c_double(beta), _ptr(S))
+    return S
+
+
+def compact_field(grid, device=None):
+    """One momentum's columns (NE x Nb^2) of a momentum-independent field."""
+    return torch.empty(field_shape(grid.NE, 1, grid.Nb), dtype=torch.complex128,
+                       device=device or grid.ctx.device)
+
+
+def _check_compact(grid, t, name):
+    if t.dtype != torch.complex128 or not t.is_contiguous():
+        raise _lib.ArgumentError(f"{name} must be a contiguous complex128 tensor")
+    if tuple(t.shape) != field_shape(grid.NE, 1, grid.Nb):
+        raise _lib.DimensionMismatch(f"{name} has shape {tuple(t.shape)}, expected "
+                                     f"{field_shape(grid.NE, 1, grid.Nb)}")
+
+
+def momentum_independent_contraction_compact(kernel: ProductionKernel, G, qK: float = 1.0, alpha: float = 1.0,
+                                             Y=None):
+    """The momentum-independent branch of _production_static_contraction!
+    (production_operators.jl:535-595) without its broadcast: the NE x Nb^2
+    columns that every momentum of the result holds (bit for bit)."""
+    _check_field(kernel.grid, G, "G")
+    Y = compact_field(kernel.grid) if Y is None else Y
+    _check_compact(kernel.grid, Y, "Y")
+    kernel.grid.ctx.call("qclhip_contract_compact", kernel.handle, _ptr(G), ct.c_double(qK), ct.c_double(alpha),
+                         _ptr(Y))
+    return Y
+
+
+def broadcast_momenta(grid: Grid, Y, S=None):
+    """S(e, m, a, b) = Y(e, a, b) for every momentum m (the broadcast of the
+    momentum-independent contraction)."""
+    _check_compact(grid, Y, "Y")
+    S = empty_field(grid) if S is None else S
+    _check_field(grid, S, "S")
+    grid.ctx.call("qclhip_broadcast_momenta", grid.handle, _ptr(Y), _ptr(S))
     return S
 
 

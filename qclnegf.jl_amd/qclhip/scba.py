@@ -26,6 +26,33 @@ class Mechanism:
     is_lo: bool = False
 
 
+def _compact_mechanism(m) -> bool:
+    """A mechanism whose self-energy is the same at every momentum: a
+    momentum-independent kernel outside the LO map."""
+    return not m.is_lo and m.kernel.kind == "momentum_independent"
+
+
+def _mi_candidate(p, mech, green, fam):
+    """Candidate of a momentum-independent mechanism: S< and S> contracted to
+    the one column set that every momentum holds, S^R transformed once on it,
+    all three broadcast into the family; bit for bit the values of the
+    full-field route (production_operators.jl:535-595, 842-884), whose
+    transform repeats each column for every momentum. Returns the round-off
+    witness (None without verification)."""
+    g = p.grid
+    YR, YL, YG = p.mi_compact
+    api.momentum_independent_contraction_compact(mech.kernel, green.GL, mech.qK, Y=YL)
+    api.momentum_independent_contraction_compact(mech.kernel, green.GG, mech.qK, Y=YG)
+    r = None
+    if p.verify_roundoff:
+        _, _, r = api.retarded_self_energy_fft(p.hilbert_compact, YL, YG, return_roundoff=True, SR=YR)
+    else:
+        api.retarded_self_energy_fft(p.hilbert_compact, YL, YG, SR=YR)
+    for Y, S in ((YR, fam.R), (YL, fam.L), (YG, fam.G)):
+        api.broadcast_momenta(g, Y, S)
+    return r
+
+
 @dataclass
 class ScbaProblem:
     grid: api.Grid
@@ -52,6 +79,13 @@ class ScbaProblem:
         self.plusLO = api.EnergyShiftPlan(g, +self.hwLO)
         self.minusLO = api.EnergyShiftPlan(g, -self.hwLO)
         self.hilbert = api.FFTHilbertPlan(g, g.ncols, verify_roundoff=self.verify_roundoff)
+        # Momentum-independent mechanisms: one momentum's columns, their
+        # transform (each column standing for Nk in the witness) and buffers.
+        self.hilbert_compact, self.mi_compact = None, None
+        if any(_compact_mechanism(m) for m in self.mechanisms):
+            self.hilbert_compact = api.FFTHilbertPlan(g, g.Nb * g.Nb, verify_roundoff=self.verify_roundoff)
+            self.hilbert_compact.set_multiplicity(g.Nk)
+            self.mi_compact = tuple(api.compact_field(g) for _ in range(3))
 
 
 @dataclass
@@ -94,6 +128,11 @@ def _candidate_scattering(p: ScbaProblem, green: api.GreenState, out: dict):
     roundoff = [] if p.grid.ctx.deferring else 0.0
     for mech in p.mechanisms:
         fam = out[mech.name]
+        if _compact_mechanism(mech):
+            r = _mi_candidate(p, mech, green, fam)
+            if r is not None:
+                roundoff = roundoff + [r] if isinstance(roundoff, list) else max(roundoff, r)
+            continue
         if mech.is_lo:
             api.production_lo_contraction(mech.kernel, green.GL, green.GG, mech.qK, p.NLO, p.plusLO,
                                           p.minusLO, SL=fam.L, SG=fam.G)
@@ -305,17 +344,23 @@ def _lean_tail(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, green, info, sta
     roundoff = []
     cand = Family(tmp[0], tmp[1], tmp[2])
     for mech in p.mechanisms:
-        if mech.is_lo:
-            api.production_lo_contraction(mech.kernel, green.GL, green.GG, mech.qK, p.NLO, p.plusLO,
-                                          p.minusLO, SL=cand.L, SG=cand.G)
+        if _compact_mechanism(mech):
+            r = _mi_candidate(p, mech, green, cand)
         else:
-            api.production_static_contraction(mech.kernel, green.GL, mech.qK, S=cand.L)
-            api.production_static_contraction(mech.kernel, green.GG, mech.qK, S=cand.G)
-        if p.verify_roundoff:
-            _, _, r = api.retarded_self_energy_fft(p.hilbert, cand.L, cand.G, return_roundoff=True, SR=cand.R)
+            if mech.is_lo:
+                api.production_lo_contraction(mech.kernel, green.GL, green.GG, mech.qK, p.NLO, p.plusLO,
+                                              p.minusLO, SL=cand.L, SG=cand.G)
+            else:
+                api.production_static_contraction(mech.kernel, green.GL, mech.qK, S=cand.L)
+                api.production_static_contraction(mech.kernel, green.GG, mech.qK, S=cand.G)
+            r = None
+            if p.verify_roundoff:
+                _, _, r = api.retarded_self_energy_fft(p.hilbert, cand.L, cand.G, return_roundoff=True,
+                                                       SR=cand.R)
+            else:
+                api.retarded_self_energy_fft(p.hilbert, cand.L, cand.G, SR=cand.R)
+        if r is not None:
             roundoff.append(r)
-        else:
-            api.retarded_self_energy_fft(p.hilbert, cand.L, cand.G, SR=cand.R)
         fam = s.scattering[mech.name]
         pairs = list(zip(fam.parts(), cand.parts()))
         tot = [(totals[0], [0], "accumulate"), (totals[1], [1], "accumulate"), (totals[2], [2], "accumulate"),
