@@ -540,6 +540,17 @@ int qclhip_mix_fields_residual_accumulate(qclhip_ctx* ctx, int64_t n, int npairs
                                           const double* const* cand, double alpha, const int* partner, int ntot,
                                           const int* tot_count, const int* tot_members, const int* tot_flags,
                                           double* const* totals, double* ratios_host);
+/* The same pass where the candidates of the pairs with compact[p] != 0 are
+ * one momentum's columns (NE x Nb^2, the momentum-independent contraction
+ * without its broadcast, qclhip_contract_compact) read for every momentum
+ * of the grid's fields: the values, totals and ratios of mixing their
+ * broadcast fields, bit for bit, without writing or reading those fields
+ * (n = NE Nk Nb^2 < 2^31). */
+int qclhip_mix_fields_residual_compact(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t n, int npairs,
+                                       double* const* old_, const double* const* cand, double alpha,
+                                       const int* partner, int ntot, const int* tot_count, const int* tot_members,
+                                       const int* tot_flags, double* const* totals, const int* compact,
+                                       double* ratios_host);
 
 /* The residual suite in two passes, for an iteration that measures the state
  * before its candidates exist (production_residual_suite, residuals.jl:

@@ -838,6 +838,10 @@ struct MixTotArgs {
     // kTotCandidate: the total sums the members' candidate values and
     // neither mixes them nor measures them (another total owns the pairs)
     int flags[kMaxTotals];
+    // Bit p: the candidate of pair p holds one momentum's columns (NE x Nb^2),
+    // read for every momentum of the NE x Nk x Nb^2 field (n < 2^31).
+    unsigned cmask;
+    int NE, Nk;
 };
 constexpr int kTotAccumulate = 1;
 constexpr int kTotCandidate = 2;
@@ -856,11 +860,17 @@ __global__ __launch_bounds__(256) void k_mix_totals(long long n, MixTotArgs a, d
     const long long stride = (long long)kMixChunks * 256;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
         c64 o[kMaxFamilies], c[kMaxFamilies];
+        long long ci = i;  // the element of a one-momentum candidate
+        if (a.cmask) {
+            const unsigned ii = (unsigned)i, col = ii / (unsigned)a.NE;
+            ci = (long long)(ii - col * (unsigned)a.NE) + (long long)a.NE * (col / (unsigned)a.Nk);
+        }
 #pragma unroll
         for (int f = 0; f < kMaxFamilies; ++f)  // all loads first
             if (f < nm) {
-                if (!cand_only) o[f] = a.old_[a.mem[t][f]][i];
-                c[f] = a.cand[a.mem[t][f]][i];
+                const int pm = a.mem[t][f];
+                if (!cand_only) o[f] = a.old_[pm][i];
+                c[f] = a.cand[pm][((a.cmask >> pm) & 1u) ? ci : i];
             }
         const c64 prior = accumulate ? tot[i] : cmk(0.0, 0.0);
         c64 sum = cmk(0.0, 0.0), o1 = sum, c1 = sum, o2 = sum, c2 = sum;
@@ -1242,13 +1252,24 @@ int qclhip_mix_fields_residual(qclhip_ctx* ctx, int64_t n, int npairs, double* c
 
 static int mix_totals_impl(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_, const double* const* cand,
                            double alpha, const int* partner, int ntot, const int* tot_count, const int* tot_members,
-                           const int* tot_flags, double* const* totals, double* ratios_host) {
+                           const int* tot_flags, double* const* totals, double* ratios_host,
+                           const qclhip_grid* grid = nullptr, const int* compact = nullptr) {
     if (!ctx || !old_ || !cand || !partner || !tot_count || !tot_members || !totals || !ratios_host || n < 0)
         return QCLHIP_ERR_ARGUMENT;
     if (npairs < 1 || npairs > kMaxResPairs) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..32 mixing pairs");
     if (ntot < 1 || ntot > kMaxTotals) return fail(ctx, QCLHIP_ERR_ARGUMENT, "1..4 totals");
     if (!(alpha >= 0 && alpha <= 1)) return fail(ctx, QCLHIP_ERR_ARGUMENT, "mixing factor must lie in [0,1]");
     MixTotArgs a{};
+    if (compact) {
+        if (!grid) return QCLHIP_ERR_ARGUMENT;
+        if (n != (int64_t)grid->NE * grid->Nk * grid->Nb * grid->Nb)
+            return fail(ctx, QCLHIP_ERR_DIMENSION, "the fields do not match the grid");
+        if (n >= ((int64_t)1 << 31)) return fail(ctx, QCLHIP_ERR_UNSUPPORTED, "one-momentum candidates need n < 2^31");
+        for (int p = 0; p < npairs; ++p)
+            if (compact[p]) a.cmask |= 1u << p;
+        a.NE = grid->NE;
+        a.Nk = grid->Nk;
+    }
     std::vector<int> owner(npairs, -1), pos(npairs, -1);
     int k = 0;
     for (int t = 0; t < ntot; ++t) {
@@ -1325,6 +1346,16 @@ int qclhip_mix_fields_residual_accumulate(qclhip_ctx* ctx, int64_t n, int npairs
     if (!tot_flags) return QCLHIP_ERR_ARGUMENT;
     return mix_totals_impl(ctx, n, npairs, old_, cand, alpha, partner, ntot, tot_count, tot_members, tot_flags,
                            totals, ratios_host);
+}
+
+int qclhip_mix_fields_residual_compact(qclhip_ctx* ctx, const qclhip_grid* grid, int64_t n, int npairs,
+                                       double* const* old_, const double* const* cand, double alpha,
+                                       const int* partner, int ntot, const int* tot_count, const int* tot_members,
+                                       const int* tot_flags, double* const* totals, const int* compact,
+                                       double* ratios_host) {
+    if (!grid || !tot_flags || !compact) return QCLHIP_ERR_ARGUMENT;
+    return mix_totals_impl(ctx, n, npairs, old_, cand, alpha, partner, ntot, tot_count, tot_members, tot_flags,
+                           totals, ratios_host, grid, compact);
 }
 
 int qclhip_mix_fields_change(qclhip_ctx* ctx, int64_t n, int npairs, double* const* old_, const double* const* cand,

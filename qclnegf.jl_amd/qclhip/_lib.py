@@ -146,6 +146,10 @@ _SIGNATURES = {
     "qclhip_mix_fields_residual_accumulate": [_vp, ct.c_int64, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),
                                               ct.c_double, ct.POINTER(ct.c_int), ct.c_int, ct.POINTER(ct.c_int),
                                               ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), ct.POINTER(_vp), _dp],
+    "qclhip_mix_fields_residual_compact": [_vp, _vp, ct.c_int64, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),
+                                           ct.c_double, ct.POINTER(ct.c_int), ct.c_int, ct.POINTER(ct.c_int),
+                                           ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), ct.POINTER(_vp),
+                                           ct.POINTER(ct.c_int), _dp],
     "qclhip_residual_state": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp),
                               ct.POINTER(_vp), _dp],
     "qclhip_residual_keldysh": [_vp, _vp, _vp, _vp, ct.c_int, ct.POINTER(_vp), _dp],

@@ -697,7 +697,7 @@ def mix_fields(pairs, alpha: float):
     ctx.call("qclhip_mix_fields", ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha))
 
 
-def mix_fields_residual(pairs, alpha: float, partner=None, totals=None):
+def mix_fields_residual(pairs, alpha: float, partner=None, totals=None, compact=None, grid=None):
     """Mix every (old, candidate) pair in one pass and return the self-energy
     residual ratios of the unmixed pairs: one per pair, then one per couple
     (p, partner[p]) with partner[p] > p, for the sum of the two pairs. A
@@ -708,13 +708,21 @@ def mix_fields_residual(pairs, alpha: float, partner=None, totals=None):
     mixed values (every pair in one total). flags: "accumulate" continues
     the fold from the field's current value; "candidate" sums the members'
     candidate values instead (the pairs then also belong to a mixing total);
-    both may be given as "accumulate,candidate"."""
+    both may be given as "accumulate,candidate".
+
+    compact: optional per-pair flags (with the grid and totals): a flagged
+    pair's candidate is one momentum's columns (compact_field), read for
+    every momentum (bitwise the mixing of its broadcast field)."""
     pairs = list(pairs)
     if not pairs:
         raise _lib.ArgumentError("no pairs to mix")
     n = pairs[0][0].numel()
-    for o, c in pairs:
-        if o.dtype != torch.complex128 or c.dtype != torch.complex128 or o.numel() != n or c.numel() != n \
+    cflags = [0] * len(pairs) if compact is None else [int(bool(x)) for x in compact]
+    if compact is not None and (grid is None or totals is None or len(cflags) != len(pairs)):
+        raise _lib.ArgumentError("compact candidates need the grid, the totals and one flag per pair")
+    for (o, c), cf in zip(pairs, cflags):
+        nc = n // grid.Nk if cf else n
+        if o.dtype != torch.complex128 or c.dtype != torch.complex128 or o.numel() != n or c.numel() != nc \
                 or not o.is_contiguous() or not c.is_contiguous():
             raise _lib.ArgumentError("mixing pairs must be contiguous complex128 fields of one size")
     partner = [-1] * len(pairs) if partner is None else list(partner)
@@ -742,7 +750,10 @@ def mix_fields_residual(pairs, alpha: float, partner=None, totals=None):
         common = (ct.c_int64(n), len(pairs), olds, cands, ct.c_double(alpha), (ct.c_int * len(pairs))(*partner),
                   len(totals), (ct.c_int * len(counts))(*counts), (ct.c_int * len(members))(*members))
         tptr = (_vp * len(totals))(*[t.data_ptr() for t, _, _ in totals])
-        if any(flags):
+        if compact is not None:
+            ctx.call("qclhip_mix_fields_residual_compact", grid.handle, *common, (ct.c_int * len(flags))(*flags),
+                     tptr, (ct.c_int * len(cflags))(*cflags), out)
+        elif any(flags):
             ctx.call("qclhip_mix_fields_residual_accumulate", *common, (ct.c_int * len(flags))(*flags), tptr, out)
         else:
             ctx.call("qclhip_mix_fields_residual_totals", *common, tptr, out)

@@ -32,13 +32,14 @@ def _compact_mechanism(m) -> bool:
     return not m.is_lo and m.kernel.kind == "momentum_independent"
 
 
-def _mi_candidate(p, mech, green, fam):
+def _mi_candidate(p, mech, green, fam=None):
     """Candidate of a momentum-independent mechanism: S< and S> contracted to
     the one column set that every momentum holds, S^R transformed once on it,
     all three broadcast into the family; bit for bit the values of the
     full-field route (production_operators.jl:535-595, 842-884), whose
     transform repeats each column for every momentum. Returns the round-off
-    witness (None without verification)."""
+    witness (None without verification). Without fam the columns stay in
+    p.mi_compact (for a mixing pass that reads them for every momentum)."""
     g = p.grid
     YR, YL, YG = p.mi_compact
     api.momentum_independent_contraction_compact(mech.kernel, green.GL, mech.qK, Y=YL)
@@ -48,8 +49,9 @@ def _mi_candidate(p, mech, green, fam):
         _, _, r = api.retarded_self_energy_fft(p.hilbert_compact, YL, YG, return_roundoff=True, SR=YR)
     else:
         api.retarded_self_energy_fft(p.hilbert_compact, YL, YG, SR=YR)
-    for Y, S in ((YR, fam.R), (YL, fam.L), (YG, fam.G)):
-        api.broadcast_momenta(g, Y, S)
+    if fam is not None:
+        for Y, S in ((YR, fam.R), (YL, fam.L), (YG, fam.G)):
+            api.broadcast_momenta(g, Y, S)
     return r
 
 
@@ -344,8 +346,9 @@ def _lean_tail(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, green, info, sta
     roundoff = []
     cand = Family(tmp[0], tmp[1], tmp[2])
     for mech in p.mechanisms:
-        if _compact_mechanism(mech):
-            r = _mi_candidate(p, mech, green, cand)
+        compact = _compact_mechanism(mech)
+        if compact:
+            r = _mi_candidate(p, mech, green)  # mixed from its one-momentum columns
         else:
             if mech.is_lo:
                 api.production_lo_contraction(mech.kernel, green.GL, green.GG, mech.qK, p.NLO, p.plusLO,
@@ -362,10 +365,11 @@ def _lean_tail(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, green, info, sta
         if r is not None:
             roundoff.append(r)
         fam = s.scattering[mech.name]
-        pairs = list(zip(fam.parts(), cand.parts()))
+        pairs = list(zip(fam.parts(), p.mi_compact if compact else cand.parts()))
         tot = [(totals[0], [0], "accumulate"), (totals[1], [1], "accumulate"), (totals[2], [2], "accumulate"),
                (ws.csum, [1], "accumulate,candidate")]
-        ratios.append(api.mix_fields_residual(pairs, p.alpha_mix, [-1, -1, -1], totals=tot))
+        ratios.append(api.mix_fields_residual(pairs, p.alpha_mix, [-1, -1, -1], totals=tot,
+                                              compact=[1, 1, 1] if compact else None, grid=g))
     t4 = stamp()
     r_K = api.residual_keldysh(g, green, [ws.csum])
     ws.totals_valid = True

@@ -65,3 +65,33 @@ def test_compact_contraction_argument_errors(hip):
         hip.momentum_independent_contraction_compact(hip.ProductionKernel(g, "low_rank", l, r, rank=2), G)
     with pytest.raises(_lib.DimensionMismatch):
         hip.broadcast_momenta(g, G)
+
+
+def test_compact_mix_equals_broadcast_mix(hip):
+    """A mixing pass that reads one-momentum candidates for every momentum
+    leaves the same mixed fields, accumulated totals, candidate sum and
+    ratios, bit for bit, as the pass over their broadcast fields."""
+    rng = np.random.default_rng(11)
+    N, NE, Nk = 6, 50, 3
+    eps = np.linspace(-0.2, 0.4, NE)
+    g = hip.Grid(eps, np.full(NE, eps[1] - eps[0]), np.ones(Nk), N)
+    field = lambda: hip.to_device(_fields(rng, NE, Nk, N, 0.05)[0])
+    olds = [field() for _ in range(3)]
+    Ys = [hip.to_device(_fields(rng, NE, 1, N, 0.05)[0]) for _ in range(3)]
+    tots = [field() for _ in range(4)]
+
+    def run(compact):
+        o = [x.clone() for x in olds]
+        t = [x.clone() for x in tots]
+        c = Ys if compact else [hip.broadcast_momenta(g, Y) for Y in Ys]
+        spec = [(t[0], [0], "accumulate"), (t[1], [1], "accumulate"), (t[2], [2], "accumulate"),
+                (t[3], [1], "accumulate,candidate")]
+        r = hip.mix_fields_residual(list(zip(o, c)), 0.3, [-1, -1, -1], totals=spec,
+                                    compact=[1, 1, 1] if compact else None, grid=g)
+        return o, t, r
+
+    o1, t1, r1 = run(True)
+    o2, t2, r2 = run(False)
+    for a, b in zip(o1 + t1, o2 + t2):
+        assert torch.equal(a, b)
+    assert r1 == r2
