@@ -32,16 +32,17 @@ def _compact_mechanism(m) -> bool:
     return not m.is_lo and m.kernel.kind == "momentum_independent"
 
 
-def _mi_candidate(p, mech, green, fam=None):
+def _mi_candidate(p, mech, green, fam=None, parts=(0, 1, 2)):
     """Candidate of a momentum-independent mechanism: S< and S> contracted to
     the one column set that every momentum holds, S^R transformed once on it,
     all three broadcast into the family; bit for bit the values of the
     full-field route (production_operators.jl:535-595, 842-884), whose
     transform repeats each column for every momentum. Returns the round-off
-    witness (None without verification). Without fam the columns stay in
-    p.mi_compact (for a mixing pass that reads them for every momentum)."""
+    witness (None without verification). The columns stay in
+    p.mi_compact[mech.name] (R, L, G) for a mixing pass that reads them for
+    every momentum; with fam, the components in parts are broadcast into it."""
     g = p.grid
-    YR, YL, YG = p.mi_compact
+    YR, YL, YG = p.mi_compact[mech.name]
     api.momentum_independent_contraction_compact(mech.kernel, green.GL, mech.qK, Y=YL)
     api.momentum_independent_contraction_compact(mech.kernel, green.GG, mech.qK, Y=YG)
     r = None
@@ -50,8 +51,8 @@ def _mi_candidate(p, mech, green, fam=None):
     else:
         api.retarded_self_energy_fft(p.hilbert_compact, YL, YG, SR=YR)
     if fam is not None:
-        for Y, S in ((YR, fam.R), (YL, fam.L), (YG, fam.G)):
-            api.broadcast_momenta(g, Y, S)
+        for c in parts:
+            api.broadcast_momenta(g, (YR, YL, YG)[c], fam.parts()[c])
     return r
 
 
@@ -87,7 +88,8 @@ class ScbaProblem:
         if any(_compact_mechanism(m) for m in self.mechanisms):
             self.hilbert_compact = api.FFTHilbertPlan(g, g.Nb * g.Nb, verify_roundoff=self.verify_roundoff)
             self.hilbert_compact.set_multiplicity(g.Nk)
-            self.mi_compact = tuple(api.compact_field(g) for _ in range(3))
+            self.mi_compact = {m.name: tuple(api.compact_field(g) for _ in range(3))
+                               for m in self.mechanisms if _compact_mechanism(m)}
 
 
 @dataclass
@@ -123,15 +125,18 @@ class ScbaState:
         return out
 
 
-def _candidate_scattering(p: ScbaProblem, green: api.GreenState, out: dict):
+def _candidate_scattering(p: ScbaProblem, green: api.GreenState, out: dict, compact_rg: bool = False):
     """_scattering_candidate_production (production_operators.jl:1258-1335).
     Returns the largest round-off witness (a list of Deferred slots in
-    deferred-results mode)."""
+    deferred-results mode). compact_rg: a momentum-independent mechanism's
+    S^R and S> stay one-momentum columns (p.mi_compact), for a mixing pass
+    that reads them so; only its S< (read by the residual suite) is
+    broadcast."""
     roundoff = [] if p.grid.ctx.deferring else 0.0
     for mech in p.mechanisms:
         fam = out[mech.name]
         if _compact_mechanism(mech):
-            r = _mi_candidate(p, mech, green, fam)
+            r = _mi_candidate(p, mech, green, fam, (1,) if compact_rg else (0, 1, 2))
             if r is not None:
                 roundoff = roundoff + [r] if isinstance(roundoff, list) else max(roundoff, r)
             continue
@@ -238,7 +243,10 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
         ws.totals_valid = ws.totals is not None
     s.green = green
     t1 = stamp()
-    roundoff = _candidate_scattering(p, green, ws.scat)
+    # linear mixing with totals reads a momentum-independent mechanism's S^R
+    # and S> candidates as one-momentum columns
+    compact_rg = ws.anderson is None and p.measure
+    roundoff = _candidate_scattering(p, green, ws.scat, compact_rg)
     t2 = stamp()
     _candidate_embedding(p, green, ws.plus, ws.minus)
     t3 = stamp()
@@ -253,10 +261,17 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
     Jbar = api.boundary_flux_bar(g, s.plus.L, s.plus.G, green.GL, green.GG)
     pops = api.state_populations(g, green.GL, p.target)
     t5 = stamp()
-    pairs = []
+    pairs, compact = [], []
     for m in p.mechanisms:
-        pairs += list(zip(s.scattering[m.name].parts(), ws.scat[m.name].parts()))
+        cand = ws.scat[m.name].parts()
+        cflags = (0, 0, 0)
+        if compact_rg and _compact_mechanism(m):
+            YR, _, YG = p.mi_compact[m.name]
+            cand, cflags = (YR, cand[1], YG), (1, 0, 1)
+        pairs += list(zip(s.scattering[m.name].parts(), cand))
+        compact += cflags
     base = len(pairs)
+    compact += [0] * 6
     for fam, cand in ((s.plus, ws.plus), (s.minus, ws.minus)):
         pairs += list(zip(fam.parts(), cand.parts()))
     sigma = None
@@ -277,7 +292,8 @@ def scba_iteration(p: ScbaProblem, s: ScbaState, ws: ScbaWorkspace, timings: dic
         M = len(p.mechanisms)
         members = [[base + c, base + 3 + c] + [3 * k + c for k in range(M)] for c in range(3)]
         sigma = api.mix_fields_residual(pairs, p.alpha_mix, partner,
-                                        totals=list(zip(ws.totals.parts(), members)))
+                                        totals=list(zip(ws.totals.parts(), members)),
+                                        compact=compact if any(compact) else None, grid=g)
     else:
         api.mix_fields(pairs, p.alpha_mix)
     if g.ctx.deferring:  # captured or deferred: results are slots of the result buffer
@@ -365,7 +381,7 @@ def _lean_tail(p: ScbaProblem, s: ScbaState, ws: LeanWorkspace, green, info, sta
         if r is not None:
             roundoff.append(r)
         fam = s.scattering[mech.name]
-        pairs = list(zip(fam.parts(), p.mi_compact if compact else cand.parts()))
+        pairs = list(zip(fam.parts(), p.mi_compact[mech.name] if compact else cand.parts()))
         tot = [(totals[0], [0], "accumulate"), (totals[1], [1], "accumulate"), (totals[2], [2], "accumulate"),
                (ws.csum, [1], "accumulate,candidate")]
         ratios.append(api.mix_fields_residual(pairs, p.alpha_mix, [-1, -1, -1], totals=tot,
