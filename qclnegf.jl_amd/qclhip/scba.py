@@ -34,10 +34,10 @@ def _compact_mechanism(m) -> bool:
 
 def _mi_candidate(p, mech, green, fam=None, parts=(0, 1, 2)):
     """Candidate of a momentum-independent mechanism: S< and S> contracted to
-    the one column set that every momentum holds, S^R transformed once on it,
-    all three broadcast into the family; bit for bit the values of the
-    full-field route (production_operators.jl:535-595, 842-884), whose
-    transform repeats each column for every momentum. Returns the round-off
+    the one column set that every momentum holds and S^R transformed once on
+    it: bit for bit the values of the full-field route (production_operators.jl:
+    535-595, 842-884), whose transform repeats each column for every
+    momentum. Returns the round-off
     witness (None without verification). The columns stay in
     p.mi_compact[mech.name] (R, L, G) for a mixing pass that reads them for
     every momentum; with fam, the components in parts are broadcast into it."""
